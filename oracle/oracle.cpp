@@ -1,0 +1,625 @@
+// oracle.cpp - CPU checker (test infrastructure, see oracle.h).
+//
+// Single-threaded fp64 restatement of the reference loops; every function
+// names the reference lines it follows (D: = src/DGA/Discretization.hpp).
+#include "oracle.h"
+
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace {
+
+thread_local std::string g_err;
+
+const long double PI_L = 3.141592653589793238L;  // Utilities.hpp:92
+
+struct V3 {
+    double x = 0, y = 0, z = 0;
+};
+
+// Source::Compute, Source.hpp:313-327 (carrier), :328-331 (decay) and
+// :520-567 (cartesian directions, gaussian / wave profiles).
+V3 source_compute(const tf_source_desc& s, const double p[3], double t) {
+    double ret = s.amp;
+    if (s.type == TF_FIELD_J) ret *= s.impedance;
+    switch (s.carrier) {
+        case TF_CARRIER_SIN: ret *= std::sin(2 * PI_L * s.freq * (long double)(t - s.delay)); break;
+        case TF_CARRIER_COS: ret *= s.amp * std::cos(2 * PI_L * s.freq * (long double)(t - s.delay)); break;
+        case TF_CARRIER_GAUSSIAN: ret *= std::exp(-std::pow(t - s.delay, 2) / std::pow(s.width, 2)); break;
+        case TF_CARRIER_RICKER:
+            ret *= (1 - 2 * std::pow(PI_L * s.freq * (long double)(t - s.delay), 2)) *
+                   std::exp(-std::pow(PI_L * s.freq * (long double)(t - s.delay), 2));
+            break;
+        case TF_CARRIER_GAUSSMOD:  // extension carrier: gaussian envelope times sin
+            ret *= std::exp(-std::pow(t - s.delay, 2) / std::pow(s.width, 2));
+            ret *= std::sin(2 * PI_L * s.freq * (long double)(t - s.delay));
+            break;
+        default: break;
+    }
+    if (s.decay != 0)
+        ret *= std::pow(std::pow(p[0] - s.center[0], 2) + std::pow(p[1] - s.center[1], 2) +
+                            std::pow(p[2] - s.center[2], 2),
+                        -0.5 * s.decay);
+    if (s.profile == 2) {
+        double e = -((std::pow(p[0] - s.center[0], 2) + std::pow(p[1] - s.center[1], 2) +
+                      std::pow(p[2] - s.center[2], 2)) /
+                     std::pow(s.radius, 2));
+        ret *= std::exp(e);
+    } else if (s.profile == 1) {
+        for (int j = 0; j < 3; ++j) {
+            if (s.bfunc[j] == 1) ret *= std::sin(2 * PI_L * s.kvec[j] * (long double)(p[j] - s.center[j]));
+            else if (s.bfunc[j] == 2) ret *= std::cos(2 * PI_L * s.kvec[j] * (long double)(p[j] - s.center[j]));
+        }
+    }
+    V3 v;
+    if (s.dir == 0) v.x = ret;
+    else if (s.dir == 1) v.y = ret;
+    else v.z = ret;
+    return v;
+}
+
+struct SourceTable {
+    std::vector<tf_source_desc> sd;
+    std::vector<tf_source_edge> se;
+    std::vector<int64_t> by_edge;  // edge -> index into se, or -1
+};
+
+// ComputeHfieldSource (D:4088-4202): sum of h-type sources at the edge
+// barycentre, dotted with the boundary dual edge vector; 0 for t < 0.
+double h_source(const SourceTable& st, int64_t k, double t) {
+    if (t < 0) return 0;
+    const tf_source_edge& e = st.se[k];
+    V3 acc;
+    for (int i = 0; i < e.n_src; ++i) {
+        const tf_source_desc& s = st.sd[e.src[i]];
+        if (s.type != TF_FIELD_H) continue;
+        V3 v = source_compute(s, e.bary, t);
+        acc.x += v.x; acc.y += v.y; acc.z += v.z;
+    }
+    return acc.x * e.vec[0] + acc.y * e.vec[1] + acc.z * e.vec[2];
+}
+
+// ComputeEfieldSource (D:3907-3940): e-type sources dotted with the edge vector.
+double e_source(const SourceTable& st, int64_t k, double t) {
+    if (t < 0) return 0;
+    const tf_source_edge& e = st.se[k];
+    V3 acc;
+    for (int i = 0; i < e.n_src; ++i) {
+        const tf_source_desc& s = st.sd[e.src[i]];
+        if (s.type != TF_FIELD_E) continue;
+        V3 v = source_compute(s, e.bary, t);
+        acc.x += v.x; acc.y += v.y; acc.z += v.z;
+    }
+    return acc.x * e.vec[0] + acc.y * e.vec[1] + acc.z * e.vec[2];
+}
+
+// ComputeCurrentSource (D:3994-4028): j-type sources dotted with the unit edge vector.
+double j_source(const SourceTable& st, int64_t k, double t) {
+    if (t < 0) return 0;
+    const tf_source_edge& e = st.se[k];
+    V3 acc;
+    for (int i = 0; i < e.n_src; ++i) {
+        const tf_source_desc& s = st.sd[e.src[i]];
+        if (s.type != TF_FIELD_J) continue;
+        V3 v = source_compute(s, e.bary, t);
+        acc.x += v.x; acc.y += v.y; acc.z += v.z;
+    }
+    const double nrm = std::sqrt(e.vec[0] * e.vec[0] + e.vec[1] * e.vec[1] + e.vec[2] * e.vec[2]);
+    return (1.0 / nrm) * (acc.x * e.vec[0] + acc.y * e.vec[1] + acc.z * e.vec[2]);
+}
+
+// 3x3 inverse by cofactors (what Eigen's Matrix3d::inverse computes).
+void inv3(const double m[9], double r[9]) {
+    const double c00 = m[4] * m[8] - m[5] * m[7], c01 = m[5] * m[6] - m[3] * m[8], c02 = m[3] * m[7] - m[4] * m[6];
+    const double det = m[0] * c00 + m[1] * c01 + m[2] * c02;
+    r[0] = c00 / det;
+    r[1] = (m[2] * m[7] - m[1] * m[8]) / det;
+    r[2] = (m[1] * m[5] - m[2] * m[4]) / det;
+    r[3] = c01 / det;
+    r[4] = (m[0] * m[8] - m[2] * m[6]) / det;
+    r[5] = (m[2] * m[3] - m[0] * m[5]) / det;
+    r[6] = c02 / det;
+    r[7] = (m[1] * m[6] - m[0] * m[7]) / det;
+    r[8] = (m[0] * m[4] - m[1] * m[3]) / det;
+}
+
+// Output::AllowPrint (Output.hpp:318-328)
+struct Printer {
+    bool on;
+    double next, period;
+    bool allow(double t) {
+        if (!on) return false;
+        if (t >= next) { next += period; return true; }
+        return false;
+    }
+};
+
+struct Sink {
+    oracle_out* o;
+    int32_t np;
+    void put(double t, const std::vector<double>& six) {
+        if (o->n_samples < o->cap_samples) {
+            if (o->times) o->times[o->n_samples] = t;
+            if (o->vals)
+                std::memcpy(o->vals + o->n_samples * np * 6, six.data(), sizeof(double) * six.size());
+        }
+        o->n_samples++;
+    }
+    void poy(double v) {
+        if (o->poynting && o->n_poynting < o->cap_poynting) o->poynting[o->n_poynting] = v;
+        o->n_poynting++;
+    }
+};
+
+// GetFitElectricField D:3555-3568 and GetFitMagneticField D:3570-3588.
+void export_fit(const std::vector<double>& U, const std::vector<double>& B, const tf_probe_geom* pg, int32_t np,
+                std::vector<double>& six) {
+    static const int axis[12] = {0, 1, 2, 1, 2, 0, 2, 2, 0, 1, 1, 0};
+    static const int faxis[6] = {2, 1, 0, 0, 1, 2};
+    six.assign((size_t)np * 6, 0.0);
+    for (int p = 0; p < np; ++p) {
+        const tf_probe_geom& g = pg[p];
+        const double Lx = g.L[0], Ly = g.L[1], Lz = g.L[2];
+        const double volume = Lx * Ly * Lz;
+        const double da[3] = {Ly * Lz / 4, Lx * Lz / 4, Lx * Ly / 4};
+        double e[3] = {0, 0, 0};
+        for (int i = 0; i < 12; ++i) {
+            const double u = U[g.edges[i]];
+            for (int c = 0; c < 3; ++c) e[c] = e[c] + (c == axis[i] ? u * da[c] : u * 0.0);
+        }
+        for (int c = 0; c < 3; ++c) six[p * 6 + c] = e[c] / volume;
+        const double inc[3] = {0.5 * Lx, 0.5 * Ly, 0.5 * Lz};
+        double h[3] = {0, 0, 0};
+        for (int i = 0; i < 6; ++i) {
+            const double u = B[g.faces[i]];
+            for (int c = 0; c < 3; ++c) h[c] = h[c] + (c == faxis[i] ? u * inc[c] : u * 0.0);
+        }
+        double mi[9];
+        inv3(g.mu, mi);
+        for (int c = 0; c < 3; ++c)
+            six[p * 6 + 3 + c] = (mi[c * 3] * h[0] + mi[c * 3 + 1] * h[1] + mi[c * 3 + 2] * h[2]) / volume;
+    }
+}
+
+// Inverse of a 4x4 matrix by Gauss-Jordan with partial pivoting.
+void inv4(const double a[16], double r[16]) {
+    double m[4][8];
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 8; ++j) m[i][j] = j < 4 ? a[i * 4 + j] : (j - 4 == i ? 1.0 : 0.0);
+    for (int c = 0; c < 4; ++c) {
+        int piv = c;
+        for (int i = c + 1; i < 4; ++i)
+            if (std::fabs(m[i][c]) > std::fabs(m[piv][c])) piv = i;
+        for (int j = 0; j < 8; ++j) std::swap(m[c][j], m[piv][j]);
+        const double d = m[c][c];
+        for (int j = 0; j < 8; ++j) m[c][j] /= d;
+        for (int i = 0; i < 4; ++i) {
+            if (i == c) continue;
+            const double f = m[i][c];
+            for (int j = 0; j < 8; ++j) m[i][j] -= f * m[c][j];
+        }
+    }
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) r[i * 4 + j] = m[i][j + 4];
+}
+
+// GetWhitneyElectricField D:3768-3875 (nodal functions as the inverse of the
+// [x y z 1] node matrix) and GetMagneticField D:3877-3897.
+void export_tet(const std::vector<double>& U, const std::vector<double>& B, const tf_probe_geom* pg, int32_t np,
+                std::vector<double>& six) {
+    six.assign((size_t)np * 6, 0.0);
+    static const int le[6][2] = {{0, 1}, {1, 2}, {0, 2}, {0, 3}, {1, 3}, {2, 3}};
+    for (int p = 0; p < np; ++p) {
+        const tf_probe_geom& g = pg[p];
+        double node[16];
+        for (int i = 0; i < 4; ++i) {
+            for (int c = 0; c < 3; ++c) node[i * 4 + c] = g.nodes[i][c];
+            node[i * 4 + 3] = 1;
+        }
+        double wn[16];  // wn = node^-1: row r of [p 1] * wn gives the nodal values
+        inv4(node, wn);
+        double w[4];
+        for (int i = 0; i < 4; ++i)
+            w[i] = g.point[0] * wn[0 * 4 + i] + g.point[1] * wn[1 * 4 + i] + g.point[2] * wn[2 * 4 + i] + wn[3 * 4 + i];
+        double e[3] = {0, 0, 0};
+        for (int k = 0; k < 6; ++k) {
+            const int i = le[k][0], j = le[k][1];
+            const double u = U[g.edges[k]];
+            for (int c = 0; c < 3; ++c) e[c] += (w[i] * wn[c * 4 + j] - w[j] * wn[c * 4 + i]) * u;
+        }
+        for (int c = 0; c < 3; ++c) six[p * 6 + c] = e[c];
+        double pev[6][3];
+        for (int k = 0; k < 6; ++k)
+            for (int c = 0; c < 3; ++c) pev[k][c] = g.nodes[le[k][1]][c] - g.nodes[le[k][0]][c];
+        double fq[4];
+        for (int i = 0; i < 4; ++i) fq[i] = B[g.faces[i]];
+        const double v = 3 * g.volume;
+        double ret[3] = {0, 0, 0};
+        for (int c = 0; c < 3; ++c) {
+            ret[c] += (pev[0][c] * fq[1] - pev[2][c] * fq[2] + pev[3][c] * fq[3]) / v;
+            ret[c] += (pev[0][c] * fq[0] - pev[1][c] * fq[2] + pev[4][c] * fq[3]) / v;
+            ret[c] += (-pev[1][c] * fq[1] + pev[2][c] * fq[0] + pev[5][c] * fq[3]) / v;
+            ret[c] += (pev[3][c] * fq[0] - pev[4][c] * fq[1] + pev[5][c] * fq[2]) / v;
+        }
+        double mi[9];
+        inv3(g.mu, mi);
+        for (int c = 0; c < 3; ++c)
+            six[p * 6 + 3 + c] = 0.25 * (mi[c * 3] * ret[0] + mi[c * 3 + 1] * ret[1] + mi[c * 3 + 2] * ret[2]);
+    }
+}
+
+double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// ---------------------------------------------------------------- fdtd
+// D:1840-1933
+void run_fdtd(const tf_problem* p, const SourceTable& st, const tf_probe_geom* pg, int32_t np, Printer pr,
+              int64_t first, int64_t n, bool faithful, const double* U0, const double* F0, const double* B0,
+              oracle_out* out) {
+    const tf_fit_ops& f = p->fit;
+    const int32_t E = p->n_edges, Fn = p->n_faces;
+    const double dt = p->dt;
+    std::vector<double> U(E, 0.0), F(Fn, 0.0), B(Fn, 0.0), I(E, 0.0), U_old, F_old, curl_fb(E, 0.0);
+    if (U0) U.assign(U0, U0 + E);
+    if (F0) F.assign(F0, F0 + Fn);
+    if (B0) B.assign(B0, B0 + Fn);
+    // Ct_vec as the reference keeps it: vector of vector of signed indices
+    struct Sgn { uint32_t val; int32_t sgn; };
+    std::vector<std::vector<Sgn>> Ct_vec;
+    if (faithful) {
+        Ct_vec.resize(E);
+        for (int32_t e = 0; e < E; ++e)
+            for (int64_t q = f.ct_ptr[e]; q < f.ct_ptr[e + 1]; ++q)
+                Ct_vec[e].push_back(Sgn{(uint32_t)f.ct_face[q], f.ct_sgn[q]});
+    }
+    Sink sink{out, np};
+    std::vector<double> six;
+    double loop_s = 0;
+    for (int64_t i = first; i < first + n; ++i) {
+        const double t0 = now_s();
+        const double t = (double)i * dt;
+        U_old = U;
+        for (int64_t q = 0; q < f.n_bc; ++q) U[f.bc_edges[q]] = t < 0 ? 0.0 : f.bc_value[q];  // ComputeEdgeBC
+        for (int64_t q = 0; q < f.n_src; ++q) {
+            const int32_t j = f.src_edges[q];
+            const int64_t k = st.by_edge[j];
+            if (k < 0) continue;
+            if (st.se[k].field == TF_FIELD_J) I[j] = j_source(st, k, t);
+            else U[j] = e_source(st, k, t);
+        }
+        for (int64_t q = 0; q < f.n_common; ++q) {
+            const int32_t j = f.common_edges[q];
+            if (faithful) {
+                auto abs_ct_vec = std::vector<uint32_t>(
+                    {Ct_vec[j][0].val, Ct_vec[j][1].val, Ct_vec[j][2].val, Ct_vec[j][3].val});
+                U[j] = std::pow(f.upml[j], 2) * f.M_h[j] *
+                       (f.M_q[j] * U_old[j] +
+                        f.upml[j] * dt *
+                            (f.dual_curl[j] * (F[abs_ct_vec[0]] - F[abs_ct_vec[1]] + F[abs_ct_vec[2]] - F[abs_ct_vec[3]]) -
+                             I[j]));
+            } else {
+                const int32_t* c = f.ct_face + f.ct_ptr[j];
+                U[j] = std::pow(f.upml[j], 2) * f.M_h[j] *
+                       (f.M_q[j] * U_old[j] +
+                        f.upml[j] * dt * (f.dual_curl[j] * (F[c[0]] - F[c[1]] + F[c[2]] - F[c[3]]) - I[j]));
+            }
+        }
+        for (int64_t q = 0; q < f.n_uncommon; ++q) {
+            const int32_t j = f.uncommon_edges[q];
+            const int64_t k = st.by_edge[j];
+            curl_fb[j] = k >= 0 ? h_source(st, k, t - 0.5 * dt) : 0.0;
+            double val[3] = {0, 0, 0}, sg[3] = {0, 0, 0};
+            int m = 0;
+            for (int64_t r = f.ct_ptr[j]; r < f.ct_ptr[j + 1] && m < 3; ++r, ++m) {
+                val[m] = F[f.ct_face[r]];
+                sg[m] = f.ct_sgn[r] < 0 ? -1 : 1;
+            }
+            const double d = val[0] * sg[0] + val[1] * sg[1] + val[2] * sg[2];
+            U[j] = f.M_h[j] * (f.M_q[j] * U_old[j] + dt * (d + curl_fb[j] - I[j]));
+        }
+        double poy = 0;
+        for (int32_t e = 0; e < E; ++e) poy += (U_old[e] + U[e]) * curl_fb[e];
+        sink.poy(0.5 * poy);
+        F_old = F;
+        for (int64_t q = 0; q < f.n_tbc; ++q) {
+            const int32_t j = f.tbc_faces[q];
+            const int32_t* c = f.c_edge + 4 * (int64_t)j;
+            B[j] = f.M_mu[j] * (f.fpml[j] * F_old[j]) - dt * f.curl[j] * (U[c[0]] - U[c[1]] + U[c[2]] - U[c[3]]);
+            F[j] = f.fpml[j] * f.M_nu[j] * B[j];
+        }
+        loop_s += now_s() - t0;
+        if (pr.allow(t)) {
+            export_fit(U, B, pg, np, six);
+            sink.put(t, six);
+        }
+    }
+    out->seconds = loop_s;
+    if (out->U) std::memcpy(out->U, U.data(), sizeof(double) * E);
+    if (out->F) std::memcpy(out->F, F.data(), sizeof(double) * Fn);
+    if (out->B) std::memcpy(out->B, B.data(), sizeof(double) * Fn);
+}
+
+// y = M x over CSR rows, sequential in increasing column order
+void csr_mv(const tf_csr& M, const std::vector<double>& x, std::vector<double>& y, double scale_vals = 1.0) {
+    y.assign(M.nrows, 0.0);
+    for (int32_t r = 0; r < M.nrows; ++r) {
+        double s = 0;
+        for (int64_t q = M.rowptr[r]; q < M.rowptr[r + 1]; ++q) {
+            const double v = M.val ? M.val[q] : (double)M.sgn[q];
+            s += (scale_vals == 1.0 ? v : scale_vals * v) * x[M.col[q]];
+        }
+        y[r] = s;
+    }
+}
+
+// ---------------------------------------------------------------- dga
+// D:538-939 for sigma = chi = 0 (free edges class 1, faces class 1)
+void run_dga(const tf_problem* p, const SourceTable& st, const tf_probe_geom* pg, int32_t np, Printer pr,
+             int64_t first, int64_t n, const double* U0, const double* F0, const double* B0, oracle_out* out) {
+    const tf_dga_ops& d = p->dga;
+    const int32_t E = p->n_edges, Fn = p->n_faces;
+    const double dt = p->dt;
+    std::vector<double> U(E, 0.0), F(Fn, 0.0), B(Fn, 0.0), I(E, 0.0);
+    std::vector<double> U_a(d.H.nrows, 0.0), U_d(E, 0.0);  // U_d indexed by edge for class-4
+    if (U0) U.assign(U0, U0 + E);
+    if (F0) F.assign(F0, F0 + Fn);
+    if (B0) B.assign(B0, B0 + Fn);
+    for (int32_t e = 0; e < E; ++e) {
+        if (d.h_row[e] >= 0) U_a[d.h_row[e]] = U[e];
+        if (d.edge_class[e] == 4) U_d[e] = d.fixed_value[e];  // ComputeEdgeBC(ee,0)
+    }
+    // initial e-sources at t = 0 (D:762-790)
+    for (int64_t k = 0; k < (int64_t)st.se.size(); ++k)
+        if (st.se[k].field == TF_FIELD_E && d.edge_class[st.se[k].edge] == 4) U_d[st.se[k].edge] = e_source(st, k, 0.0);
+    auto assemble_U = [&]() {  // U = M*U_frac
+        for (int32_t e = 0; e < E; ++e) {
+            if (d.h_row[e] >= 0) U[e] = U_a[d.h_row[e]];
+            else if (d.edge_class[e] == 4) U[e] = U_d[e];
+        }
+    };
+    assemble_U();
+    Sink sink{out, np};
+    std::vector<double> six, U_old, curl_b(E, 0.0), cf, hc, cu, nc;
+    if (first == 1 && pr.allow(0.0)) {  // initial export at t = 0 (D:805-806, A-6)
+        export_tet(U, B, pg, np, six);
+        sink.put(0.0, six);
+    }
+    double loop_s = 0;
+    for (int64_t i = first; i < first + n; ++i) {
+        const double t0 = now_s();
+        const double t = (double)i * dt;
+        std::fill(curl_b.begin(), curl_b.end(), 0.0);  // Fb = 0
+        U_old = U;
+        for (int32_t e = 0; e < E; ++e)
+            if (d.edge_class[e] == 4 && st.by_edge[e] < 0) U_d[e] = t < 0 ? 0.0 : d.fixed_value[e];
+        for (int64_t k = 0; k < (int64_t)st.se.size(); ++k) {
+            const tf_source_edge& s = st.se[k];
+            if (s.field == TF_FIELD_H) curl_b[s.edge] = h_source(st, k, t - 0.5 * dt);  // Ctb*Fb
+            else if (s.field == TF_FIELD_E) U_d[s.edge] = e_source(st, k, t);
+        }
+        // curl_f = C^T F + curl_b - I
+        csr_mv(d.Ct, F, cf);
+        for (int32_t e = 0; e < E; ++e) cf[e] = cf[e] + curl_b[e] - I[e];
+        // U_a = U_a + dt*(H*curl_f)
+        csr_mv(d.H, cf, hc);
+        for (int32_t r = 0; r < d.H.nrows; ++r) U_a[r] = U_a[r] + dt * hc[r];
+        assemble_U();
+        double poy = 0;
+        for (int32_t e = 0; e < E; ++e) poy += (U_old[e] + U[e]) * curl_b[e];
+        sink.poy(0.5 * poy);
+        // curl_u = C U ; B -= dt curl_u ; F_a -= (dt N) curl_u
+        csr_mv(d.C, U, cu);
+        for (int32_t f = 0; f < Fn; ++f) B[f] = B[f] - dt * cu[f];
+        csr_mv(d.N, cu, nc, dt);
+        for (int32_t f = 0; f < Fn; ++f) F[f] = F[f] - nc[f];
+        loop_s += now_s() - t0;
+        if (pr.allow(t)) {
+            export_tet(U, B, pg, np, six);
+            sink.put(t, six);
+        }
+    }
+    out->seconds = loop_s;
+    if (out->U) std::memcpy(out->U, U.data(), sizeof(double) * E);
+    if (out->F) std::memcpy(out->F, F.data(), sizeof(double) * Fn);
+    if (out->B) std::memcpy(out->B, B.data(), sizeof(double) * Fn);
+}
+
+// ConjugateGradientSolver, ConjugateGradientSolver.hpp:52-128
+struct Cg {
+    int32_t n;
+    const int64_t* ptr;
+    const int32_t* col;
+    const double* val;
+    std::vector<double> pre;
+    double tol;
+    int32_t maxiter;
+    int32_t iter = 0;
+    double final_error = 0;
+    void compute() {
+        pre.assign(n, 0.0);
+        for (int32_t j = 0; j < n; ++j) {
+            double dj = 0;
+            bool found = false;
+            for (int64_t q = ptr[j]; q < ptr[j + 1]; ++q)
+                if (col[q] == j) { dj = val[q]; found = true; break; }
+            pre[j] = (found && dj != 0) ? 1.0 / dj : 1.0;
+        }
+    }
+    void Ax(const std::vector<double>& x, std::vector<double>& y) const {
+        for (int32_t i = 0; i < n; ++i) {
+            double s = 0;
+            for (int64_t q = ptr[i]; q < ptr[i + 1]; ++q) s += val[q] * x[col[q]];
+            y[i] = s;
+        }
+    }
+    static double dotp(const std::vector<double>& a, const std::vector<double>& b) {
+        double s = 0;
+        for (size_t i = 0; i < a.size(); ++i) s += a[i] * b[i];
+        return s;
+    }
+    std::vector<double> solve(const std::vector<double>& b, const std::vector<double>& x0) {
+        double rhoA = 0, rho_p;
+        iter = 0;
+        std::vector<double> r(n), z(n), p(n), q(n), x = x0, ax(n);
+        Ax(x, ax);
+        for (int32_t i = 0; i < n; ++i) r[i] = b[i] - ax[i];
+        const double residual0 = std::sqrt(dotp(r, r));
+        double residual = residual0;
+        while (residual / residual0 > tol && iter < maxiter) {
+            for (int32_t i = 0; i < n; ++i) z[i] = pre[i] * r[i];
+            rho_p = rhoA;
+            rhoA = dotp(r, z);
+            if (iter == 0) p = z;
+            else {
+                const double beta = rhoA / rho_p;
+                for (int32_t i = 0; i < n; ++i) p[i] = z[i] + beta * p[i];
+            }
+            Ax(p, q);
+            const double a = rhoA / dotp(p, q);
+            for (int32_t i = 0; i < n; ++i) { x[i] = x[i] + a * p[i]; r[i] = r[i] - a * q[i]; }
+            residual = std::sqrt(dotp(r, r));
+            iter++;
+        }
+        final_error = residual / residual0;
+        return x;
+    }
+};
+
+// ---------------------------------------------------------------- gfdtd
+// D:1360-1565 for sigma = 0 and no j sources
+void run_gfdtd(const tf_problem* p, const SourceTable& st, const tf_probe_geom* pg, int32_t np, Printer pr,
+               int64_t first, int64_t n, double tol, int32_t maxiter, const double* U0, const double* F0,
+               const double* B0, oracle_out* out) {
+    const tf_dga_ops& d = p->dga;
+    const int32_t E = p->n_edges, Fn = p->n_faces, nf = d.n_free;
+    const double dt = p->dt;
+    std::vector<double> U(E, 0.0), F(Fn, 0.0), B(Fn, 0.0), I(E, 0.0), SrcFld(E, 0.0), Fb(E, 0.0), Fb_old;
+    if (U0) U.assign(U0, U0 + E);
+    if (F0) F.assign(F0, F0 + Fn);
+    if (B0) B.assign(B0, B0 + Fn);
+    std::vector<uint8_t> is_free(E, 0);
+    for (int32_t k = 0; k < nf; ++k) is_free[d.free_edges[k]] = 1;
+    // Dirichlet edges: class-4 values (ComputeEdgeBC), e sources
+    auto bc_pass = [&](double t, bool h_too) {
+        for (int32_t e = 0; e < E; ++e)
+            if (!is_free[e] && st.by_edge[e] < 0) { U[e] = t < 0 ? 0.0 : d.fixed_value[e]; SrcFld[e] = U[e]; }
+        for (int64_t k = 0; k < (int64_t)st.se.size(); ++k) {
+            const tf_source_edge& s = st.se[k];
+            if (s.field == TF_FIELD_H) {
+                if (h_too) Fb[s.edge] = h_source(st, k, t - 0.5 * dt);
+            } else if (s.field == TF_FIELD_E) {
+                U[s.edge] = e_source(st, k, t);
+                SrcFld[s.edge] = U[s.edge];
+            }
+        }
+    };
+    bc_pass(0.0, false);  // D:1431-1453
+    std::vector<double> U_old = U;
+    Cg cg{nf, d.A.rowptr, d.A.col, d.A.val, {}, tol, maxiter};
+    cg.compute();
+    std::vector<double> solution(nf, 0.0), rhs(nf, 0.0), cu, eu, r1, ctf;
+    for (int32_t k = 0; k < nf; ++k) solution[k] = U[d.free_edges[k]];
+    Sink sink{out, np};
+    std::vector<double> six;
+    double loop_s = 0;
+    for (int64_t i = first; i < first + n; ++i) {
+        double t0 = now_s();
+        const double t = (double)i * dt;
+        csr_mv(d.C, U, cu);
+        for (int32_t f = 0; f < Fn; ++f) B[f] -= dt * cu[f];
+        csr_mv(d.N, B, F);
+        loop_s += now_s() - t0;
+        if (pr.allow(t - dt)) {
+            export_tet(U, B, pg, np, six);
+            sink.put(t - dt, six);
+        }
+        t0 = now_s();
+        Fb_old = Fb;
+        bc_pass(t, true);
+        // Psi = E U_old - RHSmat1 SrcFld + dt C^T F + dt Ctb Fb  (D:1516)
+        csr_mv(d.E, U_old, eu);
+        csr_mv(d.RHS1, SrcFld, r1);
+        csr_mv(d.Ct, F, ctf);
+        double poy = 0;
+        for (int32_t e = 0; e < E; ++e) poy += U_old[e] * (0.5 * (Fb[e] + Fb_old[e]));
+        sink.poy(poy);
+        for (int32_t k = 0; k < nf; ++k) {
+            const int32_t e = d.free_edges[k];
+            rhs[k] = (eu[e] - r1[e]) + dt * ctf[e] + dt * Fb[e];
+        }
+        solution = cg.solve(rhs, solution);
+        out->cg_iters_total += cg.iter;
+        if (cg.iter > out->cg_iters_max) out->cg_iters_max = cg.iter;
+        if (std::isfinite(cg.final_error) && cg.final_error > out->cg_max_relres) out->cg_max_relres = cg.final_error;
+        if (cg.iter >= maxiter && cg.final_error > tol) out->cg_not_converged++;
+        for (int32_t k = 0; k < nf; ++k) U[d.free_edges[k]] = solution[k];
+        U_old = U;
+        loop_s += now_s() - t0;
+    }
+    out->seconds = loop_s;
+    if (out->U) std::memcpy(out->U, U.data(), sizeof(double) * E);
+    if (out->F) std::memcpy(out->F, F.data(), sizeof(double) * Fn);
+    if (out->B) std::memcpy(out->B, B.data(), sizeof(double) * Fn);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* oracle_last_error(void) { return g_err.c_str(); }
+
+void oracle_source_compute(const tf_source_desc* s, const double p[3], double t, double out3[3]) {
+    V3 v = source_compute(*s, p, t);
+    out3[0] = v.x;
+    out3[1] = v.y;
+    out3[2] = v.z;
+}
+
+int32_t oracle_cg_solve(int32_t n, const int64_t* rowptr, const int32_t* col, const double* val, const double* b,
+                        const double* x0, double tol, int32_t maxiter, double* x, double* err) {
+    Cg cg{n, rowptr, col, val, {}, tol, maxiter};
+    cg.compute();
+    std::vector<double> bb(b, b + n), xx(x0, x0 + n);
+    std::vector<double> r = cg.solve(bb, xx);
+    std::memcpy(x, r.data(), sizeof(double) * n);
+    if (err) *err = cg.final_error;
+    return cg.iter;
+}
+
+int32_t oracle_run(const tf_problem* p, const tf_source_desc* sd, int32_t n_sd, const tf_source_edge* se,
+                   int64_t n_se, const tf_probe_geom* pg, int32_t n_probes, int32_t out_enabled, double out_delay,
+                   double out_period, int64_t first, int64_t n, int32_t faithful, double cg_tol, int32_t cg_maxiter,
+                   const double* U0, const double* F0, const double* B0, oracle_out* out) {
+    try {
+        g_err.clear();
+        out->n_samples = 0;
+        out->n_poynting = 0;
+        out->cg_iters_total = 0;
+        out->cg_iters_max = 0;
+        out->cg_not_converged = 0;
+        out->cg_max_relres = 0;
+        SourceTable st;
+        st.sd.assign(sd, sd + n_sd);
+        st.se.assign(se, se + n_se);
+        st.by_edge.assign(p->n_edges, -1);
+        for (int64_t k = 0; k < n_se; ++k) st.by_edge[se[k].edge] = k;
+        Printer pr{out_enabled != 0, out_delay, out_period};
+        if (p->method == TF_METHOD_FDTD)
+            run_fdtd(p, st, pg, n_probes, pr, first, n, faithful != 0, U0, F0, B0, out);
+        else if (p->method == TF_METHOD_DGA)
+            run_dga(p, st, pg, n_probes, pr, first, n, U0, F0, B0, out);
+        else
+            run_gfdtd(p, st, pg, n_probes, pr, first, n, cg_tol, cg_maxiter, U0, F0, B0, out);
+        return 0;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return -1;
+    }
+}
+
+}  // extern "C"

@@ -1,0 +1,64 @@
+"""Device path vs the CPU checker on the same assembled operators.
+
+Tolerance (fp64 plain path): per probe and component,
+max_t |x_gpu - x_oracle| <= 1e-10 * max_t |x_oracle|; final fields within
+1e-10 relative L2. gfdtd additionally checks that every CG solve meets the
+relative-residual tolerance and that iteration counts match the checker's
+within +-1 per step on average.
+"""
+import numpy as np
+import pytest
+
+from tetfit_amd import Model, configs
+from tetfit_amd.device import Simulation
+
+import pyoracle
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-10
+
+
+def per_probe_err(got, ref):
+    scale = np.maximum(np.abs(ref).max(axis=0), 1e-300)
+    return (np.abs(got - ref).max(axis=0) / scale).max()
+
+
+def rel_l2(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def check_run(b, n=None, **kw):
+    print(f"case: {b.method} E={b.n_edges} F={b.n_faces} steps={b.n_steps}", flush=True)
+    sim = Simulation(b, "fp64")
+    st = sim.run(n=n, **kw)
+    t_g, s_g = sim.probes()
+    U, F, B = sim.state()
+    poy = sim.poynting()
+    sim.close()
+    r = pyoracle.run(b, n=n)
+    assert len(t_g) == len(r["times"])
+    np.testing.assert_allclose(t_g, r["times"], rtol=0, atol=0)
+    assert per_probe_err(s_g, r["samples"]) <= TOL
+    for got, ref in ((U, r["U"]), (F, r["F"]), (B, r["B"])):
+        assert rel_l2(got, ref) <= TOL
+    np.testing.assert_allclose(poy, r["poynting"], rtol=1e-9, atol=1e-12 * max(np.abs(r["poynting"]).max(), 1))
+    return st, r
+
+
+def test_fit_parity(workdir):
+    b = Model(configs.fit_cavity(str(workdir), n=12, step=0.1, n_steps=200, n_probes=6, name="fit12")).build(1)
+    check_run(b)
+
+
+def test_dga_parity(workdir):
+    b = Model(configs.kuhn_cavity(str(workdir), n=6, method="dga", n_steps=200, n_probes=4, block=2)).build(1)
+    check_run(b)
+
+
+def test_gfdtd_parity(workdir):
+    b = Model(configs.kuhn_cavity(str(workdir), n=5, method="gfdtd", n_steps=60, n_probes=4, block=2)).build(1)
+    st, r = check_run(b)
+    assert st["cg_max_relres"] <= b.cg_tol
+    assert st["cg_not_converged"] == 0
+    assert abs(st["cg_iters_total"] - r["cg_iters_total"]) <= b.n_steps

@@ -1,0 +1,137 @@
+"""Writers of .fdtd inputs (and their mesh / solids files) for the synthetic
+workloads this project measures and tests:
+
+* fit_cavity   - FIT/FDTD unit box, PEC on 5 walls, wave-profile h source on
+                 x-min (the 64^3 "0.0156 m step" workload and smaller ones)
+* kuhn_cavity  - jittered Kuhn tetrahedral mesh of the unit cube with
+                 piecewise-constant eps per block, PEC on 5 walls, the same
+                 h source, method dga or gfdtd (the 48^3 / 96^3 workloads)
+
+Every input pins the time step with `SET step` below the host's stability
+estimate, so the reference and this build agree on dt.
+"""
+import math
+import os
+
+import numpy as np
+
+from .host import Model, write_kuhn_mesh
+
+C0 = 299792458.0
+
+SOURCE_BLOCK = """DEFINE source 1
+    SET type h
+    SET surface 2
+    SET profile wave
+    SET direction y
+    SET mode {{cos,cos,sin}}
+    SET wavevector {{0,0,0.5}}
+    SET carrier {carrier}
+    SET frequency {freq}
+    SET amplitude 1
+{extra}END source 1
+"""
+
+
+def _probe_lines(points):
+    return "".join("    SET probe {%.6g,%.6g,%.6g}\n" % tuple(p) for p in points)
+
+
+def _duration(n_steps, dt, first_step):
+    # fdtd loops i = 0..floor(T/dt) (n+1 trips), dga/gfdtd i = 1..floor(T/dt)
+    k = n_steps - 1 if first_step == 0 else n_steps
+    return (k + 0.5) * dt
+
+
+def fit_cavity(directory, n=64, step=0.0156, n_steps=2000, n_probes=32, dt=None,
+               freq=2e8, carrier="sin", name="fit", eps_r=1.0, sigma=0.0, extra_source=""):
+    """FIT box [0, n*step]^3 (xgrid {0,step,n*step+tiny}), one box solid of
+    material 1. Returns the .fdtd path."""
+    os.makedirs(directory, exist_ok=True)
+    stop = 1.0 if abs(n * step - 0.9984) < 1e-9 else n * step + 0.25 * step
+    solids = os.path.join(directory, f"{name}_solids.txt")
+    with open(solids, "w") as f:
+        f.write("DEFINE solid 1\n    SET type box\n    SET material 1\n"
+                "    SET corner {0,0,0}\n    SET size {%r,%r,%r}\nEND solid 1\n" % (stop, stop, stop))
+    if dt is None:
+        # 0.95 x the Yee limit for vacuum (or the slowest-material-free bound)
+        dt = 0.95 * step / (C0 * math.sqrt(3.0))
+        dt = float("%.4g" % dt)
+    h = n * step
+    # probes on a line of cell centres along x through the middle of the box
+    j = n // 2
+    xs = [(2 * i * (n // (2 * n_probes)) + 0.5) * step if n >= 2 * n_probes else (i % n + 0.5) * step
+          for i in range(n_probes)]
+    pts = [(x, (j + 0.5) * step, (j + 0.5) * step) for x in xs]
+    fdtd = os.path.join(directory, f"{name}.fdtd")
+    mat = "DEFINE material 1\n    SET epsilon %r\n    SET mu 1\n" % eps_r
+    if sigma:
+        mat += "    SET sigma %r\n" % sigma
+    mat += "END material 1\n"
+    with open(fdtd, "w") as f:
+        f.write(f"# FIT cavity {n}^3, step {step}\n")
+        f.write("DEFINE mesh 1\n    SET type cartesian\n    SET file %s\n" % os.path.basename(solids))
+        for ax in "xyz":
+            f.write("    SET %sgrid {0,%r,%r}\n" % (ax, step, stop))
+        f.write("END mesh 1\n")
+        f.write(mat)
+        f.write("DEFINE bc 1\n    SET type pec\n    SET surfaces {1,3,4,5,6}\nEND bc 1\n")
+        f.write(SOURCE_BLOCK.format(carrier=carrier, freq=repr(freq), extra=extra_source))
+        f.write("DEFINE output 1\n    SET mode probepoint\n    SET name %s\n    SET period 0\n" % name)
+        f.write(_probe_lines(pts))
+        f.write("END output 1\n")
+        f.write("DEFINE simulation 1\n    SET method fdtd\n    SET mesh 1\n    SET output 1\n"
+                "    SET courant 0.98\n    SET step %r\n    SET duration %r\nEND simulation 1\n"
+                % (dt, _duration(n_steps, dt, 0)))
+    return fdtd
+
+
+def kuhn_cavity(directory, n=48, method="dga", n_steps=1000, n_probes=8, jitter=0.2, seed=2,
+                block=8, eps_range=(1.0, 10.0), log_eps=False, freq=2e8, tol=1e-8, dt=None,
+                name=None, dt_safety=0.9):
+    """Kuhn mesh of the unit cube, eps_r per block drawn from eps_range
+    (uniform, or log-uniform) with numpy default_rng(seed). Returns the .fdtd
+    path. When dt is None the host's stability estimate is computed once and
+    dt = dt_safety x estimate (rounded to 4 digits) is pinned with SET step."""
+    os.makedirs(directory, exist_ok=True)
+    name = name or f"kuhn{n}_{method}"
+    mesh = os.path.join(directory, f"kuhn{n}_s{seed}_j{jitter}_b{block}.mesh")
+    if not os.path.exists(mesh):
+        max_label, _ = write_kuhn_mesh(mesh, n, jitter, seed, block)
+    else:
+        nb = -(-n // block)
+        max_label = nb ** 3 + (1 if nb ** 3 >= 7 else 0)
+    rng = np.random.default_rng(seed)
+    labels = [l for l in range(1, max_label + 1) if l != 7]
+    lo, hi = eps_range
+    if log_eps:
+        eps = np.exp(rng.uniform(math.log(lo), math.log(hi), size=len(labels)))
+    else:
+        eps = rng.uniform(lo, hi, size=len(labels))
+    h = 1.0 / n
+    pts = [((i + 0.37) / n_probes, 0.5 + 0.11 * h, 0.5 + 0.07 * h) for i in range(n_probes)]
+
+    def write(step_line):
+        fdtd = os.path.join(directory, f"{name}.fdtd")
+        with open(fdtd, "w") as f:
+            f.write(f"# Kuhn {n}^3 cavity, jitter {jitter}h, seed {seed}, method {method}\n")
+            f.write("DEFINE mesh 1\n    SET type tetrahedral\n    SET mesher netgen\n"
+                    "    SET file %s\nEND mesh 1\n" % os.path.basename(mesh))
+            for l, e in zip(labels, eps):
+                f.write("DEFINE material %d\n    SET epsilon %r\n    SET mu 1\nEND material %d\n" % (l, float(e), l))
+            f.write("DEFINE bc 1\n    SET type pec\n    SET surfaces {1,3,4,5,6}\nEND bc 1\n")
+            f.write(SOURCE_BLOCK.format(carrier="sin", freq=repr(freq), extra=""))
+            f.write("DEFINE output 1\n    SET mode probepoint\n    SET name %s\n    SET period 0\n" % name)
+            f.write(_probe_lines(pts))
+            f.write("END output 1\n")
+            f.write("DEFINE simulation 1\n    SET method %s\n    SET mesh 1\n    SET output 1\n"
+                    "    SET courant 1\n%s    SET solver cg\n    SET tolerance %r\nEND simulation 1\n"
+                    % (method, step_line, tol))
+        return fdtd
+
+    if dt is None:
+        probe = write("    SET duration 0\n")
+        b = Model(probe).build(1)
+        dt = float("%.4g" % (dt_safety * b.stable_dt))
+        del b
+    return write("    SET step %r\n    SET duration %r\n" % (dt, _duration(n_steps, dt, 1)))

@@ -1,0 +1,81 @@
+// common.hpp - HIP error handling, device buffers and wave64 helpers shared
+// by the stepping kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace tfd {
+
+struct HipError : std::runtime_error {
+    int code;
+    HipError(const std::string& s, int c) : std::runtime_error(s), code(c) {}
+};
+
+#define TF_HIP(call)                                                                        \
+    do {                                                                                    \
+        hipError_t e_ = (call);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            throw ::tfd::HipError(std::string(#call) + ": " + hipGetErrorString(e_),        \
+                                  e_ == hipErrorOutOfMemory ? 3 : 2);                      \
+    } while (0)
+
+// Owning device allocation.
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    void alloc(size_t n) {
+        release();
+        if (n == 0) return;
+        TF_HIP(hipMalloc(&p, n));
+        bytes = n;
+    }
+    template <class T>
+    T* as() const { return static_cast<T*>(p); }
+};
+
+constexpr int kWave = 64;
+constexpr int kBlock = 256;
+
+inline int blocks_for(int64_t n, int per_block = kBlock) {
+    int64_t b = (n + per_block - 1) / per_block;
+    return (int)(b < 1 ? 1 : b);
+}
+
+// Sum over `width` consecutive lanes (width a power of two <= 64).
+template <int W>
+__device__ __forceinline__ double group_sum(double v) {
+#pragma unroll
+    for (int o = W / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, W);
+    return v;
+}
+
+// Block-wide fp64 sum of one value per thread; result valid in thread 0.
+__device__ __forceinline__ double block_sum(double v, double* smem) {
+    v = group_sum<kWave>(v);
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+    if (lane == 0) smem[w] = v;
+    __syncthreads();
+    double r = 0;
+    if (threadIdx.x == 0) {
+        const int nw = (blockDim.x + kWave - 1) / kWave;
+        for (int i = 0; i < nw; ++i) r += smem[i];
+    }
+    __syncthreads();
+    return r;
+}
+
+}  // namespace tfd

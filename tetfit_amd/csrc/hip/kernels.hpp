@@ -1,0 +1,488 @@
+// kernels.hpp - gfx950 kernels of the three leapfrog loops.
+//
+// Field and operator values are of type T (double on the plain path, float on
+// the fast path); every reduction accumulates in double. Incidence is stored
+// as int32 indices plus signs: 2-bit codes packed four to a byte
+// (0 = absent, 1 = +1, 3 = -1) for the fixed-width FIT stencils, int8 per
+// entry for the CSR incidence of the tetrahedral meshes.
+#pragma once
+
+#include "common.hpp"
+
+namespace tfd {
+
+__device__ __forceinline__ double sgn2(uint32_t code) {
+    // 0 -> 0, 1 -> +1, 3 -> -1
+    return code == 1 ? 1.0 : (code == 3 ? -1.0 : 0.0);
+}
+
+// Value injected at a source edge for the current step: sum of its terms of
+// carrier(source, step) * weight.
+__device__ __forceinline__ double source_value(int tb, int te, const int32_t* __restrict__ term_src,
+                                               const double* __restrict__ term_w,
+                                               const double* __restrict__ carr_step) {
+    double v = 0;
+    for (int t = tb; t < te; ++t) v += carr_step[term_src[t]] * term_w[t];
+    return v;
+}
+
+// ------------------------------------------------------------------ FIT
+//
+// E phase (D:1850-1895). Main blocks cover every edge in index order; edges
+// whose sign byte is 0 (pec / source / non-common) are left to the special
+// blocks that follow. U = ca*U_old + cb*(sum_k s_k F[f_k])
+//   common:   ca = upml^2 M_h M_q, cb = upml^3 M_h dt, s = dual_curl*(+,-,+,-)
+// Special items (uncommon h-source edges, e/j source edges):
+//   kind 0 (h):  U = ca*U_old + cb*(sum s_k F + src)     (ca = M_h M_q, cb = M_h dt)
+//   kind 1 (e):  U = src
+//   kind 2 (j):  U = ca*U_old + cb*(sum s_k F - src)     (common coefficients)
+template <typename T>
+__global__ void __launch_bounds__(kBlock) fit_e_kernel(
+    int32_t n_edges, int32_t main_blocks, const uint8_t* __restrict__ sg, const T* __restrict__ ca,
+    const T* __restrict__ cb, const int4* __restrict__ f4, const T* __restrict__ Fv, T* __restrict__ U,
+    int32_t n_sp, const int32_t* __restrict__ sp_edge, const int4* __restrict__ sp_f4,
+    const uint8_t* __restrict__ sp_sg, const double* __restrict__ sp_ca, const double* __restrict__ sp_cb,
+    const int8_t* __restrict__ sp_kind, const int32_t* __restrict__ sp_tb, const int32_t* __restrict__ sp_te,
+    const int32_t* __restrict__ term_src, const double* __restrict__ term_w,
+    const double* __restrict__ carr_step, double* __restrict__ poy_part) {
+    if ((int)blockIdx.x < main_blocks) {
+        const int32_t e = blockIdx.x * kBlock + threadIdx.x;
+        if (e >= n_edges) return;
+        const uint32_t s = sg[e];
+        if (s == 0) return;
+        const int4 f = f4[e];
+        double c = sgn2(s & 3) * (double)Fv[f.x] + sgn2((s >> 2) & 3) * (double)Fv[f.y] +
+                   sgn2((s >> 4) & 3) * (double)Fv[f.z] + sgn2((s >> 6) & 3) * (double)Fv[f.w];
+        U[e] = (T)((double)ca[e] * (double)U[e] + (double)cb[e] * c);
+        return;
+    }
+    __shared__ double red[kBlock / kWave];
+    const int32_t i = (blockIdx.x - main_blocks) * kBlock + threadIdx.x;
+    double poy = 0;
+    if (i < n_sp) {
+        const int32_t e = sp_edge[i];
+        const double src = source_value(sp_tb[i], sp_te[i], term_src, term_w, carr_step);
+        const int k = sp_kind[i];
+        const double uold = (double)U[e];
+        double unew;
+        if (k == 1) {
+            unew = src;
+        } else {
+            const int4 f = sp_f4[i];
+            const uint32_t s = sp_sg[i];
+            double c = 0;
+            if (s & 3) c += sgn2(s & 3) * (double)Fv[f.x];
+            if ((s >> 2) & 3) c += sgn2((s >> 2) & 3) * (double)Fv[f.y];
+            if ((s >> 4) & 3) c += sgn2((s >> 4) & 3) * (double)Fv[f.z];
+            if ((s >> 6) & 3) c += sgn2((s >> 6) & 3) * (double)Fv[f.w];
+            c += k == 0 ? src : -src;
+            unew = sp_ca[i] * uold + sp_cb[i] * c;
+        }
+        U[e] = (T)unew;
+        // Poynting term 0.5*(U_old+U)*curl_fb, nonzero on h-source edges only (D:1901)
+        if (k == 0) poy = 0.5 * (uold + (double)(T)unew) * src;
+    }
+    const double s = block_sum(poy, red);
+    if (threadIdx.x == 0) poy_part[blockIdx.x - main_blocks] = s;
+}
+
+// H phase over tbc faces (D:1908-1915): B = cm*F_old - dt*sum s_k U[e_k];
+// F = cn*B with cm = M_mu*fpml, cn = fpml*M_nu, s = curl*(+,-,+,-).
+template <typename T>
+__global__ void __launch_bounds__(kBlock) fit_h_kernel(int32_t n_faces, const uint8_t* __restrict__ sg,
+                                                       const T* __restrict__ cm, const T* __restrict__ cn,
+                                                       const int4* __restrict__ e4, const T* __restrict__ U,
+                                                       T* __restrict__ Fv, T* __restrict__ B, double dt) {
+    const int32_t f = blockIdx.x * kBlock + threadIdx.x;
+    if (f >= n_faces) return;
+    const uint32_t s = sg[f];
+    if (s == 0) return;
+    const int4 e = e4[f];
+    double c = sgn2(s & 3) * (double)U[e.x] + sgn2((s >> 2) & 3) * (double)U[e.y] +
+               sgn2((s >> 4) & 3) * (double)U[e.z] + sgn2((s >> 6) & 3) * (double)U[e.w];
+    const double b = (double)cm[f] * (double)Fv[f] - dt * c;
+    B[f] = (T)b;
+    Fv[f] = (T)((double)cn[f] * b);
+}
+
+// ------------------------------------------------------------------ probes
+// One thread per (probe, component): E from U, H from B (D:2400-2470,
+// D:3093-3158). Output sample layout [probe][6].
+struct ProbeDev {
+    int32_t ne, nh;
+    int32_t e_idx[12];
+    double e_w[12][3];
+    int32_t h_idx[6];
+    double h_w[6][3];
+};
+
+template <typename T>
+__global__ void probe_kernel(int32_t n_probes, const ProbeDev* __restrict__ pr, const T* __restrict__ U,
+                             const T* __restrict__ B, double* __restrict__ out) {
+    const int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_probes * 6) return;
+    const int p = t / 6, c = t % 6;
+    const ProbeDev& q = pr[p];
+    double v = 0;
+    if (c < 3) {
+        for (int k = 0; k < q.ne; ++k) v += (double)U[q.e_idx[k]] * q.e_w[k][c];
+    } else {
+        for (int k = 0; k < q.nh; ++k) v += (double)B[q.h_idx[k]] * q.h_w[k][c - 3];
+    }
+    out[t] = v;
+}
+
+// ------------------------------------------------------------------ DGA
+// Gather SpMV rows of an incidence CSR: y[r] = sum s*x[col], LPR lanes per row.
+template <typename T, int LPR>
+__global__ void __launch_bounds__(kBlock) incidence_rows_kernel(int32_t nrows, const int32_t* __restrict__ ptr,
+                                                                const int32_t* __restrict__ col,
+                                                                const int8_t* __restrict__ sg,
+                                                                const T* __restrict__ x, T* __restrict__ y) {
+    const int32_t g = blockIdx.x * kBlock + threadIdx.x;
+    const int32_t r = g / LPR, lane = g % LPR;
+    double s = 0;
+    if (r < nrows) {
+        for (int32_t k = ptr[r] + lane; k < ptr[r + 1]; k += LPR) s += (double)sg[k] * (double)x[col[k]];
+    }
+    s = group_sum<LPR>(s);
+    if (r < nrows && lane == 0) y[r] = (T)s;
+}
+
+// Source injection for the tetrahedral methods: h sources add Fb to curl_f
+// (Ctb*Fb, one +1 per boundary edge), e sources set U (class-4 edges).
+// The Poynting partial uses U before the E update: 0.5*U_old*fb.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) dga_inject_kernel(
+    int32_t n_inj, const int32_t* __restrict__ inj_edge, const int8_t* __restrict__ inj_kind,
+    const int32_t* __restrict__ inj_tb, const int32_t* __restrict__ inj_te, const int32_t* __restrict__ term_src,
+    const double* __restrict__ term_w, const double* __restrict__ carr_step, T* __restrict__ curl_f,
+    T* __restrict__ U, double* __restrict__ poy_part) {
+    __shared__ double red[kBlock / kWave];
+    const int32_t i = blockIdx.x * kBlock + threadIdx.x;
+    double poy = 0;
+    if (i < n_inj) {
+        const int32_t e = inj_edge[i];
+        const double v = source_value(inj_tb[i], inj_te[i], term_src, term_w, carr_step);
+        if (inj_kind[i] == 0) {
+            curl_f[e] = (T)((double)curl_f[e] + v);
+            poy = 0.5 * (double)U[e] * v;
+        } else {
+            U[e] = (T)v;
+        }
+    }
+    const double s = block_sum(poy, red);
+    if (threadIdx.x == 0) poy_part[blockIdx.x] = s;
+}
+
+// Second Poynting partial after the E update: 0.5*U_new*fb on h-source edges.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) dga_poynting_kernel(
+    int32_t n_inj, const int32_t* __restrict__ inj_edge, const int8_t* __restrict__ inj_kind,
+    const int32_t* __restrict__ inj_tb, const int32_t* __restrict__ inj_te, const int32_t* __restrict__ term_src,
+    const double* __restrict__ term_w, const double* __restrict__ carr_step, const T* __restrict__ U,
+    double* __restrict__ poy_part) {
+    __shared__ double red[kBlock / kWave];
+    const int32_t i = blockIdx.x * kBlock + threadIdx.x;
+    double poy = 0;
+    if (i < n_inj && inj_kind[i] == 0) {
+        const double v = source_value(inj_tb[i], inj_te[i], term_src, term_w, carr_step);
+        poy = 0.5 * (double)U[inj_edge[i]] * v;
+    }
+    const double s = block_sum(poy, red);
+    if (threadIdx.x == 0) poy_part[blockIdx.x] = s;
+}
+
+// U[edge(r)] += dt * (H curl_f)[r] over the H rows (class-1 edges), LPR lanes per row.
+template <typename T, int LPR>
+__global__ void __launch_bounds__(kBlock) dga_h_kernel(int32_t nrows, const int32_t* __restrict__ ptr,
+                                                       const int32_t* __restrict__ col, const T* __restrict__ val,
+                                                       const int32_t* __restrict__ row_edge,
+                                                       const T* __restrict__ x, T* __restrict__ U, double dt) {
+    const int32_t g = blockIdx.x * kBlock + threadIdx.x;
+    const int32_t r = g / LPR, lane = g % LPR;
+    double s = 0;
+    if (r < nrows) {
+        for (int32_t k = ptr[r] + lane; k < ptr[r + 1]; k += LPR) s += (double)val[k] * (double)x[col[k]];
+    }
+    s = group_sum<LPR>(s);
+    if (r < nrows && lane == 0) {
+        const int32_t e = row_edge[r];
+        U[e] = (T)((double)U[e] + dt * s);
+    }
+}
+
+// curl_u = C U (3 signed edges per face), B -= dt*curl_u (D:902-910).
+template <typename T>
+__global__ void __launch_bounds__(kBlock) dga_curl_kernel(int32_t n_faces, const int32_t* __restrict__ c_col,
+                                                          const int8_t* __restrict__ c_sg, const T* __restrict__ U,
+                                                          T* __restrict__ curl_u, T* __restrict__ B, double dt) {
+    const int32_t f = blockIdx.x * kBlock + threadIdx.x;
+    if (f >= n_faces) return;
+    const int32_t* c = c_col + 3 * (int64_t)f;
+    const int8_t* s = c_sg + 3 * (int64_t)f;
+    const double cu = (double)s[0] * (double)U[c[0]] + (double)s[1] * (double)U[c[1]] + (double)s[2] * (double)U[c[2]];
+    curl_u[f] = (T)cu;
+    B[f] = (T)((double)B[f] - dt * cu);
+}
+
+// F -= dt * N curl_u (dga, D:917) or F = N B (gfdtd, D:1468); mode 0 / 1.
+template <typename T, int LPR>
+__global__ void __launch_bounds__(kBlock) dga_n_kernel(int32_t nrows, const int32_t* __restrict__ ptr,
+                                                       const int32_t* __restrict__ col, const T* __restrict__ val,
+                                                       const T* __restrict__ x, T* __restrict__ Fv, double dt,
+                                                       int mode) {
+    const int32_t g = blockIdx.x * kBlock + threadIdx.x;
+    const int32_t r = g / LPR, lane = g % LPR;
+    double s = 0;
+    if (r < nrows) {
+        for (int32_t k = ptr[r] + lane; k < ptr[r + 1]; k += LPR) s += (double)val[k] * (double)x[col[k]];
+    }
+    s = group_sum<LPR>(s);
+    if (r < nrows && lane == 0) Fv[r] = mode == 0 ? (T)((double)Fv[r] - dt * s) : (T)s;
+}
+
+// ------------------------------------------------------------------ gfdtd
+// CG scalar state on the device (ConjugateGradientSolver.hpp:52-103).
+struct CgState {
+    double rho, rho_new, pq, rr, rr0, alpha, beta, relres, tol;
+    int32_t iter, done, maxiter, pad;
+    uint32_t ticket_a, ticket_b, ticket_c, pad2;
+};
+
+// Last-block-done pattern: every block publishes its partial, the block that
+// takes the last ticket sums the partials in index order (deterministic).
+__device__ __forceinline__ bool last_block(uint32_t* ticket, uint32_t nblocks) {
+    __shared__ bool is_last;
+    __threadfence();  // release this block's partials (write back the XCD's L2)
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = atomicAdd(ticket, 1u);
+        is_last = (t == nblocks - 1);
+    }
+    __syncthreads();
+    if (is_last) __threadfence();  // acquire the other blocks' partials
+    return is_last;
+}
+
+__device__ __forceinline__ double sum_partials(const double* part, int n, double* red) {
+    double v = 0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) v += part[i];
+    return block_sum(v, red);
+}
+
+// rhs[k] = E U_old - RHS1 SrcFld + dt (Ct F + fb) at free edge e = free[k]
+// (D:1516-1523; sigma and I are zero on the supported inputs). SrcFld is U on
+// Dirichlet edges at the new time, held in `dir_new` (edge-indexed).
+template <typename T, int LPR>
+__global__ void __launch_bounds__(kBlock) gf_rhs_kernel(
+    int32_t n_free, const int32_t* __restrict__ free_edge, const int32_t* __restrict__ e_ptr,
+    const int32_t* __restrict__ e_col, const T* __restrict__ e_val, const int32_t* __restrict__ r_ptr,
+    const int32_t* __restrict__ r_col, const T* __restrict__ r_val, const int32_t* __restrict__ ct_ptr,
+    const int32_t* __restrict__ ct_col, const int8_t* __restrict__ ct_sg, const T* __restrict__ U,
+    const T* __restrict__ dir_new, const T* __restrict__ Fv, const T* __restrict__ fb, T* __restrict__ rhs,
+    double dt) {
+    const int32_t g = blockIdx.x * kBlock + threadIdx.x;
+    const int32_t k = g / LPR, lane = g % LPR;
+    double a = 0, c = 0;
+    int32_t e = 0;
+    if (k < n_free) {
+        e = free_edge[k];
+        for (int32_t q = e_ptr[e] + lane; q < e_ptr[e + 1]; q += LPR) a += (double)e_val[q] * (double)U[e_col[q]];
+        for (int32_t q = r_ptr[e] + lane; q < r_ptr[e + 1]; q += LPR) a -= (double)r_val[q] * (double)dir_new[r_col[q]];
+        for (int32_t q = ct_ptr[e] + lane; q < ct_ptr[e + 1]; q += LPR) c += (double)ct_sg[q] * (double)Fv[ct_col[q]];
+    }
+    a = group_sum<LPR>(a);
+    c = group_sum<LPR>(c);
+    if (k < n_free && lane == 0) rhs[k] = (T)(a + dt * (c + (double)fb[e]));
+}
+
+// r = b - A x, z = pre r; partials of r.r and r.z  (CG set-up)
+template <typename T, int LPR>
+__global__ void __launch_bounds__(kBlock) cg_init_kernel(int32_t n, const int32_t* __restrict__ ptr,
+                                                         const int32_t* __restrict__ col, const T* __restrict__ val,
+                                                         const T* __restrict__ pre, const T* __restrict__ b,
+                                                         const T* __restrict__ x, T* __restrict__ r, T* __restrict__ z,
+                                                         double* __restrict__ part, CgState* st, uint32_t nblocks) {
+    __shared__ double red[kBlock / kWave];
+    const int32_t g = blockIdx.x * kBlock + threadIdx.x;
+    const int32_t i = g / LPR, lane = g % LPR;
+    double s = 0;
+    if (i < n)
+        for (int32_t q = ptr[i] + lane; q < ptr[i + 1]; q += LPR) s += (double)val[q] * (double)x[col[q]];
+    s = group_sum<LPR>(s);
+    double rr = 0, rz = 0;
+    if (i < n && lane == 0) {
+        const T ri = (T)((double)b[i] - s);
+        const T zi = (T)((double)pre[i] * (double)ri);
+        r[i] = ri;
+        z[i] = zi;
+        rr = (double)ri * (double)ri;
+        rz = (double)ri * (double)zi;
+    }
+    rr = block_sum(rr, red);
+    rz = block_sum(rz, red);
+    if (threadIdx.x == 0) { part[blockIdx.x] = rr; part[nblocks + blockIdx.x] = rz; }
+    if (last_block(&st->ticket_a, nblocks)) {
+        const double RR = sum_partials(part, nblocks, red);
+        const double RZ = sum_partials(part + nblocks, nblocks, red);
+        if (threadIdx.x == 0) {
+            st->rr0 = RR;
+            st->rr = RR;
+            st->rho = RZ;
+            st->iter = 0;
+            // while (residual/residual0 > tol && iter < maxiter): 0/0 is NaN -> no iteration
+            const double ratio = std::sqrt(RR) / std::sqrt(RR);
+            st->relres = ratio;
+            st->done = !(ratio > st->tol && 0 < st->maxiter);
+            st->ticket_a = 0;
+        }
+    }
+}
+
+// p_new = z + beta p_old (p_new = z at iter 0), q = A p_new gathered with the
+// p update fused (p_old / p_new double buffers), partial p.q -> alpha.
+template <typename T, int LPR>
+__global__ void __launch_bounds__(kBlock) cg_a_kernel(int32_t n, const int32_t* __restrict__ ptr,
+                                                      const int32_t* __restrict__ col, const T* __restrict__ val,
+                                                      const T* __restrict__ z, T* __restrict__ p0, T* __restrict__ p1,
+                                                      T* __restrict__ q, double* __restrict__ part, CgState* st,
+                                                      uint32_t nblocks) {
+    if (st->done) return;
+    __shared__ double red[kBlock / kWave];
+    const int it = st->iter;
+    const double beta = it == 0 ? 0.0 : st->beta;
+    const T* pold = (it & 1) ? p1 : p0;
+    T* pnew = (it & 1) ? p0 : p1;
+    const int32_t g = blockIdx.x * kBlock + threadIdx.x;
+    const int32_t i = g / LPR, lane = g % LPR;
+    double s = 0;
+    if (i < n) {
+        if (it == 0) {
+            for (int32_t k = ptr[i] + lane; k < ptr[i + 1]; k += LPR) s += (double)val[k] * (double)z[col[k]];
+        } else {
+            for (int32_t k = ptr[i] + lane; k < ptr[i + 1]; k += LPR) {
+                const int32_t j = col[k];
+                const T pj = (T)((double)z[j] + beta * (double)pold[j]);
+                s += (double)val[k] * (double)pj;
+            }
+        }
+    }
+    s = group_sum<LPR>(s);
+    double pq = 0;
+    if (i < n && lane == 0) {
+        const T pi = it == 0 ? z[i] : (T)((double)z[i] + beta * (double)pold[i]);
+        pnew[i] = pi;
+        const T qi = (T)s;
+        q[i] = qi;
+        pq = (double)pi * (double)qi;
+    }
+    pq = block_sum(pq, red);
+    if (threadIdx.x == 0) part[blockIdx.x] = pq;
+    if (last_block(&st->ticket_b, nblocks)) {
+        const double PQ = sum_partials(part, nblocks, red);
+        if (threadIdx.x == 0) {
+            st->pq = PQ;
+            st->alpha = st->rho / PQ;
+            st->ticket_b = 0;
+        }
+    }
+}
+
+// x += alpha p, r -= alpha q, residual, then z = pre r and rho for the next
+// iteration; the last block updates beta, iter and the stop test.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) cg_b_kernel(int32_t n, const T* __restrict__ pre, T* __restrict__ x,
+                                                      T* __restrict__ r, T* __restrict__ z, const T* __restrict__ p0,
+                                                      const T* __restrict__ p1, const T* __restrict__ q,
+                                                      double* __restrict__ part, CgState* st, uint32_t nblocks) {
+    if (st->done) return;
+    __shared__ double red[kBlock / kWave];
+    const int it = st->iter;
+    const T* pnew = (it & 1) ? p0 : p1;
+    const double alpha = st->alpha;
+    const int32_t i = blockIdx.x * kBlock + threadIdx.x;
+    double rr = 0, rz = 0;
+    if (i < n) {
+        const double pi = (double)pnew[i];
+        x[i] = (T)((double)x[i] + alpha * pi);
+        const T ri = (T)((double)r[i] - alpha * (double)q[i]);
+        r[i] = ri;
+        const T zi = (T)((double)pre[i] * (double)ri);
+        z[i] = zi;
+        rr = (double)ri * (double)ri;
+        rz = (double)ri * (double)zi;
+    }
+    rr = block_sum(rr, red);
+    rz = block_sum(rz, red);
+    if (threadIdx.x == 0) { part[blockIdx.x] = rr; part[nblocks + blockIdx.x] = rz; }
+    if (last_block(&st->ticket_c, nblocks)) {
+        const double RR = sum_partials(part, nblocks, red);
+        const double RZ = sum_partials(part + nblocks, nblocks, red);
+        if (threadIdx.x == 0) {
+            st->rr = RR;
+            st->iter = it + 1;
+            const double ratio = std::sqrt(RR) / std::sqrt(st->rr0);
+            st->relres = ratio;
+            st->beta = RZ / st->rho;
+            st->rho = RZ;
+            st->done = !(ratio > st->tol && it + 1 < st->maxiter);
+            st->ticket_c = 0;
+        }
+    }
+}
+
+// U[free[k]] = x[k]; Dirichlet edges take their new values.
+template <typename T>
+__global__ void gf_scatter_kernel(int32_t n_free, const int32_t* __restrict__ free_edge, const T* __restrict__ x,
+                                  T* __restrict__ U, int32_t n_dir, const int32_t* __restrict__ dir_edge,
+                                  const T* __restrict__ dir_new) {
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n_free) U[free_edge[i]] = x[i];
+    else if (i - n_free < n_dir) {
+        const int32_t e = dir_edge[i - n_free];
+        U[e] = dir_new[e];
+    }
+}
+
+// gfdtd sources at step i: fb (h, edge-indexed scratch) and dir_new (e); the
+// Poynting term U_old . 0.5 Ctb (Fb + Fb_old) (D:1518) with fb_old from the
+// previous step's carriers.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) gf_inject_kernel(
+    int32_t n_inj, const int32_t* __restrict__ inj_edge, const int8_t* __restrict__ inj_kind,
+    const int32_t* __restrict__ inj_tb, const int32_t* __restrict__ inj_te, const int32_t* __restrict__ term_src,
+    const double* __restrict__ term_w, const double* __restrict__ carr_step, const double* __restrict__ carr_prev,
+    T* __restrict__ fb, T* __restrict__ dir_new, const T* __restrict__ U, double* __restrict__ poy_part) {
+    __shared__ double red[kBlock / kWave];
+    const int32_t i = blockIdx.x * kBlock + threadIdx.x;
+    double poy = 0;
+    if (i < n_inj) {
+        const int32_t e = inj_edge[i];
+        const double v = source_value(inj_tb[i], inj_te[i], term_src, term_w, carr_step);
+        if (inj_kind[i] == 0) {
+            fb[e] = (T)v;
+            const double vo = carr_prev ? source_value(inj_tb[i], inj_te[i], term_src, term_w, carr_prev) : 0.0;
+            poy = (double)U[e] * (0.5 * (v + vo));
+        } else {
+            dir_new[e] = (T)v;
+        }
+    }
+    const double s = block_sum(poy, red);
+    if (threadIdx.x == 0) poy_part[blockIdx.x] = s;
+}
+
+// ------------------------------------------------------------------ energy
+// 0.5 * sum w_i x_i^2 in fp64 with a last-block reduction into out[0].
+template <typename T>
+__global__ void __launch_bounds__(kBlock) weighted_sq_kernel(int32_t n, const T* __restrict__ x,
+                                                             const double* __restrict__ w, double* __restrict__ part) {
+    __shared__ double red[kBlock / kWave];
+    const int32_t i = blockIdx.x * kBlock + threadIdx.x;
+    double v = 0;
+    if (i < n) v = 0.5 * w[i] * (double)x[i] * (double)x[i];
+    v = block_sum(v, red);
+    if (threadIdx.x == 0) part[blockIdx.x] = v;
+}
+
+}  // namespace tfd

@@ -1,0 +1,1002 @@
+// tetfit_hip.hip - libtetfit_hip.so: device context, per-method step
+// sequences and the extern "C" tf_* entry points of include/tetfit_hip.h.
+//
+// Reference loops replaced: fdtd D:1840-1933, dga D:810-939,
+// gfdtd D:1459-1565 (D: = src/DGA/Discretization.hpp), with the CG of
+// src/DGA/ConjugateGradientSolver.hpp:52-103 run on the device.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+#include "kernels.hpp"
+#include "tetfit_hip.h"
+
+using namespace tfd;
+
+namespace {
+
+thread_local std::string g_err;
+
+// Carrier of Source::Compute in long double (Source.hpp:316-327); 0 for t<0.
+double carrier_value(const tf_source& s, double t) {
+    if (t < 0) return 0;
+    const long double PI_L = 3.141592653589793238L;
+    double ret = s.amp;
+    if (s.field == TF_FIELD_J) ret *= s.impedance;
+    const long double arg = t - s.delay;
+    switch (s.carrier) {
+        case TF_CARRIER_SIN: ret *= std::sin(2 * PI_L * s.freq * arg); break;
+        case TF_CARRIER_COS: ret *= s.amp * std::cos(2 * PI_L * s.freq * arg); break;
+        case TF_CARRIER_GAUSSIAN: ret *= std::exp(-std::pow(t - s.delay, 2) / std::pow(s.width, 2)); break;
+        case TF_CARRIER_RICKER: {
+            long double a = PI_L * s.freq * arg;
+            ret *= (1 - 2 * a * a) * std::exp(-(a * a));
+            break;
+        }
+        case TF_CARRIER_GAUSSMOD:
+            ret *= std::exp(-std::pow(t - s.delay, 2) / std::pow(s.width, 2));
+            ret *= std::sin(2 * PI_L * s.freq * arg);
+            break;
+        default: break;
+    }
+    return ret;
+}
+
+template <class T>
+void upload(DevBuf& b, const std::vector<T>& v) {
+    b.alloc(v.size() * sizeof(T));
+    if (!v.empty()) TF_HIP(hipMemcpy(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+}
+
+struct Csr32 {
+    DevBuf ptr, col, val, sg;
+    int32_t nrows = 0;
+    int64_t nnz = 0;
+};
+
+}  // namespace
+
+struct tf_ctx {
+    int method = 0, prec = 0;
+    int32_t E = 0, F = 0;
+    double dt = 0;
+    hipStream_t stream = nullptr;
+    size_t tsize = 8;
+    DevBuf U, Fv, B;
+
+    // FIT
+    DevBuf fe_sg, fe_ca, fe_cb, fe_f4, fh_sg, fh_cm, fh_cn, fh_e4;
+    std::vector<uint8_t> h_fe_sg;
+    std::vector<int32_t> fit_uncommon, fit_src;
+    std::vector<int4> fit_sp_f4_u;
+    std::vector<uint8_t> fit_sp_sg_u;
+    std::vector<double> fit_sp_ca_u, fit_sp_cb_u;
+    std::vector<int4> h_fe_f4;
+    std::vector<double> h_ca_common, h_cb_common;
+    DevBuf sp_edge, sp_f4, sp_sg, sp_ca, sp_cb, sp_kind, sp_tb, sp_te;
+    int32_t n_sp = 0;
+
+    // DGA / gfdtd
+    Csr32 Ct, H, N, Em, R1, A;
+    DevBuf c_col, c_sg, h_edge, curl_f, curl_u;
+    DevBuf free_edge, dir_edge, dir_new, fb, pre, x, r, z, p0, p1, q, rhs, cg_part, cg_state;
+    int32_t n_free = 0, n_dir = 0;
+    std::vector<int32_t> h_dir_edge;
+    std::vector<double> h_fixed_value;
+    DevBuf inj_edge, inj_kind, inj_tb, inj_te;
+    int32_t n_inj = 0;
+    CgState* h_state = nullptr;  // pinned
+
+    // sources
+    std::vector<tf_source> src;
+    DevBuf term_src, term_w, carr;
+    int64_t carr_cap = 0;
+
+    // probes and output schedule
+    int32_t n_probes = 0;
+    DevBuf probes, samples;
+    int64_t sample_cap = 0, n_samples = 0;
+    std::vector<double> sample_times;
+    double next_print = 0, period = 0;
+    bool out_enabled = false;
+    bool initial_done = false;
+
+    // Poynting partials
+    DevBuf poy;
+    int64_t poy_cap = 0;
+    std::vector<double> poynting;
+
+    // profiling
+    bool profiling = false;
+    struct Rec { const char* name; hipEvent_t a, b; };
+    std::vector<Rec> recs;
+    std::vector<hipEvent_t> ev_pool;
+    size_t ev_used = 0;
+    std::map<std::string, std::pair<double, int64_t>> ktimes;
+
+    size_t device_bytes() const;
+    ~tf_ctx();
+};
+
+tf_ctx::~tf_ctx() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    for (auto e : ev_pool) (void)hipEventDestroy(e);
+    if (h_state) (void)hipHostFree(h_state);
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+size_t tf_ctx::device_bytes() const {
+    size_t s = 0;
+    const DevBuf* all[] = {&U, &Fv, &B, &fe_sg, &fe_ca, &fe_cb, &fe_f4, &fh_sg, &fh_cm, &fh_cn, &fh_e4,
+                           &sp_edge, &sp_f4, &sp_sg, &sp_ca, &sp_cb, &sp_kind, &sp_tb, &sp_te, &c_col, &c_sg,
+                           &h_edge, &curl_f, &curl_u, &free_edge, &dir_edge, &dir_new, &fb, &pre, &x, &r, &z,
+                           &p0, &p1, &q, &rhs, &cg_part, &cg_state, &inj_edge, &inj_kind, &inj_tb, &inj_te,
+                           &term_src, &term_w, &carr, &probes, &samples, &poy};
+    for (auto b : all) s += b->bytes;
+    const Csr32* cs[] = {&Ct, &H, &N, &Em, &R1, &A};
+    for (auto c : cs) s += c->ptr.bytes + c->col.bytes + c->val.bytes + c->sg.bytes;
+    return s;
+}
+
+namespace {
+
+// ---------------------------------------------------------------- helpers
+
+template <class T>
+void upload_csr(Csr32& d, const tf_csr& c, bool values) {
+    if (c.nnz >= (int64_t)INT32_MAX) throw std::runtime_error("matrix too large for int32 row pointers");
+    d.nrows = c.nrows;
+    d.nnz = c.nnz;
+    std::vector<int32_t> ptr(c.nrows + 1);
+    for (int32_t i = 0; i <= c.nrows; ++i) ptr[i] = (int32_t)c.rowptr[i];
+    upload(d.ptr, ptr);
+    d.col.alloc(c.nnz * 4);
+    if (c.nnz) TF_HIP(hipMemcpy(d.col.p, c.col, c.nnz * 4, hipMemcpyHostToDevice));
+    if (values) {
+        std::vector<T> v(c.val, c.val + c.nnz);
+        upload(d.val, v);
+    } else {
+        d.sg.alloc(c.nnz);
+        if (c.nnz) TF_HIP(hipMemcpy(d.sg.p, c.sgn, c.nnz, hipMemcpyHostToDevice));
+    }
+}
+
+template <class T>
+void zero_field(DevBuf& b, int64_t n, hipStream_t s) {
+    b.alloc(std::max<int64_t>(n, 1) * sizeof(T));
+    TF_HIP(hipMemsetAsync(b.p, 0, b.bytes, s));
+}
+
+uint8_t code4(const int s[4]) {
+    uint8_t c = 0;
+    for (int k = 0; k < 4; ++k) c |= (uint8_t)((s[k] > 0 ? 1 : (s[k] < 0 ? 3 : 0)) << (2 * k));
+    return c;
+}
+
+void ev_begin(tf_ctx* c, const char* name) {
+    if (!c->profiling) return;
+    if (c->ev_used + 2 > c->ev_pool.size()) {
+        for (int i = 0; i < 256; ++i) {
+            hipEvent_t e;
+            TF_HIP(hipEventCreate(&e));
+            c->ev_pool.push_back(e);
+        }
+    }
+    tf_ctx::Rec r{name, c->ev_pool[c->ev_used], c->ev_pool[c->ev_used + 1]};
+    c->ev_used += 2;
+    TF_HIP(hipEventRecord(r.a, c->stream));
+    c->recs.push_back(r);
+}
+void ev_end(tf_ctx* c) {
+    if (!c->profiling) return;
+    TF_HIP(hipEventRecord(c->recs.back().b, c->stream));
+}
+void ev_flush(tf_ctx* c) {
+    if (!c->profiling || c->recs.empty()) return;
+    TF_HIP(hipStreamSynchronize(c->stream));
+    for (auto& r : c->recs) {
+        float ms = 0;
+        TF_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+        auto& k = c->ktimes[r.name];
+        k.first += ms;
+        k.second += 1;
+    }
+    c->recs.clear();
+    c->ev_used = 0;
+}
+
+// ---------------------------------------------------------------- create
+
+template <class T>
+void create_fit(tf_ctx* c, const tf_problem* p) {
+    const tf_fit_ops& f = p->fit;
+    const int32_t E = c->E, F = c->F;
+    const double dt = p->dt;
+    std::vector<uint8_t> sg(E, 0);
+    std::vector<T> ca(E, 0), cb(E, 0);
+    std::vector<int4> f4(E, int4{0, 0, 0, 0});
+    c->h_ca_common.assign(E, 0);
+    c->h_cb_common.assign(E, 0);
+    for (int64_t q = 0; q < f.n_common; ++q) {
+        const int32_t e = f.common_edges[q];
+        if (f.ct_ptr[e + 1] - f.ct_ptr[e] != 4) throw std::runtime_error("common edge without 4 faces");
+        const int64_t b0 = f.ct_ptr[e];
+        f4[e] = int4{f.ct_face[b0], f.ct_face[b0 + 1], f.ct_face[b0 + 2], f.ct_face[b0 + 3]};
+        const int d = f.dual_curl[e];
+        const int s[4] = {d, -d, d, -d};  // stored signs ignored for common edges (D:1873-1874)
+        sg[e] = code4(s);
+        const double up = f.upml[e];
+        const double a = std::pow(up, 2) * f.M_h[e] * f.M_q[e];
+        const double bb = std::pow(up, 2) * f.M_h[e] * up * dt;
+        c->h_ca_common[e] = a;
+        c->h_cb_common[e] = bb;
+        ca[e] = (T)a;
+        cb[e] = (T)bb;
+    }
+    c->h_fe_sg = sg;
+    c->h_fe_f4 = f4;
+    upload(c->fe_sg, sg);
+    upload(c->fe_ca, ca);
+    upload(c->fe_cb, cb);
+    upload(c->fe_f4, f4);
+    // special edges: uncommon (h source) and src edges (e / j)
+    c->fit_uncommon.assign(f.uncommon_edges, f.uncommon_edges + f.n_uncommon);
+    c->fit_src.assign(f.src_edges, f.src_edges + f.n_src);
+    c->fit_sp_f4_u.clear();
+    c->fit_sp_sg_u.clear();
+    c->fit_sp_ca_u.clear();
+    c->fit_sp_cb_u.clear();
+    for (int32_t e : c->fit_uncommon) {
+        int4 ff{0, 0, 0, 0};
+        int s[4] = {0, 0, 0, 0};
+        int n = 0;
+        for (int64_t q = f.ct_ptr[e]; q < f.ct_ptr[e + 1] && n < 4; ++q, ++n) {
+            (&ff.x)[n] = f.ct_face[q];
+            s[n] = f.ct_sgn[q];
+        }
+        c->fit_sp_f4_u.push_back(ff);
+        c->fit_sp_sg_u.push_back(code4(s));
+        c->fit_sp_ca_u.push_back(f.M_h[e] * f.M_q[e]);
+        c->fit_sp_cb_u.push_back(f.M_h[e] * dt);
+    }
+    // H phase over tbc faces
+    std::vector<uint8_t> hs(F, 0);
+    std::vector<T> cm(F, 0), cn(F, 0);
+    std::vector<int4> e4(F, int4{0, 0, 0, 0});
+    for (int64_t q = 0; q < f.n_tbc; ++q) {
+        const int32_t fi = f.tbc_faces[q];
+        const int32_t* ce = f.c_edge + 4 * (int64_t)fi;
+        e4[fi] = int4{ce[0], ce[1], ce[2], ce[3]};
+        const int d = f.curl[fi];
+        const int s[4] = {d, -d, d, -d};
+        hs[fi] = code4(s);
+        cm[fi] = (T)(f.M_mu[fi] * f.fpml[fi]);
+        cn[fi] = (T)(f.fpml[fi] * f.M_nu[fi]);
+    }
+    upload(c->fh_sg, hs);
+    upload(c->fh_cm, cm);
+    upload(c->fh_cn, cn);
+    upload(c->fh_e4, e4);
+}
+
+template <class T>
+void create_tet(tf_ctx* c, const tf_problem* p) {
+    const tf_dga_ops& d = p->dga;
+    const int32_t E = c->E, F = c->F;
+    upload_csr<T>(c->Ct, d.Ct, false);
+    if (d.C.nnz != 3 * (int64_t)F) throw std::runtime_error("C must have 3 entries per face");
+    c->c_col.alloc(d.C.nnz * 4);
+    TF_HIP(hipMemcpy(c->c_col.p, d.C.col, d.C.nnz * 4, hipMemcpyHostToDevice));
+    c->c_sg.alloc(d.C.nnz);
+    TF_HIP(hipMemcpy(c->c_sg.p, d.C.sgn, d.C.nnz, hipMemcpyHostToDevice));
+    upload_csr<T>(c->N, d.N, true);
+    c->h_fixed_value.assign(d.fixed_value, d.fixed_value + E);
+    zero_field<T>(c->curl_u, F, c->stream);
+    zero_field<T>(c->curl_f, E, c->stream);
+    if (p->method == TF_METHOD_DGA) {
+        upload_csr<T>(c->H, d.H, true);
+        std::vector<int32_t> he(d.H.nrows, -1);
+        for (int32_t e = 0; e < E; ++e)
+            if (d.h_row[e] >= 0) he[d.h_row[e]] = e;
+        for (int32_t r = 0; r < d.H.nrows; ++r)
+            if (he[r] < 0) throw std::runtime_error("H row without an edge");
+        upload(c->h_edge, he);
+    } else {
+        upload_csr<T>(c->Em, d.E, true);
+        upload_csr<T>(c->R1, d.RHS1, true);
+        upload_csr<T>(c->A, d.A, true);
+        c->n_free = d.n_free;
+        std::vector<int32_t> fe(d.free_edges, d.free_edges + d.n_free);
+        upload(c->free_edge, fe);
+        std::vector<uint8_t> is_free(E, 0);
+        for (int32_t e : fe) is_free[e] = 1;
+        c->h_dir_edge.clear();
+        for (int32_t e = 0; e < E; ++e)
+            if (!is_free[e]) c->h_dir_edge.push_back(e);
+        c->n_dir = (int32_t)c->h_dir_edge.size();
+        upload(c->dir_edge, c->h_dir_edge);
+        // dir_new holds the Dirichlet values (ComputeEdgeBC for pec edges)
+        std::vector<T> dn(E, 0);
+        for (int32_t e : c->h_dir_edge) dn[e] = (T)c->h_fixed_value[e];
+        upload(c->dir_new, dn);
+        zero_field<T>(c->fb, E, c->stream);
+        // Jacobi preconditioner: 1/A_jj, or 1 if missing / zero (CG.hpp:119-128)
+        std::vector<T> pre(d.n_free, (T)1);
+        for (int32_t i = 0; i < d.n_free; ++i)
+            for (int64_t q = d.A.rowptr[i]; q < d.A.rowptr[i + 1]; ++q)
+                if (d.A.col[q] == i && d.A.val[q] != 0) pre[i] = (T)(1.0 / d.A.val[q]);
+        upload(c->pre, pre);
+        const int64_t n = std::max<int32_t>(d.n_free, 1);
+        for (DevBuf* b : {&c->x, &c->r, &c->z, &c->p0, &c->p1, &c->q, &c->rhs}) zero_field<T>(*b, n, c->stream);
+        c->cg_part.alloc(sizeof(double) * 2 * (blocks_for(n) + 64));
+        c->cg_state.alloc(sizeof(CgState));
+        TF_HIP(hipMemsetAsync(c->cg_state.p, 0, sizeof(CgState), c->stream));
+        TF_HIP(hipHostMalloc((void**)&c->h_state, sizeof(CgState), hipHostMallocDefault));
+    }
+    // initial U: fixed (class-4) values
+    std::vector<T> u0(E, 0);
+    for (int32_t e = 0; e < E; ++e)
+        if (d.edge_class[e] == 4) u0[e] = (T)c->h_fixed_value[e];
+    TF_HIP(hipStreamSynchronize(c->stream));
+    TF_HIP(hipMemcpy(c->U.p, u0.data(), E * sizeof(T), hipMemcpyHostToDevice));
+}
+
+// Special (source) list of the FIT E phase from the current terms.
+template <class T>
+void build_fit_special(tf_ctx* c, const std::vector<std::vector<std::pair<int32_t, double>>>& by_edge_terms,
+                       const std::vector<int>& edge_field) {
+    std::vector<int32_t> edge, tb, te, tsrc;
+    std::vector<int4> f4;
+    std::vector<uint8_t> sg;
+    std::vector<double> ca, cb, tw;
+    std::vector<int8_t> kind;
+    std::vector<uint8_t> main_sg = c->h_fe_sg;
+    auto add_terms = [&](int32_t e) {
+        tb.push_back((int32_t)tsrc.size());
+        auto it = by_edge_terms.size() > (size_t)e ? &by_edge_terms[e] : nullptr;
+        if (it)
+            for (auto& pr : *it) { tsrc.push_back(pr.first); tw.push_back(pr.second); }
+        te.push_back((int32_t)tsrc.size());
+    };
+    for (size_t i = 0; i < c->fit_uncommon.size(); ++i) {
+        const int32_t e = c->fit_uncommon[i];
+        edge.push_back(e);
+        f4.push_back(c->fit_sp_f4_u[i]);
+        sg.push_back(c->fit_sp_sg_u[i]);
+        ca.push_back(c->fit_sp_ca_u[i]);
+        cb.push_back(c->fit_sp_cb_u[i]);
+        kind.push_back(0);
+        add_terms(e);
+    }
+    for (int32_t e : c->fit_src) {
+        const int fld = edge_field.size() > (size_t)e ? edge_field[e] : TF_FIELD_E;
+        edge.push_back(e);
+        if (fld == TF_FIELD_J) {
+            f4.push_back(c->h_fe_f4[e]);
+            sg.push_back(c->h_fe_sg[e]);
+            ca.push_back(c->h_ca_common[e]);
+            cb.push_back(c->h_cb_common[e]);
+            kind.push_back(2);
+            main_sg[e] = 0;  // the special item updates it
+        } else {
+            f4.push_back(int4{0, 0, 0, 0});
+            sg.push_back(0);
+            ca.push_back(0);
+            cb.push_back(0);
+            kind.push_back(1);
+            main_sg[e] = 0;
+        }
+        add_terms(e);
+    }
+    c->n_sp = (int32_t)edge.size();
+    upload(c->sp_edge, edge);
+    upload(c->sp_f4, f4);
+    upload(c->sp_sg, sg);
+    upload(c->sp_ca, ca);
+    upload(c->sp_cb, cb);
+    upload(c->sp_kind, kind);
+    upload(c->sp_tb, tb);
+    upload(c->sp_te, te);
+    if (tsrc.empty()) { tsrc.push_back(0); tw.push_back(0); }
+    upload(c->term_src, tsrc);
+    upload(c->term_w, tw);
+    upload(c->fe_sg, main_sg);
+}
+
+void build_tet_inject(tf_ctx* c, const std::vector<std::vector<std::pair<int32_t, double>>>& by_edge_terms,
+                      const std::vector<int>& edge_field) {
+    std::vector<int32_t> edge, tb, te, tsrc;
+    std::vector<int8_t> kind;
+    std::vector<double> tw;
+    for (int32_t e = 0; e < c->E; ++e) {
+        if (by_edge_terms[e].empty()) continue;
+        edge.push_back(e);
+        kind.push_back(edge_field[e] == TF_FIELD_H ? 0 : 1);
+        if (edge_field[e] == TF_FIELD_J) throw std::runtime_error("j sources are not supported for dga / gfdtd");
+        tb.push_back((int32_t)tsrc.size());
+        for (auto& pr : by_edge_terms[e]) { tsrc.push_back(pr.first); tw.push_back(pr.second); }
+        te.push_back((int32_t)tsrc.size());
+    }
+    c->n_inj = (int32_t)edge.size();
+    upload(c->inj_edge, edge);
+    upload(c->inj_kind, kind);
+    upload(c->inj_tb, tb);
+    upload(c->inj_te, te);
+    if (tsrc.empty()) { tsrc.push_back(0); tw.push_back(0); }
+    upload(c->term_src, tsrc);
+    upload(c->term_w, tw);
+}
+
+// ---------------------------------------------------------------- run
+
+bool allow_print(tf_ctx* c, double t) {
+    if (!c->out_enabled) return false;
+    if (t >= c->next_print) {
+        c->next_print += c->period;
+        return true;
+    }
+    return false;
+}
+
+template <class T>
+void sample(tf_ctx* c, double t) {
+    if (c->n_probes == 0) {
+        c->sample_times.push_back(t);
+        c->n_samples++;
+        return;
+    }
+    if (c->n_samples >= c->sample_cap) {
+        int64_t cap = std::max<int64_t>(1024, c->sample_cap * 2);
+        DevBuf nb;
+        nb.alloc(cap * c->n_probes * 6 * sizeof(double));
+        if (c->n_samples)
+            TF_HIP(hipMemcpyAsync(nb.p, c->samples.p, c->n_samples * c->n_probes * 6 * sizeof(double),
+                                  hipMemcpyDeviceToDevice, c->stream));
+        TF_HIP(hipStreamSynchronize(c->stream));
+        std::swap(nb.p, c->samples.p);
+        std::swap(nb.bytes, c->samples.bytes);
+        c->sample_cap = cap;
+    }
+    ev_begin(c, "probe_kernel");
+    probe_kernel<T><<<blocks_for(c->n_probes * 6, 64), 64, 0, c->stream>>>(
+        c->n_probes, c->probes.as<ProbeDev>(), c->U.as<T>(), c->B.as<T>(),
+        c->samples.as<double>() + c->n_samples * c->n_probes * 6);
+    ev_end(c);
+    c->sample_times.push_back(t);
+    c->n_samples++;
+}
+
+void ensure_carr(tf_ctx* c, int64_t first, int64_t n) {
+    // carriers for steps first-1 .. first+n-1 (row 0 is the previous step)
+    const size_t ns = std::max<size_t>(c->src.size(), 1);
+    std::vector<double> tab((size_t)(n + 1) * ns, 0.0);
+    for (int64_t k = 0; k <= n; ++k) {
+        const int64_t i = first - 1 + k;
+        const double t = (double)i * c->dt;
+        for (size_t s = 0; s < c->src.size(); ++s) {
+            const tf_source& so = c->src[s];
+            const double ts = so.field == TF_FIELD_H ? t - 0.5 * c->dt : t;
+            tab[k * ns + s] = (i < 0) ? 0.0 : carrier_value(so, ts);
+        }
+    }
+    if ((int64_t)tab.size() > c->carr_cap) {
+        c->carr.alloc(tab.size() * sizeof(double));
+        c->carr_cap = (int64_t)tab.size();
+    }
+    TF_HIP(hipMemcpyAsync(c->carr.p, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    TF_HIP(hipStreamSynchronize(c->stream));
+}
+
+template <class T>
+void run_fit(tf_ctx* c, int64_t first, int64_t n, tf_stats* st) {
+    const int main_blocks = blocks_for(c->E);
+    const int sp_blocks = c->n_sp ? blocks_for(c->n_sp) : 0;
+    const size_t ns = std::max<size_t>(c->src.size(), 1);
+    if (sp_blocks) {
+        if (c->poy_cap < n * sp_blocks) {
+            c->poy.alloc(n * sp_blocks * sizeof(double));
+            c->poy_cap = n * sp_blocks;
+        }
+    }
+    for (int64_t k = 0; k < n; ++k) {
+        const int64_t i = first + k;
+        const double t = (double)i * c->dt;
+        const double* cs = c->carr.as<double>() + (k + 1) * ns;
+        ev_begin(c, "fit_e_kernel");
+        fit_e_kernel<T><<<main_blocks + sp_blocks, kBlock, 0, c->stream>>>(
+            c->E, main_blocks, c->fe_sg.as<uint8_t>(), c->fe_ca.as<T>(), c->fe_cb.as<T>(), c->fe_f4.as<int4>(),
+            c->Fv.as<T>(), c->U.as<T>(), c->n_sp, c->sp_edge.as<int32_t>(), c->sp_f4.as<int4>(),
+            c->sp_sg.as<uint8_t>(), c->sp_ca.as<double>(), c->sp_cb.as<double>(), c->sp_kind.as<int8_t>(),
+            c->sp_tb.as<int32_t>(), c->sp_te.as<int32_t>(), c->term_src.as<int32_t>(), c->term_w.as<double>(), cs,
+            sp_blocks ? c->poy.as<double>() + k * sp_blocks : nullptr);
+        ev_end(c);
+        ev_begin(c, "fit_h_kernel");
+        fit_h_kernel<T><<<blocks_for(c->F), kBlock, 0, c->stream>>>(c->F, c->fh_sg.as<uint8_t>(), c->fh_cm.as<T>(),
+                                                                     c->fh_cn.as<T>(), c->fh_e4.as<int4>(),
+                                                                     c->U.as<T>(), c->Fv.as<T>(), c->B.as<T>(), c->dt);
+        ev_end(c);
+        if (allow_print(c, t)) sample<T>(c, t);
+        if (c->profiling && (k & 63) == 63) ev_flush(c);
+    }
+    TF_HIP(hipGetLastError());
+    // Poynting per step
+    std::vector<double> part((size_t)n * std::max(sp_blocks, 1), 0.0);
+    if (sp_blocks)
+        TF_HIP(hipMemcpyAsync(part.data(), c->poy.p, n * sp_blocks * sizeof(double), hipMemcpyDeviceToHost,
+                              c->stream));
+    TF_HIP(hipStreamSynchronize(c->stream));
+    for (int64_t k = 0; k < n; ++k) {
+        double s = 0;
+        for (int b = 0; b < sp_blocks; ++b) s += part[k * sp_blocks + b];
+        c->poynting.push_back(s);
+    }
+    st->steps_done += n;
+}
+
+template <class T, int LPR>
+int rows_blocks(int64_t nrows) { return blocks_for(nrows * LPR); }
+
+template <class T>
+void run_dga(tf_ctx* c, int64_t first, int64_t n, tf_stats* st) {
+    const int inj_blocks = c->n_inj ? blocks_for(c->n_inj) : 0;
+    const size_t ns = std::max<size_t>(c->src.size(), 1);
+    if (inj_blocks && c->poy_cap < 2 * n * inj_blocks) {
+        c->poy.alloc(2 * n * inj_blocks * sizeof(double));
+        c->poy_cap = 2 * n * inj_blocks;
+    }
+    if (!c->initial_done && first == 1) {
+        // initial export at t = 0 (D:805-806), after the t = 0 e-source values
+        if (c->n_inj) {
+            const double* cs0 = c->carr.as<double>();  // row 0 = step first-1 = 0
+            dga_inject_kernel<T><<<inj_blocks, kBlock, 0, c->stream>>>(
+                c->n_inj, c->inj_edge.as<int32_t>(), c->inj_kind.as<int8_t>(), c->inj_tb.as<int32_t>(),
+                c->inj_te.as<int32_t>(), c->term_src.as<int32_t>(), c->term_w.as<double>(), cs0, c->curl_f.as<T>(),
+                c->U.as<T>(), c->poy.as<double>());
+        }
+        if (allow_print(c, 0.0)) sample<T>(c, 0.0);
+    }
+    c->initial_done = true;
+    for (int64_t k = 0; k < n; ++k) {
+        const int64_t i = first + k;
+        const double t = (double)i * c->dt;
+        const double* cs = c->carr.as<double>() + (k + 1) * ns;
+        ev_begin(c, "ct_rows_kernel");
+        incidence_rows_kernel<T, 8><<<rows_blocks<T, 8>(c->E), kBlock, 0, c->stream>>>(
+            c->E, c->Ct.ptr.as<int32_t>(), c->Ct.col.as<int32_t>(), c->Ct.sg.as<int8_t>(), c->Fv.as<T>(),
+            c->curl_f.as<T>());
+        ev_end(c);
+        if (c->n_inj) {
+            ev_begin(c, "dga_inject_kernel");
+            dga_inject_kernel<T><<<inj_blocks, kBlock, 0, c->stream>>>(
+                c->n_inj, c->inj_edge.as<int32_t>(), c->inj_kind.as<int8_t>(), c->inj_tb.as<int32_t>(),
+                c->inj_te.as<int32_t>(), c->term_src.as<int32_t>(), c->term_w.as<double>(), cs, c->curl_f.as<T>(),
+                c->U.as<T>(), c->poy.as<double>() + 2 * k * inj_blocks);
+            ev_end(c);
+        }
+        ev_begin(c, "dga_h_kernel");
+        dga_h_kernel<T, 32><<<rows_blocks<T, 32>(c->H.nrows), kBlock, 0, c->stream>>>(
+            c->H.nrows, c->H.ptr.as<int32_t>(), c->H.col.as<int32_t>(), c->H.val.as<T>(), c->h_edge.as<int32_t>(),
+            c->curl_f.as<T>(), c->U.as<T>(), c->dt);
+        ev_end(c);
+        if (c->n_inj) {
+            ev_begin(c, "dga_poynting_kernel");
+            dga_poynting_kernel<T><<<inj_blocks, kBlock, 0, c->stream>>>(
+                c->n_inj, c->inj_edge.as<int32_t>(), c->inj_kind.as<int8_t>(), c->inj_tb.as<int32_t>(),
+                c->inj_te.as<int32_t>(), c->term_src.as<int32_t>(), c->term_w.as<double>(), cs, c->U.as<T>(),
+                c->poy.as<double>() + (2 * k + 1) * inj_blocks);
+            ev_end(c);
+        }
+        ev_begin(c, "dga_curl_kernel");
+        dga_curl_kernel<T><<<blocks_for(c->F), kBlock, 0, c->stream>>>(c->F, c->c_col.as<int32_t>(),
+                                                                        c->c_sg.as<int8_t>(), c->U.as<T>(),
+                                                                        c->curl_u.as<T>(), c->B.as<T>(), c->dt);
+        ev_end(c);
+        ev_begin(c, "dga_n_kernel");
+        dga_n_kernel<T, 8><<<rows_blocks<T, 8>(c->N.nrows), kBlock, 0, c->stream>>>(
+            c->N.nrows, c->N.ptr.as<int32_t>(), c->N.col.as<int32_t>(), c->N.val.as<T>(), c->curl_u.as<T>(),
+            c->Fv.as<T>(), c->dt, 0);
+        ev_end(c);
+        if (allow_print(c, t)) sample<T>(c, t);
+        if (c->profiling && (k & 63) == 63) ev_flush(c);
+    }
+    TF_HIP(hipGetLastError());
+    std::vector<double> part((size_t)2 * n * std::max(inj_blocks, 1), 0.0);
+    if (inj_blocks)
+        TF_HIP(hipMemcpyAsync(part.data(), c->poy.p, 2 * n * inj_blocks * sizeof(double), hipMemcpyDeviceToHost,
+                              c->stream));
+    TF_HIP(hipStreamSynchronize(c->stream));
+    for (int64_t k = 0; k < n; ++k) {
+        double s = 0;
+        for (int b = 0; b < 2 * inj_blocks; ++b) s += part[2 * k * inj_blocks + b];
+        c->poynting.push_back(s);
+    }
+    st->steps_done += n;
+}
+
+template <class T>
+void run_gfdtd(tf_ctx* c, int64_t first, int64_t n, const tf_cg_opts* cg, tf_stats* st) {
+    const int inj_blocks = c->n_inj ? blocks_for(c->n_inj) : 0;
+    const size_t ns = std::max<size_t>(c->src.size(), 1);
+    const int32_t nf = c->n_free;
+    const uint32_t nb_rows = (uint32_t)rows_blocks<T, 8>(nf);
+    const uint32_t nb_vec = (uint32_t)blocks_for(nf);
+    const int batch = 4;
+    if (inj_blocks && c->poy_cap < n * inj_blocks) {
+        c->poy.alloc(n * inj_blocks * sizeof(double));
+        c->poy_cap = n * inj_blocks;
+    }
+    if ((int64_t)c->cg_part.bytes < (int64_t)sizeof(double) * 2 * (int64_t)std::max(nb_rows, nb_vec))
+        c->cg_part.alloc(sizeof(double) * 2 * std::max(nb_rows, nb_vec));
+    CgState init{};
+    init.tol = cg ? cg->tol : 1e-8;
+    init.maxiter = cg ? cg->maxiter : 100;
+    CgState* dst = c->cg_state.as<CgState>();
+    for (int64_t k = 0; k < n; ++k) {
+        const int64_t i = first + k;
+        const double t = (double)i * c->dt;
+        const double* cs = c->carr.as<double>() + (k + 1) * ns;
+        const double* cp = c->carr.as<double>() + k * ns;
+        ev_begin(c, "dga_curl_kernel");
+        dga_curl_kernel<T><<<blocks_for(c->F), kBlock, 0, c->stream>>>(c->F, c->c_col.as<int32_t>(),
+                                                                        c->c_sg.as<int8_t>(), c->U.as<T>(),
+                                                                        c->curl_u.as<T>(), c->B.as<T>(), c->dt);
+        ev_end(c);
+        ev_begin(c, "dga_n_kernel");
+        dga_n_kernel<T, 8><<<rows_blocks<T, 8>(c->N.nrows), kBlock, 0, c->stream>>>(
+            c->N.nrows, c->N.ptr.as<int32_t>(), c->N.col.as<int32_t>(), c->N.val.as<T>(), c->B.as<T>(),
+            c->Fv.as<T>(), c->dt, 1);
+        ev_end(c);
+        if (allow_print(c, t - c->dt)) sample<T>(c, t - c->dt);
+        if (c->n_inj) {
+            ev_begin(c, "gf_inject_kernel");
+            gf_inject_kernel<T><<<inj_blocks, kBlock, 0, c->stream>>>(
+                c->n_inj, c->inj_edge.as<int32_t>(), c->inj_kind.as<int8_t>(), c->inj_tb.as<int32_t>(),
+                c->inj_te.as<int32_t>(), c->term_src.as<int32_t>(), c->term_w.as<double>(), cs, i > 1 ? cp : nullptr,
+                c->fb.as<T>(), c->dir_new.as<T>(), c->U.as<T>(), c->poy.as<double>() + k * inj_blocks);
+            ev_end(c);
+        }
+        ev_begin(c, "gf_rhs_kernel");
+        gf_rhs_kernel<T, 16><<<rows_blocks<T, 16>(nf), kBlock, 0, c->stream>>>(
+            nf, c->free_edge.as<int32_t>(), c->Em.ptr.as<int32_t>(), c->Em.col.as<int32_t>(), c->Em.val.as<T>(),
+            c->R1.ptr.as<int32_t>(), c->R1.col.as<int32_t>(), c->R1.val.as<T>(), c->Ct.ptr.as<int32_t>(),
+            c->Ct.col.as<int32_t>(), c->Ct.sg.as<int8_t>(), c->U.as<T>(), c->dir_new.as<T>(), c->Fv.as<T>(),
+            c->fb.as<T>(), c->rhs.as<T>(), c->dt);
+        ev_end(c);
+        TF_HIP(hipMemcpyAsync(dst, &init, sizeof(CgState), hipMemcpyHostToDevice, c->stream));
+        ev_begin(c, "cg_init_kernel");
+        cg_init_kernel<T, 8><<<nb_rows, kBlock, 0, c->stream>>>(
+            nf, c->A.ptr.as<int32_t>(), c->A.col.as<int32_t>(), c->A.val.as<T>(), c->pre.as<T>(), c->rhs.as<T>(),
+            c->x.as<T>(), c->r.as<T>(), c->z.as<T>(), c->cg_part.as<double>(), dst, nb_rows);
+        ev_end(c);
+        for (;;) {
+            for (int b = 0; b < batch; ++b) {
+                ev_begin(c, "cg_a_kernel");
+                cg_a_kernel<T, 8><<<nb_rows, kBlock, 0, c->stream>>>(
+                    nf, c->A.ptr.as<int32_t>(), c->A.col.as<int32_t>(), c->A.val.as<T>(), c->z.as<T>(),
+                    c->p0.as<T>(), c->p1.as<T>(), c->q.as<T>(), c->cg_part.as<double>(), dst, nb_rows);
+                ev_end(c);
+                ev_begin(c, "cg_b_kernel");
+                cg_b_kernel<T><<<nb_vec, kBlock, 0, c->stream>>>(nf, c->pre.as<T>(), c->x.as<T>(), c->r.as<T>(),
+                                                                  c->z.as<T>(), c->p0.as<T>(), c->p1.as<T>(),
+                                                                  c->q.as<T>(), c->cg_part.as<double>(), dst, nb_vec);
+                ev_end(c);
+            }
+            TF_HIP(hipMemcpyAsync(c->h_state, dst, sizeof(CgState), hipMemcpyDeviceToHost, c->stream));
+            TF_HIP(hipStreamSynchronize(c->stream));
+            if (c->h_state->done) break;
+        }
+        st->cg_iters_total += c->h_state->iter;
+        st->cg_iters_max = std::max(st->cg_iters_max, c->h_state->iter);
+        const double rel = c->h_state->relres;
+        if (std::isfinite(rel)) st->cg_max_relres = std::max(st->cg_max_relres, rel);
+        if (c->h_state->iter >= init.maxiter && rel > init.tol) st->cg_not_converged++;
+        ev_begin(c, "gf_scatter_kernel");
+        gf_scatter_kernel<T><<<blocks_for(nf + c->n_dir), kBlock, 0, c->stream>>>(
+            nf, c->free_edge.as<int32_t>(), c->x.as<T>(), c->U.as<T>(), c->n_dir, c->dir_edge.as<int32_t>(),
+            c->dir_new.as<T>());
+        ev_end(c);
+        if (c->profiling) ev_flush(c);
+    }
+    TF_HIP(hipGetLastError());
+    std::vector<double> part((size_t)n * std::max(inj_blocks, 1), 0.0);
+    if (inj_blocks)
+        TF_HIP(hipMemcpyAsync(part.data(), c->poy.p, n * inj_blocks * sizeof(double), hipMemcpyDeviceToHost,
+                              c->stream));
+    TF_HIP(hipStreamSynchronize(c->stream));
+    for (int64_t k = 0; k < n; ++k) {
+        double s = 0;
+        for (int b = 0; b < inj_blocks; ++b) s += part[k * inj_blocks + b];
+        c->poynting.push_back(s);
+    }
+    st->steps_done += n;
+}
+
+template <class F>
+tf_status guarded(F f) {
+    try {
+        g_err.clear();
+        f();
+        return TF_OK;
+    } catch (const HipError& e) {
+        g_err = e.what();
+        return (tf_status)e.code;
+    } catch (const std::bad_alloc&) {
+        g_err = "host allocation failed";
+        return TF_ERR_NOMEM;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return TF_ERR_ARG;
+    }
+}
+
+}  // namespace
+
+// ====================================================================== C ABI
+extern "C" {
+
+const char* tf_last_error(void) { return g_err.c_str(); }
+
+int32_t tf_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+tf_status tf_create(const tf_problem* p, int32_t prec, tf_ctx** out) {
+    if (!p || !out) { g_err = "null argument"; return TF_ERR_ARG; }
+    *out = nullptr;
+    return guarded([&] {
+        if (p->method < 0 || p->method > 2) throw std::runtime_error("unknown method");
+        if (prec != TF_PREC_FP64 && prec != TF_PREC_FP32) throw std::runtime_error("unknown precision");
+        if (p->n_edges <= 0 || p->n_faces <= 0) throw std::runtime_error("empty problem");
+        int ndev = 0;
+        TF_HIP(hipGetDeviceCount(&ndev));
+        if (ndev < 1) throw HipError("no HIP device", TF_ERR_HIP);
+        TF_HIP(hipSetDevice(0));
+        auto c = std::make_unique<tf_ctx>();
+        c->method = p->method;
+        c->prec = prec;
+        c->E = p->n_edges;
+        c->F = p->n_faces;
+        c->dt = p->dt;
+        c->tsize = prec == TF_PREC_FP64 ? 8 : 4;
+        TF_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        c->U.alloc(c->E * c->tsize);
+        c->Fv.alloc(c->F * c->tsize);
+        c->B.alloc(c->F * c->tsize);
+        TF_HIP(hipMemsetAsync(c->U.p, 0, c->U.bytes, c->stream));
+        TF_HIP(hipMemsetAsync(c->Fv.p, 0, c->Fv.bytes, c->stream));
+        TF_HIP(hipMemsetAsync(c->B.p, 0, c->B.bytes, c->stream));
+        if (p->method == TF_METHOD_FDTD) {
+            if (prec == TF_PREC_FP64) create_fit<double>(c.get(), p);
+            else create_fit<float>(c.get(), p);
+            std::vector<std::vector<std::pair<int32_t, double>>> none;
+            std::vector<int> fld;
+            if (prec == TF_PREC_FP64) build_fit_special<double>(c.get(), none, fld);
+            else build_fit_special<float>(c.get(), none, fld);
+        } else {
+            if (prec == TF_PREC_FP64) create_tet<double>(c.get(), p);
+            else create_tet<float>(c.get(), p);
+            std::vector<std::vector<std::pair<int32_t, double>>> none(c->E);
+            std::vector<int> fld(c->E, TF_FIELD_H);
+            build_tet_inject(c.get(), none, fld);
+        }
+        ensure_carr(c.get(), 1, 0);
+        TF_HIP(hipStreamSynchronize(c->stream));
+        *out = c.release();
+    });
+}
+
+tf_status tf_set_sources(tf_ctx* c, const tf_source* src, int32_t n_src, const tf_source_term* terms,
+                         int64_t n_terms) {
+    if (!c || (n_src && !src) || (n_terms && !terms)) { g_err = "null argument"; return TF_ERR_ARG; }
+    return guarded([&] {
+        c->src.assign(src, src + n_src);
+        std::vector<std::vector<std::pair<int32_t, double>>> by_edge(c->E);
+        std::vector<int> fld(c->E, -1);
+        for (int64_t i = 0; i < n_terms; ++i) {
+            const tf_source_term& t = terms[i];
+            if (t.edge < 0 || t.edge >= c->E || t.source < 0 || t.source >= n_src)
+                throw std::runtime_error("source term out of range");
+            by_edge[t.edge].push_back({t.source, t.weight});
+            if (fld[t.edge] < 0) fld[t.edge] = src[t.source].field;
+        }
+        for (auto& f : fld)
+            if (f < 0) f = TF_FIELD_E;
+        if (c->method == TF_METHOD_FDTD) {
+            for (int32_t e : c->fit_uncommon) fld[e] = TF_FIELD_H;
+            if (c->prec == TF_PREC_FP64) build_fit_special<double>(c, by_edge, fld);
+            else build_fit_special<float>(c, by_edge, fld);
+        } else {
+            build_tet_inject(c, by_edge, fld);
+        }
+    });
+}
+
+tf_status tf_set_probes(tf_ctx* c, const tf_probe* probes, int32_t n) {
+    if (!c || (n && !probes)) { g_err = "null argument"; return TF_ERR_ARG; }
+    return guarded([&] {
+        std::vector<ProbeDev> v(n);
+        for (int32_t i = 0; i < n; ++i) {
+            const tf_probe& p = probes[i];
+            if (p.ne < 0 || p.ne > 12 || p.nh < 0 || p.nh > 6) throw std::runtime_error("probe term count");
+            ProbeDev& d = v[i];
+            std::memset(&d, 0, sizeof(d));
+            d.ne = p.ne;
+            d.nh = p.nh;
+            for (int k = 0; k < p.ne; ++k) {
+                if (p.e_idx[k] < 0 || p.e_idx[k] >= c->E) throw std::runtime_error("probe edge out of range");
+                d.e_idx[k] = p.e_idx[k];
+                for (int q = 0; q < 3; ++q) d.e_w[k][q] = p.e_w[k][q];
+            }
+            for (int k = 0; k < p.nh; ++k) {
+                if (p.h_idx[k] < 0 || p.h_idx[k] >= c->F) throw std::runtime_error("probe face out of range");
+                d.h_idx[k] = p.h_idx[k];
+                for (int q = 0; q < 3; ++q) d.h_w[k][q] = p.h_w[k][q];
+            }
+        }
+        c->n_probes = n;
+        upload(c->probes, v);
+        c->n_samples = 0;
+        c->sample_times.clear();
+        c->sample_cap = 0;
+        c->samples.release();
+    });
+}
+
+tf_status tf_set_output(tf_ctx* c, double delay, double period) {
+    if (!c) { g_err = "null argument"; return TF_ERR_ARG; }
+    c->out_enabled = true;
+    c->next_print = delay;  // Output::Initialize (Output.hpp:54-57)
+    c->period = period;
+    return TF_OK;
+}
+
+tf_status tf_run(tf_ctx* c, int64_t first, int64_t n, const tf_cg_opts* cg, tf_stats* st) {
+    if (!c || n < 0) { g_err = "bad argument"; return TF_ERR_ARG; }
+    tf_stats local{};
+    if (!st) st = &local;
+    *st = tf_stats{};
+    return guarded([&] {
+        if (n == 0) return;
+        if (c->method == TF_METHOD_GFDTD && cg && cg->precond != TF_PRECOND_JACOBI)
+            throw std::runtime_error("learned preconditioner not trained for this context");
+        ensure_carr(c, first, n);
+        c->ktimes.clear();
+        hipEvent_t e0, e1;
+        TF_HIP(hipEventCreate(&e0));
+        TF_HIP(hipEventCreate(&e1));
+        TF_HIP(hipEventRecord(e0, c->stream));
+        const bool f64 = c->prec == TF_PREC_FP64;
+        if (c->method == TF_METHOD_FDTD) {
+            if (f64) run_fit<double>(c, first, n, st); else run_fit<float>(c, first, n, st);
+        } else if (c->method == TF_METHOD_DGA) {
+            if (f64) run_dga<double>(c, first, n, st); else run_dga<float>(c, first, n, st);
+        } else {
+            if (f64) run_gfdtd<double>(c, first, n, cg, st); else run_gfdtd<float>(c, first, n, cg, st);
+        }
+        TF_HIP(hipEventRecord(e1, c->stream));
+        TF_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        TF_HIP(hipEventElapsedTime(&ms, e0, e1));
+        st->seconds = ms * 1e-3;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        ev_flush(c);
+    });
+}
+
+tf_status tf_read_state(tf_ctx* c, double* U, double* F, double* B) {
+    if (!c) { g_err = "null argument"; return TF_ERR_ARG; }
+    return guarded([&] {
+        TF_HIP(hipStreamSynchronize(c->stream));
+        auto rd = [&](const DevBuf& b, double* dst, int64_t n) {
+            if (!dst) return;
+            if (c->prec == TF_PREC_FP64) {
+                TF_HIP(hipMemcpy(dst, b.p, n * 8, hipMemcpyDeviceToHost));
+            } else {
+                std::vector<float> tmp(n);
+                TF_HIP(hipMemcpy(tmp.data(), b.p, n * 4, hipMemcpyDeviceToHost));
+                for (int64_t i = 0; i < n; ++i) dst[i] = tmp[i];
+            }
+        };
+        rd(c->U, U, c->E);
+        rd(c->Fv, F, c->F);
+        rd(c->B, B, c->F);
+    });
+}
+
+tf_status tf_write_state(tf_ctx* c, const double* U, const double* F, const double* B) {
+    if (!c) { g_err = "null argument"; return TF_ERR_ARG; }
+    return guarded([&] {
+        TF_HIP(hipStreamSynchronize(c->stream));
+        auto wr = [&](DevBuf& b, const double* src, int64_t n) {
+            if (!src) return;
+            if (c->prec == TF_PREC_FP64) {
+                TF_HIP(hipMemcpy(b.p, src, n * 8, hipMemcpyHostToDevice));
+            } else {
+                std::vector<float> tmp(src, src + n);
+                TF_HIP(hipMemcpy(b.p, tmp.data(), n * 4, hipMemcpyHostToDevice));
+            }
+        };
+        wr(c->U, U, c->E);
+        wr(c->Fv, F, c->F);
+        wr(c->B, B, c->F);
+    });
+}
+
+tf_status tf_read_probes(tf_ctx* c, double* times, double* vals, int64_t cap, int64_t* n) {
+    if (!c || !n) { g_err = "null argument"; return TF_ERR_ARG; }
+    return guarded([&] {
+        TF_HIP(hipStreamSynchronize(c->stream));
+        *n = c->n_samples;
+        const int64_t m = std::min(cap, c->n_samples);
+        if (times)
+            for (int64_t k = 0; k < m; ++k) times[k] = c->sample_times[k];
+        if (vals && m > 0 && c->n_probes > 0)
+            TF_HIP(hipMemcpy(vals, c->samples.p, m * c->n_probes * 6 * sizeof(double), hipMemcpyDeviceToHost));
+    });
+}
+
+tf_status tf_read_poynting(tf_ctx* c, double* vals, int64_t cap, int64_t* n) {
+    if (!c || !n) { g_err = "null argument"; return TF_ERR_ARG; }
+    *n = (int64_t)c->poynting.size();
+    if (vals)
+        for (int64_t k = 0; k < std::min(cap, *n); ++k) vals[k] = c->poynting[k];
+    return TF_OK;
+}
+
+tf_status tf_energy(tf_ctx* c, double* we, double* wh) {
+    if (!c) { g_err = "null argument"; return TF_ERR_ARG; }
+    g_err = "energy diagnostic is computed by the host from tf_read_state in this build";
+    (void)we;
+    (void)wh;
+    return TF_ERR_STATE;
+}
+
+tf_status tf_kernel_times(tf_ctx* c, char* names, int32_t cap_names, double* ms_avg, int64_t* launches, int32_t cap,
+                          int32_t* n) {
+    if (!c || !n) { g_err = "null argument"; return TF_ERR_ARG; }
+    int32_t k = 0;
+    int32_t off = 0;
+    for (auto& kv : c->ktimes) {
+        if (k >= cap) break;
+        const std::string& nm = kv.first;
+        if (names && off + (int32_t)nm.size() + 1 <= cap_names) {
+            std::memcpy(names + off, nm.c_str(), nm.size() + 1);
+            off += (int32_t)nm.size() + 1;
+        }
+        if (ms_avg) ms_avg[k] = kv.second.first / std::max<int64_t>(kv.second.second, 1);
+        if (launches) launches[k] = kv.second.second;
+        ++k;
+    }
+    *n = k;
+    return TF_OK;
+}
+
+tf_status tf_set_profiling(tf_ctx* c, int32_t on) {
+    if (!c) { g_err = "null argument"; return TF_ERR_ARG; }
+    c->profiling = on != 0;
+    return TF_OK;
+}
+
+tf_status tf_device_bytes(tf_ctx* c, int64_t* bytes) {
+    if (!c || !bytes) { g_err = "null argument"; return TF_ERR_ARG; }
+    *bytes = (int64_t)c->device_bytes();
+    return TF_OK;
+}
+
+tf_status tf_destroy(tf_ctx* c) {
+    delete c;
+    return TF_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,240 @@
+// common.cpp - CSR assembly helpers, source evaluation split, dt power
+// iteration and output schedule shared by the FIT and tetrahedral builders.
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+
+#include "build.hpp"
+
+namespace tfh {
+
+tf_csr CsrStore::view() const {
+    tf_csr c{};
+    c.nrows = nrows;
+    c.ncols = ncols;
+    c.nnz = rowptr.empty() ? 0 : rowptr.back();
+    c.rowptr = rowptr.empty() ? nullptr : rowptr.data();
+    c.col = col.empty() ? nullptr : col.data();
+    c.val = val.empty() ? nullptr : val.data();
+    c.sgn = sgn.empty() ? nullptr : sgn.data();
+    return c;
+}
+
+CsrStore Triplets::to_csr(int32_t nrows, int32_t ncols, bool overwrite) const {
+    CsrStore m;
+    m.nrows = nrows;
+    m.ncols = ncols;
+    const size_t n = r.size();
+    // Counting sort by row keeps insertion order within a row.
+    std::vector<int64_t> cnt(nrows + 1, 0);
+    for (size_t k = 0; k < n; ++k) cnt[r[k] + 1]++;
+    for (int32_t i = 0; i < nrows; ++i) cnt[i + 1] += cnt[i];
+    std::vector<int64_t> pos(cnt.begin(), cnt.end() - 1);
+    std::vector<int32_t> order(n);
+    for (size_t k = 0; k < n; ++k) order[pos[r[k]]++] = (int32_t)k;
+    m.rowptr.assign(nrows + 1, 0);
+    m.col.reserve(n);
+    m.val.reserve(n);
+    std::vector<int32_t> idx;
+    for (int32_t i = 0; i < nrows; ++i) {
+        idx.assign(order.begin() + cnt[i], order.begin() + cnt[i + 1]);
+        std::stable_sort(idx.begin(), idx.end(), [&](int32_t a, int32_t b) { return c[a] < c[b]; });
+        for (size_t q = 0; q < idx.size();) {
+            int32_t j = c[idx[q]];
+            double acc = v[idx[q]];
+            size_t w = q + 1;
+            for (; w < idx.size() && c[idx[w]] == j; ++w) acc = overwrite ? v[idx[w]] : acc + v[idx[w]];
+            m.col.push_back(j);
+            m.val.push_back(acc);
+            q = w;
+        }
+        m.rowptr[i + 1] = (int64_t)m.col.size();
+    }
+    return m;
+}
+
+// ---------------------------------------------------------------- sources
+
+double SourceEval::carrier(const SourceDef& s, double t) {
+    if (t < 0) return 0;  // heaviside of Compute*Source (D:3909, D:4091)
+    double ret = s.amp;
+    if (s.st == "j") ret *= s.impedance;
+    const long double arg = t - s.delay;
+    if (s.carrier == "sin") ret *= std::sin(2 * PI_L * s.freq * arg);
+    else if (s.carrier == "cos") ret *= s.amp * std::cos(2 * PI_L * s.freq * arg);
+    else if (s.carrier == "gaussian") ret *= std::exp(-std::pow(t - s.delay, 2) / std::pow(s.width, 2));
+    else if (s.carrier == "ricker") {
+        long double a = PI_L * s.freq * arg;
+        ret *= (1 - 2 * a * a) * std::exp(-(a * a));
+    } else if (s.carrier == "gaussmod") {
+        ret *= std::exp(-std::pow(t - s.delay, 2) / std::pow(s.width, 2));
+        ret *= std::sin(2 * PI_L * s.freq * arg);
+    }
+    return ret;
+}
+
+Vec3 SourceEval::space(const SourceDef& s, const Vec3& p) {
+    double ret = 1.0;
+    if (s.decay != 0)
+        ret *= std::pow(std::pow(p[0] - s.center[0], 2) + std::pow(p[1] - s.center[1], 2) +
+                            std::pow(p[2] - s.center[2], 2),
+                        -0.5 * s.decay);
+    if (s.dir != "x" && s.dir != "y" && s.dir != "z")
+        throw Error("source direction '" + s.dir + "' (curvilinear) is not supported by this build");
+    if (s.prof == "gaussian") {
+        double e = -((std::pow(p[0] - s.center[0], 2) + std::pow(p[1] - s.center[1], 2) +
+                      std::pow(p[2] - s.center[2], 2)) /
+                     std::pow(s.radius, 2));
+        ret *= std::exp(e);
+    } else if (s.prof == "wave") {
+        for (int j = 0; j < 3; ++j) {
+            const std::string& f = s.bfuncs[j];
+            if (f == "sin") ret *= std::sin(2 * PI_L * s.kvec[j] * (long double)(p[j] - s.center[j]));
+            else if (f == "cos") ret *= std::cos(2 * PI_L * s.kvec[j] * (long double)(p[j] - s.center[j]));
+            else if (!f.empty() && (f[0] == 'J' || f[0] == 'Y' || f[0] == 'j' || f[0] == 'y' || f[0] == 'P'))
+                throw Error("source mode '" + f + "' needs Bessel/Legendre functions (GSL), not supported");
+        }
+    }
+    if (s.dir == "x") return Vec3(ret, 0, 0);
+    if (s.dir == "y") return Vec3(0, ret, 0);
+    return Vec3(0, 0, ret);
+}
+
+static int carrier_kind(const std::string& c) {
+    if (c == "sin") return TF_CARRIER_SIN;
+    if (c == "cos") return TF_CARRIER_COS;
+    if (c == "gaussian") return TF_CARRIER_GAUSSIAN;
+    if (c == "ricker") return TF_CARRIER_RICKER;
+    if (c == "gaussmod") return TF_CARRIER_GAUSSMOD;
+    return TF_CARRIER_DC;
+}
+
+static int field_kind(const std::string& st) {
+    if (st == "e") return TF_FIELD_E;
+    if (st == "j") return TF_FIELD_J;
+    return TF_FIELD_H;
+}
+
+tf_source make_source(const SourceDef& s) {
+    tf_source r{};
+    r.carrier = carrier_kind(s.carrier);
+    r.field = field_kind(s.st);
+    r.amp = s.amp;
+    r.freq = s.freq;
+    r.delay = s.delay;
+    r.width = s.width;
+    r.impedance = s.impedance;
+    return r;
+}
+
+tf_source_desc describe_source(const SourceDef& s) {
+    tf_source_desc d{};
+    d.type = field_kind(s.st);
+    d.carrier = carrier_kind(s.carrier);
+    d.profile = s.prof == "wave" ? 1 : (s.prof == "gaussian" ? 2 : 0);
+    d.dir = s.dir == "x" ? 0 : (s.dir == "y" ? 1 : 2);
+    for (int j = 0; j < 3; ++j) {
+        d.bfunc[j] = s.bfuncs[j] == "sin" ? 1 : (s.bfuncs[j] == "cos" ? 2 : 0);
+        d.kvec[j] = s.kvec[j];
+        d.center[j] = s.center[j];
+    }
+    d.amp = s.amp;
+    d.freq = s.freq;
+    d.delay = s.delay;
+    d.width = s.width;
+    d.impedance = s.impedance;
+    d.radius = s.radius;
+    d.decay = s.decay;
+    return d;
+}
+
+// ---------------------------------------------------------------- dt
+
+double power_lambda_max(size_t n,
+                        const std::function<void(const std::vector<double>&, std::vector<double>&)>& apply,
+                        int iters) {
+    // Deterministic start vector with all modes present.
+    std::vector<double> x(n), y(n);
+    uint64_t st = 0x9E3779B97F4A7C15ull;
+    for (size_t i = 0; i < n; ++i) {
+        st ^= st << 13; st ^= st >> 7; st ^= st << 17;
+        x[i] = (double)(st >> 11) * (1.0 / 9007199254740992.0) - 0.5;
+    }
+    double lam = 0;
+    for (int it = 0; it < iters; ++it) {
+        double nx = std::sqrt(std::inner_product(x.begin(), x.end(), x.begin(), 0.0));
+        if (nx == 0) return 0;
+        for (auto& v : x) v /= nx;
+        apply(x, y);
+        lam = std::inner_product(x.begin(), x.end(), y.begin(), 0.0);
+        std::swap(x, y);
+    }
+    return lam;
+}
+
+// ---------------------------------------------------------------- output
+
+void set_output(const Model& m, const SimulationDef& sim, Build& b) {
+    auto it = m.outputs.find(sim.output);
+    if (it == m.outputs.end()) {
+        b.out_mode = 0;
+        b.out_name = "simulation";
+        return;
+    }
+    const OutputDef& o = it->second;
+    const std::string& md = o.mode;
+    b.out_mode = md == "probepoint" ? 1 : md == "maxerror" ? 2 : md == "l2norm" ? 3 : md == "silo" ? 4 : 0;
+    b.out_delay = o.delay;
+    b.out_period = o.period;
+    b.out_name = o.name;
+}
+
+void Build::finalize_views() {
+    prob.n_edges = (int32_t)(method == "fdtd" ? M_h.size() : edge_class.size());
+    tf_fit_ops& f = prob.fit;
+    f = tf_fit_ops{};
+    if (method == "fdtd") {
+        f.ct_ptr = ct_ptr.data();
+        f.ct_face = ct_face.data();
+        f.ct_sgn = ct_sgn.data();
+        f.c_edge = c_edge.data();
+        f.curl = curl.data();
+        f.dual_curl = dual_curl.data();
+        f.M_h = M_h.data();
+        f.M_q = M_q.data();
+        f.upml = upml.data();
+        f.M_mu = M_mu.data();
+        f.M_nu = M_nu.data();
+        f.fpml = fpml.data();
+        f.common_edges = common_edges.data();
+        f.n_common = (int64_t)common_edges.size();
+        f.uncommon_edges = uncommon_edges.data();
+        f.n_uncommon = (int64_t)uncommon_edges.size();
+        f.bc_edges = bc_edges.data();
+        f.n_bc = (int64_t)bc_edges.size();
+        f.bc_value = bc_value.data();
+        f.src_edges = src_edges.data();
+        f.n_src = (int64_t)src_edges.size();
+        f.tbc_faces = tbc_faces.data();
+        f.n_tbc = (int64_t)tbc_faces.size();
+    } else {
+        tf_dga_ops& d = prob.dga;
+        d = tf_dga_ops{};
+        d.C = C.view();
+        d.Ct = Ct.view();
+        d.bnd_edge = bnd_edge.data();
+        d.n_bnd = n_bnd;
+        d.edge_class = edge_class.data();
+        d.fixed_value = fixed_value.data();
+        d.H = H.view();
+        d.h_row = h_row.empty() ? nullptr : h_row.data();
+        d.N = N.view();
+        d.E = E.view();
+        d.A = A.view();
+        d.RHS1 = RHS1.view();
+        d.free_edges = free_edges.empty() ? nullptr : free_edges.data();
+        d.n_free = (int32_t)free_edges.size();
+    }
+}
+
+}  // namespace tfh

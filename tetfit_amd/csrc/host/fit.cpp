@@ -1,0 +1,566 @@
+// fit.cpp - cartesian mesher and FIT operator producer (method fdtd).
+// Restates ReadStructuredMesh (src/DGA/Discretization.hpp:4347-5690):
+// cube-by-cube construction in k, j, i order with reuse of the bottom (z-),
+// left (y-) and back (x-) neighbours' entities, signed incidences C_vec /
+// Ct_vec / curl / dual_curl, material averages, edge classes
+// (common / uncommon / bc / src) and the FIT coefficients M_h, M_q, M_mu,
+// M_nu. Also the FIT probe location (FindFitProbe D:3590-3643), probe weights
+// (GetFitElectricField D:3555, GetFitMagneticField D:3570) and the boundary
+// dual edge vectors of h sources (ComputeHfieldSource D:4148-4199).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+
+#include "build.hpp"
+
+namespace tfh {
+
+namespace {
+
+// Local numbering of a cube (D:4643-4934):
+//   nodes 0..7 at offset (p&1)x + (p>>1&1)y + (p>>2&1)z
+//   faces 0 z-, 1 y-, 2 x-, 3 x+, 4 y+, 5 z+
+//   edges 0 (P0P1,x) 1 (P0P2,y) 2 (P0P4,z) 3 (P1P3,y) 4 (P1P5,z) 5 (P2P3,x)
+//         6 (P2P6,z) 7 (P3P7,z) 8 (P4P5,x) 9 (P4P6,y) 10 (P5P7,y) 11 (P6P7,x)
+const int kEdgeNodes[12][2] = {{0, 1}, {0, 2}, {0, 4}, {1, 3}, {1, 5}, {2, 3},
+                               {2, 6}, {3, 7}, {4, 5}, {4, 6}, {5, 7}, {6, 7}};
+const int kEdgeAxis[12] = {0, 1, 2, 1, 2, 0, 2, 2, 0, 1, 1, 0};
+// dual_curl pushed for a new edge: +1 for x and z edges, -1 for y (D:4960-5080)
+const int kDualCurl[3] = {1, -1, 1};
+const int kFaceEdges[6][4] = {{0, 1, 3, 5}, {0, 2, 4, 8}, {1, 2, 6, 9},
+                              {3, 4, 7, 10}, {5, 6, 7, 11}, {8, 9, 10, 11}};
+const int kFaceSigns[6][4] = {{1, -1, 1, -1}, {-1, 1, -1, 1}, {1, -1, 1, -1},
+                              {1, -1, 1, -1}, {-1, 1, -1, 1}, {1, -1, 1, -1}};
+const int kFaceCurl[6] = {1, -1, 1, 1, -1, 1};
+const int kFaceAxis[6] = {2, 1, 0, 0, 1, 2};      // normal axis
+const int kFaceLabel[6] = {1, 5, 2, 4, 3, 6};     // boundary label at creation
+// Reuse from neighbours, in priority order bottom > left > back:
+// (neighbour kind 0 bottom / 1 left / 2 back, position in the neighbour).
+struct Src { int nb, pos; };
+const std::vector<Src> kEdgeReuse[12] = {
+    {{0, 8}, {1, 5}}, {{0, 9}, {2, 3}}, {{1, 6}, {2, 4}}, {{0, 10}}, {{1, 7}}, {{0, 11}},
+    {{2, 7}},         {},               {{1, 11}},        {{2, 10}}, {},        {}};
+const std::vector<Src> kNodeReuse[8] = {{{0, 4}, {1, 2}, {2, 1}}, {{0, 5}, {1, 3}}, {{0, 6}, {2, 3}}, {{0, 7}},
+                                        {{1, 6}, {2, 5}},         {{1, 7}},         {{2, 7}},         {}};
+const std::vector<Src> kFaceReuse[6] = {{{0, 5}}, {{1, 4}}, {{2, 3}}, {}, {}, {}};
+
+// The 12 E-probe weights use the dual area of each edge's axis (D:3555-3568).
+}  // namespace
+
+void build_fit(const Model& m, const SimulationDef& sim, Build& b) {
+    auto mit = m.meshes.find(sim.mesh);
+    if (mit == m.meshes.end()) throw Error("simulation refers to an undefined mesh");
+    const MeshDef& msh = mit->second;
+    for (auto& kv : m.bcs)
+        if (kv.second.type == "pml") throw Error("PML boundary conditions are not supported by this build");
+    std::map<uint32_t, SolidDef> solids = read_solids_file(msh.file);
+    std::map<uint32_t, Material> materials = m.materials;  // operator[] may insert defaults
+    std::map<uint32_t, BoundaryCondition> bcs = m.bcs;
+    std::map<uint32_t, SourceDef> sources = m.sources;
+
+    // Material adjacency labels for interface faces (D:330-347).
+    uint32_t maxmat = materials.empty() ? 0 : materials.rbegin()->first;
+    std::vector<uint32_t> matad((size_t)(maxmat + 1) * (maxmat + 1), 0);
+    {
+        uint32_t lab = 6;
+        for (uint32_t i = 0; i <= maxmat; ++i)
+            for (uint32_t j = i + 1; j <= maxmat; ++j) {
+                matad[i * (maxmat + 1) + j] = ++lab;
+                matad[j * (maxmat + 1) + i] = lab;
+            }
+    }
+    auto matad_at = [&](uint32_t a, uint32_t c) -> uint32_t {
+        if (a > maxmat || c > maxmat) return 0;
+        return matad[a * (maxmat + 1) + c];
+    };
+
+    const double scale = msh.scale;
+    const double xmin = scale * msh.xmin, ymin = scale * msh.ymin, zmin = scale * msh.zmin;
+    const double xmax = scale * msh.xmax, ymax = scale * msh.ymax, zmax = scale * msh.zmax;
+    double Lx = scale * msh.xstep, Ly = scale * msh.ystep, Lz = scale * msh.zstep;
+    for (auto& r : m.refinements) { Lx /= r.second.x; Ly /= r.second.y; Lz /= r.second.z; }
+    const uint32_t Nx = (uint32_t)(std::fabs(xmax - xmin) / Lx);
+    const uint32_t Ny = (uint32_t)(std::fabs(ymax - ymin) / Ly);
+    const uint32_t Nz = (uint32_t)(std::fabs(zmax - zmin) / Lz);
+    if (Nx == 0 || Ny == 0 || Nz == 0) throw Error("cartesian grid has no cells");
+    const uint64_t totE = (uint64_t)Nx * (Ny + 1) * (Nz + 1) + (uint64_t)(Nx + 1) * Ny * (Nz + 1) +
+                          (uint64_t)(Nx + 1) * (Ny + 1) * Nz;
+    const uint64_t totF = (uint64_t)Nx * Ny * (Nz + 1) + (uint64_t)Nz * Nx * (Ny + 1) + (uint64_t)Ny * Nz * (Nx + 1);
+    const uint64_t totV = (uint64_t)Nx * Ny * Nz;
+    const uint64_t totP = (uint64_t)(Nx + 1) * (Ny + 1) * (Nz + 1);
+    if (totE >= (1ull << 31) || totF >= (1ull << 31)) throw Error("cartesian grid too large for int32 indices");
+
+    b.L[0] = Lx; b.L[1] = Ly; b.L[2] = Lz;
+    const double area[3] = {Ly * Lz, Lx * Lz, Lx * Ly};  // area_x, area_y, area_z
+    const double da[3] = {area[0] / 4, area[1] / 4, area[2] / 4};
+    const double len[3] = {Lx, Ly, Lz};
+
+    std::vector<int32_t>& E_cluster = b.e_cluster;  // 12 per cube
+    std::vector<int32_t>& D = b.d_faces;            // 6 per cube
+    std::vector<int32_t> P_cluster;                 // 8 per cube
+    E_cluster.reserve(totV * 12);
+    D.reserve(totV * 6);
+    P_cluster.reserve(totV * 8);
+    b.pts.reserve(totP * 3);
+    std::vector<int32_t> Dt(totF * 2, -1);   // signed cube ids, up to 2 per face: value = cube, sign separate
+    std::vector<int8_t> Dt_sgn(totF * 2, 0);
+    std::vector<uint8_t> Dt_n(totF, 0);
+    std::vector<int32_t>& G = b.g_nodes;  // 2 per edge
+    G.assign(totE * 2, -1);
+    std::vector<uint8_t> edge_has_G(totE, 0), face_has_C(totF, 0);
+    b.c_edge.assign(totF * 4, 0);
+    std::vector<int32_t> ct_face(totE * 4, 0);
+    std::vector<int8_t> ct_sgn(totE * 4, 0);
+    std::vector<uint8_t> ct_n(totE, 0);
+    b.curl.assign(totF, 1);
+    b.dual_curl.assign(totE, 0);
+    std::vector<double> edge_len(totE, 0), face_area(totF, 0);
+    std::vector<double> avg_eps(totE, 0), avg_sigma(totE, 0), avg_ni(totF, 0), avg_mag_sigma(totF, 0);
+    std::vector<uint8_t> ele_lossy(totE, 0), mag_lossy(totF, 0);
+    std::vector<uint32_t> boundary_face;
+    boundary_face.reserve(totF);
+    std::vector<uint8_t> tbc;
+    tbc.reserve(totF);
+    std::vector<uint32_t> vol_material;
+    vol_material.reserve(totV);
+
+    uint32_t nv = 0, nf = 0, ne = 0, np = 0;
+    double px, py, pz = zmin;
+    for (uint32_t k = 0; k < Nz; ++k) {
+        py = ymin;
+        for (uint32_t j = 0; j < Ny; ++j) {
+            px = xmin;
+            for (uint32_t i = 0; i < Nx; ++i) {
+                // WhichSolid D:10110: first solid (by label) containing the centre
+                uint32_t mat = 0;
+                for (auto& s : solids)
+                    if (s.second.rule(px + 0.5 * Lx, py + 0.5 * Ly, pz + 0.5 * Lz)) { mat = s.second.material; break; }
+                vol_material.push_back(mat);
+                const Material& mt = materials[mat];
+
+                uint32_t nb[3] = {0, 0, 0};  // 1-based neighbour cube ids
+                if (pz > zmin) nb[0] = nv + 1 - Nx * Ny;
+                if (py > ymin) nb[1] = nv + 1 - Nx;
+                if (px > xmin) nb[2] = nv;
+
+                int32_t dcube[6], ecube[12], pcube[8];
+                for (int f = 0; f < 6; ++f) {
+                    dcube[f] = -1;
+                    for (const Src& s : kFaceReuse[f])
+                        if (nb[s.nb]) { dcube[f] = D[(size_t)(nb[s.nb] - 1) * 6 + s.pos]; break; }
+                }
+                for (int f = 0; f < 6; ++f)
+                    if (dcube[f] < 0) dcube[f] = (int32_t)nf++;
+                for (int e = 0; e < 12; ++e) {
+                    ecube[e] = -1;
+                    for (const Src& s : kEdgeReuse[e])
+                        if (nb[s.nb]) { ecube[e] = E_cluster[(size_t)(nb[s.nb] - 1) * 12 + s.pos]; break; }
+                }
+                for (int e = 0; e < 12; ++e)
+                    if (ecube[e] < 0) ecube[e] = (int32_t)ne++;
+                const Vec3 pp(px, py, pz);
+                for (int q = 0; q < 8; ++q) {
+                    pcube[q] = -1;
+                    for (const Src& s : kNodeReuse[q])
+                        if (nb[s.nb]) { pcube[q] = P_cluster[(size_t)(nb[s.nb] - 1) * 8 + s.pos]; break; }
+                }
+                for (int q = 0; q < 8; ++q)
+                    if (pcube[q] < 0) {
+                        pcube[q] = (int32_t)np++;
+                        Vec3 p = pp;
+                        if (q & 4) p = p + Vec3(0, 0, Lz);
+                        if (q & 1) p = p + Vec3(Lx, 0, 0);
+                        if (q & 2) p = p + Vec3(0, Ly, 0);
+                        b.pts.push_back(p.x); b.pts.push_back(p.y); b.pts.push_back(p.z);
+                    }
+                for (int f = 0; f < 6; ++f) D.push_back(dcube[f]);
+                for (int e = 0; e < 12; ++e) E_cluster.push_back(ecube[e]);
+                for (int q = 0; q < 8; ++q) P_cluster.push_back(pcube[q]);
+
+                for (int f = 0; f < 6; ++f) {
+                    int32_t fid = dcube[f];
+                    uint8_t& n = Dt_n[fid];
+                    Dt[(size_t)fid * 2 + n] = (int32_t)nv;
+                    Dt_sgn[(size_t)fid * 2 + n] = f < 3 ? -1 : 1;
+                    ++n;
+                }
+                for (int e = 0; e < 12; ++e) {
+                    int32_t eid = ecube[e];
+                    if (!edge_has_G[eid]) {
+                        edge_has_G[eid] = 1;
+                        G[(size_t)eid * 2] = pcube[kEdgeNodes[e][0]];
+                        G[(size_t)eid * 2 + 1] = pcube[kEdgeNodes[e][1]];
+                        edge_len[eid] = len[kEdgeAxis[e]];
+                        b.dual_curl[eid] = (int8_t)kDualCurl[kEdgeAxis[e]];
+                    }
+                }
+                for (int f = 0; f < 6; ++f) {
+                    int32_t fid = dcube[f];
+                    if (!face_has_C[fid]) {
+                        face_has_C[fid] = 1;
+                        for (int q = 0; q < 4; ++q) {
+                            int32_t eid = ecube[kFaceEdges[f][q]];
+                            b.c_edge[(size_t)fid * 4 + q] = eid;
+                            uint8_t& n = ct_n[eid];
+                            ct_face[(size_t)eid * 4 + n] = fid;
+                            ct_sgn[(size_t)eid * 4 + n] = (int8_t)kFaceSigns[f][q];
+                            ++n;
+                        }
+                        b.curl[fid] = (int8_t)kFaceCurl[f];
+                        face_area[fid] = area[kFaceAxis[f]];
+                        boundary_face.push_back((uint32_t)kFaceLabel[f]);
+                        tbc.push_back(mat != 0);
+                    } else {
+                        uint32_t v1 = (uint32_t)Dt[(size_t)fid * 2], v2 = (uint32_t)Dt[(size_t)fid * 2 + 1];
+                        boundary_face[fid] = matad_at(vol_material[v1], vol_material[v2]);
+                        tbc[fid] = tbc[fid] || mat != 0;
+                    }
+                }
+                // material averages (D:5244-5330)
+                const double mu[3] = {mt.mu(0, 0), mt.mu(1, 1), mt.mu(2, 2)};
+                const double ch[3] = {mt.chi(0, 0), mt.chi(1, 1), mt.chi(2, 2)};
+                const double ep[3] = {mt.epsilon(0, 0), mt.epsilon(1, 1), mt.epsilon(2, 2)};
+                const double si[3] = {mt.sigma(0, 0), mt.sigma(1, 1), mt.sigma(2, 2)};
+                // faces by normal axis: x -> 2,3 ; y -> 1,4 ; z -> 0,5
+                const int fax[3][2] = {{2, 3}, {1, 4}, {0, 5}};
+                for (int a = 0; a < 3; ++a) {
+                    if (mu[a] != 0)
+                        for (int q = 0; q < 2; ++q) avg_ni[dcube[fax[a][q]]] += len[a] / 2 / mu[a];
+                }
+                for (int a = 0; a < 3; ++a) {
+                    if (ch[a] != 0)
+                        for (int q = 0; q < 2; ++q) {
+                            avg_mag_sigma[dcube[fax[a][q]]] += len[a] / 2 / ch[a];
+                            mag_lossy[dcube[fax[a][q]]]++;
+                        }
+                }
+                const int eax[3][4] = {{0, 5, 8, 11}, {1, 3, 9, 10}, {2, 4, 6, 7}};
+                for (int a = 0; a < 3; ++a)
+                    if (ep[a] != 0)
+                        for (int q = 0; q < 4; ++q) avg_eps[ecube[eax[a][q]]] += da[a] * ep[a];
+                for (int a = 0; a < 3; ++a)
+                    if (si[a] != 0)
+                        for (int q = 0; q < 4; ++q) {
+                            avg_sigma[ecube[eax[a][q]]] += da[a] * si[a];
+                            ele_lossy[ecube[eax[a][q]]]++;
+                        }
+                ++nv;
+                px += Lx;
+            }
+            py += Ly;
+        }
+        pz += Lz;
+    }
+    if (nf != totF || ne != totE || np != totP) throw Error("cartesian mesher: entity count mismatch");
+    b.n_cubes = (int32_t)nv;
+
+    // ---- Ct_vec as CSR
+    b.ct_ptr.assign(totE + 1, 0);
+    for (uint64_t e = 0; e < totE; ++e) b.ct_ptr[e + 1] = b.ct_ptr[e] + ct_n[e];
+    b.ct_face.resize(b.ct_ptr[totE]);
+    b.ct_sgn.resize(b.ct_ptr[totE]);
+    for (uint64_t e = 0; e < totE; ++e)
+        for (int q = 0; q < ct_n[e]; ++q) {
+            b.ct_face[b.ct_ptr[e] + q] = ct_face[e * 4 + q];
+            b.ct_sgn[b.ct_ptr[e] + q] = ct_sgn[e * 4 + q];
+        }
+    std::vector<int32_t>().swap(ct_face);
+
+    // ---- edge classes (D:5382-5471)
+    std::vector<std::vector<uint32_t>> edge_src(totE);
+    std::vector<uint32_t> edge_bcs(totE, 0);
+    auto bc_of = [&](uint32_t label) -> const BoundaryCondition& {
+        uint32_t l = label < m.bnd_to_bc.size() ? m.bnd_to_bc[label] : 0;
+        return bcs[l];
+    };
+    for (uint32_t e = 0; e < totE; ++e) {
+        uint32_t in_b = 0, brk = 0;
+        for (int64_t q = b.ct_ptr[e]; q < b.ct_ptr[e + 1]; ++q) {
+            uint32_t ff = (uint32_t)b.ct_face[q];
+            uint32_t bid = (uint8_t)boundary_face[ff];
+            if (boundary_face[ff] > 0) {
+                if (bc_of(bid).type == "pec") {
+                    ++in_b;
+                    edge_bcs[e] = bid < m.bnd_to_bc.size() ? m.bnd_to_bc[bid] : 0;
+                    edge_src[e].clear();
+                    b.bc_edges.push_back((int32_t)e);
+                    if (brk == 1) b.src_edges.pop_back();
+                    else if (brk == 2) b.uncommon_edges.pop_back();
+                    break;
+                } else {
+                    for (uint32_t sl : sim.sources) {
+                        const SourceDef& src = sources[sl];
+                        if (std::find(src.surface_label.begin(), src.surface_label.end(), bid) == src.surface_label.end())
+                            continue;
+                        ++in_b;
+                        if (src.st == "e") {
+                            if (brk == 2) break;
+                            if (std::find(edge_src[e].begin(), edge_src[e].end(), sl) == edge_src[e].end()) {
+                                edge_src[e].push_back(sl);
+                                brk = 1;
+                                if (b.src_edges.empty() || b.src_edges.back() != (int32_t)e) b.src_edges.push_back((int32_t)e);
+                            }
+                        } else if (src.st == "h") {
+                            if (brk == 1) break;
+                            if (std::find(edge_src[e].begin(), edge_src[e].end(), sl) == edge_src[e].end()) {
+                                edge_src[e].push_back(sl);
+                                brk = 2;
+                                if (b.uncommon_edges.empty() || b.uncommon_edges.back() != (int32_t)e)
+                                    b.uncommon_edges.push_back((int32_t)e);
+                            }
+                        }
+                    }
+                }
+            }
+        }
+        if (in_b == 0) b.common_edges.push_back((int32_t)e);
+    }
+    for (int32_t e : b.common_edges)
+        if (b.ct_ptr[e + 1] - b.ct_ptr[e] != 4)
+            throw Error("common edge without 4 faces: every outer wall must be pec or a source surface (D:1872)");
+    // ComputeEdgeBC: BCs[BndToBC[edge_bcs[e]]].GetValue() (D:3899-3905)
+    for (int32_t e : b.bc_edges) b.bc_value.push_back(bc_of(edge_bcs[e]).val);
+    for (uint32_t f = 0; f < totF; ++f)
+        if (tbc[f]) b.tbc_faces.push_back((int32_t)f);
+
+    // ---- time step: power iteration on the FIT curl-curl (D:7036-7119)
+    {
+        std::vector<double> Nv(totF), Hv(totE);
+        for (uint32_t f = 0; f < totF; ++f) Nv[f] = avg_ni[f] / face_area[f];
+        for (uint32_t e = 0; e < totE; ++e) Hv[e] = edge_len[e] / avg_eps[e];
+        std::vector<double> Fw(totF);
+        auto apply = [&](const std::vector<double>& X, std::vector<double>& Y) {
+            for (uint32_t f = 0; f < totF; ++f) {
+                const int32_t* c = &b.c_edge[(size_t)f * 4];
+                Fw[f] = Nv[f] * b.curl[f] * (X[c[0]] - X[c[1]] + X[c[2]] - X[c[3]]);
+            }
+            for (uint32_t e = 0; e < totE; ++e) {
+                double s = 0;
+                for (int64_t q = b.ct_ptr[e]; q < b.ct_ptr[e + 1]; ++q) s += b.ct_sgn[q] * Fw[b.ct_face[q]];
+                Y[e] = Hv[e] * s;
+            }
+        };
+        // The operator Hv*Ct*Nv*C is similar to a symmetric one; power
+        // iteration approaches lambda_max from below, so pad by 2%.
+        double lam = power_lambda_max(totE, apply, 60) * 1.02;
+        b.stable_dt = lam > 0 ? 2.0 / std::sqrt(lam) : 0;
+    }
+    double t_step = sim.courant * b.stable_dt;
+    if (sim.step != 0) {
+        if (sim.step > t_step) b.possibly_unstable = true;
+        else t_step = sim.step;
+    }
+    b.prob.dt = t_step;
+
+    // ---- coefficients (D:5487-5649). PML factors are 1 without PML.
+    b.M_nu.resize(totF); b.M_mu.resize(totF); b.fpml.assign(totF, 1.0);
+    for (uint32_t f = 0; f < totF; ++f) {
+        if (avg_ni[f] != 0) {
+            if (mag_lossy[f]) {
+                b.M_nu[f] = 1 / (face_area[f] * (1 / avg_ni[f] + 0.5 * t_step / avg_mag_sigma[f]));
+                b.M_mu[f] = face_area[f] * (1 / avg_ni[f] - 0.5 * t_step / avg_mag_sigma[f]);
+            } else {
+                b.M_nu[f] = avg_ni[f] / face_area[f];
+                b.M_mu[f] = face_area[f] / avg_ni[f];
+            }
+        } else { b.M_nu[f] = 0; b.M_mu[f] = 0; }
+    }
+    b.M_h.resize(totE); b.M_q.resize(totE); b.M_e.resize(totE); b.upml.assign(totE, 1.0);
+    for (uint32_t e = 0; e < totE; ++e) {
+        if (!edge_src[e].empty()) {  // source consistency (D:5573-5586)
+            std::vector<uint32_t> keep;
+            const SourceDef& s0 = sources[edge_src[e].front()];
+            uint32_t surf0 = s0.surface_label.empty() ? 0 : s0.surface_label.front();
+            for (uint32_t k : edge_src[e]) {
+                const SourceDef& sk = sources[k];
+                uint32_t sf = sk.surface_label.empty() ? 0 : sk.surface_label.front();
+                if (sk.st == s0.st && sf == surf0) keep.push_back(k);
+            }
+            edge_src[e] = keep;
+        }
+        if (avg_eps[e] != 0) {
+            if (ele_lossy[e]) {
+                b.M_h[e] = edge_len[e] / (avg_eps[e] + 0.5 * t_step * avg_sigma[e]);
+                b.M_q[e] = (avg_eps[e] - 0.5 * t_step * avg_sigma[e]) / edge_len[e];
+            } else {
+                b.M_h[e] = edge_len[e] / avg_eps[e];
+                b.M_q[e] = avg_eps[e] / edge_len[e];
+            }
+            b.M_e[e] = avg_eps[e] / edge_len[e];
+        } else { b.M_h[e] = b.M_q[e] = b.M_e[e] = 0; }
+    }
+    b.boundary_face.assign(boundary_face.begin(), boundary_face.end());
+
+    // ---- geometry helpers
+    auto P = [&](int32_t n) { return Vec3(b.pts[(size_t)n * 3], b.pts[(size_t)n * 3 + 1], b.pts[(size_t)n * 3 + 2]); };
+    auto edge_bar = [&](uint32_t e) {
+        Vec3 bc(0, 0, 0);
+        bc = bc + P(G[(size_t)e * 2]);
+        bc = bc + P(G[(size_t)e * 2 + 1]);
+        return bc / 2.0;
+    };
+    auto face_bar = [&](uint32_t f) {
+        Vec3 bc(0, 0, 0);
+        for (int q = 0; q < 4; ++q) {
+            int32_t e = b.c_edge[(size_t)f * 4 + q];
+            bc = bc + 0.5 * P(G[(size_t)e * 2]);
+            bc = bc + 0.5 * P(G[(size_t)e * 2 + 1]);
+        }
+        return bc / 4.0;
+    };
+    auto vol_bar = [&](uint32_t v) {
+        Vec3 bc(0, 0, 0);
+        for (int q = 0; q < 8; ++q) bc = bc + P(P_cluster[(size_t)v * 8 + q]);
+        return bc / 8.0;
+    };
+
+    // ---- probes (FindFitProbe, D:3590-3643) and j dipoles (D:447-492)
+    set_output(m, sim, b);
+    uint32_t root = 0;
+    auto find_fit_probe = [&](const Vec3& pv) -> uint32_t {
+        const double tol = 1e-12;
+        std::vector<uint8_t> colour(nv, 0);
+        std::vector<uint32_t> queue;
+        queue.push_back(root);
+        colour[0]++;
+        for (size_t k = 0; k < queue.size(); ++k) {
+            uint32_t v = queue[k];
+            Vec3 diff = pv - P(P_cluster[(size_t)v * 8]);
+            bool c1 = diff.x >= -tol && diff.x < Lx + tol;
+            bool c2 = diff.y >= -tol && diff.y < Ly + tol;
+            bool c3 = diff.z >= -tol && diff.z < Lz + tol;
+            if (c1 && c2 && c3) { root = v; return v; }
+            for (int f = 0; f < 6; ++f) {
+                int32_t fid = D[(size_t)v * 6 + f];
+                uint32_t cn = (uint32_t)Dt[(size_t)fid * 2];
+                if (cn == v) cn = (uint32_t)Dt[(size_t)fid * 2 + Dt_n[fid] - 1];
+                if (colour[cn] == 0) { queue.push_back(cn); colour[cn]++; }
+            }
+            colour[v]++;
+        }
+        return nv;
+    };
+    if (b.out_mode == 1 || b.out_mode == 2) {
+        const OutputDef& o = m.outputs.at(sim.output);
+        const double vol = Lx * Ly * Lz;
+        const Vec3 dual_area[3] = {Vec3(Ly * Lz / 4, 0, 0), Vec3(0, Lx * Lz / 4, 0), Vec3(0, 0, Lx * Ly / 4)};
+        const Vec3 inc[3] = {Vec3(0.5 * Lx, 0, 0), Vec3(0, 0.5 * Ly, 0), Vec3(0, 0, 0.5 * Lz)};
+        for (const Vec3& pv : o.probes) {
+            uint32_t c = find_fit_probe(pv);
+            if (c >= nv) continue;  // out of mesh: ignored with a warning
+            tf_probe pr{};
+            tf_probe_geom g{};
+            pr.ne = 12;
+            pr.nh = 6;
+            for (int q = 0; q < 12; ++q) {
+                pr.e_idx[q] = E_cluster[(size_t)c * 12 + q];
+                Vec3 w = dual_area[kEdgeAxis[q]] / vol;
+                pr.e_w[q][0] = w.x; pr.e_w[q][1] = w.y; pr.e_w[q][2] = w.z;
+                g.edges[q] = pr.e_idx[q];
+            }
+            Mat3 mui = materials[vol_material[c]].mu.inverse();
+            for (int q = 0; q < 6; ++q) {
+                pr.h_idx[q] = D[(size_t)c * 6 + q];
+                Vec3 w = (mui * inc[kFaceAxis[q]]) / vol;
+                pr.h_w[q][0] = w.x; pr.h_w[q][1] = w.y; pr.h_w[q][2] = w.z;
+                g.faces[q] = pr.h_idx[q];
+            }
+            g.elem = (int32_t)c;
+            g.point[0] = pv.x; g.point[1] = pv.y; g.point[2] = pv.z;
+            g.L[0] = Lx; g.L[1] = Ly; g.L[2] = Lz;
+            std::memcpy(g.mu, materials[vol_material[c]].mu.m, sizeof(g.mu));
+            b.probes.push_back(pr);
+            b.probe_geom.push_back(g);
+            b.probe_points.push_back(pv.x); b.probe_points.push_back(pv.y); b.probe_points.push_back(pv.z);
+        }
+        for (uint32_t sl : sim.sources) {
+            SourceDef& s = sources[sl];
+            if (s.st != "j") continue;
+            Vec3 loc(s.location[0], s.location[1], s.location[2]);
+            uint32_t c = find_fit_probe(loc);
+            if (c >= nv) continue;
+            int32_t best = E_cluster[(size_t)c * 12];
+            double bd = norm(edge_bar(best) - loc);
+            for (int q = 1; q < 12; ++q) {
+                int32_t e = E_cluster[(size_t)c * 12 + q];
+                double d = norm(edge_bar(e) - loc);
+                if (d < bd) { bd = d; best = e; }
+            }
+            if (edge_src[best].empty()) b.src_edges.push_back(best);
+            edge_src[best].push_back(sl);
+        }
+    }
+
+    // ---- sources: per-edge weights. h sources on uncommon edges use the
+    // boundary dual edge vector (D:4148-4199); e/j sources the edge vector.
+    std::map<uint32_t, int32_t> src_index;
+    for (uint32_t sl : sim.sources) {
+        src_index[sl] = (int32_t)b.sources.size();
+        b.sources.push_back(make_source(sources[sl]));
+        b.src_desc.push_back(describe_source(sources[sl]));
+    }
+    auto bnd_dual = [&](uint32_t e) {
+        Vec3 eb = edge_bar(e), v(0, 0, 0), vb(0, 0, 0), backup(0, 0, 0);
+        int nfb = 0, ni = 0;
+        for (int64_t q = b.ct_ptr[e]; q < b.ct_ptr[e + 1]; ++q) {
+            uint32_t ff = (uint32_t)b.ct_face[q];
+            if (boundary_face[ff] > 0) {
+                ++nfb;
+                backup = vol_bar((uint32_t)Dt[(size_t)ff * 2]) - eb;
+                Vec3 fb = face_bar(ff);
+                if (nfb == 1) v = fb;
+                else if (nfb == 2) v = fb - v;
+            } else {
+                vb = face_bar(ff) - eb;
+                ++ni;
+            }
+        }
+        if (!ni) vb = backup;
+        Vec3 ev = P(G[(size_t)e * 2 + 1]) - P(G[(size_t)e * 2]);
+        if (dot(cross(ev, v), vb) < 0) v = -v;
+        return v;
+    };
+    auto add_edge = [&](uint32_t e) {
+        if (edge_src[e].empty()) return;
+        tf_source_edge d{};
+        d.edge = (int32_t)e;
+        const SourceDef& s0 = sources[edge_src[e].front()];
+        d.field = s0.st == "e" ? TF_FIELD_E : (s0.st == "j" ? TF_FIELD_J : TF_FIELD_H);
+        Vec3 eb = edge_bar(e);
+        Vec3 ev = P(G[(size_t)e * 2 + 1]) - P(G[(size_t)e * 2]);
+        Vec3 vec = d.field == TF_FIELD_H ? bnd_dual(e) : ev;
+        d.bary[0] = eb.x; d.bary[1] = eb.y; d.bary[2] = eb.z;
+        d.vec[0] = vec.x; d.vec[1] = vec.y; d.vec[2] = vec.z;
+        for (uint32_t sl : edge_src[e]) {
+            const SourceDef& s = sources[sl];
+            int want = s.st == "e" ? TF_FIELD_E : (s.st == "j" ? TF_FIELD_J : TF_FIELD_H);
+            if (d.n_src < 4) d.src[d.n_src++] = src_index[sl];
+            if (want != d.field) continue;  // the loop only evaluates sources of the edge's kind
+            Vec3 sp = SourceEval::space(s, eb);
+            double w = dot(sp, vec);
+            if (d.field == TF_FIELD_J) w = (1.0 / norm(ev)) * w;
+            tf_source_term t{};
+            t.edge = (int32_t)e;
+            t.source = src_index[sl];
+            t.weight = w;
+            b.terms.push_back(t);
+        }
+        b.src_edge_desc.push_back(d);
+    };
+    for (int32_t e : b.uncommon_edges) add_edge((uint32_t)e);
+    for (int32_t e : b.src_edges) add_edge((uint32_t)e);
+
+    b.prob.method = TF_METHOD_FDTD;
+    b.prob.n_faces = (int32_t)totF;
+    b.prob.n_vols = (int32_t)nv;
+    b.prob.duration = sim.duration;
+    b.n_pts = (int32_t)np;
+    b.first_step = 0;
+    // for (i=0; i*t_step <= simulation_time; ++i)  (D:1840)
+    int64_t i = 0;
+    while ((double)i * t_step <= sim.duration) ++i;
+    b.n_steps = i;
+}
+
+}  // namespace tfh

@@ -1,0 +1,995 @@
+// tet.cpp - netgen tetrahedral mesh reader and the DGA operator producers.
+// Restates ReadUnstructuredMesh (src/DGA/Discretization.hpp:5868-6926):
+// node-sorted tets sorted by node tuple, faces and edges numbered by
+// sort+unique of node tuples, signed incidences (fte +1,-1,+1), boundary dual
+// edge vectors and Ctb, pec / source marking and edge classes; then
+// ConstructCodecasaMaterialMatrices (D:8694-9513, method dga) and
+// ConstructGFDTDMaterialMatrices (D:8132-8692, method gfdtd), FindProbe
+// (D:3645-3747) and the Whitney / face probe weights (D:3768-3897).
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <numeric>
+#include <sstream>
+
+#include "build.hpp"
+
+namespace tfh {
+
+namespace {
+
+struct Reader {
+    std::vector<char> buf;
+    const char* p = nullptr;
+    const char* end = nullptr;
+    explicit Reader(const std::string& path) {
+        std::ifstream f(path, std::ios::binary);
+        if (!f.is_open()) throw Error("cannot open mesh file " + path);
+        f.seekg(0, std::ios::end);
+        size_t n = (size_t)f.tellg();
+        f.seekg(0);
+        buf.resize(n + 1);
+        f.read(buf.data(), (std::streamsize)n);
+        buf[n] = 0;
+        p = buf.data();
+        end = p + n;
+    }
+    double num() {
+        char* e;
+        double v = std::strtod(p, &e);
+        if (e == p) throw Error("malformed netgen mesh file");
+        p = e;
+        return v;
+    }
+    uint64_t uint() {
+        char* e;
+        unsigned long long v = std::strtoull(p, &e, 10);
+        if (e == p) throw Error("malformed netgen mesh file");
+        p = e;
+        return v;
+    }
+};
+
+struct Tri { uint32_t a, b, c, slot; };
+struct Ed { uint32_t a, b, slot; };
+
+// Cholesky solve of an SPD matrix (n x n, row-major) for its inverse.
+bool spd_inverse(const std::vector<double>& A, int n, std::vector<double>& inv) {
+    std::vector<double> L(n * n, 0.0);
+    for (int j = 0; j < n; ++j) {
+        double s = A[j * n + j];
+        for (int k = 0; k < j; ++k) s -= L[j * n + k] * L[j * n + k];
+        if (!(s > 0)) return false;
+        double d = std::sqrt(s);
+        L[j * n + j] = d;
+        for (int i = j + 1; i < n; ++i) {
+            double t = A[i * n + j];
+            for (int k = 0; k < j; ++k) t -= L[i * n + k] * L[j * n + k];
+            L[i * n + j] = t / d;
+        }
+    }
+    inv.assign(n * n, 0.0);
+    std::vector<double> y(n), x(n);
+    for (int c = 0; c < n; ++c) {
+        for (int i = 0; i < n; ++i) {
+            double t = i == c ? 1.0 : 0.0;
+            for (int k = 0; k < i; ++k) t -= L[i * n + k] * y[k];
+            y[i] = t / L[i * n + i];
+        }
+        for (int i = n - 1; i >= 0; --i) {
+            double t = y[i];
+            for (int k = i + 1; k < n; ++k) t -= L[k * n + i] * x[k];
+            x[i] = t / L[i * n + i];
+        }
+        for (int i = 0; i < n; ++i) inv[i * n + c] = x[i];
+    }
+    return true;
+}
+
+// CSR pattern from (row, col) pairs: rows sorted, duplicates removed.
+CsrStore pattern_from_pairs(int32_t nrows, int32_t ncols, const std::vector<int32_t>& pr,
+                            const std::vector<int32_t>& pc) {
+    CsrStore m;
+    m.nrows = nrows;
+    m.ncols = ncols;
+    std::vector<int64_t> cnt(nrows + 1, 0);
+    for (int32_t r : pr) cnt[r + 1]++;
+    for (int32_t i = 0; i < nrows; ++i) cnt[i + 1] += cnt[i];
+    std::vector<int32_t> cols(pr.size());
+    std::vector<int64_t> pos(cnt.begin(), cnt.end() - 1);
+    for (size_t k = 0; k < pr.size(); ++k) cols[pos[pr[k]]++] = pc[k];
+    m.rowptr.assign(nrows + 1, 0);
+    m.col.reserve(pr.size() / 2);
+    for (int32_t i = 0; i < nrows; ++i) {
+        auto b = cols.begin() + cnt[i], e = cols.begin() + cnt[i + 1];
+        std::sort(b, e);
+        auto u = std::unique(b, e);
+        m.col.insert(m.col.end(), b, u);
+        m.rowptr[i + 1] = (int64_t)m.col.size();
+    }
+    m.val.assign(m.col.size(), 0.0);
+    return m;
+}
+
+inline double& csr_at(CsrStore& m, int32_t r, int32_t c) {
+    auto b = m.col.begin() + m.rowptr[r], e = m.col.begin() + m.rowptr[r + 1];
+    auto it = std::lower_bound(b, e, c);
+    if (it == e || *it != c) throw Error("internal: sparse pattern miss");
+    return m.val[it - m.col.begin()];
+}
+
+}  // namespace
+
+void build_tet(const Model& m, const SimulationDef& sim, Build& b) {
+    auto mit = m.meshes.find(sim.mesh);
+    if (mit == m.meshes.end()) throw Error("simulation refers to an undefined mesh");
+    const MeshDef& msh = mit->second;
+    if (msh.mesher == "gmsh") throw Error("gmsh meshes are not supported by this build (write netgen neutral)");
+    for (auto& kv : m.bcs)
+        if (kv.second.type == "pml") throw Error("PML boundary conditions are not supported by this build");
+    std::map<uint32_t, Material> materials = m.materials;
+    std::map<uint32_t, BoundaryCondition> bcs = m.bcs;
+    std::map<uint32_t, SourceDef> sources = m.sources;
+    const bool is_dga = sim.method == "dga";
+    const double scale = msh.scale;
+
+    Reader rd(msh.file);
+    // ---- points
+    const uint64_t np = rd.uint();
+    std::vector<Vec3> pts(np);
+    for (uint64_t i = 0; i < np; ++i) {
+        double x = rd.num(), y = rd.num(), z = rd.num();
+        pts[i] = Vec3(x, y, z) * scale;
+    }
+    // ---- tetrahedra (node-sorted, label 7 dropped, sorted by node tuple)
+    const uint64_t nt_file = rd.uint();
+    struct TetIn { uint32_t n[4]; uint32_t mat; };
+    std::vector<TetIn> tin;
+    tin.reserve(nt_file);
+    for (uint64_t i = 0; i < nt_file; ++i) {
+        TetIn t;
+        t.mat = (uint32_t)rd.uint();
+        for (int k = 0; k < 4; ++k) {
+            uint64_t v = rd.uint();
+            if (v < 1 || v > np) throw Error("netgen mesh: tetrahedron node out of range");
+            t.n[k] = (uint32_t)(v - 1);
+        }
+        std::sort(t.n, t.n + 4);
+        if (t.mat != 7) tin.push_back(t);
+    }
+    std::sort(tin.begin(), tin.end(), [](const TetIn& a, const TetIn& c) {
+        return std::lexicographical_compare(a.n, a.n + 4, c.n, c.n + 4);
+    });
+    const uint32_t T = (uint32_t)tin.size();
+    if (T == 0) throw Error("netgen mesh has no tetrahedra");
+    b.vols.resize((size_t)T * 4);
+    b.vol_material.resize(T);
+    b.cell_volumes.resize(T);
+    std::vector<Mat3> dvp_eps(np), dvp_sig(np);
+    std::vector<uint8_t> cls_node(np, 0);
+    std::vector<uint32_t> av_cnt(np + 1, 0);
+    std::vector<Tri> tri((size_t)T * 4);
+    for (uint32_t t = 0; t < T; ++t) {
+        const uint32_t* n = tin[t].n;
+        for (int k = 0; k < 4; ++k) { b.vols[(size_t)t * 4 + k] = (int32_t)n[k]; av_cnt[n[k] + 1]++; }
+        b.vol_material[t] = (int32_t)tin[t].mat;
+        tri[t] = {n[0], n[1], n[2], t};
+        tri[t + T] = {n[0], n[1], n[3], t + T};
+        tri[t + 2 * (size_t)T] = {n[0], n[2], n[3], t + 2 * T};
+        tri[t + 3 * (size_t)T] = {n[1], n[2], n[3], t + 3 * T};
+        Vec3 v1 = pts[n[1]] - pts[n[0]], v2 = pts[n[2]] - pts[n[0]], v3 = pts[n[3]] - pts[n[0]];
+        b.cell_volumes[t] = dot(v1, cross(v2, v3)) / 6.0;
+        const Material& mt = materials[tin[t].mat];
+        for (int k = 0; k < 4; ++k) {
+            uint32_t p = n[k];
+            if (!dvp_eps[p].is_zero()) {
+                if (mt.sigma != dvp_sig[p]) cls_node[p] |= 4;
+                if (mt.epsilon != dvp_eps[p]) cls_node[p] |= 2;
+                if (!dvp_sig[p].is_zero()) cls_node[p] |= 1;
+            } else {
+                dvp_eps[p] = mt.epsilon;
+                dvp_sig[p] = mt.sigma;
+                if (!dvp_sig[p].is_zero()) cls_node[p] |= 1;
+            }
+        }
+    }
+    std::vector<TetIn>().swap(tin);
+    // associated_volumes as CSR (tets in increasing order per node)
+    for (uint64_t i = 0; i < np; ++i) av_cnt[i + 1] += av_cnt[i];
+    std::vector<uint32_t> av(av_cnt[np]);
+    {
+        std::vector<uint32_t> pos(av_cnt.begin(), av_cnt.end() - 1);
+        for (uint32_t t = 0; t < T; ++t)
+            for (int k = 0; k < 4; ++k) av[pos[b.vols[(size_t)t * 4 + k]]++] = t;
+    }
+
+    // ---- faces: sort + unique (unique() D:10173-10219)
+    std::vector<uint32_t> flab((size_t)T * 4);
+    std::sort(tri.begin(), tri.end(), [](const Tri& x, const Tri& y) {
+        if (x.a != y.a) return x.a < y.a;
+        if (x.b != y.b) return x.b < y.b;
+        return x.c < y.c;
+    });
+    std::vector<uint32_t> fn;
+    {
+        uint32_t it = 0;
+        flab[tri[0].slot] = 0;
+        fn.push_back(tri[0].a); fn.push_back(tri[0].b); fn.push_back(tri[0].c);
+        for (size_t k = 1; k < tri.size(); ++k) {
+            if (!(tri[k].a == tri[k - 1].a && tri[k].b == tri[k - 1].b && tri[k].c == tri[k - 1].c)) {
+                ++it;
+                fn.push_back(tri[k].a); fn.push_back(tri[k].b); fn.push_back(tri[k].c);
+            }
+            flab[tri[k].slot] = it;
+        }
+    }
+    std::vector<Tri>().swap(tri);
+    const uint32_t F = (uint32_t)(fn.size() / 3);
+    // ftv (up to 2 tets per face, with signs) and vtf (4 faces per tet)
+    std::vector<int32_t> ftv_t((size_t)F * 2, -1);
+    std::vector<int8_t> ftv_s((size_t)F * 2, 0);
+    std::vector<uint8_t> ftv_n(F, 0);
+    b.vtf.resize((size_t)T * 4);
+    std::vector<int8_t> vtf_sgn((size_t)T * 4);
+    for (uint32_t k = 0; k < T; ++k) {
+        const uint32_t l[4] = {flab[k], flab[k + T], flab[k + 2 * (size_t)T], flab[k + 3 * (size_t)T]};
+        const int8_t s[4] = {-1, 1, -1, 1};  // vol_signs is always +1 (D:6186)
+        for (int q = 0; q < 4; ++q) {
+            uint8_t& n = ftv_n[l[q]];
+            if (n >= 2) throw Error("non-manifold tetrahedral mesh: a face shared by more than 2 tets");
+            ftv_t[(size_t)l[q] * 2 + n] = (int32_t)k;
+            ftv_s[(size_t)l[q] * 2 + n] = s[q];
+            ++n;
+        }
+        // vtf_list[k] = {f4, f3, f2, f1} (D:6263-6266)
+        b.vtf[(size_t)k * 4 + 0] = (int32_t)l[3]; vtf_sgn[(size_t)k * 4 + 0] = 1;
+        b.vtf[(size_t)k * 4 + 1] = (int32_t)l[2]; vtf_sgn[(size_t)k * 4 + 1] = -1;
+        b.vtf[(size_t)k * 4 + 2] = (int32_t)l[1]; vtf_sgn[(size_t)k * 4 + 2] = 1;
+        b.vtf[(size_t)k * 4 + 3] = (int32_t)l[0]; vtf_sgn[(size_t)k * 4 + 3] = -1;
+    }
+    std::vector<uint32_t>().swap(flab);
+
+    // ---- edges: sort + unique with etn / nte / dual_star_offsets (D:10221-10299)
+    std::vector<Ed> ed((size_t)F * 3);
+    for (uint32_t f = 0; f < F; ++f) {
+        uint32_t p0 = fn[(size_t)f * 3], p1 = fn[(size_t)f * 3 + 1], p2 = fn[(size_t)f * 3 + 2];
+        ed[f] = {p0, p1, f};
+        ed[f + F] = {p0, p2, f + F};
+        ed[f + 2 * (size_t)F] = {p1, p2, f + 2 * F};
+    }
+    std::sort(ed.begin(), ed.end(), [](const Ed& x, const Ed& y) {
+        return x.a != y.a ? x.a < y.a : x.b < y.b;
+    });
+    std::vector<uint32_t> elab((size_t)F * 3);
+    std::vector<uint32_t> en;
+    {
+        uint32_t it = 0;
+        elab[ed[0].slot] = 0;
+        en.push_back(ed[0].a); en.push_back(ed[0].b);
+        for (size_t k = 1; k < ed.size(); ++k) {
+            if (!(ed[k].a == ed[k - 1].a && ed[k].b == ed[k - 1].b)) {
+                ++it;
+                en.push_back(ed[k].a); en.push_back(ed[k].b);
+            }
+            elab[ed[k].slot] = it;
+        }
+    }
+    std::vector<Ed>().swap(ed);
+    const uint32_t E = (uint32_t)(en.size() / 2);
+    // nte (edges around a node, increasing edge id) as CSR
+    std::vector<uint32_t> nte_ptr(np + 1, 0);
+    for (uint32_t e = 0; e < E; ++e) { nte_ptr[en[2 * (size_t)e] + 1]++; nte_ptr[en[2 * (size_t)e + 1] + 1]++; }
+    for (uint64_t i = 0; i < np; ++i) nte_ptr[i + 1] += nte_ptr[i];
+    std::vector<uint32_t> nte(nte_ptr[np]);
+    {
+        std::vector<uint32_t> pos(nte_ptr.begin(), nte_ptr.end() - 1);
+        for (uint32_t e = 0; e < E; ++e) {
+            nte[pos[en[2 * (size_t)e]]++] = e;
+            nte[pos[en[2 * (size_t)e + 1]]++] = e;
+        }
+    }
+    // fte: 3 signed edges per face, +1,-1,+1 (D:6345-6352)
+    std::vector<int32_t> fte((size_t)F * 3);
+    for (uint32_t f = 0; f < F; ++f) {
+        fte[(size_t)f * 3] = (int32_t)elab[f];
+        fte[(size_t)f * 3 + 1] = (int32_t)elab[f + F];
+        fte[(size_t)f * 3 + 2] = (int32_t)elab[f + 2 * (size_t)F];
+    }
+    std::vector<uint32_t>().swap(elab);
+
+    auto P = [&](uint32_t i) -> const Vec3& { return pts[i]; };
+    std::vector<Vec3> face_bars(F), edge_bars(E);
+    for (uint32_t f = 0; f < F; ++f) {
+        Vec3 bc = P(fn[(size_t)f * 3]) + P(fn[(size_t)f * 3 + 1]) + P(fn[(size_t)f * 3 + 2]);
+        face_bars[f] = bc / 3.0;
+    }
+    for (uint32_t e = 0; e < E; ++e) edge_bars[e] = 0.5 * (P(en[2 * (size_t)e]) + P(en[2 * (size_t)e + 1]));
+    auto vol_bar = [&](uint32_t t) {
+        const int32_t* n = &b.vols[(size_t)t * 4];
+        return (P(n[0]) + P(n[1]) + P(n[2]) + P(n[3])) / 4.0;
+    };
+
+    // ---- C, boundary faces, bnd edges and Ctb, face classes (D:6320-6546)
+    b.boundary_face.assign(F, 0);
+    std::vector<int32_t> bnd_edges(E, -1);
+    std::vector<Vec3> bnd_dual;
+    std::vector<uint8_t> face_class(F, 1);
+    for (uint32_t k = 0; k < F; ++k) {
+        const int nv = ftv_n[k];
+        bool recombine = true;
+        if (nv == 2) {
+            const Material& m1 = materials[b.vol_material[ftv_t[(size_t)k * 2]]];
+            const Material& m2 = materials[b.vol_material[ftv_t[(size_t)k * 2 + 1]]];
+            if (!m1.chi.is_zero() || !m2.chi.is_zero()) {
+                if (m1.chi != m2.chi) recombine = false;
+                else if (m1.mu != m2.mu) recombine = false;
+            }
+        } else if (nv == 1) {
+            b.boundary_face[k] = 1;
+            for (int q = 0; q < 3; ++q) {
+                int32_t e = fte[(size_t)k * 3 + q];
+                if (bnd_edges[e] < 0) {
+                    bnd_edges[e] = (int32_t)bnd_dual.size();
+                    bnd_dual.push_back(face_bars[k]);
+                } else {
+                    Vec3& v = bnd_dual[bnd_edges[e]];
+                    v = face_bars[k] - v;
+                    Vec3 edgvec = P(en[2 * (size_t)e + 1]) - P(en[2 * (size_t)e]);
+                    Vec3 a = P(fn[(size_t)k * 3]), c1 = P(fn[(size_t)k * 3 + 1]), c2 = P(fn[(size_t)k * 3 + 2]);
+                    Vec3 facvec = cross(c1 - a, c2 - a);
+                    Vec3 volvec = vol_bar((uint32_t)ftv_t[(size_t)k * 2]) - face_bars[k];
+                    if (dot(facvec, volvec) >= 0) facvec = -facvec;
+                    if (dot(cross(edgvec, facvec), v) > 0) v = -v;
+                    v = -v;
+                }
+            }
+        } else {
+            throw Error("Conductor boundary cannot be on mesh boundary!");
+        }
+        const Material& m0 = materials[b.vol_material[ftv_t[(size_t)k * 2]]];
+        if (recombine && m0.chi.is_zero()) face_class[k] = 1;
+        else if (recombine) face_class[k] = 2;
+        else face_class[k] = 3;
+    }
+    b.n_bnd = (int32_t)bnd_dual.size();
+
+    // ---- boundary triangles: labels, pec, sources (D:6599-6740)
+    std::vector<uint32_t> edge_bcs(E, 0);
+    std::vector<std::vector<uint32_t>> edge_src(E);
+    std::vector<uint8_t> is_pec(E, 0);
+    auto bc_type = [&](uint32_t label) -> const std::string& { return bcs[label].type; };
+    const uint64_t ntri = rd.uint();
+    for (uint64_t i = 0; i < ntri; ++i) {
+        uint32_t bid = (uint32_t)rd.uint();
+        uint32_t v[3];
+        for (int q = 0; q < 3; ++q) v[q] = (uint32_t)(rd.uint() - 1);
+        std::sort(v, v + 3);
+        // lower_bound in the sorted face list
+        uint32_t lo = 0, hi = F;
+        while (lo < hi) {
+            uint32_t mid = (lo + hi) / 2;
+            const uint32_t* c = &fn[(size_t)mid * 3];
+            bool less = c[0] != v[0] ? c[0] < v[0] : (c[1] != v[1] ? c[1] < v[1] : c[2] < v[2]);
+            if (less) lo = mid + 1; else hi = mid;
+        }
+        if (lo >= F) continue;
+        const uint32_t* c = &fn[(size_t)lo * 3];
+        if (!(c[0] == v[0] && c[1] == v[1] && c[2] == v[2])) continue;
+        const uint32_t fl = lo;
+        b.boundary_face[fl] = (int32_t)bid;
+        const uint32_t which_bc = bid < m.bnd_to_bc.size() ? m.bnd_to_bc[bid] : 0;
+        if (which_bc != 0) {
+            if (bc_type(which_bc) == "pec") {
+                for (int q = 0; q < 3; ++q) {
+                    int32_t e = fte[(size_t)fl * 3 + q];
+                    edge_bcs[e] = which_bc;
+                    edge_src[e].clear();
+                    is_pec[e] = 1;
+                    if (bnd_edges[e] >= 0) {
+                        bnd_dual[bnd_edges[e]] = Vec3(0, 0, 0);
+                        bnd_edges[e] = -1;
+                    }
+                }
+            }
+        } else {
+            std::vector<int32_t> edgs;
+            for (int q = 0; q < 3; ++q) {
+                int32_t e = fte[(size_t)fl * 3 + q];
+                if (!is_pec[e]) edgs.push_back(e);
+            }
+            for (uint32_t sl : sim.sources) {
+                const SourceDef& src = sources[sl];
+                if (std::find(src.surface_label.begin(), src.surface_label.end(), bid) == src.surface_label.end()) continue;
+                if (src.st == "b") continue;  // face sources are not used by the time loops
+                for (int32_t e : edgs)
+                    if (std::find(edge_src[e].begin(), edge_src[e].end(), sl) == edge_src[e].end()) edge_src[e].push_back(sl);
+            }
+        }
+    }
+
+    // ---- edge classes (D:6770-6905)
+    b.edge_class.assign(E, 0);
+    b.h_row.assign(E, -1);
+    b.fixed_value.assign(E, 0.0);
+    int32_t H_size = 0;
+    for (uint32_t k = 0; k < E; ++k) {
+        if (!edge_src[k].empty()) {
+            std::vector<uint32_t> keep;
+            const SourceDef& s0 = sources[edge_src[k].front()];
+            uint32_t surf0 = s0.surface_label.empty() ? 0 : s0.surface_label.front();
+            for (uint32_t j : edge_src[k]) {
+                const SourceDef& sj = sources[j];
+                uint32_t sf = sj.surface_label.empty() ? 0 : sj.surface_label.front();
+                if (sj.st == s0.st && sf == surf0) keep.push_back(j);
+            }
+            edge_src[k] = keep;
+        }
+        const uint32_t p0 = en[2 * (size_t)k], p1 = en[2 * (size_t)k + 1];
+        const uint32_t rec0 = cls_node[p0], rec1 = cls_node[p1];
+        bool treated = false;
+        if (edge_bcs[k] != 0 && bc_type(edge_bcs[k]) == "pec") {
+            b.edge_class[k] = 4;
+            treated = true;
+        } else if (!edge_src[k].empty()) {
+            for (uint32_t s : edge_src[k])
+                if (sources[s].st == "e") { b.edge_class[k] = 4; treated = true; break; }
+        }
+        if (!treated) {
+            if (rec0 > 2 || rec1 > 2 || rec0 + rec1 == 1 || rec0 + rec1 == 3 || dvp_sig[p0] != dvp_sig[p1])
+                b.edge_class[k] = 3;
+            else if (rec0 == 2 || rec1 == 2 || rec0 == 0 || rec1 == 0) {
+                b.edge_class[k] = 1;
+                b.h_row[k] = H_size++;
+            } else
+                b.edge_class[k] = 2;
+        }
+        if (b.edge_class[k] == 4 && edge_bcs[k] != 0) {
+            // ComputeEdgeBC: BCs[BndToBC[edge_bcs[e]]].GetValue() (D:3899-3905)
+            uint32_t l = edge_bcs[k] < m.bnd_to_bc.size() ? m.bnd_to_bc[edge_bcs[k]] : 0;
+            b.fixed_value[k] = bcs[l].val;
+        }
+    }
+    if (is_dga) {
+        for (uint32_t k = 0; k < E; ++k)
+            if (b.edge_class[k] == 2 || b.edge_class[k] == 3)
+                throw Error("method dga with lossy media (edge classes 2/3) is not supported by this build");
+        for (uint32_t f = 0; f < F; ++f)
+            if (face_class[f] != 1)
+                throw Error("method dga with magnetic losses (face classes 2/3) is not supported by this build");
+    }
+
+    // ---- C (faces x edges) and its transpose
+    {
+        CsrStore& C = b.C;
+        C.nrows = (int32_t)F;
+        C.ncols = (int32_t)E;
+        C.rowptr.resize(F + 1);
+        C.col.resize((size_t)F * 3);
+        C.sgn.resize((size_t)F * 3);
+        for (uint32_t f = 0; f < F; ++f) {
+            C.rowptr[f] = (int64_t)f * 3;
+            for (int q = 0; q < 3; ++q) {
+                C.col[(size_t)f * 3 + q] = fte[(size_t)f * 3 + q];
+                C.sgn[(size_t)f * 3 + q] = q == 1 ? -1 : 1;
+            }
+        }
+        C.rowptr[F] = (int64_t)F * 3;
+        CsrStore& Ct = b.Ct;
+        Ct.nrows = (int32_t)E;
+        Ct.ncols = (int32_t)F;
+        Ct.rowptr.assign(E + 1, 0);
+        for (size_t q = 0; q < C.col.size(); ++q) Ct.rowptr[C.col[q] + 1]++;
+        for (uint32_t e = 0; e < E; ++e) Ct.rowptr[e + 1] += Ct.rowptr[e];
+        Ct.col.resize(C.col.size());
+        Ct.sgn.resize(C.col.size());
+        std::vector<int64_t> pos(Ct.rowptr.begin(), Ct.rowptr.end() - 1);
+        for (uint32_t f = 0; f < F; ++f)  // increasing face order within each edge row
+            for (int q = 0; q < 3; ++q) {
+                int32_t e = C.col[(size_t)f * 3 + q];
+                Ct.col[pos[e]] = (int32_t)f;
+                Ct.sgn[pos[e]++] = C.sgn[(size_t)f * 3 + q];
+            }
+    }
+    b.bnd_edge = bnd_edges;
+
+    // ---- local geometry per tet
+    // local edge order after sort_unique + swap(1,2) + swap(1,3) (D:10314-10316):
+    // 0:(n0n1) 1:(n1n2) 2:(n0n2) 3:(n0n3) 4:(n1n3) 5:(n2n3)
+    const int kLocEdge[6][2] = {{0, 1}, {1, 2}, {0, 2}, {0, 3}, {1, 3}, {2, 3}};
+    auto edge_id = [&](uint32_t a, uint32_t c) -> int32_t {
+        // a < c; nte of a is sorted by edge id and edges are sorted by (a,c)
+        for (uint32_t q = nte_ptr[a]; q < nte_ptr[a + 1]; ++q) {
+            uint32_t e = nte[q];
+            if (en[2 * (size_t)e] == a && en[2 * (size_t)e + 1] == c) return (int32_t)e;
+        }
+        throw Error("internal: edge lookup");
+    };
+    std::vector<int32_t> tet_edges((size_t)T * 6);
+    for (uint32_t t = 0; t < T; ++t) {
+        const int32_t* n = &b.vols[(size_t)t * 4];
+        for (int q = 0; q < 6; ++q) tet_edges[(size_t)t * 6 + q] = edge_id(n[kLocEdge[q][0]], n[kLocEdge[q][1]]);
+    }
+    // dual_area_vectors (D:10301-10351)
+    auto dual_area_vectors = [&](uint32_t t, Vec3 out[6]) {
+        Vec3 ebs[6], fbs[4];
+        for (int q = 0; q < 6; ++q) ebs[q] = edge_bars[tet_edges[(size_t)t * 6 + q]];
+        for (int q = 0; q < 4; ++q) fbs[q] = face_bars[b.vtf[(size_t)t * 4 + q]];
+        Vec3 vb = vol_bar(t);
+        double sign = b.cell_volumes[t] < 0 ? -0.5 : 0.5;
+        out[0] = cross(vb - ebs[0], fbs[2] - fbs[3]) * sign;
+        out[1] = cross(vb - ebs[1], fbs[0] - fbs[3]) * sign;
+        out[2] = cross(vb - ebs[2], fbs[3] - fbs[1]) * sign;
+        out[3] = cross(vb - ebs[3], fbs[1] - fbs[2]) * sign;
+        out[4] = cross(vb - ebs[4], fbs[2] - fbs[0]) * sign;
+        out[5] = cross(vb - ebs[5], fbs[0] - fbs[1]) * sign;
+    };
+    // primal_area_vectors (D:10397-10418)
+    auto primal_area_vectors = [&](uint32_t t, Vec3 out[4]) {
+        const int32_t* n = &b.vols[(size_t)t * 4];
+        Vec3 ev0 = P(n[1]) - P(n[0]), ev1 = P(n[2]) - P(n[1]), ev2 = P(n[2]) - P(n[0]);
+        Vec3 ev3 = P(n[3]) - P(n[0]), ev4 = P(n[3]) - P(n[1]);
+        out[0] = cross(ev1, ev4) / 2.0;
+        out[1] = cross(ev2, ev3) / 2.0;
+        out[2] = cross(ev0, ev3) / 2.0;
+        out[3] = cross(ev0, ev2) / 2.0;
+    };
+    // local_M of the Codecasa / gfdtd mu block (D:8757-8776)
+    auto local_M = [&](uint32_t t, double M[16]) {
+        Vec3 fv[6];
+        dual_area_vectors(t, fv);
+        const Mat3& mu = materials[b.vol_material[t]].mu;
+        auto d = [&](int i, int j) { return dot(fv[i], mu * fv[j]); };
+        double coeff = 36.0 / std::fabs(b.cell_volumes[t]);
+        M[0] = (d(0, 0) + d(2, 2) + d(3, 3)) * coeff;
+        M[5] = (d(0, 0) + d(1, 1) + d(4, 4)) * coeff;
+        M[10] = (d(1, 1) + d(2, 2) + d(5, 5)) * coeff;
+        M[15] = (d(3, 3) + d(4, 4) + d(5, 5)) * coeff;
+        M[1] = (-d(1, 2) - d(3, 4)) * coeff;
+        M[2] = (-d(0, 1) + d(3, 5)) * coeff;
+        M[3] = (d(0, 4) + d(2, 5)) * coeff;
+        M[6] = (-d(0, 2) - d(4, 5)) * coeff;
+        M[7] = (d(0, 3) - d(1, 5)) * coeff;
+        M[11] = (-d(2, 3) - d(1, 4)) * coeff;
+        M[4] = M[1]; M[8] = M[2]; M[12] = M[3]; M[9] = M[6]; M[13] = M[7]; M[14] = M[11];
+    };
+
+    // ---- N = sum over tets of local_M^-1 (rows n_index = face id for class-1 faces)
+    {
+        std::vector<int32_t> pr, pc;
+        pr.reserve((size_t)T * 16);
+        pc.reserve((size_t)T * 16);
+        for (uint32_t t = 0; t < T; ++t)
+            for (int j = 0; j < 4; ++j)
+                for (int k = 0; k < 4; ++k) { pr.push_back(b.vtf[(size_t)t * 4 + j]); pc.push_back(b.vtf[(size_t)t * 4 + k]); }
+        b.N = pattern_from_pairs((int32_t)F, (int32_t)F, pr, pc);
+        std::vector<double> Mv(16), inv;
+        double M[16];
+        for (uint32_t t = 0; t < T; ++t) {
+            local_M(t, M);
+            Mv.assign(M, M + 16);
+            if (!spd_inverse(Mv, 4, inv)) throw Error("local mu block not positive definite (degenerate tet)");
+            for (int j = 0; j < 4; ++j)
+                for (int k = 0; k < 4; ++k)
+                    if (inv[j * 4 + k] != 0) csr_at(b.N, b.vtf[(size_t)t * 4 + j], b.vtf[(size_t)t * 4 + k]) += inv[j * 4 + k];
+        }
+    }
+
+    // Corner data of tet t at its local node k (D:8913-9078): local edges and
+    // the face area vector / sign factor that define the dual basis f_i.
+    // k=0: edges (0,2,3) faces (1,2,3) s=-1 ; k=1: (0,1,4) faces (0,2,3) s=(+,-,-)
+    // k=2: (1,2,5) faces (1,0,3) s=(+,+,-) ; k=3: (3,4,5) faces (0,1,2) s=+
+    const int kCornerEdges[4][3] = {{0, 2, 3}, {0, 1, 4}, {1, 2, 5}, {3, 4, 5}};
+    const int kCornerFaces[4][3] = {{1, 2, 3}, {0, 2, 3}, {1, 0, 3}, {0, 1, 2}};
+    const double kCornerSign[4][3] = {{-1, -1, -1}, {1, -1, -1}, {1, 1, -1}, {1, 1, 1}};
+
+    const double sim_dt_request = sim.step;
+    double t_step = 0;
+    set_output(m, sim, b);
+
+    if (is_dga) {
+        // ---- node-star eps blocks, local inverses and H (D:8819-9121, 9263-9371)
+        // H row pattern of a class-1 edge (a,b): star(a) U star(b) without class-4 edges.
+        std::vector<int32_t> pr, pc;
+        {
+            CsrStore& H = b.H;
+            H.nrows = H_size;
+            H.ncols = (int32_t)E;
+            H.rowptr.assign(H_size + 1, 0);
+            std::vector<int32_t> row;
+            for (uint32_t e = 0; e < E; ++e) {
+                if (b.h_row[e] < 0) continue;
+                row.clear();
+                for (int s = 0; s < 2; ++s) {
+                    uint32_t n = en[2 * (size_t)e + s];
+                    for (uint32_t q = nte_ptr[n]; q < nte_ptr[n + 1]; ++q)
+                        if (b.edge_class[nte[q]] != 4) row.push_back((int32_t)nte[q]);
+                }
+                std::sort(row.begin(), row.end());
+                row.erase(std::unique(row.begin(), row.end()), row.end());
+                H.col.insert(H.col.end(), row.begin(), row.end());
+                H.rowptr[b.h_row[e] + 1] = (int64_t)H.col.size();
+            }
+            H.val.assign(H.col.size(), 0.0);
+        }
+        std::vector<double> LE, LH;
+        Vec3 fa[4];
+        for (uint32_t nid = 0; nid < np; ++nid) {
+            const uint32_t deg = nte_ptr[nid + 1] - nte_ptr[nid];
+            if (deg == 0) continue;
+            const uint32_t* gi = &nte[nte_ptr[nid]];
+            LE.assign((size_t)deg * deg, 0.0);
+            auto loc = [&](int32_t e) { return (uint32_t)(std::lower_bound(gi, gi + deg, (uint32_t)e) - gi); };
+            for (uint32_t q = av_cnt[nid]; q < av_cnt[nid + 1]; ++q) {
+                const uint32_t t = av[q];
+                const int32_t* n = &b.vols[(size_t)t * 4];
+                int k = 0;
+                while (n[k] != (int32_t)nid) ++k;
+                primal_area_vectors(t, fa);
+                const double vol = std::fabs(b.cell_volumes[t]);
+                const Mat3& eps = materials[b.vol_material[t]].epsilon;
+                Vec3 f[3];
+                uint32_t le[3];
+                for (int i = 0; i < 3; ++i) {
+                    int face = kCornerFaces[k][i];
+                    double sg = vtf_sgn[(size_t)t * 4 + face];
+                    f[i] = fa[face] * (kCornerSign[k][i] * 2.0 / 3.0 / vol * sg);
+                    le[i] = loc(tet_edges[(size_t)t * 6 + kCornerEdges[k][i]]);
+                }
+                for (int i = 0; i < 3; ++i) LE[(size_t)le[i] * deg + le[i]] += dot(f[i], eps * f[i]) * vol / 4;
+                const int pairs[3][2] = {{0, 1}, {0, 2}, {1, 2}};
+                for (auto& pq : pairs) {
+                    double v = dot(f[pq[0]], eps * f[pq[1]]) * vol / 4;
+                    double& a = LE[(size_t)le[pq[0]] * deg + le[pq[1]]];
+                    a = a + v;
+                    LE[(size_t)le[pq[1]] * deg + le[pq[0]]] = a;
+                }
+            }
+            // class-4 rows/cols -> identity, invert, zero them again (D:9270-9333)
+            for (uint32_t j = 0; j < deg; ++j)
+                if (b.edge_class[gi[j]] == 4) {
+                    for (uint32_t c = 0; c < deg; ++c) LE[(size_t)j * deg + c] = LE[(size_t)c * deg + j] = 0;
+                    LE[(size_t)j * deg + j] = 1;
+                }
+            if (!spd_inverse(LE, (int)deg, LH)) throw Error("node-star eps block not positive definite");
+            for (uint32_t j = 0; j < deg; ++j)
+                if (b.edge_class[gi[j]] == 4)
+                    for (uint32_t c = 0; c < deg; ++c) LH[(size_t)j * deg + c] = LH[(size_t)c * deg + j] = 0;
+            for (uint32_t jj = 0; jj < deg; ++jj) {
+                if (b.edge_class[gi[jj]] != 1) continue;
+                for (uint32_t kk = jj; kk < deg; ++kk) {
+                    if (b.edge_class[gi[kk]] == 4 || LH[(size_t)jj * deg + kk] == 0) continue;
+                    const double v = LH[(size_t)jj * deg + kk];
+                    csr_at(b.H, b.h_row[gi[jj]], (int32_t)gi[kk]) += v;
+                    if (jj != kk) csr_at(b.H, b.h_row[gi[kk]], (int32_t)gi[jj]) += v;
+                }
+            }
+        }
+        // ---- dt: power iteration on H * Ct * N * C (ComputeDGATimeStep D:7121)
+        {
+            std::vector<double> U(E), cu(F), Fw(F), cf(E);
+            auto apply = [&](const std::vector<double>& x, std::vector<double>& y) {
+                // x, y live on H rows (class-1 edges)
+                std::fill(U.begin(), U.end(), 0.0);
+                for (uint32_t e = 0; e < E; ++e) if (b.h_row[e] >= 0) U[e] = x[b.h_row[e]];
+                for (uint32_t f = 0; f < F; ++f) {
+                    double s = 0;
+                    for (int q = 0; q < 3; ++q) s += b.C.sgn[(size_t)f * 3 + q] * U[b.C.col[(size_t)f * 3 + q]];
+                    cu[f] = s;
+                }
+                for (uint32_t f = 0; f < F; ++f) {
+                    double s = 0;
+                    for (int64_t q = b.N.rowptr[f]; q < b.N.rowptr[f + 1]; ++q) s += b.N.val[q] * cu[b.N.col[q]];
+                    Fw[f] = s;
+                }
+                for (uint32_t e = 0; e < E; ++e) {
+                    double s = 0;
+                    for (int64_t q = b.Ct.rowptr[e]; q < b.Ct.rowptr[e + 1]; ++q) s += b.Ct.sgn[q] * Fw[b.Ct.col[q]];
+                    cf[e] = s;
+                }
+                for (int32_t r = 0; r < H_size; ++r) {
+                    double s = 0;
+                    for (int64_t q = b.H.rowptr[r]; q < b.H.rowptr[r + 1]; ++q) s += b.H.val[q] * cf[b.H.col[q]];
+                    y[r] = s;
+                }
+            };
+            double lam = power_lambda_max((size_t)H_size, apply, 60) * 1.02;
+            b.stable_dt = lam > 0 ? 2.0 / std::sqrt(lam) : 0;
+        }
+    } else {
+        // ---- gfdtd: E from the tet-corner dual basis (D:8379-8420), A, RHSmat1
+        std::vector<int32_t> pr, pc;
+        pr.reserve((size_t)T * 36);
+        pc.reserve((size_t)T * 36);
+        for (uint32_t t = 0; t < T; ++t)
+            for (int k = 0; k < 4; ++k)
+                for (int i = 0; i < 3; ++i)
+                    for (int j = 0; j < 3; ++j) {
+                        pr.push_back(tet_edges[(size_t)t * 6 + kCornerEdges[k][i]]);
+                        pc.push_back(tet_edges[(size_t)t * 6 + kCornerEdges[k][j]]);
+                    }
+        b.E = pattern_from_pairs((int32_t)E, (int32_t)E, pr, pc);
+        std::vector<int32_t>().swap(pr);
+        std::vector<int32_t>().swap(pc);
+        for (uint32_t nid = 0; nid < np; ++nid) {
+            for (uint32_t q = av_cnt[nid]; q < av_cnt[nid + 1]; ++q) {
+                const uint32_t t = av[q];
+                const int32_t* n = &b.vols[(size_t)t * 4];
+                int k = 0;
+                while (n[k] != (int32_t)nid) ++k;
+                const double omega = std::fabs(b.cell_volumes[t]) / double(4);
+                const Mat3& eps = materials[b.vol_material[t]].epsilon;
+                int32_t e[3];
+                for (int i = 0; i < 3; ++i) e[i] = tet_edges[(size_t)t * 6 + kCornerEdges[k][i]];
+                Vec3 a1 = edge_bars[e[0]] - P(nid), a2 = edge_bars[e[1]] - P(nid), a3 = edge_bars[e[2]] - P(nid);
+                double nc = std::pow(dot(a1, cross(a2, a3)), -1);
+                Vec3 f[3] = {nc * cross(a2, a3), nc * cross(a3, a1), nc * cross(a1, a2)};
+                Mat3 oe;
+                for (int z = 0; z < 9; ++z) oe.m[z] = omega * eps.m[z];
+                // push order of D:8420-8429: (1,1),(2,2),(3,3),(1,2),(2,1),(1,3),(3,1),(2,3),(3,2)
+                const int order[9][2] = {{0, 0}, {1, 1}, {2, 2}, {0, 1}, {1, 0}, {0, 2}, {2, 0}, {1, 2}, {2, 1}};
+                for (auto& o : order) csr_at(b.E, e[o[0]], e[o[1]]) += dot(f[o[0]], oe * f[o[1]]);
+            }
+        }
+        // free (non-Dirichlet) edges: compressed_dirichlet (D:8145-8171)
+        std::vector<int32_t> is_dir(E, -1);
+        for (uint32_t i = 0; i < E; ++i) {
+            bool no_bc = edge_bcs[i] == 0 || bc_type(edge_bcs[i]) == "none";
+            if (!no_bc) continue;
+            bool dirich = false;
+            for (uint32_t s : edge_src[i])
+                if (sources[s].st == "e") { dirich = true; break; }
+            if (!dirich) {
+                is_dir[i] = (int32_t)b.free_edges.size();
+                b.free_edges.push_back((int32_t)i);
+            }
+        }
+        const int32_t nfree = (int32_t)b.free_edges.size();
+        // A = E restricted to free edges; RHSmat1 = E(free rows, dirichlet cols)
+        b.A.nrows = b.A.ncols = nfree;
+        b.A.rowptr.assign(nfree + 1, 0);
+        b.RHS1.nrows = b.RHS1.ncols = (int32_t)E;
+        b.RHS1.rowptr.assign(E + 1, 0);
+        for (uint32_t r = 0; r < E; ++r) {
+            if (is_dir[r] >= 0) {
+                for (int64_t q = b.E.rowptr[r]; q < b.E.rowptr[r + 1]; ++q) {
+                    int32_t c = b.E.col[q];
+                    if (is_dir[c] >= 0) { b.A.col.push_back(is_dir[c]); b.A.val.push_back(b.E.val[q]); }
+                    else { b.RHS1.col.push_back(c); b.RHS1.val.push_back(b.E.val[q]); }
+                }
+                b.A.rowptr[is_dir[r] + 1] = (int64_t)b.A.col.size();
+            }
+            b.RHS1.rowptr[r + 1] = (int64_t)b.RHS1.col.size();
+        }
+        // sigma (SigMat) is needed only for lossy media; not supported in gfdtd here
+        for (auto& kv : materials)
+            if (!kv.second.sigma.is_zero()) {
+                for (uint32_t t = 0; t < T; ++t)
+                    if ((uint32_t)b.vol_material[t] == kv.first)
+                        throw Error("method gfdtd with lossy media (SigMat) is not supported by this build");
+            }
+        // ---- dt: generalized power iteration on (Ct N C, A) (ComputeFEMTimeStep D:6928)
+        {
+            std::vector<double> U(E), cu(F), Fw(F), cf(E), r(nfree), z(nfree), pp(nfree), qq(nfree), y(nfree);
+            std::vector<double> dinv(nfree, 1.0);
+            for (int32_t i = 0; i < nfree; ++i)
+                for (int64_t q = b.A.rowptr[i]; q < b.A.rowptr[i + 1]; ++q)
+                    if (b.A.col[q] == i && b.A.val[q] != 0) dinv[i] = 1.0 / b.A.val[q];
+            auto Kx = [&](const std::vector<double>& x, std::vector<double>& out) {
+                std::fill(U.begin(), U.end(), 0.0);
+                for (int32_t i = 0; i < nfree; ++i) U[b.free_edges[i]] = x[i];
+                for (uint32_t f = 0; f < F; ++f) {
+                    double s = 0;
+                    for (int q = 0; q < 3; ++q) s += b.C.sgn[(size_t)f * 3 + q] * U[b.C.col[(size_t)f * 3 + q]];
+                    cu[f] = s;
+                }
+                for (uint32_t f = 0; f < F; ++f) {
+                    double s = 0;
+                    for (int64_t q = b.N.rowptr[f]; q < b.N.rowptr[f + 1]; ++q) s += b.N.val[q] * cu[b.N.col[q]];
+                    Fw[f] = s;
+                }
+                for (int32_t i = 0; i < nfree; ++i) {
+                    int32_t e = b.free_edges[i];
+                    double s = 0;
+                    for (int64_t q = b.Ct.rowptr[e]; q < b.Ct.rowptr[e + 1]; ++q) s += b.Ct.sgn[q] * Fw[b.Ct.col[q]];
+                    out[i] = s;
+                }
+            };
+            auto Ax = [&](const std::vector<double>& x, std::vector<double>& out) {
+                for (int32_t i = 0; i < nfree; ++i) {
+                    double s = 0;
+                    for (int64_t q = b.A.rowptr[i]; q < b.A.rowptr[i + 1]; ++q) s += b.A.val[q] * x[b.A.col[q]];
+                    out[i] = s;
+                }
+            };
+            auto solveA = [&](const std::vector<double>& rhs, std::vector<double>& x) {
+                std::fill(x.begin(), x.end(), 0.0);
+                r = rhs;
+                double r0 = std::sqrt(std::inner_product(r.begin(), r.end(), r.begin(), 0.0)), rho = 0, rho_p;
+                if (r0 == 0) return;
+                for (int it = 0; it < 200; ++it) {
+                    for (int32_t i = 0; i < nfree; ++i) z[i] = dinv[i] * r[i];
+                    rho_p = rho;
+                    rho = std::inner_product(r.begin(), r.end(), z.begin(), 0.0);
+                    if (it == 0) pp = z;
+                    else for (int32_t i = 0; i < nfree; ++i) pp[i] = z[i] + rho / rho_p * pp[i];
+                    Ax(pp, qq);
+                    double a = rho / std::inner_product(pp.begin(), pp.end(), qq.begin(), 0.0);
+                    for (int32_t i = 0; i < nfree; ++i) { x[i] += a * pp[i]; r[i] -= a * qq[i]; }
+                    if (std::sqrt(std::inner_product(r.begin(), r.end(), r.begin(), 0.0)) / r0 < 1e-6) break;
+                }
+            };
+            std::vector<double> kx(nfree), w(nfree);
+            auto apply = [&](const std::vector<double>& x, std::vector<double>& yy) {
+                Kx(x, kx);
+                solveA(kx, yy);
+            };
+            // Rayleigh quotient in the A inner product after power steps
+            std::vector<double> x(nfree);
+            uint64_t st = 0x9E3779B97F4A7C15ull;
+            for (auto& v : x) { st ^= st << 13; st ^= st >> 7; st ^= st << 17; v = (double)(st >> 11) * 1.1102230246251565e-16 - 0.5; }
+            double lam = 0;
+            for (int it = 0; it < 40; ++it) {
+                apply(x, y);
+                double n2 = std::sqrt(std::inner_product(y.begin(), y.end(), y.begin(), 0.0));
+                if (n2 == 0) break;
+                for (int32_t i = 0; i < nfree; ++i) x[i] = y[i] / n2;
+                Kx(x, kx);
+                Ax(x, w);
+                lam = std::inner_product(x.begin(), x.end(), kx.begin(), 0.0) /
+                      std::inner_product(x.begin(), x.end(), w.begin(), 0.0);
+            }
+            lam *= 1.02;
+            b.stable_dt = lam > 0 ? 2.0 / std::sqrt(lam) : 0;
+        }
+        b.cg_tol = sim.tol;
+    }
+    t_step = sim.courant * b.stable_dt;
+    if (sim_dt_request != 0) {
+        if (sim_dt_request > t_step) b.possibly_unstable = true;
+        else t_step = sim_dt_request;
+    }
+    b.prob.dt = t_step;
+
+    // ---- probes (FindProbe D:3645) with Whitney E and face H weights
+    uint32_t root = 0;
+    auto det4 = [](const Vec3 r[4]) {
+        // det of rows [x y z 1]
+        Vec3 a = r[1] - r[0], c = r[2] - r[0], d = r[3] - r[0];
+        return -dot(a, cross(c, d));
+    };
+    auto find_probe = [&](const Vec3& pv) -> uint32_t {
+        const double tol = 1e-12;
+        std::vector<uint8_t> colour(T, 0);
+        std::vector<uint32_t> queue;
+        queue.push_back(root);
+        colour[0]++;
+        for (size_t k = 0; k < queue.size(); ++k) {
+            uint32_t v = queue[k];
+            const int32_t* n = &b.vols[(size_t)v * 4];
+            Vec3 r[4] = {P(n[0]), P(n[1]), P(n[2]), P(n[3])};
+            double d0 = det4(r);
+            bool ok = true;
+            for (int s = 0; s < 4 && ok; ++s) {
+                Vec3 rr[4] = {r[0], r[1], r[2], r[3]};
+                rr[s] = pv;
+                double d1 = det4(rr);
+                ok = (d0 >= -tol && d1 >= -tol) || (d0 <= tol && d1 <= tol);
+            }
+            if (ok) { root = v; return v; }
+            for (int q = 0; q < 4; ++q) {
+                int32_t f = b.vtf[(size_t)v * 4 + q];
+                uint32_t cn = (uint32_t)ftv_t[(size_t)f * 2];
+                if (cn == v) cn = (uint32_t)ftv_t[(size_t)f * 2 + ftv_n[f] - 1];
+                if (colour[cn] == 0) { queue.push_back(cn); colour[cn]++; }
+            }
+            colour[v]++;
+        }
+        return T;
+    };
+    if (b.out_mode == 1 || b.out_mode == 2) {
+        const OutputDef& o = m.outputs.at(sim.output);
+        for (const Vec3& pv : o.probes) {
+            uint32_t t = find_probe(pv);
+            if (t >= T) continue;
+            const int32_t* n = &b.vols[(size_t)t * 4];
+            Vec3 r[4] = {P(n[0]), P(n[1]), P(n[2]), P(n[3])};
+            // barycentric coordinates and gradients: lambda_i(p) with grad
+            // lambda_i = rows of the inverse of [r1-r0, r2-r0, r3-r0]
+            Vec3 a = r[1] - r[0], c = r[2] - r[0], d = r[3] - r[0];
+            double det = dot(a, cross(c, d));
+            Vec3 g1 = cross(c, d) / det, g2 = cross(d, a) / det, g3 = cross(a, c) / det;
+            Vec3 g0 = -(g1 + g2 + g3);
+            Vec3 grad[4] = {g0, g1, g2, g3};
+            double lam[4];
+            Vec3 dp = pv - r[0];
+            lam[1] = dot(g1, dp); lam[2] = dot(g2, dp); lam[3] = dot(g3, dp);
+            lam[0] = 1 - lam[1] - lam[2] - lam[3];
+            tf_probe pr{};
+            tf_probe_geom g{};
+            pr.ne = 6;
+            pr.nh = 4;
+            for (int q = 0; q < 6; ++q) {
+                int i = kLocEdge[q][0], j = kLocEdge[q][1];
+                Vec3 w = lam[i] * grad[j] - lam[j] * grad[i];
+                pr.e_idx[q] = tet_edges[(size_t)t * 6 + q];
+                pr.e_w[q][0] = w.x; pr.e_w[q][1] = w.y; pr.e_w[q][2] = w.z;
+                g.edges[q] = pr.e_idx[q];
+            }
+            // GetMagneticField D:3877-3897
+            Vec3 pev[6];
+            for (int q = 0; q < 6; ++q) pev[q] = P(n[kLocEdge[q][1]]) - P(n[kLocEdge[q][0]]);
+            Vec3 coef[4] = {pev[0] + pev[2] + pev[3], pev[0] - pev[1] - pev[4], -pev[2] - pev[1] + pev[5],
+                            pev[3] + pev[4] + pev[5]};
+            Mat3 mui = materials[b.vol_material[t]].mu.inverse();
+            double v3 = 3 * b.cell_volumes[t];
+            for (int q = 0; q < 4; ++q) {
+                Vec3 w = (mui * (coef[q] / v3)) * 0.25;
+                pr.h_idx[q] = b.vtf[(size_t)t * 4 + q];
+                pr.h_w[q][0] = w.x; pr.h_w[q][1] = w.y; pr.h_w[q][2] = w.z;
+                g.faces[q] = pr.h_idx[q];
+            }
+            g.elem = (int32_t)t;
+            g.point[0] = pv.x; g.point[1] = pv.y; g.point[2] = pv.z;
+            for (int q = 0; q < 4; ++q) { g.nodes[q][0] = r[q].x; g.nodes[q][1] = r[q].y; g.nodes[q][2] = r[q].z; }
+            std::memcpy(g.mu, materials[b.vol_material[t]].mu.m, sizeof(g.mu));
+            g.volume = b.cell_volumes[t];
+            b.probes.push_back(pr);
+            b.probe_geom.push_back(g);
+            b.probe_points.push_back(pv.x); b.probe_points.push_back(pv.y); b.probe_points.push_back(pv.z);
+        }
+        for (uint32_t sl : sim.sources)
+            if (sources[sl].st == "j") throw Error("j dipole sources in dga/gfdtd are not supported by this build (D:835-865 reads an uninitialised value)");
+    }
+
+    // ---- sources on edges: h (bnd dual vector), e (edge vector)
+    std::map<uint32_t, int32_t> src_index;
+    for (uint32_t sl : sim.sources) {
+        src_index[sl] = (int32_t)b.sources.size();
+        b.sources.push_back(make_source(sources[sl]));
+        b.src_desc.push_back(describe_source(sources[sl]));
+    }
+    for (uint32_t e = 0; e < E; ++e) {
+        if (edge_src[e].empty()) continue;
+        const SourceDef& s0 = sources[edge_src[e].front()];
+        tf_source_edge d{};
+        d.edge = (int32_t)e;
+        d.field = s0.st == "e" ? TF_FIELD_E : (s0.st == "j" ? TF_FIELD_J : TF_FIELD_H);
+        if (d.field == TF_FIELD_H && bnd_edges[e] < 0) continue;  // gfdtd checks bnd_edges >= 0 (D:1488)
+        Vec3 ev = P(en[2 * (size_t)e + 1]) - P(en[2 * (size_t)e]);
+        Vec3 vec = d.field == TF_FIELD_H ? bnd_dual[bnd_edges[e]] : ev;
+        const Vec3& eb = edge_bars[e];
+        d.bary[0] = eb.x; d.bary[1] = eb.y; d.bary[2] = eb.z;
+        d.vec[0] = vec.x; d.vec[1] = vec.y; d.vec[2] = vec.z;
+        for (uint32_t sl : edge_src[e]) {
+            const SourceDef& s = sources[sl];
+            int want = s.st == "e" ? TF_FIELD_E : (s.st == "j" ? TF_FIELD_J : TF_FIELD_H);
+            if (d.n_src < 4) d.src[d.n_src++] = src_index[sl];
+            if (want != d.field) continue;
+            Vec3 sp = SourceEval::space(s, eb);
+            double w = dot(sp, vec);
+            if (d.field == TF_FIELD_J) w = (1.0 / norm(ev)) * w;
+            b.terms.push_back(tf_source_term{(int32_t)e, src_index[sl], w});
+        }
+        b.src_edge_desc.push_back(d);
+    }
+
+    b.face_nodes.assign(fn.begin(), fn.end());
+    b.edge_nodes.assign(en.begin(), en.end());
+    b.pts.resize(np * 3);
+    for (uint64_t i = 0; i < np; ++i) { b.pts[i * 3] = pts[i].x; b.pts[i * 3 + 1] = pts[i].y; b.pts[i * 3 + 2] = pts[i].z; }
+    b.n_pts = (int32_t)np;
+    b.prob.method = is_dga ? TF_METHOD_DGA : TF_METHOD_GFDTD;
+    b.prob.n_faces = (int32_t)F;
+    b.prob.n_vols = (int32_t)T;
+    b.prob.duration = sim.duration;
+    b.first_step = 1;
+    // for (i=1; i*t_step <= simulation_time; ++i)  (D:810, D:1459)
+    int64_t i = 1;
+    while ((double)i * t_step <= sim.duration) ++i;
+    b.n_steps = i - 1;
+}
+
+}  // namespace tfh

@@ -1,0 +1,116 @@
+"""Python view of the stepping backend (libtetfit_hip.so): one device context
+per simulation, created from a host Build (tf_create / tf_set_sources /
+tf_set_probes / tf_set_output / tf_run / tf_read_*)."""
+import ctypes as C
+
+import numpy as np
+
+from . import _ffi
+from ._ffi import hip_lib
+
+PREC = {"fp64": 0, "fp32": 1}
+PRECOND = {"jacobi": 0, "learned": 1}
+
+
+class DeviceError(RuntimeError):
+    pass
+
+
+def _check(rc):
+    if rc != 0:
+        raise DeviceError(f"tf status {rc}: {hip_lib().tf_last_error().decode()}")
+
+
+def device_count():
+    return hip_lib().tf_device_count()
+
+
+class Simulation:
+    """Device context for one assembled simulation."""
+
+    def __init__(self, build, precision="fp64", output=True):
+        L = hip_lib()
+        self.build = build
+        self.precision = precision
+        h = C.c_void_p()
+        _check(L.tf_create(build.problem_ptr, PREC[precision], C.byref(h)))
+        self._h = h
+        s, ns, t, nt = build.sources_raw()
+        _check(L.tf_set_sources(h, s, ns, t, nt))
+        p, _, npr = build.probes_raw()
+        self.n_probes = npr
+        _check(L.tf_set_probes(h, p, npr))
+        if output and build.output_mode in (1, 2):
+            _check(L.tf_set_output(h, build.output_delay, build.output_period))
+        self.next_step = build.first_step
+
+    def close(self):
+        if getattr(self, "_h", None):
+            hip_lib().tf_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def run(self, n=None, first=None, cg_tol=None, cg_maxiter=100, precond="jacobi"):
+        L = hip_lib()
+        first = self.next_step if first is None else first
+        n = self.build.n_steps - (first - self.build.first_step) if n is None else n
+        opts = _ffi.tf_cg_opts(self.build.cg_tol if cg_tol is None else cg_tol, int(cg_maxiter), PRECOND[precond])
+        st = _ffi.tf_stats()
+        _check(L.tf_run(self._h, int(first), int(n), C.byref(opts), C.byref(st)))
+        self.next_step = first + n
+        return dict(steps=st.steps_done, cg_iters_total=st.cg_iters_total, cg_iters_max=st.cg_iters_max,
+                    cg_not_converged=st.cg_not_converged, cg_max_relres=st.cg_max_relres, seconds=st.seconds)
+
+    def state(self):
+        U = np.zeros(self.build.n_edges)
+        F = np.zeros(self.build.n_faces)
+        B = np.zeros(self.build.n_faces)
+        dp = C.POINTER(C.c_double)
+        _check(hip_lib().tf_read_state(self._h, U.ctypes.data_as(dp), F.ctypes.data_as(dp), B.ctypes.data_as(dp)))
+        return U, F, B
+
+    def set_state(self, U=None, F=None, B=None):
+        dp = C.POINTER(C.c_double)
+        conv = lambda a: None if a is None else np.ascontiguousarray(a, dtype=np.float64).ctypes.data_as(dp)
+        keep = [U, F, B]
+        _check(hip_lib().tf_write_state(self._h, conv(U), conv(F), conv(B)))
+        del keep
+
+    def probes(self):
+        L = hip_lib()
+        n = C.c_int64()
+        _check(L.tf_read_probes(self._h, None, None, 0, C.byref(n)))
+        k = n.value
+        times = np.zeros(k)
+        vals = np.zeros(max(k * self.n_probes * 6, 1))
+        dp = C.POINTER(C.c_double)
+        _check(L.tf_read_probes(self._h, times.ctypes.data_as(dp), vals.ctypes.data_as(dp), k, C.byref(n)))
+        return times, vals[:k * self.n_probes * 6].reshape(k, self.n_probes, 6)
+
+    def poynting(self):
+        L = hip_lib()
+        n = C.c_int64()
+        _check(L.tf_read_poynting(self._h, None, 0, C.byref(n)))
+        v = np.zeros(n.value)
+        _check(L.tf_read_poynting(self._h, v.ctypes.data_as(C.POINTER(C.c_double)), n.value, C.byref(n)))
+        return v
+
+    def set_profiling(self, on=True):
+        _check(hip_lib().tf_set_profiling(self._h, 1 if on else 0))
+
+    def kernel_times(self):
+        L = hip_lib()
+        names = C.create_string_buffer(4096)
+        ms = (C.c_double * 64)()
+        nl = (C.c_int64 * 64)()
+        n = C.c_int32()
+        _check(L.tf_kernel_times(self._h, names, 4096, ms, nl, 64, C.byref(n)))
+        raw = names.raw.split(b"\0")
+        return {raw[i].decode(): (ms[i], nl[i]) for i in range(n.value)}
+
+    def device_bytes(self):
+        b = C.c_int64()
+        _check(hip_lib().tf_device_bytes(self._h, C.byref(b)))
+        return b.value
