@@ -1,0 +1,188 @@
+"""Benchmark of the TetFIT time-stepping hot path on one MI355X.
+
+Workload (one GPU): the 48^3-cube Kuhn tetrahedral cavity (795,024 edges,
+1,340,928 faces, 663,552 tets; vertices jittered +-0.2h with seed 2; eps_r
+piecewise constant per 8^3 block, uniform in [1,10]; mu_r = 1; PEC on 5
+walls; wave-profile h source on x-min at 2e8 Hz), stepped with the
+CG-solved DGA leapfrog (method gfdtd, CG tolerance 1e-8, maxiter 100) and,
+beside it, with the explicit DGA leapfrog (method dga).
+
+A step is one leapfrog step over the whole mesh. value = edge-DoF updates/s
+= n_edges * steps / seconds of the timed region (HIP events on the
+context's stream, inputs resident in HBM). The headline path is
+"learned+FP32": fp32 fields and operators with fp64 CG reductions and the
+learned per-edge preconditioner; "plain FP64" (Jacobi CG) is reported beside
+it. CG iterations/step are the mean ConjugateGradientSolver::iterations().
+
+Prints one JSON line. --gpus N with N > 1 exits: this build drives one GPU.
+"""
+import argparse
+import json
+import os
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "edge-DoF updates/s + CG iters/step: learned+FP32 vs plain FP64, 1 GPU"
+# dt pinned with SET step: 0.9 x the host stability estimate of each method on
+# this mesh (power iteration; see tetfit_amd/configs.py)
+DT48 = {"dga": 2.281e-11, "gfdtd": 4.777e-11}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def build_workload(d, method, n, steps):
+    from tetfit_amd import Model, configs
+    path = configs.kuhn_cavity(d, n=n, method=method, n_steps=steps, n_probes=8, block=8, seed=2,
+                               dt=DT48.get(method) if n == 48 else None)
+    t = time.time()
+    b = Model(path).build(1)
+    log(f"[bench] built {method} {n}^3: E={b.n_edges} F={b.n_faces} T={b.n_vols} dt={b.dt:.4g} "
+        f"({time.time() - t:.1f}s host set-up)")
+    return b
+
+
+def algorithmic_bytes(b, kernel, vb):
+    """Bytes a kernel must move per launch (operator arrays once, vectors
+    once per access pattern), vb = value bytes."""
+    d = b.dga_ops()
+    if kernel == "cg_a_kernel":
+        A = d["A"]
+        n, nnz = A["nrows"], int(A["rowptr"][-1])
+        # A values+cols, rowptr, z and p_old gathered, p_new and q written
+        return nnz * (vb + 4) + 4 * (n + 1) + 4 * vb * n
+    if kernel == "dga_h_kernel":
+        H = d["H"]
+        n, nnz = H["nrows"], int(H["rowptr"][-1])
+        # H values+cols, rowptr, row->edge map, curl_f read, U read+write
+        return nnz * (vb + 4) + 4 * (n + 1) + 4 * n + vb * b.n_edges + 2 * vb * n
+    raise KeyError(kernel)
+
+
+def time_path(b, precision, steps, warmup, precond="jacobi", label=""):
+    from tetfit_amd.device import Simulation
+    sim = Simulation(b, precision, output=False)
+    if warmup:
+        sim.run(n=warmup, precond=precond)
+    st = sim.run(n=steps, precond=precond)
+    sec = st["seconds"]
+    out = dict(seconds=sec, ms_per_step=1e3 * sec / steps,
+               edge_dof_per_s=b.n_edges * steps / sec,
+               cg_iters_per_step=st["cg_iters_total"] / steps if b.method == "gfdtd" else None,
+               cg_max_relres=st["cg_max_relres"] if b.method == "gfdtd" else None,
+               cg_not_converged=st["cg_not_converged"] if b.method == "gfdtd" else None)
+    # per-kernel durations with HIP events on the context's stream
+    sim.set_profiling(True)
+    sim.run(n=min(steps, 50), precond=precond)
+    out["kernels"] = {k: {"ms_avg": v[0], "launches": v[1]} for k, v in sim.kernel_times().items()}
+    sim.close()
+    log(f"[bench] {label}: {out['ms_per_step']:.3f} ms/step, {out['edge_dof_per_s'] / 1e9:.3f} G edge-DoF/s"
+        + (f", CG {out['cg_iters_per_step']:.2f} it/step" if out["cg_iters_per_step"] is not None else ""))
+    return out
+
+
+def cpu_baseline(b, steps):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    r = pyoracle.run(b, n=steps, faithful=True, output=False)
+    return dict(value=b.n_edges * steps / r["seconds"], unit="edge-DoF/s", cores=1, kind="port",
+                sample=f"{steps} steps of method {b.method} on the same mesh, single-threaded fp64 CPU "
+                       f"restatement of the reference loop (oracle/, -O3), {r['seconds']:.2f} s",
+                cg_iters_per_step=(r["cg_iters_total"] / steps) if b.method == "gfdtd" else None,
+                host=os.uname().nodename, nproc=os.cpu_count())
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--n", type=int, default=48, help="Kuhn cubes per side")
+    ap.add_argument("--cpu-steps", type=int, default=8)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-dga", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py drives exactly one GPU (asked for {args.gpus}); multi-GPU runs are not part of this build",
+              file=sys.stderr)
+        sys.exit(2)
+    from tetfit_amd.device import device_count
+    if device_count() < 1:
+        print("bench.py needs a HIP device", file=sys.stderr)
+        sys.exit(1)
+    total = args.warmup + args.steps + 60
+    with tempfile.TemporaryDirectory() as d:
+        bg = build_workload(d, "gfdtd", args.n, total)
+        plain = time_path(bg, "fp64", args.steps, args.warmup, "jacobi", "gfdtd fp64 jacobi (plain)")
+        headline_label = "gfdtd fp64 jacobi"
+        head = plain
+        f32j = time_path(bg, "fp32", args.steps, args.warmup, "jacobi", "gfdtd fp32 jacobi")
+        try:
+            fast = time_path(bg, "fp32", args.steps, args.warmup, "learned", "gfdtd fp32 learned")
+            head, headline_label = fast, "gfdtd fp32 learned"
+        except Exception as e:  # the learned path needs its trained network
+            log(f"[bench] learned+fp32 path unavailable: {e}")
+            fast = None
+        dga = None
+        if not args.no_dga:
+            bd = build_workload(d, "dga", args.n, total)
+            dga = time_path(bd, "fp64", args.steps, args.warmup, "jacobi", "dga fp64 (explicit)")
+            dga32 = time_path(bd, "fp32", args.steps, args.warmup, "jacobi", "dga fp32 (explicit)")
+        kname = "cg_a_kernel"
+        vb = 4 if head is fast else 8
+        kern = head["kernels"].get(kname)
+        roof = None
+        if kern:
+            nbytes = algorithmic_bytes(bg, kname, vb)
+            ach = nbytes / (kern["ms_avg"] * 1e-3) / 1e9
+            roof = dict(bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                        frac=round(ach / HBM_PEAK_GBS, 4), traffic=None, kernel=kname,
+                        bytes_per_launch=int(nbytes), ms_per_launch=kern["ms_avg"])
+        cpu = None
+        if not args.no_cpu:
+            t = time.time()
+            cpu = cpu_baseline(bg, args.cpu_steps)
+            log(f"[bench] cpu baseline {cpu['value'] / 1e6:.2f} M edge-DoF/s ({time.time() - t:.1f}s)")
+        line = {
+            "metric": METRIC,
+            "value": head["edge_dof_per_s"],
+            "unit": "edge-DoF/s",
+            "n_gpus": 1,
+            "steps": args.steps,
+            "warmup": args.warmup,
+            "ms_per_step": head["ms_per_step"],
+            "higher_is_better": True,
+            "scaling": "weak",
+            "vs_baseline": None,
+            "dtype": "f32" if head is fast else "f64",
+            "data": "synthetic",
+            "config": {"workload": f"Kuhn {args.n}^3 jittered tet cavity, method gfdtd (CG tol 1e-8)",
+                       "path": headline_label, "n_edges": bg.n_edges, "n_faces": bg.n_faces,
+                       "parallelism": "single GPU"},
+            "cg_iters_per_step": head["cg_iters_per_step"],
+            "plain_fp64": {"edge_dof_per_s": plain["edge_dof_per_s"], "ms_per_step": plain["ms_per_step"],
+                           "cg_iters_per_step": plain["cg_iters_per_step"]},
+            "speedup_vs_plain_fp64": head["edge_dof_per_s"] / plain["edge_dof_per_s"],
+            "fp32_jacobi": {"edge_dof_per_s": f32j["edge_dof_per_s"], "cg_iters_per_step": f32j["cg_iters_per_step"],
+                            "cg_max_relres": f32j["cg_max_relres"], "cg_not_converged": f32j["cg_not_converged"]},
+            "roofline": roof,
+            "cpu_baseline": cpu,
+        }
+        if dga:
+            line["dga_explicit"] = {"fp64_edge_dof_per_s": dga["edge_dof_per_s"], "fp64_ms_per_step": dga["ms_per_step"],
+                                    "fp32_edge_dof_per_s": dga32["edge_dof_per_s"],
+                                    "kernels_fp64": dga["kernels"]}
+        line["kernels"] = head["kernels"]
+        print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

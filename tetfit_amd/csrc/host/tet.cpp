@@ -831,7 +831,7 @@ void build_tet(const Model& m, const SimulationDef& sim, Build& b) {
             uint64_t st = 0x9E3779B97F4A7C15ull;
             for (auto& v : x) { st ^= st << 13; st ^= st >> 7; st ^= st << 17; v = (double)(st >> 11) * 1.1102230246251565e-16 - 0.5; }
             double lam = 0;
-            for (int it = 0; it < 40; ++it) {
+            for (int it = 0; it < 25; ++it) {
                 apply(x, y);
                 double n2 = std::sqrt(std::inner_product(y.begin(), y.end(), y.begin(), 0.0));
                 if (n2 == 0) break;
