@@ -69,3 +69,39 @@ def test_gfdtd_parity(workdir):
     assert st["cg_max_relres"] <= b.cg_tol
     assert st["cg_not_converged"] == 0
     assert abs(st["cg_iters_total"] - r["cg_iters_total"]) <= b.n_steps
+
+
+# ---- fp32 fields, fp64 reductions: error stated against the fp64 checker.
+# Bar of the fast path: relative L2 over all probe samples <= 1e-4.
+FP32_TOL = 1e-4
+
+
+def fp32_err(b, n=None):
+    print(f"case: fp32 {b.method} E={b.n_edges} steps={b.n_steps}", flush=True)
+    sim = Simulation(b, "fp32")
+    st = sim.run(n=n)
+    _, s = sim.probes()
+    U, F, B = sim.state()
+    sim.close()
+    r = pyoracle.run(b, n=n)
+    err = rel_l2(s, r["samples"])
+    print(f"fp32 {b.method}: probe rel L2 {err:.3e}, U rel L2 {rel_l2(U, r['U']):.3e}", flush=True)
+    return err, st
+
+
+def test_fit_fp32(workdir):
+    b = Model(configs.fit_cavity(str(workdir), n=12, step=0.1, n_steps=200, n_probes=6, name="fit12")).build(1)
+    err, _ = fp32_err(b)
+    assert err <= FP32_TOL
+
+
+def test_dga_fp32(workdir):
+    b = Model(configs.kuhn_cavity(str(workdir), n=6, method="dga", n_steps=200, n_probes=4, block=2)).build(1)
+    err, _ = fp32_err(b)
+    assert err <= FP32_TOL
+
+
+def test_gfdtd_fp32(workdir):
+    b = Model(configs.kuhn_cavity(str(workdir), n=5, method="gfdtd", n_steps=60, n_probes=4, block=2)).build(1)
+    err, st = fp32_err(b)
+    assert err <= FP32_TOL

@@ -243,193 +243,31 @@ __global__ void __launch_bounds__(kBlock) dga_n_kernel(int32_t nrows, const int3
 }
 
 // ------------------------------------------------------------------ gfdtd
-// CG scalar state on the device (ConjugateGradientSolver.hpp:52-103).
+// CG of ConjugateGradientSolver.hpp:52-103 split over two kernels per
+// iteration. Each kernel writes one fp64 partial per block; the next kernel
+// sums them (every block, in index order, so all blocks take the same
+// decision) - no cross-block fences inside a kernel.
 struct CgState {
-    double rho, rho_new, pq, rr, rr0, alpha, beta, relres, tol;
-    int32_t iter, done, maxiter, pad;
-    uint32_t ticket_a, ticket_b, ticket_c, pad2;
+    double rr0;        // ||r0||^2 of this solve
+    double rho[2];     // r.z of iteration it, slot it&1
+    double relres;     // ||r||/||r0|| at the last stop test
+    double tol;
+    int32_t maxiter;
+    int32_t iter;      // iterations done when the solve stopped
+    int32_t done;
+    int32_t pad;
 };
 
-// Last-block-done pattern: every block publishes its partial, the block that
-// takes the last ticket sums the partials in index order (deterministic).
-__device__ __forceinline__ bool last_block(uint32_t* ticket, uint32_t nblocks) {
-    __shared__ bool is_last;
-    __threadfence();  // release this block's partials (write back the XCD's L2)
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t t = atomicAdd(ticket, 1u);
-        is_last = (t == nblocks - 1);
-    }
-    __syncthreads();
-    if (is_last) __threadfence();  // acquire the other blocks' partials
-    return is_last;
-}
-
-__device__ __forceinline__ double sum_partials(const double* part, int n, double* red) {
+__device__ __forceinline__ double sum_partials(const double* __restrict__ part, int n, double* red) {
     double v = 0;
     for (int i = threadIdx.x; i < n; i += blockDim.x) v += part[i];
-    return block_sum(v, red);
-}
-
-// rhs[k] = E U_old - RHS1 SrcFld + dt (Ct F + fb) at free edge e = free[k]
-// (D:1516-1523; sigma and I are zero on the supported inputs). SrcFld is U on
-// Dirichlet edges at the new time, held in `dir_new` (edge-indexed).
-template <typename T, int LPR>
-__global__ void __launch_bounds__(kBlock) gf_rhs_kernel(
-    int32_t n_free, const int32_t* __restrict__ free_edge, const int32_t* __restrict__ e_ptr,
-    const int32_t* __restrict__ e_col, const T* __restrict__ e_val, const int32_t* __restrict__ r_ptr,
-    const int32_t* __restrict__ r_col, const T* __restrict__ r_val, const int32_t* __restrict__ ct_ptr,
-    const int32_t* __restrict__ ct_col, const int8_t* __restrict__ ct_sg, const T* __restrict__ U,
-    const T* __restrict__ dir_new, const T* __restrict__ Fv, const T* __restrict__ fb, T* __restrict__ rhs,
-    double dt) {
-    const int32_t g = blockIdx.x * kBlock + threadIdx.x;
-    const int32_t k = g / LPR, lane = g % LPR;
-    double a = 0, c = 0;
-    int32_t e = 0;
-    if (k < n_free) {
-        e = free_edge[k];
-        for (int32_t q = e_ptr[e] + lane; q < e_ptr[e + 1]; q += LPR) a += (double)e_val[q] * (double)U[e_col[q]];
-        for (int32_t q = r_ptr[e] + lane; q < r_ptr[e + 1]; q += LPR) a -= (double)r_val[q] * (double)dir_new[r_col[q]];
-        for (int32_t q = ct_ptr[e] + lane; q < ct_ptr[e + 1]; q += LPR) c += (double)ct_sg[q] * (double)Fv[ct_col[q]];
-    }
-    a = group_sum<LPR>(a);
-    c = group_sum<LPR>(c);
-    if (k < n_free && lane == 0) rhs[k] = (T)(a + dt * (c + (double)fb[e]));
-}
-
-// r = b - A x, z = pre r; partials of r.r and r.z  (CG set-up)
-template <typename T, int LPR>
-__global__ void __launch_bounds__(kBlock) cg_init_kernel(int32_t n, const int32_t* __restrict__ ptr,
-                                                         const int32_t* __restrict__ col, const T* __restrict__ val,
-                                                         const T* __restrict__ pre, const T* __restrict__ b,
-                                                         const T* __restrict__ x, T* __restrict__ r, T* __restrict__ z,
-                                                         double* __restrict__ part, CgState* st, uint32_t nblocks) {
-    __shared__ double red[kBlock / kWave];
-    const int32_t g = blockIdx.x * kBlock + threadIdx.x;
-    const int32_t i = g / LPR, lane = g % LPR;
-    double s = 0;
-    if (i < n)
-        for (int32_t q = ptr[i] + lane; q < ptr[i + 1]; q += LPR) s += (double)val[q] * (double)x[col[q]];
-    s = group_sum<LPR>(s);
-    double rr = 0, rz = 0;
-    if (i < n && lane == 0) {
-        const T ri = (T)((double)b[i] - s);
-        const T zi = (T)((double)pre[i] * (double)ri);
-        r[i] = ri;
-        z[i] = zi;
-        rr = (double)ri * (double)ri;
-        rz = (double)ri * (double)zi;
-    }
-    rr = block_sum(rr, red);
-    rz = block_sum(rz, red);
-    if (threadIdx.x == 0) { part[blockIdx.x] = rr; part[nblocks + blockIdx.x] = rz; }
-    if (last_block(&st->ticket_a, nblocks)) {
-        const double RR = sum_partials(part, nblocks, red);
-        const double RZ = sum_partials(part + nblocks, nblocks, red);
-        if (threadIdx.x == 0) {
-            st->rr0 = RR;
-            st->rr = RR;
-            st->rho = RZ;
-            st->iter = 0;
-            // while (residual/residual0 > tol && iter < maxiter): 0/0 is NaN -> no iteration
-            const double ratio = std::sqrt(RR) / std::sqrt(RR);
-            st->relres = ratio;
-            st->done = !(ratio > st->tol && 0 < st->maxiter);
-            st->ticket_a = 0;
-        }
-    }
-}
-
-// p_new = z + beta p_old (p_new = z at iter 0), q = A p_new gathered with the
-// p update fused (p_old / p_new double buffers), partial p.q -> alpha.
-template <typename T, int LPR>
-__global__ void __launch_bounds__(kBlock) cg_a_kernel(int32_t n, const int32_t* __restrict__ ptr,
-                                                      const int32_t* __restrict__ col, const T* __restrict__ val,
-                                                      const T* __restrict__ z, T* __restrict__ p0, T* __restrict__ p1,
-                                                      T* __restrict__ q, double* __restrict__ part, CgState* st,
-                                                      uint32_t nblocks) {
-    if (st->done) return;
-    __shared__ double red[kBlock / kWave];
-    const int it = st->iter;
-    const double beta = it == 0 ? 0.0 : st->beta;
-    const T* pold = (it & 1) ? p1 : p0;
-    T* pnew = (it & 1) ? p0 : p1;
-    const int32_t g = blockIdx.x * kBlock + threadIdx.x;
-    const int32_t i = g / LPR, lane = g % LPR;
-    double s = 0;
-    if (i < n) {
-        if (it == 0) {
-            for (int32_t k = ptr[i] + lane; k < ptr[i + 1]; k += LPR) s += (double)val[k] * (double)z[col[k]];
-        } else {
-            for (int32_t k = ptr[i] + lane; k < ptr[i + 1]; k += LPR) {
-                const int32_t j = col[k];
-                const T pj = (T)((double)z[j] + beta * (double)pold[j]);
-                s += (double)val[k] * (double)pj;
-            }
-        }
-    }
-    s = group_sum<LPR>(s);
-    double pq = 0;
-    if (i < n && lane == 0) {
-        const T pi = it == 0 ? z[i] : (T)((double)z[i] + beta * (double)pold[i]);
-        pnew[i] = pi;
-        const T qi = (T)s;
-        q[i] = qi;
-        pq = (double)pi * (double)qi;
-    }
-    pq = block_sum(pq, red);
-    if (threadIdx.x == 0) part[blockIdx.x] = pq;
-    if (last_block(&st->ticket_b, nblocks)) {
-        const double PQ = sum_partials(part, nblocks, red);
-        if (threadIdx.x == 0) {
-            st->pq = PQ;
-            st->alpha = st->rho / PQ;
-            st->ticket_b = 0;
-        }
-    }
-}
-
-// x += alpha p, r -= alpha q, residual, then z = pre r and rho for the next
-// iteration; the last block updates beta, iter and the stop test.
-template <typename T>
-__global__ void __launch_bounds__(kBlock) cg_b_kernel(int32_t n, const T* __restrict__ pre, T* __restrict__ x,
-                                                      T* __restrict__ r, T* __restrict__ z, const T* __restrict__ p0,
-                                                      const T* __restrict__ p1, const T* __restrict__ q,
-                                                      double* __restrict__ part, CgState* st, uint32_t nblocks) {
-    if (st->done) return;
-    __shared__ double red[kBlock / kWave];
-    const int it = st->iter;
-    const T* pnew = (it & 1) ? p0 : p1;
-    const double alpha = st->alpha;
-    const int32_t i = blockIdx.x * kBlock + threadIdx.x;
-    double rr = 0, rz = 0;
-    if (i < n) {
-        const double pi = (double)pnew[i];
-        x[i] = (T)((double)x[i] + alpha * pi);
-        const T ri = (T)((double)r[i] - alpha * (double)q[i]);
-        r[i] = ri;
-        const T zi = (T)((double)pre[i] * (double)ri);
-        z[i] = zi;
-        rr = (double)ri * (double)ri;
-        rz = (double)ri * (double)zi;
-    }
-    rr = block_sum(rr, red);
-    rz = block_sum(rz, red);
-    if (threadIdx.x == 0) { part[blockIdx.x] = rr; part[nblocks + blockIdx.x] = rz; }
-    if (last_block(&st->ticket_c, nblocks)) {
-        const double RR = sum_partials(part, nblocks, red);
-        const double RZ = sum_partials(part + nblocks, nblocks, red);
-        if (threadIdx.x == 0) {
-            st->rr = RR;
-            st->iter = it + 1;
-            const double ratio = std::sqrt(RR) / std::sqrt(st->rr0);
-            st->relres = ratio;
-            st->beta = RZ / st->rho;
-            st->rho = RZ;
-            st->done = !(ratio > st->tol && it + 1 < st->maxiter);
-            st->ticket_c = 0;
-        }
-    }
+    v = block_sum(v, red);
+    __shared__ double bcast;
+    if (threadIdx.x == 0) bcast = v;
+    __syncthreads();
+    v = bcast;
+    __syncthreads();
+    return v;
 }
 
 // U[free[k]] = x[k]; Dirichlet edges take their new values.

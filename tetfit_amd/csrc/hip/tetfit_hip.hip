@@ -17,6 +17,7 @@
 
 #include "common.hpp"
 #include "kernels.hpp"
+#include "sell.hpp"
 #include "tetfit_hip.h"
 
 using namespace tfd;
@@ -24,6 +25,12 @@ using namespace tfd;
 namespace {
 
 thread_local std::string g_err;
+
+// Grid caps of the grid-stride kernels: the CG partial buffers hold one
+// fp64 per block of the producing kernel.
+constexpr int kCgPartA = 2048;   // blocks of cg_a (SpMV with A)
+constexpr int kCgPartB = 1024;   // blocks of cg_init / cg_b
+constexpr int kSpmvCap = 8192;   // blocks of the other SELL kernels
 
 // Carrier of Source::Compute in long double (Source.hpp:316-327); 0 for t<0.
 double carrier_value(const tf_source& s, double t) {
@@ -62,6 +69,12 @@ struct Csr32 {
     int64_t nnz = 0;
 };
 
+struct Sell {
+    DevBuf sptr, col, val, sg;
+    int32_t nrows = 0, nslices = 0;
+    int64_t nnz = 0, entries = 0;
+};
+
 }  // namespace
 
 struct tf_ctx {
@@ -85,7 +98,8 @@ struct tf_ctx {
     int32_t n_sp = 0;
 
     // DGA / gfdtd
-    Csr32 Ct, H, N, Em, R1, A;
+    Csr32 Ct, Em, R1;
+    Sell sCt, H, N, A;
     DevBuf c_col, c_sg, h_edge, curl_f, curl_u;
     DevBuf free_edge, dir_edge, dir_new, fb, pre, x, r, z, p0, p1, q, rhs, cg_part, cg_state;
     int32_t n_free = 0, n_dir = 0;
@@ -94,6 +108,7 @@ struct tf_ctx {
     DevBuf inj_edge, inj_kind, inj_tb, inj_te;
     int32_t n_inj = 0;
     CgState* h_state = nullptr;  // pinned
+    int cg_guess = 30;           // iterations of the previous solve
 
     // sources
     std::vector<tf_source> src;
@@ -141,8 +156,10 @@ size_t tf_ctx::device_bytes() const {
                            &p0, &p1, &q, &rhs, &cg_part, &cg_state, &inj_edge, &inj_kind, &inj_tb, &inj_te,
                            &term_src, &term_w, &carr, &probes, &samples, &poy};
     for (auto b : all) s += b->bytes;
-    const Csr32* cs[] = {&Ct, &H, &N, &Em, &R1, &A};
+    const Csr32* cs[] = {&Ct, &Em, &R1};
     for (auto c : cs) s += c->ptr.bytes + c->col.bytes + c->val.bytes + c->sg.bytes;
+    const Sell* ss[] = {&sCt, &H, &N, &A};
+    for (auto c : ss) s += c->sptr.bytes + c->col.bytes + c->val.bytes + c->sg.bytes;
     return s;
 }
 
@@ -167,6 +184,54 @@ void upload_csr(Csr32& d, const tf_csr& c, bool values) {
         d.sg.alloc(c.nnz);
         if (c.nnz) TF_HIP(hipMemcpy(d.sg.p, c.sgn, c.nnz, hipMemcpyHostToDevice));
     }
+}
+
+// CSR -> SELL-64 (slices of 64 rows, column-major inside a slice, rows
+// padded with value 0 on their last column).
+template <class T>
+void upload_sell(Sell& d, const tf_csr& c, bool values) {
+    d.nrows = c.nrows;
+    d.nnz = c.nnz;
+    d.nslices = (c.nrows + kSlice - 1) / kSlice;
+    std::vector<int64_t> sptr(d.nslices + 1, 0);
+    for (int32_t s = 0; s < d.nslices; ++s) {
+        int64_t w = 0;
+        for (int32_t r = s * kSlice; r < std::min(c.nrows, (s + 1) * kSlice); ++r)
+            w = std::max<int64_t>(w, c.rowptr[r + 1] - c.rowptr[r]);
+        sptr[s + 1] = sptr[s] + w * kSlice;
+    }
+    d.entries = sptr[d.nslices];
+    std::vector<int32_t> col(d.entries, 0);
+    std::vector<T> val(values ? d.entries : 0, (T)0);
+    std::vector<int8_t> sg(values ? 0 : d.entries, 0);
+    for (int32_t s = 0; s < d.nslices; ++s) {
+        const int64_t w = (sptr[s + 1] - sptr[s]) / kSlice;
+        for (int lane = 0; lane < kSlice; ++lane) {
+            const int32_t r = s * kSlice + lane;
+            int64_t b = 0, e = 0;
+            if (r < c.nrows) { b = c.rowptr[r]; e = c.rowptr[r + 1]; }
+            const int32_t padc = e > b ? c.col[e - 1] : 0;
+            for (int64_t k = 0; k < w; ++k) {
+                const int64_t o = sptr[s] + k * kSlice + lane;
+                if (b + k < e) {
+                    col[o] = c.col[b + k];
+                    if (values) val[o] = (T)c.val[b + k];
+                    else sg[o] = c.sgn[b + k];
+                } else {
+                    col[o] = padc;
+                }
+            }
+        }
+    }
+    upload(d.sptr, sptr);
+    upload(d.col, col);
+    if (values) upload(d.val, val);
+    else upload(d.sg, sg);
+}
+
+int sell_blocks(const Sell& m, int cap) {
+    const int need = (m.nslices + kBlock / kWave - 1) / (kBlock / kWave);
+    return std::max(1, std::min(need, cap));
 }
 
 template <class T>
@@ -291,18 +356,19 @@ template <class T>
 void create_tet(tf_ctx* c, const tf_problem* p) {
     const tf_dga_ops& d = p->dga;
     const int32_t E = c->E, F = c->F;
-    upload_csr<T>(c->Ct, d.Ct, false);
+    if (p->method == TF_METHOD_DGA) upload_sell<T>(c->sCt, d.Ct, false);
+    else upload_csr<T>(c->Ct, d.Ct, false);
     if (d.C.nnz != 3 * (int64_t)F) throw std::runtime_error("C must have 3 entries per face");
     c->c_col.alloc(d.C.nnz * 4);
     TF_HIP(hipMemcpy(c->c_col.p, d.C.col, d.C.nnz * 4, hipMemcpyHostToDevice));
     c->c_sg.alloc(d.C.nnz);
     TF_HIP(hipMemcpy(c->c_sg.p, d.C.sgn, d.C.nnz, hipMemcpyHostToDevice));
-    upload_csr<T>(c->N, d.N, true);
+    upload_sell<T>(c->N, d.N, true);
     c->h_fixed_value.assign(d.fixed_value, d.fixed_value + E);
     zero_field<T>(c->curl_u, F, c->stream);
     zero_field<T>(c->curl_f, E, c->stream);
     if (p->method == TF_METHOD_DGA) {
-        upload_csr<T>(c->H, d.H, true);
+        upload_sell<T>(c->H, d.H, true);
         std::vector<int32_t> he(d.H.nrows, -1);
         for (int32_t e = 0; e < E; ++e)
             if (d.h_row[e] >= 0) he[d.h_row[e]] = e;
@@ -312,7 +378,7 @@ void create_tet(tf_ctx* c, const tf_problem* p) {
     } else {
         upload_csr<T>(c->Em, d.E, true);
         upload_csr<T>(c->R1, d.RHS1, true);
-        upload_csr<T>(c->A, d.A, true);
+        upload_sell<T>(c->A, d.A, true);
         c->n_free = d.n_free;
         std::vector<int32_t> fe(d.free_edges, d.free_edges + d.n_free);
         upload(c->free_edge, fe);
@@ -336,7 +402,7 @@ void create_tet(tf_ctx* c, const tf_problem* p) {
         upload(c->pre, pre);
         const int64_t n = std::max<int32_t>(d.n_free, 1);
         for (DevBuf* b : {&c->x, &c->r, &c->z, &c->p0, &c->p1, &c->q, &c->rhs}) zero_field<T>(*b, n, c->stream);
-        c->cg_part.alloc(sizeof(double) * 2 * (blocks_for(n) + 64));
+        c->cg_part.alloc(sizeof(double) * (kCgPartA + 2 * kCgPartB));
         c->cg_state.alloc(sizeof(CgState));
         TF_HIP(hipMemsetAsync(c->cg_state.p, 0, sizeof(CgState), c->stream));
         TF_HIP(hipHostMalloc((void**)&c->h_state, sizeof(CgState), hipHostMallocDefault));
@@ -541,9 +607,6 @@ void run_fit(tf_ctx* c, int64_t first, int64_t n, tf_stats* st) {
     st->steps_done += n;
 }
 
-template <class T, int LPR>
-int rows_blocks(int64_t nrows) { return blocks_for(nrows * LPR); }
-
 template <class T>
 void run_dga(tf_ctx* c, int64_t first, int64_t n, tf_stats* st) {
     const int inj_blocks = c->n_inj ? blocks_for(c->n_inj) : 0;
@@ -564,14 +627,16 @@ void run_dga(tf_ctx* c, int64_t first, int64_t n, tf_stats* st) {
         if (allow_print(c, 0.0)) sample<T>(c, 0.0);
     }
     c->initial_done = true;
+    const int gb_ct = sell_blocks(c->sCt, kSpmvCap), gb_h = sell_blocks(c->H, kSpmvCap);
+    const int gb_n = sell_blocks(c->N, kSpmvCap);
     for (int64_t k = 0; k < n; ++k) {
         const int64_t i = first + k;
         const double t = (double)i * c->dt;
         const double* cs = c->carr.as<double>() + (k + 1) * ns;
-        ev_begin(c, "ct_rows_kernel");
-        incidence_rows_kernel<T, 8><<<rows_blocks<T, 8>(c->E), kBlock, 0, c->stream>>>(
-            c->E, c->Ct.ptr.as<int32_t>(), c->Ct.col.as<int32_t>(), c->Ct.sg.as<int8_t>(), c->Fv.as<T>(),
-            c->curl_f.as<T>());
+        ev_begin(c, "sell_ct_kernel");
+        sell_ct_kernel<T><<<gb_ct, kBlock, 0, c->stream>>>(c->E, c->sCt.nslices, c->sCt.sptr.as<int64_t>(),
+                                                           c->sCt.col.as<int32_t>(), c->sCt.sg.as<int8_t>(),
+                                                           c->Fv.as<T>(), c->curl_f.as<T>());
         ev_end(c);
         if (c->n_inj) {
             ev_begin(c, "dga_inject_kernel");
@@ -581,10 +646,11 @@ void run_dga(tf_ctx* c, int64_t first, int64_t n, tf_stats* st) {
                 c->U.as<T>(), c->poy.as<double>() + 2 * k * inj_blocks);
             ev_end(c);
         }
-        ev_begin(c, "dga_h_kernel");
-        dga_h_kernel<T, 32><<<rows_blocks<T, 32>(c->H.nrows), kBlock, 0, c->stream>>>(
-            c->H.nrows, c->H.ptr.as<int32_t>(), c->H.col.as<int32_t>(), c->H.val.as<T>(), c->h_edge.as<int32_t>(),
-            c->curl_f.as<T>(), c->U.as<T>(), c->dt);
+        ev_begin(c, "sell_h_kernel");
+        sell_h_kernel<T><<<gb_h, kBlock, 0, c->stream>>>(c->H.nrows, c->H.nslices, c->H.sptr.as<int64_t>(),
+                                                         c->H.col.as<int32_t>(), c->H.val.as<T>(),
+                                                         c->h_edge.as<int32_t>(), c->curl_f.as<T>(), c->U.as<T>(),
+                                                         c->dt);
         ev_end(c);
         if (c->n_inj) {
             ev_begin(c, "dga_poynting_kernel");
@@ -599,10 +665,10 @@ void run_dga(tf_ctx* c, int64_t first, int64_t n, tf_stats* st) {
                                                                         c->c_sg.as<int8_t>(), c->U.as<T>(),
                                                                         c->curl_u.as<T>(), c->B.as<T>(), c->dt);
         ev_end(c);
-        ev_begin(c, "dga_n_kernel");
-        dga_n_kernel<T, 8><<<rows_blocks<T, 8>(c->N.nrows), kBlock, 0, c->stream>>>(
-            c->N.nrows, c->N.ptr.as<int32_t>(), c->N.col.as<int32_t>(), c->N.val.as<T>(), c->curl_u.as<T>(),
-            c->Fv.as<T>(), c->dt, 0);
+        ev_begin(c, "sell_n_kernel");
+        sell_n_kernel<T><<<gb_n, kBlock, 0, c->stream>>>(c->N.nrows, c->N.nslices, c->N.sptr.as<int64_t>(),
+                                                         c->N.col.as<int32_t>(), c->N.val.as<T>(), c->curl_u.as<T>(),
+                                                         c->Fv.as<T>(), c->dt, 0);
         ev_end(c);
         if (allow_print(c, t)) sample<T>(c, t);
         if (c->profiling && (k & 63) == 63) ev_flush(c);
@@ -621,24 +687,63 @@ void run_dga(tf_ctx* c, int64_t first, int64_t n, tf_stats* st) {
     st->steps_done += n;
 }
 
+// One CG solve on the device: cg_init, then iterations it = 0, 1, ... as
+// (cg_a, cg_b) pairs. Launches are issued in batches sized from the previous
+// solve's count; kernels past the stop test return at once. Returns the
+// final state (read back after the solve).
+template <class T>
+void cg_solve(tf_ctx* c, const CgState& init, int& guess) {
+    const int32_t nf = c->n_free;
+    const int nb_a = sell_blocks(c->A, kCgPartA);
+    const int nb_b = std::max(1, std::min(blocks_for(nf), kCgPartB));
+    double* part_a = c->cg_part.as<double>();
+    double* part_b = part_a + kCgPartA;
+    CgState* dst = c->cg_state.as<CgState>();
+    TF_HIP(hipMemcpyAsync(dst, &init, sizeof(CgState), hipMemcpyHostToDevice, c->stream));
+    ev_begin(c, "cg_init_kernel");
+    cg_init_kernel<T><<<nb_b, kBlock, 0, c->stream>>>(nf, c->A.nslices, c->A.sptr.as<int64_t>(), c->A.col.as<int32_t>(),
+                                                      c->A.val.as<T>(), c->pre.as<T>(), c->rhs.as<T>(), c->x.as<T>(),
+                                                      c->r.as<T>(), c->z.as<T>(), part_b, dst);
+    ev_end(c);
+    int it = 0;
+    int batch = std::max(1, guess + 1);
+    for (;;) {
+        const int stop = std::min(it + batch, init.maxiter + 1);
+        for (; it < stop; ++it) {
+            ev_begin(c, "cg_a_kernel");
+            cg_a_kernel<T><<<nb_a, kBlock, 0, c->stream>>>(it, nf, c->A.nslices, c->A.sptr.as<int64_t>(),
+                                                           c->A.col.as<int32_t>(), c->A.val.as<T>(), c->z.as<T>(),
+                                                           c->p0.as<T>(), c->p1.as<T>(), c->q.as<T>(), part_b, nb_b,
+                                                           part_a, dst);
+            ev_end(c);
+            if (it == init.maxiter) break;  // cg_a(maxiter) only runs the stop test
+            ev_begin(c, "cg_b_kernel");
+            cg_b_kernel<T><<<nb_b, kBlock, 0, c->stream>>>(it, nf, c->pre.as<T>(), c->x.as<T>(), c->r.as<T>(),
+                                                           c->z.as<T>(), c->p0.as<T>(), c->p1.as<T>(), c->q.as<T>(),
+                                                           part_a, nb_a, part_b, dst);
+            ev_end(c);
+        }
+        TF_HIP(hipMemcpyAsync(c->h_state, dst, sizeof(CgState), hipMemcpyDeviceToHost, c->stream));
+        TF_HIP(hipStreamSynchronize(c->stream));
+        if (c->h_state->done || it > init.maxiter) break;
+        batch = 4;
+    }
+    guess = c->h_state->iter;
+}
+
 template <class T>
 void run_gfdtd(tf_ctx* c, int64_t first, int64_t n, const tf_cg_opts* cg, tf_stats* st) {
     const int inj_blocks = c->n_inj ? blocks_for(c->n_inj) : 0;
     const size_t ns = std::max<size_t>(c->src.size(), 1);
     const int32_t nf = c->n_free;
-    const uint32_t nb_rows = (uint32_t)rows_blocks<T, 8>(nf);
-    const uint32_t nb_vec = (uint32_t)blocks_for(nf);
-    const int batch = 4;
     if (inj_blocks && c->poy_cap < n * inj_blocks) {
         c->poy.alloc(n * inj_blocks * sizeof(double));
         c->poy_cap = n * inj_blocks;
     }
-    if ((int64_t)c->cg_part.bytes < (int64_t)sizeof(double) * 2 * (int64_t)std::max(nb_rows, nb_vec))
-        c->cg_part.alloc(sizeof(double) * 2 * std::max(nb_rows, nb_vec));
     CgState init{};
     init.tol = cg ? cg->tol : 1e-8;
     init.maxiter = cg ? cg->maxiter : 100;
-    CgState* dst = c->cg_state.as<CgState>();
+    const int gb_n = sell_blocks(c->N, kSpmvCap);
     for (int64_t k = 0; k < n; ++k) {
         const int64_t i = first + k;
         const double t = (double)i * c->dt;
@@ -649,10 +754,10 @@ void run_gfdtd(tf_ctx* c, int64_t first, int64_t n, const tf_cg_opts* cg, tf_sta
                                                                         c->c_sg.as<int8_t>(), c->U.as<T>(),
                                                                         c->curl_u.as<T>(), c->B.as<T>(), c->dt);
         ev_end(c);
-        ev_begin(c, "dga_n_kernel");
-        dga_n_kernel<T, 8><<<rows_blocks<T, 8>(c->N.nrows), kBlock, 0, c->stream>>>(
-            c->N.nrows, c->N.ptr.as<int32_t>(), c->N.col.as<int32_t>(), c->N.val.as<T>(), c->B.as<T>(),
-            c->Fv.as<T>(), c->dt, 1);
+        ev_begin(c, "sell_n_kernel");
+        sell_n_kernel<T><<<gb_n, kBlock, 0, c->stream>>>(c->N.nrows, c->N.nslices, c->N.sptr.as<int64_t>(),
+                                                         c->N.col.as<int32_t>(), c->N.val.as<T>(), c->B.as<T>(),
+                                                         c->Fv.as<T>(), c->dt, 1);
         ev_end(c);
         if (allow_print(c, t - c->dt)) sample<T>(c, t - c->dt);
         if (c->n_inj) {
@@ -664,40 +769,18 @@ void run_gfdtd(tf_ctx* c, int64_t first, int64_t n, const tf_cg_opts* cg, tf_sta
             ev_end(c);
         }
         ev_begin(c, "gf_rhs_kernel");
-        gf_rhs_kernel<T, 16><<<rows_blocks<T, 16>(nf), kBlock, 0, c->stream>>>(
+        gf_rhs_kernel<T, 16><<<blocks_for((int64_t)nf * 16), kBlock, 0, c->stream>>>(
             nf, c->free_edge.as<int32_t>(), c->Em.ptr.as<int32_t>(), c->Em.col.as<int32_t>(), c->Em.val.as<T>(),
             c->R1.ptr.as<int32_t>(), c->R1.col.as<int32_t>(), c->R1.val.as<T>(), c->Ct.ptr.as<int32_t>(),
             c->Ct.col.as<int32_t>(), c->Ct.sg.as<int8_t>(), c->U.as<T>(), c->dir_new.as<T>(), c->Fv.as<T>(),
             c->fb.as<T>(), c->rhs.as<T>(), c->dt);
         ev_end(c);
-        TF_HIP(hipMemcpyAsync(dst, &init, sizeof(CgState), hipMemcpyHostToDevice, c->stream));
-        ev_begin(c, "cg_init_kernel");
-        cg_init_kernel<T, 8><<<nb_rows, kBlock, 0, c->stream>>>(
-            nf, c->A.ptr.as<int32_t>(), c->A.col.as<int32_t>(), c->A.val.as<T>(), c->pre.as<T>(), c->rhs.as<T>(),
-            c->x.as<T>(), c->r.as<T>(), c->z.as<T>(), c->cg_part.as<double>(), dst, nb_rows);
-        ev_end(c);
-        for (;;) {
-            for (int b = 0; b < batch; ++b) {
-                ev_begin(c, "cg_a_kernel");
-                cg_a_kernel<T, 8><<<nb_rows, kBlock, 0, c->stream>>>(
-                    nf, c->A.ptr.as<int32_t>(), c->A.col.as<int32_t>(), c->A.val.as<T>(), c->z.as<T>(),
-                    c->p0.as<T>(), c->p1.as<T>(), c->q.as<T>(), c->cg_part.as<double>(), dst, nb_rows);
-                ev_end(c);
-                ev_begin(c, "cg_b_kernel");
-                cg_b_kernel<T><<<nb_vec, kBlock, 0, c->stream>>>(nf, c->pre.as<T>(), c->x.as<T>(), c->r.as<T>(),
-                                                                  c->z.as<T>(), c->p0.as<T>(), c->p1.as<T>(),
-                                                                  c->q.as<T>(), c->cg_part.as<double>(), dst, nb_vec);
-                ev_end(c);
-            }
-            TF_HIP(hipMemcpyAsync(c->h_state, dst, sizeof(CgState), hipMemcpyDeviceToHost, c->stream));
-            TF_HIP(hipStreamSynchronize(c->stream));
-            if (c->h_state->done) break;
-        }
+        cg_solve<T>(c, init, c->cg_guess);
         st->cg_iters_total += c->h_state->iter;
         st->cg_iters_max = std::max(st->cg_iters_max, c->h_state->iter);
         const double rel = c->h_state->relres;
         if (std::isfinite(rel)) st->cg_max_relres = std::max(st->cg_max_relres, rel);
-        if (c->h_state->iter >= init.maxiter && rel > init.tol) st->cg_not_converged++;
+        if (c->h_state->iter >= init.maxiter && !(rel <= init.tol)) st->cg_not_converged++;
         ev_begin(c, "gf_scatter_kernel");
         gf_scatter_kernel<T><<<blocks_for(nf + c->n_dir), kBlock, 0, c->stream>>>(
             nf, c->free_edge.as<int32_t>(), c->x.as<T>(), c->U.as<T>(), c->n_dir, c->dir_edge.as<int32_t>(),
