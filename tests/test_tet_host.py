@@ -1,0 +1,201 @@
+"""Tetrahedral reader and DGA / gfdtd operator producers (host side), pinned
+by closed-form counts, exact integer identities, a uniform-field patch test
+and SPD / symmetry properties."""
+import numpy as np
+import pytest
+
+from tetfit_amd import Model, configs, write_kuhn_mesh
+
+EPS0 = 8.854187817e-12
+
+
+def material_eps(fdtd_path):
+    """eps_r per material label, read from the .fdtd text."""
+    eps, lab = {}, None
+    for line in open(fdtd_path):
+        tok = line.split()
+        if tok[:2] == ["DEFINE", "material"]:
+            lab = int(tok[2])
+        elif tok[:2] == ["SET", "epsilon"] and lab is not None:
+            eps[lab] = float(tok[2])
+    return eps
+
+
+@pytest.fixture(scope="module")
+def kuhn(tmp_path_factory):
+    d = tmp_path_factory.mktemp("kuhn")
+    out = {}
+    for m in ("dga", "gfdtd"):
+        p = configs.kuhn_cavity(str(d), n=4, method=m, n_steps=5, n_probes=2, block=2, seed=11)
+        out[m] = Model(p).build(1)
+        out[m + "_eps"] = material_eps(p)
+    return out
+
+
+def test_kuhn_counts_closed_form(kuhn):
+    n = 4
+    b = kuhn["dga"]
+    assert b.n_vols == 6 * n ** 3
+    assert b.n_edges == 7 * n ** 3 + 9 * n ** 2 + 3 * n
+    assert b.n_faces == 12 * n ** 3 + 6 * n ** 2
+    t = b.tet_tables()
+    assert len(t["pts"]) == (n + 1) ** 3
+    assert int((t["boundary_face"] > 0).sum()) == 12 * n ** 2
+
+
+def test_numbering_sorted_unique(kuhn):
+    """Nodes sorted in each tet, tets sorted by node tuple, faces and edges
+    numbered by lexicographic order of their sorted node tuples (D:6081,
+    D:6120-6127, D:10173-10299)."""
+    t = kuhn["dga"].tet_tables()
+    v = t["vols"]
+    assert np.all(np.diff(v, axis=1) > 0)
+    keys = [tuple(r) for r in v]
+    assert keys == sorted(keys)
+    fn, en = t["face_nodes"], t["edge_nodes"]
+    assert [tuple(r) for r in fn] == sorted(set(tuple(r) for r in fn))
+    assert [tuple(r) for r in en] == sorted(set(tuple(r) for r in en))
+    # every tet face / edge is in the tables
+    fset = set(tuple(r) for r in fn)
+    eset = set(tuple(r) for r in en)
+    for a, b_, c, d in v:
+        for f in ((a, b_, c), (a, b_, d), (a, c, d), (b_, c, d)):
+            assert f in fset
+        for e in ((a, b_), (a, c), (a, d), (b_, c), (b_, d), (c, d)):
+            assert e in eset
+
+
+def test_curl_of_gradient_vanishes(kuhn):
+    b = kuhn["dga"]
+    d = b.dga_ops()
+    t = b.tet_tables()
+    C = d["C"]
+    en = t["edge_nodes"]
+    rng = np.random.default_rng(0)
+    phi = rng.integers(-1000, 1000, size=len(t["pts"]))
+    grad = phi[en[:, 1]] - phi[en[:, 0]]  # etn signs -1 / +1 (D:10245-10246)
+    cg = np.zeros(b.n_faces, dtype=np.int64)
+    for f in range(b.n_faces):
+        for q in range(C["rowptr"][f], C["rowptr"][f + 1]):
+            cg[f] += int(C["sgn"][q]) * int(grad[C["col"][q]])
+    assert not cg.any()
+    # Ct is the exact transpose of C
+    Ct = d["Ct"]
+    pairs = set()
+    for f in range(b.n_faces):
+        for q in range(C["rowptr"][f], C["rowptr"][f + 1]):
+            pairs.add((C["col"][q], f, int(C["sgn"][q])))
+    pairs_t = set()
+    for e in range(b.n_edges):
+        for q in range(Ct["rowptr"][e], Ct["rowptr"][e + 1]):
+            pairs_t.add((e, Ct["col"][q], int(Ct["sgn"][q])))
+    assert pairs == pairs_t
+
+
+def test_face_tet_adjacency(kuhn):
+    t = kuhn["dga"].tet_tables()
+    cnt = np.bincount(t["vtf"].ravel(), minlength=kuhn["dga"].n_faces)
+    bnd = t["boundary_face"] > 0
+    assert np.all(cnt[bnd] == 1) and np.all(cnt[~bnd] == 2)
+
+
+def test_boundary_labels_and_classes(kuhn):
+    b = kuhn["dga"]
+    t = b.tet_tables()
+    d = b.dga_ops()
+    pts = t["pts"]
+    fn = t["face_nodes"]
+    for f in np.where(t["boundary_face"] > 0)[0]:
+        p = pts[fn[f]]
+        lab = 0
+        for ax, lo, hi in ((2, 1, 6), (0, 2, 4), (1, 5, 3)):
+            if np.all(np.abs(p[:, ax]) < 1e-12):
+                lab = lo
+            elif np.all(np.abs(p[:, ax] - 1) < 1e-12):
+                lab = hi
+        assert t["boundary_face"][f] == lab
+    # class-4 (pec) edges are exactly the edges lying in a pec wall
+    en = t["edge_nodes"]
+    pe = pts[en]
+    in_pec = np.zeros(b.n_edges, bool)
+    for ax, val in ((2, 0), (2, 1), (0, 1), (1, 0), (1, 1)):
+        in_pec |= np.all(np.abs(pe[:, :, ax] - val) < 1e-12, axis=1)
+    assert np.array_equal(d["edge_class"] == 4, in_pec)
+    assert np.all(d["edge_class"][~in_pec] == 1)
+
+
+def corner_mass(t, eps_of_tet, oriented):
+    """Independent dense assembly of the gfdtd eps mass (D:8379-8429): at
+    every tet corner n with edges e1..e3, a_i = edge midpoint - p_n, dual
+    basis f_i = (a_j x a_k) / (a_i . (a_j x a_k)), E(ei,ej) += f_i . (|V|/4 eps) f_j.
+    oriented=True multiplies f_i by the orientation of e_i seen from n (+1
+    when n is its lower node); the reference does not (oriented=False)."""
+    pts, vols, en = t["pts"], t["vols"], t["edge_nodes"]
+    eid = {tuple(r): i for i, r in enumerate(en)}
+    E = np.zeros((len(en), len(en)))
+    for ti, v in enumerate(vols):
+        P = pts[v]
+        vol = abs(np.dot(P[1] - P[0], np.cross(P[2] - P[0], P[3] - P[0]))) / 6
+        w = vol / 4 * eps_of_tet[ti]
+        for k in range(4):
+            n = v[k]
+            others = [v[j] for j in range(4) if j != k]
+            es = [eid[(min(n, m), max(n, m))] for m in others]
+            a = [0.5 * (pts[m] + pts[n]) - pts[n] for m in others]
+            nc = 1.0 / np.dot(a[0], np.cross(a[1], a[2]))
+            f = [nc * np.cross(a[1], a[2]), nc * np.cross(a[2], a[0]), nc * np.cross(a[0], a[1])]
+            if oriented:
+                f = [fi * (1.0 if n < m else -1.0) for fi, m in zip(f, others)]
+            for i in range(3):
+                for j in range(3):
+                    E[es[i], es[j]] += w * np.dot(f[i], f[j])
+    return E
+
+
+def test_gfdtd_mass_matches_independent_assembly(kuhn):
+    b = kuhn["gfdtd"]
+    t = b.tet_tables()
+    Em = b.dga_ops()["E"]
+    eps = kuhn["gfdtd_eps"]
+    epst = EPS0 * np.array([eps[m] for m in t["vol_material"]])
+    ref = corner_mass(t, epst, oriented=False)
+    got = np.zeros_like(ref)
+    for i in range(b.n_edges):
+        sl = slice(Em["rowptr"][i], Em["rowptr"][i + 1])
+        got[i, Em["col"][sl]] = Em["val"][sl]
+    assert np.abs(got - ref).max() <= 1e-13 * np.abs(ref).max()
+    # The dual basis itself is pinned by the uniform-field patch test: with
+    # orientation signs, U_e = E0.(p_b - p_a) gives U^T E U = 4 sum eps |E0|^2 |V|.
+    ori = corner_mass(t, epst, oriented=True)
+    pts, en, vols = t["pts"], t["edge_nodes"], t["vols"]
+    V = np.abs(np.einsum("ij,ij->i", pts[vols[:, 1]] - pts[vols[:, 0]],
+                         np.cross(pts[vols[:, 2]] - pts[vols[:, 0]], pts[vols[:, 3]] - pts[vols[:, 0]]))) / 6
+    rng = np.random.default_rng(5)
+    for _ in range(3):
+        E0 = rng.standard_normal(3)
+        U = (pts[en[:, 1]] - pts[en[:, 0]]) @ E0
+        rhs = 4 * (E0 @ E0) * np.sum(epst * V)
+        assert abs(U @ ori @ U - rhs) <= 1e-12 * rhs
+        # the reference's unoriented basis does not reproduce a uniform field
+        assert abs(U @ ref @ U - rhs) > 1e-6 * rhs
+
+
+def test_gfdtd_system_spd(kuhn):
+    d = kuhn["gfdtd"].dga_ops()
+    A = d["A"]
+    n = A["nrows"]
+    M = np.zeros((n, n))
+    for i in range(n):
+        sl = slice(A["rowptr"][i], A["rowptr"][i + 1])
+        M[i, A["col"][sl]] = A["val"][sl]
+    assert np.abs(M - M.T).max() <= 1e-14 * np.abs(M).max()
+    assert np.linalg.eigvalsh(M).min() > 0
+
+
+def test_kuhn_writer_rejects_inversion(tmp_path):
+    # jitter 0.2h never inverts a Kuhn tet at this size/seed; 0.49h eventually does
+    lab, r = write_kuhn_mesh(str(tmp_path / "ok.mesh"), 4, 0.2, 1, 2)
+    assert r > 0
+    with pytest.raises(Exception):
+        for s in range(50):
+            write_kuhn_mesh(str(tmp_path / "bad.mesh"), 6, 0.49, s, 2)

@@ -44,7 +44,8 @@ def _duration(n_steps, dt, first_step):
 
 
 def fit_cavity(directory, n=64, step=0.0156, n_steps=2000, n_probes=32, dt=None,
-               freq=2e8, carrier="sin", name="fit", eps_r=1.0, sigma=0.0, extra_source=""):
+               freq=2e8, carrier="sin", name="fit", eps_r=1.0, sigma=0.0, extra_source="",
+               pec_surfaces="1,3,4,5,6", source_surface=2):
     """FIT box [0, n*step]^3 (xgrid {0,step,n*step+tiny}), one box solid of
     material 1. Returns the .fdtd path."""
     os.makedirs(directory, exist_ok=True)
@@ -75,8 +76,9 @@ def fit_cavity(directory, n=64, step=0.0156, n_steps=2000, n_probes=32, dt=None,
             f.write("    SET %sgrid {0,%r,%r}\n" % (ax, step, stop))
         f.write("END mesh 1\n")
         f.write(mat)
-        f.write("DEFINE bc 1\n    SET type pec\n    SET surfaces {1,3,4,5,6}\nEND bc 1\n")
-        f.write(SOURCE_BLOCK.format(carrier=carrier, freq=repr(freq), extra=extra_source))
+        f.write("DEFINE bc 1\n    SET type pec\n    SET surfaces {%s}\nEND bc 1\n" % pec_surfaces)
+        f.write(SOURCE_BLOCK.format(carrier=carrier, freq=repr(freq), extra=extra_source)
+                .replace("SET surface 2", "SET surface %d" % source_surface))
         f.write("DEFINE output 1\n    SET mode probepoint\n    SET name %s\n    SET period 0\n" % name)
         f.write(_probe_lines(pts))
         f.write("END output 1\n")
@@ -88,7 +90,7 @@ def fit_cavity(directory, n=64, step=0.0156, n_steps=2000, n_probes=32, dt=None,
 
 def kuhn_cavity(directory, n=48, method="dga", n_steps=1000, n_probes=8, jitter=0.2, seed=2,
                 block=8, eps_range=(1.0, 10.0), log_eps=False, freq=2e8, tol=1e-8, dt=None,
-                name=None, dt_safety=0.9):
+                name=None, dt_safety=0.9, pec_surfaces="1,3,4,5,6", source_surface=2):
     """Kuhn mesh of the unit cube, eps_r per block drawn from eps_range
     (uniform, or log-uniform) with numpy default_rng(seed). Returns the .fdtd
     path. When dt is None the host's stability estimate is computed once and
@@ -119,8 +121,9 @@ def kuhn_cavity(directory, n=48, method="dga", n_steps=1000, n_probes=8, jitter=
                     "    SET file %s\nEND mesh 1\n" % os.path.basename(mesh))
             for l, e in zip(labels, eps):
                 f.write("DEFINE material %d\n    SET epsilon %r\n    SET mu 1\nEND material %d\n" % (l, float(e), l))
-            f.write("DEFINE bc 1\n    SET type pec\n    SET surfaces {1,3,4,5,6}\nEND bc 1\n")
-            f.write(SOURCE_BLOCK.format(carrier="sin", freq=repr(freq), extra=""))
+            f.write("DEFINE bc 1\n    SET type pec\n    SET surfaces {%s}\nEND bc 1\n" % pec_surfaces)
+            f.write(SOURCE_BLOCK.format(carrier="sin", freq=repr(freq), extra="")
+                    .replace("SET surface 2", "SET surface %d" % source_surface))
             f.write("DEFINE output 1\n    SET mode probepoint\n    SET name %s\n    SET period 0\n" % name)
             f.write(_probe_lines(pts))
             f.write("END output 1\n")

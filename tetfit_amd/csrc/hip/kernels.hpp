@@ -220,8 +220,14 @@ __global__ void __launch_bounds__(kBlock) dga_curl_kernel(int32_t n_faces, const
     const int32_t f = blockIdx.x * kBlock + threadIdx.x;
     if (f >= n_faces) return;
     const int32_t* c = c_col + 3 * (int64_t)f;
-    const int8_t* s = c_sg + 3 * (int64_t)f;
-    const double cu = (double)s[0] * (double)U[c[0]] + (double)s[1] * (double)U[c[1]] + (double)s[2] * (double)U[c[2]];
+    double cu;
+    if (c_sg) {
+        const int8_t* s = c_sg + 3 * (int64_t)f;
+        cu = (double)s[0] * (double)U[c[0]] + (double)s[1] * (double)U[c[1]] + (double)s[2] * (double)U[c[2]];
+    } else {
+        // every tet face has the fixed pattern +1,-1,+1 on (p0p1, p0p2, p1p2) (D:6345-6352)
+        cu = (double)U[c[0]] - (double)U[c[1]] + (double)U[c[2]];
+    }
     curl_u[f] = (T)cu;
     B[f] = (T)((double)B[f] - dt * cu);
 }

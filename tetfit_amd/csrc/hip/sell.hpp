@@ -16,30 +16,31 @@ namespace tfd {
 
 constexpr int kSlice = 64;
 
-// Sum over the slice row of val*gather(col), unrolled 4 deep so that each
-// lane has several independent loads in flight.
+// Sum over the slice row of val*gather(col). Loads are issued in batches of
+// 8 entries (16 independent loads, then 8 independent gathers) so that each
+// lane keeps enough memory traffic in flight; the tail batch is predicated.
+// Accumulation stays in column order.
 template <typename T, class G>
 __device__ __forceinline__ double sell_dot(int64_t base, int w, const int32_t* __restrict__ col,
                                            const T* __restrict__ val, int lane, G gather) {
+    constexpr int U = 8;
     const int32_t* c = col + base + lane;
     const T* v = val + base + lane;
     double acc = 0;
-    int k = 0;
-    for (; k + 4 <= w; k += 4) {
-        const int32_t c0 = c[0], c1 = c[64], c2 = c[128], c3 = c[192];
-        const double v0 = (double)v[0], v1 = (double)v[64], v2 = (double)v[128], v3 = (double)v[192];
-        const double g0 = gather(c0), g1 = gather(c1), g2 = gather(c2), g3 = gather(c3);
-        acc += v0 * g0;
-        acc += v1 * g1;
-        acc += v2 * g2;
-        acc += v3 * g3;
-        c += 256;
-        v += 256;
-    }
-    for (; k < w; ++k) {
-        acc += (double)v[0] * gather(c[0]);
-        c += 64;
-        v += 64;
+    for (int k0 = 0; k0 < w; k0 += U) {
+        int32_t ci[U];
+        double vi[U], gi[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int kk = min(k0 + u, w - 1);  // clamped: every load stays inside the slice
+            ci[u] = c[kk * 64];
+            vi[u] = (double)v[kk * 64];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) gi[u] = gather(ci[u]);
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (k0 + u < w) acc += vi[u] * gi[u];
     }
     return acc;
 }
@@ -48,13 +49,24 @@ __device__ __forceinline__ double sell_dot(int64_t base, int w, const int32_t* _
 template <typename T>
 __device__ __forceinline__ double sell_sgn_dot(int64_t base, int w, const int32_t* __restrict__ col,
                                                const int8_t* __restrict__ sg, int lane, const T* __restrict__ x) {
+    constexpr int U = 8;
     const int32_t* c = col + base + lane;
     const int8_t* s = sg + base + lane;
     double acc = 0;
-    for (int k = 0; k < w; ++k) {
-        acc += (double)s[0] * (double)x[c[0]];
-        c += 64;
-        s += 64;
+    for (int k0 = 0; k0 < w; k0 += U) {
+        int32_t ci[U];
+        double si[U], gi[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int kk = min(k0 + u, w - 1);
+            ci[u] = c[kk * 64];
+            si[u] = (double)s[kk * 64];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) gi[u] = (double)x[ci[u]];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (k0 + u < w) acc += si[u] * gi[u];
     }
     return acc;
 }
@@ -113,29 +125,28 @@ __global__ void __launch_bounds__(kBlock) sell_n_kernel(int32_t nrows, int32_t n
 }
 
 // rhs[k] = E U_old - RHS1 SrcFld + dt (Ct F + fb) at free edge e = free[k]
-// (D:1516-1523; sigma and I are zero on the supported inputs). SrcFld is U on
+// (D:1516-1523; sigma and I are zero on the supported inputs). The three
+// operators are stored as SELL-64 over the free rows; SrcFld is U on
 // Dirichlet edges at the new time, held in `dir_new` (edge-indexed).
-template <typename T, int LPR>
+template <typename T>
 __global__ void __launch_bounds__(kBlock) gf_rhs_kernel(
-    int32_t n_free, const int32_t* __restrict__ free_edge, const int32_t* __restrict__ e_ptr,
-    const int32_t* __restrict__ e_col, const T* __restrict__ e_val, const int32_t* __restrict__ r_ptr,
-    const int32_t* __restrict__ r_col, const T* __restrict__ r_val, const int32_t* __restrict__ ct_ptr,
-    const int32_t* __restrict__ ct_col, const int8_t* __restrict__ ct_sg, const T* __restrict__ U,
+    int32_t n_free, int32_t nslices, const int32_t* __restrict__ free_edge, const int64_t* __restrict__ e_ptr,
+    const int32_t* __restrict__ e_col, const T* __restrict__ e_val, const int64_t* __restrict__ r_ptr,
+    const int32_t* __restrict__ r_col, const T* __restrict__ r_val, const int64_t* __restrict__ c_ptr,
+    const int32_t* __restrict__ c_col, const int8_t* __restrict__ c_sg, const T* __restrict__ U,
     const T* __restrict__ dir_new, const T* __restrict__ Fv, const T* __restrict__ fb, T* __restrict__ rhs,
     double dt) {
-    const int32_t g = blockIdx.x * kBlock + threadIdx.x;
-    const int32_t k = g / LPR, lane = g % LPR;
-    double a = 0, c = 0;
-    int32_t e = 0;
-    if (k < n_free) {
-        e = free_edge[k];
-        for (int32_t q = e_ptr[e] + lane; q < e_ptr[e + 1]; q += LPR) a += (double)e_val[q] * (double)U[e_col[q]];
-        for (int32_t q = r_ptr[e] + lane; q < r_ptr[e + 1]; q += LPR) a -= (double)r_val[q] * (double)dir_new[r_col[q]];
-        for (int32_t q = ct_ptr[e] + lane; q < ct_ptr[e + 1]; q += LPR) c += (double)ct_sg[q] * (double)Fv[ct_col[q]];
+    const int lane = lane_id();
+    for (int s = wave_id(); s < nslices; s += wave_count()) {
+        const int64_t eb = e_ptr[s], rb = r_ptr[s], cb = c_ptr[s];
+        const int ew = (int)((e_ptr[s + 1] - eb) / kSlice), rw = (int)((r_ptr[s + 1] - rb) / kSlice);
+        const int cw = (int)((c_ptr[s + 1] - cb) / kSlice);
+        const double eu = sell_dot<T>(eb, ew, e_col, e_val, lane, [&](int32_t j) { return (double)U[j]; });
+        const double rs = rw ? sell_dot<T>(rb, rw, r_col, r_val, lane, [&](int32_t j) { return (double)dir_new[j]; }) : 0.0;
+        const double cf = sell_sgn_dot<T>(cb, cw, c_col, c_sg, lane, Fv);
+        const int32_t k = s * kSlice + lane;
+        if (k < n_free) rhs[k] = (T)((eu - rs) + dt * cf + dt * (double)fb[free_edge[k]]);
     }
-    a = group_sum<LPR>(a);
-    c = group_sum<LPR>(c);
-    if (k < n_free && lane == 0) rhs[k] = (T)(a + dt * (c + (double)fb[e]));
 }
 
 // ---- CG (ConjugateGradientSolver.hpp:52-103) ------------------------------

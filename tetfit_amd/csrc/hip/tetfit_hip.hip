@@ -98,8 +98,9 @@ struct tf_ctx {
     int32_t n_sp = 0;
 
     // DGA / gfdtd
-    Csr32 Ct, Em, R1;
-    Sell sCt, H, N, A;
+    Csr32 Ct;
+    Sell sCt, H, N, A, Ef, R1f, Ctf;
+    bool c_fixed_signs = false;
     DevBuf c_col, c_sg, h_edge, curl_f, curl_u;
     DevBuf free_edge, dir_edge, dir_new, fb, pre, x, r, z, p0, p1, q, rhs, cg_part, cg_state;
     int32_t n_free = 0, n_dir = 0;
@@ -156,9 +157,9 @@ size_t tf_ctx::device_bytes() const {
                            &p0, &p1, &q, &rhs, &cg_part, &cg_state, &inj_edge, &inj_kind, &inj_tb, &inj_te,
                            &term_src, &term_w, &carr, &probes, &samples, &poy};
     for (auto b : all) s += b->bytes;
-    const Csr32* cs[] = {&Ct, &Em, &R1};
+    const Csr32* cs[] = {&Ct};
     for (auto c : cs) s += c->ptr.bytes + c->col.bytes + c->val.bytes + c->sg.bytes;
-    const Sell* ss[] = {&sCt, &H, &N, &A};
+    const Sell* ss[] = {&sCt, &H, &N, &A, &Ef, &R1f, &Ctf};
     for (auto c : ss) s += c->sptr.bytes + c->col.bytes + c->val.bytes + c->sg.bytes;
     return s;
 }
@@ -357,12 +358,16 @@ void create_tet(tf_ctx* c, const tf_problem* p) {
     const tf_dga_ops& d = p->dga;
     const int32_t E = c->E, F = c->F;
     if (p->method == TF_METHOD_DGA) upload_sell<T>(c->sCt, d.Ct, false);
-    else upload_csr<T>(c->Ct, d.Ct, false);
     if (d.C.nnz != 3 * (int64_t)F) throw std::runtime_error("C must have 3 entries per face");
     c->c_col.alloc(d.C.nnz * 4);
     TF_HIP(hipMemcpy(c->c_col.p, d.C.col, d.C.nnz * 4, hipMemcpyHostToDevice));
-    c->c_sg.alloc(d.C.nnz);
-    TF_HIP(hipMemcpy(c->c_sg.p, d.C.sgn, d.C.nnz, hipMemcpyHostToDevice));
+    c->c_fixed_signs = true;
+    for (int64_t q = 0; q < d.C.nnz && c->c_fixed_signs; ++q)
+        if (d.C.sgn[q] != (q % 3 == 1 ? -1 : 1)) c->c_fixed_signs = false;
+    if (!c->c_fixed_signs) {
+        c->c_sg.alloc(d.C.nnz);
+        TF_HIP(hipMemcpy(c->c_sg.p, d.C.sgn, d.C.nnz, hipMemcpyHostToDevice));
+    }
     upload_sell<T>(c->N, d.N, true);
     c->h_fixed_value.assign(d.fixed_value, d.fixed_value + E);
     zero_field<T>(c->curl_u, F, c->stream);
@@ -376,9 +381,44 @@ void create_tet(tf_ctx* c, const tf_problem* p) {
             if (he[r] < 0) throw std::runtime_error("H row without an edge");
         upload(c->h_edge, he);
     } else {
-        upload_csr<T>(c->Em, d.E, true);
-        upload_csr<T>(c->R1, d.RHS1, true);
         upload_sell<T>(c->A, d.A, true);
+        // free rows of E, RHSmat1 and Ct for the right-hand side
+        {
+            auto free_rows = [&](const tf_csr& m, std::vector<int64_t>& rp, std::vector<int32_t>& col,
+                                 std::vector<double>& val, std::vector<int8_t>& sg) {
+                rp.assign(1, 0);
+                for (int32_t k = 0; k < d.n_free; ++k) {
+                    const int32_t e = d.free_edges[k];
+                    for (int64_t q = m.rowptr[e]; q < m.rowptr[e + 1]; ++q) {
+                        col.push_back(m.col[q]);
+                        if (m.val) val.push_back(m.val[q]);
+                        if (m.sgn) sg.push_back(m.sgn[q]);
+                    }
+                    rp.push_back((int64_t)col.size());
+                }
+                tf_csr o{};
+                o.nrows = d.n_free;
+                o.ncols = m.ncols;
+                o.nnz = rp.back();
+                o.rowptr = rp.data();
+                o.col = col.data();
+                o.val = m.val ? val.data() : nullptr;
+                o.sgn = m.sgn ? sg.data() : nullptr;
+                return o;
+            };
+            std::vector<int64_t> rp;
+            std::vector<int32_t> col;
+            std::vector<double> val;
+            std::vector<int8_t> sg;
+            tf_csr ef = free_rows(d.E, rp, col, val, sg);
+            upload_sell<T>(c->Ef, ef, true);
+            rp.clear(); col.clear(); val.clear(); sg.clear();
+            tf_csr rf = free_rows(d.RHS1, rp, col, val, sg);
+            upload_sell<T>(c->R1f, rf, true);
+            rp.clear(); col.clear(); val.clear(); sg.clear();
+            tf_csr cf = free_rows(d.Ct, rp, col, val, sg);
+            upload_sell<T>(c->Ctf, cf, false);
+        }
         c->n_free = d.n_free;
         std::vector<int32_t> fe(d.free_edges, d.free_edges + d.n_free);
         upload(c->free_edge, fe);
@@ -662,7 +702,7 @@ void run_dga(tf_ctx* c, int64_t first, int64_t n, tf_stats* st) {
         }
         ev_begin(c, "dga_curl_kernel");
         dga_curl_kernel<T><<<blocks_for(c->F), kBlock, 0, c->stream>>>(c->F, c->c_col.as<int32_t>(),
-                                                                        c->c_sg.as<int8_t>(), c->U.as<T>(),
+                                                                        c->c_fixed_signs ? nullptr : c->c_sg.as<int8_t>(), c->U.as<T>(),
                                                                         c->curl_u.as<T>(), c->B.as<T>(), c->dt);
         ev_end(c);
         ev_begin(c, "sell_n_kernel");
@@ -751,7 +791,7 @@ void run_gfdtd(tf_ctx* c, int64_t first, int64_t n, const tf_cg_opts* cg, tf_sta
         const double* cp = c->carr.as<double>() + k * ns;
         ev_begin(c, "dga_curl_kernel");
         dga_curl_kernel<T><<<blocks_for(c->F), kBlock, 0, c->stream>>>(c->F, c->c_col.as<int32_t>(),
-                                                                        c->c_sg.as<int8_t>(), c->U.as<T>(),
+                                                                        c->c_fixed_signs ? nullptr : c->c_sg.as<int8_t>(), c->U.as<T>(),
                                                                         c->curl_u.as<T>(), c->B.as<T>(), c->dt);
         ev_end(c);
         ev_begin(c, "sell_n_kernel");
@@ -769,11 +809,11 @@ void run_gfdtd(tf_ctx* c, int64_t first, int64_t n, const tf_cg_opts* cg, tf_sta
             ev_end(c);
         }
         ev_begin(c, "gf_rhs_kernel");
-        gf_rhs_kernel<T, 16><<<blocks_for((int64_t)nf * 16), kBlock, 0, c->stream>>>(
-            nf, c->free_edge.as<int32_t>(), c->Em.ptr.as<int32_t>(), c->Em.col.as<int32_t>(), c->Em.val.as<T>(),
-            c->R1.ptr.as<int32_t>(), c->R1.col.as<int32_t>(), c->R1.val.as<T>(), c->Ct.ptr.as<int32_t>(),
-            c->Ct.col.as<int32_t>(), c->Ct.sg.as<int8_t>(), c->U.as<T>(), c->dir_new.as<T>(), c->Fv.as<T>(),
-            c->fb.as<T>(), c->rhs.as<T>(), c->dt);
+        gf_rhs_kernel<T><<<sell_blocks(c->Ef, kSpmvCap), kBlock, 0, c->stream>>>(
+            nf, c->Ef.nslices, c->free_edge.as<int32_t>(), c->Ef.sptr.as<int64_t>(), c->Ef.col.as<int32_t>(),
+            c->Ef.val.as<T>(), c->R1f.sptr.as<int64_t>(), c->R1f.col.as<int32_t>(), c->R1f.val.as<T>(),
+            c->Ctf.sptr.as<int64_t>(), c->Ctf.col.as<int32_t>(), c->Ctf.sg.as<int8_t>(), c->U.as<T>(),
+            c->dir_new.as<T>(), c->Fv.as<T>(), c->fb.as<T>(), c->rhs.as<T>(), c->dt);
         ev_end(c);
         cg_solve<T>(c, init, c->cg_guess);
         st->cg_iters_total += c->h_state->iter;
