@@ -11,7 +11,7 @@ A step is one leapfrog step over the whole mesh. value = edge-DoF updates/s
 = n_edges * steps / seconds of the timed region (HIP events on the
 context's stream, inputs resident in HBM). The headline path is
 "learned+FP32": fp32 fields and operators with fp64 CG reductions and the
-learned per-edge preconditioner; "plain FP64" (Jacobi CG) is reported beside
+learned sparse-approximate-inverse preconditioner; "plain FP64" (Jacobi CG) is reported beside
 it. CG iterations/step are the mean ConjugateGradientSolver::iterations().
 
 Prints one JSON line. --gpus N with N > 1 exits: this build drives one GPU.
@@ -67,9 +67,20 @@ def algorithmic_bytes(b, kernel, vb):
     raise KeyError(kernel)
 
 
-def time_path(b, precision, steps, warmup, precond="jacobi", label=""):
+def time_path(b, precision, steps, warmup, precond="jacobi", label="", learn_steps=200):
+    """Times `steps` steps after `warmup` untimed ones. For the learned
+    preconditioner the context first runs its `learn_steps` Jacobi-PCG
+    collection steps and the on-device training (untimed, reported apart)."""
     from tetfit_amd.device import Simulation
     sim = Simulation(b, precision, output=False)
+    learn = None
+    if precond == "learned":
+        ls = sim.run(n=learn_steps, precond="learned", learn_steps=learn_steps)
+        learn = dict(learn_steps=learn_steps, collection_seconds=ls["seconds"] - ls["pc_train_seconds"],
+                     train_seconds=ls["pc_train_seconds"], pairs=ls["pc_pairs"], loss0=ls["pc_loss0"],
+                     loss=ls["pc_loss"], jacobi_iters_per_step=ls["pc_jacobi_mean_iters"], state=ls["pc_state"])
+        log(f"[bench] {label}: trained on {ls['pc_pairs']} pairs in {ls['pc_train_seconds']:.2f} s, "
+            f"loss {ls['pc_loss0']:.4g} -> {ls['pc_loss']:.4g}")
     if warmup:
         sim.run(n=warmup, precond=precond)
     st = sim.run(n=steps, precond=precond)
@@ -79,6 +90,9 @@ def time_path(b, precision, steps, warmup, precond="jacobi", label=""):
                cg_iters_per_step=st["cg_iters_total"] / steps if b.method == "gfdtd" else None,
                cg_max_relres=st["cg_max_relres"] if b.method == "gfdtd" else None,
                cg_not_converged=st["cg_not_converged"] if b.method == "gfdtd" else None)
+    if learn is not None:
+        learn.update(state_after=st["pc_state"], fallbacks=st["pc_fallbacks"])
+        out["learned"] = learn
     # per-kernel durations with HIP events on the context's stream
     sim.set_profiling(True)
     sim.run(n=min(steps, 50), precond=precond)
@@ -118,19 +132,16 @@ def main():
     if device_count() < 1:
         print("bench.py needs a HIP device", file=sys.stderr)
         sys.exit(1)
-    total = args.warmup + args.steps + 60
+    total = args.warmup + args.steps + 260
     with tempfile.TemporaryDirectory() as d:
         bg = build_workload(d, "gfdtd", args.n, total)
         plain = time_path(bg, "fp64", args.steps, args.warmup, "jacobi", "gfdtd fp64 jacobi (plain)")
         headline_label = "gfdtd fp64 jacobi"
         head = plain
         f32j = time_path(bg, "fp32", args.steps, args.warmup, "jacobi", "gfdtd fp32 jacobi")
-        try:
-            fast = time_path(bg, "fp32", args.steps, args.warmup, "learned", "gfdtd fp32 learned")
-            head, headline_label = fast, "gfdtd fp32 learned"
-        except Exception as e:  # the learned path needs its trained network
-            log(f"[bench] learned+fp32 path unavailable: {e}")
-            fast = None
+        fast = time_path(bg, "fp32", args.steps, args.warmup, "learned", "gfdtd fp32 learned")
+        head, headline_label = fast, "gfdtd fp32 learned"
+        l64 = time_path(bg, "fp64", args.steps, args.warmup, "learned", "gfdtd fp64 learned")
         dga = None
         if not args.no_dga:
             bd = build_workload(d, "dga", args.n, total)
@@ -173,6 +184,8 @@ def main():
             "speedup_vs_plain_fp64": head["edge_dof_per_s"] / plain["edge_dof_per_s"],
             "fp32_jacobi": {"edge_dof_per_s": f32j["edge_dof_per_s"], "cg_iters_per_step": f32j["cg_iters_per_step"],
                             "cg_max_relres": f32j["cg_max_relres"], "cg_not_converged": f32j["cg_not_converged"]},
+            "fp64_learned": {"edge_dof_per_s": l64["edge_dof_per_s"], "cg_iters_per_step": l64["cg_iters_per_step"]},
+            "learned": head.get("learned"),
             "roofline": roof,
             "cpu_baseline": cpu,
         }

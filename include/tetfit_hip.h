@@ -46,13 +46,31 @@ typedef enum {
 
 typedef enum {
     TF_PRECOND_JACOBI  = 0,    /* 1/diag(A), ConjugateGradientSolver.hpp:119  */
-    TF_PRECOND_LEARNED = 1     /* per-edge diagonal predicted by the MLP      */
+    TF_PRECOND_LEARNED = 1     /* learned sparse approximate inverse, below   */
 } tf_precond_kind;
 
+/* TF_PRECOND_LEARNED (gfdtd only; it replaces the diagonal preconditioner of
+ * ConjugateGradientSolver.hpp:119-128 and the AGMG option of agmg.hpp):
+ * G is symmetric with the pattern of A,
+ *     G_ej = (base_ej + y_ej) / sqrt(A_ee A_jj),
+ *     base = 1 on the diagonal, -A_ej / sqrt(A_ee A_jj) off it,
+ * and y_ej is the output of a 16 -> 64 -> 64 -> 1 ReLU MLP on 16 features of
+ * the entry pair {e, j} (symmetric in e, j). The first `learn_steps` solves
+ * of the context run Jacobi-PCG and capture (residual, correction) pairs
+ * (r_k, x* - x_k) at CG iterations 0, 3, 7 and 12 of a dozen of them; the
+ * network is then trained on the device (bf16 MFMA forward/backward, Adam)
+ * to minimise the A-norm of the error left by one step x += G r, and frozen.
+ * Later solves use G; a solve that misses the tolerance is redone with
+ * Jacobi-PCG from the same start, and plain CG also judges the gain (no fewer
+ * iterations than Jacobi over the first 8 learned solves turns G off). The
+ * stopping test is the reference's (||r|| / ||r0|| <= tol) in every case. */
 typedef struct {
     double  tol;               /* relative residual vs warm-start residual    */
     int32_t maxiter;           /* 100 in the reference callers (D:1388)       */
     int32_t precond;           /* tf_precond_kind                             */
+    int32_t learn_steps;       /* LEARNED: solves that supply training pairs (0: 200) */
+    int32_t train_iters;       /* LEARNED: Adam steps (0: 150)                */
+    double  lr;                /* LEARNED: Adam learning rate (0: 3e-3)       */
 } tf_cg_opts;
 
 typedef struct {
@@ -62,7 +80,22 @@ typedef struct {
     int32_t cg_not_converged;  /* steps that stopped at maxiter                */
     double  cg_max_relres;     /* max over steps of final ||r||/||r0||          */
     double  seconds;           /* wall time of the step loop (hip events)      */
+    /* learned preconditioner, cumulative since the context was created */
+    int32_t pc_state;          /* 0 unused, 1 collecting, 2 active, 3 turned off */
+    int32_t pc_pairs;          /* captured (residual, correction) pairs        */
+    int64_t pc_fallbacks;      /* learned solves redone with Jacobi-PCG        */
+    int64_t pc_learned_solves;
+    int64_t pc_learned_iters;  /* CG iterations of the learned solves          */
+    double  pc_jacobi_mean_iters; /* mean iterations of the collection solves  */
+    double  pc_train_seconds;
+    double  pc_loss0, pc_loss; /* training loss at the start / best            */
 } tf_stats;
+
+/* Parameter vector of the preconditioner MLP (fp32): W1[64][16], b1[64],
+ * W2[64][64], b2[64], w3[64], b3. */
+#define TF_MLP_IN 16
+#define TF_MLP_HIDDEN 64
+#define TF_MLP_PARAMS 5313
 
 typedef struct tf_ctx tf_ctx;
 
@@ -123,6 +156,14 @@ tf_status tf_set_profiling(tf_ctx* ctx, int32_t on);
 tf_status tf_device_bytes(tf_ctx* ctx, int64_t* bytes);
 
 tf_status tf_destroy(tf_ctx* ctx);
+
+/* One forward and backward pass of the preconditioner MLP on the device
+ * (the kernels training uses), for gradient checks: y[i] = MLP(x[i]) and
+ * grad = sum_i gy[i] dy[i]/dparams, for n feature rows x (n x TF_MLP_IN,
+ * rounded to bf16 on upload). No reference counterpart: the reference has no
+ * learned component. */
+tf_status tf_mlp_grad(const float* params, const float* x, const float* gy,
+                      int64_t n, float* y, float* grad);
 
 #ifdef __cplusplus
 }

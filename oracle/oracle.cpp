@@ -622,4 +622,50 @@ int32_t oracle_run(const tf_problem* p, const tf_source_desc* sd, int32_t n_sd, 
     }
 }
 
+// Preconditioner MLP in fp64 (no reference counterpart: the reference has no
+// learned component). Parameter layout W1[64][16], b1[64], W2[64][64], b2[64],
+// w3[64], b3. y[i] = w3 . relu(W2 relu(W1 x_i + b1) + b2) + b3 and
+// grad = sum_i gy[i] dy[i]/dparams, by hand-written backpropagation.
+void oracle_mlp(const float* P, const float* X, const double* gy, int64_t n, double* y, double* grad) {
+    const int I = 16, H = 64;
+    const int oW1 = 0, oB1 = oW1 + H * I, oW2 = oB1 + H, oB2 = oW2 + H * H, oW3 = oB2 + H, oB3 = oW3 + H;
+    if (grad)
+        for (int k = 0; k <= oB3; ++k) grad[k] = 0;
+    std::vector<double> z1(H), h1(H), z2(H), h2(H), g2(H), g1(H);
+    for (int64_t i = 0; i < n; ++i) {
+        const float* x = X + i * I;
+        for (int a = 0; a < H; ++a) {
+            double t = P[oB1 + a];
+            for (int k = 0; k < I; ++k) t += (double)P[oW1 + a * I + k] * x[k];
+            z1[a] = t;
+            h1[a] = t > 0 ? t : 0;
+        }
+        double out = P[oB3];
+        for (int a = 0; a < H; ++a) {
+            double t = P[oB2 + a];
+            for (int b = 0; b < H; ++b) t += (double)P[oW2 + a * H + b] * h1[b];
+            z2[a] = t;
+            h2[a] = t > 0 ? t : 0;
+            out += (double)P[oW3 + a] * h2[a];
+        }
+        if (y) y[i] = out;
+        if (!grad) continue;
+        const double g = gy[i];
+        grad[oB3] += g;
+        for (int a = 0; a < H; ++a) {
+            grad[oW3 + a] += g * h2[a];
+            g2[a] = z2[a] > 0 ? g * P[oW3 + a] : 0;
+            grad[oB2 + a] += g2[a];
+            for (int b = 0; b < H; ++b) grad[oW2 + a * H + b] += g2[a] * h1[b];
+        }
+        for (int b = 0; b < H; ++b) {
+            double t = 0;
+            for (int a = 0; a < H; ++a) t += (double)P[oW2 + a * H + b] * g2[a];
+            g1[b] = z1[b] > 0 ? t : 0;
+            grad[oB1 + b] += g1[b];
+            for (int k = 0; k < I; ++k) grad[oW1 + b * I + k] += g1[b] * x[k];
+        }
+    }
+}
+
 }  // extern "C"

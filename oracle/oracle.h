@@ -76,6 +76,10 @@ int32_t oracle_cg_solve(int32_t n, const int64_t* rowptr, const int32_t* col, co
 
 const char* oracle_last_error(void);
 
+/* Preconditioner MLP (16 -> 64 -> 64 -> 1, ReLU) forward and backward in
+ * fp64: y[i] and grad = sum_i gy[i] dy[i]/dparams. */
+void oracle_mlp(const float* params, const float* x, const double* gy, int64_t n, double* y, double* grad);
+
 #ifdef __cplusplus
 }
 #endif

@@ -43,6 +43,9 @@ def lib():
     L.oracle_cg_solve.argtypes = [C.c_int32, C.POINTER(C.c_int64), C.POINTER(C.c_int32), dp, dp, dp,
                                   C.c_double, C.c_int32, dp, dp]
     L.oracle_cg_solve.restype = C.c_int32
+    fp = C.POINTER(C.c_float)
+    L.oracle_mlp.argtypes = [fp, fp, dp, C.c_int64, dp, dp]
+    L.oracle_mlp.restype = None
     _lib = L
     return L
 
@@ -117,3 +120,17 @@ def cg_solve(rowptr, col, val, b, x0, tol, maxiter):
     it = L.oracle_cg_solve(n, rowptr.ctypes.data_as(C.POINTER(C.c_int64)), col.ctypes.data_as(C.POINTER(C.c_int32)),
                            _dp(val), _dp(b), _dp(x0), float(tol), int(maxiter), _dp(x), C.byref(err))
     return x, it, err.value
+
+
+def mlp(params, x, gy=None):
+    """FP64 preconditioner MLP: returns (y, grad) for rows x [n, 16]."""
+    L = lib()
+    fp = C.POINTER(C.c_float)
+    P = np.ascontiguousarray(params, dtype=np.float32)
+    X = np.ascontiguousarray(x, dtype=np.float32)
+    n = X.shape[0]
+    G = np.zeros(n) if gy is None else np.ascontiguousarray(gy, dtype=np.float64)
+    y = np.zeros(n)
+    g = np.zeros(P.size)
+    L.oracle_mlp(P.ctypes.data_as(fp), X.ctypes.data_as(fp), _dp(G), n, _dp(y), _dp(g))
+    return y, g

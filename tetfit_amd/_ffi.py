@@ -84,13 +84,21 @@ class tf_probe_geom(C.Structure):
 
 
 class tf_cg_opts(C.Structure):
-    _fields_ = [("tol", C.c_double), ("maxiter", C.c_int32), ("precond", C.c_int32)]
+    _fields_ = [("tol", C.c_double), ("maxiter", C.c_int32), ("precond", C.c_int32),
+                ("learn_steps", C.c_int32), ("train_iters", C.c_int32), ("lr", C.c_double)]
 
 
 class tf_stats(C.Structure):
     _fields_ = [("steps_done", C.c_int64), ("cg_iters_total", C.c_int64),
                 ("cg_iters_max", C.c_int32), ("cg_not_converged", C.c_int32),
-                ("cg_max_relres", C.c_double), ("seconds", C.c_double)]
+                ("cg_max_relres", C.c_double), ("seconds", C.c_double),
+                ("pc_state", C.c_int32), ("pc_pairs", C.c_int32), ("pc_fallbacks", C.c_int64),
+                ("pc_learned_solves", C.c_int64), ("pc_learned_iters", C.c_int64),
+                ("pc_jacobi_mean_iters", C.c_double), ("pc_train_seconds", C.c_double),
+                ("pc_loss0", C.c_double), ("pc_loss", C.c_double)]
+
+
+MLP_PARAMS = 5313
 
 
 _host = None
@@ -176,9 +184,12 @@ def hip_lib():
     L.tf_set_profiling.argtypes = [vp, C.c_int32]
     L.tf_device_bytes.argtypes = [vp, i64p]
     L.tf_destroy.argtypes = [vp]
+    f32p = C.POINTER(C.c_float)
+    L.tf_mlp_grad.argtypes = [f32p, f32p, f32p, C.c_int64, f32p, f32p]
     for n in ["tf_create", "tf_set_sources", "tf_set_probes", "tf_set_output", "tf_run",
               "tf_read_state", "tf_write_state", "tf_read_probes", "tf_read_poynting",
-              "tf_energy", "tf_kernel_times", "tf_set_profiling", "tf_device_bytes", "tf_destroy"]:
+              "tf_energy", "tf_kernel_times", "tf_set_profiling", "tf_device_bytes", "tf_destroy",
+              "tf_mlp_grad"]:
         getattr(L, n).restype = C.c_int32
     _hip = L
     return L

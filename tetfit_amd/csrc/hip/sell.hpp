@@ -151,12 +151,13 @@ __global__ void __launch_bounds__(kBlock) gf_rhs_kernel(
 
 // ---- CG (ConjugateGradientSolver.hpp:52-103) ------------------------------
 // r = b - A x, z = pre r, partials of r.r and r.z into part_b[0..nb), [nb..2nb)
+// With jacobi = false z is left to cg_c_kernel (z = G r) and the r.z partials are 0.
 template <typename T>
 __global__ void __launch_bounds__(kBlock) cg_init_kernel(int32_t n, int32_t nslices, const int64_t* __restrict__ sptr,
                                                          const int32_t* __restrict__ col, const T* __restrict__ val,
                                                          const T* __restrict__ pre, const T* __restrict__ b,
                                                          const T* __restrict__ x, T* __restrict__ r, T* __restrict__ z,
-                                                         double* __restrict__ part_b, CgState* st) {
+                                                         double* __restrict__ part_b, CgState* st, int jacobi) {
     __shared__ double red[kBlock / kWave];
     const int lane = lane_id();
     double rr = 0, rz = 0;
@@ -167,11 +168,13 @@ __global__ void __launch_bounds__(kBlock) cg_init_kernel(int32_t n, int32_t nsli
         const int32_t i = s * kSlice + lane;
         if (i < n) {
             const T ri = (T)((double)b[i] - ax);
-            const T zi = (T)((double)pre[i] * (double)ri);
             r[i] = ri;
-            z[i] = zi;
             rr += (double)ri * (double)ri;
-            rz += (double)ri * (double)zi;
+            if (jacobi) {
+                const T zi = (T)((double)pre[i] * (double)ri);
+                z[i] = zi;
+                rz += (double)ri * (double)zi;
+            }
         }
     }
     rr = block_sum(rr, red);
@@ -192,12 +195,13 @@ __global__ void __launch_bounds__(kBlock) cg_a_kernel(int32_t it, int32_t n, int
                                                       const int64_t* __restrict__ sptr, const int32_t* __restrict__ col,
                                                       const T* __restrict__ val, const T* __restrict__ z,
                                                       T* __restrict__ p0, T* __restrict__ p1, T* __restrict__ q,
-                                                      const double* __restrict__ part_b, int32_t nb_b,
+                                                      const double* __restrict__ part_rr, int32_t nb_rr,
+                                                      const double* __restrict__ part_rz, int32_t nb_rz,
                                                       double* __restrict__ part_a, CgState* st) {
     if (st->done) return;
     __shared__ double red[kBlock / kWave];
-    const double RR = sum_partials(part_b, nb_b, red);
-    const double RZ = sum_partials(part_b + nb_b, nb_b, red);
+    const double RR = sum_partials(part_rr, nb_rr, red);
+    const double RZ = sum_partials(part_rz, nb_rz, red);
     const double rr0 = it == 0 ? RR : st->rr0;
     const double ratio = std::sqrt(RR) / std::sqrt(rr0);
     const bool stop = !(ratio > st->tol && it < st->maxiter);
@@ -241,7 +245,8 @@ __global__ void __launch_bounds__(kBlock) cg_b_kernel(int32_t it, int32_t n, con
                                                       T* __restrict__ x, T* __restrict__ r, T* __restrict__ z,
                                                       const T* __restrict__ p0, const T* __restrict__ p1,
                                                       const T* __restrict__ q, const double* __restrict__ part_a,
-                                                      int32_t nb_a, double* __restrict__ part_b, CgState* st) {
+                                                      int32_t nb_a, double* __restrict__ part_b, CgState* st,
+                                                      int jacobi) {
     if (st->done) return;
     __shared__ double red[kBlock / kWave];
     const double PQ = sum_partials(part_a, nb_a, red);
@@ -252,10 +257,12 @@ __global__ void __launch_bounds__(kBlock) cg_b_kernel(int32_t it, int32_t n, con
         x[i] = (T)((double)x[i] + alpha * (double)pnew[i]);
         const T ri = (T)((double)r[i] - alpha * (double)q[i]);
         r[i] = ri;
-        const T zi = (T)((double)pre[i] * (double)ri);
-        z[i] = zi;
         rr += (double)ri * (double)ri;
-        rz += (double)ri * (double)zi;
+        if (jacobi) {
+            const T zi = (T)((double)pre[i] * (double)ri);
+            z[i] = zi;
+            rz += (double)ri * (double)zi;
+        }
     }
     rr = block_sum(rr, red);
     rz = block_sum(rz, red);
@@ -263,6 +270,32 @@ __global__ void __launch_bounds__(kBlock) cg_b_kernel(int32_t it, int32_t n, con
         part_b[blockIdx.x] = rr;
         part_b[gridDim.x + blockIdx.x] = rz;
     }
+}
+
+// z = G r with G the learned approximate inverse stored on A's SELL pattern
+// (same column array, its own values in fp32), partial r.z into part_c.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) cg_c_kernel(int32_t n, int32_t nslices, const int64_t* __restrict__ sptr,
+                                                      const int32_t* __restrict__ col, const float* __restrict__ gval,
+                                                      const T* __restrict__ r, T* __restrict__ z,
+                                                      double* __restrict__ part_c, const CgState* st) {
+    if (st->done) return;
+    __shared__ double red[kBlock / kWave];
+    const int lane = lane_id();
+    double rz = 0;
+    for (int s = wave_id(); s < nslices; s += wave_count()) {
+        const int64_t bs = sptr[s];
+        const int w = (int)((sptr[s + 1] - bs) / kSlice);
+        const double zv = sell_dot<float>(bs, w, col, gval, lane, [&](int32_t j) { return (double)r[j]; });
+        const int32_t i = s * kSlice + lane;
+        if (i < n) {
+            const T zi = (T)zv;
+            z[i] = zi;
+            rz += (double)r[i] * (double)zi;
+        }
+    }
+    rz = block_sum(rz, red);
+    if (threadIdx.x == 0) part_c[blockIdx.x] = rz;
 }
 
 }  // namespace tfd

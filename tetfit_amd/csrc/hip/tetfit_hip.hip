@@ -17,6 +17,8 @@
 
 #include "common.hpp"
 #include "kernels.hpp"
+#include "learned.hpp"
+#include "mlp.hpp"
 #include "sell.hpp"
 #include "tetfit_hip.h"
 
@@ -30,6 +32,7 @@ thread_local std::string g_err;
 // fp64 per block of the producing kernel.
 constexpr int kCgPartA = 2048;   // blocks of cg_a (SpMV with A)
 constexpr int kCgPartB = 1024;   // blocks of cg_init / cg_b
+constexpr int kCgPartC = 2048;   // blocks of cg_c (SpMV with the learned G)
 constexpr int kSpmvCap = 8192;   // blocks of the other SELL kernels
 
 // Carrier of Source::Compute in long double (Source.hpp:316-327); 0 for t<0.
@@ -75,6 +78,27 @@ struct Sell {
     int64_t nnz = 0, entries = 0;
 };
 
+// State of the learned preconditioner of a gfdtd context.
+struct LearnedPc {
+    enum { kOff = 0, kCollecting = 1, kActive = 2, kDisabled = 3 };
+    int state = kOff;
+    int64_t steps = 0;              // solves seen since set-up
+    int learn_steps = 200, train_iters = 150;
+    double lr = 3e-3;
+    int32_t n_pairs = 0, n_pad = 0;
+    DevBuf X, base, scale, pa, pb, pos2pair, y, ghat, gval, ahat, isd;
+    DevBuf P, Pbest, m, v, part, gy, gpos, lpart, cpart;
+    DevBuf Rb, Cb, T1, T2;
+    std::vector<int> slot;          // 0 free, 1 captured correction pair, 2 random probe
+    std::vector<int> jac_iters, pc_iters;
+    double train_seconds = 0, loss0 = 0, loss = 0;
+    int64_t fallbacks = 0, iters = 0, solves = 0;
+    std::vector<const DevBuf*> bufs() const {
+        return {&X, &base, &scale, &pa, &pb, &pos2pair, &y, &ghat, &gval, &ahat, &isd, &P, &Pbest,
+                &m, &v, &part, &gy, &gpos, &lpart, &cpart, &Rb, &Cb, &T1, &T2};
+    }
+};
+
 }  // namespace
 
 struct tf_ctx {
@@ -110,6 +134,13 @@ struct tf_ctx {
     int32_t n_inj = 0;
     CgState* h_state = nullptr;  // pinned
     int cg_guess = 30;           // iterations of the previous solve
+    int cg_guess_pc = 15;        // same, for solves with the learned preconditioner
+    // host copy of A (free rows, CSR) and its SELL slice pointers, kept for
+    // the learned preconditioner's pair list and features
+    std::vector<int64_t> hA_ptr, hA_sptr;
+    std::vector<int32_t> hA_col;
+    std::vector<double> hA_val;
+    LearnedPc lp;
 
     // sources
     std::vector<tf_source> src;
@@ -161,6 +192,7 @@ size_t tf_ctx::device_bytes() const {
     for (auto c : cs) s += c->ptr.bytes + c->col.bytes + c->val.bytes + c->sg.bytes;
     const Sell* ss[] = {&sCt, &H, &N, &A, &Ef, &R1f, &Ctf};
     for (auto c : ss) s += c->sptr.bytes + c->col.bytes + c->val.bytes + c->sg.bytes;
+    for (auto b : lp.bufs()) s += b->bytes;
     return s;
 }
 
@@ -189,18 +221,24 @@ void upload_csr(Csr32& d, const tf_csr& c, bool values) {
 
 // CSR -> SELL-64 (slices of 64 rows, column-major inside a slice, rows
 // padded with value 0 on their last column).
-template <class T>
-void upload_sell(Sell& d, const tf_csr& c, bool values) {
-    d.nrows = c.nrows;
-    d.nnz = c.nnz;
-    d.nslices = (c.nrows + kSlice - 1) / kSlice;
-    std::vector<int64_t> sptr(d.nslices + 1, 0);
-    for (int32_t s = 0; s < d.nslices; ++s) {
+std::vector<int64_t> sell_layout(const tf_csr& c) {
+    const int32_t nslices = (c.nrows + kSlice - 1) / kSlice;
+    std::vector<int64_t> sptr(nslices + 1, 0);
+    for (int32_t s = 0; s < nslices; ++s) {
         int64_t w = 0;
         for (int32_t r = s * kSlice; r < std::min(c.nrows, (s + 1) * kSlice); ++r)
             w = std::max<int64_t>(w, c.rowptr[r + 1] - c.rowptr[r]);
         sptr[s + 1] = sptr[s] + w * kSlice;
     }
+    return sptr;
+}
+
+template <class T>
+void upload_sell(Sell& d, const tf_csr& c, bool values) {
+    d.nrows = c.nrows;
+    d.nnz = c.nnz;
+    d.nslices = (c.nrows + kSlice - 1) / kSlice;
+    const std::vector<int64_t> sptr = sell_layout(c);
     d.entries = sptr[d.nslices];
     std::vector<int32_t> col(d.entries, 0);
     std::vector<T> val(values ? d.entries : 0, (T)0);
@@ -382,6 +420,10 @@ void create_tet(tf_ctx* c, const tf_problem* p) {
         upload(c->h_edge, he);
     } else {
         upload_sell<T>(c->A, d.A, true);
+        c->hA_ptr.assign(d.A.rowptr, d.A.rowptr + d.A.nrows + 1);
+        c->hA_col.assign(d.A.col, d.A.col + d.A.nnz);
+        c->hA_val.assign(d.A.val, d.A.val + d.A.nnz);
+        c->hA_sptr = sell_layout(d.A);
         // free rows of E, RHSmat1 and Ct for the right-hand side
         {
             auto free_rows = [&](const tf_csr& m, std::vector<int64_t>& rp, std::vector<int32_t>& col,
@@ -442,7 +484,7 @@ void create_tet(tf_ctx* c, const tf_problem* p) {
         upload(c->pre, pre);
         const int64_t n = std::max<int32_t>(d.n_free, 1);
         for (DevBuf* b : {&c->x, &c->r, &c->z, &c->p0, &c->p1, &c->q, &c->rhs}) zero_field<T>(*b, n, c->stream);
-        c->cg_part.alloc(sizeof(double) * (kCgPartA + 2 * kCgPartB));
+        c->cg_part.alloc(sizeof(double) * (kCgPartA + 2 * kCgPartB + kCgPartC));
         c->cg_state.alloc(sizeof(CgState));
         TF_HIP(hipMemsetAsync(c->cg_state.p, 0, sizeof(CgState), c->stream));
         TF_HIP(hipHostMalloc((void**)&c->h_state, sizeof(CgState), hipHostMallocDefault));
@@ -728,23 +770,47 @@ void run_dga(tf_ctx* c, int64_t first, int64_t n, tf_stats* st) {
 }
 
 // One CG solve on the device: cg_init, then iterations it = 0, 1, ... as
-// (cg_a, cg_b) pairs. Launches are issued in batches sized from the previous
-// solve's count; kernels past the stop test return at once. Returns the
-// final state (read back after the solve).
+// (cg_a, cg_b) pairs, or (cg_a, cg_b, cg_c) with the learned preconditioner
+// (cg_c applies z = G r; cg_b then leaves z alone). Launches are issued in
+// batches sized from the previous solve's count; kernels past the stop test
+// return at once. `snaps` lists (iteration k, bank slot) captures of the
+// iterate (r_k, x_k), taken in stream order after the update that produces
+// it. The final state is read back into h_state.
 template <class T>
-void cg_solve(tf_ctx* c, const CgState& init, int& guess) {
+void cg_solve(tf_ctx* c, const CgState& init, bool learned, const std::vector<std::pair<int, int>>* snaps) {
     const int32_t nf = c->n_free;
+    int& guess = learned ? c->cg_guess_pc : c->cg_guess;
     const int nb_a = sell_blocks(c->A, kCgPartA);
     const int nb_b = std::max(1, std::min(blocks_for(nf), kCgPartB));
+    const int nb_c = sell_blocks(c->A, kCgPartC);
     double* part_a = c->cg_part.as<double>();
     double* part_b = part_a + kCgPartA;
+    double* part_c = part_b + 2 * kCgPartB;
+    const double* part_rz = learned ? part_c : part_b + nb_b;
+    const int nb_rz = learned ? nb_c : nb_b;
+    const int jac = learned ? 0 : 1;
     CgState* dst = c->cg_state.as<CgState>();
+    auto capture = [&](int k) {
+        if (!snaps) return;
+        for (auto& sn : *snaps)
+            if (sn.first == k)
+                bank_capture_kernel<T><<<blocks_for(nf), kBlock, 0, c->stream>>>(
+                    nf, sn.second, c->r.as<T>(), c->x.as<T>(), c->lp.Rb.as<float>(), c->lp.Cb.as<float>());
+    };
+    auto apply_g = [&] {
+        ev_begin(c, "cg_c_kernel");
+        cg_c_kernel<T><<<nb_c, kBlock, 0, c->stream>>>(nf, c->A.nslices, c->A.sptr.as<int64_t>(), c->A.col.as<int32_t>(),
+                                                       c->lp.gval.as<float>(), c->r.as<T>(), c->z.as<T>(), part_c, dst);
+        ev_end(c);
+    };
     TF_HIP(hipMemcpyAsync(dst, &init, sizeof(CgState), hipMemcpyHostToDevice, c->stream));
     ev_begin(c, "cg_init_kernel");
     cg_init_kernel<T><<<nb_b, kBlock, 0, c->stream>>>(nf, c->A.nslices, c->A.sptr.as<int64_t>(), c->A.col.as<int32_t>(),
                                                       c->A.val.as<T>(), c->pre.as<T>(), c->rhs.as<T>(), c->x.as<T>(),
-                                                      c->r.as<T>(), c->z.as<T>(), part_b, dst);
+                                                      c->r.as<T>(), c->z.as<T>(), part_b, dst, jac);
     ev_end(c);
+    capture(0);
+    if (learned) apply_g();
     int it = 0;
     int batch = std::max(1, guess + 1);
     for (;;) {
@@ -754,14 +820,16 @@ void cg_solve(tf_ctx* c, const CgState& init, int& guess) {
             cg_a_kernel<T><<<nb_a, kBlock, 0, c->stream>>>(it, nf, c->A.nslices, c->A.sptr.as<int64_t>(),
                                                            c->A.col.as<int32_t>(), c->A.val.as<T>(), c->z.as<T>(),
                                                            c->p0.as<T>(), c->p1.as<T>(), c->q.as<T>(), part_b, nb_b,
-                                                           part_a, dst);
+                                                           part_rz, nb_rz, part_a, dst);
             ev_end(c);
             if (it == init.maxiter) break;  // cg_a(maxiter) only runs the stop test
             ev_begin(c, "cg_b_kernel");
             cg_b_kernel<T><<<nb_b, kBlock, 0, c->stream>>>(it, nf, c->pre.as<T>(), c->x.as<T>(), c->r.as<T>(),
                                                            c->z.as<T>(), c->p0.as<T>(), c->p1.as<T>(), c->q.as<T>(),
-                                                           part_a, nb_a, part_b, dst);
+                                                           part_a, nb_a, part_b, dst, jac);
             ev_end(c);
+            capture(it + 1);
+            if (learned) apply_g();
         }
         TF_HIP(hipMemcpyAsync(c->h_state, dst, sizeof(CgState), hipMemcpyDeviceToHost, c->stream));
         TF_HIP(hipStreamSynchronize(c->stream));
@@ -769,6 +837,337 @@ void cg_solve(tf_ctx* c, const CgState& init, int& guess) {
         batch = 4;
     }
     guess = c->h_state->iter;
+}
+
+// ------------------------------------------------- learned preconditioner
+//
+// G is a symmetric sparse approximate inverse of A with A's pattern:
+//   G_ej = (base_p + y_p) / sqrt(A_ee A_jj),  p = pair {e, j},
+//   base_p = 1 (e == j), -Â_ej (e != j)   (y = 0: G = D^-1/2 (2I - Â) D^-1/2),
+// y_p = MLP(features_p), features symmetric in (e, j). The network is trained
+// on the device from correction pairs captured during the first Jacobi-PCG
+// solves of the run, then frozen: G is a fixed linear operator for CG.
+
+uint16_t bf16_bits(float f) {
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+constexpr int kCgSlots = 48;                      // bank slots for captured pairs
+constexpr int kCaptureIters[4] = {0, 3, 7, 12};   // CG iterations captured per chosen step
+
+// Pair list, features and parameter set-up (host, once per context).
+void learned_setup(tf_ctx* c, const tf_cg_opts* cg) {
+    LearnedPc& L = c->lp;
+    const int32_t n = c->n_free;
+    if (n <= 0) throw std::runtime_error("learned preconditioner needs free edges");
+    const auto& rp = c->hA_ptr;
+    const auto& col = c->hA_col;
+    const auto& val = c->hA_val;
+    std::vector<double> d(n, 0.0);
+    for (int32_t e = 0; e < n; ++e) {
+        for (int64_t q = rp[e]; q < rp[e + 1]; ++q) {
+            if (q > rp[e] && col[q] <= col[q - 1]) throw std::runtime_error("A rows must have sorted columns");
+            if (col[q] == e) d[e] = val[q];
+        }
+        if (!(d[e] > 0)) throw std::runtime_error("A has a non-positive diagonal entry");
+    }
+    std::vector<double> is(n);
+    for (int32_t e = 0; e < n; ++e) is[e] = 1.0 / std::sqrt(d[e]);
+    auto find = [&](int32_t row, int32_t cl) -> int64_t {
+        const int32_t* b = col.data() + rp[row];
+        const int32_t* en = col.data() + rp[row + 1];
+        const int32_t* it = std::lower_bound(b, en, cl);
+        if (it == en || *it != cl) throw std::runtime_error("A pattern is not symmetric");
+        return it - col.data();
+    };
+    auto sell_pos = [&](int32_t row, int64_t q) {
+        return c->hA_sptr[row / kSlice] + (q - rp[row]) * kSlice + row % kSlice;
+    };
+    // pairs: upper triangle incl. diagonal, in CSR order
+    std::vector<int32_t> pair_of(col.size(), -1), pe, pj;
+    for (int32_t e = 0; e < n; ++e)
+        for (int64_t q = rp[e]; q < rp[e + 1]; ++q)
+            if (col[q] >= e) {
+                pair_of[q] = (int32_t)pe.size();
+                pe.push_back(e);
+                pj.push_back(col[q]);
+            }
+    for (int32_t e = 0; e < n; ++e)
+        for (int64_t q = rp[e]; q < rp[e + 1]; ++q)
+            if (col[q] < e) pair_of[q] = pair_of[find(col[q], e)];
+    const int32_t np = (int32_t)pe.size();
+    L.n_pairs = np;
+    L.n_pad = (np + 15) / 16 * 16;
+    // row statistics of Â (off-diagonal part)
+    std::vector<double> rho(n, 0), sig(n, 0), ssum(n, 0), nnz(n, 0), ld(n, 0);
+    double ldm = 0, nm = 0;
+    for (int32_t e = 0; e < n; ++e) {
+        for (int64_t q = rp[e]; q < rp[e + 1]; ++q) {
+            if (col[q] == e) continue;
+            const double a = val[q] * is[e] * is[col[q]];
+            rho[e] += std::fabs(a);
+            sig[e] += a * a;
+            ssum[e] += a;
+        }
+        nnz[e] = (double)(rp[e + 1] - rp[e]);
+        ld[e] = std::log(d[e]);
+        ldm += ld[e];
+        nm += nnz[e];
+    }
+    ldm /= n;
+    nm /= n;
+    for (int32_t e = 0; e < n; ++e) { ld[e] -= ldm; nnz[e] /= nm; }
+    // features (16 per pair, symmetric in e, j), then standardised
+    std::vector<float> F((size_t)np * kMlpIn);
+    std::vector<float> base(np), scale(np);
+    std::vector<int64_t> pa(np), pb(np);
+    for (int32_t p = 0; p < np; ++p) {
+        const int32_t e = pe[p], j = pj[p];
+        const bool dg = e == j;
+        const int64_t qe = find(e, j);
+        const double ah = val[qe] * is[e] * is[j];
+        // (Â^2)_ej: merge of rows e and j
+        double a2 = 0;
+        for (int64_t x = rp[e], y = rp[j]; x < rp[e + 1] && y < rp[j + 1];) {
+            if (col[x] < col[y]) ++x;
+            else if (col[y] < col[x]) ++y;
+            else {
+                a2 += val[x] * is[e] * is[col[x]] * val[y] * is[j] * is[col[y]];
+                ++x;
+                ++y;
+            }
+        }
+        const double id = dg ? 1.0 : 0.0;
+        float* f = &F[(size_t)p * kMlpIn];
+        const double mean2[5][2] = {{0.5 * (ld[e] + ld[j]), std::fabs(ld[e] - ld[j])},
+                                    {0.5 * (rho[e] + rho[j]), std::fabs(rho[e] - rho[j])},
+                                    {0.5 * (sig[e] + sig[j]), std::fabs(sig[e] - sig[j])},
+                                    {0.5 * (ssum[e] + ssum[j]), std::fabs(ssum[e] - ssum[j])},
+                                    {0.5 * (nnz[e] + nnz[j]), std::fabs(nnz[e] - nnz[j])}};
+        f[0] = (float)id;
+        f[1] = (float)(dg ? 0.0 : ah);
+        f[2] = (float)a2;
+        f[3] = (float)(dg ? 0.0 : std::fabs(ah));
+        for (int k = 0; k < 5; ++k) { f[4 + 2 * k] = (float)mean2[k][0]; f[5 + 2 * k] = (float)mean2[k][1]; }
+        f[14] = (float)(id * a2);
+        f[15] = (float)((1 - id) * a2);
+        base[p] = (float)(dg ? 1.0 : -ah);
+        scale[p] = (float)(is[e] * is[j]);
+        pa[p] = sell_pos(e, qe);
+        pb[p] = dg ? -1 : sell_pos(j, find(j, e));
+    }
+    std::vector<uint16_t> X((size_t)L.n_pad * kMlpIn, 0);
+    for (int k = 0; k < kMlpIn; ++k) {
+        double mu = 0, m2 = 0;
+        for (int32_t p = 0; p < np; ++p) mu += F[(size_t)p * kMlpIn + k];
+        mu /= np;
+        for (int32_t p = 0; p < np; ++p) {
+            const double t = F[(size_t)p * kMlpIn + k] - mu;
+            m2 += t * t;
+        }
+        double sd = std::sqrt(m2 / np);
+        if (!(sd > 1e-12)) sd = 1;
+        for (int32_t p = 0; p < np; ++p)
+            X[(size_t)p * kMlpIn + k] = bf16_bits((float)((F[(size_t)p * kMlpIn + k] - mu) / sd));
+    }
+    // SELL position -> pair, and Â on the SELL pattern
+    const int64_t entries = c->A.entries;
+    std::vector<int32_t> pos2pair(entries, -1);
+    std::vector<float> ahat(entries, 0.f);
+    for (int32_t e = 0; e < n; ++e)
+        for (int64_t q = rp[e]; q < rp[e + 1]; ++q) {
+            const int64_t o = sell_pos(e, q);
+            pos2pair[o] = pair_of[q];
+            ahat[o] = (float)(val[q] * is[e] * is[col[q]]);
+        }
+    std::vector<float> isd(n);
+    for (int32_t e = 0; e < n; ++e) isd[e] = (float)is[e];
+    upload(L.X, X);
+    upload(L.base, base);
+    upload(L.scale, scale);
+    upload(L.pa, pa);
+    upload(L.pb, pb);
+    upload(L.pos2pair, pos2pair);
+    upload(L.ahat, ahat);
+    upload(L.isd, isd);
+    L.y.alloc((size_t)L.n_pad * 4);
+    TF_HIP(hipMemsetAsync(L.y.p, 0, L.y.bytes, c->stream));
+    L.gy.alloc((size_t)L.n_pad * 4);
+    TF_HIP(hipMemsetAsync(L.gy.p, 0, L.gy.bytes, c->stream));
+    L.ghat.alloc(entries * 4);
+    L.gval.alloc(entries * 4);
+    L.gpos.alloc(entries * 4);
+    // parameters: He-uniform W1, W2 from a fixed LCG, zero biases, zero output layer
+    std::vector<float> P(kMlpParams, 0.f);
+    uint64_t st = 0x243F6A8885A308D3ULL;
+    auto unif = [&] {
+        st = st * 6364136223846793005ULL + 1442695040888963407ULL;
+        return (double)(st >> 11) * (1.0 / 9007199254740992.0);
+    };
+    for (int k = 0; k < kMlpHid * kMlpIn; ++k) P[kW1 + k] = (float)((2 * unif() - 1) * std::sqrt(6.0 / kMlpIn));
+    for (int k = 0; k < kMlpHid * kMlpHid; ++k) P[kW2 + k] = (float)((2 * unif() - 1) * std::sqrt(6.0 / kMlpHid));
+    upload(L.P, P);
+    upload(L.Pbest, P);
+    std::vector<float> zeros(kMlpParams, 0.f);
+    upload(L.m, zeros);
+    upload(L.v, zeros);
+    const int nbb = 512;
+    L.part.alloc((size_t)nbb * kMlpParams * 4);
+    L.lpart.alloc(kSpmvCap * sizeof(double));
+    L.cpart.alloc((size_t)kSpmvCap * kBank * sizeof(double));
+    const size_t bank = (size_t)n * kBank * 4;
+    for (DevBuf* b : {&L.Rb, &L.Cb, &L.T1, &L.T2}) {
+        b->alloc(bank);
+        TF_HIP(hipMemsetAsync(b->p, 0, bank, c->stream));
+    }
+    L.slot.assign(kBank, 0);
+    L.learn_steps = cg->learn_steps > 0 ? cg->learn_steps : 200;
+    L.train_iters = cg->train_iters > 0 ? cg->train_iters : 150;
+    L.lr = cg->lr > 0 ? cg->lr : 3e-3;
+    L.state = LearnedPc::kCollecting;
+    L.steps = 0;
+    TF_HIP(hipStreamSynchronize(c->stream));
+}
+
+int row_blocks(int32_t n) { return std::max(1, std::min(kSpmvCap, (n + kBlock / kWave - 1) / (kBlock / kWave))); }
+
+// Trains the MLP on the bank (Adam, train_iters steps), keeps the parameters
+// of the lowest loss and writes G's values for CG.
+void learned_train(tf_ctx* c) {
+    LearnedPc& L = c->lp;
+    const int32_t n = c->n_free;
+    const auto t0 = std::chrono::steady_clock::now();
+    hipStream_t s = c->stream;
+    const int64_t* sptr = c->A.sptr.as<int64_t>();
+    const int32_t* col = c->A.col.as<int32_t>();
+    const int rb = row_blocks(n);
+    const int64_t nb_elems = (int64_t)n * kBank;
+    // free slots become random probes: c~ = unit-variance noise, r~ = Â c~
+    uint64_t mask = 0;
+    for (int k = 0; k < kBank; ++k)
+        if (L.slot[k] == 0) { mask |= 1ULL << k; L.slot[k] = 2; }
+    const int eb = (int)std::min<int64_t>((nb_elems + kBlock - 1) / kBlock, INT32_MAX);
+    if (mask) bank_random_kernel<<<eb, kBlock, 0, s>>>(n, mask, 0x5EEDULL, L.Cb.as<float>());
+    bank_spmm_kernel<<<rb, kBlock, 0, s>>>(n, sptr, col, L.ahat.as<float>(), L.Cb.as<float>(), nullptr,
+                                           L.T1.as<float>());
+    if (mask) bank_copy_cols_kernel<<<eb, kBlock, 0, s>>>(n, mask, L.T1.as<float>(), L.Rb.as<float>());
+    // weights: w_s = 1 / (S c~_s^T Â c~_s)
+    bank_coldot_kernel<<<rb, kBlock, 0, s>>>(n, L.Cb.as<float>(), L.T1.as<float>(), L.cpart.as<double>());
+    std::vector<double> cp((size_t)rb * kBank);
+    TF_HIP(hipMemcpyAsync(cp.data(), L.cpart.p, cp.size() * 8, hipMemcpyDeviceToHost, s));
+    TF_HIP(hipStreamSynchronize(s));
+    std::vector<float> w(kBank, 0.f);
+    for (int k = 0; k < kBank; ++k) {
+        double t = 0;
+        for (int b = 0; b < rb; ++b) t += cp[(size_t)b * kBank + k];
+        w[k] = (t > 0 && std::isfinite(t)) ? (float)(1.0 / (kBank * t)) : 0.f;
+    }
+    DevBuf wts;
+    upload(wts, w);
+    const int64_t tiles = L.n_pad / 16;
+    const int nbb = (int)std::min<int64_t>(512, std::max<int64_t>(1, (tiles + 3) / 4));
+    const int fb = (int)std::min<int64_t>(4096, std::max<int64_t>(1, (tiles + 3) / 4));
+    const int64_t entries = c->A.entries;
+    const int xb = (int)((entries + kBlock - 1) / kBlock);
+    const int pb = blocks_for(L.n_pairs);
+    std::vector<double> lsum(rb);
+    double best = INFINITY;
+    for (int it = 0; it <= L.train_iters; ++it) {
+        mlp_forward_kernel<<<fb, kBlock, 0, s>>>(tiles, L.P.as<float>(), L.X.as<short>(), L.y.as<float>());
+        expand_kernel<<<xb, kBlock, 0, s>>>(entries, L.pos2pair.as<int32_t>(), L.base.as<float>(),
+                                            L.scale.as<float>(), L.y.as<float>(), L.ghat.as<float>(), nullptr);
+        bank_spmm_kernel<<<rb, kBlock, 0, s>>>(n, sptr, col, L.ghat.as<float>(), L.Rb.as<float>(),
+                                               L.Cb.as<float>(), L.T1.as<float>());
+        bank_spmm_kernel<<<rb, kBlock, 0, s>>>(n, sptr, col, L.ahat.as<float>(), L.T1.as<float>(), nullptr,
+                                               L.T2.as<float>());
+        bank_grad_kernel<<<rb, kBlock, 0, s>>>(n, sptr, col, L.Rb.as<float>(), L.T1.as<float>(), L.T2.as<float>(),
+                                               wts.as<float>(), 2.f, L.gpos.as<float>(), L.lpart.as<double>());
+        TF_HIP(hipMemcpyAsync(lsum.data(), L.lpart.p, rb * 8, hipMemcpyDeviceToHost, s));
+        TF_HIP(hipStreamSynchronize(s));
+        double loss = 0;
+        for (double v : lsum) loss += v;
+        if (it == 0) L.loss0 = loss;
+        if (std::isfinite(loss) && loss < best) {
+            best = loss;
+            TF_HIP(hipMemcpyAsync(L.Pbest.p, L.P.p, kMlpParams * 4, hipMemcpyDeviceToDevice, s));
+        }
+        if (it == L.train_iters) break;
+        pair_grad_kernel<<<pb, kBlock, 0, s>>>(L.n_pairs, L.pa.as<int64_t>(), L.pb.as<int64_t>(),
+                                               L.gpos.as<float>(), L.gy.as<float>());
+        mlp_backward_kernel<<<nbb, kBlock, 0, s>>>(tiles, L.P.as<float>(), L.X.as<short>(), L.gy.as<float>(),
+                                                   L.part.as<float>());
+        mlp_adam_kernel<<<blocks_for(kMlpParams), kBlock, 0, s>>>(nbb, L.part.as<float>(), L.P.as<float>(),
+                                                                  L.m.as<float>(), L.v.as<float>(), nullptr,
+                                                                  (float)L.lr, 0.9f, 0.999f, 1e-8f, it + 1, 1.f);
+    }
+    L.loss = best;
+    // frozen G from the best parameters
+    mlp_forward_kernel<<<fb, kBlock, 0, s>>>(tiles, L.Pbest.as<float>(), L.X.as<short>(), L.y.as<float>());
+    expand_kernel<<<xb, kBlock, 0, s>>>(entries, L.pos2pair.as<int32_t>(), L.base.as<float>(), L.scale.as<float>(),
+                                        L.y.as<float>(), nullptr, L.gval.as<float>());
+    TF_HIP(hipGetLastError());
+    TF_HIP(hipStreamSynchronize(s));
+    L.train_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    L.state = LearnedPc::kActive;
+}
+
+// The CG solve of one gfdtd step under the requested preconditioner, with
+// capture during collection, and the Jacobi redo when a learned solve does
+// not reach the tolerance.
+template <class T>
+void gfdtd_solve(tf_ctx* c, const CgState& init, bool want_learned, tf_stats* st) {
+    LearnedPc& L = c->lp;
+    const int32_t nf = c->n_free;
+    if (!want_learned || L.state == LearnedPc::kDisabled) {
+        cg_solve<T>(c, init, false, nullptr);
+        return;
+    }
+    if (L.state == LearnedPc::kCollecting) {
+        std::vector<std::pair<int, int>> snaps;
+        const int stride = std::max(1, L.learn_steps / (kCgSlots / 4));
+        if (L.steps % stride == stride - 1) {
+            int k = 0;
+            for (int sl = 0; sl < kCgSlots && k < 4; ++sl)
+                if (L.slot[sl] == 0) snaps.push_back({kCaptureIters[k++], sl});
+            if (k < 4) snaps.clear();
+        }
+        cg_solve<T>(c, init, false, snaps.empty() ? nullptr : &snaps);
+        const int iters = c->h_state->iter;
+        L.jac_iters.push_back(iters);
+        if (std::isfinite(c->h_state->relres) && c->h_state->relres <= init.tol)
+            for (auto& sn : snaps)
+                if (sn.first < iters) {
+                    bank_finalize_kernel<T><<<blocks_for(nf), kBlock, 0, c->stream>>>(
+                        nf, sn.second, c->x.as<T>(), L.isd.as<float>(), L.Rb.as<float>(), L.Cb.as<float>());
+                    L.slot[sn.second] = 1;
+                }
+        if (++L.steps >= L.learn_steps) learned_train(c);
+        return;
+    }
+    // active
+    cg_solve<T>(c, init, true, nullptr);
+    const bool ok = std::isfinite(c->h_state->relres) && c->h_state->relres <= init.tol;
+    L.solves++;
+    L.iters += c->h_state->iter;
+    L.pc_iters.push_back(c->h_state->iter);
+    if (!ok) {
+        // Jacobi-PCG redo from the same warm start (U still holds x_old on free edges)
+        st->cg_iters_total += c->h_state->iter;
+        gather_free_kernel<T><<<blocks_for(nf), kBlock, 0, c->stream>>>(nf, c->free_edge.as<int32_t>(),
+                                                                        c->U.as<T>(), c->x.as<T>());
+        cg_solve<T>(c, init, false, nullptr);
+        if (++L.fallbacks > 3) L.state = LearnedPc::kDisabled;
+    }
+    // plain CG judges the gain: after 8 learned solves, no fewer iterations
+    // than the collection's Jacobi solves disables the learned operator
+    if (L.state == LearnedPc::kActive && L.pc_iters.size() == 8 && L.jac_iters.size() >= 8) {
+        double a = 0, b = 0;
+        for (int k = 0; k < 8; ++k) { a += L.pc_iters[k]; b += L.jac_iters[L.jac_iters.size() - 8 + k]; }
+        if (a >= b) L.state = LearnedPc::kDisabled;
+    }
 }
 
 template <class T>
@@ -783,6 +1182,8 @@ void run_gfdtd(tf_ctx* c, int64_t first, int64_t n, const tf_cg_opts* cg, tf_sta
     CgState init{};
     init.tol = cg ? cg->tol : 1e-8;
     init.maxiter = cg ? cg->maxiter : 100;
+    const bool learned = cg && cg->precond == TF_PRECOND_LEARNED;
+    if (learned && c->lp.state == LearnedPc::kOff) learned_setup(c, cg);
     const int gb_n = sell_blocks(c->N, kSpmvCap);
     for (int64_t k = 0; k < n; ++k) {
         const int64_t i = first + k;
@@ -815,7 +1216,7 @@ void run_gfdtd(tf_ctx* c, int64_t first, int64_t n, const tf_cg_opts* cg, tf_sta
             c->Ctf.sptr.as<int64_t>(), c->Ctf.col.as<int32_t>(), c->Ctf.sg.as<int8_t>(), c->U.as<T>(),
             c->dir_new.as<T>(), c->Fv.as<T>(), c->fb.as<T>(), c->rhs.as<T>(), c->dt);
         ev_end(c);
-        cg_solve<T>(c, init, c->cg_guess);
+        gfdtd_solve<T>(c, init, learned, st);
         st->cg_iters_total += c->h_state->iter;
         st->cg_iters_max = std::max(st->cg_iters_max, c->h_state->iter);
         const double rel = c->h_state->relres;
@@ -840,6 +1241,20 @@ void run_gfdtd(tf_ctx* c, int64_t first, int64_t n, const tf_cg_opts* cg, tf_sta
         c->poynting.push_back(s);
     }
     st->steps_done += n;
+    const LearnedPc& L = c->lp;
+    st->pc_state = L.state;
+    st->pc_fallbacks = L.fallbacks;
+    st->pc_learned_solves = L.solves;
+    st->pc_learned_iters = L.iters;
+    st->pc_train_seconds = L.train_seconds;
+    st->pc_loss0 = L.loss0;
+    st->pc_loss = L.loss;
+    int np = 0;
+    for (int v : L.slot) np += v == 1;
+    st->pc_pairs = np;
+    double jm = 0;
+    for (int v : L.jac_iters) jm += v;
+    st->pc_jacobi_mean_iters = L.jac_iters.empty() ? 0.0 : jm / L.jac_iters.size();
 }
 
 template <class F>
@@ -990,8 +1405,9 @@ tf_status tf_run(tf_ctx* c, int64_t first, int64_t n, const tf_cg_opts* cg, tf_s
     *st = tf_stats{};
     return guarded([&] {
         if (n == 0) return;
-        if (c->method == TF_METHOD_GFDTD && cg && cg->precond != TF_PRECOND_JACOBI)
-            throw std::runtime_error("learned preconditioner not trained for this context");
+        if (cg && cg->precond != TF_PRECOND_JACOBI && cg->precond != TF_PRECOND_LEARNED)
+            throw std::runtime_error("unknown preconditioner");
+        if (cg && (cg->maxiter < 0 || !(cg->tol >= 0))) throw std::runtime_error("bad CG options");
         ensure_carr(c, first, n);
         c->ktimes.clear();
         hipEvent_t e0, e1;
@@ -1120,6 +1536,42 @@ tf_status tf_device_bytes(tf_ctx* c, int64_t* bytes) {
 tf_status tf_destroy(tf_ctx* c) {
     delete c;
     return TF_OK;
+}
+
+tf_status tf_mlp_grad(const float* params, const float* x, const float* gy, int64_t n, float* y, float* grad) {
+    if (!params || n < 0 || (n && (!x || !gy))) { g_err = "bad argument"; return TF_ERR_ARG; }
+    static_assert(kMlpParams == TF_MLP_PARAMS && kMlpIn == TF_MLP_IN && kMlpHid == TF_MLP_HIDDEN,
+                  "MLP layout mismatch");
+    return guarded([&] {
+        int ndev = 0;
+        TF_HIP(hipGetDeviceCount(&ndev));
+        if (ndev < 1) throw HipError("no HIP device", TF_ERR_HIP);
+        TF_HIP(hipSetDevice(0));
+        const int64_t npad = std::max<int64_t>(16, (n + 15) / 16 * 16);
+        std::vector<uint16_t> xb((size_t)npad * kMlpIn, 0);
+        for (int64_t k = 0; k < n * kMlpIn; ++k) xb[k] = bf16_bits(x[k]);
+        std::vector<float> g(npad, 0.f);
+        std::copy(gy, gy + n, g.begin());
+        std::vector<float> P(params, params + kMlpParams);
+        DevBuf dP, dX, dG, dY, dPart, dGrad;
+        upload(dP, P);
+        upload(dX, xb);
+        upload(dG, g);
+        dY.alloc(npad * 4);
+        const int64_t tiles = npad / 16;
+        const int nbb = (int)std::min<int64_t>(512, (tiles + 3) / 4);
+        dPart.alloc((size_t)nbb * kMlpParams * 4);
+        dGrad.alloc(kMlpParams * 4);
+        mlp_forward_kernel<<<(int)std::min<int64_t>(4096, (tiles + 3) / 4), kBlock>>>(tiles, dP.as<float>(),
+                                                                                     dX.as<short>(), dY.as<float>());
+        mlp_backward_kernel<<<nbb, kBlock>>>(tiles, dP.as<float>(), dX.as<short>(), dG.as<float>(), dPart.as<float>());
+        mlp_adam_kernel<<<blocks_for(kMlpParams), kBlock>>>(nbb, dPart.as<float>(), dP.as<float>(), nullptr, nullptr,
+                                                            dGrad.as<float>(), 0.f, 0.9f, 0.999f, 1e-8f, 1, 1.f);
+        TF_HIP(hipGetLastError());
+        TF_HIP(hipDeviceSynchronize());
+        if (y && n) TF_HIP(hipMemcpy(y, dY.p, n * 4, hipMemcpyDeviceToHost));
+        if (grad) TF_HIP(hipMemcpy(grad, dGrad.p, kMlpParams * 4, hipMemcpyDeviceToHost));
+    });
 }
 
 }  // extern "C"

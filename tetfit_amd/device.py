@@ -52,16 +52,27 @@ class Simulation:
     def __del__(self):
         self.close()
 
-    def run(self, n=None, first=None, cg_tol=None, cg_maxiter=100, precond="jacobi"):
+    def run(self, n=None, first=None, cg_tol=None, cg_maxiter=100, precond="jacobi", learn_steps=0,
+            train_iters=0, lr=0.0):
+        """Advances n steps from `first` (default: where the last run ended).
+        precond "learned" trains the preconditioner during the context's first
+        `learn_steps` solves (0: 200) and uses it afterwards."""
         L = hip_lib()
         first = self.next_step if first is None else first
         n = self.build.n_steps - (first - self.build.first_step) if n is None else n
-        opts = _ffi.tf_cg_opts(self.build.cg_tol if cg_tol is None else cg_tol, int(cg_maxiter), PRECOND[precond])
+        opts = _ffi.tf_cg_opts(self.build.cg_tol if cg_tol is None else cg_tol, int(cg_maxiter), PRECOND[precond],
+                               int(learn_steps), int(train_iters), float(lr))
         st = _ffi.tf_stats()
         _check(L.tf_run(self._h, int(first), int(n), C.byref(opts), C.byref(st)))
         self.next_step = first + n
-        return dict(steps=st.steps_done, cg_iters_total=st.cg_iters_total, cg_iters_max=st.cg_iters_max,
-                    cg_not_converged=st.cg_not_converged, cg_max_relres=st.cg_max_relres, seconds=st.seconds)
+        out = dict(steps=st.steps_done, cg_iters_total=st.cg_iters_total, cg_iters_max=st.cg_iters_max,
+                   cg_not_converged=st.cg_not_converged, cg_max_relres=st.cg_max_relres, seconds=st.seconds)
+        if precond == "learned":
+            out.update(pc_state=st.pc_state, pc_pairs=st.pc_pairs, pc_fallbacks=st.pc_fallbacks,
+                       pc_learned_solves=st.pc_learned_solves, pc_learned_iters=st.pc_learned_iters,
+                       pc_jacobi_mean_iters=st.pc_jacobi_mean_iters, pc_train_seconds=st.pc_train_seconds,
+                       pc_loss0=st.pc_loss0, pc_loss=st.pc_loss)
+        return out
 
     def state(self):
         U = np.zeros(self.build.n_edges)
@@ -114,3 +125,19 @@ class Simulation:
         b = C.c_int64()
         _check(hip_lib().tf_device_bytes(self._h, C.byref(b)))
         return b.value
+
+
+def mlp_grad(params, x, gy):
+    """Device forward + backward of the preconditioner MLP (tf_mlp_grad):
+    returns (y, grad) for feature rows x [n, 16], output gradients gy [n]."""
+    f32 = C.POINTER(C.c_float)
+    P = np.ascontiguousarray(params, dtype=np.float32)
+    X = np.ascontiguousarray(x, dtype=np.float32)
+    G = np.ascontiguousarray(gy, dtype=np.float32)
+    n = len(G)
+    assert P.size == _ffi.MLP_PARAMS and X.shape == (n, 16)
+    y = np.zeros(n, np.float32)
+    g = np.zeros(_ffi.MLP_PARAMS, np.float32)
+    _check(hip_lib().tf_mlp_grad(P.ctypes.data_as(f32), X.ctypes.data_as(f32), G.ctypes.data_as(f32), n,
+                                 y.ctypes.data_as(f32), g.ctypes.data_as(f32)))
+    return y, g
