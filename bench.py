@@ -54,11 +54,13 @@ def algorithmic_bytes(b, kernel, vb):
     """Bytes a kernel must move per launch (operator arrays once, vectors
     once per access pattern), vb = value bytes."""
     d = b.dga_ops()
-    if kernel == "cg_a_kernel":
+    if kernel in ("cg_w_kernel", "cg_g_kernel"):
         A = d["A"]
         n, nnz = A["nrows"], int(A["rowptr"][-1])
-        # A values+cols, rowptr, z and p_old gathered, p_new and q written
-        return nnz * (vb + 4) + 4 * (n + 1) + 4 * vb * n
+        # operator values + column indices once (cg_g: fp32 G values on A's
+        # pattern), slice pointers, the input vector once, the output once
+        ob = 4 if kernel == "cg_g_kernel" else vb
+        return nnz * (ob + 4) + 8 * (n // 64 + 1) + 2 * vb * n
     if kernel == "dga_h_kernel":
         H = d["H"]
         n, nnz = H["nrows"], int(H["rowptr"][-1])
@@ -147,7 +149,7 @@ def main():
             bd = build_workload(d, "dga", args.n, total)
             dga = time_path(bd, "fp64", args.steps, args.warmup, "jacobi", "dga fp64 (explicit)")
             dga32 = time_path(bd, "fp32", args.steps, args.warmup, "jacobi", "dga fp32 (explicit)")
-        kname = "cg_a_kernel"
+        kname = "cg_w_kernel"
         vb = 4 if head is fast else 8
         kern = head["kernels"].get(kname)
         roof = None

@@ -199,3 +199,29 @@ def test_kuhn_writer_rejects_inversion(tmp_path):
     with pytest.raises(Exception):
         for s in range(50):
             write_kuhn_mesh(str(tmp_path / "bad.mesh"), 6, 0.49, s, 2)
+
+
+def test_dga_reluctance_patch_test(tmp_path):
+    """N (sum over tets of the inverse Codecasa mu block, D:8757-8776) is
+    exact for uniform fields: with face fluxes phi_f = B0 . a_f (a_f the
+    area vector of the sorted face (p0,p1,p2)), phi^T N phi = |B0|^2 V / mu0
+    on a jittered mesh. A wrong dual-area vector, coefficient or index in
+    the local block breaks the identity."""
+    import scipy.sparse as sp
+    p = configs.kuhn_cavity(str(tmp_path), n=3, method="dga", n_steps=5, n_probes=1, block=2, seed=11,
+                            eps_range=(1.0, 1.0))
+    b = Model(p).build(1)
+    t = b.tet_tables()
+    N = b.dga_ops()["N"]
+    Nm = sp.csr_matrix((N["val"], N["col"], N["rowptr"]), shape=(b.n_faces, b.n_faces))
+    pts, fn, vols = t["pts"], t["face_nodes"], t["vols"]
+    area = 0.5 * np.cross(pts[fn[:, 1]] - pts[fn[:, 0]], pts[fn[:, 2]] - pts[fn[:, 0]])
+    V = np.abs(np.einsum("ij,ij->i", pts[vols[:, 1]] - pts[vols[:, 0]],
+                         np.cross(pts[vols[:, 2]] - pts[vols[:, 0]], pts[vols[:, 3]] - pts[vols[:, 0]]))).sum() / 6
+    mu0 = float(np.float64(4 * np.longdouble(np.pi) * np.longdouble(1e-7)))
+    rng = np.random.default_rng(0)
+    for _ in range(3):
+        B0 = rng.standard_normal(3)
+        phi = area @ B0
+        expect = (B0 @ B0) * V / mu0
+        assert abs(phi @ (Nm @ phi) - expect) <= 1e-13 * expect
