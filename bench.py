@@ -83,6 +83,7 @@ def time_path(b, precision, steps, warmup, precond="jacobi", label="", learn_ste
                      loss=ls["pc_loss"], jacobi_iters_per_step=ls["pc_jacobi_mean_iters"], state=ls["pc_state"])
         log(f"[bench] {label}: trained on {ls['pc_pairs']} pairs in {ls['pc_train_seconds']:.2f} s, "
             f"loss {ls['pc_loss0']:.4g} -> {ls['pc_loss']:.4g}")
+    tuned = sim.autotune()
     if warmup:
         sim.run(n=warmup, precond=precond)
     st = sim.run(n=steps, precond=precond)
@@ -92,6 +93,7 @@ def time_path(b, precision, steps, warmup, precond="jacobi", label="", learn_ste
                cg_iters_per_step=st["cg_iters_total"] / steps if b.method == "gfdtd" else None,
                cg_max_relres=st["cg_max_relres"] if b.method == "gfdtd" else None,
                cg_not_converged=st["cg_not_converged"] if b.method == "gfdtd" else None)
+    out["tuning"] = tuned
     if learn is not None:
         learn.update(state_after=st["pc_state"], fallbacks=st["pc_fallbacks"])
         out["learned"] = learn
@@ -116,6 +118,81 @@ def cpu_baseline(b, steps):
                 host=os.uname().nodename, nproc=os.cpu_count())
 
 
+def batch_workload(d, count=256, steps=2000, seed=4, workers=16):
+    """Batched dga workload: `count` Kuhn cavities with n ~ U{8..32}, jitter
+    0.2h, one eps_r ~ U[1, 10] and one source frequency ~ U[1e8, 5e8] Hz per
+    instance (numpy default_rng(seed)); dt = 0.9 x each instance's own
+    stability estimate. Host builds run in a thread pool."""
+    from concurrent.futures import ThreadPoolExecutor
+    from tetfit_amd import Model, configs
+    rng = np.random.default_rng(seed)
+    ns = rng.integers(8, 33, size=count)
+    eps = rng.uniform(1.0, 10.0, size=count)
+    freq = rng.uniform(1e8, 5e8, size=count)
+
+    def one(k):
+        p = configs.kuhn_cavity(d, n=int(ns[k]), method="dga", n_steps=steps, n_probes=1, block=4,
+                                seed=1000 + k, eps_range=(float(eps[k]), float(eps[k])), freq=float(freq[k]),
+                                dt="courant", name=f"inst{k}")
+        return Model(p).build(1)
+
+    t = time.time()
+    with ThreadPoolExecutor(max_workers=workers) as ex:
+        builds = list(ex.map(one, range(count)))
+    log(f"[bench] built {count} dga instances, n in [{ns.min()}, {ns.max()}], "
+        f"sum E = {sum(b.n_edges for b in builds)}, sum F = {sum(b.n_faces for b in builds)} "
+        f"({time.time() - t:.1f}s host set-up)")
+    return builds
+
+
+def bench_batch(args):
+    from tetfit_amd.device import BatchSimulation
+    steps = args.steps if args.steps_given else 2000
+    with tempfile.TemporaryDirectory() as d:
+        builds = batch_workload(d, count=args.batch, steps=steps)
+        E = sum(b.n_edges for b in builds)
+        res = {}
+        for prec in ("fp32", "fp64"):
+            sim = BatchSimulation(builds, prec)
+            tuned = sim.autotune()
+            log(f"[bench] batch {prec} tuning: {tuned}")
+            if args.warmup:
+                sim.run(args.warmup)
+            st = sim.run(steps)
+            sim.set_profiling(True)
+            sim.run(min(steps, 50))
+            kern = {k: {"ms_avg": v[0], "launches": v[1]} for k, v in sim.kernel_times().items()}
+            sim.close()
+            res[prec] = dict(seconds=st["seconds"], ms_per_step=1e3 * st["seconds"] / steps,
+                             edge_dof_per_s=E * steps / st["seconds"], kernels=kern, tuning=tuned)
+            log(f"[bench] batch {prec}: {res[prec]['ms_per_step']:.3f} ms/step, "
+                f"{res[prec]['edge_dof_per_s'] / 1e9:.3f} G edge-DoF/s")
+        head = res["fp32"]
+        nnz_h = sum(int(b.dga_ops()["H"]["rowptr"][-1]) for b in builds)
+        n_h = sum(int(b.dga_ops()["H"]["nrows"]) for b in builds)
+        kh = head["kernels"].get("sell_h_kernel")
+        roof = None
+        if kh:
+            # H values+cols, row->edge map, row dt, curl_f gathered, U read+write
+            nbytes = nnz_h * 8 + 4 * n_h + 8 * n_h + 4 * E + 2 * 4 * n_h
+            ach = nbytes / (kh["ms_avg"] * 1e-3) / 1e9
+            roof = dict(bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                        frac=round(ach / HBM_PEAK_GBS, 4), traffic=None, kernel="sell_h_kernel",
+                        bytes_per_launch=int(nbytes), ms_per_launch=kh["ms_avg"])
+        line = {
+            "metric": "edge-DoF updates/s, batched dga (all instances), 1 GPU",
+            "value": head["edge_dof_per_s"], "unit": "edge-DoF/s", "n_gpus": 1, "steps": steps,
+            "warmup": args.warmup, "ms_per_step": head["ms_per_step"], "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": f"{len(builds)} Kuhn dga cavities, n~U{{8..32}}, eps~U[1,10], f~U[1e8,5e8] Hz",
+                       "n_edges_total": E, "n_faces_total": sum(b.n_faces for b in builds),
+                       "parallelism": "single GPU, one launch per phase for the whole batch"},
+            "fp64": {"edge_dof_per_s": res["fp64"]["edge_dof_per_s"], "ms_per_step": res["fp64"]["ms_per_step"]},
+            "roofline": roof, "cpu_baseline": None, "kernels": head["kernels"], "tuning": head["tuning"],
+        }
+        print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -125,7 +202,11 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-dga", action="store_true")
+    ap.add_argument("--workload", choices=["gfdtd", "batch"], default="gfdtd",
+                    help="gfdtd: the 48^3 headline; batch: batched dga instances (own JSON line)")
+    ap.add_argument("--batch", type=int, default=256, help="instances of the batch workload")
     args = ap.parse_args()
+    args.steps_given = any(a.startswith("--steps") for a in sys.argv[1:])
     if args.gpus != 1:
         print(f"bench.py drives exactly one GPU (asked for {args.gpus}); multi-GPU runs are not part of this build",
               file=sys.stderr)
@@ -134,6 +215,9 @@ def main():
     if device_count() < 1:
         print("bench.py needs a HIP device", file=sys.stderr)
         sys.exit(1)
+    if args.workload == "batch":
+        bench_batch(args)
+        return
     total = args.warmup + args.steps + 260
     with tempfile.TemporaryDirectory() as d:
         bg = build_workload(d, "gfdtd", args.n, total)

@@ -157,6 +157,37 @@ tf_status tf_device_bytes(tf_ctx* ctx, int64_t* bytes);
 
 tf_status tf_destroy(tf_ctx* ctx);
 
+/* Autotuner: times the context's SELL kernels (Ct, H, N for dga; N, A and
+ * the learned G for gfdtd) over grid sizes and gather batch depths and keeps
+ * the fastest for later tf_run calls. Results are identical for every
+ * choice; they are cached per (kernel, precision, rows, entries) in the
+ * process and in the file named by $TETFIT_TUNE_CACHE if set. No reference
+ * counterpart. tf_get_tuning reports the choice per kernel (order: Ct, H, N,
+ * A, G; blocks 0 = default). */
+tf_status tf_autotune(tf_ctx* ctx);
+tf_status tf_get_tuning(tf_ctx* ctx, int32_t* blocks, int32_t* unroll, int32_t cap, int32_t* n);
+
+/* ---- batches ----------------------------------------------------------
+ * Many independent dga simulations (uneven meshes, materials, sources and
+ * time steps) stepped together: the problems are concatenated into one
+ * block-diagonal context so that each phase of the leapfrog is one launch
+ * over every instance; rows and faces carry their own simulation's dt and
+ * each source its own time base (t = i * dt_instance). No reference
+ * counterpart: the reference runs simulations one after another
+ * (dga_main.cpp:58-62). tf_run then advances all instances by the same
+ * number of steps (cg may be NULL); Poynting values are summed over the
+ * batch. Edge, face and source indices in the per-instance calls are local
+ * to the instance. */
+tf_status tf_create_batch(const tf_problem* const* probs, int32_t n, int32_t prec, tf_ctx** out);
+tf_status tf_batch_set_sources(tf_ctx* ctx, int32_t inst, const tf_source* src, int32_t n_src,
+                               const tf_source_term* terms, int64_t n_terms);
+tf_status tf_batch_set_probes(tf_ctx* ctx, int32_t inst, const tf_probe* probes, int32_t n);
+/* Probe samples every `every` steps (and at t = 0), 0 = none. */
+tf_status tf_batch_set_output(tf_ctx* ctx, int64_t every);
+tf_status tf_batch_read_probes(tf_ctx* ctx, int32_t inst, double* times, double* vals,
+                               int64_t cap_samples, int64_t* n);
+tf_status tf_batch_read_state(tf_ctx* ctx, int32_t inst, double* U, double* F, double* B);
+
 /* One forward and backward pass of the preconditioner MLP on the device
  * (the kernels training uses), for gradient checks: y[i] = MLP(x[i]) and
  * grad = sum_i gy[i] dy[i]/dparams, for n feature rows x (n x TF_MLP_IN,

@@ -1,0 +1,54 @@
+"""Batched dga (many independent simulations in one context, one launch per
+phase): every instance against its own run of the CPU checker.
+
+Instances differ in mesh size, jitter seed, eps range, source frequency and
+therefore dt; all advance the same number of steps. Bars as for single
+simulations: fp64 per probe and component <= 1e-10 of the component's (or
+field's) peak and final U, F, B within 1e-10 relative L2; fp32 within 1e-4
+relative L2."""
+import numpy as np
+import pytest
+
+from tetfit_amd import Model, configs
+
+import pyoracle
+from test_gpu_parity import per_probe_err, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+STEPS = 80
+
+
+@pytest.fixture(scope="module")
+def instances(tmp_path_factory):
+    d = tmp_path_factory.mktemp("batch")
+    specs = [(3, 1, (1.0, 4.0), 2e8), (5, 2, (1.0, 10.0), 3e8), (4, 3, (2.0, 6.0), 1.5e8), (6, 4, (1.0, 10.0), 4e8)]
+    out = []
+    for k, (n, seed, er, f) in enumerate(specs):
+        p = configs.kuhn_cavity(str(d), n=n, method="dga", n_steps=STEPS, n_probes=3, block=2, seed=seed,
+                                eps_range=er, freq=f, name=f"b{k}")
+        out.append(Model(p).build(1))
+    return out
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_batch_matches_each_instance(instances, prec):
+    from tetfit_amd.device import BatchSimulation
+    dts = sorted({b.dt for b in instances})
+    print(f"case: batch {prec} of {len(instances)} dga instances, dts {dts}", flush=True)
+    assert len(dts) == len(instances)  # the instances really run on different dt
+    sim = BatchSimulation(instances, prec, sample_every=1)
+    sim.run(STEPS)
+    for k, b in enumerate(instances):
+        t, s = sim.probes(k)
+        U, F, B = sim.state(k)
+        r = pyoracle.run(b, n=STEPS)
+        np.testing.assert_allclose(t, r["times"], rtol=1e-15, atol=0)
+        if prec == "fp64":
+            assert per_probe_err(s, r["samples"]) <= 1e-10, k
+            for got, ref in ((U, r["U"]), (F, r["F"]), (B, r["B"])):
+                assert rel_l2(got, ref) <= 1e-10, k
+        else:
+            assert rel_l2(s, r["samples"]) <= 1e-4, k
+            assert rel_l2(U, r["U"]) <= 1e-4, k
+    sim.close()

@@ -105,3 +105,24 @@ def test_gfdtd_fp32(workdir):
     b = Model(configs.kuhn_cavity(str(workdir), n=5, method="gfdtd", n_steps=60, n_probes=4, block=2)).build(1)
     err, st = fp32_err(b)
     assert err <= FP32_TOL
+
+
+@pytest.mark.parametrize("method", ["dga", "gfdtd"])
+def test_autotune_keeps_results_bitwise(workdir, method):
+    """The autotuner changes only launch shapes (grid size, gather batch);
+    every row is still summed by one lane in column order, so fp64 fields
+    are bitwise identical with and without tuning."""
+    b = Model(configs.kuhn_cavity(str(workdir), n=5, method=method, n_steps=40, n_probes=2, block=2,
+                                  name=f"tune_{method}")).build(1)
+    print(f"case: autotune {method} E={b.n_edges}", flush=True)
+    a = Simulation(b, "fp64")
+    a.run()
+    Ua, Fa, Ba = a.state()
+    a.close()
+    t = Simulation(b, "fp64")
+    tuned = t.autotune()
+    print(tuned, flush=True)
+    t.run()
+    Ut, Ft, Bt = t.state()
+    t.close()
+    assert np.array_equal(Ua, Ut) and np.array_equal(Fa, Ft) and np.array_equal(Ba, Bt)

@@ -186,10 +186,21 @@ def hip_lib():
     L.tf_destroy.argtypes = [vp]
     f32p = C.POINTER(C.c_float)
     L.tf_mlp_grad.argtypes = [f32p, f32p, f32p, C.c_int64, f32p, f32p]
+    L.tf_create_batch.argtypes = [C.POINTER(C.POINTER(tf_problem)), C.c_int32, C.c_int32, C.POINTER(vp)]
+    L.tf_batch_set_sources.argtypes = [vp, C.c_int32, C.POINTER(tf_source), C.c_int32,
+                                       C.POINTER(tf_source_term), C.c_int64]
+    L.tf_batch_set_probes.argtypes = [vp, C.c_int32, C.POINTER(tf_probe), C.c_int32]
+    L.tf_batch_set_output.argtypes = [vp, C.c_int64]
+    L.tf_batch_read_probes.argtypes = [vp, C.c_int32, f64p, f64p, C.c_int64, i64p]
+    L.tf_batch_read_state.argtypes = [vp, C.c_int32, f64p, f64p, f64p]
+    L.tf_autotune.argtypes = [vp]
+    L.tf_get_tuning.argtypes = [vp, i32p, i32p, C.c_int32, i32p]
     for n in ["tf_create", "tf_set_sources", "tf_set_probes", "tf_set_output", "tf_run",
               "tf_read_state", "tf_write_state", "tf_read_probes", "tf_read_poynting",
               "tf_energy", "tf_kernel_times", "tf_set_profiling", "tf_device_bytes", "tf_destroy",
-              "tf_mlp_grad"]:
+              "tf_mlp_grad", "tf_create_batch", "tf_batch_set_sources", "tf_batch_set_probes",
+              "tf_batch_set_output", "tf_batch_read_probes", "tf_batch_read_state", "tf_autotune",
+              "tf_get_tuning"]:
         getattr(L, n).restype = C.c_int32
     _hip = L
     return L

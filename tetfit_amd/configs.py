@@ -94,7 +94,10 @@ def kuhn_cavity(directory, n=48, method="dga", n_steps=1000, n_probes=8, jitter=
     """Kuhn mesh of the unit cube, eps_r per block drawn from eps_range
     (uniform, or log-uniform) with numpy default_rng(seed). Returns the .fdtd
     path. When dt is None the host's stability estimate is computed once and
-    dt = dt_safety x estimate (rounded to 4 digits) is pinned with SET step."""
+    dt = dt_safety x estimate (rounded to 4 digits) is pinned with SET step.
+    dt="courant" writes no step and `SET courant dt_safety` instead (one host
+    build; dt = dt_safety x the estimate, the duration then only bounds the
+    step count)."""
     os.makedirs(directory, exist_ok=True)
     name = name or f"kuhn{n}_{method}"
     mesh = os.path.join(directory, f"kuhn{n}_s{seed}_j{jitter}_b{block}.mesh")
@@ -128,10 +131,15 @@ def kuhn_cavity(directory, n=48, method="dga", n_steps=1000, n_probes=8, jitter=
             f.write(_probe_lines(pts))
             f.write("END output 1\n")
             f.write("DEFINE simulation 1\n    SET method %s\n    SET mesh 1\n    SET output 1\n"
-                    "    SET courant 1\n%s    SET solver cg\n    SET tolerance %r\nEND simulation 1\n"
-                    % (method, step_line, tol))
+                    "    SET courant %r\n%s    SET solver cg\n    SET tolerance %r\nEND simulation 1\n"
+                    % (method, courant, step_line, tol))
         return fdtd
 
+    courant = 1
+    if dt == "courant":
+        courant = dt_safety
+        # an upper bound of the step count: dt >= 0.05 h / c0 on these meshes
+        return write("    SET duration %r\n" % (n_steps * 0.05 / (n * C0)))
     if dt is None:
         probe = write("    SET duration 0\n")
         b = Model(probe).build(1)

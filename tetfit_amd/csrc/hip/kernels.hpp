@@ -133,22 +133,6 @@ __global__ void probe_kernel(int32_t n_probes, const ProbeDev* __restrict__ pr, 
 }
 
 // ------------------------------------------------------------------ DGA
-// Gather SpMV rows of an incidence CSR: y[r] = sum s*x[col], LPR lanes per row.
-template <typename T, int LPR>
-__global__ void __launch_bounds__(kBlock) incidence_rows_kernel(int32_t nrows, const int32_t* __restrict__ ptr,
-                                                                const int32_t* __restrict__ col,
-                                                                const int8_t* __restrict__ sg,
-                                                                const T* __restrict__ x, T* __restrict__ y) {
-    const int32_t g = blockIdx.x * kBlock + threadIdx.x;
-    const int32_t r = g / LPR, lane = g % LPR;
-    double s = 0;
-    if (r < nrows) {
-        for (int32_t k = ptr[r] + lane; k < ptr[r + 1]; k += LPR) s += (double)sg[k] * (double)x[col[k]];
-    }
-    s = group_sum<LPR>(s);
-    if (r < nrows && lane == 0) y[r] = (T)s;
-}
-
 // Source injection for the tetrahedral methods: h sources add Fb to curl_f
 // (Ctb*Fb, one +1 per boundary edge), e sources set U (class-4 edges).
 // The Poynting partial uses U before the E update: 0.5*U_old*fb.
@@ -193,30 +177,12 @@ __global__ void __launch_bounds__(kBlock) dga_poynting_kernel(
     if (threadIdx.x == 0) poy_part[blockIdx.x] = s;
 }
 
-// U[edge(r)] += dt * (H curl_f)[r] over the H rows (class-1 edges), LPR lanes per row.
-template <typename T, int LPR>
-__global__ void __launch_bounds__(kBlock) dga_h_kernel(int32_t nrows, const int32_t* __restrict__ ptr,
-                                                       const int32_t* __restrict__ col, const T* __restrict__ val,
-                                                       const int32_t* __restrict__ row_edge,
-                                                       const T* __restrict__ x, T* __restrict__ U, double dt) {
-    const int32_t g = blockIdx.x * kBlock + threadIdx.x;
-    const int32_t r = g / LPR, lane = g % LPR;
-    double s = 0;
-    if (r < nrows) {
-        for (int32_t k = ptr[r] + lane; k < ptr[r + 1]; k += LPR) s += (double)val[k] * (double)x[col[k]];
-    }
-    s = group_sum<LPR>(s);
-    if (r < nrows && lane == 0) {
-        const int32_t e = row_edge[r];
-        U[e] = (T)((double)U[e] + dt * s);
-    }
-}
-
 // curl_u = C U (3 signed edges per face), B -= dt*curl_u (D:902-910).
 template <typename T>
 __global__ void __launch_bounds__(kBlock) dga_curl_kernel(int32_t n_faces, const int32_t* __restrict__ c_col,
                                                           const int8_t* __restrict__ c_sg, const T* __restrict__ U,
-                                                          T* __restrict__ curl_u, T* __restrict__ B, double dt) {
+                                                          T* __restrict__ curl_u, T* __restrict__ B, double dt,
+                                                          const double* __restrict__ face_dt) {
     const int32_t f = blockIdx.x * kBlock + threadIdx.x;
     if (f >= n_faces) return;
     const int32_t* c = c_col + 3 * (int64_t)f;
@@ -229,33 +195,17 @@ __global__ void __launch_bounds__(kBlock) dga_curl_kernel(int32_t n_faces, const
         cu = (double)U[c[0]] - (double)U[c[1]] + (double)U[c[2]];
     }
     curl_u[f] = (T)cu;
-    B[f] = (T)((double)B[f] - dt * cu);
-}
-
-// F -= dt * N curl_u (dga, D:917) or F = N B (gfdtd, D:1468); mode 0 / 1.
-template <typename T, int LPR>
-__global__ void __launch_bounds__(kBlock) dga_n_kernel(int32_t nrows, const int32_t* __restrict__ ptr,
-                                                       const int32_t* __restrict__ col, const T* __restrict__ val,
-                                                       const T* __restrict__ x, T* __restrict__ Fv, double dt,
-                                                       int mode) {
-    const int32_t g = blockIdx.x * kBlock + threadIdx.x;
-    const int32_t r = g / LPR, lane = g % LPR;
-    double s = 0;
-    if (r < nrows) {
-        for (int32_t k = ptr[r] + lane; k < ptr[r + 1]; k += LPR) s += (double)val[k] * (double)x[col[k]];
-    }
-    s = group_sum<LPR>(s);
-    if (r < nrows && lane == 0) Fv[r] = mode == 0 ? (T)((double)Fv[r] - dt * s) : (T)s;
+    B[f] = (T)((double)B[f] - (face_dt ? face_dt[f] : dt) * cu);
 }
 
 // ------------------------------------------------------------------ gfdtd
-// CG of ConjugateGradientSolver.hpp:52-103 split over two kernels per
-// iteration. Each kernel writes one fp64 partial per block; the next kernel
-// sums them (every block, in index order, so all blocks take the same
-// decision) - no cross-block fences inside a kernel.
+// Device state of one CG solve (kernels in sell.hpp). Each kernel writes one
+// fp64 partial per block; the next kernel sums them (every block, in index
+// order, so all blocks take the same decision) - no cross-block fences.
 struct CgState {
     double rr0;        // ||r0||^2 of this solve
-    double rho[2];     // r.z of iteration it, slot it&1
+    double gam[2];     // r.u of pass it, slot it&1
+    double alpha[2];   // step length of pass it, slot it&1
     double relres;     // ||r||/||r0|| at the last stop test
     double tol;
     int32_t maxiter;
@@ -314,19 +264,6 @@ __global__ void __launch_bounds__(kBlock) gf_inject_kernel(
     }
     const double s = block_sum(poy, red);
     if (threadIdx.x == 0) poy_part[blockIdx.x] = s;
-}
-
-// ------------------------------------------------------------------ energy
-// 0.5 * sum w_i x_i^2 in fp64 with a last-block reduction into out[0].
-template <typename T>
-__global__ void __launch_bounds__(kBlock) weighted_sq_kernel(int32_t n, const T* __restrict__ x,
-                                                             const double* __restrict__ w, double* __restrict__ part) {
-    __shared__ double red[kBlock / kWave];
-    const int32_t i = blockIdx.x * kBlock + threadIdx.x;
-    double v = 0;
-    if (i < n) v = 0.5 * w[i] * (double)x[i] * (double)x[i];
-    v = block_sum(v, red);
-    if (threadIdx.x == 0) part[blockIdx.x] = v;
 }
 
 }  // namespace tfd

@@ -8,6 +8,8 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -30,9 +32,9 @@ thread_local std::string g_err;
 
 // Grid caps of the grid-stride kernels: the CG partial buffers hold one
 // fp64 per block of the producing kernel.
-constexpr int kCgPartA = 2048;   // blocks of cg_a (SpMV with A)
-constexpr int kCgPartB = 1024;   // blocks of cg_init / cg_b
-constexpr int kCgPartC = 2048;   // blocks of cg_c (SpMV with the learned G)
+constexpr int kCgPartA = 2048;   // blocks of cg_w (SpMV with A)
+constexpr int kCgPartB = 1024;   // blocks of cg_init / cg_upd
+constexpr int kCgPartC = 2048;   // blocks of cg_g (SpMV with the learned G)
 constexpr int kSpmvCap = 8192;   // blocks of the other SELL kernels
 
 // Carrier of Source::Compute in long double (Source.hpp:316-327); 0 for t<0.
@@ -77,6 +79,26 @@ struct Sell {
     int32_t nrows = 0, nslices = 0;
     int64_t nnz = 0, entries = 0;
 };
+
+// Launch shape of the tuned SELL kernels: grid size (0 = one slice per
+// wave) and entries per gather batch (4, 8 or 16). Results do not depend on
+// either (each row is summed by one lane in column order).
+enum TuneKernel { TK_CT = 0, TK_H, TK_N, TK_W, TK_G, TK_COUNT };
+const char* const kTuneName[TK_COUNT] = {"sell_ct_kernel", "sell_h_kernel", "sell_n_kernel", "cg_w_kernel",
+                                         "cg_g_kernel"};
+struct KTune {
+    int blocks = 0;
+    int unroll = 8;
+};
+
+#define TF_LAUNCH_NB(nb, kern, T, grid, stream, ...)                               \
+    do {                                                                          \
+        switch (nb) {                                                             \
+            case 4: kern<T, 4><<<(grid), kBlock, 0, (stream)>>>(__VA_ARGS__); break;   \
+            case 16: kern<T, 16><<<(grid), kBlock, 0, (stream)>>>(__VA_ARGS__); break; \
+            default: kern<T, 8><<<(grid), kBlock, 0, (stream)>>>(__VA_ARGS__); break;  \
+        }                                                                         \
+    } while (0)
 
 // State of the learned preconditioner of a gfdtd context.
 struct LearnedPc {
@@ -141,6 +163,24 @@ struct tf_ctx {
     std::vector<int32_t> hA_col;
     std::vector<double> hA_val;
     LearnedPc lp;
+    KTune tune[TK_COUNT];
+
+    // batch of independent dga simulations (tf_create_batch): the instances
+    // are concatenated (block-diagonal operators), each row/face carries its
+    // own simulation's dt, each source its own carrier time base
+    struct Batch {
+        int32_t n = 0;
+        std::vector<int32_t> eoff, foff, poff;   // n+1 prefix sums: edges, faces, probes
+        std::vector<double> dt;
+        std::vector<std::vector<tf_source>> src;
+        std::vector<std::vector<tf_source_term>> terms;
+        std::vector<std::vector<tf_probe>> probes;
+        std::vector<int64_t> sample_steps;
+        int64_t every = 0;                       // probe sample period in steps (0: off)
+    };
+    std::unique_ptr<Batch> batch;
+    DevBuf row_dt_h, row_dt_f;                   // per H row / per face dt (batches)
+    std::vector<double> src_dt;                  // per source dt of the carrier table
 
     // sources
     std::vector<tf_source> src;
@@ -186,7 +226,7 @@ size_t tf_ctx::device_bytes() const {
                            &sp_edge, &sp_f4, &sp_sg, &sp_ca, &sp_cb, &sp_kind, &sp_tb, &sp_te, &c_col, &c_sg,
                            &h_edge, &curl_f, &curl_u, &free_edge, &dir_edge, &dir_new, &fb, &pre, &x, &r, &z,
                            &p0, &p1, &q, &rhs, &cg_part, &cg_state, &inj_edge, &inj_kind, &inj_tb, &inj_te,
-                           &term_src, &term_w, &carr, &probes, &samples, &poy};
+                           &term_src, &term_w, &carr, &probes, &samples, &poy, &row_dt_h, &row_dt_f};
     for (auto b : all) s += b->bytes;
     const Csr32* cs[] = {&Ct};
     for (auto c : cs) s += c->ptr.bytes + c->col.bytes + c->val.bytes + c->sg.bytes;
@@ -271,6 +311,11 @@ void upload_sell(Sell& d, const tf_csr& c, bool values) {
 int sell_blocks(const Sell& m, int cap) {
     const int need = (m.nslices + kBlock / kWave - 1) / (kBlock / kWave);
     return std::max(1, std::min(need, cap));
+}
+
+int tuned_blocks(const tf_ctx* c, int k, const Sell& m, int cap) {
+    const int b = c->tune[k].blocks;
+    return b > 0 ? std::max(1, std::min(b, std::min(cap, sell_blocks(m, INT32_MAX)))) : sell_blocks(m, cap);
 }
 
 template <class T>
@@ -484,7 +529,7 @@ void create_tet(tf_ctx* c, const tf_problem* p) {
         upload(c->pre, pre);
         const int64_t n = std::max<int32_t>(d.n_free, 1);
         for (DevBuf* b : {&c->x, &c->r, &c->z, &c->p0, &c->p1, &c->q, &c->rhs}) zero_field<T>(*b, n, c->stream);
-        c->cg_part.alloc(sizeof(double) * (kCgPartA + 2 * kCgPartB + kCgPartC));
+        c->cg_part.alloc(sizeof(double) * (kCgPartA + 3 * kCgPartB + kCgPartC));
         c->cg_state.alloc(sizeof(CgState));
         TF_HIP(hipMemsetAsync(c->cg_state.p, 0, sizeof(CgState), c->stream));
         TF_HIP(hipHostMalloc((void**)&c->h_state, sizeof(CgState), hipHostMallocDefault));
@@ -594,6 +639,17 @@ bool allow_print(tf_ctx* c, double t) {
     return false;
 }
 
+// Output schedule: AllowPrint for single simulations; a fixed step period
+// for batches (each instance's time is step * its own dt).
+bool want_sample(tf_ctx* c, int64_t i, double t) {
+    if (c->batch) {
+        if (c->batch->every <= 0 || i % c->batch->every != 0) return false;
+        c->batch->sample_steps.push_back(i);
+        return true;
+    }
+    return allow_print(c, t);
+}
+
 template <class T>
 void sample(tf_ctx* c, double t) {
     if (c->n_probes == 0) {
@@ -626,12 +682,14 @@ void ensure_carr(tf_ctx* c, int64_t first, int64_t n) {
     // carriers for steps first-1 .. first+n-1 (row 0 is the previous step)
     const size_t ns = std::max<size_t>(c->src.size(), 1);
     std::vector<double> tab((size_t)(n + 1) * ns, 0.0);
+    const bool per_src = c->src_dt.size() == c->src.size();
     for (int64_t k = 0; k <= n; ++k) {
         const int64_t i = first - 1 + k;
-        const double t = (double)i * c->dt;
         for (size_t s = 0; s < c->src.size(); ++s) {
             const tf_source& so = c->src[s];
-            const double ts = so.field == TF_FIELD_H ? t - 0.5 * c->dt : t;
+            const double dt = per_src ? c->src_dt[s] : c->dt;
+            const double t = (double)i * dt;
+            const double ts = so.field == TF_FIELD_H ? t - 0.5 * dt : t;
             tab[k * ns + s] = (i < 0) ? 0.0 : carrier_value(so, ts);
         }
     }
@@ -706,17 +764,19 @@ void run_dga(tf_ctx* c, int64_t first, int64_t n, tf_stats* st) {
                 c->inj_te.as<int32_t>(), c->term_src.as<int32_t>(), c->term_w.as<double>(), cs0, c->curl_f.as<T>(),
                 c->U.as<T>(), c->poy.as<double>());
         }
-        if (allow_print(c, 0.0)) sample<T>(c, 0.0);
+        if (want_sample(c, 0, 0.0)) sample<T>(c, 0.0);
     }
     c->initial_done = true;
-    const int gb_ct = sell_blocks(c->sCt, kSpmvCap), gb_h = sell_blocks(c->H, kSpmvCap);
-    const int gb_n = sell_blocks(c->N, kSpmvCap);
+    const double* hdt = c->batch ? c->row_dt_h.as<double>() : nullptr;
+    const double* fdt = c->batch ? c->row_dt_f.as<double>() : nullptr;
+    const int gb_ct = tuned_blocks(c, TK_CT, c->sCt, kSpmvCap), gb_h = tuned_blocks(c, TK_H, c->H, kSpmvCap);
+    const int gb_n = tuned_blocks(c, TK_N, c->N, kSpmvCap);
     for (int64_t k = 0; k < n; ++k) {
         const int64_t i = first + k;
         const double t = (double)i * c->dt;
         const double* cs = c->carr.as<double>() + (k + 1) * ns;
         ev_begin(c, "sell_ct_kernel");
-        sell_ct_kernel<T><<<gb_ct, kBlock, 0, c->stream>>>(c->E, c->sCt.nslices, c->sCt.sptr.as<int64_t>(),
+        TF_LAUNCH_NB(c->tune[TK_CT].unroll, sell_ct_kernel, T, gb_ct, c->stream, c->E, c->sCt.nslices, c->sCt.sptr.as<int64_t>(),
                                                            c->sCt.col.as<int32_t>(), c->sCt.sg.as<int8_t>(),
                                                            c->Fv.as<T>(), c->curl_f.as<T>());
         ev_end(c);
@@ -729,10 +789,10 @@ void run_dga(tf_ctx* c, int64_t first, int64_t n, tf_stats* st) {
             ev_end(c);
         }
         ev_begin(c, "sell_h_kernel");
-        sell_h_kernel<T><<<gb_h, kBlock, 0, c->stream>>>(c->H.nrows, c->H.nslices, c->H.sptr.as<int64_t>(),
+        TF_LAUNCH_NB(c->tune[TK_H].unroll, sell_h_kernel, T, gb_h, c->stream, c->H.nrows, c->H.nslices, c->H.sptr.as<int64_t>(),
                                                          c->H.col.as<int32_t>(), c->H.val.as<T>(),
                                                          c->h_edge.as<int32_t>(), c->curl_f.as<T>(), c->U.as<T>(),
-                                                         c->dt);
+                                                         c->dt, hdt);
         ev_end(c);
         if (c->n_inj) {
             ev_begin(c, "dga_poynting_kernel");
@@ -745,14 +805,14 @@ void run_dga(tf_ctx* c, int64_t first, int64_t n, tf_stats* st) {
         ev_begin(c, "dga_curl_kernel");
         dga_curl_kernel<T><<<blocks_for(c->F), kBlock, 0, c->stream>>>(c->F, c->c_col.as<int32_t>(),
                                                                         c->c_fixed_signs ? nullptr : c->c_sg.as<int8_t>(), c->U.as<T>(),
-                                                                        c->curl_u.as<T>(), c->B.as<T>(), c->dt);
+                                                                        c->curl_u.as<T>(), c->B.as<T>(), c->dt, fdt);
         ev_end(c);
         ev_begin(c, "sell_n_kernel");
-        sell_n_kernel<T><<<gb_n, kBlock, 0, c->stream>>>(c->N.nrows, c->N.nslices, c->N.sptr.as<int64_t>(),
+        TF_LAUNCH_NB(c->tune[TK_N].unroll, sell_n_kernel, T, gb_n, c->stream, c->N.nrows, c->N.nslices, c->N.sptr.as<int64_t>(),
                                                          c->N.col.as<int32_t>(), c->N.val.as<T>(), c->curl_u.as<T>(),
-                                                         c->Fv.as<T>(), c->dt, 0);
+                                                         c->Fv.as<T>(), c->dt, 0, fdt);
         ev_end(c);
-        if (allow_print(c, t)) sample<T>(c, t);
+        if (want_sample(c, i, t)) sample<T>(c, t);
         if (c->profiling && (k & 63) == 63) ev_flush(c);
     }
     TF_HIP(hipGetLastError());
@@ -769,67 +829,72 @@ void run_dga(tf_ctx* c, int64_t first, int64_t n, tf_stats* st) {
     st->steps_done += n;
 }
 
-// One CG solve on the device: cg_init, then iterations it = 0, 1, ... as
-// (cg_a, cg_b) pairs, or (cg_a, cg_b, cg_c) with the learned preconditioner
-// (cg_c applies z = G r; cg_b then leaves z alone). Launches are issued in
-// batches sized from the previous solve's count; kernels past the stop test
-// return at once. `snaps` lists (iteration k, bank slot) captures of the
-// iterate (r_k, x_k), taken in stream order after the update that produces
-// it. The final state is read back into h_state.
+// One CG solve on the device (single-reduction PCG, see sell.hpp): cg_init,
+// then passes it = 0, 1, ... of (cg_w, cg_upd) with Jacobi, or (cg_g, cg_w,
+// cg_upd) with the learned preconditioner. Launches are issued in batches
+// sized from the previous solve's count; kernels past the stop test return
+// at once. `snaps` lists (iteration k, bank slot) captures of the iterate
+// (r_k, x_k), taken in stream order after the update that produces it. The
+// final state is read back into h_state.
+//
+// Vectors: u = z, w = q, p = p0, s = p1. Partials (fp64, one per block):
+// [delta: kCgPartA][rr: kCgPartB][gamma ping: kCgPartB][gamma pong: kCgPartB][gamma of cg_g: kCgPartC]
 template <class T>
 void cg_solve(tf_ctx* c, const CgState& init, bool learned, const std::vector<std::pair<int, int>>* snaps) {
     const int32_t nf = c->n_free;
     int& guess = learned ? c->cg_guess_pc : c->cg_guess;
-    const int nb_a = sell_blocks(c->A, kCgPartA);
+    const int nb_w = tuned_blocks(c, TK_W, c->A, kCgPartA);
     const int nb_b = std::max(1, std::min(blocks_for(nf), kCgPartB));
-    const int nb_c = sell_blocks(c->A, kCgPartC);
-    double* part_a = c->cg_part.as<double>();
-    double* part_b = part_a + kCgPartA;
-    double* part_c = part_b + 2 * kCgPartB;
-    const double* part_rz = learned ? part_c : part_b + nb_b;
-    const int nb_rz = learned ? nb_c : nb_b;
+    const int nb_g = tuned_blocks(c, TK_G, c->A, kCgPartC);
+    double* part_del = c->cg_part.as<double>();
+    double* part_rr = part_del + kCgPartA;
+    double* part_gj[2] = {part_rr + kCgPartB, part_rr + 2 * kCgPartB};
+    double* part_gg = part_rr + 3 * kCgPartB;
     const int jac = learned ? 0 : 1;
     CgState* dst = c->cg_state.as<CgState>();
+    T *x = c->x.as<T>(), *r = c->r.as<T>(), *u = c->z.as<T>(), *w = c->q.as<T>(), *p = c->p0.as<T>(),
+      *sv = c->p1.as<T>();
     auto capture = [&](int k) {
         if (!snaps) return;
         for (auto& sn : *snaps)
             if (sn.first == k)
-                bank_capture_kernel<T><<<blocks_for(nf), kBlock, 0, c->stream>>>(
-                    nf, sn.second, c->r.as<T>(), c->x.as<T>(), c->lp.Rb.as<float>(), c->lp.Cb.as<float>());
-    };
-    auto apply_g = [&] {
-        ev_begin(c, "cg_c_kernel");
-        cg_c_kernel<T><<<nb_c, kBlock, 0, c->stream>>>(nf, c->A.nslices, c->A.sptr.as<int64_t>(), c->A.col.as<int32_t>(),
-                                                       c->lp.gval.as<float>(), c->r.as<T>(), c->z.as<T>(), part_c, dst);
-        ev_end(c);
+                bank_capture_kernel<T><<<blocks_for(nf), kBlock, 0, c->stream>>>(nf, sn.second, r, x,
+                                                                                   c->lp.Rb.as<float>(),
+                                                                                   c->lp.Cb.as<float>());
     };
     TF_HIP(hipMemcpyAsync(dst, &init, sizeof(CgState), hipMemcpyHostToDevice, c->stream));
     ev_begin(c, "cg_init_kernel");
     cg_init_kernel<T><<<nb_b, kBlock, 0, c->stream>>>(nf, c->A.nslices, c->A.sptr.as<int64_t>(), c->A.col.as<int32_t>(),
-                                                      c->A.val.as<T>(), c->pre.as<T>(), c->rhs.as<T>(), c->x.as<T>(),
-                                                      c->r.as<T>(), c->z.as<T>(), part_b, dst, jac);
+                                                      c->A.val.as<T>(), c->pre.as<T>(), c->rhs.as<T>(), x, r, u,
+                                                      part_rr, part_gj[0], dst, jac);
     ev_end(c);
     capture(0);
-    if (learned) apply_g();
     int it = 0;
     int batch = std::max(1, guess + 1);
     for (;;) {
         const int stop = std::min(it + batch, init.maxiter + 1);
         for (; it < stop; ++it) {
-            ev_begin(c, "cg_a_kernel");
-            cg_a_kernel<T><<<nb_a, kBlock, 0, c->stream>>>(it, nf, c->A.nslices, c->A.sptr.as<int64_t>(),
-                                                           c->A.col.as<int32_t>(), c->A.val.as<T>(), c->z.as<T>(),
-                                                           c->p0.as<T>(), c->p1.as<T>(), c->q.as<T>(), part_b, nb_b,
-                                                           part_rz, nb_rz, part_a, dst);
+            if (learned) {
+                ev_begin(c, "cg_g_kernel");
+                TF_LAUNCH_NB(c->tune[TK_G].unroll, cg_g_kernel, T, nb_g, c->stream, it, nf, c->A.nslices, c->A.sptr.as<int64_t>(),
+                                                               c->A.col.as<int32_t>(), c->lp.gval.as<float>(), r, u, 1,
+                                                               part_rr, nb_b, part_gg, dst);
+                ev_end(c);
+                if (it == init.maxiter) break;  // the pass at maxiter only runs the stop test
+            }
+            ev_begin(c, "cg_w_kernel");
+            TF_LAUNCH_NB(c->tune[TK_W].unroll, cg_w_kernel, T, nb_w, c->stream, it, nf, c->A.nslices, c->A.sptr.as<int64_t>(),
+                                                           c->A.col.as<int32_t>(), c->A.val.as<T>(), u, w, jac,
+                                                           part_rr, nb_b, part_del, dst);
             ev_end(c);
-            if (it == init.maxiter) break;  // cg_a(maxiter) only runs the stop test
-            ev_begin(c, "cg_b_kernel");
-            cg_b_kernel<T><<<nb_b, kBlock, 0, c->stream>>>(it, nf, c->pre.as<T>(), c->x.as<T>(), c->r.as<T>(),
-                                                           c->z.as<T>(), c->p0.as<T>(), c->p1.as<T>(), c->q.as<T>(),
-                                                           part_a, nb_a, part_b, dst, jac);
+            if (it == init.maxiter) break;
+            ev_begin(c, "cg_upd_kernel");
+            cg_upd_kernel<T><<<nb_b, kBlock, 0, c->stream>>>(it, nf, c->pre.as<T>(), x, r, u, w, p, sv,
+                                                             learned ? part_gg : part_gj[it & 1],
+                                                             learned ? nb_g : nb_b, part_del, nb_w, part_rr,
+                                                             part_gj[(it + 1) & 1], dst, jac);
             ev_end(c);
             capture(it + 1);
-            if (learned) apply_g();
         }
         TF_HIP(hipMemcpyAsync(c->h_state, dst, sizeof(CgState), hipMemcpyDeviceToHost, c->stream));
         TF_HIP(hipStreamSynchronize(c->stream));
@@ -1184,7 +1249,7 @@ void run_gfdtd(tf_ctx* c, int64_t first, int64_t n, const tf_cg_opts* cg, tf_sta
     init.maxiter = cg ? cg->maxiter : 100;
     const bool learned = cg && cg->precond == TF_PRECOND_LEARNED;
     if (learned && c->lp.state == LearnedPc::kOff) learned_setup(c, cg);
-    const int gb_n = sell_blocks(c->N, kSpmvCap);
+    const int gb_n = tuned_blocks(c, TK_N, c->N, kSpmvCap);
     for (int64_t k = 0; k < n; ++k) {
         const int64_t i = first + k;
         const double t = (double)i * c->dt;
@@ -1193,12 +1258,12 @@ void run_gfdtd(tf_ctx* c, int64_t first, int64_t n, const tf_cg_opts* cg, tf_sta
         ev_begin(c, "dga_curl_kernel");
         dga_curl_kernel<T><<<blocks_for(c->F), kBlock, 0, c->stream>>>(c->F, c->c_col.as<int32_t>(),
                                                                         c->c_fixed_signs ? nullptr : c->c_sg.as<int8_t>(), c->U.as<T>(),
-                                                                        c->curl_u.as<T>(), c->B.as<T>(), c->dt);
+                                                                        c->curl_u.as<T>(), c->B.as<T>(), c->dt, nullptr);
         ev_end(c);
         ev_begin(c, "sell_n_kernel");
-        sell_n_kernel<T><<<gb_n, kBlock, 0, c->stream>>>(c->N.nrows, c->N.nslices, c->N.sptr.as<int64_t>(),
+        TF_LAUNCH_NB(c->tune[TK_N].unroll, sell_n_kernel, T, gb_n, c->stream, c->N.nrows, c->N.nslices, c->N.sptr.as<int64_t>(),
                                                          c->N.col.as<int32_t>(), c->N.val.as<T>(), c->B.as<T>(),
-                                                         c->Fv.as<T>(), c->dt, 1);
+                                                         c->Fv.as<T>(), c->dt, 1, nullptr);
         ev_end(c);
         if (allow_print(c, t - c->dt)) sample<T>(c, t - c->dt);
         if (c->n_inj) {
@@ -1275,6 +1340,291 @@ tf_status guarded(F f) {
     }
 }
 
+std::unique_ptr<tf_ctx> make_ctx(const tf_problem* p, int32_t prec) {
+    if (p->method < 0 || p->method > 2) throw std::runtime_error("unknown method");
+    if (prec != TF_PREC_FP64 && prec != TF_PREC_FP32) throw std::runtime_error("unknown precision");
+    if (p->n_edges <= 0 || p->n_faces <= 0) throw std::runtime_error("empty problem");
+    int ndev = 0;
+    TF_HIP(hipGetDeviceCount(&ndev));
+    if (ndev < 1) throw HipError("no HIP device", TF_ERR_HIP);
+    TF_HIP(hipSetDevice(0));
+    auto c = std::make_unique<tf_ctx>();
+    c->method = p->method;
+    c->prec = prec;
+    c->E = p->n_edges;
+    c->F = p->n_faces;
+    c->dt = p->dt;
+    c->tsize = prec == TF_PREC_FP64 ? 8 : 4;
+    TF_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    c->U.alloc(c->E * c->tsize);
+    c->Fv.alloc(c->F * c->tsize);
+    c->B.alloc(c->F * c->tsize);
+    TF_HIP(hipMemsetAsync(c->U.p, 0, c->U.bytes, c->stream));
+    TF_HIP(hipMemsetAsync(c->Fv.p, 0, c->Fv.bytes, c->stream));
+    TF_HIP(hipMemsetAsync(c->B.p, 0, c->B.bytes, c->stream));
+    if (p->method == TF_METHOD_FDTD) {
+        if (prec == TF_PREC_FP64) create_fit<double>(c.get(), p);
+        else create_fit<float>(c.get(), p);
+        std::vector<std::vector<std::pair<int32_t, double>>> none;
+        std::vector<int> fld;
+        if (prec == TF_PREC_FP64) build_fit_special<double>(c.get(), none, fld);
+        else build_fit_special<float>(c.get(), none, fld);
+    } else {
+        if (prec == TF_PREC_FP64) create_tet<double>(c.get(), p);
+        else create_tet<float>(c.get(), p);
+        std::vector<std::vector<std::pair<int32_t, double>>> none(c->E);
+        std::vector<int> fld(c->E, TF_FIELD_H);
+        build_tet_inject(c.get(), none, fld);
+    }
+    ensure_carr(c.get(), 1, 0);
+    TF_HIP(hipStreamSynchronize(c->stream));
+    return c;
+}
+
+// Block-diagonal concatenation of n dga problems: edge, face and H-row
+// indices of instance k are offset by the sizes of instances 0..k-1.
+struct MergedDga {
+    std::vector<int64_t> c_ptr, ct_ptr, h_ptr, n_ptr;
+    std::vector<int32_t> c_col, ct_col, h_col, n_col, h_row, bnd_edge;
+    std::vector<int8_t> c_sg, ct_sg, edge_class;
+    std::vector<double> h_val, n_val, fixed_value, h_dt, f_dt;
+    std::vector<int32_t> eoff, foff;
+    tf_problem p{};
+};
+
+void append_csr(const tf_csr& m, int32_t col_off, std::vector<int64_t>& ptr, std::vector<int32_t>& col,
+                std::vector<int8_t>* sg, std::vector<double>* val) {
+    if (ptr.empty()) ptr.push_back(0);
+    const int64_t base = ptr.back();
+    for (int32_t r = 0; r < m.nrows; ++r) ptr.push_back(base + m.rowptr[r + 1]);
+    for (int64_t q = 0; q < m.nnz; ++q) {
+        const int64_t c64 = (int64_t)m.col[q] + col_off;
+        if (c64 > INT32_MAX) throw std::runtime_error("batch too large for int32 indices");
+        col.push_back((int32_t)c64);
+    }
+    if (sg) {
+        if (!m.sgn) throw std::runtime_error("incidence without signs");
+        sg->insert(sg->end(), m.sgn, m.sgn + m.nnz);
+    }
+    if (val) {
+        if (!m.val) throw std::runtime_error("operator without values");
+        val->insert(val->end(), m.val, m.val + m.nnz);
+    }
+}
+
+tf_csr csr_view(int32_t nrows, int32_t ncols, const std::vector<int64_t>& ptr, const std::vector<int32_t>& col,
+                const std::vector<int8_t>* sg, const std::vector<double>* val) {
+    tf_csr o{};
+    o.nrows = nrows;
+    o.ncols = ncols;
+    o.nnz = ptr.empty() ? 0 : ptr.back();
+    o.rowptr = ptr.data();
+    o.col = col.data();
+    o.sgn = sg ? sg->data() : nullptr;
+    o.val = val ? val->data() : nullptr;
+    return o;
+}
+
+void merge_dga(const tf_problem* const* probs, int32_t n, MergedDga& m) {
+    m.eoff.assign(1, 0);
+    m.foff.assign(1, 0);
+    int32_t hoff = 0, boff = 0;
+    for (int32_t k = 0; k < n; ++k) {
+        const tf_problem* p = probs[k];
+        if (!p) throw std::runtime_error("null problem in batch");
+        if (p->method != TF_METHOD_DGA) throw std::runtime_error("batches hold method dga problems");
+        if (p->n_edges <= 0 || p->n_faces <= 0) throw std::runtime_error("empty problem in batch");
+        if (!(p->dt > 0)) throw std::runtime_error("batch instance without a time step");
+        const tf_dga_ops& d = p->dga;
+        const int32_t E = p->n_edges, F = p->n_faces, eo = m.eoff.back(), fo = m.foff.back();
+        if ((int64_t)eo + E > INT32_MAX || (int64_t)fo + F > INT32_MAX) throw std::runtime_error("batch too large");
+        if (d.C.nrows != F || d.Ct.nrows != E || d.N.nrows != F) throw std::runtime_error("inconsistent operators");
+        append_csr(d.C, eo, m.c_ptr, m.c_col, &m.c_sg, nullptr);
+        append_csr(d.Ct, fo, m.ct_ptr, m.ct_col, &m.ct_sg, nullptr);
+        append_csr(d.H, eo, m.h_ptr, m.h_col, nullptr, &m.h_val);
+        append_csr(d.N, fo, m.n_ptr, m.n_col, nullptr, &m.n_val);
+        for (int32_t e = 0; e < E; ++e) {
+            m.h_row.push_back(d.h_row[e] >= 0 ? d.h_row[e] + hoff : -1);
+            m.bnd_edge.push_back(d.bnd_edge && d.bnd_edge[e] >= 0 ? d.bnd_edge[e] + boff : -1);
+            m.edge_class.push_back(d.edge_class[e]);
+            m.fixed_value.push_back(d.fixed_value[e]);
+        }
+        m.h_dt.insert(m.h_dt.end(), (size_t)d.H.nrows, p->dt);
+        m.f_dt.insert(m.f_dt.end(), (size_t)F, p->dt);
+        hoff += d.H.nrows;
+        boff += d.n_bnd;
+        m.eoff.push_back(eo + E);
+        m.foff.push_back(fo + F);
+    }
+    const int32_t E = m.eoff.back(), F = m.foff.back();
+    tf_problem& p = m.p;
+    p.method = TF_METHOD_DGA;
+    p.n_edges = E;
+    p.n_faces = F;
+    p.dt = probs[0]->dt;
+    p.duration = probs[0]->duration;
+    for (int32_t k = 0; k < n; ++k) p.n_vols += probs[k]->n_vols;
+    tf_dga_ops& d = p.dga;
+    d.C = csr_view(F, E, m.c_ptr, m.c_col, &m.c_sg, nullptr);
+    d.Ct = csr_view(E, F, m.ct_ptr, m.ct_col, &m.ct_sg, nullptr);
+    d.H = csr_view(hoff, E, m.h_ptr, m.h_col, nullptr, &m.h_val);
+    d.N = csr_view(F, F, m.n_ptr, m.n_col, nullptr, &m.n_val);
+    d.h_row = m.h_row.data();
+    d.bnd_edge = m.bnd_edge.data();
+    d.n_bnd = boff;
+    d.edge_class = m.edge_class.data();
+    d.fixed_value = m.fixed_value.data();
+}
+
+// Rebuilds the merged source and probe tables of a batch from its
+// per-instance lists (edge, face and source indices offset per instance).
+void batch_sources(tf_ctx* c) {
+    auto& b = *c->batch;
+    std::vector<std::vector<std::pair<int32_t, double>>> by_edge(c->E);
+    std::vector<int> fld(c->E, -1);
+    c->src.clear();
+    c->src_dt.clear();
+    for (int32_t k = 0; k < b.n; ++k) {
+        const int32_t so = (int32_t)c->src.size();
+        for (const tf_source& s : b.src[k]) {
+            c->src.push_back(s);
+            c->src_dt.push_back(b.dt[k]);
+        }
+        for (const tf_source_term& t : b.terms[k]) {
+            const int32_t e = t.edge + b.eoff[k];
+            by_edge[e].push_back({t.source + so, t.weight});
+            if (fld[e] < 0) fld[e] = b.src[k][t.source].field;
+        }
+    }
+    for (auto& f : fld)
+        if (f < 0) f = TF_FIELD_E;
+    build_tet_inject(c, by_edge, fld);
+}
+
+// ------------------------------------------------------------- autotuner
+//
+// For each SELL kernel of the context: candidate grid sizes (one slice per
+// wave, 1/2, 1/4, 1/8 of that, and 256..8192 blocks) x gather batches
+// {4, 8, 16}; each candidate is launched twice to warm up and timed over
+// five launches with HIP events; the fastest is kept. Results are cached per
+// (kernel, precision, rows, entries) in the process and, when the
+// environment names a file in TETFIT_TUNE_CACHE, in that file.
+std::map<std::string, KTune> g_tune_cache;
+bool g_tune_loaded = false;
+
+std::string tune_key(const tf_ctx* c, int k, const Sell& m) {
+    return std::string(kTuneName[k]) + "|" + (c->prec == TF_PREC_FP64 ? "f64" : "f32") + "|" +
+           std::to_string(m.nrows) + "|" + std::to_string(m.entries);
+}
+
+void tune_cache_load() {
+    if (g_tune_loaded) return;
+    g_tune_loaded = true;
+    const char* path = std::getenv("TETFIT_TUNE_CACHE");
+    if (!path) return;
+    FILE* f = std::fopen(path, "r");
+    if (!f) return;
+    char key[256];
+    int b = 0, u = 0;
+    while (std::fscanf(f, "%255s %d %d", key, &b, &u) == 3) g_tune_cache[key] = KTune{b, u};
+    std::fclose(f);
+}
+
+void tune_cache_store(const std::string& key, const KTune& t) {
+    g_tune_cache[key] = t;
+    const char* path = std::getenv("TETFIT_TUNE_CACHE");
+    if (!path) return;
+    FILE* f = std::fopen(path, "a");
+    if (!f) return;
+    std::fprintf(f, "%s %d %d\n", key.c_str(), t.blocks, t.unroll);
+    std::fclose(f);
+}
+
+template <class L>
+KTune tune_one(tf_ctx* c, int k, const Sell& m, int cap, L launch) {
+    tune_cache_load();
+    const std::string key = tune_key(c, k, m);
+    auto it = g_tune_cache.find(key);
+    if (it != g_tune_cache.end()) return it->second;
+    const int need = sell_blocks(m, INT32_MAX);
+    std::vector<int> cand;
+    for (int b : {need, (need + 1) / 2, (need + 3) / 4, (need + 7) / 8, 256, 512, 1024, 2048, 4096, 8192})
+        if (b >= 1 && b <= std::min(need, cap)) cand.push_back(b);
+    std::sort(cand.begin(), cand.end());
+    cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
+    hipEvent_t e0, e1;
+    TF_HIP(hipEventCreate(&e0));
+    TF_HIP(hipEventCreate(&e1));
+    KTune best{sell_blocks(m, cap), 8};
+    float best_ms = INFINITY;
+    for (int b : cand)
+        for (int u : {4, 8, 16}) {
+            launch(b, u);
+            launch(b, u);
+            TF_HIP(hipEventRecord(e0, c->stream));
+            for (int r = 0; r < 5; ++r) launch(b, u);
+            TF_HIP(hipEventRecord(e1, c->stream));
+            TF_HIP(hipEventSynchronize(e1));
+            float ms = 0;
+            TF_HIP(hipEventElapsedTime(&ms, e0, e1));
+            if (ms < best_ms) { best_ms = ms; best = KTune{b, u}; }
+        }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    TF_HIP(hipGetLastError());
+    tune_cache_store(key, best);
+    return best;
+}
+
+// Launches used for tuning leave the simulation state as it was: Ct F only
+// rewrites curl_f (recomputed first in every dga step), H and N updates run
+// with dt = 0, F = N B (gfdtd) rewrites F with the value it already has,
+// and the CG kernels run on a scratch state with the outputs in scratch
+// vectors (w, u), which every solve recomputes before use.
+template <class T>
+void autotune(tf_ctx* c) {
+    hipStream_t s = c->stream;
+    if (c->method == TF_METHOD_DGA) {
+        c->tune[TK_CT] = tune_one(c, TK_CT, c->sCt, kSpmvCap, [&](int b, int u) {
+            TF_LAUNCH_NB(u, sell_ct_kernel, T, b, s, c->E, c->sCt.nslices, c->sCt.sptr.as<int64_t>(),
+                         c->sCt.col.as<int32_t>(), c->sCt.sg.as<int8_t>(), c->Fv.as<T>(), c->curl_f.as<T>());
+        });
+        c->tune[TK_H] = tune_one(c, TK_H, c->H, kSpmvCap, [&](int b, int u) {
+            TF_LAUNCH_NB(u, sell_h_kernel, T, b, s, c->H.nrows, c->H.nslices, c->H.sptr.as<int64_t>(),
+                         c->H.col.as<int32_t>(), c->H.val.as<T>(), c->h_edge.as<int32_t>(), c->curl_f.as<T>(),
+                         c->U.as<T>(), 0.0, nullptr);
+        });
+        c->tune[TK_N] = tune_one(c, TK_N, c->N, kSpmvCap, [&](int b, int u) {
+            TF_LAUNCH_NB(u, sell_n_kernel, T, b, s, c->N.nrows, c->N.nslices, c->N.sptr.as<int64_t>(),
+                         c->N.col.as<int32_t>(), c->N.val.as<T>(), c->curl_u.as<T>(), c->Fv.as<T>(), 0.0, 0,
+                         nullptr);
+        });
+    } else if (c->method == TF_METHOD_GFDTD) {
+        c->tune[TK_N] = tune_one(c, TK_N, c->N, kSpmvCap, [&](int b, int u) {
+            TF_LAUNCH_NB(u, sell_n_kernel, T, b, s, c->N.nrows, c->N.nslices, c->N.sptr.as<int64_t>(),
+                         c->N.col.as<int32_t>(), c->N.val.as<T>(), c->B.as<T>(), c->Fv.as<T>(), c->dt, 1,
+                         nullptr);
+        });
+        DevBuf scratch;
+        scratch.alloc(sizeof(CgState));
+        TF_HIP(hipMemsetAsync(scratch.p, 0, sizeof(CgState), s));
+        double* part = c->cg_part.as<double>();
+        const int32_t nf = c->n_free;
+        c->tune[TK_W] = tune_one(c, TK_W, c->A, kCgPartA, [&](int b, int u) {
+            TF_LAUNCH_NB(u, cg_w_kernel, T, b, s, 1, nf, c->A.nslices, c->A.sptr.as<int64_t>(),
+                         c->A.col.as<int32_t>(), c->A.val.as<T>(), c->z.as<T>(), c->q.as<T>(), 0, part, 1, part,
+                         scratch.as<CgState>());
+        });
+        if (c->lp.state == LearnedPc::kActive)
+            c->tune[TK_G] = tune_one(c, TK_G, c->A, kCgPartC, [&](int b, int u) {
+                TF_LAUNCH_NB(u, cg_g_kernel, T, b, s, 1, nf, c->A.nslices, c->A.sptr.as<int64_t>(),
+                             c->A.col.as<int32_t>(), c->lp.gval.as<float>(), c->r.as<T>(), c->z.as<T>(), 0, part,
+                             1, part, scratch.as<CgState>());
+            });
+        TF_HIP(hipStreamSynchronize(s));
+    }
+}
+
 }  // namespace
 
 // ====================================================================== C ABI
@@ -1291,44 +1641,28 @@ int32_t tf_device_count(void) {
 tf_status tf_create(const tf_problem* p, int32_t prec, tf_ctx** out) {
     if (!p || !out) { g_err = "null argument"; return TF_ERR_ARG; }
     *out = nullptr;
+    return guarded([&] { *out = make_ctx(p, prec).release(); });
+}
+
+tf_status tf_create_batch(const tf_problem* const* probs, int32_t n, int32_t prec, tf_ctx** out) {
+    if (!probs || !out || n < 1) { g_err = "bad argument"; return TF_ERR_ARG; }
+    *out = nullptr;
     return guarded([&] {
-        if (p->method < 0 || p->method > 2) throw std::runtime_error("unknown method");
-        if (prec != TF_PREC_FP64 && prec != TF_PREC_FP32) throw std::runtime_error("unknown precision");
-        if (p->n_edges <= 0 || p->n_faces <= 0) throw std::runtime_error("empty problem");
-        int ndev = 0;
-        TF_HIP(hipGetDeviceCount(&ndev));
-        if (ndev < 1) throw HipError("no HIP device", TF_ERR_HIP);
-        TF_HIP(hipSetDevice(0));
-        auto c = std::make_unique<tf_ctx>();
-        c->method = p->method;
-        c->prec = prec;
-        c->E = p->n_edges;
-        c->F = p->n_faces;
-        c->dt = p->dt;
-        c->tsize = prec == TF_PREC_FP64 ? 8 : 4;
-        TF_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-        c->U.alloc(c->E * c->tsize);
-        c->Fv.alloc(c->F * c->tsize);
-        c->B.alloc(c->F * c->tsize);
-        TF_HIP(hipMemsetAsync(c->U.p, 0, c->U.bytes, c->stream));
-        TF_HIP(hipMemsetAsync(c->Fv.p, 0, c->Fv.bytes, c->stream));
-        TF_HIP(hipMemsetAsync(c->B.p, 0, c->B.bytes, c->stream));
-        if (p->method == TF_METHOD_FDTD) {
-            if (prec == TF_PREC_FP64) create_fit<double>(c.get(), p);
-            else create_fit<float>(c.get(), p);
-            std::vector<std::vector<std::pair<int32_t, double>>> none;
-            std::vector<int> fld;
-            if (prec == TF_PREC_FP64) build_fit_special<double>(c.get(), none, fld);
-            else build_fit_special<float>(c.get(), none, fld);
-        } else {
-            if (prec == TF_PREC_FP64) create_tet<double>(c.get(), p);
-            else create_tet<float>(c.get(), p);
-            std::vector<std::vector<std::pair<int32_t, double>>> none(c->E);
-            std::vector<int> fld(c->E, TF_FIELD_H);
-            build_tet_inject(c.get(), none, fld);
-        }
-        ensure_carr(c.get(), 1, 0);
-        TF_HIP(hipStreamSynchronize(c->stream));
+        MergedDga m;
+        merge_dga(probs, n, m);
+        auto c = make_ctx(&m.p, prec);
+        auto bt = std::make_unique<tf_ctx::Batch>();
+        bt->n = n;
+        bt->eoff = m.eoff;
+        bt->foff = m.foff;
+        bt->poff.assign(n + 1, 0);
+        bt->src.resize(n);
+        bt->terms.resize(n);
+        bt->probes.resize(n);
+        for (int32_t k = 0; k < n; ++k) bt->dt.push_back(probs[k]->dt);
+        upload(c->row_dt_h, m.h_dt);
+        upload(c->row_dt_f, m.f_dt);
+        c->batch = std::move(bt);
         *out = c.release();
     });
 }
@@ -1571,6 +1905,117 @@ tf_status tf_mlp_grad(const float* params, const float* x, const float* gy, int6
         TF_HIP(hipDeviceSynchronize());
         if (y && n) TF_HIP(hipMemcpy(y, dY.p, n * 4, hipMemcpyDeviceToHost));
         if (grad) TF_HIP(hipMemcpy(grad, dGrad.p, kMlpParams * 4, hipMemcpyDeviceToHost));
+    });
+}
+
+tf_status tf_autotune(tf_ctx* c) {
+    if (!c) { g_err = "null argument"; return TF_ERR_ARG; }
+    return guarded([&] {
+        if (c->prec == TF_PREC_FP64) autotune<double>(c);
+        else autotune<float>(c);
+    });
+}
+
+tf_status tf_get_tuning(tf_ctx* c, int32_t* blocks, int32_t* unroll, int32_t cap, int32_t* n) {
+    if (!c || !n) { g_err = "null argument"; return TF_ERR_ARG; }
+    *n = TK_COUNT;
+    for (int k = 0; k < std::min<int32_t>(cap, TK_COUNT); ++k) {
+        if (blocks) blocks[k] = c->tune[k].blocks;
+        if (unroll) unroll[k] = c->tune[k].unroll;
+    }
+    return TF_OK;
+}
+
+tf_status tf_batch_set_sources(tf_ctx* c, int32_t inst, const tf_source* src, int32_t n_src,
+                               const tf_source_term* terms, int64_t n_terms) {
+    if (!c || !c->batch || inst < 0 || inst >= c->batch->n || n_src < 0 || n_terms < 0 || (n_src && !src) ||
+        (n_terms && !terms)) {
+        g_err = "bad argument";
+        return TF_ERR_ARG;
+    }
+    return guarded([&] {
+        auto& b = *c->batch;
+        const int32_t E = b.eoff[inst + 1] - b.eoff[inst];
+        for (int64_t i = 0; i < n_terms; ++i)
+            if (terms[i].edge < 0 || terms[i].edge >= E || terms[i].source < 0 || terms[i].source >= n_src)
+                throw std::runtime_error("source term out of range");
+        b.src[inst].assign(src, src + n_src);
+        b.terms[inst].assign(terms, terms + n_terms);
+        batch_sources(c);
+    });
+}
+
+tf_status tf_batch_set_probes(tf_ctx* c, int32_t inst, const tf_probe* probes, int32_t n) {
+    if (!c || !c->batch || inst < 0 || inst >= c->batch->n || n < 0 || (n && !probes)) {
+        g_err = "bad argument";
+        return TF_ERR_ARG;
+    }
+    return guarded([&] {
+        auto& b = *c->batch;
+        const int32_t eo = b.eoff[inst], fo = b.foff[inst];
+        const int32_t E = b.eoff[inst + 1] - eo, F = b.foff[inst + 1] - fo;
+        std::vector<tf_probe> v(probes, probes + n);
+        for (tf_probe& p : v) {
+            if (p.ne < 0 || p.ne > TF_PROBE_MAX_E || p.nh < 0 || p.nh > TF_PROBE_MAX_H)
+                throw std::runtime_error("probe term count");
+            for (int k = 0; k < p.ne; ++k) {
+                if (p.e_idx[k] < 0 || p.e_idx[k] >= E) throw std::runtime_error("probe edge out of range");
+                p.e_idx[k] += eo;
+            }
+            for (int k = 0; k < p.nh; ++k) {
+                if (p.h_idx[k] < 0 || p.h_idx[k] >= F) throw std::runtime_error("probe face out of range");
+                p.h_idx[k] += fo;
+            }
+        }
+        b.probes[inst] = std::move(v);
+        std::vector<tf_probe> all;
+        b.poff.assign(1, 0);
+        for (int32_t k = 0; k < b.n; ++k) {
+            all.insert(all.end(), b.probes[k].begin(), b.probes[k].end());
+            b.poff.push_back((int32_t)all.size());
+        }
+        b.sample_steps.clear();
+        const tf_status rc = tf_set_probes(c, all.data(), (int32_t)all.size());
+        if (rc != TF_OK) throw std::runtime_error(g_err);
+    });
+}
+
+tf_status tf_batch_set_output(tf_ctx* c, int64_t every) {
+    if (!c || !c->batch || every < 0) { g_err = "bad argument"; return TF_ERR_ARG; }
+    c->batch->every = every;
+    return TF_OK;
+}
+
+tf_status tf_batch_read_probes(tf_ctx* c, int32_t inst, double* times, double* vals, int64_t cap, int64_t* n) {
+    if (!c || !c->batch || !n || inst < 0 || inst >= c->batch->n) { g_err = "bad argument"; return TF_ERR_ARG; }
+    return guarded([&] {
+        auto& b = *c->batch;
+        TF_HIP(hipStreamSynchronize(c->stream));
+        *n = c->n_samples;
+        const int64_t m = std::min(cap, c->n_samples);
+        if (times)
+            for (int64_t k = 0; k < m; ++k) times[k] = (double)b.sample_steps[k] * b.dt[inst];
+        const int32_t P = c->n_probes, p0 = b.poff[inst], np = b.poff[inst + 1] - p0;
+        if (vals && m > 0 && np > 0) {
+            std::vector<double> all((size_t)m * P * 6);
+            TF_HIP(hipMemcpy(all.data(), c->samples.p, all.size() * sizeof(double), hipMemcpyDeviceToHost));
+            for (int64_t k = 0; k < m; ++k)
+                std::memcpy(vals + (size_t)k * np * 6, all.data() + ((size_t)k * P + p0) * 6, (size_t)np * 6 * 8);
+        }
+    });
+}
+
+tf_status tf_batch_read_state(tf_ctx* c, int32_t inst, double* U, double* F, double* B) {
+    if (!c || !c->batch || inst < 0 || inst >= c->batch->n) { g_err = "bad argument"; return TF_ERR_ARG; }
+    return guarded([&] {
+        auto& b = *c->batch;
+        std::vector<double> u(c->E), f(c->F), bb(c->F);
+        const tf_status rc = tf_read_state(c, u.data(), f.data(), bb.data());
+        if (rc != TF_OK) throw std::runtime_error(g_err);
+        const int32_t eo = b.eoff[inst], fo = b.foff[inst];
+        if (U) std::copy(u.begin() + eo, u.begin() + b.eoff[inst + 1], U);
+        if (F) std::copy(f.begin() + fo, f.begin() + b.foff[inst + 1], F);
+        if (B) std::copy(bb.begin() + fo, bb.begin() + b.foff[inst + 1], B);
     });
 }
 

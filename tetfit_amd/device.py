@@ -126,6 +126,85 @@ class Simulation:
         _check(hip_lib().tf_device_bytes(self._h, C.byref(b)))
         return b.value
 
+    def autotune(self):
+        """tf_autotune; returns {kernel: (blocks, unroll)}."""
+        L = hip_lib()
+        _check(L.tf_autotune(self._h))
+        bl, un, n = (C.c_int32 * 8)(), (C.c_int32 * 8)(), C.c_int32()
+        _check(L.tf_get_tuning(self._h, bl, un, 8, C.byref(n)))
+        names = ["sell_ct_kernel", "sell_h_kernel", "sell_n_kernel", "cg_w_kernel", "cg_g_kernel"]
+        return {names[k]: (bl[k], un[k]) for k in range(min(n.value, len(names)))}
+
+
+class BatchSimulation:
+    """One device context stepping many independent dga simulations together
+    (tf_create_batch): operators concatenated block-diagonally, one launch per
+    phase for the whole batch, each instance on its own dt and sources."""
+
+    def __init__(self, builds, precision="fp32", sample_every=0):
+        L = hip_lib()
+        self.builds = list(builds)
+        arr = (C.POINTER(_ffi.tf_problem) * len(self.builds))(*[b.problem_ptr for b in self.builds])
+        h = C.c_void_p()
+        _check(L.tf_create_batch(arr, len(self.builds), PREC[precision], C.byref(h)))
+        self._h = h
+        self.n_probes = []
+        for k, b in enumerate(self.builds):
+            s, ns, t, nt = b.sources_raw()
+            _check(L.tf_batch_set_sources(h, k, s, ns, t, nt))
+            p, _, npr = b.probes_raw()
+            _check(L.tf_batch_set_probes(h, k, p, npr))
+            self.n_probes.append(npr)
+        _check(L.tf_batch_set_output(h, int(sample_every)))
+        self.next_step = 1
+
+    def close(self):
+        if getattr(self, "_h", None):
+            hip_lib().tf_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def run(self, n, first=None):
+        first = self.next_step if first is None else first
+        st = _ffi.tf_stats()
+        _check(hip_lib().tf_run(self._h, int(first), int(n), None, C.byref(st)))
+        self.next_step = first + n
+        return dict(steps=st.steps_done, seconds=st.seconds)
+
+    def state(self, k):
+        b = self.builds[k]
+        U, F, B = np.zeros(b.n_edges), np.zeros(b.n_faces), np.zeros(b.n_faces)
+        dp = C.POINTER(C.c_double)
+        _check(hip_lib().tf_batch_read_state(self._h, k, U.ctypes.data_as(dp), F.ctypes.data_as(dp),
+                                             B.ctypes.data_as(dp)))
+        return U, F, B
+
+    def probes(self, k):
+        L = hip_lib()
+        n = C.c_int64()
+        _check(L.tf_batch_read_probes(self._h, k, None, None, 0, C.byref(n)))
+        m, npr = n.value, self.n_probes[k]
+        times = np.zeros(m)
+        vals = np.zeros(max(m * npr * 6, 1))
+        dp = C.POINTER(C.c_double)
+        _check(L.tf_batch_read_probes(self._h, k, times.ctypes.data_as(dp), vals.ctypes.data_as(dp), m,
+                                      C.byref(n)))
+        return times, vals[:m * npr * 6].reshape(m, npr, 6)
+
+    def set_profiling(self, on=True):
+        _check(hip_lib().tf_set_profiling(self._h, 1 if on else 0))
+
+    def kernel_times(self):
+        return Simulation.kernel_times(self)
+
+    def autotune(self):
+        return Simulation.autotune(self)
+
+    def device_bytes(self):
+        return Simulation.device_bytes(self)
+
 
 def mlp_grad(params, x, gy):
     """Device forward + backward of the preconditioner MLP (tf_mlp_grad):
