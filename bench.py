@@ -39,10 +39,16 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def build_workload(d, method, n, steps):
+def build_workload(d, method, n, steps, contrast=False):
+    """The 48^3 workload (block 8, seed 2, eps_r U[1,10]); contrast=True is the
+    high-contrast variant (6^3-cube blocks, seed 5, eps_r log-uniform [1,80])."""
     from tetfit_amd import Model, configs
-    path = configs.kuhn_cavity(d, n=n, method=method, n_steps=steps, n_probes=8, block=8, seed=2,
-                               dt=DT48.get(method) if n == 48 else None)
+    if contrast:
+        path = configs.kuhn_cavity(d, n=n, method=method, n_steps=steps, n_probes=8, block=6, seed=5,
+                                   eps_range=(1.0, 80.0), log_eps=True, name=f"kuhn{n}_{method}_contrast")
+    else:
+        path = configs.kuhn_cavity(d, n=n, method=method, n_steps=steps, n_probes=8, block=8, seed=2,
+                                   dt=DT48.get(method) if n == 48 else None)
     t = time.time()
     b = Model(path).build(1)
     log(f"[bench] built {method} {n}^3: E={b.n_edges} F={b.n_faces} T={b.n_vols} dt={b.dt:.4g} "
@@ -145,6 +151,142 @@ def batch_workload(d, count=256, steps=2000, seed=4, workers=16):
     return builds
 
 
+def fit_bytes(b, kernel, vb):
+    """Algorithmic bytes per launch of the FIT kernels: E phase reads U, ca,
+    cb, the packed faces (16 B) and sign byte per edge, writes U, gathers F
+    once; H phase reads B, cm, cn, packed edges and signs per face, writes B
+    and F, gathers U once."""
+    E, F = b.n_edges, b.n_faces
+    if kernel == "fit_e_kernel":
+        return E * (4 * vb + 17) + F * vb
+    return F * (5 * vb + 17) + E * vb
+
+
+def bench_fit(args):
+    """FIT/FDTD cavity: 64^3 cells of 0.0156 m (xgrid {0,0.0156,1}), PEC on
+    5 walls, wave-profile h source on x-min, 32 probes; fp64 and fp32."""
+    from tetfit_amd import Model, configs
+    from tetfit_amd.device import Simulation
+    steps = args.steps if args.steps_given else 2000
+    with tempfile.TemporaryDirectory() as d:
+        t = time.time()
+        b = Model(configs.fit_cavity(d, n=64, step=0.0156, n_steps=steps + args.warmup + 60, n_probes=32)).build(1)
+        log(f"[bench] built fdtd 64^3: E={b.n_edges} F={b.n_faces} dt={b.dt:.4g} ({time.time() - t:.1f}s)")
+        res = {}
+        for prec in ("fp64", "fp32"):
+            sim = Simulation(b, prec, output=False)
+            sim.run(n=args.warmup)
+            st = sim.run(n=steps)
+            sim.set_profiling(True)
+            sim.run(n=50)
+            kern = {k: {"ms_avg": v[0], "launches": v[1]} for k, v in sim.kernel_times().items()}
+            sim.close()
+            res[prec] = dict(ms_per_step=1e3 * st["seconds"] / steps, edge_dof_per_s=b.n_edges * steps / st["seconds"],
+                             kernels=kern)
+            log(f"[bench] fdtd {prec}: {res[prec]['ms_per_step']:.3f} ms/step, "
+                f"{res[prec]['edge_dof_per_s'] / 1e9:.3f} G edge-DoF/s")
+        roof = {}
+        for kname in ("fit_e_kernel", "fit_h_kernel"):
+            k = res["fp64"]["kernels"].get(kname)
+            if k:
+                nb = fit_bytes(b, kname, 8)
+                ach = nb / (k["ms_avg"] * 1e-3) / 1e9
+                roof[kname] = dict(bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                                   frac=round(ach / HBM_PEAK_GBS, 4), traffic=None, bytes_per_launch=int(nb),
+                                   ms_per_launch=k["ms_avg"])
+        cpu = None
+        if not args.no_cpu:
+            sys.path.insert(0, os.path.join(ROOT, "oracle"))
+            import pyoracle
+            r = pyoracle.run(b, n=args.cpu_steps * 4, faithful=True, output=False)
+            cpu = dict(value=b.n_edges * args.cpu_steps * 4 / r["seconds"], unit="edge-DoF/s", cores=1, kind="port",
+                       sample=f"{args.cpu_steps * 4} fdtd steps, loop-faithful single-thread fp64 restatement, "
+                              f"{r['seconds']:.2f} s")
+        line = {"metric": "edge-DoF updates/s, FIT/FDTD 64^3, 1 GPU", "value": res["fp64"]["edge_dof_per_s"],
+                "unit": "edge-DoF/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
+                "ms_per_step": res["fp64"]["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+                "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+                "config": {"workload": "FIT 64^3 cavity, 0.0156 m cells, PEC on 5 walls, h source on x-min",
+                           "n_edges": b.n_edges, "n_faces": b.n_faces},
+                "fp32": {"edge_dof_per_s": res["fp32"]["edge_dof_per_s"], "ms_per_step": res["fp32"]["ms_per_step"]},
+                "roofline": roof.get("fit_e_kernel"), "roofline_h": roof.get("fit_h_kernel"), "cpu_baseline": cpu,
+                "kernels": res["fp64"]["kernels"]}
+        print(json.dumps(line), flush=True)
+
+
+def bench_fitprec(args):
+    """Precision study: FIT n^3 random media (eps_r U[1,4) per cell, seed 3),
+    sigma 0.05 S/m slab 0.4 <= x < 0.6, gaussmod carrier at 3e8 Hz, 16
+    probes; the same steps in fp32 and fp64 on the GPU; errors of fp32 vs the
+    fp64 run per probe (relative L2 over time), globally over U, F, B at the
+    end, and of the fp64-reduced energy series."""
+    from tetfit_amd import Model, configs
+    from tetfit_amd.device import Simulation
+    steps = args.steps if args.steps_given else 4000
+    n = args.n if args.n != 48 else 256
+    chunk = 100
+    with tempfile.TemporaryDirectory() as d:
+        t = time.time()
+        b = Model(configs.fit_random_media(d, n=n, n_steps=steps, n_probes=16)).build(1)
+        log(f"[bench] built fdtd random media {n}^3: E={b.n_edges} F={b.n_faces} dt={b.dt:.4g} "
+            f"({time.time() - t:.1f}s host set-up)")
+        out = {}
+        for prec in ("fp64", "fp32"):
+            sim = Simulation(b, prec)
+            secs, energy = 0.0, []
+            done = 0
+            while done < steps:
+                k = min(chunk, steps - done)
+                st = sim.run(n=k)
+                secs += st["seconds"]
+                done += k
+                energy.append(sim.energy())
+            _, samples = sim.probes()
+            U, F, B = sim.state()
+            sim.close()
+            out[prec] = dict(samples=samples, U=U, F=F, B=B, energy=np.array(energy), seconds=secs)
+            log(f"[bench] fitprec {prec}: {1e3 * secs / steps:.3f} ms/step")
+        ref, got = out["fp64"], out["fp32"]
+        rl2 = lambda a, r: float(np.linalg.norm(a - r) / max(np.linalg.norm(r), 1e-300))
+        per_probe = [rl2(got["samples"][:, p], ref["samples"][:, p]) for p in range(ref["samples"].shape[1])]
+        W64 = ref["energy"].sum(axis=1)
+        W32 = got["energy"].sum(axis=1)
+        line = {"metric": "fp32 vs fp64 relative L2 error, FIT random media", "value": max(per_probe),
+                "unit": "relative L2 (max over probes)", "n_gpus": 1, "steps": steps, "warmup": 0,
+                "ms_per_step": 1e3 * got["seconds"] / steps, "higher_is_better": False, "scaling": "weak",
+                "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+                "config": {"workload": f"FIT {n}^3 random media, eps_r U[1,4) per cell, sigma 0.05 slab, gaussmod 3e8 Hz",
+                           "n_edges": b.n_edges, "n_faces": b.n_faces},
+                "per_probe_rel_l2": per_probe, "global_rel_l2": {"U": rl2(got["U"], ref["U"]), "F": rl2(got["F"], ref["F"]),
+                                                                 "B": rl2(got["B"], ref["B"])},
+                "energy_rel_diff_max": float(np.abs(W32 - W64).max() / max(np.abs(W64).max(), 1e-300)),
+                "energy_fp64_final": float(W64[-1]), "energy_fp64_max": float(W64.max()),
+                "edge_dof_per_s": {"fp32": b.n_edges * steps / got["seconds"], "fp64": b.n_edges * steps / ref["seconds"]},
+                "roofline": None, "cpu_baseline": None}
+        print(json.dumps(line), flush=True)
+
+
+def bench_copy(args):
+    """1 GiB device-to-device copy with torch (one elementwise copy kernel of
+    known size): the reference for scaling FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    n = 1 << 28
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(args.steps):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / args.steps
+    print(json.dumps({"workload": "copy", "bytes_read": 4 * n, "bytes_written": 4 * n, "ms": ms,
+                      "GBps": 8 * n / (ms * 1e-3) / 1e9}), flush=True)
+
+
 def bench_batch(args):
     from tetfit_amd.device import BatchSimulation
     steps = args.steps if args.steps_given else 2000
@@ -202,9 +344,15 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-dga", action="store_true")
-    ap.add_argument("--workload", choices=["gfdtd", "batch"], default="gfdtd",
-                    help="gfdtd: the 48^3 headline; batch: batched dga instances (own JSON line)")
+    ap.add_argument("--workload", choices=["gfdtd", "contrast", "batch", "fit", "fitprec", "copy"], default="gfdtd",
+                    help="gfdtd: the 48^3 headline; contrast: gfdtd on the high-contrast Kuhn mesh (--n, default "
+                         "96); batch: batched dga instances; fit: FIT 64^3; fitprec: FIT random-media fp32 vs "
+                         "fp64 study (--n, default 256); copy: a 1 GiB device copy (the reference kernel for "
+                         "HBM counter scaling). Every workload but gfdtd prints its own JSON line.")
     ap.add_argument("--batch", type=int, default=256, help="instances of the batch workload")
+    ap.add_argument("--paths", default="plain64,jacobi32,learned32,learned64",
+                    help="gfdtd paths to run (profiling runs pick one)")
+    ap.add_argument("--learn-steps", type=int, default=200, help="collection solves of the learned paths")
     args = ap.parse_args()
     args.steps_given = any(a.startswith("--steps") for a in sys.argv[1:])
     if args.gpus != 1:
@@ -218,18 +366,38 @@ def main():
     if args.workload == "batch":
         bench_batch(args)
         return
+    if args.workload == "copy":
+        bench_copy(args)
+        return
+    if args.workload == "fit":
+        bench_fit(args)
+        return
+    if args.workload == "fitprec":
+        bench_fitprec(args)
+        return
+    contrast = args.workload == "contrast"
+    if contrast and args.n == 48:
+        args.n = 96
+    paths = set(args.paths.split(","))
     total = args.warmup + args.steps + 260
     with tempfile.TemporaryDirectory() as d:
-        bg = build_workload(d, "gfdtd", args.n, total)
-        plain = time_path(bg, "fp64", args.steps, args.warmup, "jacobi", "gfdtd fp64 jacobi (plain)")
-        headline_label = "gfdtd fp64 jacobi"
-        head = plain
-        f32j = time_path(bg, "fp32", args.steps, args.warmup, "jacobi", "gfdtd fp32 jacobi")
-        fast = time_path(bg, "fp32", args.steps, args.warmup, "learned", "gfdtd fp32 learned")
-        head, headline_label = fast, "gfdtd fp32 learned"
-        l64 = time_path(bg, "fp64", args.steps, args.warmup, "learned", "gfdtd fp64 learned")
+        bg = build_workload(d, "gfdtd", args.n, total + args.learn_steps, contrast)
+        none = dict(edge_dof_per_s=None, ms_per_step=None, cg_iters_per_step=None, cg_max_relres=None,
+                    cg_not_converged=None, kernels={})
+        plain = f32j = fast = l64 = none
+        if "plain64" in paths:
+            plain = time_path(bg, "fp64", args.steps, args.warmup, "jacobi", "gfdtd fp64 jacobi (plain)")
+        if "jacobi32" in paths:
+            f32j = time_path(bg, "fp32", args.steps, args.warmup, "jacobi", "gfdtd fp32 jacobi")
+        if "learned32" in paths:
+            fast = time_path(bg, "fp32", args.steps, args.warmup, "learned", "gfdtd fp32 learned",
+                             learn_steps=args.learn_steps)
+        if "learned64" in paths:
+            l64 = time_path(bg, "fp64", args.steps, args.warmup, "learned", "gfdtd fp64 learned",
+                            learn_steps=args.learn_steps)
+        head, headline_label = (fast, "gfdtd fp32 learned") if fast is not none else (plain, "gfdtd fp64 jacobi")
         dga = None
-        if not args.no_dga:
+        if not args.no_dga and not contrast:
             bd = build_workload(d, "dga", args.n, total)
             dga = time_path(bd, "fp64", args.steps, args.warmup, "jacobi", "dga fp64 (explicit)")
             dga32 = time_path(bd, "fp32", args.steps, args.warmup, "jacobi", "dga fp32 (explicit)")
@@ -261,13 +429,15 @@ def main():
             "vs_baseline": None,
             "dtype": "f32" if head is fast else "f64",
             "data": "synthetic",
-            "config": {"workload": f"Kuhn {args.n}^3 jittered tet cavity, method gfdtd (CG tol 1e-8)",
+            "config": {"workload": f"Kuhn {args.n}^3 jittered tet cavity, method gfdtd (CG tol 1e-8)"
+                                   + (", eps_r log-uniform [1,80] per 6^3 block" if contrast else ""),
                        "path": headline_label, "n_edges": bg.n_edges, "n_faces": bg.n_faces,
                        "parallelism": "single GPU"},
             "cg_iters_per_step": head["cg_iters_per_step"],
             "plain_fp64": {"edge_dof_per_s": plain["edge_dof_per_s"], "ms_per_step": plain["ms_per_step"],
                            "cg_iters_per_step": plain["cg_iters_per_step"]},
-            "speedup_vs_plain_fp64": head["edge_dof_per_s"] / plain["edge_dof_per_s"],
+            "speedup_vs_plain_fp64": (head["edge_dof_per_s"] / plain["edge_dof_per_s"]
+                                      if plain["edge_dof_per_s"] and head["edge_dof_per_s"] else None),
             "fp32_jacobi": {"edge_dof_per_s": f32j["edge_dof_per_s"], "cg_iters_per_step": f32j["cg_iters_per_step"],
                             "cg_max_relres": f32j["cg_max_relres"], "cg_not_converged": f32j["cg_not_converged"]},
             "fp64_learned": {"edge_dof_per_s": l64["edge_dof_per_s"], "cg_iters_per_step": l64["cg_iters_per_step"]},

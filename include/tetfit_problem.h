@@ -65,6 +65,10 @@ typedef struct {
     const double*  bc_value;      /* per bc edge: BCs[...].GetValue()         */
     const int32_t* src_edges;      int64_t n_src;       /* e / j source edges */
     const int32_t* tbc_faces;      int64_t n_tbc;       /* tbc_surfaces       */
+    /* energy diagnostic weights (D:3468-3470): We = 1/2 U.(ep_vec U),
+     * Wh = 1/2 B.(nu_vec B); ep_vec = eps_bar/L (D:5596), nu_vec = nu_bar/A (D:5507) */
+    const double*  ep_vec;
+    const double*  nu_vec;
 } tf_fit_ops;
 
 /* DGA operators (methods dga and gfdtd). */

@@ -198,3 +198,44 @@ def test_source_weights_closed_form(fit4):
         expect = math.sin(math.pi * e["bary"][2]) * e["vec"][1]
         assert abs(w - expect) <= 1e-15
         assert abs(abs(e["vec"][1]) - h) < 1e-15 or abs(e["vec"][1]) < 1e-15
+
+
+def cell_uniform(seed, index):
+    """The epsilon_random draw: SplitMix64 finalisation of seed*golden+index."""
+    m = (1 << 64) - 1
+    z = (seed * 0x9E3779B97F4A7C15 + index + 0x632BE59BD9B4E019) & m
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & m
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & m
+    z ^= z >> 31
+    return (z >> 11) * (1.0 / 9007199254740992.0)
+
+
+def test_random_media_and_lossy_coefficients(tmp_path):
+    """Per-cell eps (epsilon_random) and the lossy slab: for every edge,
+    eps_bar = sum over its cells of eps0*eps_r(cell)*h^2/4 and sigma_bar the
+    same over the slab cells; lossless edges M_q = eps_bar/h, lossy edges
+    M_h = h/(eps_bar + dt sigma_bar/2), M_q = (eps_bar - dt sigma_bar/2)/h
+    (D:5581-5586)."""
+    from tetfit_amd import configs as cf
+    n, seed, lo, hi, sig = 6, 7, 1.0, 4.0, 0.05
+    b = Model(cf.fit_random_media(str(tmp_path), n=n, n_steps=4, n_probes=1, eps_range=(lo, hi), seed=seed,
+                                  sigma=sig, slab=(0.35, 0.65))).build(1)
+    f, t = b.fit_ops(), b.fit_tables()
+    h = 1.0 / n
+    ebar = np.zeros(b.n_edges)
+    sbar = np.zeros(b.n_edges)
+    for c, edges in enumerate(t["e_cluster"]):
+        i = c % n
+        er = lo + (hi - lo) * cell_uniform(seed, c)
+        xc = (i + 0.5) * h
+        for e in edges:
+            ebar[e] += EPS0 * er * h * h / 4
+            if 0.35 <= xc < 0.65:
+                sbar[e] += sig * h * h / 4
+    lossy = sbar > 0
+    assert lossy.any() and (~lossy).any()
+    np.testing.assert_allclose(f["M_q"][~lossy], ebar[~lossy] / h, rtol=1e-13)
+    np.testing.assert_allclose(f["M_h"][lossy], h / (ebar[lossy] + 0.5 * b.dt * sbar[lossy]), rtol=1e-13)
+    np.testing.assert_allclose(f["M_q"][lossy], (ebar[lossy] - 0.5 * b.dt * sbar[lossy]) / h, rtol=1e-13)
+    # the draws really differ per cell
+    assert np.unique(np.round(f["M_q"][~lossy] * h / (EPS0 * h * h), 12)).size > 10

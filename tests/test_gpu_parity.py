@@ -126,3 +126,32 @@ def test_autotune_keeps_results_bitwise(workdir, method):
     Ut, Ft, Bt = t.state()
     t.close()
     assert np.array_equal(Ua, Ut) and np.array_equal(Fa, Ft) and np.array_equal(Ba, Bt)
+
+
+def test_fit_random_lossy_parity(workdir):
+    """Per-cell random eps_r in [1, 4), a sigma = 0.05 S/m slab (lossy M_h,
+    M_q) and the gaussmod carrier: the ingredients of the 512^3 precision
+    study at a size the checker runs in seconds. fp64 at the 1e-10 bar,
+    fp32 at the 1e-4 bar."""
+    b = Model(configs.fit_random_media(str(workdir), n=12, n_steps=160, n_probes=4, name="fitrand12")).build(1)
+    check_run(b)
+    err, _ = fp32_err(b)
+    assert err <= FP32_TOL
+
+
+def test_fit_energy_diagnostic(workdir):
+    """tf_energy (fp64 reductions on the device) equals 1/2 U.(ep U) and
+    1/2 B.(nu B) computed on the host from the read-back state (D:3468-3470),
+    for fp64 and fp32 fields."""
+    b = Model(configs.fit_cavity(str(workdir), n=10, step=0.1, n_steps=120, n_probes=2, name="fit_energy")).build(1)
+    f = b.fit_ops()
+    for prec in ("fp64", "fp32"):
+        print(f"case: energy {prec}", flush=True)
+        sim = Simulation(b, prec)
+        sim.run()
+        we, wh = sim.energy()
+        U, _, B = sim.state()
+        sim.close()
+        ref_we, ref_wh = 0.5 * U @ (f["ep_vec"] * U), 0.5 * B @ (f["nu_vec"] * B)
+        assert ref_we > 0 and ref_wh > 0
+        assert abs(we - ref_we) <= 1e-12 * ref_we and abs(wh - ref_wh) <= 1e-12 * ref_wh

@@ -31,7 +31,7 @@ class tf_fit_ops(C.Structure):
                 ("uncommon_edges", i32p), ("n_uncommon", C.c_int64),
                 ("bc_edges", i32p), ("n_bc", C.c_int64), ("bc_value", f64p),
                 ("src_edges", i32p), ("n_src", C.c_int64),
-                ("tbc_faces", i32p), ("n_tbc", C.c_int64)]
+                ("tbc_faces", i32p), ("n_tbc", C.c_int64), ("ep_vec", f64p), ("nu_vec", f64p)]
 
 
 class tf_dga_ops(C.Structure):

@@ -88,6 +88,55 @@ def fit_cavity(directory, n=64, step=0.0156, n_steps=2000, n_probes=32, dt=None,
     return fdtd
 
 
+def fit_random_media(directory, n=512, n_steps=4000, n_probes=16, eps_range=(1.0, 4.0), seed=3, sigma=0.05,
+                     slab=(0.4, 0.6), freq=3e8, name=None, dt=None):
+    """FIT unit box with n^3 cells of step 1/n: every cell its own eps_r ~
+    U[eps_range) (SET epsilon_random, seed), sigma S/m in the slab
+    slab[0] <= x < slab[1], PEC on 5 walls, a wave-profile h source on x-min
+    with the Gaussian-modulated sine carrier (gaussmod, f = freq, width
+    1.5/f, delay 3/f), probes on a line through the centre. Returns the
+    .fdtd path."""
+    os.makedirs(directory, exist_ok=True)
+    name = name or f"fitrand{n}"
+    step = 1.0 / n
+    stop = 1.0 + 0.25 * step
+    lo, hi = eps_range
+    solids = os.path.join(directory, f"{name}_solids.txt")
+    with open(solids, "w") as f:
+        f.write("DEFINE solid 1\n    SET type box\n    SET material 2\n"
+                "    SET corner {%r,0,0}\n    SET size {%r,%r,%r}\nEND solid 1\n"
+                % (slab[0], slab[1] - slab[0], stop, stop))
+        f.write("DEFINE solid 2\n    SET type box\n    SET material 1\n"
+                "    SET corner {0,0,0}\n    SET size {%r,%r,%r}\nEND solid 2\n" % (stop, stop, stop))
+    if dt is None:
+        # 0.95 x the Yee limit of the fastest (eps_r = lo) medium
+        dt = float("%.4g" % (0.95 * step * math.sqrt(lo) / (C0 * math.sqrt(3.0))))
+    j = n // 2
+    pts = [(((i + 0.5) / n_probes), (j + 0.5) * step, (j + 0.5) * step) for i in range(n_probes)]
+    extra = "    SET width %r\n    SET delay %r\n" % (1.5 / freq, 3.0 / freq)
+    fdtd = os.path.join(directory, f"{name}.fdtd")
+    with open(fdtd, "w") as f:
+        f.write(f"# FIT {n}^3 random media, eps_r U[{lo},{hi}) per cell (seed {seed}), "
+                f"sigma {sigma} in {slab[0]} <= x < {slab[1]}\n")
+        f.write("DEFINE mesh 1\n    SET type cartesian\n    SET file %s\n" % os.path.basename(solids))
+        for ax in "xyz":
+            f.write("    SET %sgrid {0,%r,%r}\n" % (ax, step, stop))
+        f.write("END mesh 1\n")
+        f.write("DEFINE material 1\n    SET epsilon_random {%r,%r,%d}\n    SET mu 1\nEND material 1\n"
+                % (lo, hi, seed))
+        f.write("DEFINE material 2\n    SET epsilon_random {%r,%r,%d}\n    SET mu 1\n    SET sigma %r\n"
+                "END material 2\n" % (lo, hi, seed, sigma))
+        f.write("DEFINE bc 1\n    SET type pec\n    SET surfaces {1,3,4,5,6}\nEND bc 1\n")
+        f.write(SOURCE_BLOCK.format(carrier="gaussmod", freq=repr(freq), extra=extra))
+        f.write("DEFINE output 1\n    SET mode probepoint\n    SET name %s\n    SET period 0\n" % name)
+        f.write(_probe_lines(pts))
+        f.write("END output 1\n")
+        f.write("DEFINE simulation 1\n    SET method fdtd\n    SET mesh 1\n    SET output 1\n"
+                "    SET courant 0.98\n    SET step %r\n    SET duration %r\nEND simulation 1\n"
+                % (dt, _duration(n_steps, dt, 0)))
+    return fdtd
+
+
 def kuhn_cavity(directory, n=48, method="dga", n_steps=1000, n_probes=8, jitter=0.2, seed=2,
                 block=8, eps_range=(1.0, 10.0), log_eps=False, freq=2e8, tol=1e-8, dt=None,
                 name=None, dt_safety=0.9, pec_surfaces="1,3,4,5,6", source_surface=2):

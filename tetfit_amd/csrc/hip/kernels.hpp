@@ -266,4 +266,20 @@ __global__ void __launch_bounds__(kBlock) gf_inject_kernel(
     if (threadIdx.x == 0) poy_part[blockIdx.x] = s;
 }
 
+// ------------------------------------------------------------------ energy
+// part[block] = sum over this block's entries of w_i x_i^2 (fp64).
+template <typename T>
+__global__ void __launch_bounds__(kBlock) weighted_sq_kernel(int64_t n, const T* __restrict__ x,
+                                                             const double* __restrict__ w,
+                                                             double* __restrict__ part) {
+    __shared__ double red[kBlock / kWave];
+    double acc = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const double v = (double)x[i];
+        acc += w[i] * v * v;
+    }
+    acc = block_sum(acc, red);
+    if (threadIdx.x == 0) part[blockIdx.x] = acc;
+}
+
 }  // namespace tfd

@@ -133,6 +133,7 @@ struct tf_ctx {
 
     // FIT
     DevBuf fe_sg, fe_ca, fe_cb, fe_f4, fh_sg, fh_cm, fh_cn, fh_e4;
+    DevBuf fit_ep, fit_nu;  // fp64 energy weights
     std::vector<uint8_t> h_fe_sg;
     std::vector<int32_t> fit_uncommon, fit_src;
     std::vector<int4> fit_sp_f4_u;
@@ -222,7 +223,7 @@ tf_ctx::~tf_ctx() {
 
 size_t tf_ctx::device_bytes() const {
     size_t s = 0;
-    const DevBuf* all[] = {&U, &Fv, &B, &fe_sg, &fe_ca, &fe_cb, &fe_f4, &fh_sg, &fh_cm, &fh_cn, &fh_e4,
+    const DevBuf* all[] = {&U, &Fv, &B, &fit_ep, &fit_nu, &fe_sg, &fe_ca, &fe_cb, &fe_f4, &fh_sg, &fh_cm, &fh_cn, &fh_e4,
                            &sp_edge, &sp_f4, &sp_sg, &sp_ca, &sp_cb, &sp_kind, &sp_tb, &sp_te, &c_col, &c_sg,
                            &h_edge, &curl_f, &curl_u, &free_edge, &dir_edge, &dir_new, &fb, &pre, &x, &r, &z,
                            &p0, &p1, &q, &rhs, &cg_part, &cg_state, &inj_edge, &inj_kind, &inj_tb, &inj_te,
@@ -368,6 +369,10 @@ template <class T>
 void create_fit(tf_ctx* c, const tf_problem* p) {
     const tf_fit_ops& f = p->fit;
     const int32_t E = c->E, F = c->F;
+    if (f.ep_vec && f.nu_vec) {
+        upload(c->fit_ep, std::vector<double>(f.ep_vec, f.ep_vec + E));
+        upload(c->fit_nu, std::vector<double>(f.nu_vec, f.nu_vec + F));
+    }
     const double dt = p->dt;
     std::vector<uint8_t> sg(E, 0);
     std::vector<T> ca(E, 0), cb(E, 0);
@@ -1829,10 +1834,32 @@ tf_status tf_read_poynting(tf_ctx* c, double* vals, int64_t cap, int64_t* n) {
 
 tf_status tf_energy(tf_ctx* c, double* we, double* wh) {
     if (!c) { g_err = "null argument"; return TF_ERR_ARG; }
-    g_err = "energy diagnostic is computed by the host from tf_read_state in this build";
-    (void)we;
-    (void)wh;
-    return TF_ERR_STATE;
+    if (c->method != TF_METHOD_FDTD || !c->fit_ep.p) {
+        g_err = "the device energy diagnostic covers fdtd contexts (dga/gfdtd: from tf_read_state on the host)";
+        return TF_ERR_STATE;
+    }
+    return guarded([&] {
+        const int nb = 1024;
+        DevBuf part;
+        part.alloc(2 * nb * sizeof(double));
+        double* pp = part.as<double>();
+        if (c->prec == TF_PREC_FP64) {
+            weighted_sq_kernel<double><<<nb, kBlock, 0, c->stream>>>(c->E, c->U.as<double>(), c->fit_ep.as<double>(), pp);
+            weighted_sq_kernel<double><<<nb, kBlock, 0, c->stream>>>(c->F, c->B.as<double>(), c->fit_nu.as<double>(),
+                                                                     pp + nb);
+        } else {
+            weighted_sq_kernel<float><<<nb, kBlock, 0, c->stream>>>(c->E, c->U.as<float>(), c->fit_ep.as<double>(), pp);
+            weighted_sq_kernel<float><<<nb, kBlock, 0, c->stream>>>(c->F, c->B.as<float>(), c->fit_nu.as<double>(),
+                                                                    pp + nb);
+        }
+        std::vector<double> h(2 * nb);
+        TF_HIP(hipMemcpyAsync(h.data(), pp, h.size() * 8, hipMemcpyDeviceToHost, c->stream));
+        TF_HIP(hipStreamSynchronize(c->stream));
+        double a = 0, b = 0;
+        for (int k = 0; k < nb; ++k) { a += h[k]; b += h[nb + k]; }
+        if (we) *we = 0.5 * a;
+        if (wh) *wh = 0.5 * b;
+    });
 }
 
 tf_status tf_kernel_times(tf_ctx* c, char* names, int32_t cap_names, double* ms_avg, int64_t* launches, int32_t cap,

@@ -51,7 +51,7 @@ struct Build {
     std::vector<int8_t> ct_sgn;
     std::vector<int32_t> c_edge;
     std::vector<int8_t> curl, dual_curl;
-    std::vector<double> M_h, M_q, upml, M_mu, M_nu, fpml, M_e;
+    std::vector<double> M_h, M_q, upml, M_mu, M_nu, fpml, M_e, Nu_vec;
     std::vector<int32_t> common_edges, uncommon_edges, bc_edges, src_edges, tbc_faces;
     std::vector<double> bc_value;
     std::vector<int32_t> e_cluster, d_faces, g_nodes;  // 12/cube, 6/cube, 2/edge
@@ -107,5 +107,15 @@ void build_tet(const Model& m, const SimulationDef& sim, Build& b);
 
 // Output::AllowPrint-style schedule parameters from the simulation's output.
 void set_output(const Model& m, const SimulationDef& sim, Build& b);
+
+// Uniform [0, 1) from (seed, index): one SplitMix64 finalisation of
+// seed * golden + index (order-independent per-cell draws).
+inline double cell_uniform(uint64_t seed, uint64_t index) {
+    uint64_t z = seed * 0x9E3779B97F4A7C15ull + index + 0x632BE59BD9B4E019ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+}
 
 }  // namespace tfh

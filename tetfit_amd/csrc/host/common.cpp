@@ -217,6 +217,8 @@ void Build::finalize_views() {
         f.n_src = (int64_t)src_edges.size();
         f.tbc_faces = tbc_faces.data();
         f.n_tbc = (int64_t)tbc_faces.size();
+        f.ep_vec = M_e.data();
+        f.nu_vec = Nu_vec.data();
     } else {
         tf_dga_ops& d = prob.dga;
         d = tf_dga_ops{};

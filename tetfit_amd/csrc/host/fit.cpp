@@ -220,7 +220,11 @@ void build_fit(const Model& m, const SimulationDef& sim, Build& b) {
                 // material averages (D:5244-5330)
                 const double mu[3] = {mt.mu(0, 0), mt.mu(1, 1), mt.mu(2, 2)};
                 const double ch[3] = {mt.chi(0, 0), mt.chi(1, 1), mt.chi(2, 2)};
-                const double ep[3] = {mt.epsilon(0, 0), mt.epsilon(1, 1), mt.epsilon(2, 2)};
+                double ep[3] = {mt.epsilon(0, 0), mt.epsilon(1, 1), mt.epsilon(2, 2)};
+                if (mt.eps_random) {
+                    const double er = mt.eps_lo + (mt.eps_hi - mt.eps_lo) * cell_uniform(mt.eps_seed, nv);
+                    ep[0] = ep[1] = ep[2] = EPSILON0 * er;
+                }
                 const double si[3] = {mt.sigma(0, 0), mt.sigma(1, 1), mt.sigma(2, 2)};
                 // faces by normal axis: x -> 2,3 ; y -> 1,4 ; z -> 0,5
                 const int fax[3][2] = {{2, 3}, {1, 4}, {0, 5}};
@@ -354,9 +358,10 @@ void build_fit(const Model& m, const SimulationDef& sim, Build& b) {
     b.prob.dt = t_step;
 
     // ---- coefficients (D:5487-5649). PML factors are 1 without PML.
-    b.M_nu.resize(totF); b.M_mu.resize(totF); b.fpml.assign(totF, 1.0);
+    b.M_nu.resize(totF); b.M_mu.resize(totF); b.fpml.assign(totF, 1.0); b.Nu_vec.assign(totF, 0.0);
     for (uint32_t f = 0; f < totF; ++f) {
         if (avg_ni[f] != 0) {
+            b.Nu_vec[f] = avg_ni[f] / face_area[f];
             if (mag_lossy[f]) {
                 b.M_nu[f] = 1 / (face_area[f] * (1 / avg_ni[f] + 0.5 * t_step / avg_mag_sigma[f]));
                 b.M_mu[f] = face_area[f] * (1 / avg_ni[f] - 0.5 * t_step / avg_mag_sigma[f]);

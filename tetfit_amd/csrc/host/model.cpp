@@ -113,6 +113,16 @@ void Material::set(uint32_t line, const std::string& p, const std::string& v) {
     else if (p == "mu_vec") diag_list(mu, MU0);
     else if (p == "sigma_vec") diag_list(sigma, 1.0);
     else if (p == "chi_vec") diag_list(chi, 1.0);
+    else if (p == "epsilon_random") {
+        auto l = parse_brace_list(line, v, false);
+        if (l.size() != 3 || !(l[0] > 0) || !(l[1] >= l[0]) || l[2] < 0)
+            input_error(line, "epsilon_random expects {lo,hi,seed} with 0 < lo <= hi, seed >= 0");
+        eps_random = true;
+        eps_lo = l[0];
+        eps_hi = l[1];
+        eps_seed = (uint64_t)l[2];
+        epsilon = Mat3::diag(EPSILON0 * eps_lo);
+    }
     else input_error(line, "Unrecognized material parameter! Available: epsilon, mu, sigma, chi");
 }
 

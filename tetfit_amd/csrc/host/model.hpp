@@ -78,6 +78,13 @@ struct Material {
     Mat3 mu = Mat3::diag(MU0);
     Mat3 sigma, chi;
     bool pml = true;
+    // Extension (no reference counterpart): `SET epsilon_random {lo,hi,seed}`
+    // gives every Cartesian cell of this material its own isotropic eps_r,
+    // uniform in [lo, hi), drawn from (seed, cell index) - per-cell random
+    // media that the reference's input language cannot express.
+    bool eps_random = false;
+    double eps_lo = 1, eps_hi = 1;
+    uint64_t eps_seed = 0;
     void set(uint32_t line, const std::string& p, const std::string& v);
 };
 

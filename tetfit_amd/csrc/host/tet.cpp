@@ -183,6 +183,7 @@ void build_tet(const Model& m, const SimulationDef& sim, Build& b) {
         Vec3 v1 = pts[n[1]] - pts[n[0]], v2 = pts[n[2]] - pts[n[0]], v3 = pts[n[3]] - pts[n[0]];
         b.cell_volumes[t] = dot(v1, cross(v2, v3)) / 6.0;
         const Material& mt = materials[tin[t].mat];
+        if (mt.eps_random) throw Error("epsilon_random is supported on cartesian meshes only");
         for (int k = 0; k < 4; ++k) {
             uint32_t p = n[k];
             if (!dvp_eps[p].is_zero()) {

@@ -100,6 +100,12 @@ class Simulation:
         _check(L.tf_read_probes(self._h, times.ctypes.data_as(dp), vals.ctypes.data_as(dp), k, C.byref(n)))
         return times, vals[:k * self.n_probes * 6].reshape(k, self.n_probes, 6)
 
+    def energy(self):
+        """(We, Wh) of the current state, fp64 reductions on the device (fdtd)."""
+        we, wh = C.c_double(), C.c_double()
+        _check(hip_lib().tf_energy(self._h, C.byref(we), C.byref(wh)))
+        return we.value, wh.value
+
     def poynting(self):
         L = hip_lib()
         n = C.c_int64()

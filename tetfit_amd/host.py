@@ -178,7 +178,8 @@ class Build:
             uncommon_edges=_arr(f.uncommon_edges, f.n_uncommon, np.int32),
             bc_edges=_arr(f.bc_edges, f.n_bc, np.int32), bc_value=_arr(f.bc_value, f.n_bc, np.float64),
             src_edges=_arr(f.src_edges, f.n_src, np.int32),
-            tbc_faces=_arr(f.tbc_faces, f.n_tbc, np.int32))
+            tbc_faces=_arr(f.tbc_faces, f.n_tbc, np.int32),
+            ep_vec=_arr(f.ep_vec, E, np.float64), nu_vec=_arr(f.nu_vec, F, np.float64))
 
     def fit_tables(self):
         L = host_lib()
