@@ -49,6 +49,20 @@ struct DevBuf {
 
 constexpr int kWave = 64;
 constexpr int kBlock = 256;
+constexpr int kXcd = 8;  // MI355X: 8 XCDs, each with its own L2
+
+// The hardware hands workgroup b to XCD b % 8. This maps the workgroup to a
+// logical block index such that XCD x holds one contiguous range of logical
+// blocks: XCD x < r gets [x(q+1), (x+1)(q+1)), XCD x >= r gets
+// [xq + r, (x+1)q + r), with q = grid / 8, r = grid % 8 (a bijection on
+// [0, grid)). Kernels that gather neighbours index rows by the logical block,
+// so each L2 caches the part of the gathered vector near its own rows.
+__device__ __forceinline__ int xcd_block() {
+    const int g = (int)gridDim.x, b = (int)blockIdx.x;
+    const int x = b % kXcd, i = b / kXcd;
+    const int q = g / kXcd, r = g % kXcd;
+    return x * q + (x < r ? x : r) + i;
+}
 
 inline int blocks_for(int64_t n, int per_block = kBlock) {
     int64_t b = (n + per_block - 1) / per_block;

@@ -45,8 +45,9 @@ __global__ void __launch_bounds__(kBlock) fit_e_kernel(
     const int8_t* __restrict__ sp_kind, const int32_t* __restrict__ sp_tb, const int32_t* __restrict__ sp_te,
     const int32_t* __restrict__ term_src, const double* __restrict__ term_w,
     const double* __restrict__ carr_step, double* __restrict__ poy_part) {
-    if ((int)blockIdx.x < main_blocks) {
-        const int32_t e = blockIdx.x * kBlock + threadIdx.x;
+    const int lb = xcd_block();
+    if (lb < main_blocks) {
+        const int32_t e = lb * kBlock + threadIdx.x;
         if (e >= n_edges) return;
         const uint32_t s = sg[e];
         if (s == 0) return;
@@ -57,7 +58,7 @@ __global__ void __launch_bounds__(kBlock) fit_e_kernel(
         return;
     }
     __shared__ double red[kBlock / kWave];
-    const int32_t i = (blockIdx.x - main_blocks) * kBlock + threadIdx.x;
+    const int32_t i = (lb - main_blocks) * kBlock + threadIdx.x;
     double poy = 0;
     if (i < n_sp) {
         const int32_t e = sp_edge[i];
@@ -83,7 +84,7 @@ __global__ void __launch_bounds__(kBlock) fit_e_kernel(
         if (k == 0) poy = 0.5 * (uold + (double)(T)unew) * src;
     }
     const double s = block_sum(poy, red);
-    if (threadIdx.x == 0) poy_part[blockIdx.x - main_blocks] = s;
+    if (threadIdx.x == 0) poy_part[lb - main_blocks] = s;
 }
 
 // H phase over tbc faces (D:1908-1915): B = cm*F_old - dt*sum s_k U[e_k];
@@ -93,7 +94,7 @@ __global__ void __launch_bounds__(kBlock) fit_h_kernel(int32_t n_faces, const ui
                                                        const T* __restrict__ cm, const T* __restrict__ cn,
                                                        const int4* __restrict__ e4, const T* __restrict__ U,
                                                        T* __restrict__ Fv, T* __restrict__ B, double dt) {
-    const int32_t f = blockIdx.x * kBlock + threadIdx.x;
+    const int32_t f = xcd_block() * kBlock + threadIdx.x;
     if (f >= n_faces) return;
     const uint32_t s = sg[f];
     if (s == 0) return;
@@ -183,7 +184,7 @@ __global__ void __launch_bounds__(kBlock) dga_curl_kernel(int32_t n_faces, const
                                                           const int8_t* __restrict__ c_sg, const T* __restrict__ U,
                                                           T* __restrict__ curl_u, T* __restrict__ B, double dt,
                                                           const double* __restrict__ face_dt) {
-    const int32_t f = blockIdx.x * kBlock + threadIdx.x;
+    const int32_t f = xcd_block() * kBlock + threadIdx.x;
     if (f >= n_faces) return;
     const int32_t* c = c_col + 3 * (int64_t)f;
     double cu;

@@ -75,13 +75,28 @@ __device__ __forceinline__ int wave_id() { return (blockIdx.x * blockDim.x + thr
 __device__ __forceinline__ int wave_count() { return gridDim.x * blockDim.x / kWave; }
 __device__ __forceinline__ int lane_id() { return threadIdx.x & (kWave - 1); }
 
+// Slices [s0, s1) of this workgroup: the logical blocks (xcd_block) split
+// the slices into contiguous chunks, so each XCD streams one contiguous
+// part of the matrix and its L2 holds the part of the gathered vector that
+// those rows touch. The waves of a block take the chunk's slices in turn.
+struct SliceRange {
+    int s, end, step;
+};
+__device__ __forceinline__ SliceRange block_slices(int nslices) {
+    const int lb = xcd_block();
+    const int s0 = (int)((int64_t)nslices * lb / gridDim.x);
+    const int s1 = (int)((int64_t)nslices * (lb + 1) / gridDim.x);
+    return SliceRange{s0 + (int)(threadIdx.x / kWave), s1, (int)(blockDim.x / kWave)};
+}
+
 // curl_f = Ct F over edges (dga, D:878-879).
 template <typename T, int NB>
 __global__ void __launch_bounds__(kBlock) sell_ct_kernel(int32_t nrows, int32_t nslices, const int64_t* __restrict__ sptr,
                                                          const int32_t* __restrict__ col, const int8_t* __restrict__ sg,
                                                          const T* __restrict__ x, T* __restrict__ y) {
     const int lane = lane_id();
-    for (int s = wave_id(); s < nslices; s += wave_count()) {
+    for (SliceRange sr = block_slices(nslices); sr.s < sr.end; sr.s += sr.step) {
+        const int s = sr.s;
         const int64_t b = sptr[s];
         const int w = (int)((sptr[s + 1] - b) / kSlice);
         const double v = sell_sgn_dot<T, NB>(b, w, col, sg, lane, x);
@@ -99,7 +114,8 @@ __global__ void __launch_bounds__(kBlock) sell_h_kernel(int32_t nrows, int32_t n
                                                         T* __restrict__ U, double dt,
                                                         const double* __restrict__ row_dt) {
     const int lane = lane_id();
-    for (int s = wave_id(); s < nslices; s += wave_count()) {
+    for (SliceRange sr = block_slices(nslices); sr.s < sr.end; sr.s += sr.step) {
+        const int s = sr.s;
         const int64_t b = sptr[s];
         const int w = (int)((sptr[s + 1] - b) / kSlice);
         const double v = sell_dot<T, NB>(b, w, col, val, lane, [&](int32_t j) { return x[j]; });
@@ -119,7 +135,8 @@ __global__ void __launch_bounds__(kBlock) sell_n_kernel(int32_t nrows, int32_t n
                                                         const T* __restrict__ x, T* __restrict__ Fv, double dt, int mode,
                                                         const double* __restrict__ row_dt) {
     const int lane = lane_id();
-    for (int s = wave_id(); s < nslices; s += wave_count()) {
+    for (SliceRange sr = block_slices(nslices); sr.s < sr.end; sr.s += sr.step) {
+        const int s = sr.s;
         const int64_t b = sptr[s];
         const int w = (int)((sptr[s + 1] - b) / kSlice);
         const double v = sell_dot<T, NB>(b, w, col, val, lane, [&](int32_t j) { return x[j]; });
@@ -141,7 +158,8 @@ __global__ void __launch_bounds__(kBlock) gf_rhs_kernel(
     const T* __restrict__ dir_new, const T* __restrict__ Fv, const T* __restrict__ fb, T* __restrict__ rhs,
     double dt) {
     const int lane = lane_id();
-    for (int s = wave_id(); s < nslices; s += wave_count()) {
+    for (SliceRange sr = block_slices(nslices); sr.s < sr.end; sr.s += sr.step) {
+        const int s = sr.s;
         const int64_t eb = e_ptr[s], rb = r_ptr[s], cb = c_ptr[s];
         const int ew = (int)((e_ptr[s + 1] - eb) / kSlice), rw = (int)((r_ptr[s + 1] - rb) / kSlice);
         const int cw = (int)((c_ptr[s + 1] - cb) / kSlice);
@@ -194,7 +212,8 @@ __global__ void __launch_bounds__(kBlock) cg_init_kernel(int32_t n, int32_t nsli
     __shared__ double red[kBlock / kWave];
     const int lane = lane_id();
     double rr = 0, ru = 0;
-    for (int s = wave_id(); s < nslices; s += wave_count()) {
+    for (SliceRange sr = block_slices(nslices); sr.s < sr.end; sr.s += sr.step) {
+        const int s = sr.s;
         const int64_t bs = sptr[s];
         const int w = (int)((sptr[s + 1] - bs) / kSlice);
         const double ax = sell_dot<T, 8>(bs, w, col, val, lane, [&](int32_t j) { return x[j]; });
@@ -234,7 +253,8 @@ __global__ void __launch_bounds__(kBlock) cg_g_kernel(int32_t it, int32_t n, int
     if (test_stop && cg_stop_test(it, part_rr, nb_rr, st, red)) return;
     const int lane = lane_id();
     double ru = 0;
-    for (int s = wave_id(); s < nslices; s += wave_count()) {
+    for (SliceRange sr = block_slices(nslices); sr.s < sr.end; sr.s += sr.step) {
+        const int s = sr.s;
         const int64_t bs = sptr[s];
         const int w = (int)((sptr[s + 1] - bs) / kSlice);
         const double uv = sell_dot<T, NB>(bs, w, col, gval, lane, [&](int32_t j) { return r[j]; });
@@ -262,7 +282,8 @@ __global__ void __launch_bounds__(kBlock) cg_w_kernel(int32_t it, int32_t n, int
     if (test_stop && cg_stop_test(it, part_rr, nb_rr, st, red)) return;
     const int lane = lane_id();
     double wu = 0;
-    for (int s = wave_id(); s < nslices; s += wave_count()) {
+    for (SliceRange sr = block_slices(nslices); sr.s < sr.end; sr.s += sr.step) {
+        const int s = sr.s;
         const int64_t bs = sptr[s];
         const int wd = (int)((sptr[s + 1] - bs) / kSlice);
         const double wv = sell_dot<T, NB>(bs, wd, col, val, lane, [&](int32_t j) { return u[j]; });

@@ -875,9 +875,13 @@ void cg_solve(tf_ctx* c, const CgState& init, bool learned, const std::vector<st
     ev_end(c);
     capture(0);
     int it = 0;
-    int batch = std::max(1, guess + 1);
+    // With per-kernel profiling on, passes go one at a time and the pass in
+    // which the stop test fires (its kernels return after the test) is timed
+    // under its own name, so kernel averages cover working launches only.
+    int batch = c->profiling ? 1 : std::max(1, guess + 1);
     for (;;) {
         const int stop = std::min(it + batch, init.maxiter + 1);
+        const size_t rec0 = c->recs.size();
         for (; it < stop; ++it) {
             if (learned) {
                 ev_begin(c, "cg_g_kernel");
@@ -903,8 +907,10 @@ void cg_solve(tf_ctx* c, const CgState& init, bool learned, const std::vector<st
         }
         TF_HIP(hipMemcpyAsync(c->h_state, dst, sizeof(CgState), hipMemcpyDeviceToHost, c->stream));
         TF_HIP(hipStreamSynchronize(c->stream));
+        if (c->profiling && c->h_state->done && c->h_state->iter == it - 1)
+            for (size_t k = rec0; k < c->recs.size(); ++k) c->recs[k].name = "cg_stop_pass";
         if (c->h_state->done || it > init.maxiter) break;
-        batch = 4;
+        batch = c->profiling ? 1 : 4;
     }
     guess = c->h_state->iter;
 }
