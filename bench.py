@@ -56,23 +56,38 @@ def build_workload(d, method, n, steps, contrast=False):
     return b
 
 
-def algorithmic_bytes(b, kernel, vb):
-    """Bytes a kernel must move per launch (operator arrays once, vectors
-    once per access pattern), vb = value bytes."""
-    d = b.dga_ops()
-    if kernel in ("cg_w_kernel", "cg_g_kernel"):
-        A = d["A"]
-        n, nnz = A["nrows"], int(A["rowptr"][-1])
-        # operator values + column indices once (cg_g: fp32 G values on A's
-        # pattern), slice pointers, the input vector once, the output once
-        ob = 4 if kernel == "cg_g_kernel" else vb
-        return nnz * (ob + 4) + 8 * (n // 64 + 1) + 2 * vb * n
-    if kernel == "dga_h_kernel":
-        H = d["H"]
-        n, nnz = H["nrows"], int(H["rowptr"][-1])
-        # H values+cols, rowptr, row->edge map, curl_f read, U read+write
-        return nnz * (vb + 4) + 4 * (n + 1) + 4 * n + vb * b.n_edges + 2 * vb * n
+def algorithmic_bytes(info, kernel, vb):
+    """Bytes a CG kernel must move per launch: the operator's non-zeros once
+    (values + stored column index; padding entries are not counted), the
+    slice pointer and column base of each 64-row slice, and each vector once
+    per access (gathered vectors once). `info` is the device storage of A
+    (Simulation.sell_info("a")), vb the value bytes of the vectors."""
+    n, nnz, cb = info["rows"], info["nnz"], info["col_bytes"]
+    per_slice = 12 * info["slices"]
+    if kernel == "cg_w_kernel":      # w = A u: u gathered, w written
+        return nnz * (vb + cb) + per_slice + 2 * vb * n
+    if kernel == "cg_wu_kernel":     # w = A u, p and s updated: u gathered, p, s read and written
+        return nnz * (vb + cb) + per_slice + 5 * vb * n
+    if kernel == "cg_pg_kernel":     # u = G r' (bf16 G): r, s gathered, x r/w, p read, r', u written
+        return nnz * (2 + cb) + per_slice + 7 * vb * n
     raise KeyError(kernel)
+
+
+def roofline_of(info, kernels, names, vb):
+    """Roofline entries of the listed CG kernels measured in a run; the
+    first is the one with the largest total time."""
+    out = []
+    for k in names:
+        t = kernels.get(k)
+        if not t:
+            continue
+        nb = algorithmic_bytes(info, k, vb)
+        ach = nb / (t["ms_avg"] * 1e-3) / 1e9
+        out.append(dict(bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                        frac=round(ach / HBM_PEAK_GBS, 4), traffic=None, kernel=k, bytes_per_launch=int(nb),
+                        ms_per_launch=t["ms_avg"], launches=t["launches"]))
+    out.sort(key=lambda r: -r["ms_per_launch"] * r["launches"])
+    return out
 
 
 def time_path(b, precision, steps, warmup, precond="jacobi", label="", learn_steps=200):
@@ -100,6 +115,8 @@ def time_path(b, precision, steps, warmup, precond="jacobi", label="", learn_ste
                cg_max_relres=st["cg_max_relres"] if b.method == "gfdtd" else None,
                cg_not_converged=st["cg_not_converged"] if b.method == "gfdtd" else None)
     out["tuning"] = tuned
+    if b.method == "gfdtd":
+        out["A"] = sim.sell_info("a")
     if learn is not None:
         learn.update(state_after=st["pc_state"], fallbacks=st["pc_fallbacks"])
         out["learned"] = learn
@@ -401,16 +418,10 @@ def main():
             bd = build_workload(d, "dga", args.n, total)
             dga = time_path(bd, "fp64", args.steps, args.warmup, "jacobi", "dga fp64 (explicit)")
             dga32 = time_path(bd, "fp32", args.steps, args.warmup, "jacobi", "dga fp32 (explicit)")
-        kname = "cg_w_kernel"
         vb = 4 if head is fast else 8
-        kern = head["kernels"].get(kname)
-        roof = None
-        if kern:
-            nbytes = algorithmic_bytes(bg, kname, vb)
-            ach = nbytes / (kern["ms_avg"] * 1e-3) / 1e9
-            roof = dict(bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                        frac=round(ach / HBM_PEAK_GBS, 4), traffic=None, kernel=kname,
-                        bytes_per_launch=int(nbytes), ms_per_launch=kern["ms_avg"])
+        roofs = roofline_of(head["A"], head["kernels"], ["cg_wu_kernel", "cg_pg_kernel", "cg_w_kernel"], vb)
+        roof = roofs[0] if roofs else None
+        plain_roofs = roofline_of(plain["A"], plain["kernels"], ["cg_w_kernel"], 8) if plain is not none else []
         cpu = None
         if not args.no_cpu:
             t = time.time()
@@ -443,6 +454,9 @@ def main():
             "fp64_learned": {"edge_dof_per_s": l64["edge_dof_per_s"], "cg_iters_per_step": l64["cg_iters_per_step"]},
             "learned": head.get("learned"),
             "roofline": roof,
+            "roofline_kernels": roofs,
+            "plain_fp64_roofline": plain_roofs[0] if plain_roofs else None,
+            "A_storage": head.get("A"),
             "cpu_baseline": cpu,
         }
         if dga:

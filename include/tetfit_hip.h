@@ -159,15 +159,27 @@ tf_status tf_device_bytes(tf_ctx* ctx, int64_t* bytes);
 
 tf_status tf_destroy(tf_ctx* ctx);
 
-/* Autotuner: times the context's SELL kernels (Ct, H, N for dga; N, A and
- * the learned G for gfdtd) over grid sizes and gather batch depths and keeps
- * the fastest for later tf_run calls. Results are identical for every
- * choice; they are cached per (kernel, precision, rows, entries) in the
+/* Autotuner: times the context's SELL kernels (Ct, H, N for dga; N, the
+ * Jacobi pass's A product and, once the learned preconditioner is active,
+ * its two passes (G and A products) for gfdtd) over grid sizes and gather
+ * batch depths and keeps the fastest for later tf_run calls. Fields are
+ * identical for every choice (each row is summed by one lane in entry
+ * order). Choices are cached per (kernel, precision, rows, entries) in the
  * process and in the file named by $TETFIT_TUNE_CACHE if set. No reference
- * counterpart. tf_get_tuning reports the choice per kernel (order: Ct, H, N,
- * A, G; blocks 0 = default). */
+ * counterpart. tf_get_tuning reports the choice per kernel (order:
+ * sell_ct, sell_h, sell_n, cg_w, cg_pg, cg_wu; blocks 0 = default). */
 tf_status tf_autotune(tf_ctx* ctx);
 tf_status tf_get_tuning(tf_ctx* ctx, int32_t* blocks, int32_t* unroll, int32_t cap, int32_t* n);
+/* Sets the launch shape of the first n kernels in tf_get_tuning's order
+ * (blocks 0 = default grid; unroll 4, 8 or 16), e.g. to replay a choice. */
+tf_status tf_set_tuning(tf_ctx* ctx, const int32_t* blocks, const int32_t* unroll, int32_t n);
+
+/* Storage of one device operator, for byte accounting: which = 0 Ct (dga),
+ * 1 H, 2 N, 3 A (gfdtd free rows), 4 E, 5 RHSmat1, 6 Ct (gfdtd free rows);
+ * info[0..4] = rows, 64-row slices, non-zeros, stored entries (with
+ * padding), bytes per stored column index (2: per-slice base + 16-bit
+ * offset, 4: int32). No reference counterpart. */
+tf_status tf_sell_info(tf_ctx* ctx, int32_t which, int64_t* info);
 
 /* ---- batches ----------------------------------------------------------
  * Many independent dga simulations (uneven meshes, materials, sources and

@@ -207,6 +207,7 @@ struct CgState {
     double rr0;        // ||r0||^2 of this solve
     double gam[2];     // r.u of pass it, slot it&1
     double alpha[2];   // step length of pass it, slot it&1
+    double beta[2];    // beta of pass it, slot it&1 (learned passes)
     double relres;     // ||r||/||r0|| at the last stop test
     double tol;
     int32_t maxiter;

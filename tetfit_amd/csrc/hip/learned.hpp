@@ -102,16 +102,25 @@ __global__ void pair_grad_kernel(int32_t n_pairs, const int64_t* __restrict__ pa
     gy[p] = g;
 }
 
-// Ĝ (training, scaled) and G (CG, unscaled) values at every SELL position.
+// fp32 -> bf16 bits, round to nearest even (finite inputs).
+__device__ __forceinline__ uint16_t float_to_bf16(float f) {
+    uint32_t u = __float_as_uint(f);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+// Ĝ (training, scaled, fp32) and G (CG, unscaled, bf16) values at every
+// SELL position. The two positions of a pair get the same value, so G stays
+// exactly symmetric after rounding.
 __global__ void expand_kernel(int64_t entries, const int32_t* __restrict__ pos2pair, const float* __restrict__ base,
                               const float* __restrict__ scale, const float* __restrict__ y, float* __restrict__ ghat,
-                              float* __restrict__ gval) {
+                              uint16_t* __restrict__ gval) {
     const int64_t pos = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (pos >= entries) return;
     const int32_t p = pos2pair[pos];
     const float g = p >= 0 ? base[p] + y[p] : 0.f;
     if (ghat) ghat[pos] = g;
-    if (gval) gval[pos] = p >= 0 ? scale[p] * g : 0.f;
+    if (gval) gval[pos] = float_to_bf16(p >= 0 ? scale[p] * g : 0.f);
 }
 
 // Bank column `slot` <- r, x (the CG iterate being captured).

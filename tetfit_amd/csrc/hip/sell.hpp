@@ -5,8 +5,12 @@
 // sptr[s] + 64*k + lane, so every load of values, column indices and signs
 // is a fully coalesced 64-lane access. Short rows are padded with value 0
 // and their own last column (a valid index). Within a row the entries keep
-// the CSR column order, so the sums run in the same order as a sorted-CSR
-// sequential product.
+// the order of the CSR row they come from, so a row sum runs in the same
+// order as a sequential CSR product.
+//
+// Column indices are stored either as int32, or, when every slice's columns
+// lie within 65536 of the slice's smallest column, as a per-slice int32 base
+// plus a uint16 offset per entry (2 bytes instead of 4 on every entry).
 #pragma once
 
 #include "common.hpp"
@@ -16,57 +20,81 @@ namespace tfd {
 
 constexpr int kSlice = 64;
 
-// Sum over the slice row of val*gather(col), accumulated in A (the field
-// type: fp64 on the plain path, fp32 on the fast path). Loads are issued in
-// batches of U entries (2U independent loads, then U independent gathers)
-// so that each lane keeps enough memory traffic in flight; the tail batch is
-// clamped to the row's last entry and predicated. Accumulation stays in
-// column order, so the result does not depend on U or on the launch shape.
-template <typename A, int U, typename T, class G>
-__device__ __forceinline__ double sell_dot(int64_t base, int w, const int32_t* __restrict__ col,
-                                           const T* __restrict__ val, int lane, G gather) {
-    const int32_t* c = col + base + lane;
-    const T* v = val + base + lane;
-    A acc = 0;
-    for (int k0 = 0; k0 < w; k0 += U) {
-        int32_t ci[U];
-        A vi[U], gi[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int kk = min(k0 + u, w - 1);  // clamped: every load stays inside the slice
-            ci[u] = c[kk * 64];
-            vi[u] = (A)v[kk * 64];
-        }
-#pragma unroll
-        for (int u = 0; u < U; ++u) gi[u] = (A)gather(ci[u]);
-#pragma unroll
-        for (int u = 0; u < U; ++u)
-            if (k0 + u < w) acc += vi[u] * gi[u];
-    }
-    return (double)acc;
+// Device view of a SELL-64 matrix with values of type V (int8 for signed
+// incidence, uint16 for bf16 bits). Exactly one of col / col16 is set.
+template <typename V>
+struct SellView {
+    int32_t nrows, nslices;
+    const int64_t* sptr;
+    const int32_t* col;     // int32 columns, or null
+    const uint16_t* col16;  // uint16 offsets from cbase[slice], or null
+    const int32_t* cbase;
+    const V* val;
+};
+
+__device__ __forceinline__ float bf16_to_float(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
+
+template <typename A, typename V>
+__device__ __forceinline__ A sell_value(V v) {
+    return (A)v;
+}
+template <>
+__device__ __forceinline__ float sell_value<float, uint16_t>(uint16_t v) {
+    return bf16_to_float(v);
+}
+template <>
+__device__ __forceinline__ double sell_value<double, uint16_t>(uint16_t v) {
+    return (double)bf16_to_float(v);
 }
 
-// Same with int8 signs instead of values (incidence matrices).
-template <typename T, int U>
-__device__ __forceinline__ double sell_sgn_dot(int64_t base, int w, const int32_t* __restrict__ col,
-                                               const int8_t* __restrict__ sg, int lane, const T* __restrict__ x) {
-    const int32_t* c = col + base + lane;
-    const int8_t* s = sg + base + lane;
-    T acc = 0;
-    for (int k0 = 0; k0 < w; k0 += U) {
-        int32_t ci[U];
-        T si[U], gi[U];
+// Sum over row (slice s, lane) of val * gather(col), accumulated in A (the
+// field type: fp64 on the plain path, fp32 on the fast path). Loads are
+// issued in batches of U entries (2U independent loads, then U independent
+// gathers) so that each lane keeps enough memory traffic in flight; the tail
+// batch is clamped to the row's last entry and predicated. Accumulation
+// stays in entry order, so the result does not depend on U or on the launch
+// shape.
+template <typename A, int U, typename V, class G>
+__device__ __forceinline__ double sell_dot(const SellView<V>& m, int s, int lane, G gather) {
+    const int64_t base = m.sptr[s];
+    const int w = (int)((m.sptr[s + 1] - base) / kSlice);
+    const V* v = m.val + base + lane;
+    A acc = 0;
+    if (m.col16) {
+        const uint16_t* c = m.col16 + base + lane;
+        const int32_t cb = m.cbase[s];
+        for (int k0 = 0; k0 < w; k0 += U) {
+            int32_t ci[U];
+            A vi[U], gi[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int kk = min(k0 + u, w - 1);
-            ci[u] = c[kk * 64];
-            si[u] = (T)s[kk * 64];
+            for (int u = 0; u < U; ++u) {
+                const int kk = min(k0 + u, w - 1);  // clamped: every load stays inside the slice
+                ci[u] = cb + (int32_t)c[kk * 64];
+                vi[u] = sell_value<A, V>(v[kk * 64]);
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) gi[u] = (A)gather(ci[u]);
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                if (k0 + u < w) acc += vi[u] * gi[u];
         }
+    } else {
+        const int32_t* c = m.col + base + lane;
+        for (int k0 = 0; k0 < w; k0 += U) {
+            int32_t ci[U];
+            A vi[U], gi[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) gi[u] = x[ci[u]];
+            for (int u = 0; u < U; ++u) {
+                const int kk = min(k0 + u, w - 1);
+                ci[u] = c[kk * 64];
+                vi[u] = sell_value<A, V>(v[kk * 64]);
+            }
 #pragma unroll
-        for (int u = 0; u < U; ++u)
-            if (k0 + u < w) acc += si[u] * gi[u];
+            for (int u = 0; u < U; ++u) gi[u] = (A)gather(ci[u]);
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                if (k0 + u < w) acc += vi[u] * gi[u];
+        }
     }
     return (double)acc;
 }
@@ -89,38 +117,34 @@ __device__ __forceinline__ SliceRange block_slices(int nslices) {
     return SliceRange{s0 + (int)(threadIdx.x / kWave), s1, (int)(blockDim.x / kWave)};
 }
 
+#define TF_FOR_SLICES(m)                                                               \
+    for (SliceRange sr_ = block_slices((m).nslices); sr_.s < sr_.end; sr_.s += sr_.step)
+
 // curl_f = Ct F over edges (dga, D:878-879).
 template <typename T, int NB>
-__global__ void __launch_bounds__(kBlock) sell_ct_kernel(int32_t nrows, int32_t nslices, const int64_t* __restrict__ sptr,
-                                                         const int32_t* __restrict__ col, const int8_t* __restrict__ sg,
-                                                         const T* __restrict__ x, T* __restrict__ y) {
+__global__ void __launch_bounds__(kBlock) sell_ct_kernel(SellView<int8_t> m, const T* __restrict__ x,
+                                                         T* __restrict__ y) {
     const int lane = lane_id();
-    for (SliceRange sr = block_slices(nslices); sr.s < sr.end; sr.s += sr.step) {
-        const int s = sr.s;
-        const int64_t b = sptr[s];
-        const int w = (int)((sptr[s + 1] - b) / kSlice);
-        const double v = sell_sgn_dot<T, NB>(b, w, col, sg, lane, x);
+    TF_FOR_SLICES(m) {
+        const int s = sr_.s;
+        const double v = sell_dot<T, NB>(m, s, lane, [&](int32_t j) { return x[j]; });
         const int32_t r = s * kSlice + lane;
-        if (r < nrows) y[r] = (T)v;
+        if (r < m.nrows) y[r] = (T)v;
     }
 }
 
 // U[edge(r)] += dt * (H curl_f)[r] (dga, D:889); row_dt (batches) gives
 // each row its own simulation's dt.
 template <typename T, int NB>
-__global__ void __launch_bounds__(kBlock) sell_h_kernel(int32_t nrows, int32_t nslices, const int64_t* __restrict__ sptr,
-                                                        const int32_t* __restrict__ col, const T* __restrict__ val,
-                                                        const int32_t* __restrict__ row_edge, const T* __restrict__ x,
-                                                        T* __restrict__ U, double dt,
+__global__ void __launch_bounds__(kBlock) sell_h_kernel(SellView<T> m, const int32_t* __restrict__ row_edge,
+                                                        const T* __restrict__ x, T* __restrict__ U, double dt,
                                                         const double* __restrict__ row_dt) {
     const int lane = lane_id();
-    for (SliceRange sr = block_slices(nslices); sr.s < sr.end; sr.s += sr.step) {
-        const int s = sr.s;
-        const int64_t b = sptr[s];
-        const int w = (int)((sptr[s + 1] - b) / kSlice);
-        const double v = sell_dot<T, NB>(b, w, col, val, lane, [&](int32_t j) { return x[j]; });
+    TF_FOR_SLICES(m) {
+        const int s = sr_.s;
+        const double v = sell_dot<T, NB>(m, s, lane, [&](int32_t j) { return x[j]; });
         const int32_t r = s * kSlice + lane;
-        if (r < nrows) {
+        if (r < m.nrows) {
             const int32_t e = row_edge[r];
             U[e] = (T)((double)U[e] + (row_dt ? row_dt[r] : dt) * v);
         }
@@ -130,44 +154,14 @@ __global__ void __launch_bounds__(kBlock) sell_h_kernel(int32_t nrows, int32_t n
 // mode 0: F -= dt * N x (dga, D:917); mode 1: F = N x (gfdtd, D:1468);
 // row_dt as in sell_h_kernel.
 template <typename T, int NB>
-__global__ void __launch_bounds__(kBlock) sell_n_kernel(int32_t nrows, int32_t nslices, const int64_t* __restrict__ sptr,
-                                                        const int32_t* __restrict__ col, const T* __restrict__ val,
-                                                        const T* __restrict__ x, T* __restrict__ Fv, double dt, int mode,
-                                                        const double* __restrict__ row_dt) {
+__global__ void __launch_bounds__(kBlock) sell_n_kernel(SellView<T> m, const T* __restrict__ x, T* __restrict__ Fv,
+                                                        double dt, int mode, const double* __restrict__ row_dt) {
     const int lane = lane_id();
-    for (SliceRange sr = block_slices(nslices); sr.s < sr.end; sr.s += sr.step) {
-        const int s = sr.s;
-        const int64_t b = sptr[s];
-        const int w = (int)((sptr[s + 1] - b) / kSlice);
-        const double v = sell_dot<T, NB>(b, w, col, val, lane, [&](int32_t j) { return x[j]; });
+    TF_FOR_SLICES(m) {
+        const int s = sr_.s;
+        const double v = sell_dot<T, NB>(m, s, lane, [&](int32_t j) { return x[j]; });
         const int32_t r = s * kSlice + lane;
-        if (r < nrows) Fv[r] = mode == 0 ? (T)((double)Fv[r] - (row_dt ? row_dt[r] : dt) * v) : (T)v;
-    }
-}
-
-// rhs[k] = E U_old - RHS1 SrcFld + dt (Ct F + fb) at free edge e = free[k]
-// (D:1516-1523; sigma and I are zero on the supported inputs). The three
-// operators are stored as SELL-64 over the free rows; SrcFld is U on
-// Dirichlet edges at the new time, held in `dir_new` (edge-indexed).
-template <typename T>
-__global__ void __launch_bounds__(kBlock) gf_rhs_kernel(
-    int32_t n_free, int32_t nslices, const int32_t* __restrict__ free_edge, const int64_t* __restrict__ e_ptr,
-    const int32_t* __restrict__ e_col, const T* __restrict__ e_val, const int64_t* __restrict__ r_ptr,
-    const int32_t* __restrict__ r_col, const T* __restrict__ r_val, const int64_t* __restrict__ c_ptr,
-    const int32_t* __restrict__ c_col, const int8_t* __restrict__ c_sg, const T* __restrict__ U,
-    const T* __restrict__ dir_new, const T* __restrict__ Fv, const T* __restrict__ fb, T* __restrict__ rhs,
-    double dt) {
-    const int lane = lane_id();
-    for (SliceRange sr = block_slices(nslices); sr.s < sr.end; sr.s += sr.step) {
-        const int s = sr.s;
-        const int64_t eb = e_ptr[s], rb = r_ptr[s], cb = c_ptr[s];
-        const int ew = (int)((e_ptr[s + 1] - eb) / kSlice), rw = (int)((r_ptr[s + 1] - rb) / kSlice);
-        const int cw = (int)((c_ptr[s + 1] - cb) / kSlice);
-        const double eu = sell_dot<T, 8>(eb, ew, e_col, e_val, lane, [&](int32_t j) { return U[j]; });
-        const double rs = rw ? sell_dot<T, 8>(rb, rw, r_col, r_val, lane, [&](int32_t j) { return dir_new[j]; }) : 0.0;
-        const double cf = sell_sgn_dot<T, 8>(cb, cw, c_col, c_sg, lane, Fv);
-        const int32_t k = s * kSlice + lane;
-        if (k < n_free) rhs[k] = (T)((eu - rs) + dt * cf + dt * (double)fb[free_edge[k]]);
+        if (r < m.nrows) Fv[r] = mode == 0 ? (T)((double)Fv[r] - (row_dt ? row_dt[r] : dt) * v) : (T)v;
     }
 }
 
@@ -177,22 +171,81 @@ __global__ void __launch_bounds__(kBlock) gf_rhs_kernel(
 //   beta_i  = gamma_i / gamma_{i-1},  alpha_i = gamma_i / (delta_i - beta_i gamma_i / alpha_{i-1}),
 //   p = u + beta p,  s = w + beta s  (s = A p),  x += alpha p,  r -= alpha s.
 // In exact arithmetic the iterates are those of the reference's PCG, and the
-// stop test is the reference's: ||r|| / ||r0|| > tol && it < maxiter. Its
-// point on this device: every SpMV gathers one vector (A u, G r), and the
-// vector updates are one streaming kernel, so an iteration is
+// stop test is the reference's: ||r|| / ||r0|| > tol && it < maxiter.
+// Kernels per iteration:
 //   Jacobi (M = 1/diag A):  cg_w (stop test, w = A u), cg_upd (updates, u = M r)
-//   SpMV M = G (learned):   cg_g (stop test, u = G r), cg_w, cg_upd.
+//   learned (M = G, SpMV):  cg_pg (x, r updates, u = G r), cg_wu (stop test,
+//                           w = A u, p and s updates)
 // Reductions go through per-block fp64 partials that the next kernel sums in
 // a fixed order (no atomics, no grid-wide fences).
 
-// Stop test at the start of pass `it` on the residual left by the previous
-// update (the partials part_rr). Every block computes the same sums.
-__device__ __forceinline__ bool cg_stop_test(int32_t it, const double* __restrict__ part_rr, int32_t nb_rr,
-                                             CgState* st, double* red) {
-    const double RR = sum_partials(part_rr, nb_rr, red);
-    const double rr0 = it == 0 ? RR : st->rr0;
+// ---- scalar prologue of the CG kernels
+// A kernel needs global scalars (sums of the previous kernel's per-block
+// partials, and values in the CgState) before its first store. Every load
+// of the prologue is issued before any of them is used: each thread holds
+// its share of up to kMaxPart partials per array in registers (the same
+// per-thread order as a strided loop, so the sums are unchanged), and both
+// arrays are reduced in one block reduction. The prologue then costs one
+// memory round trip instead of one per loop trip and per scalar.
+constexpr int kMaxPart = 2048;
+constexpr int kPartPer = kMaxPart / kBlock;
+
+struct Partials {
+    double v[kPartPer];
+};
+__device__ __forceinline__ Partials load_partials(const double* __restrict__ p, int n) {
+    Partials r;
+#pragma unroll
+    for (int k = 0; k < kPartPer; ++k) {
+        const int i = threadIdx.x + k * kBlock;
+        r.v[k] = i < n ? p[i] : 0.0;
+    }
+    return r;
+}
+__device__ __forceinline__ double thread_sum(const Partials& r) {
+    double s = 0;
+#pragma unroll
+    for (int k = 0; k < kPartPer; ++k) s += r.v[k];
+    return s;
+}
+
+// Block-wide sums of two per-thread values, returned to every thread.
+// red holds 2 * (waves per block) + 2 doubles.
+__device__ __forceinline__ void block_sum2_all(double& a, double& b, double* red) {
+    constexpr int nw = kBlock / kWave;
+    a = group_sum<kWave>(a);
+    b = group_sum<kWave>(b);
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+    if (lane == 0) {
+        red[w] = a;
+        red[nw + w] = b;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double sa = 0, sb = 0;
+        for (int i = 0; i < nw; ++i) {
+            sa += red[i];
+            sb += red[nw + i];
+        }
+        red[2 * nw] = sa;
+        red[2 * nw + 1] = sb;
+    }
+    __syncthreads();
+    a = red[2 * nw];
+    b = red[2 * nw + 1];
+    __syncthreads();
+}
+constexpr int kRed2 = 2 * (kBlock / kWave) + 2;
+
+// Stop decision of pass `it` from the residual norm RR = ||r_it||^2 (the
+// reference's test, ConjugateGradientSolver.hpp:68-74): stop unless
+// ||r|| / ||r0|| > tol and it < maxiter. Every block decides the same;
+// block 0 records it.
+__device__ __forceinline__ bool cg_decide_stop(int32_t it, double RR, double rr0_prev, double tol, int32_t maxiter,
+                                               CgState* st) {
+    const double rr0 = it == 0 ? RR : rr0_prev;
     const double ratio = std::sqrt(RR) / std::sqrt(rr0);
-    const bool stop = !(ratio > st->tol && it < st->maxiter);
+    const bool stop = !(ratio > tol && it < maxiter);
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         if (it == 0) st->rr0 = RR;
         st->relres = ratio;
@@ -201,26 +254,39 @@ __device__ __forceinline__ bool cg_stop_test(int32_t it, const double* __restric
     return stop;
 }
 
-// r = b - A x, partial r.r; with jacobi also u = pre r and partial r.u.
+// Residual and search-direction image of one row, stored side by side for
+// the learned passes: cg_pg gathers both with one 8- (fp32) or 16-byte
+// (fp64) load per matrix entry.
 template <typename T>
-__global__ void __launch_bounds__(kBlock) cg_init_kernel(int32_t n, int32_t nslices, const int64_t* __restrict__ sptr,
-                                                         const int32_t* __restrict__ col, const T* __restrict__ val,
-                                                         const T* __restrict__ pre, const T* __restrict__ b,
-                                                         const T* __restrict__ x, T* __restrict__ r, T* __restrict__ u,
+struct alignas(2 * sizeof(T)) RS {
+    T r, s;
+};
+
+// The residual update r - alpha s in one rounding, used identically by the
+// lane that owns a row and by every lane that gathers it.
+template <typename T>
+__device__ __forceinline__ T cg_r_next(T r, T s, double alpha) {
+    return (T)__builtin_fma(-alpha, (double)s, (double)r);
+}
+
+// r = b - A x (r[i * r_stride]), partial r.r; with jacobi also u = pre r
+// and partial r.u.
+template <typename T, int NB>
+__global__ void __launch_bounds__(kBlock) cg_init_kernel(SellView<T> m, const T* __restrict__ pre,
+                                                         const T* __restrict__ b, const T* __restrict__ x,
+                                                         T* __restrict__ r, int r_stride, T* __restrict__ u,
                                                          double* __restrict__ part_rr, double* __restrict__ part_gam,
                                                          CgState* st, int jacobi) {
-    __shared__ double red[kBlock / kWave];
+    __shared__ double red[kRed2];
     const int lane = lane_id();
     double rr = 0, ru = 0;
-    for (SliceRange sr = block_slices(nslices); sr.s < sr.end; sr.s += sr.step) {
-        const int s = sr.s;
-        const int64_t bs = sptr[s];
-        const int w = (int)((sptr[s + 1] - bs) / kSlice);
-        const double ax = sell_dot<T, 8>(bs, w, col, val, lane, [&](int32_t j) { return x[j]; });
+    TF_FOR_SLICES(m) {
+        const int s = sr_.s;
+        const double ax = sell_dot<T, NB>(m, s, lane, [&](int32_t j) { return x[j]; });
         const int32_t i = s * kSlice + lane;
-        if (i < n) {
+        if (i < m.nrows) {
             const T ri = (T)((double)b[i] - ax);
-            r[i] = ri;
+            r[(int64_t)i * r_stride] = ri;
             rr += (double)ri * (double)ri;
             if (jacobi) {
                 const T ui = (T)((double)pre[i] * (double)ri);
@@ -229,8 +295,7 @@ __global__ void __launch_bounds__(kBlock) cg_init_kernel(int32_t n, int32_t nsli
             }
         }
     }
-    rr = block_sum(rr, red);
-    ru = block_sum(ru, red);
+    block_sum2_all(rr, ru, red);
     if (threadIdx.x == 0) {
         part_rr[blockIdx.x] = rr;
         if (jacobi) part_gam[blockIdx.x] = ru;
@@ -238,68 +303,42 @@ __global__ void __launch_bounds__(kBlock) cg_init_kernel(int32_t n, int32_t nsli
     }
 }
 
-// Learned preconditioner pass: stop test (test_stop = 1 in solves), then
-// u = G r with G on A's SELL pattern (same column array, fp32 values),
-// partial r.u.
+// w = A u, partial w.u; with test_stop the pass starts with the stop test on
+// the partials part_rr of the previous update.
 template <typename T, int NB>
-__global__ void __launch_bounds__(kBlock) cg_g_kernel(int32_t it, int32_t n, int32_t nslices,
-                                                      const int64_t* __restrict__ sptr, const int32_t* __restrict__ col,
-                                                      const float* __restrict__ gval, const T* __restrict__ r,
-                                                      T* __restrict__ u, int test_stop,
-                                                      const double* __restrict__ part_rr, int32_t nb_rr,
-                                                      double* __restrict__ part_gam, CgState* st) {
-    if (st->done) return;
-    __shared__ double red[kBlock / kWave];
-    if (test_stop && cg_stop_test(it, part_rr, nb_rr, st, red)) return;
-    const int lane = lane_id();
-    double ru = 0;
-    for (SliceRange sr = block_slices(nslices); sr.s < sr.end; sr.s += sr.step) {
-        const int s = sr.s;
-        const int64_t bs = sptr[s];
-        const int w = (int)((sptr[s + 1] - bs) / kSlice);
-        const double uv = sell_dot<T, NB>(bs, w, col, gval, lane, [&](int32_t j) { return r[j]; });
-        const int32_t i = s * kSlice + lane;
-        if (i < n) {
-            const T ui = (T)uv;
-            u[i] = ui;
-            ru += (double)r[i] * (double)ui;
-        }
-    }
-    ru = block_sum(ru, red);
-    if (threadIdx.x == 0) part_gam[blockIdx.x] = ru;
-}
-
-// w = A u, partial w.u; with test_stop the pass starts with the stop test.
-template <typename T, int NB>
-__global__ void __launch_bounds__(kBlock) cg_w_kernel(int32_t it, int32_t n, int32_t nslices,
-                                                      const int64_t* __restrict__ sptr, const int32_t* __restrict__ col,
-                                                      const T* __restrict__ val, const T* __restrict__ u,
+__global__ void __launch_bounds__(kBlock) cg_w_kernel(int32_t it, SellView<T> m, const T* __restrict__ u,
                                                       T* __restrict__ w, int test_stop,
                                                       const double* __restrict__ part_rr, int32_t nb_rr,
                                                       double* __restrict__ part_del, CgState* st) {
-    if (st->done) return;
-    __shared__ double red[kBlock / kWave];
-    if (test_stop && cg_stop_test(it, part_rr, nb_rr, st, red)) return;
+    __shared__ double red[kRed2];
+    const Partials prr = load_partials(part_rr, test_stop ? nb_rr : 0);
+    const int done = st->done;
+    const double rr0 = st->rr0, tol = st->tol;
+    const int32_t maxiter = st->maxiter;
+    if (done) return;
+    if (test_stop) {
+        double RR = thread_sum(prr), unused = 0;
+        block_sum2_all(RR, unused, red);
+        if (cg_decide_stop(it, RR, rr0, tol, maxiter, st)) return;
+    }
     const int lane = lane_id();
-    double wu = 0;
-    for (SliceRange sr = block_slices(nslices); sr.s < sr.end; sr.s += sr.step) {
-        const int s = sr.s;
-        const int64_t bs = sptr[s];
-        const int wd = (int)((sptr[s + 1] - bs) / kSlice);
-        const double wv = sell_dot<T, NB>(bs, wd, col, val, lane, [&](int32_t j) { return u[j]; });
+    double wu = 0, none = 0;
+    TF_FOR_SLICES(m) {
+        const int s = sr_.s;
+        const double wv = sell_dot<T, NB>(m, s, lane, [&](int32_t j) { return u[j]; });
         const int32_t i = s * kSlice + lane;
-        if (i < n) {
+        if (i < m.nrows) {
             const T wi = (T)wv;
             w[i] = wi;
             wu += (double)wi * (double)u[i];
         }
     }
-    wu = block_sum(wu, red);
+    block_sum2_all(wu, none, red);
     if (threadIdx.x == 0) part_del[blockIdx.x] = wu;
 }
 
-// Vector updates of pass `it`; gamma and delta from their partials, the
-// previous gamma and alpha from slot (it-1)&1 of the state. With jacobi the
+// Jacobi vector updates of pass `it`; gamma and delta from their partials,
+// the previous gamma and alpha from slot (it-1)&1 of the state; then the
 // next u = pre r and its partial r.u go to part_gam_next.
 template <typename T>
 __global__ void __launch_bounds__(kBlock) cg_upd_kernel(int32_t it, int32_t n, const T* __restrict__ pre,
@@ -308,18 +347,21 @@ __global__ void __launch_bounds__(kBlock) cg_upd_kernel(int32_t it, int32_t n, c
                                                         const double* __restrict__ part_gam, int32_t nb_gam,
                                                         const double* __restrict__ part_del, int32_t nb_del,
                                                         double* __restrict__ part_rr, double* __restrict__ part_gam_next,
-                                                        CgState* st, int jacobi) {
-    if (st->done) return;
-    __shared__ double red[kBlock / kWave];
-    const double gam = sum_partials(part_gam, nb_gam, red);
-    const double del = sum_partials(part_del, nb_del, red);
+                                                        CgState* st) {
+    __shared__ double red[kRed2];
+    const Partials pg = load_partials(part_gam, nb_gam), pd = load_partials(part_del, nb_del);
+    const int done = st->done;
+    const double gam_prev = st->gam[(it - 1) & 1], alpha_prev = st->alpha[(it - 1) & 1];
+    if (done) return;
+    double gam = thread_sum(pg), del = thread_sum(pd);
+    block_sum2_all(gam, del, red);
     double alpha, beta;
     if (it == 0) {
         beta = 0.0;
         alpha = gam / del;
     } else {
-        beta = gam / st->gam[(it - 1) & 1];
-        alpha = gam / (del - beta * gam / st->alpha[(it - 1) & 1]);
+        beta = gam / gam_prev;
+        alpha = gam / (del - beta * gam / alpha_prev);
     }
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         st->gam[it & 1] = gam;
@@ -335,17 +377,145 @@ __global__ void __launch_bounds__(kBlock) cg_upd_kernel(int32_t it, int32_t n, c
         const T ri = (T)((double)r[i] - alpha * (double)si);
         r[i] = ri;
         rr += (double)ri * (double)ri;
-        if (jacobi) {
-            const T ui = (T)((double)pre[i] * (double)ri);
-            u[i] = ui;
-            ru += (double)ri * (double)ui;
-        }
+        const T ui = (T)((double)pre[i] * (double)ri);
+        u[i] = ui;
+        ru += (double)ri * (double)ui;
     }
-    rr = block_sum(rr, red);
-    ru = block_sum(ru, red);
+    block_sum2_all(rr, ru, red);
     if (threadIdx.x == 0) {
         part_rr[blockIdx.x] = rr;
-        if (jacobi) part_gam_next[blockIdx.x] = ru;
+        part_gam_next[blockIdx.x] = ru;
+    }
+}
+
+// Learned pass `it`, first kernel. For it >= 1: alpha_{it-1} from gamma and
+// beta of pass it-1 (state slot (it-1)&1), the partials of delta_{it-1}
+// (part_del) and alpha_{it-2}; then on the owned rows x += alpha p and
+// r_it = r_{it-1} - alpha s_{it-1} (rs_in holds (r_{it-1}, s_{it-1}); r_it
+// goes to rs_out[i].r), and u = G r_it where the gathered r_it[j] is formed
+// from rs_in[j] with the same rounding as the owner's; partials r.u and r.r.
+// For it = 0, u = G r with r = rs_in[].r. G: bf16 values on A's pattern.
+template <typename T, int NB>
+__global__ void __launch_bounds__(kBlock) cg_pg_kernel(int32_t it, SellView<uint16_t> g,
+                                                       const RS<T>* __restrict__ rs_in, RS<T>* __restrict__ rs_out,
+                                                       const T* __restrict__ p, T* __restrict__ x, T* __restrict__ u,
+                                                       const double* __restrict__ part_del, int32_t nb_del,
+                                                       double* __restrict__ part_gam, double* __restrict__ part_rr,
+                                                       CgState* st) {
+    __shared__ double red[kRed2];
+    const int k = (it - 1) & 1;
+    const Partials pd = load_partials(part_del, it > 0 ? nb_del : 0);
+    const int done = st->done;
+    const double gam = st->gam[k], beta = st->beta[k], alpha_pp = st->alpha[k ^ 1];
+    if (done) return;
+    double alpha = 0;
+    if (it > 0) {
+        double del = thread_sum(pd), unused = 0;
+        block_sum2_all(del, unused, red);
+        alpha = it == 1 ? gam / del : gam / (del - beta * gam / alpha_pp);
+        if (threadIdx.x == 0 && blockIdx.x == 0) st->alpha[k] = alpha;
+    }
+    const int lane = lane_id();
+    double ru = 0, rr = 0;
+    TF_FOR_SLICES(g) {
+        const int sl = sr_.s;
+        double uv;
+        if (it > 0)
+            uv = sell_dot<T, NB>(g, sl, lane, [&](int32_t j) {
+                const RS<T> v = rs_in[j];
+                return cg_r_next(v.r, v.s, alpha);
+            });
+        else
+            uv = sell_dot<T, NB>(g, sl, lane, [&](int32_t j) { return rs_in[j].r; });
+        const int32_t i = sl * kSlice + lane;
+        if (i < g.nrows) {
+            const RS<T> o = rs_in[i];
+            T ri = o.r;
+            if (it > 0) {
+                ri = cg_r_next(o.r, o.s, alpha);
+                rs_out[i].r = ri;
+                x[i] = (T)((double)x[i] + alpha * (double)p[i]);
+            }
+            const T ui = (T)uv;
+            u[i] = ui;
+            ru += (double)ri * (double)ui;
+            rr += (double)ri * (double)ri;
+        }
+    }
+    block_sum2_all(ru, rr, red);
+    if (threadIdx.x == 0) {
+        part_gam[blockIdx.x] = ru;
+        part_rr[blockIdx.x] = rr;
+    }
+}
+
+// Learned pass `it`, second kernel: stop test on r_it (part_rr), gamma_it
+// and beta_it (stored in slot it&1), w = A u, p = u + beta p and
+// s_it = w + beta s_{it-1} on the owned rows (s_{it-1} from rs_prev[i].s,
+// s_it into rs_cur[i].s), partial w.u (delta_it).
+template <typename T, int NB>
+__global__ void __launch_bounds__(kBlock) cg_wu_kernel(int32_t it, SellView<T> m, const T* __restrict__ u,
+                                                       T* __restrict__ p, const RS<T>* __restrict__ rs_prev,
+                                                       RS<T>* __restrict__ rs_cur,
+                                                       const double* __restrict__ part_rr,
+                                                       const double* __restrict__ part_gam, int32_t nb_pg,
+                                                       double* __restrict__ part_del, CgState* st) {
+    __shared__ double red[kRed2];
+    const Partials prr = load_partials(part_rr, nb_pg), pga = load_partials(part_gam, nb_pg);
+    const int done = st->done;
+    const double rr0 = st->rr0, tol = st->tol, gam_prev = st->gam[(it - 1) & 1];
+    const int32_t maxiter = st->maxiter;
+    if (done) return;
+    double RR = thread_sum(prr), gam = thread_sum(pga);
+    block_sum2_all(RR, gam, red);
+    if (cg_decide_stop(it, RR, rr0, tol, maxiter, st)) return;
+    const double beta = it == 0 ? 0.0 : gam / gam_prev;
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        st->gam[it & 1] = gam;
+        st->beta[it & 1] = beta;
+    }
+    const int lane = lane_id();
+    double wu = 0, none = 0;
+    TF_FOR_SLICES(m) {
+        const int sl = sr_.s;
+        const double wv = sell_dot<T, NB>(m, sl, lane, [&](int32_t j) { return u[j]; });
+        const int32_t i = sl * kSlice + lane;
+        if (i < m.nrows) {
+            const T wi = (T)wv;
+            const T ui = u[i];
+            if (it == 0) {
+                p[i] = ui;
+                rs_cur[i].s = wi;
+            } else {
+                p[i] = (T)((double)ui + beta * (double)p[i]);
+                rs_cur[i].s = (T)((double)wi + beta * (double)rs_prev[i].s);
+            }
+            wu += (double)wi * (double)ui;
+        }
+    }
+    block_sum2_all(wu, none, red);
+    if (threadIdx.x == 0) part_del[blockIdx.x] = wu;
+}
+
+// rhs[k] = E U_old - RHS1 SrcFld + dt (Ct F + fb) at free edge e = free[k]
+// (D:1516-1523; sigma and I are zero on the supported inputs). The three
+// operators are stored as SELL-64 over the free rows (in the CG's row
+// order); SrcFld is U on Dirichlet edges at the new time, held in `dir_new`
+// (edge-indexed).
+template <typename T>
+__global__ void __launch_bounds__(kBlock) gf_rhs_kernel(SellView<T> e, SellView<T> r1, SellView<int8_t> ct,
+                                                        const int32_t* __restrict__ free_edge, const T* __restrict__ U,
+                                                        const T* __restrict__ dir_new, const T* __restrict__ Fv,
+                                                        const T* __restrict__ fb, T* __restrict__ rhs, double dt) {
+    const int lane = lane_id();
+    TF_FOR_SLICES(e) {
+        const int s = sr_.s;
+        const double eu = sell_dot<T, 8>(e, s, lane, [&](int32_t j) { return U[j]; });
+        const bool has_r = r1.sptr[s + 1] > r1.sptr[s];
+        const double rs = has_r ? sell_dot<T, 8>(r1, s, lane, [&](int32_t j) { return dir_new[j]; }) : 0.0;
+        const double cf = sell_dot<T, 8>(ct, s, lane, [&](int32_t j) { return Fv[j]; });
+        const int32_t k = s * kSlice + lane;
+        if (k < e.nrows) rhs[k] = (T)((eu - rs) + dt * cf + dt * (double)fb[free_edge[k]]);
     }
 }
 

@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -34,7 +35,12 @@ thread_local std::string g_err;
 // fp64 per block of the producing kernel.
 constexpr int kCgPartA = 2048;   // blocks of cg_w (SpMV with A)
 constexpr int kCgPartB = 1024;   // blocks of cg_init / cg_upd
-constexpr int kCgPartC = 2048;   // blocks of cg_g (SpMV with the learned G)
+constexpr int kCgPartC = 2048;   // blocks of cg_pg (SpMV with the learned G)
+static_assert(kCgPartA <= kMaxPart && kCgPartB <= kMaxPart && kCgPartC <= kMaxPart,
+              "CG kernels read at most kMaxPart partials per array");
+// SELL-C-sigma: rows of A (gfdtd) and H (dga) are sorted by decreasing length
+// within windows of this many rows, so the 64 rows of a slice pad little.
+constexpr int kSigma = 1024;
 constexpr int kSpmvCap = 8192;   // blocks of the other SELL kernels
 
 // Carrier of Source::Compute in long double (Source.hpp:316-327); 0 for t<0.
@@ -75,17 +81,26 @@ struct Csr32 {
 };
 
 struct Sell {
-    DevBuf sptr, col, val, sg;
+    DevBuf sptr, col, val, sg, col16, cbase;
     int32_t nrows = 0, nslices = 0;
     int64_t nnz = 0, entries = 0;
+    bool c16 = false;  // 16-bit column offsets (col16 + cbase) in the kernels' view
+    template <class V>
+    SellView<V> view(const DevBuf& v) const {
+        return SellView<V>{nrows, nslices, sptr.as<int64_t>(), c16 ? nullptr : col.as<int32_t>(),
+                           c16 ? col16.as<uint16_t>() : nullptr, cbase.as<int32_t>(), v.as<V>()};
+    }
+    template <class T>
+    SellView<T> vals() const { return view<T>(val); }
+    SellView<int8_t> signs() const { return view<int8_t>(sg); }
 };
 
 // Launch shape of the tuned SELL kernels: grid size (0 = one slice per
 // wave) and entries per gather batch (4, 8 or 16). Results do not depend on
 // either (each row is summed by one lane in column order).
-enum TuneKernel { TK_CT = 0, TK_H, TK_N, TK_W, TK_G, TK_COUNT };
+enum TuneKernel { TK_CT = 0, TK_H, TK_N, TK_W, TK_G, TK_WU, TK_COUNT };
 const char* const kTuneName[TK_COUNT] = {"sell_ct_kernel", "sell_h_kernel", "sell_n_kernel", "cg_w_kernel",
-                                         "cg_g_kernel"};
+                                         "cg_pg_kernel", "cg_wu_kernel"};
 struct KTune {
     int blocks = 0;
     int unroll = 8;
@@ -150,6 +165,8 @@ struct tf_ctx {
     bool c_fixed_signs = false;
     DevBuf c_col, c_sg, h_edge, curl_f, curl_u;
     DevBuf free_edge, dir_edge, dir_new, fb, pre, x, r, z, p0, p1, q, rhs, cg_part, cg_state;
+    DevBuf rs0, rs1;  // (r, s) pairs of the learned passes, even / odd pass
+    std::vector<int32_t> h_perm;  // SELL row r of H holds row h_perm[r] of the problem's H
     int32_t n_free = 0, n_dir = 0;
     std::vector<int32_t> h_dir_edge;
     std::vector<double> h_fixed_value;
@@ -225,14 +242,14 @@ size_t tf_ctx::device_bytes() const {
     size_t s = 0;
     const DevBuf* all[] = {&U, &Fv, &B, &fit_ep, &fit_nu, &fe_sg, &fe_ca, &fe_cb, &fe_f4, &fh_sg, &fh_cm, &fh_cn, &fh_e4,
                            &sp_edge, &sp_f4, &sp_sg, &sp_ca, &sp_cb, &sp_kind, &sp_tb, &sp_te, &c_col, &c_sg,
-                           &h_edge, &curl_f, &curl_u, &free_edge, &dir_edge, &dir_new, &fb, &pre, &x, &r, &z,
+                           &h_edge, &curl_f, &curl_u, &free_edge, &dir_edge, &dir_new, &fb, &pre, &x, &r, &rs0, &rs1, &z,
                            &p0, &p1, &q, &rhs, &cg_part, &cg_state, &inj_edge, &inj_kind, &inj_tb, &inj_te,
                            &term_src, &term_w, &carr, &probes, &samples, &poy, &row_dt_h, &row_dt_f};
     for (auto b : all) s += b->bytes;
     const Csr32* cs[] = {&Ct};
     for (auto c : cs) s += c->ptr.bytes + c->col.bytes + c->val.bytes + c->sg.bytes;
     const Sell* ss[] = {&sCt, &H, &N, &A, &Ef, &R1f, &Ctf};
-    for (auto c : ss) s += c->sptr.bytes + c->col.bytes + c->val.bytes + c->sg.bytes;
+    for (auto c : ss) s += c->sptr.bytes + c->col.bytes + c->val.bytes + c->sg.bytes + c->col16.bytes + c->cbase.bytes;
     for (auto b : lp.bufs()) s += b->bytes;
     return s;
 }
@@ -286,11 +303,16 @@ void upload_sell(Sell& d, const tf_csr& c, bool values) {
     std::vector<int8_t> sg(values ? 0 : d.entries, 0);
     for (int32_t s = 0; s < d.nslices; ++s) {
         const int64_t w = (sptr[s + 1] - sptr[s]) / kSlice;
+        // rows without entries (and lanes past the last row) pad with a column
+        // the slice already uses, so that padding does not widen its span
+        int32_t slice_col = 0;
+        for (int32_t r = s * kSlice; r < std::min(c.nrows, (s + 1) * kSlice); ++r)
+            if (c.rowptr[r + 1] > c.rowptr[r]) { slice_col = c.col[c.rowptr[r]]; break; }
         for (int lane = 0; lane < kSlice; ++lane) {
             const int32_t r = s * kSlice + lane;
             int64_t b = 0, e = 0;
             if (r < c.nrows) { b = c.rowptr[r]; e = c.rowptr[r + 1]; }
-            const int32_t padc = e > b ? c.col[e - 1] : 0;
+            const int32_t padc = e > b ? c.col[e - 1] : slice_col;
             for (int64_t k = 0; k < w; ++k) {
                 const int64_t o = sptr[s] + k * kSlice + lane;
                 if (b + k < e) {
@@ -307,6 +329,71 @@ void upload_sell(Sell& d, const tf_csr& c, bool values) {
     upload(d.col, col);
     if (values) upload(d.val, val);
     else upload(d.sg, sg);
+    // 16-bit columns when every slice spans fewer than 65536 columns
+    std::vector<int32_t> cb(std::max(d.nslices, 1), 0);
+    bool fits = true;
+    for (int32_t s = 0; s < d.nslices && fits; ++s) {
+        int32_t lo = INT32_MAX, hi = 0;
+        for (int64_t o = sptr[s]; o < sptr[s + 1]; ++o) { lo = std::min(lo, col[o]); hi = std::max(hi, col[o]); }
+        if (sptr[s + 1] == sptr[s]) lo = hi = 0;
+        cb[s] = lo;
+        fits = (int64_t)hi - lo <= 65535;
+    }
+    d.c16 = fits;
+    if (fits) {
+        std::vector<uint16_t> c16(std::max<int64_t>(d.entries, 1), 0);
+        for (int32_t s = 0; s < d.nslices; ++s)
+            for (int64_t o = sptr[s]; o < sptr[s + 1]; ++o) c16[o] = (uint16_t)(col[o] - cb[s]);
+        upload(d.col16, c16);
+    }
+    upload(d.cbase, cb);
+}
+
+// Row order for SELL-C-sigma: within each window of kSigma consecutive rows,
+// rows sorted by decreasing length (stable). perm[new] = old.
+std::vector<int32_t> sigma_order(const tf_csr& m) {
+    std::vector<int32_t> perm(m.nrows);
+    for (int32_t i = 0; i < m.nrows; ++i) perm[i] = i;
+    for (int32_t w0 = 0; w0 < m.nrows; w0 += kSigma) {
+        const int32_t w1 = std::min(m.nrows, w0 + kSigma);
+        std::stable_sort(perm.begin() + w0, perm.begin() + w1, [&](int32_t a, int32_t b) {
+            return m.rowptr[a + 1] - m.rowptr[a] > m.rowptr[b + 1] - m.rowptr[b];
+        });
+    }
+    return perm;
+}
+
+// Rows rows[0..n) of m in that order (entries keep their order), columns
+// relabeled through cmap when given.
+struct CsrBuf {
+    std::vector<int64_t> rp;
+    std::vector<int32_t> col;
+    std::vector<double> val;
+    std::vector<int8_t> sg;
+    tf_csr v{};
+};
+void select_rows(const tf_csr& m, const int32_t* rows, int32_t n, const int32_t* cmap, CsrBuf& o) {
+    o.rp.assign(1, 0);
+    o.col.clear();
+    o.val.clear();
+    o.sg.clear();
+    for (int32_t k = 0; k < n; ++k) {
+        const int32_t e = rows[k];
+        for (int64_t q = m.rowptr[e]; q < m.rowptr[e + 1]; ++q) {
+            o.col.push_back(cmap ? cmap[m.col[q]] : m.col[q]);
+            if (m.val) o.val.push_back(m.val[q]);
+            if (m.sgn) o.sg.push_back(m.sgn[q]);
+        }
+        o.rp.push_back((int64_t)o.col.size());
+    }
+    o.v = tf_csr{};
+    o.v.nrows = n;
+    o.v.ncols = m.ncols;
+    o.v.nnz = o.rp.back();
+    o.v.rowptr = o.rp.data();
+    o.v.col = o.col.data();
+    o.v.val = m.val ? o.val.data() : nullptr;
+    o.v.sgn = m.sgn ? o.sg.data() : nullptr;
 }
 
 int sell_blocks(const Sell& m, int cap) {
@@ -461,58 +548,48 @@ void create_tet(tf_ctx* c, const tf_problem* p) {
     zero_field<T>(c->curl_u, F, c->stream);
     zero_field<T>(c->curl_f, E, c->stream);
     if (p->method == TF_METHOD_DGA) {
-        upload_sell<T>(c->H, d.H, true);
-        std::vector<int32_t> he(d.H.nrows, -1);
+        // H rows in SELL-C-sigma order; h_edge maps a SELL row to its edge
+        c->h_perm = sigma_order(d.H);
+        CsrBuf hs;
+        select_rows(d.H, c->h_perm.data(), d.H.nrows, nullptr, hs);
+        upload_sell<T>(c->H, hs.v, true);
+        std::vector<int32_t> row_edge(d.H.nrows, -1);
         for (int32_t e = 0; e < E; ++e)
-            if (d.h_row[e] >= 0) he[d.h_row[e]] = e;
-        for (int32_t r = 0; r < d.H.nrows; ++r)
+            if (d.h_row[e] >= 0) row_edge[d.h_row[e]] = e;
+        std::vector<int32_t> he(d.H.nrows);
+        for (int32_t r = 0; r < d.H.nrows; ++r) {
+            he[r] = row_edge[c->h_perm[r]];
             if (he[r] < 0) throw std::runtime_error("H row without an edge");
+        }
         upload(c->h_edge, he);
     } else {
-        upload_sell<T>(c->A, d.A, true);
-        c->hA_ptr.assign(d.A.rowptr, d.A.rowptr + d.A.nrows + 1);
-        c->hA_col.assign(d.A.col, d.A.col + d.A.nnz);
-        c->hA_val.assign(d.A.val, d.A.val + d.A.nnz);
-        c->hA_sptr = sell_layout(d.A);
-        // free rows of E, RHSmat1 and Ct for the right-hand side
+        // The CG runs over the free rows in SELL-C-sigma order of A: row k of
+        // every free-row operator (A, and the free rows of E, RHSmat1 and Ct
+        // for the right-hand side) is free edge free_edges[perm[k]], and A's
+        // columns are relabeled to match, so all CG vectors use that order.
+        if (d.A.nrows != d.n_free) throw std::runtime_error("A must have one row per free edge");
+        const std::vector<int32_t> perm = sigma_order(d.A);
+        std::vector<int32_t> inv(d.n_free);
+        for (int32_t k = 0; k < d.n_free; ++k) inv[perm[k]] = k;
+        CsrBuf a;
+        select_rows(d.A, perm.data(), d.n_free, inv.data(), a);
+        upload_sell<T>(c->A, a.v, true);
+        c->hA_ptr = a.rp;
+        c->hA_col = a.col;
+        c->hA_val = a.val;
+        c->hA_sptr = sell_layout(a.v);
+        std::vector<int32_t> fe(d.n_free);
+        for (int32_t k = 0; k < d.n_free; ++k) fe[k] = d.free_edges[perm[k]];
         {
-            auto free_rows = [&](const tf_csr& m, std::vector<int64_t>& rp, std::vector<int32_t>& col,
-                                 std::vector<double>& val, std::vector<int8_t>& sg) {
-                rp.assign(1, 0);
-                for (int32_t k = 0; k < d.n_free; ++k) {
-                    const int32_t e = d.free_edges[k];
-                    for (int64_t q = m.rowptr[e]; q < m.rowptr[e + 1]; ++q) {
-                        col.push_back(m.col[q]);
-                        if (m.val) val.push_back(m.val[q]);
-                        if (m.sgn) sg.push_back(m.sgn[q]);
-                    }
-                    rp.push_back((int64_t)col.size());
-                }
-                tf_csr o{};
-                o.nrows = d.n_free;
-                o.ncols = m.ncols;
-                o.nnz = rp.back();
-                o.rowptr = rp.data();
-                o.col = col.data();
-                o.val = m.val ? val.data() : nullptr;
-                o.sgn = m.sgn ? sg.data() : nullptr;
-                return o;
-            };
-            std::vector<int64_t> rp;
-            std::vector<int32_t> col;
-            std::vector<double> val;
-            std::vector<int8_t> sg;
-            tf_csr ef = free_rows(d.E, rp, col, val, sg);
-            upload_sell<T>(c->Ef, ef, true);
-            rp.clear(); col.clear(); val.clear(); sg.clear();
-            tf_csr rf = free_rows(d.RHS1, rp, col, val, sg);
-            upload_sell<T>(c->R1f, rf, true);
-            rp.clear(); col.clear(); val.clear(); sg.clear();
-            tf_csr cf = free_rows(d.Ct, rp, col, val, sg);
-            upload_sell<T>(c->Ctf, cf, false);
+            CsrBuf m;
+            select_rows(d.E, fe.data(), d.n_free, nullptr, m);
+            upload_sell<T>(c->Ef, m.v, true);
+            select_rows(d.RHS1, fe.data(), d.n_free, nullptr, m);
+            upload_sell<T>(c->R1f, m.v, true);
+            select_rows(d.Ct, fe.data(), d.n_free, nullptr, m);
+            upload_sell<T>(c->Ctf, m.v, false);
         }
         c->n_free = d.n_free;
-        std::vector<int32_t> fe(d.free_edges, d.free_edges + d.n_free);
         upload(c->free_edge, fe);
         std::vector<uint8_t> is_free(E, 0);
         for (int32_t e : fe) is_free[e] = 1;
@@ -529,12 +606,13 @@ void create_tet(tf_ctx* c, const tf_problem* p) {
         // Jacobi preconditioner: 1/A_jj, or 1 if missing / zero (CG.hpp:119-128)
         std::vector<T> pre(d.n_free, (T)1);
         for (int32_t i = 0; i < d.n_free; ++i)
-            for (int64_t q = d.A.rowptr[i]; q < d.A.rowptr[i + 1]; ++q)
-                if (d.A.col[q] == i && d.A.val[q] != 0) pre[i] = (T)(1.0 / d.A.val[q]);
+            for (int64_t q = a.rp[i]; q < a.rp[i + 1]; ++q)
+                if (a.col[q] == i && a.val[q] != 0) pre[i] = (T)(1.0 / a.val[q]);
         upload(c->pre, pre);
         const int64_t n = std::max<int32_t>(d.n_free, 1);
         for (DevBuf* b : {&c->x, &c->r, &c->z, &c->p0, &c->p1, &c->q, &c->rhs}) zero_field<T>(*b, n, c->stream);
-        c->cg_part.alloc(sizeof(double) * (kCgPartA + 3 * kCgPartB + kCgPartC));
+        for (DevBuf* b : {&c->rs0, &c->rs1}) zero_field<RS<T>>(*b, n, c->stream);
+        c->cg_part.alloc(sizeof(double) * (kCgPartA + 3 * kCgPartB + 2 * kCgPartC));
         c->cg_state.alloc(sizeof(CgState));
         TF_HIP(hipMemsetAsync(c->cg_state.p, 0, sizeof(CgState), c->stream));
         TF_HIP(hipHostMalloc((void**)&c->h_state, sizeof(CgState), hipHostMallocDefault));
@@ -781,9 +859,8 @@ void run_dga(tf_ctx* c, int64_t first, int64_t n, tf_stats* st) {
         const double t = (double)i * c->dt;
         const double* cs = c->carr.as<double>() + (k + 1) * ns;
         ev_begin(c, "sell_ct_kernel");
-        TF_LAUNCH_NB(c->tune[TK_CT].unroll, sell_ct_kernel, T, gb_ct, c->stream, c->E, c->sCt.nslices, c->sCt.sptr.as<int64_t>(),
-                                                           c->sCt.col.as<int32_t>(), c->sCt.sg.as<int8_t>(),
-                                                           c->Fv.as<T>(), c->curl_f.as<T>());
+        TF_LAUNCH_NB(c->tune[TK_CT].unroll, sell_ct_kernel, T, gb_ct, c->stream, c->sCt.signs(), c->Fv.as<T>(),
+                     c->curl_f.as<T>());
         ev_end(c);
         if (c->n_inj) {
             ev_begin(c, "dga_inject_kernel");
@@ -794,10 +871,8 @@ void run_dga(tf_ctx* c, int64_t first, int64_t n, tf_stats* st) {
             ev_end(c);
         }
         ev_begin(c, "sell_h_kernel");
-        TF_LAUNCH_NB(c->tune[TK_H].unroll, sell_h_kernel, T, gb_h, c->stream, c->H.nrows, c->H.nslices, c->H.sptr.as<int64_t>(),
-                                                         c->H.col.as<int32_t>(), c->H.val.as<T>(),
-                                                         c->h_edge.as<int32_t>(), c->curl_f.as<T>(), c->U.as<T>(),
-                                                         c->dt, hdt);
+        TF_LAUNCH_NB(c->tune[TK_H].unroll, sell_h_kernel, T, gb_h, c->stream, c->H.vals<T>(), c->h_edge.as<int32_t>(),
+                     c->curl_f.as<T>(), c->U.as<T>(), c->dt, hdt);
         ev_end(c);
         if (c->n_inj) {
             ev_begin(c, "dga_poynting_kernel");
@@ -813,9 +888,8 @@ void run_dga(tf_ctx* c, int64_t first, int64_t n, tf_stats* st) {
                                                                         c->curl_u.as<T>(), c->B.as<T>(), c->dt, fdt);
         ev_end(c);
         ev_begin(c, "sell_n_kernel");
-        TF_LAUNCH_NB(c->tune[TK_N].unroll, sell_n_kernel, T, gb_n, c->stream, c->N.nrows, c->N.nslices, c->N.sptr.as<int64_t>(),
-                                                         c->N.col.as<int32_t>(), c->N.val.as<T>(), c->curl_u.as<T>(),
-                                                         c->Fv.as<T>(), c->dt, 0, fdt);
+        TF_LAUNCH_NB(c->tune[TK_N].unroll, sell_n_kernel, T, gb_n, c->stream, c->N.vals<T>(), c->curl_u.as<T>(),
+                     c->Fv.as<T>(), c->dt, 0, fdt);
         ev_end(c);
         if (want_sample(c, i, t)) sample<T>(c, t);
         if (c->profiling && (k & 63) == 63) ev_flush(c);
@@ -835,82 +909,99 @@ void run_dga(tf_ctx* c, int64_t first, int64_t n, tf_stats* st) {
 }
 
 // One CG solve on the device (single-reduction PCG, see sell.hpp): cg_init,
-// then passes it = 0, 1, ... of (cg_w, cg_upd) with Jacobi, or (cg_g, cg_w,
-// cg_upd) with the learned preconditioner. Launches are issued in batches
+// then passes it = 0, 1, ... of (cg_w, cg_upd) with Jacobi, or (cg_pg,
+// cg_wu) with the learned preconditioner. Launches are issued in batches
 // sized from the previous solve's count; kernels past the stop test return
 // at once. `snaps` lists (iteration k, bank slot) captures of the iterate
-// (r_k, x_k), taken in stream order after the update that produces it. The
-// final state is read back into h_state.
+// (r_k, x_k) (Jacobi solves), taken in stream order after the update that
+// produces it. The final state is read back into h_state.
 //
-// Vectors: u = z, w = q, p = p0, s = p1. Partials (fp64, one per block):
-// [delta: kCgPartA][rr: kCgPartB][gamma ping: kCgPartB][gamma pong: kCgPartB][gamma of cg_g: kCgPartC]
+// Vectors: u = z, w = q, p = p0, s = p1; the learned passes keep (r, s)
+// pairs in rs0 (even passes) and rs1 (odd passes). Partials (fp64, one per
+// block): [delta: kCgPartA][rr: kCgPartB][gamma ping, pong: 2 kCgPartB]
+// [gamma of cg_pg: kCgPartC][rr of cg_pg: kCgPartC].
 template <class T>
 void cg_solve(tf_ctx* c, const CgState& init, bool learned, const std::vector<std::pair<int, int>>* snaps) {
     const int32_t nf = c->n_free;
     int& guess = learned ? c->cg_guess_pc : c->cg_guess;
-    const int nb_w = tuned_blocks(c, TK_W, c->A, kCgPartA);
+    const int nb_w = tuned_blocks(c, learned ? TK_WU : TK_W, c->A, kCgPartA);
     const int nb_b = std::max(1, std::min(blocks_for(nf), kCgPartB));
     const int nb_g = tuned_blocks(c, TK_G, c->A, kCgPartC);
     double* part_del = c->cg_part.as<double>();
     double* part_rr = part_del + kCgPartA;
     double* part_gj[2] = {part_rr + kCgPartB, part_rr + 2 * kCgPartB};
-    double* part_gg = part_rr + 3 * kCgPartB;
-    const int jac = learned ? 0 : 1;
+    double* part_gl = part_rr + 3 * kCgPartB;
+    double* part_rl = part_gl + kCgPartC;
     CgState* dst = c->cg_state.as<CgState>();
-    T *x = c->x.as<T>(), *r = c->r.as<T>(), *u = c->z.as<T>(), *w = c->q.as<T>(), *p = c->p0.as<T>(),
-      *sv = c->p1.as<T>();
+    T *x = c->x.as<T>(), *u = c->z.as<T>(), *w = c->q.as<T>(), *p = c->p0.as<T>(), *sv = c->p1.as<T>();
+    T* rj = c->r.as<T>();
+    RS<T>* rs[2] = {c->rs0.as<RS<T>>(), c->rs1.as<RS<T>>()};
+    const SellView<T> A = c->A.vals<T>();
+    const SellView<uint16_t> G = c->A.view<uint16_t>(c->lp.gval);
     auto capture = [&](int k) {
         if (!snaps) return;
         for (auto& sn : *snaps)
             if (sn.first == k)
-                bank_capture_kernel<T><<<blocks_for(nf), kBlock, 0, c->stream>>>(nf, sn.second, r, x,
+                bank_capture_kernel<T><<<blocks_for(nf), kBlock, 0, c->stream>>>(nf, sn.second, rj, x,
                                                                                    c->lp.Rb.as<float>(),
                                                                                    c->lp.Cb.as<float>());
     };
+    // profiled launches of this solve: (record, pass, 1 for the learned pass's first kernel)
+    std::vector<std::array<int64_t, 3>> tags;
+    auto tag = [&](int it, int first) {
+        if (c->profiling) tags.push_back({(int64_t)c->recs.size() - 1, it, first});
+    };
     TF_HIP(hipMemcpyAsync(dst, &init, sizeof(CgState), hipMemcpyHostToDevice, c->stream));
     ev_begin(c, "cg_init_kernel");
-    cg_init_kernel<T><<<nb_b, kBlock, 0, c->stream>>>(nf, c->A.nslices, c->A.sptr.as<int64_t>(), c->A.col.as<int32_t>(),
-                                                      c->A.val.as<T>(), c->pre.as<T>(), c->rhs.as<T>(), x, r, u,
-                                                      part_rr, part_gj[0], dst, jac);
+    TF_LAUNCH_NB(8, cg_init_kernel, T, nb_b, c->stream, A, c->pre.as<T>(), c->rhs.as<T>(), x,
+                 learned ? &rs[0]->r : rj, learned ? 2 : 1, u, part_rr, part_gj[0], dst, learned ? 0 : 1);
     ev_end(c);
     capture(0);
     int it = 0;
-    // With per-kernel profiling on, passes go one at a time and the pass in
-    // which the stop test fires (its kernels return after the test) is timed
-    // under its own name, so kernel averages cover working launches only.
-    int batch = c->profiling ? 1 : std::max(1, guess + 1);
+    int batch = std::max(1, guess + 1);
     for (;;) {
         const int stop = std::min(it + batch, init.maxiter + 1);
-        const size_t rec0 = c->recs.size();
         for (; it < stop; ++it) {
             if (learned) {
-                ev_begin(c, "cg_g_kernel");
-                TF_LAUNCH_NB(c->tune[TK_G].unroll, cg_g_kernel, T, nb_g, c->stream, it, nf, c->A.nslices, c->A.sptr.as<int64_t>(),
-                                                               c->A.col.as<int32_t>(), c->lp.gval.as<float>(), r, u, 1,
-                                                               part_rr, nb_b, part_gg, dst);
+                ev_begin(c, "cg_pg_kernel");
+                TF_LAUNCH_NB(c->tune[TK_G].unroll, cg_pg_kernel, T, nb_g, c->stream, it, G,
+                             rs[it == 0 ? 0 : (it - 1) & 1], rs[it & 1], p, x, u, part_del, nb_w, part_gl, part_rl,
+                             dst);
                 ev_end(c);
-                if (it == init.maxiter) break;  // the pass at maxiter only runs the stop test
+                tag(it, 1);
+                ev_begin(c, "cg_wu_kernel");
+                TF_LAUNCH_NB(c->tune[TK_WU].unroll, cg_wu_kernel, T, nb_w, c->stream, it, A, u, p,
+                             rs[(it + 1) & 1], rs[it & 1], part_rl, part_gl, nb_g, part_del, dst);
+                ev_end(c);
+                tag(it, 0);
+                continue;  // at it == maxiter the stop test of cg_wu ends the solve
             }
             ev_begin(c, "cg_w_kernel");
-            TF_LAUNCH_NB(c->tune[TK_W].unroll, cg_w_kernel, T, nb_w, c->stream, it, nf, c->A.nslices, c->A.sptr.as<int64_t>(),
-                                                           c->A.col.as<int32_t>(), c->A.val.as<T>(), u, w, jac,
-                                                           part_rr, nb_b, part_del, dst);
+            TF_LAUNCH_NB(c->tune[TK_W].unroll, cg_w_kernel, T, nb_w, c->stream, it, A, u, w, 1, part_rr, nb_b,
+                         part_del, dst);
             ev_end(c);
-            if (it == init.maxiter) break;
+            tag(it, 0);
+            if (it == init.maxiter) break;  // the pass at maxiter only runs the stop test
             ev_begin(c, "cg_upd_kernel");
-            cg_upd_kernel<T><<<nb_b, kBlock, 0, c->stream>>>(it, nf, c->pre.as<T>(), x, r, u, w, p, sv,
-                                                             learned ? part_gg : part_gj[it & 1],
-                                                             learned ? nb_g : nb_b, part_del, nb_w, part_rr,
-                                                             part_gj[(it + 1) & 1], dst, jac);
+            cg_upd_kernel<T><<<nb_b, kBlock, 0, c->stream>>>(it, nf, c->pre.as<T>(), x, rj, u, w, p, sv,
+                                                             part_gj[it & 1], nb_b, part_del, nb_w, part_rr,
+                                                             part_gj[(it + 1) & 1], dst);
             ev_end(c);
+            tag(it, 0);
             capture(it + 1);
         }
         TF_HIP(hipMemcpyAsync(c->h_state, dst, sizeof(CgState), hipMemcpyDeviceToHost, c->stream));
         TF_HIP(hipStreamSynchronize(c->stream));
-        if (c->profiling && c->h_state->done && c->h_state->iter == it - 1)
-            for (size_t k = rec0; k < c->recs.size(); ++k) c->recs[k].name = "cg_stop_pass";
         if (c->h_state->done || it > init.maxiter) break;
-        batch = c->profiling ? 1 : 4;
+        batch = 4;
+    }
+    // Launches past the stop test did no work: timed under their own name,
+    // so that per-kernel averages cover working launches only (the learned
+    // pass that reaches the stop test still updates x and r in cg_pg).
+    if (c->profiling && c->h_state->done) {
+        const int iters = c->h_state->iter;
+        for (auto& t : tags)
+            if (t[1] > iters || (t[1] == iters && !t[2])) c->recs[t[0]].name = "cg_stop_pass";
     }
     guess = c->h_state->iter;
 }
@@ -944,20 +1035,27 @@ void learned_setup(tf_ctx* c, const tf_cg_opts* cg) {
     const auto& val = c->hA_val;
     std::vector<double> d(n, 0.0);
     for (int32_t e = 0; e < n; ++e) {
-        for (int64_t q = rp[e]; q < rp[e + 1]; ++q) {
-            if (q > rp[e] && col[q] <= col[q - 1]) throw std::runtime_error("A rows must have sorted columns");
+        for (int64_t q = rp[e]; q < rp[e + 1]; ++q)
             if (col[q] == e) d[e] = val[q];
-        }
         if (!(d[e] > 0)) throw std::runtime_error("A has a non-positive diagonal entry");
     }
     std::vector<double> is(n);
     for (int32_t e = 0; e < n; ++e) is[e] = 1.0 / std::sqrt(d[e]);
+    // rows keep their CSR entry order (that of the SELL layout); ord lists
+    // each row's positions by increasing column for lookups and merges
+    std::vector<int64_t> ord(col.size());
+    for (int32_t e = 0; e < n; ++e) {
+        for (int64_t q = rp[e]; q < rp[e + 1]; ++q) ord[q] = q;
+        std::sort(ord.begin() + rp[e], ord.begin() + rp[e + 1], [&](int64_t a, int64_t b) { return col[a] < col[b]; });
+        for (int64_t q = rp[e] + 1; q < rp[e + 1]; ++q)
+            if (col[ord[q]] == col[ord[q - 1]]) throw std::runtime_error("A has a repeated column in a row");
+    }
     auto find = [&](int32_t row, int32_t cl) -> int64_t {
-        const int32_t* b = col.data() + rp[row];
-        const int32_t* en = col.data() + rp[row + 1];
-        const int32_t* it = std::lower_bound(b, en, cl);
-        if (it == en || *it != cl) throw std::runtime_error("A pattern is not symmetric");
-        return it - col.data();
+        const int64_t* b = ord.data() + rp[row];
+        const int64_t* en = ord.data() + rp[row + 1];
+        const int64_t* it = std::lower_bound(b, en, cl, [&](int64_t q, int32_t v) { return col[q] < v; });
+        if (it == en || col[*it] != cl) throw std::runtime_error("A pattern is not symmetric");
+        return *it;
     };
     auto sell_pos = [&](int32_t row, int64_t q) {
         return c->hA_sptr[row / kSlice] + (q - rp[row]) * kSlice + row % kSlice;
@@ -1008,10 +1106,11 @@ void learned_setup(tf_ctx* c, const tf_cg_opts* cg) {
         // (Â^2)_ej: merge of rows e and j
         double a2 = 0;
         for (int64_t x = rp[e], y = rp[j]; x < rp[e + 1] && y < rp[j + 1];) {
-            if (col[x] < col[y]) ++x;
-            else if (col[y] < col[x]) ++y;
+            const int64_t qx = ord[x], qy = ord[y];
+            if (col[qx] < col[qy]) ++x;
+            else if (col[qy] < col[qx]) ++y;
             else {
-                a2 += val[x] * is[e] * is[col[x]] * val[y] * is[j] * is[col[y]];
+                a2 += val[qx] * is[e] * is[col[qx]] * val[qy] * is[j] * is[col[qy]];
                 ++x;
                 ++y;
             }
@@ -1074,7 +1173,7 @@ void learned_setup(tf_ctx* c, const tf_cg_opts* cg) {
     L.gy.alloc((size_t)L.n_pad * 4);
     TF_HIP(hipMemsetAsync(L.gy.p, 0, L.gy.bytes, c->stream));
     L.ghat.alloc(entries * 4);
-    L.gval.alloc(entries * 4);
+    L.gval.alloc(entries * 2);  // bf16 bits
     L.gpos.alloc(entries * 4);
     // parameters: He-uniform W1, W2 from a fixed LCG, zero biases, zero output layer
     std::vector<float> P(kMlpParams, 0.f);
@@ -1183,7 +1282,7 @@ void learned_train(tf_ctx* c) {
     // frozen G from the best parameters
     mlp_forward_kernel<<<fb, kBlock, 0, s>>>(tiles, L.Pbest.as<float>(), L.X.as<short>(), L.y.as<float>());
     expand_kernel<<<xb, kBlock, 0, s>>>(entries, L.pos2pair.as<int32_t>(), L.base.as<float>(), L.scale.as<float>(),
-                                        L.y.as<float>(), nullptr, L.gval.as<float>());
+                                        L.y.as<float>(), nullptr, L.gval.as<uint16_t>());
     TF_HIP(hipGetLastError());
     TF_HIP(hipStreamSynchronize(s));
     L.train_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -1272,9 +1371,8 @@ void run_gfdtd(tf_ctx* c, int64_t first, int64_t n, const tf_cg_opts* cg, tf_sta
                                                                         c->curl_u.as<T>(), c->B.as<T>(), c->dt, nullptr);
         ev_end(c);
         ev_begin(c, "sell_n_kernel");
-        TF_LAUNCH_NB(c->tune[TK_N].unroll, sell_n_kernel, T, gb_n, c->stream, c->N.nrows, c->N.nslices, c->N.sptr.as<int64_t>(),
-                                                         c->N.col.as<int32_t>(), c->N.val.as<T>(), c->B.as<T>(),
-                                                         c->Fv.as<T>(), c->dt, 1, nullptr);
+        TF_LAUNCH_NB(c->tune[TK_N].unroll, sell_n_kernel, T, gb_n, c->stream, c->N.vals<T>(), c->B.as<T>(),
+                     c->Fv.as<T>(), c->dt, 1, nullptr);
         ev_end(c);
         if (allow_print(c, t - c->dt)) sample<T>(c, t - c->dt);
         if (c->n_inj) {
@@ -1287,9 +1385,7 @@ void run_gfdtd(tf_ctx* c, int64_t first, int64_t n, const tf_cg_opts* cg, tf_sta
         }
         ev_begin(c, "gf_rhs_kernel");
         gf_rhs_kernel<T><<<sell_blocks(c->Ef, kSpmvCap), kBlock, 0, c->stream>>>(
-            nf, c->Ef.nslices, c->free_edge.as<int32_t>(), c->Ef.sptr.as<int64_t>(), c->Ef.col.as<int32_t>(),
-            c->Ef.val.as<T>(), c->R1f.sptr.as<int64_t>(), c->R1f.col.as<int32_t>(), c->R1f.val.as<T>(),
-            c->Ctf.sptr.as<int64_t>(), c->Ctf.col.as<int32_t>(), c->Ctf.sg.as<int8_t>(), c->U.as<T>(),
+            c->Ef.vals<T>(), c->R1f.vals<T>(), c->Ctf.signs(), c->free_edge.as<int32_t>(), c->U.as<T>(),
             c->dir_new.as<T>(), c->Fv.as<T>(), c->fb.as<T>(), c->rhs.as<T>(), c->dt);
         ev_end(c);
         gfdtd_solve<T>(c, init, learned, st);
@@ -1597,41 +1693,47 @@ void autotune(tf_ctx* c) {
     hipStream_t s = c->stream;
     if (c->method == TF_METHOD_DGA) {
         c->tune[TK_CT] = tune_one(c, TK_CT, c->sCt, kSpmvCap, [&](int b, int u) {
-            TF_LAUNCH_NB(u, sell_ct_kernel, T, b, s, c->E, c->sCt.nslices, c->sCt.sptr.as<int64_t>(),
-                         c->sCt.col.as<int32_t>(), c->sCt.sg.as<int8_t>(), c->Fv.as<T>(), c->curl_f.as<T>());
+            TF_LAUNCH_NB(u, sell_ct_kernel, T, b, s, c->sCt.signs(), c->Fv.as<T>(), c->curl_f.as<T>());
         });
         c->tune[TK_H] = tune_one(c, TK_H, c->H, kSpmvCap, [&](int b, int u) {
-            TF_LAUNCH_NB(u, sell_h_kernel, T, b, s, c->H.nrows, c->H.nslices, c->H.sptr.as<int64_t>(),
-                         c->H.col.as<int32_t>(), c->H.val.as<T>(), c->h_edge.as<int32_t>(), c->curl_f.as<T>(),
+            TF_LAUNCH_NB(u, sell_h_kernel, T, b, s, c->H.vals<T>(), c->h_edge.as<int32_t>(), c->curl_f.as<T>(),
                          c->U.as<T>(), 0.0, nullptr);
         });
         c->tune[TK_N] = tune_one(c, TK_N, c->N, kSpmvCap, [&](int b, int u) {
-            TF_LAUNCH_NB(u, sell_n_kernel, T, b, s, c->N.nrows, c->N.nslices, c->N.sptr.as<int64_t>(),
-                         c->N.col.as<int32_t>(), c->N.val.as<T>(), c->curl_u.as<T>(), c->Fv.as<T>(), 0.0, 0,
-                         nullptr);
+            TF_LAUNCH_NB(u, sell_n_kernel, T, b, s, c->N.vals<T>(), c->curl_u.as<T>(), c->Fv.as<T>(), 0.0, 0, nullptr);
         });
     } else if (c->method == TF_METHOD_GFDTD) {
         c->tune[TK_N] = tune_one(c, TK_N, c->N, kSpmvCap, [&](int b, int u) {
-            TF_LAUNCH_NB(u, sell_n_kernel, T, b, s, c->N.nrows, c->N.nslices, c->N.sptr.as<int64_t>(),
-                         c->N.col.as<int32_t>(), c->N.val.as<T>(), c->B.as<T>(), c->Fv.as<T>(), c->dt, 1,
-                         nullptr);
+            TF_LAUNCH_NB(u, sell_n_kernel, T, b, s, c->N.vals<T>(), c->B.as<T>(), c->Fv.as<T>(), c->dt, 1, nullptr);
         });
-        DevBuf scratch;
-        scratch.alloc(sizeof(CgState));
-        TF_HIP(hipMemsetAsync(scratch.p, 0, sizeof(CgState), s));
+        // Tuning launches run pass 1 (the full work of a pass) on a scratch
+        // CG state that never stops (tol 0, no iteration cap) with unit
+        // partials. Their outputs go to vectors every solve rewrites before
+        // use: u (z), w (q, also standing in for x), p (p0) and the (r, s)
+        // pairs (rs0, rs1).
+        CgState sc{};
+        sc.tol = 0.0;
+        sc.maxiter = INT32_MAX;
+        sc.gam[0] = sc.gam[1] = sc.alpha[0] = sc.alpha[1] = 1.0;
+        DevBuf scratch, ones;
+        upload(scratch, std::vector<CgState>{sc});
+        upload(ones, std::vector<double>(std::max(kCgPartA, kCgPartC), 1.0));
         double* part = c->cg_part.as<double>();
-        const int32_t nf = c->n_free;
+        const double* one = ones.as<double>();
+        CgState* st = scratch.as<CgState>();
         c->tune[TK_W] = tune_one(c, TK_W, c->A, kCgPartA, [&](int b, int u) {
-            TF_LAUNCH_NB(u, cg_w_kernel, T, b, s, 1, nf, c->A.nslices, c->A.sptr.as<int64_t>(),
-                         c->A.col.as<int32_t>(), c->A.val.as<T>(), c->z.as<T>(), c->q.as<T>(), 0, part, 1, part,
-                         scratch.as<CgState>());
+            TF_LAUNCH_NB(u, cg_w_kernel, T, b, s, 1, c->A.vals<T>(), c->z.as<T>(), c->q.as<T>(), 0, one, 1, part, st);
         });
-        if (c->lp.state == LearnedPc::kActive)
+        if (c->lp.state == LearnedPc::kActive) {
             c->tune[TK_G] = tune_one(c, TK_G, c->A, kCgPartC, [&](int b, int u) {
-                TF_LAUNCH_NB(u, cg_g_kernel, T, b, s, 1, nf, c->A.nslices, c->A.sptr.as<int64_t>(),
-                             c->A.col.as<int32_t>(), c->lp.gval.as<float>(), c->r.as<T>(), c->z.as<T>(), 0, part,
-                             1, part, scratch.as<CgState>());
+                TF_LAUNCH_NB(u, cg_pg_kernel, T, b, s, 1, c->A.view<uint16_t>(c->lp.gval), c->rs0.as<RS<T>>(),
+                             c->rs1.as<RS<T>>(), c->p0.as<T>(), c->q.as<T>(), c->z.as<T>(), one, 1, part, part, st);
             });
+            c->tune[TK_WU] = tune_one(c, TK_WU, c->A, kCgPartA, [&](int b, int u) {
+                TF_LAUNCH_NB(u, cg_wu_kernel, T, b, s, 1, c->A.vals<T>(), c->z.as<T>(), c->p0.as<T>(),
+                             c->rs0.as<RS<T>>(), c->rs1.as<RS<T>>(), one, one, 1, part, st);
+            });
+        }
         TF_HIP(hipStreamSynchronize(s));
     }
 }
@@ -1671,7 +1773,9 @@ tf_status tf_create_batch(const tf_problem* const* probs, int32_t n, int32_t pre
         bt->terms.resize(n);
         bt->probes.resize(n);
         for (int32_t k = 0; k < n; ++k) bt->dt.push_back(probs[k]->dt);
-        upload(c->row_dt_h, m.h_dt);
+        std::vector<double> hdt(m.h_dt.size());
+        for (size_t r = 0; r < hdt.size(); ++r) hdt[r] = m.h_dt[c->h_perm[r]];  // H rows are in SELL order
+        upload(c->row_dt_h, hdt);
         upload(c->row_dt_f, m.f_dt);
         c->batch = std::move(bt);
         *out = c.release();
@@ -1947,6 +2051,30 @@ tf_status tf_autotune(tf_ctx* c) {
         if (c->prec == TF_PREC_FP64) autotune<double>(c);
         else autotune<float>(c);
     });
+}
+
+tf_status tf_sell_info(tf_ctx* c, int32_t which, int64_t* info) {
+    if (!c || !info || which < 0 || which > 6) { g_err = "bad argument"; return TF_ERR_ARG; }
+    const Sell* m[7] = {&c->sCt, &c->H, &c->N, &c->A, &c->Ef, &c->R1f, &c->Ctf};
+    const Sell& s = *m[which];
+    info[0] = s.nrows;
+    info[1] = s.nslices;
+    info[2] = s.nnz;
+    info[3] = s.entries;
+    info[4] = s.c16 ? 2 : 4;
+    return TF_OK;
+}
+
+tf_status tf_set_tuning(tf_ctx* c, const int32_t* blocks, const int32_t* unroll, int32_t n) {
+    if (!c || !blocks || !unroll || n < 0 || n > TK_COUNT) { g_err = "bad argument"; return TF_ERR_ARG; }
+    for (int k = 0; k < n; ++k) {
+        if (blocks[k] < 0 || (unroll[k] != 4 && unroll[k] != 8 && unroll[k] != 16)) {
+            g_err = "tuning: blocks >= 0 (0 = default), unroll 4, 8 or 16";
+            return TF_ERR_ARG;
+        }
+        c->tune[k] = KTune{blocks[k], unroll[k]};
+    }
+    return TF_OK;
 }
 
 tf_status tf_get_tuning(tf_ctx* c, int32_t* blocks, int32_t* unroll, int32_t cap, int32_t* n) {

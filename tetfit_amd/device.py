@@ -138,8 +138,27 @@ class Simulation:
         _check(L.tf_autotune(self._h))
         bl, un, n = (C.c_int32 * 8)(), (C.c_int32 * 8)(), C.c_int32()
         _check(L.tf_get_tuning(self._h, bl, un, 8, C.byref(n)))
-        names = ["sell_ct_kernel", "sell_h_kernel", "sell_n_kernel", "cg_w_kernel", "cg_g_kernel"]
+        names = ["sell_ct_kernel", "sell_h_kernel", "sell_n_kernel", "cg_w_kernel", "cg_pg_kernel", "cg_wu_kernel"]
         return {names[k]: (bl[k], un[k]) for k in range(min(n.value, len(names)))}
+
+    def set_tuning(self, choice):
+        """tf_set_tuning from {kernel: (blocks, unroll)} (kernels as in autotune())."""
+        names = ["sell_ct_kernel", "sell_h_kernel", "sell_n_kernel", "cg_w_kernel", "cg_pg_kernel", "cg_wu_kernel"]
+        cur = {}
+        bl, un, n = (C.c_int32 * 8)(), (C.c_int32 * 8)(), C.c_int32()
+        _check(hip_lib().tf_get_tuning(self._h, bl, un, 8, C.byref(n)))
+        for k in range(len(names)):
+            b, u = choice.get(names[k], (bl[k], un[k]))
+            bl[k], un[k] = int(b), int(u)
+        _check(hip_lib().tf_set_tuning(self._h, bl, un, len(names)))
+
+    def sell_info(self, which):
+        """tf_sell_info: storage of a device operator ("ct", "h", "n", "a",
+        "e", "rhs1", "ctf") as dict(rows, slices, nnz, entries, col_bytes)."""
+        k = ["ct", "h", "n", "a", "e", "rhs1", "ctf"].index(which)
+        info = (C.c_int64 * 5)()
+        _check(hip_lib().tf_sell_info(self._h, k, info))
+        return dict(rows=info[0], slices=info[1], nnz=info[2], entries=info[3], col_bytes=info[4])
 
 
 class BatchSimulation:
