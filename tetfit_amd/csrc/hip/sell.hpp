@@ -48,53 +48,48 @@ __device__ __forceinline__ double sell_value<double, uint16_t>(uint16_t v) {
 }
 
 // Sum over row (slice s, lane) of val * gather(col), accumulated in A (the
-// field type: fp64 on the plain path, fp32 on the fast path). Loads are
-// issued in batches of U entries (2U independent loads, then U independent
-// gathers) so that each lane keeps enough memory traffic in flight; the tail
-// batch is clamped to the row's last entry and predicated. Accumulation
-// stays in entry order, so the result does not depend on U or on the launch
-// shape.
+// field type: fp64 on the plain path, fp32 on the fast path). The slice is
+// uniform across the wave, so its base, width and column base live in
+// scalar registers and each entry's loads use one lane offset plus a
+// constant offset. Loads are issued in batches of U entries (2U independent
+// loads, then U independent gathers) so that each lane keeps enough memory
+// traffic in flight; the tail batch is clamped to the row's last entry and
+// predicated. Accumulation stays in entry order, so the result does not
+// depend on U or on the launch shape.
 template <typename A, int U, typename V, class G>
 __device__ __forceinline__ double sell_dot(const SellView<V>& m, int s, int lane, G gather) {
+    s = __builtin_amdgcn_readfirstlane(s);
     const int64_t base = m.sptr[s];
-    const int w = (int)((m.sptr[s + 1] - base) / kSlice);
-    const V* v = m.val + base + lane;
+    const int w = __builtin_amdgcn_readfirstlane((int)((m.sptr[s + 1] - base) / kSlice));
+    const V* v = m.val + base;
     A acc = 0;
+    auto batch = [&](int k0, auto col_at, bool full) {
+        int32_t ci[U];
+        A vi[U], gi[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int kk = full ? k0 + u : min(k0 + u, w - 1);  // clamped: loads stay inside the slice
+            ci[u] = col_at(kk * kSlice + lane);
+            vi[u] = sell_value<A, V>(v[kk * kSlice + lane]);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) gi[u] = (A)gather(ci[u]);
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (full || k0 + u < w) acc += vi[u] * gi[u];
+    };
+    auto rows = [&](auto col_at) {
+        int k0 = 0;
+        for (; k0 + U <= w; k0 += U) batch(k0, col_at, true);
+        if (k0 < w) batch(k0, col_at, false);
+    };
     if (m.col16) {
-        const uint16_t* c = m.col16 + base + lane;
-        const int32_t cb = m.cbase[s];
-        for (int k0 = 0; k0 < w; k0 += U) {
-            int32_t ci[U];
-            A vi[U], gi[U];
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const int kk = min(k0 + u, w - 1);  // clamped: every load stays inside the slice
-                ci[u] = cb + (int32_t)c[kk * 64];
-                vi[u] = sell_value<A, V>(v[kk * 64]);
-            }
-#pragma unroll
-            for (int u = 0; u < U; ++u) gi[u] = (A)gather(ci[u]);
-#pragma unroll
-            for (int u = 0; u < U; ++u)
-                if (k0 + u < w) acc += vi[u] * gi[u];
-        }
+        const uint16_t* c = m.col16 + base;
+        const int32_t cb = __builtin_amdgcn_readfirstlane(m.cbase[s]);
+        rows([&](int o) { return cb + (int32_t)c[o]; });
     } else {
-        const int32_t* c = m.col + base + lane;
-        for (int k0 = 0; k0 < w; k0 += U) {
-            int32_t ci[U];
-            A vi[U], gi[U];
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const int kk = min(k0 + u, w - 1);
-                ci[u] = c[kk * 64];
-                vi[u] = sell_value<A, V>(v[kk * 64]);
-            }
-#pragma unroll
-            for (int u = 0; u < U; ++u) gi[u] = (A)gather(ci[u]);
-#pragma unroll
-            for (int u = 0; u < U; ++u)
-                if (k0 + u < w) acc += vi[u] * gi[u];
-        }
+        const int32_t* c = m.col + base;
+        rows([&](int o) { return c[o]; });
     }
     return (double)acc;
 }
@@ -193,13 +188,18 @@ constexpr int kPartPer = kMaxPart / kBlock;
 struct Partials {
     double v[kPartPer];
 };
+// Loads without branches: an index past n reads p[0] (every partial array
+// is allocated with at least one entry) and is masked to zero.
 __device__ __forceinline__ Partials load_partials(const double* __restrict__ p, int n) {
     Partials r;
+    double t[kPartPer];
 #pragma unroll
     for (int k = 0; k < kPartPer; ++k) {
         const int i = threadIdx.x + k * kBlock;
-        r.v[k] = i < n ? p[i] : 0.0;
+        t[k] = p[i < n ? i : 0];
     }
+#pragma unroll
+    for (int k = 0; k < kPartPer; ++k) r.v[k] = threadIdx.x + k * kBlock < n ? t[k] : 0.0;
     return r;
 }
 __device__ __forceinline__ double thread_sum(const Partials& r) {
@@ -498,24 +498,47 @@ __global__ void __launch_bounds__(kBlock) cg_wu_kernel(int32_t it, SellView<T> m
 }
 
 // rhs[k] = E U_old - RHS1 SrcFld + dt (Ct F + fb) at free edge e = free[k]
-// (D:1516-1523; sigma and I are zero on the supported inputs). The three
-// operators are stored as SELL-64 over the free rows (in the CG's row
-// order); SrcFld is U on Dirichlet edges at the new time, held in `dir_new`
-// (edge-indexed).
-template <typename T>
-__global__ void __launch_bounds__(kBlock) gf_rhs_kernel(SellView<T> e, SellView<T> r1, SellView<int8_t> ct,
-                                                        const int32_t* __restrict__ free_edge, const T* __restrict__ U,
-                                                        const T* __restrict__ dir_new, const T* __restrict__ Fv,
-                                                        const T* __restrict__ fb, T* __restrict__ rhs, double dt) {
+// (D:1516-1523; sigma and I are zero on the supported inputs), fused with
+// the start of the CG solve (cg_init_kernel): r = rhs - A x, partial r.r,
+// and with jacobi u = pre r and partial r.u. E, RHSmat1, Ct and A are
+// stored as SELL-64 over the free rows in the CG's row order; SrcFld is U
+// on Dirichlet edges at the new time, held in `dir_new` (edge-indexed).
+template <typename T, int NB>
+__global__ void __launch_bounds__(kBlock) gf_rhs_init_kernel(
+    SellView<T> e, SellView<T> r1, SellView<int8_t> ct, SellView<T> a, const int32_t* __restrict__ free_edge,
+    const T* __restrict__ U, const T* __restrict__ dir_new, const T* __restrict__ Fv, const T* __restrict__ fb,
+    T* __restrict__ rhs, double dt, const T* __restrict__ pre, const T* __restrict__ x, T* __restrict__ r,
+    int r_stride, T* __restrict__ u, double* __restrict__ part_rr, double* __restrict__ part_gam, CgState* st,
+    int jacobi) {
+    __shared__ double red[kRed2];
     const int lane = lane_id();
+    double rr = 0, ru = 0;
     TF_FOR_SLICES(e) {
         const int s = sr_.s;
         const double eu = sell_dot<T, 8>(e, s, lane, [&](int32_t j) { return U[j]; });
         const bool has_r = r1.sptr[s + 1] > r1.sptr[s];
         const double rs = has_r ? sell_dot<T, 8>(r1, s, lane, [&](int32_t j) { return dir_new[j]; }) : 0.0;
         const double cf = sell_dot<T, 8>(ct, s, lane, [&](int32_t j) { return Fv[j]; });
+        const double ax = sell_dot<T, NB>(a, s, lane, [&](int32_t j) { return x[j]; });
         const int32_t k = s * kSlice + lane;
-        if (k < e.nrows) rhs[k] = (T)((eu - rs) + dt * cf + dt * (double)fb[free_edge[k]]);
+        if (k < e.nrows) {
+            const T bk = (T)((eu - rs) + dt * cf + dt * (double)fb[free_edge[k]]);
+            rhs[k] = bk;
+            const T rk = (T)((double)bk - ax);
+            r[(int64_t)k * r_stride] = rk;
+            rr += (double)rk * (double)rk;
+            if (jacobi) {
+                const T uk = (T)((double)pre[k] * (double)rk);
+                u[k] = uk;
+                ru += (double)rk * (double)uk;
+            }
+        }
+    }
+    block_sum2_all(rr, ru, red);
+    if (threadIdx.x == 0) {
+        part_rr[blockIdx.x] = rr;
+        if (jacobi) part_gam[blockIdx.x] = ru;
+        if (blockIdx.x == 0) { st->done = 0; st->iter = 0; }
     }
 }
 

@@ -921,7 +921,8 @@ void run_dga(tf_ctx* c, int64_t first, int64_t n, tf_stats* st) {
 // block): [delta: kCgPartA][rr: kCgPartB][gamma ping, pong: 2 kCgPartB]
 // [gamma of cg_pg: kCgPartC][rr of cg_pg: kCgPartC].
 template <class T>
-void cg_solve(tf_ctx* c, const CgState& init, bool learned, const std::vector<std::pair<int, int>>* snaps) {
+void cg_solve(tf_ctx* c, const CgState& init, bool learned, const std::vector<std::pair<int, int>>* snaps,
+              bool with_rhs) {
     const int32_t nf = c->n_free;
     int& guess = learned ? c->cg_guess_pc : c->cg_guess;
     const int nb_w = tuned_blocks(c, learned ? TK_WU : TK_W, c->A, kCgPartA);
@@ -952,9 +953,20 @@ void cg_solve(tf_ctx* c, const CgState& init, bool learned, const std::vector<st
         if (c->profiling) tags.push_back({(int64_t)c->recs.size() - 1, it, first});
     };
     TF_HIP(hipMemcpyAsync(dst, &init, sizeof(CgState), hipMemcpyHostToDevice, c->stream));
-    ev_begin(c, "cg_init_kernel");
-    TF_LAUNCH_NB(8, cg_init_kernel, T, nb_b, c->stream, A, c->pre.as<T>(), c->rhs.as<T>(), x,
-                 learned ? &rs[0]->r : rj, learned ? 2 : 1, u, part_rr, part_gj[0], dst, learned ? 0 : 1);
+    T* r0 = learned ? &rs[0]->r : rj;
+    const int r0_stride = learned ? 2 : 1;
+    if (with_rhs) {
+        // the step's right-hand side and r = b - A x in one pass
+        ev_begin(c, "gf_rhs_init_kernel");
+        TF_LAUNCH_NB(8, gf_rhs_init_kernel, T, nb_b, c->stream, c->Ef.vals<T>(), c->R1f.vals<T>(), c->Ctf.signs(), A,
+                     c->free_edge.as<int32_t>(), c->U.as<T>(), c->dir_new.as<T>(), c->Fv.as<T>(), c->fb.as<T>(),
+                     c->rhs.as<T>(), c->dt, c->pre.as<T>(), x, r0, r0_stride, u, part_rr, part_gj[0], dst,
+                     learned ? 0 : 1);
+    } else {
+        ev_begin(c, "cg_init_kernel");
+        TF_LAUNCH_NB(8, cg_init_kernel, T, nb_b, c->stream, A, c->pre.as<T>(), c->rhs.as<T>(), x, r0, r0_stride, u,
+                     part_rr, part_gj[0], dst, learned ? 0 : 1);
+    }
     ev_end(c);
     capture(0);
     int it = 0;
@@ -1297,7 +1309,7 @@ void gfdtd_solve(tf_ctx* c, const CgState& init, bool want_learned, tf_stats* st
     LearnedPc& L = c->lp;
     const int32_t nf = c->n_free;
     if (!want_learned || L.state == LearnedPc::kDisabled) {
-        cg_solve<T>(c, init, false, nullptr);
+        cg_solve<T>(c, init, false, nullptr, true);
         return;
     }
     if (L.state == LearnedPc::kCollecting) {
@@ -1309,7 +1321,7 @@ void gfdtd_solve(tf_ctx* c, const CgState& init, bool want_learned, tf_stats* st
                 if (L.slot[sl] == 0) snaps.push_back({kCaptureIters[k++], sl});
             if (k < 4) snaps.clear();
         }
-        cg_solve<T>(c, init, false, snaps.empty() ? nullptr : &snaps);
+        cg_solve<T>(c, init, false, snaps.empty() ? nullptr : &snaps, true);
         const int iters = c->h_state->iter;
         L.jac_iters.push_back(iters);
         if (std::isfinite(c->h_state->relres) && c->h_state->relres <= init.tol)
@@ -1323,7 +1335,7 @@ void gfdtd_solve(tf_ctx* c, const CgState& init, bool want_learned, tf_stats* st
         return;
     }
     // active
-    cg_solve<T>(c, init, true, nullptr);
+    cg_solve<T>(c, init, true, nullptr, true);
     const bool ok = std::isfinite(c->h_state->relres) && c->h_state->relres <= init.tol;
     L.solves++;
     L.iters += c->h_state->iter;
@@ -1333,7 +1345,7 @@ void gfdtd_solve(tf_ctx* c, const CgState& init, bool want_learned, tf_stats* st
         st->cg_iters_total += c->h_state->iter;
         gather_free_kernel<T><<<blocks_for(nf), kBlock, 0, c->stream>>>(nf, c->free_edge.as<int32_t>(),
                                                                         c->U.as<T>(), c->x.as<T>());
-        cg_solve<T>(c, init, false, nullptr);
+        cg_solve<T>(c, init, false, nullptr, false);
         if (++L.fallbacks > 3) L.state = LearnedPc::kDisabled;
     }
     // plain CG judges the gain: after 8 learned solves, no fewer iterations
@@ -1383,11 +1395,6 @@ void run_gfdtd(tf_ctx* c, int64_t first, int64_t n, const tf_cg_opts* cg, tf_sta
                 c->fb.as<T>(), c->dir_new.as<T>(), c->U.as<T>(), c->poy.as<double>() + k * inj_blocks);
             ev_end(c);
         }
-        ev_begin(c, "gf_rhs_kernel");
-        gf_rhs_kernel<T><<<sell_blocks(c->Ef, kSpmvCap), kBlock, 0, c->stream>>>(
-            c->Ef.vals<T>(), c->R1f.vals<T>(), c->Ctf.signs(), c->free_edge.as<int32_t>(), c->U.as<T>(),
-            c->dir_new.as<T>(), c->Fv.as<T>(), c->fb.as<T>(), c->rhs.as<T>(), c->dt);
-        ev_end(c);
         gfdtd_solve<T>(c, init, learned, st);
         st->cg_iters_total += c->h_state->iter;
         st->cg_iters_max = std::max(st->cg_iters_max, c->h_state->iter);
