@@ -73,6 +73,23 @@ def algorithmic_bytes(info, kernel, vb):
     raise KeyError(kernel)
 
 
+def measured_traffic(kernel, vb, nbytes):
+    """HBM bytes per dispatch of `kernel` from the committed rocprofv3 PMC
+    summary (profiles/traffic.json: FETCH_SIZE and WRITE_SIZE scaled by a
+    copy of known size), or None when no measurement exists for the same
+    kernel, value type and storage (same algorithmic bytes)."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
+            table = json.load(f)["kernels"]
+    except (OSError, ValueError, KeyError):
+        return None
+    t = "double" if vb == 8 else "float"
+    for e in table:
+        if e["kernel"] == kernel and e["type"] == t and e["algorithmic_bytes"] == int(nbytes):
+            return e["traffic_bytes"]
+    return None
+
+
 def roofline_of(info, kernels, names, vb):
     """Roofline entries of the listed CG kernels measured in a run; the
     first is the one with the largest total time."""
@@ -84,8 +101,8 @@ def roofline_of(info, kernels, names, vb):
         nb = algorithmic_bytes(info, k, vb)
         ach = nb / (t["ms_avg"] * 1e-3) / 1e9
         out.append(dict(bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                        frac=round(ach / HBM_PEAK_GBS, 4), traffic=None, kernel=k, bytes_per_launch=int(nb),
-                        ms_per_launch=t["ms_avg"], launches=t["launches"]))
+                        frac=round(ach / HBM_PEAK_GBS, 4), traffic=measured_traffic(k, vb, nb), kernel=k,
+                        bytes_per_launch=int(nb), ms_per_launch=t["ms_avg"], launches=t["launches"]))
     out.sort(key=lambda r: -r["ms_per_launch"] * r["launches"])
     return out
 
@@ -209,8 +226,8 @@ def bench_fit(args):
                 nb = fit_bytes(b, kname, 8)
                 ach = nb / (k["ms_avg"] * 1e-3) / 1e9
                 roof[kname] = dict(bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                                   frac=round(ach / HBM_PEAK_GBS, 4), traffic=None, bytes_per_launch=int(nb),
-                                   ms_per_launch=k["ms_avg"])
+                                   frac=round(ach / HBM_PEAK_GBS, 4), traffic=measured_traffic(kname, 8, nb),
+                                   kernel=kname, bytes_per_launch=int(nb), ms_per_launch=k["ms_avg"])
         cpu = None
         if not args.no_cpu:
             sys.path.insert(0, os.path.join(ROOT, "oracle"))
