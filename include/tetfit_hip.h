@@ -139,9 +139,11 @@ tf_status tf_read_probes(tf_ctx* ctx, double* times, double* vals,
 /* Per-step Poynting scalar (the first column of poynting.dat). */
 tf_status tf_read_poynting(tf_ctx* ctx, double* vals, int64_t cap, int64_t* n);
 
-/* Discrete energy of the current state, fp64 reductions (D:3468-3470):
- * We = 1/2 U.(ep_vec U), Wh = 1/2 B.(nu_vec B); fdtd contexts (TF_ERR_STATE
- * otherwise). */
+/* Discrete energy of the current state, fp64 reductions: fdtd
+ * We = 1/2 U.(ep_vec U), Wh = 1/2 B.(nu_vec B) (D:3468-3470); dga and
+ * gfdtd We = 1/2 U.(E U), Wh = 1/2 B.(N B) with E the problem's eps mass
+ * matrix (D:2773-2775). TF_ERR_STATE for contexts without these matrices
+ * (batches). */
 tf_status tf_energy(tf_ctx* ctx, double* we, double* wh);
 
 /* Average duration of each kernel launched by the last tf_run, measured with

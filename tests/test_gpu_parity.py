@@ -155,3 +155,29 @@ def test_fit_energy_diagnostic(workdir):
         ref_we, ref_wh = 0.5 * U @ (f["ep_vec"] * U), 0.5 * B @ (f["nu_vec"] * B)
         assert ref_we > 0 and ref_wh > 0
         assert abs(we - ref_we) <= 1e-12 * ref_we and abs(wh - ref_wh) <= 1e-12 * ref_wh
+
+
+@pytest.mark.parametrize("method", ["dga", "gfdtd"])
+def test_tet_energy_diagnostic(workdir, method):
+    """tf_energy on dga / gfdtd contexts: We = 1/2 U.(E U), Wh = 1/2 B.(N B)
+    (D:2773-2775) reduced in fp64 on the device, equal to the same forms
+    computed on the host from the read-back state and the host's E and N
+    (for fp32 contexts with E and N rounded to fp32, as the device stores
+    them)."""
+    import scipy.sparse as sp
+    b = Model(configs.kuhn_cavity(str(workdir), n=4, method=method, n_steps=40, n_probes=2, block=2,
+                                  name=f"energy_{method}")).build(1)
+    d = b.dga_ops()
+    for prec in ("fp64", "fp32"):
+        print(f"case: tet energy {method} {prec}", flush=True)
+        sim = Simulation(b, prec)
+        sim.run()
+        we, wh = sim.energy()
+        U, _, B = sim.state()
+        sim.close()
+        cast = (lambda v: v) if prec == "fp64" else (lambda v: v.astype(np.float32).astype(np.float64))
+        Em = sp.csr_matrix((cast(d["E"]["val"]), d["E"]["col"], d["E"]["rowptr"]), shape=(b.n_edges, b.n_edges))
+        Nm = sp.csr_matrix((cast(d["N"]["val"]), d["N"]["col"], d["N"]["rowptr"]), shape=(b.n_faces, b.n_faces))
+        ref_we, ref_wh = 0.5 * U @ (Em @ U), 0.5 * B @ (Nm @ B)
+        assert ref_we > 0 and ref_wh > 0
+        assert abs(we - ref_we) <= 1e-12 * ref_we and abs(wh - ref_wh) <= 1e-12 * ref_wh

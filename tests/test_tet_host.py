@@ -225,3 +225,32 @@ def test_dga_reluctance_patch_test(tmp_path):
         phi = area @ B0
         expect = (B0 @ B0) * V / mu0
         assert abs(phi @ (Nm @ phi) - expect) <= 1e-13 * expect
+
+
+def test_dga_eps_mass_patch_test(kuhn):
+    """E of the dga energy diagnostic (the node-star eps blocks summed,
+    D:9435-9445) is exact for uniform fields: the corner vectors
+    f_i = 2 a_i / (3 V) (a_i the signed face area vectors, D:8926-9078) are
+    twice the dual basis of the corner's edges, so with U_e = E0.(p_b - p_a)
+    they give 2 E0 in every corner and U^T E U = 4 sum_t eps_t |E0|^2 |V_t|,
+    the same factor as the oriented gfdtd corner basis. E is symmetric and
+    positive definite; a wrong sign, area vector or corner weight breaks the
+    identity."""
+    import scipy.sparse as sp
+    b = kuhn["dga"]
+    t = b.tet_tables()
+    Em = b.dga_ops()["E"]
+    M = sp.csr_matrix((Em["val"], Em["col"], Em["rowptr"]), shape=(b.n_edges, b.n_edges))
+    assert abs(M - M.T).max() <= 1e-14 * abs(M).max()
+    eps = kuhn["dga_eps"]
+    epst = EPS0 * np.array([eps[m] for m in t["vol_material"]])
+    pts, en, vols = t["pts"], t["edge_nodes"], t["vols"]
+    V = np.abs(np.einsum("ij,ij->i", pts[vols[:, 1]] - pts[vols[:, 0]],
+                         np.cross(pts[vols[:, 2]] - pts[vols[:, 0]], pts[vols[:, 3]] - pts[vols[:, 0]]))) / 6
+    rng = np.random.default_rng(7)
+    for _ in range(3):
+        E0 = rng.standard_normal(3)
+        U = (pts[en[:, 1]] - pts[en[:, 0]]) @ E0
+        expect = 4 * (E0 @ E0) * np.sum(epst * V)
+        assert abs(U @ (M @ U) - expect) <= 1e-12 * expect
+    assert np.linalg.eigvalsh(M.toarray()).min() > 0

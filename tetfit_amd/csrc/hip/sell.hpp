@@ -160,6 +160,25 @@ __global__ void __launch_bounds__(kBlock) sell_n_kernel(SellView<T> m, const T* 
     }
 }
 
+// part[block] = sum over the block's rows r of x[r] (M x)[r], accumulated
+// in fp64: twice the discrete energy x.(M x) / 2 of a symmetric mass matrix
+// (We = 1/2 U.E U, Wh = 1/2 B.N B, D:2773-2775).
+template <typename T, int NB>
+__global__ void __launch_bounds__(kBlock) quad_form_kernel(SellView<T> m, const T* __restrict__ x,
+                                                           double* __restrict__ part) {
+    __shared__ double red[kBlock / kWave];
+    const int lane = lane_id();
+    double acc = 0;
+    TF_FOR_SLICES(m) {
+        const int s = sr_.s;
+        const double mx = sell_dot<double, NB>(m, s, lane, [&](int32_t j) { return x[j]; });
+        const int32_t r = s * kSlice + lane;
+        if (r < m.nrows) acc += (double)x[r] * mx;
+    }
+    acc = block_sum(acc, red);
+    if (threadIdx.x == 0) part[blockIdx.x] = acc;
+}
+
 // ---- CG (ConjugateGradientSolver.hpp:52-103) ------------------------------
 // Preconditioned CG in the single-reduction form of Chronopoulos and Gear
 // (1989): with u = M r, w = A u, gamma = r.u, delta = w.u,

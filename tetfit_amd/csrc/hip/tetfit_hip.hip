@@ -162,6 +162,7 @@ struct tf_ctx {
     // DGA / gfdtd
     Csr32 Ct;
     Sell sCt, H, N, A, Ef, R1f, Ctf;
+    Sell Efull;  // eps mass E of the energy diagnostic (all edges), tet methods
     bool c_fixed_signs = false;
     DevBuf c_col, c_sg, h_edge, curl_f, curl_u;
     DevBuf free_edge, dir_edge, dir_new, fb, pre, x, r, z, p0, p1, q, rhs, cg_part, cg_state;
@@ -248,7 +249,7 @@ size_t tf_ctx::device_bytes() const {
     for (auto b : all) s += b->bytes;
     const Csr32* cs[] = {&Ct};
     for (auto c : cs) s += c->ptr.bytes + c->col.bytes + c->val.bytes + c->sg.bytes;
-    const Sell* ss[] = {&sCt, &H, &N, &A, &Ef, &R1f, &Ctf};
+    const Sell* ss[] = {&sCt, &H, &N, &A, &Ef, &R1f, &Ctf, &Efull};
     for (auto c : ss) s += c->sptr.bytes + c->col.bytes + c->val.bytes + c->sg.bytes + c->col16.bytes + c->cbase.bytes;
     for (auto b : lp.bufs()) s += b->bytes;
     return s;
@@ -544,6 +545,7 @@ void create_tet(tf_ctx* c, const tf_problem* p) {
         TF_HIP(hipMemcpy(c->c_sg.p, d.C.sgn, d.C.nnz, hipMemcpyHostToDevice));
     }
     upload_sell<T>(c->N, d.N, true);
+    if (d.E.nrows == E && d.E.nnz > 0) upload_sell<T>(c->Efull, d.E, true);
     c->h_fixed_value.assign(d.fixed_value, d.fixed_value + E);
     zero_field<T>(c->curl_u, F, c->stream);
     zero_field<T>(c->curl_f, E, c->stream);
@@ -1951,8 +1953,8 @@ tf_status tf_read_poynting(tf_ctx* c, double* vals, int64_t cap, int64_t* n) {
 
 tf_status tf_energy(tf_ctx* c, double* we, double* wh) {
     if (!c) { g_err = "null argument"; return TF_ERR_ARG; }
-    if (c->method != TF_METHOD_FDTD || !c->fit_ep.p) {
-        g_err = "the device energy diagnostic covers fdtd contexts (dga/gfdtd: from tf_read_state on the host)";
+    if (c->method == TF_METHOD_FDTD ? !c->fit_ep.p : c->Efull.nrows != c->E) {
+        g_err = "no energy matrices in this context (batches of dga problems carry none)";
         return TF_ERR_STATE;
     }
     return guarded([&] {
@@ -1960,7 +1962,19 @@ tf_status tf_energy(tf_ctx* c, double* we, double* wh) {
         DevBuf part;
         part.alloc(2 * nb * sizeof(double));
         double* pp = part.as<double>();
-        if (c->prec == TF_PREC_FP64) {
+        int ge = nb, gn = nb;  // partials written by each reduction
+        if (c->method != TF_METHOD_FDTD) {
+            // We = 1/2 U.E U, Wh = 1/2 B.N B (D:2773-2775)
+            ge = sell_blocks(c->Efull, nb);
+            gn = sell_blocks(c->N, nb);
+            if (c->prec == TF_PREC_FP64) {
+                quad_form_kernel<double, 8><<<ge, kBlock, 0, c->stream>>>(c->Efull.vals<double>(), c->U.as<double>(), pp);
+                quad_form_kernel<double, 8><<<gn, kBlock, 0, c->stream>>>(c->N.vals<double>(), c->B.as<double>(), pp + nb);
+            } else {
+                quad_form_kernel<float, 8><<<ge, kBlock, 0, c->stream>>>(c->Efull.vals<float>(), c->U.as<float>(), pp);
+                quad_form_kernel<float, 8><<<gn, kBlock, 0, c->stream>>>(c->N.vals<float>(), c->B.as<float>(), pp + nb);
+            }
+        } else if (c->prec == TF_PREC_FP64) {
             weighted_sq_kernel<double><<<nb, kBlock, 0, c->stream>>>(c->E, c->U.as<double>(), c->fit_ep.as<double>(), pp);
             weighted_sq_kernel<double><<<nb, kBlock, 0, c->stream>>>(c->F, c->B.as<double>(), c->fit_nu.as<double>(),
                                                                      pp + nb);
@@ -1973,7 +1987,8 @@ tf_status tf_energy(tf_ctx* c, double* we, double* wh) {
         TF_HIP(hipMemcpyAsync(h.data(), pp, h.size() * 8, hipMemcpyDeviceToHost, c->stream));
         TF_HIP(hipStreamSynchronize(c->stream));
         double a = 0, b = 0;
-        for (int k = 0; k < nb; ++k) { a += h[k]; b += h[nb + k]; }
+        for (int k = 0; k < ge; ++k) a += h[k];
+        for (int k = 0; k < gn; ++k) b += h[nb + k];
         if (we) *we = 0.5 * a;
         if (wh) *wh = 0.5 * b;
     });

@@ -615,6 +615,22 @@ void build_tet(const Model& m, const SimulationDef& sim, Build& b) {
             }
             H.val.assign(H.col.size(), 0.0);
         }
+        // E: the node-star eps mass blocks summed without the class-4
+        // identity rows (E_fracs assembled into E_trip, D:9435-9445), the
+        // matrix of the energy diagnostic We = 1/2 U.E U (D:2773)
+        {
+            std::vector<int32_t> er, ec;
+            er.reserve((size_t)T * 36);
+            ec.reserve((size_t)T * 36);
+            for (uint32_t t = 0; t < T; ++t)
+                for (int k = 0; k < 4; ++k)
+                    for (int i = 0; i < 3; ++i)
+                        for (int j = 0; j < 3; ++j) {
+                            er.push_back(tet_edges[(size_t)t * 6 + kCornerEdges[k][i]]);
+                            ec.push_back(tet_edges[(size_t)t * 6 + kCornerEdges[k][j]]);
+                        }
+            b.E = pattern_from_pairs((int32_t)E, (int32_t)E, er, ec);
+        }
         std::vector<double> LE, LH;
         Vec3 fa[4];
         for (uint32_t nid = 0; nid < np; ++nid) {
@@ -647,6 +663,9 @@ void build_tet(const Model& m, const SimulationDef& sim, Build& b) {
                     a = a + v;
                     LE[(size_t)le[pq[1]] * deg + le[pq[0]]] = a;
                 }
+                for (int i = 0; i < 3; ++i)
+                    for (int j = 0; j < 3; ++j)
+                        csr_at(b.E, (int32_t)gi[le[i]], (int32_t)gi[le[j]]) += dot(f[i], eps * f[j]) * vol / 4;
             }
             // class-4 rows/cols -> identity, invert, zero them again (D:9270-9333)
             for (uint32_t j = 0; j < deg; ++j)

@@ -101,7 +101,8 @@ class Simulation:
         return times, vals[:k * self.n_probes * 6].reshape(k, self.n_probes, 6)
 
     def energy(self):
-        """(We, Wh) of the current state, fp64 reductions on the device (fdtd)."""
+        """(We, Wh) of the current state, fp64 reductions on the device
+        (fdtd: ep_vec / nu_vec weights; dga, gfdtd: U.E U / 2, B.N B / 2)."""
         we, wh = C.c_double(), C.c_double()
         _check(hip_lib().tf_energy(self._h, C.byref(we), C.byref(wh)))
         return we.value, wh.value
