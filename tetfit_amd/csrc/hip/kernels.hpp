@@ -26,6 +26,32 @@ __device__ __forceinline__ double source_value(int tb, int te, const int32_t* __
     return v;
 }
 
+// Device state of one CG solve (kernels in sell.hpp). Each kernel writes one
+// fp64 partial per block; the next kernel sums them (every block, in index
+// order, so all blocks take the same decision) - no cross-block fences.
+struct CgState {
+    double rr0;        // ||r0||^2 of this solve
+    double gam[2];     // r.u of pass it, slot it&1
+    double alpha[2];   // step length of pass it, slot it&1
+    double beta[2];    // beta of pass it, slot it&1 (learned passes)
+    double relres;     // ||r||/||r0|| at the last stop test
+    double tol;
+    int32_t maxiter;
+    int32_t iter;      // iterations done when the solve stopped
+    int32_t done;
+    int32_t need_conv;  // the solve must meet tol (learned solves; Jacobi ones may stop at maxiter)
+};
+
+// Guard of the kernels a gfdtd step enqueues ahead of the host's check of
+// the previous CG solve (gf_scatter of that step, the next step's
+// dga_curl, sell_n, probe_kernel, gf_inject and gf_rhs_init): they run only
+// if that solve has stopped, and, for a solve that must meet the tolerance,
+// met it. Otherwise they return at once and the host enqueues them again
+// once the solve is complete. A null guard always passes.
+__device__ __forceinline__ bool cg_guard_ok(const CgState* g) {
+    return !g || (g->done && (!g->need_conv || g->relres <= g->tol));
+}
+
 // ------------------------------------------------------------------ FIT
 //
 // E phase (D:1850-1895). Main blocks cover every edge in index order; edges
@@ -119,7 +145,8 @@ struct ProbeDev {
 
 template <typename T>
 __global__ void probe_kernel(int32_t n_probes, const ProbeDev* __restrict__ pr, const T* __restrict__ U,
-                             const T* __restrict__ B, double* __restrict__ out) {
+                             const T* __restrict__ B, double* __restrict__ out, const CgState* guard) {
+    if (!cg_guard_ok(guard)) return;
     const int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n_probes * 6) return;
     const int p = t / 6, c = t % 6;
@@ -183,7 +210,8 @@ template <typename T>
 __global__ void __launch_bounds__(kBlock) dga_curl_kernel(int32_t n_faces, const int32_t* __restrict__ c_col,
                                                           const int8_t* __restrict__ c_sg, const T* __restrict__ U,
                                                           T* __restrict__ curl_u, T* __restrict__ B, double dt,
-                                                          const double* __restrict__ face_dt) {
+                                                          const double* __restrict__ face_dt, const CgState* guard) {
+    if (!cg_guard_ok(guard)) return;
     const int32_t f = xcd_block() * kBlock + threadIdx.x;
     if (f >= n_faces) return;
     const int32_t* c = c_col + 3 * (int64_t)f;
@@ -200,22 +228,6 @@ __global__ void __launch_bounds__(kBlock) dga_curl_kernel(int32_t n_faces, const
 }
 
 // ------------------------------------------------------------------ gfdtd
-// Device state of one CG solve (kernels in sell.hpp). Each kernel writes one
-// fp64 partial per block; the next kernel sums them (every block, in index
-// order, so all blocks take the same decision) - no cross-block fences.
-struct CgState {
-    double rr0;        // ||r0||^2 of this solve
-    double gam[2];     // r.u of pass it, slot it&1
-    double alpha[2];   // step length of pass it, slot it&1
-    double beta[2];    // beta of pass it, slot it&1 (learned passes)
-    double relres;     // ||r||/||r0|| at the last stop test
-    double tol;
-    int32_t maxiter;
-    int32_t iter;      // iterations done when the solve stopped
-    int32_t done;
-    int32_t pad;
-};
-
 __device__ __forceinline__ double sum_partials(const double* __restrict__ part, int n, double* red) {
     double v = 0;
     for (int i = threadIdx.x; i < n; i += blockDim.x) v += part[i];
@@ -232,7 +244,8 @@ __device__ __forceinline__ double sum_partials(const double* __restrict__ part, 
 template <typename T>
 __global__ void gf_scatter_kernel(int32_t n_free, const int32_t* __restrict__ free_edge, const T* __restrict__ x,
                                   T* __restrict__ U, int32_t n_dir, const int32_t* __restrict__ dir_edge,
-                                  const T* __restrict__ dir_new) {
+                                  const T* __restrict__ dir_new, const CgState* guard) {
+    if (!cg_guard_ok(guard)) return;
     const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n_free) U[free_edge[i]] = x[i];
     else if (i - n_free < n_dir) {
@@ -249,7 +262,9 @@ __global__ void __launch_bounds__(kBlock) gf_inject_kernel(
     int32_t n_inj, const int32_t* __restrict__ inj_edge, const int8_t* __restrict__ inj_kind,
     const int32_t* __restrict__ inj_tb, const int32_t* __restrict__ inj_te, const int32_t* __restrict__ term_src,
     const double* __restrict__ term_w, const double* __restrict__ carr_step, const double* __restrict__ carr_prev,
-    T* __restrict__ fb, T* __restrict__ dir_new, const T* __restrict__ U, double* __restrict__ poy_part) {
+    T* __restrict__ fb, T* __restrict__ dir_new, const T* __restrict__ U, double* __restrict__ poy_part,
+    const CgState* guard) {
+    if (!cg_guard_ok(guard)) return;
     __shared__ double red[kBlock / kWave];
     const int32_t i = blockIdx.x * kBlock + threadIdx.x;
     double poy = 0;

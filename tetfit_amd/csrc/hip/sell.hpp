@@ -147,10 +147,12 @@ __global__ void __launch_bounds__(kBlock) sell_h_kernel(SellView<T> m, const int
 }
 
 // mode 0: F -= dt * N x (dga, D:917); mode 1: F = N x (gfdtd, D:1468);
-// row_dt as in sell_h_kernel.
+// row_dt as in sell_h_kernel; guard as in cg_guard_ok.
 template <typename T, int NB>
 __global__ void __launch_bounds__(kBlock) sell_n_kernel(SellView<T> m, const T* __restrict__ x, T* __restrict__ Fv,
-                                                        double dt, int mode, const double* __restrict__ row_dt) {
+                                                        double dt, int mode, const double* __restrict__ row_dt,
+                                                        const CgState* guard) {
+    if (!cg_guard_ok(guard)) return;
     const int lane = lane_id();
     TF_FOR_SLICES(m) {
         const int s = sr_.s;
@@ -519,7 +521,7 @@ __global__ void __launch_bounds__(kBlock) cg_wu_kernel(int32_t it, SellView<T> m
 // rhs[k] = E U_old - RHS1 SrcFld + dt (Ct F + fb) at free edge e = free[k]
 // (D:1516-1523; sigma and I are zero on the supported inputs), fused with
 // the start of the CG solve (cg_init_kernel): r = rhs - A x, partial r.r,
-// and with jacobi u = pre r and partial r.u. E, RHSmat1, Ct and A are
+// and with jacobi u = pre r and partial r.u. guard as in cg_guard_ok. E, RHSmat1, Ct and A are
 // stored as SELL-64 over the free rows in the CG's row order; SrcFld is U
 // on Dirichlet edges at the new time, held in `dir_new` (edge-indexed).
 template <typename T, int NB>
@@ -528,7 +530,8 @@ __global__ void __launch_bounds__(kBlock) gf_rhs_init_kernel(
     const T* __restrict__ U, const T* __restrict__ dir_new, const T* __restrict__ Fv, const T* __restrict__ fb,
     T* __restrict__ rhs, double dt, const T* __restrict__ pre, const T* __restrict__ x, T* __restrict__ r,
     int r_stride, T* __restrict__ u, double* __restrict__ part_rr, double* __restrict__ part_gam, CgState* st,
-    int jacobi) {
+    int jacobi, const CgState* guard) {
+    if (!cg_guard_ok(guard)) return;
     __shared__ double red[kRed2];
     const int lane = lane_id();
     double rr = 0, ru = 0;

@@ -173,7 +173,9 @@ struct tf_ctx {
     std::vector<double> h_fixed_value;
     DevBuf inj_edge, inj_kind, inj_tb, inj_te;
     int32_t n_inj = 0;
-    CgState* h_state = nullptr;  // pinned
+    CgState* h_state = nullptr;  // pinned, two slots (as cg_state)
+    CgState init_src[2] = {};     // host sources of the initial-state copies, per slot
+    hipEvent_t solve_ev = nullptr;  // recorded after a solve's first batch of passes
     int cg_guess = 30;           // iterations of the previous solve
     int cg_guess_pc = 15;        // same, for solves with the learned preconditioner
     // host copy of A (free rows, CSR) and its SELL slice pointers, kept for
@@ -234,6 +236,7 @@ struct tf_ctx {
 
 tf_ctx::~tf_ctx() {
     if (stream) (void)hipStreamSynchronize(stream);
+    if (solve_ev) (void)hipEventDestroy(solve_ev);
     for (auto e : ev_pool) (void)hipEventDestroy(e);
     if (h_state) (void)hipHostFree(h_state);
     if (stream) (void)hipStreamDestroy(stream);
@@ -615,9 +618,13 @@ void create_tet(tf_ctx* c, const tf_problem* p) {
         for (DevBuf* b : {&c->x, &c->r, &c->z, &c->p0, &c->p1, &c->q, &c->rhs}) zero_field<T>(*b, n, c->stream);
         for (DevBuf* b : {&c->rs0, &c->rs1}) zero_field<RS<T>>(*b, n, c->stream);
         c->cg_part.alloc(sizeof(double) * (kCgPartA + 3 * kCgPartB + 2 * kCgPartC));
-        c->cg_state.alloc(sizeof(CgState));
-        TF_HIP(hipMemsetAsync(c->cg_state.p, 0, sizeof(CgState), c->stream));
-        TF_HIP(hipHostMalloc((void**)&c->h_state, sizeof(CgState), hipHostMallocDefault));
+        // two CG states (device and pinned host copies): the solve of step k
+        // uses slot k & 1, so a step's kernels can run ahead of the host's
+        // check of the previous solve (run_gfdtd)
+        c->cg_state.alloc(2 * sizeof(CgState));
+        TF_HIP(hipMemsetAsync(c->cg_state.p, 0, 2 * sizeof(CgState), c->stream));
+        TF_HIP(hipHostMalloc((void**)&c->h_state, 2 * sizeof(CgState), hipHostMallocDefault));
+        std::memset((void*)c->h_state, 0, 2 * sizeof(CgState));
     }
     // initial U: fixed (class-4) values
     std::vector<T> u0(E, 0);
@@ -735,32 +742,40 @@ bool want_sample(tf_ctx* c, int64_t i, double t) {
     return allow_print(c, t);
 }
 
-template <class T>
-void sample(tf_ctx* c, double t) {
-    if (c->n_probes == 0) {
-        c->sample_times.push_back(t);
-        c->n_samples++;
-        return;
-    }
-    if (c->n_samples >= c->sample_cap) {
+// Host bookkeeping of one probe sample at time t: its slot in the sample
+// buffer (grown when full), or -1 without probes.
+int64_t sample_reserve(tf_ctx* c, double t) {
+    c->sample_times.push_back(t);
+    const int64_t slot = c->n_samples++;
+    if (c->n_probes == 0) return -1;
+    if (c->n_samples > c->sample_cap) {
         int64_t cap = std::max<int64_t>(1024, c->sample_cap * 2);
         DevBuf nb;
         nb.alloc(cap * c->n_probes * 6 * sizeof(double));
-        if (c->n_samples)
-            TF_HIP(hipMemcpyAsync(nb.p, c->samples.p, c->n_samples * c->n_probes * 6 * sizeof(double),
+        if (slot)
+            TF_HIP(hipMemcpyAsync(nb.p, c->samples.p, slot * c->n_probes * 6 * sizeof(double),
                                   hipMemcpyDeviceToDevice, c->stream));
         TF_HIP(hipStreamSynchronize(c->stream));
         std::swap(nb.p, c->samples.p);
         std::swap(nb.bytes, c->samples.bytes);
         c->sample_cap = cap;
     }
+    return slot;
+}
+
+template <class T>
+void sample_launch(tf_ctx* c, int64_t slot, const CgState* guard) {
+    if (slot < 0) return;
     ev_begin(c, "probe_kernel");
     probe_kernel<T><<<blocks_for(c->n_probes * 6, 64), 64, 0, c->stream>>>(
         c->n_probes, c->probes.as<ProbeDev>(), c->U.as<T>(), c->B.as<T>(),
-        c->samples.as<double>() + c->n_samples * c->n_probes * 6);
+        c->samples.as<double>() + slot * c->n_probes * 6, guard);
     ev_end(c);
-    c->sample_times.push_back(t);
-    c->n_samples++;
+}
+
+template <class T>
+void sample(tf_ctx* c, double t) {
+    sample_launch<T>(c, sample_reserve(c, t), nullptr);
 }
 
 void ensure_carr(tf_ctx* c, int64_t first, int64_t n) {
@@ -887,11 +902,12 @@ void run_dga(tf_ctx* c, int64_t first, int64_t n, tf_stats* st) {
         ev_begin(c, "dga_curl_kernel");
         dga_curl_kernel<T><<<blocks_for(c->F), kBlock, 0, c->stream>>>(c->F, c->c_col.as<int32_t>(),
                                                                         c->c_fixed_signs ? nullptr : c->c_sg.as<int8_t>(), c->U.as<T>(),
-                                                                        c->curl_u.as<T>(), c->B.as<T>(), c->dt, fdt);
+                                                                        c->curl_u.as<T>(), c->B.as<T>(), c->dt, fdt,
+                                                                        nullptr);
         ev_end(c);
         ev_begin(c, "sell_n_kernel");
         TF_LAUNCH_NB(c->tune[TK_N].unroll, sell_n_kernel, T, gb_n, c->stream, c->N.vals<T>(), c->curl_u.as<T>(),
-                     c->Fv.as<T>(), c->dt, 0, fdt);
+                     c->Fv.as<T>(), c->dt, 0, fdt, nullptr);
         ev_end(c);
         if (want_sample(c, i, t)) sample<T>(c, t);
         if (c->profiling && (k & 63) == 63) ev_flush(c);
@@ -910,114 +926,172 @@ void run_dga(tf_ctx* c, int64_t first, int64_t n, tf_stats* st) {
     st->steps_done += n;
 }
 
-// One CG solve on the device (single-reduction PCG, see sell.hpp): cg_init,
-// then passes it = 0, 1, ... of (cg_w, cg_upd) with Jacobi, or (cg_pg,
-// cg_wu) with the learned preconditioner. Launches are issued in batches
-// sized from the previous solve's count; kernels past the stop test return
-// at once. `snaps` lists (iteration k, bank slot) captures of the iterate
-// (r_k, x_k) (Jacobi solves), taken in stream order after the update that
-// produces it. The final state is read back into h_state.
+// ---- CG solves on the device (single-reduction PCG, see sell.hpp) --------
+// A solve is cg_init (or the fused gf_rhs_init of a step), then passes it =
+// 0, 1, ... of (cg_w, cg_upd) with Jacobi, or (cg_pg, cg_wu) with the
+// learned preconditioner. Passes are launched in batches sized from the
+// previous solve's count; kernels past the stop test return at once.
+// `snaps` lists (iteration k, bank slot) captures of the iterate (r_k, x_k)
+// (Jacobi solves), taken in stream order after the update that produces it.
 //
 // Vectors: u = z, w = q, p = p0, s = p1; the learned passes keep (r, s)
 // pairs in rs0 (even passes) and rs1 (odd passes). Partials (fp64, one per
 // block): [delta: kCgPartA][rr: kCgPartB][gamma ping, pong: 2 kCgPartB]
 // [gamma of cg_pg: kCgPartC][rr of cg_pg: kCgPartC].
+struct CgSolve {
+    CgState init{};
+    bool learned = false;
+    const std::vector<std::pair<int, int>>* snaps = nullptr;
+    int slot = 0;  // CG state slot (device cg_state[slot], pinned h_state[slot])
+    int it = 0;    // next pass
+    std::vector<std::array<int64_t, 3>> tags;  // profiled launches: (record, pass, first kernel of a learned pass)
+};
+
 template <class T>
-void cg_solve(tf_ctx* c, const CgState& init, bool learned, const std::vector<std::pair<int, int>>* snaps,
-              bool with_rhs) {
-    const int32_t nf = c->n_free;
-    int& guess = learned ? c->cg_guess_pc : c->cg_guess;
-    const int nb_w = tuned_blocks(c, learned ? TK_WU : TK_W, c->A, kCgPartA);
-    const int nb_b = std::max(1, std::min(blocks_for(nf), kCgPartB));
-    const int nb_g = tuned_blocks(c, TK_G, c->A, kCgPartC);
-    double* part_del = c->cg_part.as<double>();
-    double* part_rr = part_del + kCgPartA;
-    double* part_gj[2] = {part_rr + kCgPartB, part_rr + 2 * kCgPartB};
-    double* part_gl = part_rr + 3 * kCgPartB;
-    double* part_rl = part_gl + kCgPartC;
-    CgState* dst = c->cg_state.as<CgState>();
-    T *x = c->x.as<T>(), *u = c->z.as<T>(), *w = c->q.as<T>(), *p = c->p0.as<T>(), *sv = c->p1.as<T>();
-    T* rj = c->r.as<T>();
-    RS<T>* rs[2] = {c->rs0.as<RS<T>>(), c->rs1.as<RS<T>>()};
-    const SellView<T> A = c->A.vals<T>();
-    const SellView<uint16_t> G = c->A.view<uint16_t>(c->lp.gval);
-    auto capture = [&](int k) {
-        if (!snaps) return;
-        for (auto& sn : *snaps)
-            if (sn.first == k)
-                bank_capture_kernel<T><<<blocks_for(nf), kBlock, 0, c->stream>>>(nf, sn.second, rj, x,
-                                                                                   c->lp.Rb.as<float>(),
-                                                                                   c->lp.Cb.as<float>());
-    };
-    // profiled launches of this solve: (record, pass, 1 for the learned pass's first kernel)
-    std::vector<std::array<int64_t, 3>> tags;
-    auto tag = [&](int it, int first) {
-        if (c->profiling) tags.push_back({(int64_t)c->recs.size() - 1, it, first});
-    };
-    TF_HIP(hipMemcpyAsync(dst, &init, sizeof(CgState), hipMemcpyHostToDevice, c->stream));
-    T* r0 = learned ? &rs[0]->r : rj;
-    const int r0_stride = learned ? 2 : 1;
+struct CgView {
+    int32_t nf;
+    int nb_w, nb_b, nb_g;
+    double *part_del, *part_rr, *part_gj[2], *part_gl, *part_rl;
+    T *x, *u, *w, *p, *sv, *rj;
+    RS<T>* rs[2];
+    SellView<T> A;
+    SellView<uint16_t> G;
+    CgView(tf_ctx* c, bool learned) {
+        nf = c->n_free;
+        nb_w = tuned_blocks(c, learned ? TK_WU : TK_W, c->A, kCgPartA);
+        nb_b = std::max(1, std::min(blocks_for(nf), kCgPartB));
+        nb_g = tuned_blocks(c, TK_G, c->A, kCgPartC);
+        part_del = c->cg_part.as<double>();
+        part_rr = part_del + kCgPartA;
+        part_gj[0] = part_rr + kCgPartB;
+        part_gj[1] = part_rr + 2 * kCgPartB;
+        part_gl = part_rr + 3 * kCgPartB;
+        part_rl = part_gl + kCgPartC;
+        x = c->x.as<T>();
+        u = c->z.as<T>();
+        w = c->q.as<T>();
+        p = c->p0.as<T>();
+        sv = c->p1.as<T>();
+        rj = c->r.as<T>();
+        rs[0] = c->rs0.as<RS<T>>();
+        rs[1] = c->rs1.as<RS<T>>();
+        A = c->A.vals<T>();
+        G = c->A.view<uint16_t>(c->lp.gval);
+    }
+};
+
+// Copies the solve's initial state to its slot and launches the start of the
+// solve: with_rhs, the step's right-hand side and r = b - A x in one pass
+// (gf_rhs_init_kernel, guarded); otherwise r = b - A x on the current rhs.
+template <class T>
+void cg_start(tf_ctx* c, CgSolve& cs, bool with_rhs, const CgState* guard) {
+    const CgView<T> v(c, cs.learned);
+    CgState* dst = c->cg_state.as<CgState>() + cs.slot;
+    cs.init.need_conv = cs.learned ? 1 : 0;
+    // the copy's source outlives the call: slot k & 1 is written again only
+    // for the solve after next, after the host has waited on this one
+    c->init_src[cs.slot] = cs.init;
+    TF_HIP(hipMemcpyAsync(dst, &c->init_src[cs.slot], sizeof(CgState), hipMemcpyHostToDevice, c->stream));
+    T* r0 = cs.learned ? &v.rs[0]->r : v.rj;
+    const int r0_stride = cs.learned ? 2 : 1;
     if (with_rhs) {
-        // the step's right-hand side and r = b - A x in one pass
         ev_begin(c, "gf_rhs_init_kernel");
-        TF_LAUNCH_NB(8, gf_rhs_init_kernel, T, nb_b, c->stream, c->Ef.vals<T>(), c->R1f.vals<T>(), c->Ctf.signs(), A,
-                     c->free_edge.as<int32_t>(), c->U.as<T>(), c->dir_new.as<T>(), c->Fv.as<T>(), c->fb.as<T>(),
-                     c->rhs.as<T>(), c->dt, c->pre.as<T>(), x, r0, r0_stride, u, part_rr, part_gj[0], dst,
-                     learned ? 0 : 1);
+        TF_LAUNCH_NB(8, gf_rhs_init_kernel, T, v.nb_b, c->stream, c->Ef.vals<T>(), c->R1f.vals<T>(), c->Ctf.signs(),
+                     v.A, c->free_edge.as<int32_t>(), c->U.as<T>(), c->dir_new.as<T>(), c->Fv.as<T>(),
+                     c->fb.as<T>(), c->rhs.as<T>(), c->dt, c->pre.as<T>(), v.x, r0, r0_stride, v.u, v.part_rr,
+                     v.part_gj[0], dst, cs.learned ? 0 : 1, guard);
     } else {
         ev_begin(c, "cg_init_kernel");
-        TF_LAUNCH_NB(8, cg_init_kernel, T, nb_b, c->stream, A, c->pre.as<T>(), c->rhs.as<T>(), x, r0, r0_stride, u,
-                     part_rr, part_gj[0], dst, learned ? 0 : 1);
+        TF_LAUNCH_NB(8, cg_init_kernel, T, v.nb_b, c->stream, v.A, c->pre.as<T>(), c->rhs.as<T>(), v.x, r0,
+                     r0_stride, v.u, v.part_rr, v.part_gj[0], dst, cs.learned ? 0 : 1);
     }
     ev_end(c);
-    capture(0);
-    int it = 0;
-    int batch = std::max(1, guess + 1);
-    for (;;) {
-        const int stop = std::min(it + batch, init.maxiter + 1);
-        for (; it < stop; ++it) {
-            if (learned) {
-                ev_begin(c, "cg_pg_kernel");
-                TF_LAUNCH_NB(c->tune[TK_G].unroll, cg_pg_kernel, T, nb_g, c->stream, it, G,
-                             rs[it == 0 ? 0 : (it - 1) & 1], rs[it & 1], p, x, u, part_del, nb_w, part_gl, part_rl,
-                             dst);
-                ev_end(c);
-                tag(it, 1);
-                ev_begin(c, "cg_wu_kernel");
-                TF_LAUNCH_NB(c->tune[TK_WU].unroll, cg_wu_kernel, T, nb_w, c->stream, it, A, u, p,
-                             rs[(it + 1) & 1], rs[it & 1], part_rl, part_gl, nb_g, part_del, dst);
-                ev_end(c);
-                tag(it, 0);
-                continue;  // at it == maxiter the stop test of cg_wu ends the solve
-            }
-            ev_begin(c, "cg_w_kernel");
-            TF_LAUNCH_NB(c->tune[TK_W].unroll, cg_w_kernel, T, nb_w, c->stream, it, A, u, w, 1, part_rr, nb_b,
-                         part_del, dst);
+    cs.it = 0;
+    cs.tags.clear();
+}
+
+template <class T>
+void cg_capture(tf_ctx* c, const CgSolve& cs, const CgView<T>& v, int k) {
+    if (!cs.snaps) return;
+    for (auto& sn : *cs.snaps)
+        if (sn.first == k)
+            bank_capture_kernel<T><<<blocks_for(v.nf), kBlock, 0, c->stream>>>(v.nf, sn.second, v.rj, v.x,
+                                                                                 c->lp.Rb.as<float>(),
+                                                                                 c->lp.Cb.as<float>());
+}
+
+// Launches passes [cs.it, min(cs.it + n, maxiter + 1)) and then an async copy
+// of the state into the pinned slot.
+template <class T>
+void cg_passes(tf_ctx* c, CgSolve& cs, int n) {
+    const CgView<T> v(c, cs.learned);
+    CgState* dst = c->cg_state.as<CgState>() + cs.slot;
+    auto tag = [&](int it, int first) {
+        if (c->profiling) cs.tags.push_back({(int64_t)c->recs.size() - 1, it, first});
+    };
+    if (cs.it == 0) cg_capture<T>(c, cs, v, 0);
+    const int stop = std::min(cs.it + n, cs.init.maxiter + 1);
+    for (int& it = cs.it; it < stop; ++it) {
+        if (cs.learned) {
+            ev_begin(c, "cg_pg_kernel");
+            TF_LAUNCH_NB(c->tune[TK_G].unroll, cg_pg_kernel, T, v.nb_g, c->stream, it, v.G,
+                         v.rs[it == 0 ? 0 : (it - 1) & 1], v.rs[it & 1], v.p, v.x, v.u, v.part_del, v.nb_w,
+                         v.part_gl, v.part_rl, dst);
+            ev_end(c);
+            tag(it, 1);
+            ev_begin(c, "cg_wu_kernel");
+            TF_LAUNCH_NB(c->tune[TK_WU].unroll, cg_wu_kernel, T, v.nb_w, c->stream, it, v.A, v.u, v.p,
+                         v.rs[(it + 1) & 1], v.rs[it & 1], v.part_rl, v.part_gl, v.nb_g, v.part_del, dst);
             ev_end(c);
             tag(it, 0);
-            if (it == init.maxiter) break;  // the pass at maxiter only runs the stop test
-            ev_begin(c, "cg_upd_kernel");
-            cg_upd_kernel<T><<<nb_b, kBlock, 0, c->stream>>>(it, nf, c->pre.as<T>(), x, rj, u, w, p, sv,
-                                                             part_gj[it & 1], nb_b, part_del, nb_w, part_rr,
-                                                             part_gj[(it + 1) & 1], dst);
-            ev_end(c);
-            tag(it, 0);
-            capture(it + 1);
+            continue;  // at it == maxiter the stop test of cg_wu ends the solve
         }
-        TF_HIP(hipMemcpyAsync(c->h_state, dst, sizeof(CgState), hipMemcpyDeviceToHost, c->stream));
-        TF_HIP(hipStreamSynchronize(c->stream));
-        if (c->h_state->done || it > init.maxiter) break;
-        batch = 4;
+        ev_begin(c, "cg_w_kernel");
+        TF_LAUNCH_NB(c->tune[TK_W].unroll, cg_w_kernel, T, v.nb_w, c->stream, it, v.A, v.u, v.w, 1, v.part_rr,
+                     v.nb_b, v.part_del, dst);
+        ev_end(c);
+        tag(it, 0);
+        if (it == cs.init.maxiter) { ++it; break; }  // the pass at maxiter only runs the stop test
+        ev_begin(c, "cg_upd_kernel");
+        cg_upd_kernel<T><<<v.nb_b, kBlock, 0, c->stream>>>(it, v.nf, c->pre.as<T>(), v.x, v.rj, v.u, v.w, v.p, v.sv,
+                                                           v.part_gj[it & 1], v.nb_b, v.part_del, v.nb_w,
+                                                           v.part_rr, v.part_gj[(it + 1) & 1], dst);
+        ev_end(c);
+        tag(it, 0);
+        cg_capture<T>(c, cs, v, it + 1);
+    }
+    TF_HIP(hipMemcpyAsync(c->h_state + cs.slot, dst, sizeof(CgState), hipMemcpyDeviceToHost, c->stream));
+}
+
+// Waits for the state copied by the last cg_passes (unless the caller has
+// already waited for it: synced), launching further batches of passes until
+// the solve has stopped; the final state is in h_state[cs.slot].
+template <class T>
+void cg_finish(tf_ctx* c, CgSolve& cs, bool synced) {
+    for (;;) {
+        if (!synced) TF_HIP(hipStreamSynchronize(c->stream));
+        synced = false;
+        const CgState& h = c->h_state[cs.slot];
+        if (h.done || cs.it > cs.init.maxiter) break;
+        cg_passes<T>(c, cs, 4);
     }
     // Launches past the stop test did no work: timed under their own name,
     // so that per-kernel averages cover working launches only (the learned
     // pass that reaches the stop test still updates x and r in cg_pg).
-    if (c->profiling && c->h_state->done) {
-        const int iters = c->h_state->iter;
-        for (auto& t : tags)
-            if (t[1] > iters || (t[1] == iters && !t[2])) c->recs[t[0]].name = "cg_stop_pass";
-    }
-    guess = c->h_state->iter;
+    const CgState& h = c->h_state[cs.slot];
+    if (c->profiling && h.done)
+        for (auto& t : cs.tags)
+            if (t[1] > h.iter || (t[1] == h.iter && !t[2])) c->recs[t[0]].name = "cg_stop_pass";
+    (cs.learned ? c->cg_guess_pc : c->cg_guess) = h.iter;
+}
+
+// A whole solve on the current right-hand side (the Jacobi redo of a
+// learned solve that missed the tolerance).
+template <class T>
+void cg_solve(tf_ctx* c, CgSolve& cs) {
+    cg_start<T>(c, cs, false, nullptr);
+    cg_passes<T>(c, cs, std::max(1, (cs.learned ? c->cg_guess_pc : c->cg_guess) + 1));
+    cg_finish<T>(c, cs, false);
 }
 
 // ------------------------------------------------- learned preconditioner
@@ -1303,60 +1377,31 @@ void learned_train(tf_ctx* c) {
     L.state = LearnedPc::kActive;
 }
 
-// The CG solve of one gfdtd step under the requested preconditioner, with
-// capture during collection, and the Jacobi redo when a learned solve does
-// not reach the tolerance.
-template <class T>
-void gfdtd_solve(tf_ctx* c, const CgState& init, bool want_learned, tf_stats* st) {
-    LearnedPc& L = c->lp;
-    const int32_t nf = c->n_free;
-    if (!want_learned || L.state == LearnedPc::kDisabled) {
-        cg_solve<T>(c, init, false, nullptr, true);
-        return;
-    }
-    if (L.state == LearnedPc::kCollecting) {
-        std::vector<std::pair<int, int>> snaps;
-        const int stride = std::max(1, L.learn_steps / (kCgSlots / 4));
-        if (L.steps % stride == stride - 1) {
-            int k = 0;
-            for (int sl = 0; sl < kCgSlots && k < 4; ++sl)
-                if (L.slot[sl] == 0) snaps.push_back({kCaptureIters[k++], sl});
-            if (k < 4) snaps.clear();
-        }
-        cg_solve<T>(c, init, false, snaps.empty() ? nullptr : &snaps, true);
-        const int iters = c->h_state->iter;
-        L.jac_iters.push_back(iters);
-        if (std::isfinite(c->h_state->relres) && c->h_state->relres <= init.tol)
-            for (auto& sn : snaps)
-                if (sn.first < iters) {
-                    bank_finalize_kernel<T><<<blocks_for(nf), kBlock, 0, c->stream>>>(
-                        nf, sn.second, c->x.as<T>(), L.isd.as<float>(), L.Rb.as<float>(), L.Cb.as<float>());
-                    L.slot[sn.second] = 1;
-                }
-        if (++L.steps >= L.learn_steps) learned_train(c);
-        return;
-    }
-    // active
-    cg_solve<T>(c, init, true, nullptr, true);
-    const bool ok = std::isfinite(c->h_state->relres) && c->h_state->relres <= init.tol;
-    L.solves++;
-    L.iters += c->h_state->iter;
-    L.pc_iters.push_back(c->h_state->iter);
-    if (!ok) {
-        // Jacobi-PCG redo from the same warm start (U still holds x_old on free edges)
-        st->cg_iters_total += c->h_state->iter;
-        gather_free_kernel<T><<<blocks_for(nf), kBlock, 0, c->stream>>>(nf, c->free_edge.as<int32_t>(),
-                                                                        c->U.as<T>(), c->x.as<T>());
-        cg_solve<T>(c, init, false, nullptr, false);
-        if (++L.fallbacks > 3) L.state = LearnedPc::kDisabled;
-    }
-    // plain CG judges the gain: after 8 learned solves, no fewer iterations
-    // than the collection's Jacobi solves disables the learned operator
-    if (L.state == LearnedPc::kActive && L.pc_iters.size() == 8 && L.jac_iters.size() >= 8) {
-        double a = 0, b = 0;
-        for (int k = 0; k < 8; ++k) { a += L.pc_iters[k]; b += L.jac_iters[L.jac_iters.size() - 8 + k]; }
-        if (a >= b) L.state = LearnedPc::kDisabled;
-    }
+// ---- the gfdtd step loop (D:1459-1565) -----------------------------------
+// Step i: B -= dt C U (dga_curl), F = N B (sell_n), probe sample at t - dt,
+// sources (gf_inject), rhs and r = b - A x (gf_rhs_init), the CG passes,
+// then U = x on free edges and the Dirichlet values (gf_scatter).
+//
+// The host enqueues step k's scatter and step k+1's kernels up to
+// gf_rhs_init right after solve k's first batch of passes, each guarded on
+// solve k's state (cg_guard_ok), and only then waits for that batch: the GPU
+// has that work queued while the host reads the state and enqueues the next
+// passes. When solve k needs more passes, or a learned solve misses the
+// tolerance and is redone with Jacobi, the guarded kernels have returned
+// without effect and are enqueued again once solve k is complete.
+
+struct GfPre {
+    int64_t sample_slot = -1;  // probe sample slot of this step (-1: none)
+    bool learned = false;      // preconditioner the step's gf_rhs_init was launched for
+};
+
+// The learned preconditioner state the next solve will see: active, or
+// about to become active when the collection ends with this solve.
+bool predict_learned(const tf_ctx* c, bool want_learned) {
+    const LearnedPc& L = c->lp;
+    if (!want_learned) return false;
+    if (L.state == LearnedPc::kActive) return true;
+    return L.state == LearnedPc::kCollecting && L.steps + 1 >= L.learn_steps;
 }
 
 template <class T>
@@ -1371,43 +1416,148 @@ void run_gfdtd(tf_ctx* c, int64_t first, int64_t n, const tf_cg_opts* cg, tf_sta
     CgState init{};
     init.tol = cg ? cg->tol : 1e-8;
     init.maxiter = cg ? cg->maxiter : 100;
-    const bool learned = cg && cg->precond == TF_PRECOND_LEARNED;
-    if (learned && c->lp.state == LearnedPc::kOff) learned_setup(c, cg);
+    const bool want_learned = cg && cg->precond == TF_PRECOND_LEARNED;
+    if (want_learned && c->lp.state == LearnedPc::kOff) learned_setup(c, cg);
+    if (!c->solve_ev) TF_HIP(hipEventCreateWithFlags(&c->solve_ev, hipEventDisableTiming));
+    LearnedPc& L = c->lp;
     const int gb_n = tuned_blocks(c, TK_N, c->N, kSpmvCap);
-    for (int64_t k = 0; k < n; ++k) {
+    CgState* states = c->cg_state.as<CgState>();
+
+    // Kernels of step k up to the start of its solve (guard: the previous solve's state).
+    auto pre = [&](int64_t k, GfPre& g, bool fresh, const CgState* guard) {
         const int64_t i = first + k;
         const double t = (double)i * c->dt;
         const double* cs = c->carr.as<double>() + (k + 1) * ns;
         const double* cp = c->carr.as<double>() + k * ns;
         ev_begin(c, "dga_curl_kernel");
-        dga_curl_kernel<T><<<blocks_for(c->F), kBlock, 0, c->stream>>>(c->F, c->c_col.as<int32_t>(),
-                                                                        c->c_fixed_signs ? nullptr : c->c_sg.as<int8_t>(), c->U.as<T>(),
-                                                                        c->curl_u.as<T>(), c->B.as<T>(), c->dt, nullptr);
+        dga_curl_kernel<T><<<blocks_for(c->F), kBlock, 0, c->stream>>>(
+            c->F, c->c_col.as<int32_t>(), c->c_fixed_signs ? nullptr : c->c_sg.as<int8_t>(), c->U.as<T>(),
+            c->curl_u.as<T>(), c->B.as<T>(), c->dt, nullptr, guard);
         ev_end(c);
         ev_begin(c, "sell_n_kernel");
         TF_LAUNCH_NB(c->tune[TK_N].unroll, sell_n_kernel, T, gb_n, c->stream, c->N.vals<T>(), c->B.as<T>(),
-                     c->Fv.as<T>(), c->dt, 1, nullptr);
+                     c->Fv.as<T>(), c->dt, 1, nullptr, guard);
         ev_end(c);
-        if (allow_print(c, t - c->dt)) sample<T>(c, t - c->dt);
+        if (fresh) g.sample_slot = allow_print(c, t - c->dt) ? sample_reserve(c, t - c->dt) : -1;
+        sample_launch<T>(c, g.sample_slot, guard);
         if (c->n_inj) {
             ev_begin(c, "gf_inject_kernel");
             gf_inject_kernel<T><<<inj_blocks, kBlock, 0, c->stream>>>(
                 c->n_inj, c->inj_edge.as<int32_t>(), c->inj_kind.as<int8_t>(), c->inj_tb.as<int32_t>(),
                 c->inj_te.as<int32_t>(), c->term_src.as<int32_t>(), c->term_w.as<double>(), cs, i > 1 ? cp : nullptr,
-                c->fb.as<T>(), c->dir_new.as<T>(), c->U.as<T>(), c->poy.as<double>() + k * inj_blocks);
+                c->fb.as<T>(), c->dir_new.as<T>(), c->U.as<T>(), c->poy.as<double>() + k * inj_blocks, guard);
             ev_end(c);
         }
-        gfdtd_solve<T>(c, init, learned, st);
-        st->cg_iters_total += c->h_state->iter;
-        st->cg_iters_max = std::max(st->cg_iters_max, c->h_state->iter);
-        const double rel = c->h_state->relres;
-        if (std::isfinite(rel)) st->cg_max_relres = std::max(st->cg_max_relres, rel);
-        if (c->h_state->iter >= init.maxiter && !(rel <= init.tol)) st->cg_not_converged++;
+        CgSolve s0;
+        s0.init = init;
+        s0.learned = g.learned;
+        s0.slot = (int)(k & 1);
+        cg_start<T>(c, s0, true, guard);
+    };
+    auto scatter = [&](const CgState* guard) {
         ev_begin(c, "gf_scatter_kernel");
         gf_scatter_kernel<T><<<blocks_for(nf + c->n_dir), kBlock, 0, c->stream>>>(
             nf, c->free_edge.as<int32_t>(), c->x.as<T>(), c->U.as<T>(), c->n_dir, c->dir_edge.as<int32_t>(),
-            c->dir_new.as<T>());
+            c->dir_new.as<T>(), guard);
         ev_end(c);
+    };
+    // collection captures of the solve of the current step
+    auto collection_snaps = [&]() {
+        std::vector<std::pair<int, int>> snaps;
+        const int stride = std::max(1, L.learn_steps / (kCgSlots / 4));
+        if (L.steps % stride == stride - 1) {
+            int k = 0;
+            for (int sl = 0; sl < kCgSlots && k < 4; ++sl)
+                if (L.slot[sl] == 0) snaps.push_back({kCaptureIters[k++], sl});
+            if (k < 4) snaps.clear();
+        }
+        return snaps;
+    };
+
+    GfPre cur;
+    cur.learned = predict_learned(c, want_learned) && L.state == LearnedPc::kActive;
+    pre(0, cur, true, nullptr);
+    for (int64_t k = 0; k < n; ++k) {
+        // the solve of step k: the mode it needs now, its start redone if the
+        // start already launched was for the other preconditioner
+        const bool learned = want_learned && L.state == LearnedPc::kActive;
+        CgSolve sol;
+        sol.init = init;
+        sol.learned = learned;
+        sol.slot = (int)(k & 1);
+        std::vector<std::pair<int, int>> snaps;
+        if (want_learned && L.state == LearnedPc::kCollecting) snaps = collection_snaps();
+        sol.snaps = snaps.empty() ? nullptr : &snaps;
+        if (learned != cur.learned) {
+            cur.learned = learned;
+            cg_start<T>(c, sol, true, nullptr);
+        } else {
+            sol.init.need_conv = learned ? 1 : 0;
+            sol.it = 0;
+        }
+        cg_passes<T>(c, sol, std::max(1, (learned ? c->cg_guess_pc : c->cg_guess) + 1));
+        TF_HIP(hipEventRecord(c->solve_ev, c->stream));
+        // ahead of the check: this step's scatter and the next step's start
+        const CgState* g = states + sol.slot;
+        scatter(g);
+        GfPre nxt;
+        nxt.learned = predict_learned(c, want_learned);
+        if (k + 1 < n) pre(k + 1, nxt, true, g);
+        // the check of solve k's first batch
+        TF_HIP(hipEventSynchronize(c->solve_ev));
+        const CgState h0 = c->h_state[sol.slot];
+        const bool ran_ahead = h0.done && (!learned || h0.relres <= init.tol);
+        cg_finish<T>(c, sol, true);
+        CgState h = c->h_state[sol.slot];
+        bool ok = std::isfinite(h.relres) && h.relres <= init.tol;
+        int64_t iters = h.iter;
+        if (learned) {
+            L.solves++;
+            L.iters += h.iter;
+            L.pc_iters.push_back(h.iter);
+            if (!ok) {
+                // Jacobi-PCG redo from the same warm start (U still holds x_old on free edges)
+                st->cg_iters_total += h.iter;
+                gather_free_kernel<T><<<blocks_for(nf), kBlock, 0, c->stream>>>(nf, c->free_edge.as<int32_t>(),
+                                                                                c->U.as<T>(), c->x.as<T>());
+                CgState* sh = c->h_state + sol.slot;
+                CgSolve redo;
+                redo.init = init;
+                redo.slot = sol.slot;
+                cg_solve<T>(c, redo);
+                h = *sh;
+                ok = std::isfinite(h.relres) && h.relres <= init.tol;
+                iters = h.iter;
+                if (++L.fallbacks > 3) L.state = LearnedPc::kDisabled;
+            }
+            // plain CG judges the gain: after 8 learned solves, no fewer
+            // iterations than the collection's Jacobi solves turns G off
+            if (L.state == LearnedPc::kActive && L.pc_iters.size() == 8 && L.jac_iters.size() >= 8) {
+                double a = 0, b = 0;
+                for (int q = 0; q < 8; ++q) { a += L.pc_iters[q]; b += L.jac_iters[L.jac_iters.size() - 8 + q]; }
+                if (a >= b) L.state = LearnedPc::kDisabled;
+            }
+        } else if (want_learned && L.state == LearnedPc::kCollecting) {
+            L.jac_iters.push_back(h.iter);
+            if (ok)
+                for (auto& sn : snaps)
+                    if (sn.first < h.iter) {
+                        bank_finalize_kernel<T><<<blocks_for(nf), kBlock, 0, c->stream>>>(
+                            nf, sn.second, c->x.as<T>(), L.isd.as<float>(), L.Rb.as<float>(), L.Cb.as<float>());
+                        L.slot[sn.second] = 1;
+                    }
+            if (++L.steps >= L.learn_steps) learned_train(c);
+        }
+        st->cg_iters_total += iters;
+        st->cg_iters_max = std::max<int32_t>(st->cg_iters_max, (int32_t)iters);
+        if (std::isfinite(h.relres)) st->cg_max_relres = std::max(st->cg_max_relres, h.relres);
+        if (iters >= init.maxiter && !ok) st->cg_not_converged++;
+        if (!ran_ahead) {
+            // the guarded kernels returned without effect: run them now
+            scatter(nullptr);
+            if (k + 1 < n) pre(k + 1, nxt, false, nullptr);
+        }
+        cur = nxt;
         if (c->profiling) ev_flush(c);
     }
     TF_HIP(hipGetLastError());
@@ -1422,7 +1572,6 @@ void run_gfdtd(tf_ctx* c, int64_t first, int64_t n, const tf_cg_opts* cg, tf_sta
         c->poynting.push_back(s);
     }
     st->steps_done += n;
-    const LearnedPc& L = c->lp;
     st->pc_state = L.state;
     st->pc_fallbacks = L.fallbacks;
     st->pc_learned_solves = L.solves;
@@ -1709,11 +1858,13 @@ void autotune(tf_ctx* c) {
                          c->U.as<T>(), 0.0, nullptr);
         });
         c->tune[TK_N] = tune_one(c, TK_N, c->N, kSpmvCap, [&](int b, int u) {
-            TF_LAUNCH_NB(u, sell_n_kernel, T, b, s, c->N.vals<T>(), c->curl_u.as<T>(), c->Fv.as<T>(), 0.0, 0, nullptr);
+            TF_LAUNCH_NB(u, sell_n_kernel, T, b, s, c->N.vals<T>(), c->curl_u.as<T>(), c->Fv.as<T>(), 0.0, 0, nullptr,
+                         nullptr);
         });
     } else if (c->method == TF_METHOD_GFDTD) {
         c->tune[TK_N] = tune_one(c, TK_N, c->N, kSpmvCap, [&](int b, int u) {
-            TF_LAUNCH_NB(u, sell_n_kernel, T, b, s, c->N.vals<T>(), c->B.as<T>(), c->Fv.as<T>(), c->dt, 1, nullptr);
+            TF_LAUNCH_NB(u, sell_n_kernel, T, b, s, c->N.vals<T>(), c->B.as<T>(), c->Fv.as<T>(), c->dt, 1, nullptr,
+                         nullptr);
         });
         // Tuning launches run pass 1 (the full work of a pass) on a scratch
         // CG state that never stops (tol 0, no iteration cap) with unit
