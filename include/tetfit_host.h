@@ -54,6 +54,8 @@ int32_t tf_build_output_mode(const tf_build* b); /* 0 none,1 probepoint,...  */
 double  tf_build_output_delay(const tf_build* b);
 double  tf_build_output_period(const tf_build* b);
 const char* tf_build_output_name(const tf_build* b);
+/* Output::ReferenceFrame (Output.hpp:331): 0 cartesian, 1 cylindrical, 2 spherical. */
+int32_t tf_build_output_frame(const tf_build* b);
 
 void tf_build_sources(const tf_build* b, const tf_source** src, int32_t* n_src,
                       const tf_source_term** terms, int64_t* n_terms);

@@ -131,7 +131,7 @@ def host_lib():
                      ("tf_build_n_steps", C.c_int64), ("tf_build_first_step", C.c_int64),
                      ("tf_build_cg_tol", C.c_double), ("tf_build_output_mode", C.c_int32),
                      ("tf_build_output_delay", C.c_double), ("tf_build_output_period", C.c_double),
-                     ("tf_build_output_name", C.c_char_p)]:
+                     ("tf_build_output_name", C.c_char_p), ("tf_build_output_frame", C.c_int32)]:
         f = getattr(L, name)
         f.argtypes = [vp]
         f.restype = rt

@@ -43,6 +43,7 @@ struct Build {
     double cg_tol = 1e-8;
     int32_t out_mode = 0;  // 0 none, 1 probepoint, 2 maxerror, 3 l2norm, 4 silo
     double out_delay = 0, out_period = 0;
+    int32_t out_frame = 0;  // Output::ReferenceFrame: 0 cartesian, 1 cylindrical, 2 spherical
     std::string out_name;
 
     // ---- FIT tables

@@ -100,6 +100,7 @@ int32_t tf_build_output_mode(const tf_build* b) { return b ? b->b.out_mode : 0; 
 double tf_build_output_delay(const tf_build* b) { return b ? b->b.out_delay : 0; }
 double tf_build_output_period(const tf_build* b) { return b ? b->b.out_period : 0; }
 const char* tf_build_output_name(const tf_build* b) { return b ? b->b.out_name.c_str() : ""; }
+int32_t tf_build_output_frame(const tf_build* b) { return b ? b->b.out_frame : 0; }
 
 void tf_build_sources(const tf_build* b, const tf_source** src, int32_t* n_src, const tf_source_term** terms,
                       int64_t* n_terms) {

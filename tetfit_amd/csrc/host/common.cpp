@@ -186,6 +186,7 @@ void set_output(const Model& m, const SimulationDef& sim, Build& b) {
     b.out_mode = md == "probepoint" ? 1 : md == "maxerror" ? 2 : md == "l2norm" ? 3 : md == "silo" ? 4 : 0;
     b.out_delay = o.delay;
     b.out_period = o.period;
+    b.out_frame = (int32_t)o.ref_sys;
     b.out_name = o.name;
 }
 
