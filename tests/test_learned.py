@@ -190,3 +190,36 @@ def test_learned_preconditioner_gain(contrast_cavity, prec, bar):
     assert rel(samples, r["samples"]) <= bar
     for got, ref in ((U, r["U"]), (F, r["F"]), (B, r["B"])):
         assert rel(got, ref) <= bar
+
+
+@pytest.mark.gpu
+def test_training_is_what_gains(tmp_path):
+    """On the 48^3 Kuhn workload the operator the network starts from does
+    not beat Jacobi: with training switched off (learning rate 1e-12, so the
+    zero-initialised output layer keeps y = 0 and G is the first-order
+    Neumann inverse D^-1/2 (2I - Â) D^-1/2, which loses definiteness when the
+    largest eigenvalue of Â exceeds 2) its solves take more iterations than
+    Jacobi-PCG, or miss the tolerance and are redone, and plain CG switches it
+    off. The trained network's G meets the tolerance in every solve with at
+    most 0.6x the Jacobi-PCG iterations. (On the small Kuhn 10^3 mesh of the
+    gain test the untrained base does about as well as the trained one.)"""
+    from tetfit_amd.device import Simulation
+    p = configs.kuhn_cavity(str(tmp_path), n=48, method="gfdtd", n_steps=120, n_probes=2, block=8, seed=2,
+                            dt=4.777e-11)
+    b = Model(p).build(1)
+    res = {}
+    for name, lr in (("trained", 3e-3), ("untrained", 1e-12)):
+        print(f"case: {name} preconditioner E={b.n_edges}", flush=True)
+        sim = Simulation(b, "fp32", output=False)
+        res[name + "_collect"] = sim.run(n=60, precond="learned", learn_steps=60, train_iters=150, lr=lr)
+        res[name] = sim.run(n=20, precond="learned")
+        sim.close()
+        print(name, res[name], flush=True)
+    t, u = res["trained"], res["untrained"]
+    jac = res["trained_collect"]["pc_jacobi_mean_iters"]
+    assert t["pc_state"] == 2 and t["pc_fallbacks"] == 0 and t["cg_not_converged"] == 0
+    assert t["cg_iters_total"] <= 0.6 * jac * 20
+    assert t["pc_loss"] < 0.9 * t["pc_loss0"]
+    assert u["pc_loss"] >= 0.99 * u["pc_loss0"]  # nothing learned
+    assert u["pc_state"] == 3  # switched off
+    assert u["pc_fallbacks"] >= 1 or u["pc_learned_iters"] > jac * u["pc_learned_solves"]
