@@ -156,6 +156,12 @@ tf_status tf_kernel_times(tf_ctx* ctx, char* names, int32_t cap_names,
 /* Enables per-kernel event timing for subsequent runs (costs launch overhead). */
 tf_status tf_set_profiling(tf_ctx* ctx, int32_t on);
 
+/* fdtd and dga contexts step in HIP graphs when no launch depends on the host
+ * (no probe sampling, no profiling): 64 steps of launches are captured once
+ * and replayed; on (default) or off (every step launched from the host).
+ * Results are identical either way. No reference counterpart. */
+tf_status tf_set_graphs(tf_ctx* ctx, int32_t on);
+
 /* Device bytes held by the context. */
 tf_status tf_device_bytes(tf_ctx* ctx, int64_t* bytes);
 

@@ -181,3 +181,48 @@ def test_tet_energy_diagnostic(workdir, method):
         ref_we, ref_wh = 0.5 * U @ (Em @ U), 0.5 * B @ (Nm @ B)
         assert ref_we > 0 and ref_wh > 0
         assert abs(we - ref_we) <= 1e-12 * ref_we and abs(wh - ref_wh) <= 1e-12 * ref_wh
+
+
+@pytest.mark.parametrize("method", ["fdtd", "dga"])
+def test_graph_steps_match_eager(workdir, method):
+    """With nothing sampled, fdtd and dga step in HIP graphs of 64 steps
+    (captured once, replayed with the carrier rows refilled and the Poynting
+    partials drained by stream copies around each replay). Several runs of
+    150 steps (graph replays with eager remainder steps and host waits in
+    between) give fields and Poynting values bitwise equal to launching
+    every step from the host."""
+    if method == "fdtd":
+        b = Model(configs.fit_cavity(str(workdir), n=10, step=0.1, n_steps=400, n_probes=2, name="graph_fit")).build(1)
+    else:
+        b = Model(configs.kuhn_cavity(str(workdir), n=5, method="dga", n_steps=400, n_probes=2, block=2,
+                                      name="graph_dga")).build(1)
+    out = {}
+    for graphs in (True, False):
+        print(f"case: {method} graphs={graphs}", flush=True)
+        sim = Simulation(b, "fp64", output=False)
+        sim.set_graphs(graphs)
+        for _ in range(2):
+            sim.run(n=150)
+        out[graphs] = (*sim.state(), sim.poynting())
+        sim.close()
+    for a, e in zip(out[True], out[False]):
+        assert np.array_equal(a, e)
+    assert np.abs(out[True][0]).max() > 0
+
+
+def test_batch_graph_steps_match_eager(workdir):
+    from tetfit_amd.device import BatchSimulation
+    builds = [Model(configs.kuhn_cavity(str(workdir), n=n, method="dga", n_steps=300, n_probes=1, block=2, seed=n,
+                                        freq=f, name=f"gb{n}")).build(1) for n, f in ((3, 2e8), (4, 3e8))]
+    out = {}
+    for graphs in (True, False):
+        print(f"case: batch graphs={graphs}", flush=True)
+        sim = BatchSimulation(builds, "fp64")
+        sim.set_graphs(graphs)
+        for _ in range(2):
+            sim.run(140)
+        out[graphs] = [sim.state(k) for k in range(len(builds))]
+        sim.close()
+    for k in range(len(builds)):
+        for a, e in zip(out[True][k], out[False][k]):
+            assert np.array_equal(a, e)

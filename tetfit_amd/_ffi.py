@@ -197,12 +197,13 @@ def hip_lib():
     L.tf_get_tuning.argtypes = [vp, i32p, i32p, C.c_int32, i32p]
     L.tf_sell_info.argtypes = [vp, C.c_int32, i64p]
     L.tf_set_tuning.argtypes = [vp, i32p, i32p, C.c_int32]
+    L.tf_set_graphs.argtypes = [vp, C.c_int32]
     for n in ["tf_create", "tf_set_sources", "tf_set_probes", "tf_set_output", "tf_run",
               "tf_read_state", "tf_write_state", "tf_read_probes", "tf_read_poynting",
               "tf_energy", "tf_kernel_times", "tf_set_profiling", "tf_device_bytes", "tf_destroy",
               "tf_mlp_grad", "tf_create_batch", "tf_batch_set_sources", "tf_batch_set_probes",
               "tf_batch_set_output", "tf_batch_read_probes", "tf_batch_read_state", "tf_autotune",
-              "tf_get_tuning", "tf_sell_info", "tf_set_tuning"]:
+              "tf_get_tuning", "tf_sell_info", "tf_set_tuning", "tf_set_graphs"]:
         getattr(L, n).restype = C.c_int32
     _hip = L
     return L

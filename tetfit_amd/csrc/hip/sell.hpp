@@ -454,7 +454,7 @@ __global__ void __launch_bounds__(kBlock) cg_pg_kernel(int32_t it, SellView<uint
             T ri = o.r;
             if (it > 0) {
                 ri = cg_r_next(o.r, o.s, alpha);
-                rs_out[i].r = ri;
+                rs_out[i] = RS<T>{ri, o.s};  // whole pair: s is replaced by cg_wu of this pass
                 x[i] = (T)((double)x[i] + alpha * (double)p[i]);
             }
             const T ui = (T)uv;
@@ -504,12 +504,13 @@ __global__ void __launch_bounds__(kBlock) cg_wu_kernel(int32_t it, SellView<T> m
         if (i < m.nrows) {
             const T wi = (T)wv;
             const T ui = u[i];
+            const T ri = rs_cur[i].r;
             if (it == 0) {
                 p[i] = ui;
-                rs_cur[i].s = wi;
+                rs_cur[i] = RS<T>{ri, wi};
             } else {
                 p[i] = (T)((double)ui + beta * (double)p[i]);
-                rs_cur[i].s = (T)((double)wi + beta * (double)rs_prev[i].s);
+                rs_cur[i] = RS<T>{ri, (T)((double)wi + beta * (double)rs_prev[i].s)};
             }
             wu += (double)wi * (double)ui;
         }

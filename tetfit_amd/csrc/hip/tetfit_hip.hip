@@ -222,6 +222,19 @@ struct tf_ctx {
     int64_t poy_cap = 0;
     std::vector<double> poynting;
 
+    // Step loops of the explicit methods captured in a HIP graph: kGraphSteps
+    // steps of launches whose per-step carrier rows and Poynting partial
+    // slots point into the graph's own buffers (carr, poy), refilled and
+    // drained around each replay. Rebuilt after anything that changes the
+    // launches (sources, tuning).
+    struct StepGraph {
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        DevBuf carr, poy;
+    };
+    StepGraph sg;
+    bool use_graphs = true;
+
     // profiling
     bool profiling = false;
     struct Rec { const char* name; hipEvent_t a, b; };
@@ -236,6 +249,8 @@ struct tf_ctx {
 
 tf_ctx::~tf_ctx() {
     if (stream) (void)hipStreamSynchronize(stream);
+    if (sg.exec) (void)hipGraphExecDestroy(sg.exec);
+    if (sg.graph) (void)hipGraphDestroy(sg.graph);
     if (solve_ev) (void)hipEventDestroy(solve_ev);
     for (auto e : ev_pool) (void)hipEventDestroy(e);
     if (h_state) (void)hipHostFree(h_state);
@@ -801,6 +816,62 @@ void ensure_carr(tf_ctx* c, int64_t first, int64_t n) {
     TF_HIP(hipStreamSynchronize(c->stream));
 }
 
+constexpr int kGraphSteps = 64;
+
+void drop_graph(tf_ctx* c) {
+    if (c->sg.exec) (void)hipGraphExecDestroy(c->sg.exec);
+    if (c->sg.graph) (void)hipGraphDestroy(c->sg.graph);
+    c->sg.exec = nullptr;
+    c->sg.graph = nullptr;
+}
+
+// Graph replay is used when no launch of the chunk depends on the host: no
+// probe sampling and no per-kernel event timing.
+bool graph_steps(const tf_ctx* c) {
+    if (!c->use_graphs || c->profiling) return false;
+    return c->batch ? c->batch->every <= 0 : !c->out_enabled;
+}
+
+// Captures kGraphSteps calls of step(local step, carrier row, Poynting slot)
+// into the context's graph (once), then runs steps [k0, k0 + m) of the
+// current tf_run in chunks of kGraphSteps by replay; per_step = Poynting
+// partials per step. Returns the number of steps run (a multiple of
+// kGraphSteps); the caller runs the rest with ordinary launches.
+template <class Step>
+int64_t run_graph_chunks(tf_ctx* c, int64_t m, int per_step, Step step) {
+    const int64_t chunks = m / kGraphSteps;
+    if (chunks == 0) return 0;
+    const size_t ns = std::max<size_t>(c->src.size(), 1);
+    if (!c->sg.exec) {
+        c->sg.carr.alloc((kGraphSteps + 1) * ns * sizeof(double));
+        c->sg.poy.alloc(std::max(1, kGraphSteps * per_step) * sizeof(double));
+        TF_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        try {
+            for (int k = 0; k < kGraphSteps; ++k)
+                step(k, c->sg.carr.as<double>() + (size_t)(k + 1) * ns, c->sg.poy.as<double>() + (size_t)k * per_step);
+        } catch (...) {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(c->stream, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        TF_HIP(hipStreamEndCapture(c->stream, &c->sg.graph));
+        TF_HIP(hipGraphInstantiate(&c->sg.exec, c->sg.graph, nullptr, nullptr, 0));
+    }
+    for (int64_t q = 0; q < chunks; ++q) {
+        const int64_t k0 = q * kGraphSteps;
+        // carrier rows k0 .. k0 + kGraphSteps of this run (row 0 = the step before)
+        TF_HIP(hipMemcpyAsync(c->sg.carr.p, c->carr.as<double>() + (size_t)k0 * ns,
+                              (kGraphSteps + 1) * ns * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+        TF_HIP(hipGraphLaunch(c->sg.exec, c->stream));
+        if (per_step)
+            TF_HIP(hipMemcpyAsync(c->poy.as<double>() + (size_t)k0 * per_step, c->sg.poy.p,
+                                  (size_t)kGraphSteps * per_step * sizeof(double), hipMemcpyDeviceToDevice,
+                                  c->stream));
+    }
+    return chunks * kGraphSteps;
+}
+
 template <class T>
 void run_fit(tf_ctx* c, int64_t first, int64_t n, tf_stats* st) {
     const int main_blocks = blocks_for(c->E);
@@ -812,23 +883,29 @@ void run_fit(tf_ctx* c, int64_t first, int64_t n, tf_stats* st) {
             c->poy_cap = n * sp_blocks;
         }
     }
-    for (int64_t k = 0; k < n; ++k) {
-        const int64_t i = first + k;
-        const double t = (double)i * c->dt;
-        const double* cs = c->carr.as<double>() + (k + 1) * ns;
+    // one step: E phase (with sources and the Poynting partials), H phase
+    auto step = [&](const double* cs, double* poy) {
         ev_begin(c, "fit_e_kernel");
         fit_e_kernel<T><<<main_blocks + sp_blocks, kBlock, 0, c->stream>>>(
             c->E, main_blocks, c->fe_sg.as<uint8_t>(), c->fe_ca.as<T>(), c->fe_cb.as<T>(), c->fe_f4.as<int4>(),
             c->Fv.as<T>(), c->U.as<T>(), c->n_sp, c->sp_edge.as<int32_t>(), c->sp_f4.as<int4>(),
             c->sp_sg.as<uint8_t>(), c->sp_ca.as<double>(), c->sp_cb.as<double>(), c->sp_kind.as<int8_t>(),
             c->sp_tb.as<int32_t>(), c->sp_te.as<int32_t>(), c->term_src.as<int32_t>(), c->term_w.as<double>(), cs,
-            sp_blocks ? c->poy.as<double>() + k * sp_blocks : nullptr);
+            sp_blocks ? poy : nullptr);
         ev_end(c);
         ev_begin(c, "fit_h_kernel");
         fit_h_kernel<T><<<blocks_for(c->F), kBlock, 0, c->stream>>>(c->F, c->fh_sg.as<uint8_t>(), c->fh_cm.as<T>(),
                                                                      c->fh_cn.as<T>(), c->fh_e4.as<int4>(),
                                                                      c->U.as<T>(), c->Fv.as<T>(), c->B.as<T>(), c->dt);
         ev_end(c);
+    };
+    const int64_t g = graph_steps(c) ? run_graph_chunks(c, n, sp_blocks, [&](int, const double* cs, double* poy) {
+        step(cs, poy);
+    }) : 0;
+    for (int64_t k = g; k < n; ++k) {
+        const int64_t i = first + k;
+        const double t = (double)i * c->dt;
+        step(c->carr.as<double>() + (k + 1) * ns, c->poy.as<double>() + k * sp_blocks);
         if (allow_print(c, t)) sample<T>(c, t);
         if (c->profiling && (k & 63) == 63) ev_flush(c);
     }
@@ -871,10 +948,10 @@ void run_dga(tf_ctx* c, int64_t first, int64_t n, tf_stats* st) {
     const double* fdt = c->batch ? c->row_dt_f.as<double>() : nullptr;
     const int gb_ct = tuned_blocks(c, TK_CT, c->sCt, kSpmvCap), gb_h = tuned_blocks(c, TK_H, c->H, kSpmvCap);
     const int gb_n = tuned_blocks(c, TK_N, c->N, kSpmvCap);
-    for (int64_t k = 0; k < n; ++k) {
-        const int64_t i = first + k;
-        const double t = (double)i * c->dt;
-        const double* cs = c->carr.as<double>() + (k + 1) * ns;
+    // one step (D:810-939): curl_f = Ct F, h sources, U += dt H curl_f,
+    // Poynting, curl_u = C U with B update, F -= dt N curl_u; poy holds the
+    // step's two Poynting partial blocks
+    auto step = [&](const double* cs, double* poy) {
         ev_begin(c, "sell_ct_kernel");
         TF_LAUNCH_NB(c->tune[TK_CT].unroll, sell_ct_kernel, T, gb_ct, c->stream, c->sCt.signs(), c->Fv.as<T>(),
                      c->curl_f.as<T>());
@@ -884,7 +961,7 @@ void run_dga(tf_ctx* c, int64_t first, int64_t n, tf_stats* st) {
             dga_inject_kernel<T><<<inj_blocks, kBlock, 0, c->stream>>>(
                 c->n_inj, c->inj_edge.as<int32_t>(), c->inj_kind.as<int8_t>(), c->inj_tb.as<int32_t>(),
                 c->inj_te.as<int32_t>(), c->term_src.as<int32_t>(), c->term_w.as<double>(), cs, c->curl_f.as<T>(),
-                c->U.as<T>(), c->poy.as<double>() + 2 * k * inj_blocks);
+                c->U.as<T>(), poy);
             ev_end(c);
         }
         ev_begin(c, "sell_h_kernel");
@@ -896,7 +973,7 @@ void run_dga(tf_ctx* c, int64_t first, int64_t n, tf_stats* st) {
             dga_poynting_kernel<T><<<inj_blocks, kBlock, 0, c->stream>>>(
                 c->n_inj, c->inj_edge.as<int32_t>(), c->inj_kind.as<int8_t>(), c->inj_tb.as<int32_t>(),
                 c->inj_te.as<int32_t>(), c->term_src.as<int32_t>(), c->term_w.as<double>(), cs, c->U.as<T>(),
-                c->poy.as<double>() + (2 * k + 1) * inj_blocks);
+                poy + inj_blocks);
             ev_end(c);
         }
         ev_begin(c, "dga_curl_kernel");
@@ -909,6 +986,14 @@ void run_dga(tf_ctx* c, int64_t first, int64_t n, tf_stats* st) {
         TF_LAUNCH_NB(c->tune[TK_N].unroll, sell_n_kernel, T, gb_n, c->stream, c->N.vals<T>(), c->curl_u.as<T>(),
                      c->Fv.as<T>(), c->dt, 0, fdt, nullptr);
         ev_end(c);
+    };
+    const int64_t g = graph_steps(c) ? run_graph_chunks(c, n, 2 * inj_blocks, [&](int, const double* cs, double* poy) {
+        step(cs, poy);
+    }) : 0;
+    for (int64_t k = g; k < n; ++k) {
+        const int64_t i = first + k;
+        const double t = (double)i * c->dt;
+        step(c->carr.as<double>() + (k + 1) * ns, c->poy.as<double>() + 2 * k * inj_blocks);
         if (want_sample(c, i, t)) sample<T>(c, t);
         if (c->profiling && (k & 63) == 63) ev_flush(c);
     }
@@ -1946,6 +2031,7 @@ tf_status tf_set_sources(tf_ctx* c, const tf_source* src, int32_t n_src, const t
                          int64_t n_terms) {
     if (!c || (n_src && !src) || (n_terms && !terms)) { g_err = "null argument"; return TF_ERR_ARG; }
     return guarded([&] {
+        drop_graph(c);
         c->src.assign(src, src + n_src);
         std::vector<std::vector<std::pair<int32_t, double>>> by_edge(c->E);
         std::vector<int> fld(c->E, -1);
@@ -2221,6 +2307,7 @@ tf_status tf_mlp_grad(const float* params, const float* x, const float* gy, int6
 tf_status tf_autotune(tf_ctx* c) {
     if (!c) { g_err = "null argument"; return TF_ERR_ARG; }
     return guarded([&] {
+        drop_graph(c);
         if (c->prec == TF_PREC_FP64) autotune<double>(c);
         else autotune<float>(c);
     });
@@ -2238,8 +2325,15 @@ tf_status tf_sell_info(tf_ctx* c, int32_t which, int64_t* info) {
     return TF_OK;
 }
 
+tf_status tf_set_graphs(tf_ctx* c, int32_t on) {
+    if (!c) { g_err = "null argument"; return TF_ERR_ARG; }
+    c->use_graphs = on != 0;
+    return TF_OK;
+}
+
 tf_status tf_set_tuning(tf_ctx* c, const int32_t* blocks, const int32_t* unroll, int32_t n) {
     if (!c || !blocks || !unroll || n < 0 || n > TK_COUNT) { g_err = "bad argument"; return TF_ERR_ARG; }
+    drop_graph(c);
     for (int k = 0; k < n; ++k) {
         if (blocks[k] < 0 || (unroll[k] != 4 && unroll[k] != 8 && unroll[k] != 16)) {
             g_err = "tuning: blocks >= 0 (0 = default), unroll 4, 8 or 16";
@@ -2268,6 +2362,7 @@ tf_status tf_batch_set_sources(tf_ctx* c, int32_t inst, const tf_source* src, in
         return TF_ERR_ARG;
     }
     return guarded([&] {
+        drop_graph(c);
         auto& b = *c->batch;
         const int32_t E = b.eoff[inst + 1] - b.eoff[inst];
         for (int64_t i = 0; i < n_terms; ++i)

@@ -142,6 +142,10 @@ class Simulation:
         names = ["sell_ct_kernel", "sell_h_kernel", "sell_n_kernel", "cg_w_kernel", "cg_pg_kernel", "cg_wu_kernel"]
         return {names[k]: (bl[k], un[k]) for k in range(min(n.value, len(names)))}
 
+    def set_graphs(self, on):
+        """tf_set_graphs: step fdtd / dga in HIP graphs when nothing is sampled (default on)."""
+        _check(hip_lib().tf_set_graphs(self._h, int(bool(on))))
+
     def set_tuning(self, choice):
         """tf_set_tuning from {kernel: (blocks, unroll)} (kernels as in autotune())."""
         names = ["sell_ct_kernel", "sell_h_kernel", "sell_n_kernel", "cg_w_kernel", "cg_pg_kernel", "cg_wu_kernel"]
@@ -191,6 +195,10 @@ class BatchSimulation:
 
     def __del__(self):
         self.close()
+
+    def set_graphs(self, on):
+        """tf_set_graphs: step the batch in HIP graphs when nothing is sampled (default on)."""
+        _check(hip_lib().tf_set_graphs(self._h, int(bool(on))))
 
     def run(self, n, first=None):
         first = self.next_step if first is None else first
