@@ -1,0 +1,70 @@
+"""Device path vs the CPU checker for the source kinds beyond the wave-profile
+h source of the other parity tests: imposed-field e sources (FIT special
+edges, D:1850-1895; dga/gfdtd class-4 / Dirichlet edges, D:821-868,
+D:1490-1510), j current dipoles on FIT (D:447-492), several sources in one
+simulation, and the gaussian, ricker and cos carriers (Source.hpp:313-567).
+Same bars as test_gpu_parity: fp64 per probe and component within 1e-10 of
+the component's (or field's) peak, final fields within 1e-10 relative L2."""
+import re
+
+import pytest
+
+from tetfit_amd import Model, configs
+
+from test_gpu_parity import check_run
+
+pytestmark = pytest.mark.gpu
+
+
+def edit(path, fn):
+    s = open(path).read()
+    open(path, "w").write(fn(s))
+    return path
+
+
+def set_source1(s, **kv):
+    """Replace / add SET lines of source 1."""
+    a = s.index("DEFINE source 1")
+    b = s.index("END source 1")
+    block = s[a:b]
+    for k, v in kv.items():
+        if re.search(r"SET %s " % k, block):
+            block = re.sub(r"SET %s [^\n]*" % k, "SET %s %s" % (k, v), block)
+        else:
+            block = block + "    SET %s %s\n" % (k, v)
+    return s[:a] + block + s[b:]
+
+
+def add_source(s, label, lines):
+    block = "DEFINE source %d\n%sEND source %d\n" % (label, "".join("    SET %s\n" % l for l in lines), label)
+    s = s.replace("DEFINE output 1", block + "DEFINE output 1", 1)
+    return s.replace("DEFINE simulation 1\n", "DEFINE simulation 1\n    SET source %d\n" % label, 1)
+
+
+def test_fit_e_source_gaussian(workdir):
+    p = configs.fit_cavity(str(workdir), n=12, step=0.1, n_steps=150, n_probes=4, name="fit_esrc")
+    f = 2e8
+    edit(p, lambda s: set_source1(s, type="e", carrier="gaussian", width=repr(1.0 / f), delay=repr(2.0 / f)))
+    check_run(Model(p).build(1))
+
+
+def test_fit_h_plus_j_dipole_ricker(workdir):
+    p = configs.fit_cavity(str(workdir), n=12, step=0.1, n_steps=150, n_probes=4, name="fit_jsrc")
+    f = 3e8
+    edit(p, lambda s: add_source(s, 2, ["type j", "location {0.55,0.65,0.45}", "direction z", "carrier ricker",
+                                        "frequency %r" % f, "delay %r" % (1.5 / f), "amplitude 2",
+                                        "impedance 3"]))
+    check_run(Model(p).build(1))
+
+
+def test_dga_e_source_cos(workdir):
+    p = configs.kuhn_cavity(str(workdir), n=5, method="dga", n_steps=150, n_probes=4, block=2, name="dga_esrc")
+    edit(p, lambda s: set_source1(s, type="e", carrier="cos"))
+    check_run(Model(p).build(1))
+
+
+def test_gfdtd_e_source(workdir):
+    p = configs.kuhn_cavity(str(workdir), n=5, method="gfdtd", n_steps=50, n_probes=4, block=2, name="gf_esrc")
+    edit(p, lambda s: set_source1(s, type="e"))
+    st, r = check_run(Model(p).build(1))
+    assert st["cg_not_converged"] == 0
