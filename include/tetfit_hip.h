@@ -183,7 +183,9 @@ tf_status tf_get_tuning(tf_ctx* ctx, int32_t* blocks, int32_t* unroll, int32_t c
 tf_status tf_set_tuning(tf_ctx* ctx, const int32_t* blocks, const int32_t* unroll, int32_t n);
 
 /* Storage of one device operator, for byte accounting: which = 0 Ct (dga),
- * 1 H, 2 N, 3 A (gfdtd free rows), 4 E, 5 RHSmat1, 6 Ct (gfdtd free rows);
+ * 1 H, 2 N, 3 A (gfdtd free rows), 4 K = E_f - [A 0] (gfdtd free rows: the
+ * couplings left once the warm start cancels, used for the initial residual),
+ * 5 RHSmat1, 6 Ct (gfdtd free rows);
  * info[0..4] = rows, 64-row slices, non-zeros, stored entries (with
  * padding), bytes per stored column index (2: per-slice base + 16-bit
  * offset, 4: int32). No reference counterpart. */

@@ -491,7 +491,7 @@ struct Cg {
 };
 
 // ---------------------------------------------------------------- gfdtd
-// D:1360-1565 for sigma = 0 and no j sources
+// D:1360-1565 for sigma = 0
 void run_gfdtd(const tf_problem* p, const SourceTable& st, const tf_probe_geom* pg, int32_t np, Printer pr,
                int64_t first, int64_t n, double tol, int32_t maxiter, const double* U0, const double* F0,
                const double* B0, oracle_out* out) {
@@ -512,6 +512,9 @@ void run_gfdtd(const tf_problem* p, const SourceTable& st, const tf_probe_geom* 
             const tf_source_edge& s = st.se[k];
             if (s.field == TF_FIELD_H) {
                 if (h_too) Fb[s.edge] = h_source(st, k, t - 0.5 * dt);
+            } else if (s.field == TF_FIELD_J) {
+                if (h_too) I[s.edge] = j_source(st, k, t);  // D:1491-1492
+                SrcFld[s.edge] = U[s.edge];
             } else if (s.field == TF_FIELD_E) {
                 U[s.edge] = e_source(st, k, t);
                 SrcFld[s.edge] = U[s.edge];
@@ -541,7 +544,7 @@ void run_gfdtd(const tf_problem* p, const SourceTable& st, const tf_probe_geom* 
         t0 = now_s();
         Fb_old = Fb;
         bc_pass(t, true);
-        // Psi = E U_old - RHSmat1 SrcFld + dt C^T F + dt Ctb Fb  (D:1516)
+        // Psi = E U_old - RHSmat1 SrcFld - dt I + dt C^T F + dt Ctb Fb  (D:1516)
         csr_mv(d.E, U_old, eu);
         csr_mv(d.RHS1, SrcFld, r1);
         csr_mv(d.Ct, F, ctf);
@@ -550,7 +553,7 @@ void run_gfdtd(const tf_problem* p, const SourceTable& st, const tf_probe_geom* 
         sink.poy(poy);
         for (int32_t k = 0; k < nf; ++k) {
             const int32_t e = d.free_edges[k];
-            rhs[k] = (eu[e] - r1[e]) + dt * ctf[e] + dt * Fb[e];
+            rhs[k] = ((eu[e] - r1[e]) - dt * I[e]) + dt * ctf[e] + dt * Fb[e];
         }
         solution = cg.solve(rhs, solution);
         out->cg_iters_total += cg.iter;

@@ -1,7 +1,7 @@
 """Device path vs the CPU checker for the source kinds beyond the wave-profile
 h source of the other parity tests: imposed-field e sources (FIT special
 edges, D:1850-1895; dga/gfdtd class-4 / Dirichlet edges, D:821-868,
-D:1490-1510), j current dipoles on FIT (D:447-492), several sources in one
+D:1490-1510), j current dipoles on FIT (D:447-492) and gfdtd (D:429-462), several sources in one
 simulation, and the gaussian, ricker and cos carriers (Source.hpp:313-567).
 Same bars as test_gpu_parity: fp64 per probe and component within 1e-10 of
 the component's (or field's) peak, final fields within 1e-10 relative L2."""
@@ -66,5 +66,16 @@ def test_dga_e_source_cos(workdir):
 def test_gfdtd_e_source(workdir):
     p = configs.kuhn_cavity(str(workdir), n=5, method="gfdtd", n_steps=50, n_probes=4, block=2, name="gf_esrc")
     edit(p, lambda s: set_source1(s, type="e"))
+    st, r = check_run(Model(p).build(1))
+    assert st["cg_not_converged"] == 0
+
+
+def test_gfdtd_h_plus_j_dipole(workdir):
+    """gfdtd with the wave-profile h source and a j dipole (-dt I in Psi,
+    D:1491-1516): the initial residual carries the current on a free edge."""
+    p = configs.kuhn_cavity(str(workdir), n=5, method="gfdtd", n_steps=50, n_probes=4, block=2, name="gf_jsrc")
+    f = 3e8
+    edit(p, lambda s: add_source(s, 2, ["type j", "location {0.43,0.57,0.61}", "direction z", "carrier ricker",
+                                        "frequency %r" % f, "delay %r" % (1.5 / f), "amplitude 2", "impedance 3"]))
     st, r = check_run(Model(p).build(1))
     assert st["cg_not_converged"] == 0

@@ -254,3 +254,52 @@ def test_dga_eps_mass_patch_test(kuhn):
         expect = 4 * (E0 @ E0) * np.sum(epst * V)
         assert abs(U @ (M @ U) - expect) <= 1e-12 * expect
     assert np.linalg.eigvalsh(M.toarray()).min() > 0
+
+
+def _with_j(path, loc, method):
+    s = open(path).read().replace("method gfdtd", "method " + method)
+    block = ("DEFINE source 2\n    SET type j\n    SET location {%r,%r,%r}\n    SET direction z\n"
+             "    SET carrier sin\n    SET frequency 3e8\nEND source 2\n" % tuple(float(x) for x in loc))
+    s = s.replace("DEFINE output 1", block + "DEFINE output 1", 1)
+    s = s.replace("DEFINE simulation 1\n", "DEFINE simulation 1\n    SET source 2\n", 1)
+    out = path.replace(".fdtd", "_j_%s.fdtd" % method)
+    open(out, "w").write(s)
+    return out
+
+
+def test_gfdtd_j_dipole_edge(tmp_path):
+    """D:429-462: a j dipole sits on the edge nearest to its location (by
+    barycentre) among the edges around the first node of the tet that holds
+    it. Found here by brute force over all tets (barycentric coordinates) and
+    all edges of that node."""
+    loc = np.array([0.43, 0.57, 0.61])
+    p = configs.kuhn_cavity(str(tmp_path), n=5, method="gfdtd", n_steps=10, n_probes=2, block=2, name="j")
+    b = Model(_with_j(p, loc, "gfdtd")).build(1)
+    tt = b.tet_tables()
+    pts, vols, en = tt["pts"], tt["vols"], tt["edge_nodes"]
+    found = None
+    for t, v in enumerate(vols):
+        r = pts[v]
+        lam = np.linalg.solve(np.vstack([r.T, np.ones(4)]), np.append(loc, 1.0))
+        if lam.min() >= -1e-12:
+            found = t
+            break
+    node = vols[found][0]
+    cand = np.where((en[:, 0] == node) | (en[:, 1] == node))[0]
+    d = np.linalg.norm(0.5 * (pts[en[cand, 0]] + pts[en[cand, 1]]) - loc, axis=1)
+    want = cand[np.argmin(d)]
+    src, terms = b.sources()
+    assert src[1]["field"] == 2  # TF_FIELD_J
+    jt = terms[terms["source"] == 1]
+    assert len(jt) == 1 and jt["edge"][0] == want
+    ev = pts[en[want, 1]] - pts[en[want, 0]]
+    # direction z, unit amplitude and profile: the weight is the z share of the unit edge vector
+    assert abs(jt["weight"][0] - ev[2] / np.linalg.norm(ev)) < 1e-12
+
+
+def test_dga_j_dipole_rejected(tmp_path):
+    """j dipoles in dga read an uninitialised value in the reference
+    (D:835-865); the build refuses them."""
+    p = configs.kuhn_cavity(str(tmp_path), n=3, method="gfdtd", n_steps=10, n_probes=2, block=2, name="jd")
+    with pytest.raises(Exception, match="j dipole"):
+        Model(_with_j(p, [0.4, 0.5, 0.6], "dga")).build(1)

@@ -254,9 +254,10 @@ __global__ void gf_scatter_kernel(int32_t n_free, const int32_t* __restrict__ fr
     }
 }
 
-// gfdtd sources at step i: fb (h, edge-indexed scratch) and dir_new (e); the
-// Poynting term U_old . 0.5 Ctb (Fb + Fb_old) (D:1518) with fb_old from the
-// previous step's carriers.
+// gfdtd sources at step i (D:1478-1502): fb, the edge-indexed forcing
+// Ctb Fb - I of Psi (h sources at t - dt/2: Fb; j dipoles at t: -I), and
+// dir_new (e sources); the Poynting term U_old . 0.5 Ctb (Fb + Fb_old)
+// (D:1518) with fb_old from the previous step's carriers.
 template <typename T>
 __global__ void __launch_bounds__(kBlock) gf_inject_kernel(
     int32_t n_inj, const int32_t* __restrict__ inj_edge, const int8_t* __restrict__ inj_kind,
@@ -275,6 +276,8 @@ __global__ void __launch_bounds__(kBlock) gf_inject_kernel(
             fb[e] = (T)v;
             const double vo = carr_prev ? source_value(inj_tb[i], inj_te[i], term_src, term_w, carr_prev) : 0.0;
             poy = (double)U[e] * (0.5 * (v + vo));
+        } else if (inj_kind[i] == 2) {
+            fb[e] = (T)(-v);  // current I at t (D:1491-1492): -dt I joins dt Ctb Fb in Psi
         } else {
             dir_new[e] = (T)v;
         }

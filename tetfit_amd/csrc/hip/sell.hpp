@@ -290,30 +290,25 @@ __device__ __forceinline__ T cg_r_next(T r, T s, double alpha) {
     return (T)__builtin_fma(-alpha, (double)s, (double)r);
 }
 
-// r = b - A x (r[i * r_stride]), partial r.r; with jacobi also u = pre r
-// and partial r.u.
-template <typename T, int NB>
-__global__ void __launch_bounds__(kBlock) cg_init_kernel(SellView<T> m, const T* __restrict__ pre,
-                                                         const T* __restrict__ b, const T* __restrict__ x,
-                                                         T* __restrict__ r, int r_stride, T* __restrict__ u,
-                                                         double* __restrict__ part_rr, double* __restrict__ part_gam,
-                                                         CgState* st, int jacobi) {
+// Restart of a solve from its saved initial residual: r = r0 (r[i * r_stride]),
+// partial r.r; with jacobi also u = pre r and partial r.u. The solver's x
+// must hold the same warm start as when r0 was formed.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) cg_restart_kernel(int32_t n, const T* __restrict__ pre,
+                                                            const T* __restrict__ r0, T* __restrict__ r,
+                                                            int r_stride, T* __restrict__ u,
+                                                            double* __restrict__ part_rr,
+                                                            double* __restrict__ part_gam, CgState* st, int jacobi) {
     __shared__ double red[kRed2];
-    const int lane = lane_id();
     double rr = 0, ru = 0;
-    TF_FOR_SLICES(m) {
-        const int s = sr_.s;
-        const double ax = sell_dot<T, NB>(m, s, lane, [&](int32_t j) { return x[j]; });
-        const int32_t i = s * kSlice + lane;
-        if (i < m.nrows) {
-            const T ri = (T)((double)b[i] - ax);
-            r[(int64_t)i * r_stride] = ri;
-            rr += (double)ri * (double)ri;
-            if (jacobi) {
-                const T ui = (T)((double)pre[i] * (double)ri);
-                u[i] = ui;
-                ru += (double)ri * (double)ui;
-            }
+    for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const T ri = r0[i];
+        r[(int64_t)i * r_stride] = ri;
+        rr += (double)ri * (double)ri;
+        if (jacobi) {
+            const T ui = (T)((double)pre[i] * (double)ri);
+            u[i] = ui;
+            ru += (double)ri * (double)ui;
         }
     }
     block_sum2_all(rr, ru, red);
@@ -476,8 +471,7 @@ __global__ void __launch_bounds__(kBlock) cg_pg_kernel(int32_t it, SellView<uint
 // s_it into rs_cur[i].s), partial w.u (delta_it).
 template <typename T, int NB>
 __global__ void __launch_bounds__(kBlock) cg_wu_kernel(int32_t it, SellView<T> m, const T* __restrict__ u,
-                                                       T* __restrict__ p, const RS<T>* __restrict__ rs_prev,
-                                                       RS<T>* __restrict__ rs_cur,
+                                                       T* __restrict__ p, RS<T>* __restrict__ rs_cur,
                                                        const double* __restrict__ part_rr,
                                                        const double* __restrict__ part_gam, int32_t nb_pg,
                                                        double* __restrict__ part_del, CgState* st) {
@@ -504,13 +498,13 @@ __global__ void __launch_bounds__(kBlock) cg_wu_kernel(int32_t it, SellView<T> m
         if (i < m.nrows) {
             const T wi = (T)wv;
             const T ui = u[i];
-            const T ri = rs_cur[i].r;
+            const RS<T> o = rs_cur[i];  // (r', s of the previous pass), as cg_pg wrote it
             if (it == 0) {
                 p[i] = ui;
-                rs_cur[i] = RS<T>{ri, wi};
+                rs_cur[i] = RS<T>{o.r, wi};
             } else {
                 p[i] = (T)((double)ui + beta * (double)p[i]);
-                rs_cur[i] = RS<T>{ri, (T)((double)wi + beta * (double)rs_prev[i].s)};
+                rs_cur[i] = RS<T>{o.r, (T)((double)wi + beta * (double)o.s)};
             }
             wu += (double)wi * (double)ui;
         }
@@ -519,35 +513,43 @@ __global__ void __launch_bounds__(kBlock) cg_wu_kernel(int32_t it, SellView<T> m
     if (threadIdx.x == 0) part_del[blockIdx.x] = wu;
 }
 
-// rhs[k] = E U_old - RHS1 SrcFld + dt (Ct F + fb) at free edge e = free[k]
-// (D:1516-1523; sigma and I are zero on the supported inputs), fused with
-// the start of the CG solve (cg_init_kernel): r = rhs - A x, partial r.r,
-// and with jacobi u = pre r and partial r.u. guard as in cg_guard_ok. E, RHSmat1, Ct and A are
-// stored as SELL-64 over the free rows in the CG's row order; SrcFld is U
-// on Dirichlet edges at the new time, held in `dir_new` (edge-indexed).
+// Initial residual of the step's solve at free edge e = free[k] (D:1516-1536:
+// Psi = E U_old - RHS1 SrcFld + dt (Ct F + Ctb Fb - I), rhs = Psi on free rows,
+// r0 = rhs - A x with the warm start x = U_old on free edges). Since
+// E_f U_old - A x = (E_f - [A 0]) U_old =: K U_old exactly, the residual is
+// formed without the cancelling products E_f U_old and A x:
+//   r0 = K U_old - RHS1 SrcFld + dt (Ct F + fb),  fb = Ctb Fb - I (edge-indexed)
+// K (free rows, full edge columns) holds E's couplings to Dirichlet edges and
+// whatever E_ff - A leaves (zero for the supported lossless media), so many
+// of its slices are empty and skipped, as are RHS1's; with dir_terms = 0 (the
+// host knows U_old and SrcFld to be zero on every Dirichlet edge) both
+// products are skipped. r0 is kept in `r0save`
+// for a restart; with jacobi also u = pre r and partial r.u. guard as in
+// cg_guard_ok. SrcFld is U on Dirichlet edges at the new time (`dir_new`).
 template <typename T, int NB>
 __global__ void __launch_bounds__(kBlock) gf_rhs_init_kernel(
-    SellView<T> e, SellView<T> r1, SellView<int8_t> ct, SellView<T> a, const int32_t* __restrict__ free_edge,
+    SellView<T> kop, SellView<T> r1, SellView<int8_t> ct, int dir_terms, const int32_t* __restrict__ free_edge,
     const T* __restrict__ U, const T* __restrict__ dir_new, const T* __restrict__ Fv, const T* __restrict__ fb,
-    T* __restrict__ rhs, double dt, const T* __restrict__ pre, const T* __restrict__ x, T* __restrict__ r,
-    int r_stride, T* __restrict__ u, double* __restrict__ part_rr, double* __restrict__ part_gam, CgState* st,
-    int jacobi, const CgState* guard) {
+    T* __restrict__ r0save, double dt, const T* __restrict__ pre, T* __restrict__ r, int r_stride,
+    T* __restrict__ u, double* __restrict__ part_rr, double* __restrict__ part_gam, CgState* st, int jacobi,
+    const CgState* guard) {
     if (!cg_guard_ok(guard)) return;
     __shared__ double red[kRed2];
     const int lane = lane_id();
     double rr = 0, ru = 0;
-    TF_FOR_SLICES(e) {
+    TF_FOR_SLICES(ct) {
         const int s = sr_.s;
-        const double eu = sell_dot<T, 8>(e, s, lane, [&](int32_t j) { return U[j]; });
-        const bool has_r = r1.sptr[s + 1] > r1.sptr[s];
-        const double rs = has_r ? sell_dot<T, 8>(r1, s, lane, [&](int32_t j) { return dir_new[j]; }) : 0.0;
-        const double cf = sell_dot<T, 8>(ct, s, lane, [&](int32_t j) { return Fv[j]; });
-        const double ax = sell_dot<T, NB>(a, s, lane, [&](int32_t j) { return x[j]; });
+        const double ku = dir_terms && kop.sptr[s + 1] > kop.sptr[s]
+                              ? sell_dot<T, 8>(kop, s, lane, [&](int32_t j) { return U[j]; })
+                              : 0.0;
+        const double rs = dir_terms && r1.sptr[s + 1] > r1.sptr[s]
+                              ? sell_dot<T, 8>(r1, s, lane, [&](int32_t j) { return dir_new[j]; })
+                              : 0.0;
+        const double cf = sell_dot<T, NB>(ct, s, lane, [&](int32_t j) { return Fv[j]; });
         const int32_t k = s * kSlice + lane;
-        if (k < e.nrows) {
-            const T bk = (T)((eu - rs) + dt * cf + dt * (double)fb[free_edge[k]]);
-            rhs[k] = bk;
-            const T rk = (T)((double)bk - ax);
+        if (k < ct.nrows) {
+            const T rk = (T)((ku - rs) + dt * cf + dt * (double)fb[free_edge[k]]);
+            r0save[k] = rk;
             r[(int64_t)k * r_stride] = rk;
             rr += (double)rk * (double)rk;
             if (jacobi) {

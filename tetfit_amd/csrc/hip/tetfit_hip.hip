@@ -34,7 +34,7 @@ thread_local std::string g_err;
 // Grid caps of the grid-stride kernels: the CG partial buffers hold one
 // fp64 per block of the producing kernel.
 constexpr int kCgPartA = 2048;   // blocks of cg_w (SpMV with A)
-constexpr int kCgPartB = 1024;   // blocks of cg_init / cg_upd
+constexpr int kCgPartB = 1024;   // blocks of gf_rhs_init / cg_restart / cg_upd
 constexpr int kCgPartC = 2048;   // blocks of cg_pg (SpMV with the learned G)
 static_assert(kCgPartA <= kMaxPart && kCgPartB <= kMaxPart && kCgPartC <= kMaxPart,
               "CG kernels read at most kMaxPart partials per array");
@@ -161,14 +161,19 @@ struct tf_ctx {
 
     // DGA / gfdtd
     Csr32 Ct;
-    Sell sCt, H, N, A, Ef, R1f, Ctf;
+    Sell sCt, H, N, A, Kf, R1f, Ctf;  // Kf: E_f - [A 0] of the gfdtd residual (gf_rhs_init_kernel)
     Sell Efull;  // eps mass E of the energy diagnostic (all edges), tet methods
     bool c_fixed_signs = false;
     DevBuf c_col, c_sg, h_edge, curl_f, curl_u;
-    DevBuf free_edge, dir_edge, dir_new, fb, pre, x, r, z, p0, p1, q, rhs, cg_part, cg_state;
+    DevBuf free_edge, dir_edge, dir_new, fb, pre, x, r, z, p0, p1, q, r0, cg_part, cg_state;
     DevBuf rs0, rs1;  // (r, s) pairs of the learned passes, even / odd pass
     std::vector<int32_t> h_perm;  // SELL row r of H holds row h_perm[r] of the problem's H
     int32_t n_free = 0, n_dir = 0;
+    // gfdtd: every Dirichlet value of every step is zero (pec values, no e
+    // sources), and U is zero on the Dirichlet edges now; both together make
+    // the Dirichlet terms of the initial residual vanish
+    bool dir_zero_static = false, ud_zero = true;
+    bool k_dir_only = true;  // K has entries in Dirichlet columns only
     std::vector<int32_t> h_dir_edge;
     std::vector<double> h_fixed_value;
     DevBuf inj_edge, inj_kind, inj_tb, inj_te;
@@ -262,12 +267,12 @@ size_t tf_ctx::device_bytes() const {
     const DevBuf* all[] = {&U, &Fv, &B, &fit_ep, &fit_nu, &fe_sg, &fe_ca, &fe_cb, &fe_f4, &fh_sg, &fh_cm, &fh_cn, &fh_e4,
                            &sp_edge, &sp_f4, &sp_sg, &sp_ca, &sp_cb, &sp_kind, &sp_tb, &sp_te, &c_col, &c_sg,
                            &h_edge, &curl_f, &curl_u, &free_edge, &dir_edge, &dir_new, &fb, &pre, &x, &r, &rs0, &rs1, &z,
-                           &p0, &p1, &q, &rhs, &cg_part, &cg_state, &inj_edge, &inj_kind, &inj_tb, &inj_te,
+                           &p0, &p1, &q, &r0, &cg_part, &cg_state, &inj_edge, &inj_kind, &inj_tb, &inj_te,
                            &term_src, &term_w, &carr, &probes, &samples, &poy, &row_dt_h, &row_dt_f};
     for (auto b : all) s += b->bytes;
     const Csr32* cs[] = {&Ct};
     for (auto c : cs) s += c->ptr.bytes + c->col.bytes + c->val.bytes + c->sg.bytes;
-    const Sell* ss[] = {&sCt, &H, &N, &A, &Ef, &R1f, &Ctf, &Efull};
+    const Sell* ss[] = {&sCt, &H, &N, &A, &Kf, &R1f, &Ctf, &Efull};
     for (auto c : ss) s += c->sptr.bytes + c->col.bytes + c->val.bytes + c->sg.bytes + c->col16.bytes + c->cbase.bytes;
     for (auto b : lp.bufs()) s += b->bytes;
     return s;
@@ -547,6 +552,55 @@ void create_fit(tf_ctx* c, const tf_problem* p) {
     upload(c->fh_e4, e4);
 }
 
+// K = E_f - [A 0]: row k is E's row of free edge fe[k] minus A's row
+// perm[k] placed at the edges of its (free) columns, in fp64, exact zeros
+// dropped. Then E_f U - A x = K U whenever x = U on the free edges, which
+// gives the step's initial residual without the cancelling products E_f U
+// and A x (gf_rhs_init_kernel). Entries keep E's row order, A-only columns
+// follow.
+void residual_operator(const tf_dga_ops& d, const std::vector<int32_t>& perm, const std::vector<int32_t>& fe,
+                       CsrBuf& o) {
+    o.rp.assign(1, 0);
+    o.col.clear();
+    o.val.clear();
+    o.sg.clear();
+    std::vector<std::pair<int32_t, double>> arow;
+    std::vector<uint8_t> used;
+    for (size_t k = 0; k < fe.size(); ++k) {
+        const int32_t e = fe[k], a = perm[k];
+        arow.clear();
+        for (int64_t q = d.A.rowptr[a]; q < d.A.rowptr[a + 1]; ++q)
+            arow.push_back({d.free_edges[d.A.col[q]], d.A.val[q]});
+        used.assign(arow.size(), 0);
+        for (int64_t q = d.E.rowptr[e]; q < d.E.rowptr[e + 1]; ++q) {
+            double v = d.E.val[q];
+            for (size_t t = 0; t < arow.size(); ++t)
+                if (!used[t] && arow[t].first == d.E.col[q]) {
+                    v -= arow[t].second;
+                    used[t] = 1;
+                    break;
+                }
+            if (v != 0) {
+                o.col.push_back(d.E.col[q]);
+                o.val.push_back(v);
+            }
+        }
+        for (size_t t = 0; t < arow.size(); ++t)
+            if (!used[t] && arow[t].second != 0) {
+                o.col.push_back(arow[t].first);
+                o.val.push_back(-arow[t].second);
+            }
+        o.rp.push_back((int64_t)o.col.size());
+    }
+    o.v = tf_csr{};
+    o.v.nrows = (int32_t)fe.size();
+    o.v.ncols = d.E.ncols;
+    o.v.nnz = o.rp.back();
+    o.v.rowptr = o.rp.data();
+    o.v.col = o.col.data();
+    o.v.val = o.val.data();
+}
+
 template <class T>
 void create_tet(tf_ctx* c, const tf_problem* p) {
     const tf_dga_ops& d = p->dga;
@@ -584,9 +638,9 @@ void create_tet(tf_ctx* c, const tf_problem* p) {
         upload(c->h_edge, he);
     } else {
         // The CG runs over the free rows in SELL-C-sigma order of A: row k of
-        // every free-row operator (A, and the free rows of E, RHSmat1 and Ct
-        // for the right-hand side) is free edge free_edges[perm[k]], and A's
-        // columns are relabeled to match, so all CG vectors use that order.
+        // every free-row operator (A, and K, RHSmat1 and Ct for the initial
+        // residual) is free edge free_edges[perm[k]], and A's columns are
+        // relabeled to match, so all CG vectors use that order.
         if (d.A.nrows != d.n_free) throw std::runtime_error("A must have one row per free edge");
         const std::vector<int32_t> perm = sigma_order(d.A);
         std::vector<int32_t> inv(d.n_free);
@@ -602,8 +656,13 @@ void create_tet(tf_ctx* c, const tf_problem* p) {
         for (int32_t k = 0; k < d.n_free; ++k) fe[k] = d.free_edges[perm[k]];
         {
             CsrBuf m;
-            select_rows(d.E, fe.data(), d.n_free, nullptr, m);
-            upload_sell<T>(c->Ef, m.v, true);
+            residual_operator(d, perm, fe, m);
+            upload_sell<T>(c->Kf, m.v, true);
+            std::vector<uint8_t> free_col(E, 0);
+            for (int32_t e : fe) free_col[e] = 1;
+            c->k_dir_only = true;
+            for (int32_t col : m.col)
+                if (free_col[col]) c->k_dir_only = false;
             select_rows(d.RHS1, fe.data(), d.n_free, nullptr, m);
             upload_sell<T>(c->R1f, m.v, true);
             select_rows(d.Ct, fe.data(), d.n_free, nullptr, m);
@@ -630,7 +689,7 @@ void create_tet(tf_ctx* c, const tf_problem* p) {
                 if (a.col[q] == i && a.val[q] != 0) pre[i] = (T)(1.0 / a.val[q]);
         upload(c->pre, pre);
         const int64_t n = std::max<int32_t>(d.n_free, 1);
-        for (DevBuf* b : {&c->x, &c->r, &c->z, &c->p0, &c->p1, &c->q, &c->rhs}) zero_field<T>(*b, n, c->stream);
+        for (DevBuf* b : {&c->x, &c->r, &c->z, &c->p0, &c->p1, &c->q, &c->r0}) zero_field<T>(*b, n, c->stream);
         for (DevBuf* b : {&c->rs0, &c->rs1}) zero_field<RS<T>>(*b, n, c->stream);
         c->cg_part.alloc(sizeof(double) * (kCgPartA + 3 * kCgPartB + 2 * kCgPartC));
         // two CG states (device and pinned host copies): the solve of step k
@@ -719,13 +778,17 @@ void build_tet_inject(tf_ctx* c, const std::vector<std::vector<std::pair<int32_t
     for (int32_t e = 0; e < c->E; ++e) {
         if (by_edge_terms[e].empty()) continue;
         edge.push_back(e);
-        kind.push_back(edge_field[e] == TF_FIELD_H ? 0 : 1);
-        if (edge_field[e] == TF_FIELD_J) throw std::runtime_error("j sources are not supported for dga / gfdtd");
+        kind.push_back(edge_field[e] == TF_FIELD_H ? 0 : edge_field[e] == TF_FIELD_J ? 2 : 1);
+        if (edge_field[e] == TF_FIELD_J && c->method != TF_METHOD_GFDTD)
+            throw std::runtime_error("j sources are not supported for dga");
         tb.push_back((int32_t)tsrc.size());
         for (auto& pr : by_edge_terms[e]) { tsrc.push_back(pr.first); tw.push_back(pr.second); }
         te.push_back((int32_t)tsrc.size());
     }
     c->n_inj = (int32_t)edge.size();
+    c->dir_zero_static = c->k_dir_only && std::find(kind.begin(), kind.end(), (int8_t)1) == kind.end();
+    for (int32_t e : c->h_dir_edge)
+        if (c->h_fixed_value[e] != 0) c->dir_zero_static = false;
     upload(c->inj_edge, edge);
     upload(c->inj_kind, kind);
     upload(c->inj_tb, tb);
@@ -1012,7 +1075,7 @@ void run_dga(tf_ctx* c, int64_t first, int64_t n, tf_stats* st) {
 }
 
 // ---- CG solves on the device (single-reduction PCG, see sell.hpp) --------
-// A solve is cg_init (or the fused gf_rhs_init of a step), then passes it =
+// A solve is gf_rhs_init (or cg_restart for a redo), then passes it =
 // 0, 1, ... of (cg_w, cg_upd) with Jacobi, or (cg_pg, cg_wu) with the
 // learned preconditioner. Passes are launched in batches sized from the
 // previous solve's count; kernels past the stop test return at once.
@@ -1028,6 +1091,7 @@ struct CgSolve {
     bool learned = false;
     const std::vector<std::pair<int, int>>* snaps = nullptr;
     int slot = 0;  // CG state slot (device cg_state[slot], pinned h_state[slot])
+    bool dir_terms = true;  // gf_rhs_init: apply K U_old - RHS1 SrcFld (false: known to be zero)
     int it = 0;    // next pass
     std::vector<std::array<int64_t, 3>> tags;  // profiled launches: (record, pass, first kernel of a learned pass)
 };
@@ -1066,8 +1130,9 @@ struct CgView {
 };
 
 // Copies the solve's initial state to its slot and launches the start of the
-// solve: with_rhs, the step's right-hand side and r = b - A x in one pass
-// (gf_rhs_init_kernel, guarded); otherwise r = b - A x on the current rhs.
+// solve: with_rhs, the step's initial residual (gf_rhs_init_kernel, guarded,
+// which also keeps it in r0); otherwise a restart from r0 (cg_restart_kernel)
+// with x back at the step's warm start.
 template <class T>
 void cg_start(tf_ctx* c, CgSolve& cs, bool with_rhs, const CgState* guard) {
     const CgView<T> v(c, cs.learned);
@@ -1081,14 +1146,15 @@ void cg_start(tf_ctx* c, CgSolve& cs, bool with_rhs, const CgState* guard) {
     const int r0_stride = cs.learned ? 2 : 1;
     if (with_rhs) {
         ev_begin(c, "gf_rhs_init_kernel");
-        TF_LAUNCH_NB(8, gf_rhs_init_kernel, T, v.nb_b, c->stream, c->Ef.vals<T>(), c->R1f.vals<T>(), c->Ctf.signs(),
-                     v.A, c->free_edge.as<int32_t>(), c->U.as<T>(), c->dir_new.as<T>(), c->Fv.as<T>(),
-                     c->fb.as<T>(), c->rhs.as<T>(), c->dt, c->pre.as<T>(), v.x, r0, r0_stride, v.u, v.part_rr,
-                     v.part_gj[0], dst, cs.learned ? 0 : 1, guard);
+        TF_LAUNCH_NB(8, gf_rhs_init_kernel, T, v.nb_b, c->stream, c->Kf.vals<T>(), c->R1f.vals<T>(), c->Ctf.signs(),
+                     cs.dir_terms ? 1 : 0, c->free_edge.as<int32_t>(), c->U.as<T>(), c->dir_new.as<T>(),
+                     c->Fv.as<T>(), c->fb.as<T>(), c->r0.as<T>(), c->dt, c->pre.as<T>(), r0, r0_stride, v.u,
+                     v.part_rr, v.part_gj[0], dst, cs.learned ? 0 : 1, guard);
     } else {
-        ev_begin(c, "cg_init_kernel");
-        TF_LAUNCH_NB(8, cg_init_kernel, T, v.nb_b, c->stream, v.A, c->pre.as<T>(), c->rhs.as<T>(), v.x, r0,
-                     r0_stride, v.u, v.part_rr, v.part_gj[0], dst, cs.learned ? 0 : 1);
+        ev_begin(c, "cg_restart_kernel");
+        cg_restart_kernel<T><<<v.nb_b, kBlock, 0, c->stream>>>(v.nf, c->pre.as<T>(), c->r0.as<T>(), r0, r0_stride,
+                                                               v.u, v.part_rr, v.part_gj[0], dst,
+                                                               cs.learned ? 0 : 1);
     }
     ev_end(c);
     cs.it = 0;
@@ -1126,7 +1192,7 @@ void cg_passes(tf_ctx* c, CgSolve& cs, int n) {
             tag(it, 1);
             ev_begin(c, "cg_wu_kernel");
             TF_LAUNCH_NB(c->tune[TK_WU].unroll, cg_wu_kernel, T, v.nb_w, c->stream, it, v.A, v.u, v.p,
-                         v.rs[(it + 1) & 1], v.rs[it & 1], v.part_rl, v.part_gl, v.nb_g, v.part_del, dst);
+                         v.rs[it & 1], v.part_rl, v.part_gl, v.nb_g, v.part_del, dst);
             ev_end(c);
             tag(it, 0);
             continue;  // at it == maxiter the stop test of cg_wu ends the solve
@@ -1170,8 +1236,9 @@ void cg_finish(tf_ctx* c, CgSolve& cs, bool synced) {
     (cs.learned ? c->cg_guess_pc : c->cg_guess) = h.iter;
 }
 
-// A whole solve on the current right-hand side (the Jacobi redo of a
-// learned solve that missed the tolerance).
+// A whole solve restarted from the step's initial residual r0 (the Jacobi
+// redo of a learned solve that missed the tolerance; x must be back at the
+// step's warm start).
 template <class T>
 void cg_solve(tf_ctx* c, CgSolve& cs) {
     cg_start<T>(c, cs, false, nullptr);
@@ -1508,6 +1575,11 @@ void run_gfdtd(tf_ctx* c, int64_t first, int64_t n, const tf_cg_opts* cg, tf_sta
     const int gb_n = tuned_blocks(c, TK_N, c->N, kSpmvCap);
     CgState* states = c->cg_state.as<CgState>();
 
+    // The Dirichlet terms of step k's residual vanish when every Dirichlet
+    // value is zero: always for steps after the first of the run (the
+    // previous scatter wrote the zero values), and for the first when U is
+    // zero there now.
+    auto dir_terms = [&](int64_t k) { return !(c->dir_zero_static && (k > 0 || c->ud_zero)); };
     // Kernels of step k up to the start of its solve (guard: the previous solve's state).
     auto pre = [&](int64_t k, GfPre& g, bool fresh, const CgState* guard) {
         const int64_t i = first + k;
@@ -1537,6 +1609,7 @@ void run_gfdtd(tf_ctx* c, int64_t first, int64_t n, const tf_cg_opts* cg, tf_sta
         s0.init = init;
         s0.learned = g.learned;
         s0.slot = (int)(k & 1);
+        s0.dir_terms = dir_terms(k);
         cg_start<T>(c, s0, true, guard);
     };
     auto scatter = [&](const CgState* guard) {
@@ -1575,6 +1648,7 @@ void run_gfdtd(tf_ctx* c, int64_t first, int64_t n, const tf_cg_opts* cg, tf_sta
         sol.snaps = snaps.empty() ? nullptr : &snaps;
         if (learned != cur.learned) {
             cur.learned = learned;
+            sol.dir_terms = dir_terms(k);
             cg_start<T>(c, sol, true, nullptr);
         } else {
             sol.init.need_conv = learned ? 1 : 0;
@@ -1645,6 +1719,7 @@ void run_gfdtd(tf_ctx* c, int64_t first, int64_t n, const tf_cg_opts* cg, tf_sta
         cur = nxt;
         if (c->profiling) ev_flush(c);
     }
+    if (n > 0) c->ud_zero = c->dir_zero_static;
     TF_HIP(hipGetLastError());
     std::vector<double> part((size_t)n * std::max(inj_blocks, 1), 0.0);
     if (inj_blocks)
@@ -1976,7 +2051,7 @@ void autotune(tf_ctx* c) {
             });
             c->tune[TK_WU] = tune_one(c, TK_WU, c->A, kCgPartA, [&](int b, int u) {
                 TF_LAUNCH_NB(u, cg_wu_kernel, T, b, s, 1, c->A.vals<T>(), c->z.as<T>(), c->p0.as<T>(),
-                             c->rs0.as<RS<T>>(), c->rs1.as<RS<T>>(), one, one, 1, part, st);
+                             c->rs1.as<RS<T>>(), one, one, 1, part, st);
             });
         }
         TF_HIP(hipStreamSynchronize(s));
@@ -2164,6 +2239,23 @@ tf_status tf_write_state(tf_ctx* c, const double* U, const double* F, const doub
         wr(c->U, U, c->E);
         wr(c->Fv, F, c->F);
         wr(c->B, B, c->F);
+        if (U) {
+            c->ud_zero = true;
+            for (int32_t e : c->h_dir_edge)
+                if (U[e] != 0) c->ud_zero = false;
+        }
+        // gfdtd: the next solve's warm start is the written field on the free
+        // edges, as after every step (the solution vector is U's free part)
+        if (U && c->method == TF_METHOD_GFDTD && c->n_free > 0) {
+            if (c->prec == TF_PREC_FP64)
+                gather_free_kernel<double><<<blocks_for(c->n_free), kBlock, 0, c->stream>>>(
+                    c->n_free, c->free_edge.as<int32_t>(), c->U.as<double>(), c->x.as<double>());
+            else
+                gather_free_kernel<float><<<blocks_for(c->n_free), kBlock, 0, c->stream>>>(
+                    c->n_free, c->free_edge.as<int32_t>(), c->U.as<float>(), c->x.as<float>());
+            TF_HIP(hipGetLastError());
+            TF_HIP(hipStreamSynchronize(c->stream));
+        }
     });
 }
 
@@ -2315,7 +2407,7 @@ tf_status tf_autotune(tf_ctx* c) {
 
 tf_status tf_sell_info(tf_ctx* c, int32_t which, int64_t* info) {
     if (!c || !info || which < 0 || which > 6) { g_err = "bad argument"; return TF_ERR_ARG; }
-    const Sell* m[7] = {&c->sCt, &c->H, &c->N, &c->A, &c->Ef, &c->R1f, &c->Ctf};
+    const Sell* m[7] = {&c->sCt, &c->H, &c->N, &c->A, &c->Kf, &c->R1f, &c->Ctf};
     const Sell& s = *m[which];
     info[0] = s.nrows;
     info[1] = s.nslices;

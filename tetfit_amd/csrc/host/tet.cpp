@@ -960,8 +960,27 @@ void build_tet(const Model& m, const SimulationDef& sim, Build& b) {
             b.probe_geom.push_back(g);
             b.probe_points.push_back(pv.x); b.probe_points.push_back(pv.y); b.probe_points.push_back(pv.z);
         }
-        for (uint32_t sl : sim.sources)
-            if (sources[sl].st == "j") throw Error("j dipole sources in dga/gfdtd are not supported by this build (D:835-865 reads an uninitialised value)");
+        // j dipoles (D:429-462, placed only with probe output as in the
+        // reference): the edge nearest to the location, by barycentre, among
+        // the edges around the first node of the tet that holds it, in
+        // increasing edge id (the first of equal distances wins)
+        for (uint32_t sl : sim.sources) {
+            const SourceDef& s = sources[sl];
+            if (s.st != "j") continue;
+            if (is_dga)
+                throw Error("j dipole sources in dga are not supported by this build (D:835-865 reads an uninitialised value)");
+            const Vec3 loc(s.location[0], s.location[1], s.location[2]);
+            const uint32_t t = find_probe(loc);
+            if (t >= T) continue;  // out of the mesh: the reference skips it with a warning
+            const uint32_t node = (uint32_t)b.vols[(size_t)t * 4];
+            int64_t best = -1;
+            double bd = 0;
+            for (uint32_t q = nte_ptr[node]; q < nte_ptr[node + 1]; ++q) {
+                const double dq = norm(edge_bars[nte[q]] - loc);
+                if (best < 0 || dq < bd) { bd = dq; best = nte[q]; }
+            }
+            if (best >= 0) edge_src[best].push_back(sl);
+        }
     }
 
     // ---- sources on edges: h (bnd dual vector), e (edge vector)

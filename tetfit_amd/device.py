@@ -159,8 +159,9 @@ class Simulation:
 
     def sell_info(self, which):
         """tf_sell_info: storage of a device operator ("ct", "h", "n", "a",
-        "e", "rhs1", "ctf") as dict(rows, slices, nnz, entries, col_bytes)."""
-        k = ["ct", "h", "n", "a", "e", "rhs1", "ctf"].index(which)
+        "k" (gfdtd residual operator E_f - [A 0]), "rhs1", "ctf") as
+        dict(rows, slices, nnz, entries, col_bytes)."""
+        k = ["ct", "h", "n", "a", "k", "rhs1", "ctf"].index(which)
         info = (C.c_int64 * 5)()
         _check(hip_lib().tf_sell_info(self._h, k, info))
         return dict(rows=info[0], slices=info[1], nnz=info[2], entries=info[3], col_bytes=info[4])
