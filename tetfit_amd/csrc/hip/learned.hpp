@@ -112,15 +112,20 @@ __device__ __forceinline__ uint16_t float_to_bf16(float f) {
 // Ĝ (training, scaled, fp32) and G (CG, unscaled, bf16) values at every
 // SELL position. The two positions of a pair get the same value, so G stays
 // exactly symmetric after rounding.
+// With c16 (A's 16-bit column offsets) the bf16 value goes to gpk packed
+// as (c16 << 16 | bf16), otherwise to gval.
 __global__ void expand_kernel(int64_t entries, const int32_t* __restrict__ pos2pair, const float* __restrict__ base,
                               const float* __restrict__ scale, const float* __restrict__ y, float* __restrict__ ghat,
-                              uint16_t* __restrict__ gval) {
+                              uint16_t* __restrict__ gval, const uint16_t* __restrict__ c16,
+                              uint32_t* __restrict__ gpk) {
     const int64_t pos = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (pos >= entries) return;
     const int32_t p = pos2pair[pos];
     const float g = p >= 0 ? base[p] + y[p] : 0.f;
     if (ghat) ghat[pos] = g;
-    if (gval) gval[pos] = float_to_bf16(p >= 0 ? scale[p] * g : 0.f);
+    const uint16_t b = float_to_bf16(p >= 0 ? scale[p] * g : 0.f);
+    if (gval) gval[pos] = b;
+    if (gpk) gpk[pos] = ((uint32_t)c16[pos] << 16) | b;
 }
 
 // Bank column `slot` <- r, x (the CG iterate being captured).

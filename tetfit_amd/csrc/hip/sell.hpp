@@ -320,10 +320,14 @@ __global__ void __launch_bounds__(kBlock) cg_restart_kernel(int32_t n, const T* 
 }
 
 // w = A u, partial w.u; with test_stop the pass starts with the stop test on
-// the partials part_rr of the previous update.
+// the partials part_rr of the previous update. With `early` the product of
+// the wave's first slice is issued before the partial sums are reduced (it
+// does not depend on them), which hides that round trip; a pass that then
+// stops has computed that slice in vain, so the host sets `early` only for
+// passes that are not expected to stop.
 template <typename T, int NB>
 __global__ void __launch_bounds__(kBlock) cg_w_kernel(int32_t it, SellView<T> m, const T* __restrict__ u,
-                                                      T* __restrict__ w, int test_stop,
+                                                      T* __restrict__ w, int test_stop, int early,
                                                       const double* __restrict__ part_rr, int32_t nb_rr,
                                                       double* __restrict__ part_del, CgState* st) {
     __shared__ double red[kRed2];
@@ -332,23 +336,30 @@ __global__ void __launch_bounds__(kBlock) cg_w_kernel(int32_t it, SellView<T> m,
     const double rr0 = st->rr0, tol = st->tol;
     const int32_t maxiter = st->maxiter;
     if (done) return;
+    const int lane = lane_id();
+    SliceRange sr = block_slices(m.nslices);
+    auto gat = [&](int32_t j) { return u[j]; };
+    const bool first = early && sr.s < sr.end;
+    const double w0 = first ? sell_dot<T, NB>(m, sr.s, lane, gat) : 0.0;
     if (test_stop) {
         double RR = thread_sum(prr), unused = 0;
         block_sum2_all(RR, unused, red);
         if (cg_decide_stop(it, RR, rr0, tol, maxiter, st)) return;
     }
-    const int lane = lane_id();
     double wu = 0, none = 0;
-    TF_FOR_SLICES(m) {
-        const int s = sr_.s;
-        const double wv = sell_dot<T, NB>(m, s, lane, [&](int32_t j) { return u[j]; });
-        const int32_t i = s * kSlice + lane;
+    auto row = [&](int sl, double wv) {
+        const int32_t i = sl * kSlice + lane;
         if (i < m.nrows) {
             const T wi = (T)wv;
             w[i] = wi;
             wu += (double)wi * (double)u[i];
         }
+    };
+    if (first) {
+        row(sr.s, w0);
+        sr.s += sr.step;
     }
+    for (; sr.s < sr.end; sr.s += sr.step) row(sr.s, sell_dot<T, NB>(m, sr.s, lane, gat));
     block_sum2_all(wu, none, red);
     if (threadIdx.x == 0) part_del[blockIdx.x] = wu;
 }
@@ -369,6 +380,12 @@ __global__ void __launch_bounds__(kBlock) cg_upd_kernel(int32_t it, int32_t n, c
     const int done = st->done;
     const double gam_prev = st->gam[(it - 1) & 1], alpha_prev = st->alpha[(it - 1) & 1];
     if (done) return;
+    // the thread's first element is loaded before the partial sums are reduced
+    const int32_t i0 = blockIdx.x * blockDim.x + threadIdx.x;
+    T u0 = 0, w0 = 0, p0 = 0, s0 = 0, x0 = 0, r0 = 0, q0 = 0;
+    if (i0 < n) {
+        u0 = u[i0]; w0 = w[i0]; p0 = p[i0]; s0 = s[i0]; x0 = x[i0]; r0 = r[i0]; q0 = pre[i0];
+    }
     double gam = thread_sum(pg), del = thread_sum(pd);
     block_sum2_all(gam, del, red);
     double alpha, beta;
@@ -384,16 +401,19 @@ __global__ void __launch_bounds__(kBlock) cg_upd_kernel(int32_t it, int32_t n, c
         st->alpha[it & 1] = alpha;
     }
     double rr = 0, ru = 0;
-    for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const T pi = it == 0 ? u[i] : (T)((double)u[i] + beta * (double)p[i]);
-        const T si = it == 0 ? w[i] : (T)((double)w[i] + beta * (double)s[i]);
+    for (int32_t i = i0; i < n; i += gridDim.x * blockDim.x) {
+        if (i != i0) {
+            u0 = u[i]; w0 = w[i]; p0 = p[i]; s0 = s[i]; x0 = x[i]; r0 = r[i]; q0 = pre[i];
+        }
+        const T pi = it == 0 ? u0 : (T)((double)u0 + beta * (double)p0);
+        const T si = it == 0 ? w0 : (T)((double)w0 + beta * (double)s0);
         p[i] = pi;
         s[i] = si;
-        x[i] = (T)((double)x[i] + alpha * (double)pi);
-        const T ri = (T)((double)r[i] - alpha * (double)si);
+        x[i] = (T)((double)x0 + alpha * (double)pi);
+        const T ri = (T)((double)r0 - alpha * (double)si);
         r[i] = ri;
         rr += (double)ri * (double)ri;
-        const T ui = (T)((double)pre[i] * (double)ri);
+        const T ui = (T)((double)q0 * (double)ri);
         u[i] = ui;
         ru += (double)ri * (double)ui;
     }
@@ -404,15 +424,78 @@ __global__ void __launch_bounds__(kBlock) cg_upd_kernel(int32_t it, int32_t n, c
     }
 }
 
+// Both products of one row over the gathered (r, s) pairs: gr = (G r)_i and
+// gs = (G s)_i, so that u = G (r - alpha s) = gr - alpha gs needs alpha only
+// after the gathers. G is stored packed, 32 bits per entry (16-bit column
+// offset << 16 | bf16 value), when A has 16-bit columns (pk), otherwise as
+// bf16 values beside A's column indices (m.val).
+template <typename T, int U>
+__device__ __forceinline__ void sell_dot_rs(const SellView<uint16_t>& m, const uint32_t* __restrict__ pk, int s,
+                                            int lane, const RS<T>* __restrict__ rs, T& gr, T& gs) {
+    s = __builtin_amdgcn_readfirstlane(s);
+    const int64_t base = m.sptr[s];
+    const int w = __builtin_amdgcn_readfirstlane((int)((m.sptr[s + 1] - base) / kSlice));
+    T ar = 0, as = 0;
+    auto batch = [&](int k0, auto entry, bool full) {
+        int32_t ci[U];
+        T vi[U];
+        RS<T> gi[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) entry((full ? k0 + u : min(k0 + u, w - 1)) * kSlice + lane, ci[u], vi[u]);
+#pragma unroll
+        for (int u = 0; u < U; ++u) gi[u] = rs[ci[u]];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (full || k0 + u < w) {
+                ar += vi[u] * gi[u].r;
+                as += vi[u] * gi[u].s;
+            }
+    };
+    auto rows = [&](auto entry) {
+        int k0 = 0;
+        for (; k0 + U <= w; k0 += U) batch(k0, entry, true);
+        if (k0 < w) batch(k0, entry, false);
+    };
+    if (pk) {
+        const uint32_t* q = pk + base;
+        const int32_t cb = __builtin_amdgcn_readfirstlane(m.cbase[s]);
+        rows([&](int o, int32_t& c, T& v) {
+            const uint32_t e = q[o];
+            c = cb + (int32_t)(e >> 16);
+            v = (T)bf16_to_float((uint16_t)(e & 0xFFFFu));
+        });
+    } else if (m.col16) {
+        const uint16_t* c16 = m.col16 + base;
+        const uint16_t* vv = m.val + base;
+        const int32_t cb = __builtin_amdgcn_readfirstlane(m.cbase[s]);
+        rows([&](int o, int32_t& c, T& v) {
+            c = cb + (int32_t)c16[o];
+            v = (T)bf16_to_float(vv[o]);
+        });
+    } else {
+        const int32_t* c32 = m.col + base;
+        const uint16_t* vv = m.val + base;
+        rows([&](int o, int32_t& c, T& v) {
+            c = c32[o];
+            v = (T)bf16_to_float(vv[o]);
+        });
+    }
+    gr = ar;
+    gs = as;
+}
+
 // Learned pass `it`, first kernel. For it >= 1: alpha_{it-1} from gamma and
 // beta of pass it-1 (state slot (it-1)&1), the partials of delta_{it-1}
 // (part_del) and alpha_{it-2}; then on the owned rows x += alpha p and
 // r_it = r_{it-1} - alpha s_{it-1} (rs_in holds (r_{it-1}, s_{it-1}); r_it
-// goes to rs_out[i].r), and u = G r_it where the gathered r_it[j] is formed
-// from rs_in[j] with the same rounding as the owner's; partials r.u and r.r.
-// For it = 0, u = G r with r = rs_in[].r. G: bf16 values on A's pattern.
+// goes to rs_out[i].r), and u = G r_it = G r_{it-1} - alpha G s_{it-1};
+// partials r.u and r.r. For it = 0, u = G r with r = rs_in[].r. The
+// products of the wave's first slice do not need alpha and are issued
+// before the partial sums are reduced. G: bf16 values on A's pattern
+// (sell_dot_rs).
 template <typename T, int NB>
 __global__ void __launch_bounds__(kBlock) cg_pg_kernel(int32_t it, SellView<uint16_t> g,
+                                                       const uint32_t* __restrict__ gpk,
                                                        const RS<T>* __restrict__ rs_in, RS<T>* __restrict__ rs_out,
                                                        const T* __restrict__ p, T* __restrict__ x, T* __restrict__ u,
                                                        const double* __restrict__ part_del, int32_t nb_del,
@@ -424,6 +507,11 @@ __global__ void __launch_bounds__(kBlock) cg_pg_kernel(int32_t it, SellView<uint
     const int done = st->done;
     const double gam = st->gam[k], beta = st->beta[k], alpha_pp = st->alpha[k ^ 1];
     if (done) return;
+    const int lane = lane_id();
+    SliceRange sr = block_slices(g.nslices);
+    const bool first = sr.s < sr.end;
+    T gr0 = 0, gs0 = 0;
+    if (first) sell_dot_rs<T, NB>(g, gpk, sr.s, lane, rs_in, gr0, gs0);
     double alpha = 0;
     if (it > 0) {
         double del = thread_sum(pd), unused = 0;
@@ -431,32 +519,31 @@ __global__ void __launch_bounds__(kBlock) cg_pg_kernel(int32_t it, SellView<uint
         alpha = it == 1 ? gam / del : gam / (del - beta * gam / alpha_pp);
         if (threadIdx.x == 0 && blockIdx.x == 0) st->alpha[k] = alpha;
     }
-    const int lane = lane_id();
     double ru = 0, rr = 0;
-    TF_FOR_SLICES(g) {
-        const int sl = sr_.s;
-        double uv;
-        if (it > 0)
-            uv = sell_dot<T, NB>(g, sl, lane, [&](int32_t j) {
-                const RS<T> v = rs_in[j];
-                return cg_r_next(v.r, v.s, alpha);
-            });
-        else
-            uv = sell_dot<T, NB>(g, sl, lane, [&](int32_t j) { return rs_in[j].r; });
+    auto row = [&](int sl, T gr, T gs) {
         const int32_t i = sl * kSlice + lane;
         if (i < g.nrows) {
             const RS<T> o = rs_in[i];
-            T ri = o.r;
+            T ri = o.r, ui = gr;
             if (it > 0) {
                 ri = cg_r_next(o.r, o.s, alpha);
                 rs_out[i] = RS<T>{ri, o.s};  // whole pair: s is replaced by cg_wu of this pass
                 x[i] = (T)((double)x[i] + alpha * (double)p[i]);
+                ui = (T)((double)gr - alpha * (double)gs);
             }
-            const T ui = (T)uv;
             u[i] = ui;
             ru += (double)ri * (double)ui;
             rr += (double)ri * (double)ri;
         }
+    };
+    if (first) {
+        row(sr.s, gr0, gs0);
+        sr.s += sr.step;
+    }
+    for (; sr.s < sr.end; sr.s += sr.step) {
+        T gr, gs;
+        sell_dot_rs<T, NB>(g, gpk, sr.s, lane, rs_in, gr, gs);
+        row(sr.s, gr, gs);
     }
     block_sum2_all(ru, rr, red);
     if (threadIdx.x == 0) {
@@ -467,11 +554,12 @@ __global__ void __launch_bounds__(kBlock) cg_pg_kernel(int32_t it, SellView<uint
 
 // Learned pass `it`, second kernel: stop test on r_it (part_rr), gamma_it
 // and beta_it (stored in slot it&1), w = A u, p = u + beta p and
-// s_it = w + beta s_{it-1} on the owned rows (s_{it-1} from rs_prev[i].s,
-// s_it into rs_cur[i].s), partial w.u (delta_it).
+// s_it = w + beta s_{it-1} on the owned rows (rs_cur[i] holds (r_it,
+// s_{it-1}) as cg_pg wrote it; s_it replaces its s), partial w.u (delta_it).
+// `early` as in cg_w_kernel.
 template <typename T, int NB>
 __global__ void __launch_bounds__(kBlock) cg_wu_kernel(int32_t it, SellView<T> m, const T* __restrict__ u,
-                                                       T* __restrict__ p, RS<T>* __restrict__ rs_cur,
+                                                       T* __restrict__ p, RS<T>* __restrict__ rs_cur, int early,
                                                        const double* __restrict__ part_rr,
                                                        const double* __restrict__ part_gam, int32_t nb_pg,
                                                        double* __restrict__ part_del, CgState* st) {
@@ -481,6 +569,11 @@ __global__ void __launch_bounds__(kBlock) cg_wu_kernel(int32_t it, SellView<T> m
     const double rr0 = st->rr0, tol = st->tol, gam_prev = st->gam[(it - 1) & 1];
     const int32_t maxiter = st->maxiter;
     if (done) return;
+    const int lane = lane_id();
+    SliceRange sr = block_slices(m.nslices);
+    auto gat = [&](int32_t j) { return u[j]; };
+    const bool first = early && sr.s < sr.end;
+    const double w0 = first ? sell_dot<T, NB>(m, sr.s, lane, gat) : 0.0;
     double RR = thread_sum(prr), gam = thread_sum(pga);
     block_sum2_all(RR, gam, red);
     if (cg_decide_stop(it, RR, rr0, tol, maxiter, st)) return;
@@ -489,16 +582,13 @@ __global__ void __launch_bounds__(kBlock) cg_wu_kernel(int32_t it, SellView<T> m
         st->gam[it & 1] = gam;
         st->beta[it & 1] = beta;
     }
-    const int lane = lane_id();
     double wu = 0, none = 0;
-    TF_FOR_SLICES(m) {
-        const int sl = sr_.s;
-        const double wv = sell_dot<T, NB>(m, sl, lane, [&](int32_t j) { return u[j]; });
+    auto row = [&](int sl, double wv) {
         const int32_t i = sl * kSlice + lane;
         if (i < m.nrows) {
             const T wi = (T)wv;
             const T ui = u[i];
-            const RS<T> o = rs_cur[i];  // (r', s of the previous pass), as cg_pg wrote it
+            const RS<T> o = rs_cur[i];
             if (it == 0) {
                 p[i] = ui;
                 rs_cur[i] = RS<T>{o.r, wi};
@@ -508,7 +598,12 @@ __global__ void __launch_bounds__(kBlock) cg_wu_kernel(int32_t it, SellView<T> m
             }
             wu += (double)wi * (double)ui;
         }
+    };
+    if (first) {
+        row(sr.s, w0);
+        sr.s += sr.step;
     }
+    for (; sr.s < sr.end; sr.s += sr.step) row(sr.s, sell_dot<T, NB>(m, sr.s, lane, gat));
     block_sum2_all(wu, none, red);
     if (threadIdx.x == 0) part_del[blockIdx.x] = wu;
 }

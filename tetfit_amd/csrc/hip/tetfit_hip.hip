@@ -123,7 +123,7 @@ struct LearnedPc {
     int learn_steps = 200, train_iters = 150;
     double lr = 3e-3;
     int32_t n_pairs = 0, n_pad = 0;
-    DevBuf X, base, scale, pa, pb, pos2pair, y, ghat, gval, ahat, isd;
+    DevBuf X, base, scale, pa, pb, pos2pair, y, ghat, gval, gpk, ahat, isd;
     DevBuf P, Pbest, m, v, part, gy, gpos, lpart, cpart;
     DevBuf Rb, Cb, T1, T2;
     std::vector<int> slot;          // 0 free, 1 captured correction pair, 2 random probe
@@ -131,7 +131,7 @@ struct LearnedPc {
     double train_seconds = 0, loss0 = 0, loss = 0;
     int64_t fallbacks = 0, iters = 0, solves = 0;
     std::vector<const DevBuf*> bufs() const {
-        return {&X, &base, &scale, &pa, &pb, &pos2pair, &y, &ghat, &gval, &ahat, &isd, &P, &Pbest,
+        return {&X, &base, &scale, &pa, &pb, &pos2pair, &y, &ghat, &gval, &gpk, &ahat, &isd, &P, &Pbest,
                 &m, &v, &part, &gy, &gpos, &lpart, &cpart, &Rb, &Cb, &T1, &T2};
     }
 };
@@ -1105,6 +1105,7 @@ struct CgView {
     RS<T>* rs[2];
     SellView<T> A;
     SellView<uint16_t> G;
+    const uint32_t* gpk;  // G packed with A's 16-bit columns, or null (G = bf16 values beside A's columns)
     CgView(tf_ctx* c, bool learned) {
         nf = c->n_free;
         nb_w = tuned_blocks(c, learned ? TK_WU : TK_W, c->A, kCgPartA);
@@ -1126,6 +1127,7 @@ struct CgView {
         rs[1] = c->rs1.as<RS<T>>();
         A = c->A.vals<T>();
         G = c->A.view<uint16_t>(c->lp.gval);
+        gpk = c->lp.gpk.p ? c->lp.gpk.as<uint32_t>() : nullptr;
     }
 };
 
@@ -1182,24 +1184,28 @@ void cg_passes(tf_ctx* c, CgSolve& cs, int n) {
     };
     if (cs.it == 0) cg_capture<T>(c, cs, v, 0);
     const int stop = std::min(cs.it + n, cs.init.maxiter + 1);
+    // passes before the previous solve's count are not expected to stop:
+    // their SpMV starts ahead of the stop test (cg_w / cg_wu `early`)
+    const int guess = cs.learned ? c->cg_guess_pc : c->cg_guess;
+    auto early = [&](int it) { return it + 1 < guess; };
     for (int& it = cs.it; it < stop; ++it) {
         if (cs.learned) {
             ev_begin(c, "cg_pg_kernel");
-            TF_LAUNCH_NB(c->tune[TK_G].unroll, cg_pg_kernel, T, v.nb_g, c->stream, it, v.G,
+            TF_LAUNCH_NB(c->tune[TK_G].unroll, cg_pg_kernel, T, v.nb_g, c->stream, it, v.G, v.gpk,
                          v.rs[it == 0 ? 0 : (it - 1) & 1], v.rs[it & 1], v.p, v.x, v.u, v.part_del, v.nb_w,
                          v.part_gl, v.part_rl, dst);
             ev_end(c);
             tag(it, 1);
             ev_begin(c, "cg_wu_kernel");
             TF_LAUNCH_NB(c->tune[TK_WU].unroll, cg_wu_kernel, T, v.nb_w, c->stream, it, v.A, v.u, v.p,
-                         v.rs[it & 1], v.part_rl, v.part_gl, v.nb_g, v.part_del, dst);
+                         v.rs[it & 1], early(it) ? 1 : 0, v.part_rl, v.part_gl, v.nb_g, v.part_del, dst);
             ev_end(c);
             tag(it, 0);
             continue;  // at it == maxiter the stop test of cg_wu ends the solve
         }
         ev_begin(c, "cg_w_kernel");
-        TF_LAUNCH_NB(c->tune[TK_W].unroll, cg_w_kernel, T, v.nb_w, c->stream, it, v.A, v.u, v.w, 1, v.part_rr,
-                     v.nb_b, v.part_del, dst);
+        TF_LAUNCH_NB(c->tune[TK_W].unroll, cg_w_kernel, T, v.nb_w, c->stream, it, v.A, v.u, v.w, 1,
+                     early(it) ? 1 : 0, v.part_rr, v.nb_b, v.part_del, dst);
         ev_end(c);
         tag(it, 0);
         if (it == cs.init.maxiter) { ++it; break; }  // the pass at maxiter only runs the stop test
@@ -1413,7 +1419,10 @@ void learned_setup(tf_ctx* c, const tf_cg_opts* cg) {
     L.gy.alloc((size_t)L.n_pad * 4);
     TF_HIP(hipMemsetAsync(L.gy.p, 0, L.gy.bytes, c->stream));
     L.ghat.alloc(entries * 4);
-    L.gval.alloc(entries * 2);  // bf16 bits
+    // bf16 values: packed with A's 16-bit column offsets (cg_pg_kernel reads
+    // one word per entry) when A has them, else beside A's int32 columns
+    if (c->A.c16) L.gpk.alloc(entries * 4);
+    else L.gval.alloc(entries * 2);
     L.gpos.alloc(entries * 4);
     // parameters: He-uniform W1, W2 from a fixed LCG, zero biases, zero output layer
     std::vector<float> P(kMlpParams, 0.f);
@@ -1493,7 +1502,8 @@ void learned_train(tf_ctx* c) {
     for (int it = 0; it <= L.train_iters; ++it) {
         mlp_forward_kernel<<<fb, kBlock, 0, s>>>(tiles, L.P.as<float>(), L.X.as<short>(), L.y.as<float>());
         expand_kernel<<<xb, kBlock, 0, s>>>(entries, L.pos2pair.as<int32_t>(), L.base.as<float>(),
-                                            L.scale.as<float>(), L.y.as<float>(), L.ghat.as<float>(), nullptr);
+                                            L.scale.as<float>(), L.y.as<float>(), L.ghat.as<float>(), nullptr,
+                                            nullptr, nullptr);
         bank_spmm_kernel<<<rb, kBlock, 0, s>>>(n, sptr, col, L.ghat.as<float>(), L.Rb.as<float>(),
                                                L.Cb.as<float>(), L.T1.as<float>());
         bank_spmm_kernel<<<rb, kBlock, 0, s>>>(n, sptr, col, L.ahat.as<float>(), L.T1.as<float>(), nullptr,
@@ -1522,7 +1532,8 @@ void learned_train(tf_ctx* c) {
     // frozen G from the best parameters
     mlp_forward_kernel<<<fb, kBlock, 0, s>>>(tiles, L.Pbest.as<float>(), L.X.as<short>(), L.y.as<float>());
     expand_kernel<<<xb, kBlock, 0, s>>>(entries, L.pos2pair.as<int32_t>(), L.base.as<float>(), L.scale.as<float>(),
-                                        L.y.as<float>(), nullptr, L.gval.as<uint16_t>());
+                                        L.y.as<float>(), nullptr, c->A.c16 ? nullptr : L.gval.as<uint16_t>(),
+                                        c->A.col16.as<uint16_t>(), c->A.c16 ? L.gpk.as<uint32_t>() : nullptr);
     TF_HIP(hipGetLastError());
     TF_HIP(hipStreamSynchronize(s));
     L.train_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -2042,16 +2053,18 @@ void autotune(tf_ctx* c) {
         const double* one = ones.as<double>();
         CgState* st = scratch.as<CgState>();
         c->tune[TK_W] = tune_one(c, TK_W, c->A, kCgPartA, [&](int b, int u) {
-            TF_LAUNCH_NB(u, cg_w_kernel, T, b, s, 1, c->A.vals<T>(), c->z.as<T>(), c->q.as<T>(), 0, one, 1, part, st);
+            TF_LAUNCH_NB(u, cg_w_kernel, T, b, s, 1, c->A.vals<T>(), c->z.as<T>(), c->q.as<T>(), 0, 1, one, 1, part,
+                         st);
         });
         if (c->lp.state == LearnedPc::kActive) {
             c->tune[TK_G] = tune_one(c, TK_G, c->A, kCgPartC, [&](int b, int u) {
-                TF_LAUNCH_NB(u, cg_pg_kernel, T, b, s, 1, c->A.view<uint16_t>(c->lp.gval), c->rs0.as<RS<T>>(),
+                TF_LAUNCH_NB(u, cg_pg_kernel, T, b, s, 1, c->A.view<uint16_t>(c->lp.gval),
+                             c->lp.gpk.p ? c->lp.gpk.as<uint32_t>() : nullptr, c->rs0.as<RS<T>>(),
                              c->rs1.as<RS<T>>(), c->p0.as<T>(), c->q.as<T>(), c->z.as<T>(), one, 1, part, part, st);
             });
             c->tune[TK_WU] = tune_one(c, TK_WU, c->A, kCgPartA, [&](int b, int u) {
                 TF_LAUNCH_NB(u, cg_wu_kernel, T, b, s, 1, c->A.vals<T>(), c->z.as<T>(), c->p0.as<T>(),
-                             c->rs1.as<RS<T>>(), one, one, 1, part, st);
+                             c->rs1.as<RS<T>>(), 1, one, one, 1, part, st);
             });
         }
         TF_HIP(hipStreamSynchronize(s));
