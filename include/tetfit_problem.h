@@ -90,6 +90,9 @@ typedef struct {
     tf_csr         RHS1;          /* RHSmat1: n_edges x n_edges               */
     const int32_t* free_edges;    /* compressed_dirichlet                      */
     int32_t        n_free;
+    tf_csr         Sig;           /* SigMat: n_edges x n_edges conductivity    */
+                                  /* mass on E's pattern (gfdtd); nnz 0 when   */
+                                  /* no tet is lossy                           */
 } tf_dga_ops;
 
 typedef struct {

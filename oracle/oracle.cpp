@@ -491,7 +491,7 @@ struct Cg {
 };
 
 // ---------------------------------------------------------------- gfdtd
-// D:1360-1565 for sigma = 0
+// D:1360-1565
 void run_gfdtd(const tf_problem* p, const SourceTable& st, const tf_probe_geom* pg, int32_t np, Printer pr,
                int64_t first, int64_t n, double tol, int32_t maxiter, const double* U0, const double* F0,
                const double* B0, oracle_out* out) {
@@ -525,7 +525,7 @@ void run_gfdtd(const tf_problem* p, const SourceTable& st, const tf_probe_geom* 
     std::vector<double> U_old = U;
     Cg cg{nf, d.A.rowptr, d.A.col, d.A.val, {}, tol, maxiter};
     cg.compute();
-    std::vector<double> solution(nf, 0.0), rhs(nf, 0.0), cu, eu, r1, ctf;
+    std::vector<double> solution(nf, 0.0), rhs(nf, 0.0), cu, eu, r1, ctf, su;
     for (int32_t k = 0; k < nf; ++k) solution[k] = U[d.free_edges[k]];
     Sink sink{out, np};
     std::vector<double> six;
@@ -544,8 +544,12 @@ void run_gfdtd(const tf_problem* p, const SourceTable& st, const tf_probe_geom* 
         t0 = now_s();
         Fb_old = Fb;
         bc_pass(t, true);
-        // Psi = E U_old - RHSmat1 SrcFld - dt I + dt C^T F + dt Ctb Fb  (D:1516)
+        // Psi = E U_old - dt/2 SigMat U_old - RHSmat1 SrcFld - dt I + dt C^T F + dt Ctb Fb  (D:1516)
         csr_mv(d.E, U_old, eu);
+        if (d.Sig.nnz > 0) {
+            csr_mv(d.Sig, U_old, su);
+            for (int32_t e = 0; e < E; ++e) eu[e] -= (0.5 * dt) * su[e];  // - (dt/2) SigMat U_old
+        }
         csr_mv(d.RHS1, SrcFld, r1);
         csr_mv(d.Ct, F, ctf);
         double poy = 0;

@@ -79,3 +79,12 @@ def test_gfdtd_h_plus_j_dipole(workdir):
                                         "frequency %r" % f, "delay %r" % (1.5 / f), "amplitude 2", "impedance 3"]))
     st, r = check_run(Model(p).build(1))
     assert st["cg_not_converged"] == 0
+
+
+def test_gfdtd_lossy(workdir):
+    """gfdtd with conducting blocks (sigma 0.5 S/m on every other eps block):
+    A = E + dt/2 SigMat and Psi's - dt/2 SigMat U_old (D:1516, D:8646)."""
+    from test_tet_host import _lossy
+    p = configs.kuhn_cavity(str(workdir), n=5, method="gfdtd", n_steps=50, n_probes=4, block=2, name="gf_lossy")
+    st, r = check_run(Model(_lossy(p, 0.5, 2)).build(1))
+    assert st["cg_not_converged"] == 0

@@ -303,3 +303,65 @@ def test_dga_j_dipole_rejected(tmp_path):
     p = configs.kuhn_cavity(str(tmp_path), n=3, method="gfdtd", n_steps=10, n_probes=2, block=2, name="jd")
     with pytest.raises(Exception, match="j dipole"):
         Model(_with_j(p, [0.4, 0.5, 0.6], "dga")).build(1)
+
+
+def _lossy(path, sigma, every=1, eps=None):
+    """Adds `SET sigma` to every `every`-th material (and optionally one eps_r
+    for all) of a generated input."""
+    import re
+    s = open(path).read()
+    k = [0]
+
+    def mat(m):
+        k[0] += 1
+        body = m.group(0)
+        if eps is not None:
+            body = re.sub(r"SET epsilon [^\n]*", "SET epsilon %r" % eps, body)
+        if (k[0] - 1) % every == 0:
+            body = body.replace("    SET mu 1\n", "    SET mu 1\n    SET sigma %r\n" % sigma)
+        return body
+    s = re.sub(r"DEFINE material \d+\n(?:    SET [^\n]*\n)+", mat, s)
+    out = path.replace(".fdtd", "_lossy.fdtd")
+    open(out, "w").write(s)
+    return out
+
+
+def test_gfdtd_sigma_mass_is_eps_mass_scaled(tmp_path):
+    """D:8586-8612: SigMat collects the same tet-corner terms as E with sigma
+    in place of eps, so with one isotropic material everywhere SigMat =
+    (sigma / eps) E; and the system is SysMat = E + dt/2 SigMat split into A
+    (free x free) and RHSmat1 (free rows, Dirichlet columns), D:8646-8668."""
+    p = configs.kuhn_cavity(str(tmp_path), n=4, method="gfdtd", n_steps=10, n_probes=2, block=2, name="ls",
+                            dt=1e-11)
+    sig, eps_r = 0.05, 2.5
+    b = Model(_lossy(p, sig, 1, eps_r)).build(1)
+    o = b.dga_ops()
+    d = b.problem_ptr.contents.dga
+    from tetfit_amd.host import _csr
+    S, E = _csr(d.Sig), o["E"]
+    np.testing.assert_array_equal(S["rowptr"], E["rowptr"])
+    np.testing.assert_array_equal(S["col"], E["col"])
+    ratio = sig / (eps_r * 8.854187817e-12)
+    np.testing.assert_allclose(S["val"], ratio * E["val"], rtol=1e-12, atol=1e-12 * ratio * np.abs(E["val"]).max())
+    # A and RHSmat1 carry E + dt/2 SigMat
+    fe = o["free_edges"]
+    isf = np.zeros(b.n_edges, bool)
+    isf[fe] = True
+    fidx = -np.ones(b.n_edges, np.int64)
+    fidx[fe] = np.arange(len(fe))
+    A, R1 = o["A"], o["RHS1"]
+    h = 0.5 * b.dt
+    for r in fe[::7]:
+        q = slice(E["rowptr"][r], E["rowptr"][r + 1])
+        cols, ev, sv = E["col"][q], E["val"][q], S["val"][q]
+        ar = slice(A["rowptr"][fidx[r]], A["rowptr"][fidx[r] + 1])
+        np.testing.assert_array_equal(A["val"][ar], (ev + h * sv)[isf[cols]])
+        rr = slice(R1["rowptr"][r], R1["rowptr"][r + 1])
+        np.testing.assert_array_equal(R1["val"][rr], (ev + h * sv)[~isf[cols]])
+
+
+def test_gfdtd_lossless_has_no_sigma(tmp_path):
+    p = configs.kuhn_cavity(str(tmp_path), n=3, method="gfdtd", n_steps=10, n_probes=2, block=2, name="nl",
+                            dt=1e-11)
+    b = Model(p).build(1)
+    assert b.problem_ptr.contents.dga.Sig.nnz == 0

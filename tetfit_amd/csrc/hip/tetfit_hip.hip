@@ -552,14 +552,19 @@ void create_fit(tf_ctx* c, const tf_problem* p) {
     upload(c->fh_e4, e4);
 }
 
-// K = E_f - [A 0]: row k is E's row of free edge fe[k] minus A's row
+// K = (E - dt/2 SigMat)_f - [A 0]: row k is the row of free edge fe[k] of
+// E - dt/2 SigMat (the operator Psi applies to U_old, D:1516) minus A's row
 // perm[k] placed at the edges of its (free) columns, in fp64, exact zeros
-// dropped. Then E_f U - A x = K U whenever x = U on the free edges, which
-// gives the step's initial residual without the cancelling products E_f U
-// and A x (gf_rhs_init_kernel). Entries keep E's row order, A-only columns
-// follow.
-void residual_operator(const tf_dga_ops& d, const std::vector<int32_t>& perm, const std::vector<int32_t>& fe,
-                       CsrBuf& o) {
+// dropped. Then Psi's U_old terms minus A x are K U_old whenever x = U_old on
+// the free edges, which gives the step's initial residual without the
+// cancelling products (gf_rhs_init_kernel). With A = E_ff + dt/2 SigMat_ff,
+// K holds E's and SigMat's Dirichlet columns and -dt SigMat_ff. Entries keep
+// E's row order, A-only columns follow.
+void residual_operator(const tf_dga_ops& d, double dt, const std::vector<int32_t>& perm,
+                       const std::vector<int32_t>& fe, CsrBuf& o) {
+    const bool lossy = d.Sig.nnz > 0;
+    if (lossy && (d.Sig.nrows != d.E.nrows || d.Sig.nnz != d.E.nnz))
+        throw std::runtime_error("SigMat must have E's pattern");
     o.rp.assign(1, 0);
     o.col.clear();
     o.val.clear();
@@ -573,7 +578,7 @@ void residual_operator(const tf_dga_ops& d, const std::vector<int32_t>& perm, co
             arow.push_back({d.free_edges[d.A.col[q]], d.A.val[q]});
         used.assign(arow.size(), 0);
         for (int64_t q = d.E.rowptr[e]; q < d.E.rowptr[e + 1]; ++q) {
-            double v = d.E.val[q];
+            double v = lossy ? d.E.val[q] - (0.5 * dt) * d.Sig.val[q] : d.E.val[q];
             for (size_t t = 0; t < arow.size(); ++t)
                 if (!used[t] && arow[t].first == d.E.col[q]) {
                     v -= arow[t].second;
@@ -656,7 +661,7 @@ void create_tet(tf_ctx* c, const tf_problem* p) {
         for (int32_t k = 0; k < d.n_free; ++k) fe[k] = d.free_edges[perm[k]];
         {
             CsrBuf m;
-            residual_operator(d, perm, fe, m);
+            residual_operator(d, p->dt, perm, fe, m);
             upload_sell<T>(c->Kf, m.v, true);
             std::vector<uint8_t> free_col(E, 0);
             for (int32_t e : fe) free_col[e] = 1;

@@ -62,7 +62,7 @@ struct Build {
     double L[3] = {0, 0, 0};
 
     // ---- DGA tables
-    CsrStore C, Ct, H, N, E, A, RHS1;
+    CsrStore C, Ct, H, N, E, A, RHS1, Sig;
     std::vector<int32_t> bnd_edge, h_row, free_edges;
     std::vector<int8_t> edge_class;
     std::vector<double> fixed_value;

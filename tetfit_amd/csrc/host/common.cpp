@@ -237,6 +237,7 @@ void Build::finalize_views() {
         d.RHS1 = RHS1.view();
         d.free_edges = free_edges.empty() ? nullptr : free_edges.data();
         d.n_free = (int32_t)free_edges.size();
+        d.Sig = Sig.view();
     }
 }
 

@@ -733,6 +733,14 @@ void build_tet(const Model& m, const SimulationDef& sim, Build& b) {
         b.E = pattern_from_pairs((int32_t)E, (int32_t)E, pr, pc);
         std::vector<int32_t>().swap(pr);
         std::vector<int32_t>().swap(pc);
+        // SigMat: the same corner terms with sigma, from the tets whose sigma
+        // tensor is invertible (D:8600-8612); kept on E's pattern
+        bool lossy = false;
+        for (uint32_t t = 0; t < T && !lossy; ++t) lossy = materials[b.vol_material[t]].sigma.det() != 0;
+        if (lossy) {
+            b.Sig = b.E;
+            std::fill(b.Sig.val.begin(), b.Sig.val.end(), 0.0);
+        }
         for (uint32_t nid = 0; nid < np; ++nid) {
             for (uint32_t q = av_cnt[nid]; q < av_cnt[nid + 1]; ++q) {
                 const uint32_t t = av[q];
@@ -751,6 +759,12 @@ void build_tet(const Model& m, const SimulationDef& sim, Build& b) {
                 // push order of D:8420-8429: (1,1),(2,2),(3,3),(1,2),(2,1),(1,3),(3,1),(2,3),(3,2)
                 const int order[9][2] = {{0, 0}, {1, 1}, {2, 2}, {0, 1}, {1, 0}, {0, 2}, {2, 0}, {1, 2}, {2, 1}};
                 for (auto& o : order) csr_at(b.E, e[o[0]], e[o[1]]) += dot(f[o[0]], oe * f[o[1]]);
+                const Mat3& sig = materials[b.vol_material[t]].sigma;
+                if (lossy && sig.det() != 0) {
+                    Mat3 os;
+                    for (int z = 0; z < 9; ++z) os.m[z] = omega * sig.m[z];
+                    for (auto& o : order) csr_at(b.Sig, e[o[0]], e[o[1]]) += dot(f[o[0]], os * f[o[1]]);
+                }
             }
         }
         // free (non-Dirichlet) edges: compressed_dirichlet (D:8145-8171)
@@ -767,7 +781,8 @@ void build_tet(const Model& m, const SimulationDef& sim, Build& b) {
             }
         }
         const int32_t nfree = (int32_t)b.free_edges.size();
-        // A = E restricted to free edges; RHSmat1 = E(free rows, dirichlet cols)
+        // A = E restricted to free edges; RHSmat1 = E(free rows, dirichlet cols);
+        // with lossy media both take E + dt/2 SigMat once dt is known (below)
         b.A.nrows = b.A.ncols = nfree;
         b.A.rowptr.assign(nfree + 1, 0);
         b.RHS1.nrows = b.RHS1.ncols = (int32_t)E;
@@ -783,13 +798,6 @@ void build_tet(const Model& m, const SimulationDef& sim, Build& b) {
             }
             b.RHS1.rowptr[r + 1] = (int64_t)b.RHS1.col.size();
         }
-        // sigma (SigMat) is needed only for lossy media; not supported in gfdtd here
-        for (auto& kv : materials)
-            if (!kv.second.sigma.is_zero()) {
-                for (uint32_t t = 0; t < T; ++t)
-                    if ((uint32_t)b.vol_material[t] == kv.first)
-                        throw Error("method gfdtd with lossy media (SigMat) is not supported by this build");
-            }
         // ---- dt: generalized power iteration on (Ct N C, A) (ComputeFEMTimeStep D:6928)
         {
             std::vector<double> U(E), cu(F), Fw(F), cf(E), r(nfree), z(nfree), pp(nfree), qq(nfree), y(nfree);
@@ -872,6 +880,22 @@ void build_tet(const Model& m, const SimulationDef& sim, Build& b) {
         else t_step = sim_dt_request;
     }
     b.prob.dt = t_step;
+    if (!is_dga && !b.Sig.rowptr.empty()) {
+        // SysMat = E + (dt/2) SigMat split into A (free x free) and RHSmat1
+        // (free rows, Dirichlet columns), D:8646-8668; SigMat has E's pattern
+        const double h = 0.5 * t_step;
+        std::vector<uint8_t> is_free(E, 0);
+        for (int32_t e : b.free_edges) is_free[e] = 1;
+        size_t qa = 0, q1 = 0;
+        for (uint32_t r = 0; r < E; ++r) {
+            if (!is_free[r]) continue;
+            for (int64_t q = b.E.rowptr[r]; q < b.E.rowptr[r + 1]; ++q) {
+                const double v = b.E.val[q] + h * b.Sig.val[q];
+                if (is_free[b.E.col[q]]) b.A.val[qa++] = v;
+                else b.RHS1.val[q1++] = v;
+            }
+        }
+    }
 
     // ---- probes (FindProbe D:3645) with Whitney E and face H weights
     uint32_t root = 0;
