@@ -139,6 +139,15 @@ tf_status tf_read_probes(tf_ctx* ctx, double* times, double* vals,
 /* Per-step Poynting scalar (the first column of poynting.dat). */
 tf_status tf_read_poynting(tf_ctx* ctx, double* vals, int64_t cap, int64_t* n);
 
+/* The losses column of poynting.dat, one value per stepped step, as the
+ * reference writes it: gfdtd pushes 1/2 U_old.SigMat U_old before each solve
+ * and, from step 2 on, U.SigMat U after it (D:1519, D:1543-1545), and the
+ * file pairs the first n of those values with the n steps (D:1573-1576);
+ * dga and fdtd give 0 (dga runs lossless media only; the FIT Joule weights
+ * are filled in l2norm output mode only, D:5593-5597). Replaces the Losses
+ * vector of Discretization::RunSimulation. */
+tf_status tf_read_losses(tf_ctx* ctx, double* vals, int64_t cap, int64_t* n);
+
 /* Discrete energy of the current state, fp64 reductions: fdtd
  * We = 1/2 U.(ep_vec U), Wh = 1/2 B.(nu_vec B) (D:3468-3470); dga and
  * gfdtd We = 1/2 U.(E U), Wh = 1/2 B.(N B) with E the problem's eps mass

@@ -528,7 +528,19 @@ void run_gfdtd(const tf_problem* p, const SourceTable& st, const tf_probe_geom* 
     std::vector<double> solution(nf, 0.0), rhs(nf, 0.0), cu, eu, r1, ctf, su;
     for (int32_t k = 0; k < nf; ++k) solution[k] = U[d.free_edges[k]];
     Sink sink{out, np};
-    std::vector<double> six;
+    std::vector<double> six, sx;
+    // U.(SigMat U) and the Losses vector (zero when every tet is lossless)
+    auto quad_sig = [&](const std::vector<double>& x) {
+        if (d.Sig.nnz == 0) return 0.0;
+        csr_mv(d.Sig, x, sx);
+        double q = 0;
+        for (int32_t e = 0; e < E; ++e) q += x[e] * sx[e];
+        return q;
+    };
+    auto loss = [&](double v) {
+        if (out->losses && out->n_losses < out->cap_losses) out->losses[out->n_losses] = v;
+        out->n_losses++;
+    };
     double loop_s = 0;
     for (int64_t i = first; i < first + n; ++i) {
         double t0 = now_s();
@@ -555,6 +567,7 @@ void run_gfdtd(const tf_problem* p, const SourceTable& st, const tf_probe_geom* 
         double poy = 0;
         for (int32_t e = 0; e < E; ++e) poy += U_old[e] * (0.5 * (Fb[e] + Fb_old[e]));
         sink.poy(poy);
+        loss(0.5 * quad_sig(U_old));  // D:1519
         for (int32_t k = 0; k < nf; ++k) {
             const int32_t e = d.free_edges[k];
             rhs[k] = ((eu[e] - r1[e]) - dt * I[e]) + dt * ctf[e] + dt * Fb[e];
@@ -565,9 +578,11 @@ void run_gfdtd(const tf_problem* p, const SourceTable& st, const tf_probe_geom* 
         if (std::isfinite(cg.final_error) && cg.final_error > out->cg_max_relres) out->cg_max_relres = cg.final_error;
         if (cg.iter >= maxiter && cg.final_error > tol) out->cg_not_converged++;
         for (int32_t k = 0; k < nf; ++k) U[d.free_edges[k]] = solution[k];
+        if (i > 1) loss(quad_sig(U));  // Joule_L, D:1543-1545
         U_old = U;
         loop_s += now_s() - t0;
     }
+    if (first + n > 1) loss(quad_sig(U));  // after the loop, D:1566-1570
     out->seconds = loop_s;
     if (out->U) std::memcpy(out->U, U.data(), sizeof(double) * E);
     if (out->F) std::memcpy(out->F, F.data(), sizeof(double) * Fn);
@@ -606,6 +621,7 @@ int32_t oracle_run(const tf_problem* p, const tf_source_desc* sd, int32_t n_sd, 
         g_err.clear();
         out->n_samples = 0;
         out->n_poynting = 0;
+        out->n_losses = 0;
         out->cg_iters_total = 0;
         out->cg_iters_max = 0;
         out->cg_not_converged = 0;

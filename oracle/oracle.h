@@ -53,6 +53,10 @@ typedef struct {
     double cg_max_relres;
     /* steady-clock seconds of the step loop, probe export excluded (D:812/926) */
     double seconds;
+    /* gfdtd Losses vector in push order (D:1519, D:1543-1545, D:1566-1570) */
+    double* losses;
+    int64_t cap_losses;
+    int64_t n_losses;
 } oracle_out;
 
 /* Runs steps [first, first+n) of the problem's method. faithful=1 keeps the

@@ -20,7 +20,8 @@ class oracle_out(C.Structure):
                 ("poynting", C.POINTER(C.c_double)), ("cap_poynting", C.c_int64), ("n_poynting", C.c_int64),
                 ("U", C.POINTER(C.c_double)), ("F", C.POINTER(C.c_double)), ("B", C.POINTER(C.c_double)),
                 ("cg_iters_total", C.c_int64), ("cg_iters_max", C.c_int32), ("cg_not_converged", C.c_int32),
-                ("cg_max_relres", C.c_double), ("seconds", C.c_double)]
+                ("cg_max_relres", C.c_double), ("seconds", C.c_double),
+                ("losses", C.POINTER(C.c_double)), ("cap_losses", C.c_int64), ("n_losses", C.c_int64)]
 
 
 def lib():
@@ -57,7 +58,8 @@ def _dp(a):
 def run(build, first=None, n=None, faithful=True, cg_tol=None, cg_maxiter=100, U0=None, F0=None, B0=None,
         output=True, max_samples=None):
     """Runs the oracle on a tetfit_amd.host.Build; returns a dict with
-    times, samples [k, probe, 6], poynting, U, F, B and CG statistics."""
+    times, samples [k, probe, 6], poynting, losses (gfdtd's Losses vector in
+    push order), U, F, B and CG statistics."""
     from tetfit_amd import _ffi
     L = lib()
     first = build.first_step if first is None else first
@@ -76,12 +78,14 @@ def run(build, first=None, n=None, faithful=True, cg_tol=None, cg_maxiter=100, U
     times = np.zeros(cap)
     vals = np.zeros(cap * max(np_, 1) * 6)
     poy = np.zeros(n + 2)
+    loss = np.zeros(2 * n + 2)
     U = np.zeros(build.n_edges)
     F = np.zeros(build.n_faces)
     B = np.zeros(build.n_faces)
     o = oracle_out()
     o.times, o.vals, o.cap_samples = _dp(times), _dp(vals), cap
     o.poynting, o.cap_poynting = _dp(poy), len(poy)
+    o.losses, o.cap_losses = _dp(loss), len(loss)
     o.U, o.F, o.B = _dp(U), _dp(F), _dp(B)
     tol = build.cg_tol if cg_tol is None else cg_tol
     u0 = None if U0 is None else np.ascontiguousarray(U0, dtype=np.float64)
@@ -94,7 +98,7 @@ def run(build, first=None, n=None, faithful=True, cg_tol=None, cg_maxiter=100, U
         raise RuntimeError(L.oracle_last_error().decode())
     ns = min(o.n_samples, cap)
     return dict(times=times[:ns], samples=vals[:ns * np_ * 6].reshape(ns, np_, 6), n_samples=o.n_samples,
-                poynting=poy[:min(o.n_poynting, len(poy))], U=U, F=F, B=B,
+                poynting=poy[:min(o.n_poynting, len(poy))], losses=loss[:min(o.n_losses, len(loss))], U=U, F=F, B=B,
                 cg_iters_total=o.cg_iters_total, cg_iters_max=o.cg_iters_max,
                 cg_not_converged=o.cg_not_converged, cg_max_relres=o.cg_max_relres, seconds=o.seconds)
 

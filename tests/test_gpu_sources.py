@@ -88,3 +88,22 @@ def test_gfdtd_lossy(workdir):
     p = configs.kuhn_cavity(str(workdir), n=5, method="gfdtd", n_steps=50, n_probes=4, block=2, name="gf_lossy")
     st, r = check_run(Model(_lossy(p, 0.5, 2)).build(1))
     assert st["cg_not_converged"] == 0
+
+
+def test_gfdtd_lossy_losses_column(workdir):
+    """tf_read_losses against the checker's Losses vector: the first n
+    values, one per step, as poynting.dat pairs them (D:1573-1576)."""
+    import numpy as np
+    import pyoracle
+    from tetfit_amd.device import Simulation
+    from test_tet_host import _lossy
+    p = configs.kuhn_cavity(str(workdir), n=5, method="gfdtd", n_steps=40, n_probes=2, block=2, name="gf_loss")
+    b = Model(_lossy(p, 0.5, 2)).build(1)
+    sim = Simulation(b, "fp64")
+    sim.run(n=15)
+    sim.run()  # the column continues across runs
+    got = sim.losses()
+    sim.close()
+    ref = pyoracle.run(b)["losses"][:b.n_steps]
+    assert len(got) == b.n_steps and np.abs(ref).max() > 0
+    np.testing.assert_allclose(got, ref, rtol=0, atol=1e-10 * np.abs(ref).max())

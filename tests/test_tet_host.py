@@ -365,3 +365,27 @@ def test_gfdtd_lossless_has_no_sigma(tmp_path):
                             dt=1e-11)
     b = Model(p).build(1)
     assert b.problem_ptr.contents.dga.Sig.nnz == 0
+
+
+def test_oracle_gfdtd_losses_push_order(tmp_path):
+    """The checker's gfdtd Losses vector follows D:1519 / D:1543-1545 /
+    D:1566-1570: 1/2 U_old.SigMat U_old before each solve, U.SigMat U after
+    the solve from step 2 on, and U.SigMat U once more after the loop. Pinned
+    by quadratic forms computed here on the final fields of 1- and 2-step
+    runs from the same start."""
+    import pyoracle
+    from tetfit_amd.host import _csr
+    p = configs.kuhn_cavity(str(tmp_path), n=3, method="gfdtd", n_steps=4, n_probes=2, block=2, name="lo")
+    b = Model(_lossy(p, 0.5, 2)).build(1)
+    S = _csr(b.problem_ptr.contents.dga.Sig)
+    import scipy.sparse as sp
+    Sig = sp.csr_matrix((S["val"], S["col"], S["rowptr"]), shape=(b.n_edges, b.n_edges))
+    q = lambda u: float(u @ (Sig @ u))
+    rng = np.random.default_rng(3)
+    U0 = rng.standard_normal(b.n_edges)
+    U0[b.dga_ops()["edge_class"] == 4] = 0.0
+    r1 = pyoracle.run(b, n=1, U0=U0)
+    r2 = pyoracle.run(b, n=2, U0=U0)
+    assert len(r1["losses"]) == 2 and len(r2["losses"]) == 4
+    np.testing.assert_allclose(r1["losses"], [0.5 * q(U0), q(r1["U"])], rtol=1e-12)
+    np.testing.assert_allclose(r2["losses"], [0.5 * q(U0), 0.5 * q(r1["U"]), q(r2["U"]), q(r2["U"])], rtol=1e-12)

@@ -179,6 +179,7 @@ def hip_lib():
     L.tf_write_state.argtypes = [vp, f64p, f64p, f64p]
     L.tf_read_probes.argtypes = [vp, f64p, f64p, C.c_int64, i64p]
     L.tf_read_poynting.argtypes = [vp, f64p, C.c_int64, i64p]
+    L.tf_read_losses.argtypes = [vp, f64p, C.c_int64, i64p]
     L.tf_energy.argtypes = [vp, f64p, f64p]
     L.tf_kernel_times.argtypes = [vp, C.c_char_p, C.c_int32, f64p, i64p, C.c_int32, i32p]
     L.tf_set_profiling.argtypes = [vp, C.c_int32]
@@ -199,7 +200,7 @@ def hip_lib():
     L.tf_set_tuning.argtypes = [vp, i32p, i32p, C.c_int32]
     L.tf_set_graphs.argtypes = [vp, C.c_int32]
     for n in ["tf_create", "tf_set_sources", "tf_set_probes", "tf_set_output", "tf_run",
-              "tf_read_state", "tf_write_state", "tf_read_probes", "tf_read_poynting",
+              "tf_read_state", "tf_write_state", "tf_read_probes", "tf_read_poynting", "tf_read_losses",
               "tf_energy", "tf_kernel_times", "tf_set_profiling", "tf_device_bytes", "tf_destroy",
               "tf_mlp_grad", "tf_create_batch", "tf_batch_set_sources", "tf_batch_set_probes",
               "tf_batch_set_output", "tf_batch_read_probes", "tf_batch_read_state", "tf_autotune",

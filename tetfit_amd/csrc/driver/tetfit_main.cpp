@@ -7,8 +7,8 @@
 //       mode, D:1948-2030: x y z t Ex Ey Ez per probe and sample, setw(15);
 //       H at t - dt/2; cylindrical frame converted as at D:2032-2074)
 //   poynting.dat  (t, Poynting flux, losses per step, D:945-950, D:1573-1578,
-//       D:1936-1941; the losses column is 0: dga and gfdtd inputs are
-//       lossless here and the device does not form the FIT Joule term)
+//       D:1936-1941; losses from tf_read_losses: gfdtd's SigMat terms, 0 for
+//       dga (lossless here) and fdtd (Joule weights only in l2norm mode))
 //
 // usage: tetfit input.fdtd [--fp32] [--learned] [--tol X]
 //   --fp32     fp32 fields and operators (fp64 reductions)
@@ -179,13 +179,14 @@ int main(int argc, char** argv) {
         }
         // poynting.dat: step k at (k + 1/2) dt (fdtd, dga) or (k + 1) dt (gfdtd)
         {
-            int64_t np = 0;
+            int64_t np = 0, nl = 0;
             check(tf_read_poynting(ctx, nullptr, 0, &np), "tf_read_poynting");
-            std::vector<double> poy(np);
+            std::vector<double> poy(np), loss(np, 0.0);
             check(tf_read_poynting(ctx, poy.data(), np, &np), "tf_read_poynting");
+            check(tf_read_losses(ctx, loss.data(), np, &nl), "tf_read_losses");
             std::ofstream pf("poynting.dat");
             const double off = prob->method == TF_METHOD_GFDTD ? 1.0 : 0.5;
-            for (int64_t k = 0; k < np; ++k) pf << double(k + off) * prob->dt << "\t" << poy[k] << "\t" << 0.0 << std::endl;
+            for (int64_t k = 0; k < np; ++k) pf << double(k + off) * prob->dt << "\t" << poy[k] << "\t" << loss[k] << std::endl;
         }
         check(tf_destroy(ctx), "tf_destroy");
         tf_build_free(b);

@@ -164,10 +164,12 @@ __global__ void __launch_bounds__(kBlock) sell_n_kernel(SellView<T> m, const T* 
 
 // part[block] = sum over the block's rows r of x[r] (M x)[r], accumulated
 // in fp64: twice the discrete energy x.(M x) / 2 of a symmetric mass matrix
-// (We = 1/2 U.E U, Wh = 1/2 B.N B, D:2773-2775).
+// (We = 1/2 U.E U, Wh = 1/2 B.N B, D:2773-2775), or the Joule term
+// U.SigMat U of gfdtd's losses (D:1519, D:1544); guard as in cg_guard_ok.
 template <typename T, int NB>
 __global__ void __launch_bounds__(kBlock) quad_form_kernel(SellView<T> m, const T* __restrict__ x,
-                                                           double* __restrict__ part) {
+                                                           double* __restrict__ part, const CgState* guard) {
+    if (!cg_guard_ok(guard)) return;
     __shared__ double red[kBlock / kWave];
     const int lane = lane_id();
     double acc = 0;

@@ -163,6 +163,10 @@ struct tf_ctx {
     Csr32 Ct;
     Sell sCt, H, N, A, Kf, R1f, Ctf;  // Kf: E_f - [A 0] of the gfdtd residual (gf_rhs_init_kernel)
     Sell Efull;  // eps mass E of the energy diagnostic (all edges), tet methods
+    Sell Sig;    // gfdtd SigMat (all edges) of the losses, lossy media only
+    DevBuf loss_part;            // per step and block: U_old.SigMat U_old partials of a run
+    int64_t loss_cap = 0;
+    std::vector<double> loss_q;  // per step i of the simulation: U_old(i).SigMat U_old(i)
     bool c_fixed_signs = false;
     DevBuf c_col, c_sg, h_edge, curl_f, curl_u;
     DevBuf free_edge, dir_edge, dir_new, fb, pre, x, r, z, p0, p1, q, r0, cg_part, cg_state;
@@ -272,7 +276,7 @@ size_t tf_ctx::device_bytes() const {
     for (auto b : all) s += b->bytes;
     const Csr32* cs[] = {&Ct};
     for (auto c : cs) s += c->ptr.bytes + c->col.bytes + c->val.bytes + c->sg.bytes;
-    const Sell* ss[] = {&sCt, &H, &N, &A, &Kf, &R1f, &Ctf, &Efull};
+    const Sell* ss[] = {&sCt, &H, &N, &A, &Kf, &R1f, &Ctf, &Efull, &Sig};
     for (auto c : ss) s += c->sptr.bytes + c->col.bytes + c->val.bytes + c->sg.bytes + c->col16.bytes + c->cbase.bytes;
     for (auto b : lp.bufs()) s += b->bytes;
     return s;
@@ -623,6 +627,7 @@ void create_tet(tf_ctx* c, const tf_problem* p) {
     }
     upload_sell<T>(c->N, d.N, true);
     if (d.E.nrows == E && d.E.nnz > 0) upload_sell<T>(c->Efull, d.E, true);
+    if (p->method == TF_METHOD_GFDTD && d.Sig.nnz > 0) upload_sell<T>(c->Sig, d.Sig, true);
     c->h_fixed_value.assign(d.fixed_value, d.fixed_value + E);
     zero_field<T>(c->curl_u, F, c->stream);
     zero_field<T>(c->curl_f, E, c->stream);
@@ -1581,6 +1586,12 @@ void run_gfdtd(tf_ctx* c, int64_t first, int64_t n, const tf_cg_opts* cg, tf_sta
         c->poy.alloc(n * inj_blocks * sizeof(double));
         c->poy_cap = n * inj_blocks;
     }
+    const bool lossy = c->Sig.nrows == c->E;
+    const int loss_blocks = lossy ? sell_blocks(c->Sig, kSpmvCap) : 0;
+    if (lossy && c->loss_cap < n * loss_blocks) {
+        c->loss_part.alloc(n * loss_blocks * sizeof(double));
+        c->loss_cap = n * loss_blocks;
+    }
     CgState init{};
     init.tol = cg ? cg->tol : 1e-8;
     init.maxiter = cg ? cg->maxiter : 100;
@@ -1613,6 +1624,13 @@ void run_gfdtd(tf_ctx* c, int64_t first, int64_t n, const tf_cg_opts* cg, tf_sta
         ev_end(c);
         if (fresh) g.sample_slot = allow_print(c, t - c->dt) ? sample_reserve(c, t - c->dt) : -1;
         sample_launch<T>(c, g.sample_slot, guard);
+        if (lossy) {
+            // U_old.SigMat U_old of the losses (D:1519)
+            ev_begin(c, "quad_form_kernel");
+            quad_form_kernel<T, 8><<<loss_blocks, kBlock, 0, c->stream>>>(
+                c->Sig.vals<T>(), c->U.as<T>(), c->loss_part.as<double>() + k * loss_blocks, guard);
+            ev_end(c);
+        }
         if (c->n_inj) {
             ev_begin(c, "gf_inject_kernel");
             gf_inject_kernel<T><<<inj_blocks, kBlock, 0, c->stream>>>(
@@ -1746,6 +1764,17 @@ void run_gfdtd(tf_ctx* c, int64_t first, int64_t n, const tf_cg_opts* cg, tf_sta
         double s = 0;
         for (int b = 0; b < inj_blocks; ++b) s += part[k * inj_blocks + b];
         c->poynting.push_back(s);
+    }
+    if (lossy) {
+        std::vector<double> lp((size_t)n * loss_blocks);
+        TF_HIP(hipMemcpy(lp.data(), c->loss_part.p, lp.size() * sizeof(double), hipMemcpyDeviceToHost));
+        for (int64_t k = 0; k < n; ++k) {
+            double s = 0;
+            for (int b = 0; b < loss_blocks; ++b) s += lp[k * loss_blocks + b];
+            c->loss_q.push_back(s);
+        }
+    } else {
+        c->loss_q.insert(c->loss_q.end(), (size_t)n, 0.0);
     }
     st->steps_done += n;
     st->pc_state = L.state;
@@ -2298,6 +2327,28 @@ tf_status tf_read_poynting(tf_ctx* c, double* vals, int64_t cap, int64_t* n) {
     return TF_OK;
 }
 
+tf_status tf_read_losses(tf_ctx* c, double* vals, int64_t cap, int64_t* n) {
+    if (!c || !n) { g_err = "null argument"; return TF_ERR_ARG; }
+    // the losses column of poynting.dat, one value per stepped step: gfdtd
+    // pushes 1/2 U_old.SigMat U_old before each solve and, from step 2 on,
+    // U.SigMat U after it (D:1519, D:1543-1545), and the file pairs the first
+    // of those values with the steps (D:1573-1576); dga / fdtd: 0 (lossless
+    // dga here; the FIT Joule weights exist only in l2norm mode, D:5593-5597)
+    *n = (int64_t)c->poynting.size();
+    std::vector<double> col;
+    if (c->method == TF_METHOD_GFDTD) {
+        const int64_t ns = (int64_t)c->loss_q.size();  // q[i-1] = U_old(i).SigMat U_old(i)
+        for (int64_t i = 1; i <= ns && (int64_t)col.size() < *n; ++i) {
+            col.push_back(0.5 * c->loss_q[i - 1]);
+            if (i > 1 && i < ns) col.push_back(c->loss_q[i]);  // U(i) = U_old(i+1)
+        }
+    }
+    col.resize(*n, 0.0);
+    if (vals)
+        for (int64_t k = 0; k < std::min(cap, *n); ++k) vals[k] = col[k];
+    return TF_OK;
+}
+
 tf_status tf_energy(tf_ctx* c, double* we, double* wh) {
     if (!c) { g_err = "null argument"; return TF_ERR_ARG; }
     if (c->method == TF_METHOD_FDTD ? !c->fit_ep.p : c->Efull.nrows != c->E) {
@@ -2315,11 +2366,15 @@ tf_status tf_energy(tf_ctx* c, double* we, double* wh) {
             ge = sell_blocks(c->Efull, nb);
             gn = sell_blocks(c->N, nb);
             if (c->prec == TF_PREC_FP64) {
-                quad_form_kernel<double, 8><<<ge, kBlock, 0, c->stream>>>(c->Efull.vals<double>(), c->U.as<double>(), pp);
-                quad_form_kernel<double, 8><<<gn, kBlock, 0, c->stream>>>(c->N.vals<double>(), c->B.as<double>(), pp + nb);
+                quad_form_kernel<double, 8><<<ge, kBlock, 0, c->stream>>>(c->Efull.vals<double>(), c->U.as<double>(), pp,
+                                                                    nullptr);
+                quad_form_kernel<double, 8><<<gn, kBlock, 0, c->stream>>>(c->N.vals<double>(), c->B.as<double>(), pp + nb,
+                                                                    nullptr);
             } else {
-                quad_form_kernel<float, 8><<<ge, kBlock, 0, c->stream>>>(c->Efull.vals<float>(), c->U.as<float>(), pp);
-                quad_form_kernel<float, 8><<<gn, kBlock, 0, c->stream>>>(c->N.vals<float>(), c->B.as<float>(), pp + nb);
+                quad_form_kernel<float, 8><<<ge, kBlock, 0, c->stream>>>(c->Efull.vals<float>(), c->U.as<float>(), pp,
+                                                                   nullptr);
+                quad_form_kernel<float, 8><<<gn, kBlock, 0, c->stream>>>(c->N.vals<float>(), c->B.as<float>(), pp + nb,
+                                                                   nullptr);
             }
         } else if (c->prec == TF_PREC_FP64) {
             weighted_sq_kernel<double><<<nb, kBlock, 0, c->stream>>>(c->E, c->U.as<double>(), c->fit_ep.as<double>(), pp);

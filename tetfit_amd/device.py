@@ -115,6 +115,15 @@ class Simulation:
         _check(L.tf_read_poynting(self._h, v.ctypes.data_as(C.POINTER(C.c_double)), n.value, C.byref(n)))
         return v
 
+    def losses(self):
+        """tf_read_losses: the losses column of poynting.dat, one per step."""
+        L = hip_lib()
+        n = C.c_int64()
+        _check(L.tf_read_losses(self._h, None, 0, C.byref(n)))
+        v = np.zeros(n.value)
+        _check(L.tf_read_losses(self._h, v.ctypes.data_as(C.POINTER(C.c_double)), n.value, C.byref(n)))
+        return v
+
     def set_profiling(self, on=True):
         _check(hip_lib().tf_set_profiling(self._h, 1 if on else 0))
 
