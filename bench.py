@@ -122,6 +122,7 @@ def time_path(b, precision, steps, warmup, precond="jacobi", label="", learn_ste
         log(f"[bench] {label}: trained on {ls['pc_pairs']} pairs in {ls['pc_train_seconds']:.2f} s, "
             f"loss {ls['pc_loss0']:.4g} -> {ls['pc_loss']:.4g}")
     tuned = sim.autotune()
+    log(f"[bench] {label}: launch shapes {tuned}")
     if warmup:
         sim.run(n=warmup, precond=precond)
     st = sim.run(n=steps, precond=precond)

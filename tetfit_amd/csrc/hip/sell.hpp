@@ -322,14 +322,10 @@ __global__ void __launch_bounds__(kBlock) cg_restart_kernel(int32_t n, const T* 
 }
 
 // w = A u, partial w.u; with test_stop the pass starts with the stop test on
-// the partials part_rr of the previous update. With `early` the product of
-// the wave's first slice is issued before the partial sums are reduced (it
-// does not depend on them), which hides that round trip; a pass that then
-// stops has computed that slice in vain, so the host sets `early` only for
-// passes that are not expected to stop.
+// the partials part_rr of the previous update.
 template <typename T, int NB>
 __global__ void __launch_bounds__(kBlock) cg_w_kernel(int32_t it, SellView<T> m, const T* __restrict__ u,
-                                                      T* __restrict__ w, int test_stop, int early,
+                                                      T* __restrict__ w, int test_stop,
                                                       const double* __restrict__ part_rr, int32_t nb_rr,
                                                       double* __restrict__ part_del, CgState* st) {
     __shared__ double red[kRed2];
@@ -338,30 +334,23 @@ __global__ void __launch_bounds__(kBlock) cg_w_kernel(int32_t it, SellView<T> m,
     const double rr0 = st->rr0, tol = st->tol;
     const int32_t maxiter = st->maxiter;
     if (done) return;
-    const int lane = lane_id();
-    SliceRange sr = block_slices(m.nslices);
-    auto gat = [&](int32_t j) { return u[j]; };
-    const bool first = early && sr.s < sr.end;
-    const double w0 = first ? sell_dot<T, NB>(m, sr.s, lane, gat) : 0.0;
     if (test_stop) {
         double RR = thread_sum(prr), unused = 0;
         block_sum2_all(RR, unused, red);
         if (cg_decide_stop(it, RR, rr0, tol, maxiter, st)) return;
     }
+    const int lane = lane_id();
     double wu = 0, none = 0;
-    auto row = [&](int sl, double wv) {
-        const int32_t i = sl * kSlice + lane;
+    TF_FOR_SLICES(m) {
+        const int s = sr_.s;
+        const double wv = sell_dot<T, NB>(m, s, lane, [&](int32_t j) { return u[j]; });
+        const int32_t i = s * kSlice + lane;
         if (i < m.nrows) {
             const T wi = (T)wv;
             w[i] = wi;
             wu += (double)wi * (double)u[i];
         }
-    };
-    if (first) {
-        row(sr.s, w0);
-        sr.s += sr.step;
     }
-    for (; sr.s < sr.end; sr.s += sr.step) row(sr.s, sell_dot<T, NB>(m, sr.s, lane, gat));
     block_sum2_all(wu, none, red);
     if (threadIdx.x == 0) part_del[blockIdx.x] = wu;
 }
@@ -558,10 +547,9 @@ __global__ void __launch_bounds__(kBlock) cg_pg_kernel(int32_t it, SellView<uint
 // and beta_it (stored in slot it&1), w = A u, p = u + beta p and
 // s_it = w + beta s_{it-1} on the owned rows (rs_cur[i] holds (r_it,
 // s_{it-1}) as cg_pg wrote it; s_it replaces its s), partial w.u (delta_it).
-// `early` as in cg_w_kernel.
 template <typename T, int NB>
 __global__ void __launch_bounds__(kBlock) cg_wu_kernel(int32_t it, SellView<T> m, const T* __restrict__ u,
-                                                       T* __restrict__ p, RS<T>* __restrict__ rs_cur, int early,
+                                                       T* __restrict__ p, RS<T>* __restrict__ rs_cur,
                                                        const double* __restrict__ part_rr,
                                                        const double* __restrict__ part_gam, int32_t nb_pg,
                                                        double* __restrict__ part_del, CgState* st) {
@@ -571,11 +559,6 @@ __global__ void __launch_bounds__(kBlock) cg_wu_kernel(int32_t it, SellView<T> m
     const double rr0 = st->rr0, tol = st->tol, gam_prev = st->gam[(it - 1) & 1];
     const int32_t maxiter = st->maxiter;
     if (done) return;
-    const int lane = lane_id();
-    SliceRange sr = block_slices(m.nslices);
-    auto gat = [&](int32_t j) { return u[j]; };
-    const bool first = early && sr.s < sr.end;
-    const double w0 = first ? sell_dot<T, NB>(m, sr.s, lane, gat) : 0.0;
     double RR = thread_sum(prr), gam = thread_sum(pga);
     block_sum2_all(RR, gam, red);
     if (cg_decide_stop(it, RR, rr0, tol, maxiter, st)) return;
@@ -584,13 +567,16 @@ __global__ void __launch_bounds__(kBlock) cg_wu_kernel(int32_t it, SellView<T> m
         st->gam[it & 1] = gam;
         st->beta[it & 1] = beta;
     }
+    const int lane = lane_id();
     double wu = 0, none = 0;
-    auto row = [&](int sl, double wv) {
+    TF_FOR_SLICES(m) {
+        const int sl = sr_.s;
+        const double wv = sell_dot<T, NB>(m, sl, lane, [&](int32_t j) { return u[j]; });
         const int32_t i = sl * kSlice + lane;
         if (i < m.nrows) {
             const T wi = (T)wv;
             const T ui = u[i];
-            const RS<T> o = rs_cur[i];
+            const RS<T> o = rs_cur[i];  // (r', s of the previous pass), as cg_pg wrote it
             if (it == 0) {
                 p[i] = ui;
                 rs_cur[i] = RS<T>{o.r, wi};
@@ -600,12 +586,7 @@ __global__ void __launch_bounds__(kBlock) cg_wu_kernel(int32_t it, SellView<T> m
             }
             wu += (double)wi * (double)ui;
         }
-    };
-    if (first) {
-        row(sr.s, w0);
-        sr.s += sr.step;
     }
-    for (; sr.s < sr.end; sr.s += sr.step) row(sr.s, sell_dot<T, NB>(m, sr.s, lane, gat));
     block_sum2_all(wu, none, red);
     if (threadIdx.x == 0) part_del[blockIdx.x] = wu;
 }

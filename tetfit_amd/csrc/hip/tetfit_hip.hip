@@ -1194,10 +1194,6 @@ void cg_passes(tf_ctx* c, CgSolve& cs, int n) {
     };
     if (cs.it == 0) cg_capture<T>(c, cs, v, 0);
     const int stop = std::min(cs.it + n, cs.init.maxiter + 1);
-    // passes before the previous solve's count are not expected to stop:
-    // their SpMV starts ahead of the stop test (cg_w / cg_wu `early`)
-    const int guess = cs.learned ? c->cg_guess_pc : c->cg_guess;
-    auto early = [&](int it) { return it + 1 < guess; };
     for (int& it = cs.it; it < stop; ++it) {
         if (cs.learned) {
             ev_begin(c, "cg_pg_kernel");
@@ -1208,14 +1204,14 @@ void cg_passes(tf_ctx* c, CgSolve& cs, int n) {
             tag(it, 1);
             ev_begin(c, "cg_wu_kernel");
             TF_LAUNCH_NB(c->tune[TK_WU].unroll, cg_wu_kernel, T, v.nb_w, c->stream, it, v.A, v.u, v.p,
-                         v.rs[it & 1], early(it) ? 1 : 0, v.part_rl, v.part_gl, v.nb_g, v.part_del, dst);
+                         v.rs[it & 1], v.part_rl, v.part_gl, v.nb_g, v.part_del, dst);
             ev_end(c);
             tag(it, 0);
             continue;  // at it == maxiter the stop test of cg_wu ends the solve
         }
         ev_begin(c, "cg_w_kernel");
-        TF_LAUNCH_NB(c->tune[TK_W].unroll, cg_w_kernel, T, v.nb_w, c->stream, it, v.A, v.u, v.w, 1,
-                     early(it) ? 1 : 0, v.part_rr, v.nb_b, v.part_del, dst);
+        TF_LAUNCH_NB(c->tune[TK_W].unroll, cg_w_kernel, T, v.nb_w, c->stream, it, v.A, v.u, v.w, 1, v.part_rr,
+                     v.nb_b, v.part_del, dst);
         ev_end(c);
         tag(it, 0);
         if (it == cs.init.maxiter) { ++it; break; }  // the pass at maxiter only runs the stop test
@@ -2074,8 +2070,10 @@ void autotune(tf_ctx* c) {
         // Tuning launches run pass 1 (the full work of a pass) on a scratch
         // CG state that never stops (tol 0, no iteration cap) with unit
         // partials. Their outputs go to vectors every solve rewrites before
-        // use: u (z), w (q, also standing in for x), p (p0) and the (r, s)
-        // pairs (rs0, rs1).
+        // use: u (z), w (q, also standing in for x), r, p (p0), s (p1) and
+        // the (r, s) pairs (rs0, rs1). A pass kernel's grid sets how many
+        // partials the next kernel sums, so each candidate is timed together
+        // with the kernel that follows it in a pass, reading that many.
         CgState sc{};
         sc.tol = 0.0;
         sc.maxiter = INT32_MAX;
@@ -2086,19 +2084,39 @@ void autotune(tf_ctx* c) {
         double* part = c->cg_part.as<double>();
         const double* one = ones.as<double>();
         CgState* st = scratch.as<CgState>();
+        const int32_t nf = c->n_free;
+        const int nbb = std::max(1, std::min(blocks_for(nf), kCgPartB));  // cg_upd's grid (CgView)
+        // Jacobi pass: cg_w (stop test on cg_upd's partials) + the cg_upd that sums its b partials
         c->tune[TK_W] = tune_one(c, TK_W, c->A, kCgPartA, [&](int b, int u) {
-            TF_LAUNCH_NB(u, cg_w_kernel, T, b, s, 1, c->A.vals<T>(), c->z.as<T>(), c->q.as<T>(), 0, 1, one, 1, part,
+            TF_LAUNCH_NB(u, cg_w_kernel, T, b, s, 1, c->A.vals<T>(), c->z.as<T>(), c->q.as<T>(), 1, one, nbb, part,
                          st);
+            cg_upd_kernel<T><<<nbb, kBlock, 0, s>>>(1, nf, c->pre.as<T>(), c->q.as<T>(), c->r.as<T>(), c->z.as<T>(),
+                                                    c->q.as<T>(), c->p0.as<T>(), c->p1.as<T>(), one, nbb, one, b,
+                                                    part, part, st);
         });
         if (c->lp.state == LearnedPc::kActive) {
-            c->tune[TK_G] = tune_one(c, TK_G, c->A, kCgPartC, [&](int b, int u) {
-                TF_LAUNCH_NB(u, cg_pg_kernel, T, b, s, 1, c->A.view<uint16_t>(c->lp.gval),
-                             c->lp.gpk.p ? c->lp.gpk.as<uint32_t>() : nullptr, c->rs0.as<RS<T>>(),
-                             c->rs1.as<RS<T>>(), c->p0.as<T>(), c->q.as<T>(), c->z.as<T>(), one, 1, part, part, st);
-            });
-            c->tune[TK_WU] = tune_one(c, TK_WU, c->A, kCgPartA, [&](int b, int u) {
+            const SellView<uint16_t> gv = c->A.view<uint16_t>(c->lp.gval);
+            const uint32_t* gpk = c->lp.gpk.p ? c->lp.gpk.as<uint32_t>() : nullptr;
+            auto pg = [&](int b, int u, int nparts) {
+                TF_LAUNCH_NB(u, cg_pg_kernel, T, b, s, 1, gv, gpk, c->rs0.as<RS<T>>(), c->rs1.as<RS<T>>(),
+                             c->p0.as<T>(), c->q.as<T>(), c->z.as<T>(), one, nparts, part, part, st);
+            };
+            auto wu = [&](int b, int u, int nparts) {
                 TF_LAUNCH_NB(u, cg_wu_kernel, T, b, s, 1, c->A.vals<T>(), c->z.as<T>(), c->p0.as<T>(),
-                             c->rs1.as<RS<T>>(), 1, one, one, 1, part, st);
+                             c->rs1.as<RS<T>>(), one, one, nparts, part, st);
+            };
+            // cg_pg (summing cg_wu's partials) + the cg_wu that sums its b partials, then
+            // cg_wu (summing the chosen cg_pg's partials) + the cg_pg that sums its b
+            const int wb0 = tuned_blocks(c, TK_WU, c->A, kCgPartA);
+            const int wu0 = c->tune[TK_WU].unroll > 0 ? c->tune[TK_WU].unroll : 8;
+            c->tune[TK_G] = tune_one(c, TK_G, c->A, kCgPartC, [&](int b, int u) {
+                pg(b, u, wb0);
+                wu(wb0, wu0, b);
+            });
+            const KTune G = c->tune[TK_G];
+            c->tune[TK_WU] = tune_one(c, TK_WU, c->A, kCgPartA, [&](int b, int u) {
+                wu(b, u, G.blocks);
+                pg(G.blocks, G.unroll, b);
             });
         }
         TF_HIP(hipStreamSynchronize(s));
