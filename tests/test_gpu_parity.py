@@ -226,3 +226,17 @@ def test_batch_graph_steps_match_eager(workdir):
     for k in range(len(builds)):
         for a, e in zip(out[True][k], out[False][k]):
             assert np.array_equal(a, e)
+
+
+@pytest.mark.parametrize("method", ["dga", "gfdtd"])
+def test_int32_columns_parity(workdir, method, monkeypatch):
+    """Every SELL operator stored with int32 column indices (the layout for
+    meshes whose slices span more than 65,536 columns) gives the same fp64
+    parity."""
+    monkeypatch.setenv("TETFIT_SELL_COL32", "1")
+    b = Model(configs.kuhn_cavity(str(workdir), n=5, method=method, n_steps=60, n_probes=4, block=2,
+                                  name=f"c32_{method}")).build(1)
+    sim = Simulation(b, "fp64")
+    assert sim.sell_info("n")["col_bytes"] == 4
+    sim.close()
+    check_run(b)

@@ -193,6 +193,31 @@ def test_learned_preconditioner_gain(contrast_cavity, prec, bar):
 
 
 @pytest.mark.gpu
+def test_learned_int32_columns(contrast_cavity, monkeypatch):
+    """The same learned run with A (and G beside it) stored with int32
+    column indices instead of 16-bit offsets: the unpacked G path of
+    cg_pg_kernel and the int32 path of the SELL kernels."""
+    from tetfit_amd.device import Simulation
+    monkeypatch.setenv("TETFIT_SELL_COL32", "1")
+    b = contrast_cavity
+    print(f"case: learned fp32 int32 columns E={b.n_edges} steps={b.n_steps}", flush=True)
+    sim = Simulation(b, "fp32")
+    assert sim.sell_info("a")["col_bytes"] == 4
+    s1 = sim.run(n=30, precond="learned", learn_steps=30, train_iters=80)
+    s2 = sim.run(n=b.n_steps - 30, precond="learned")
+    _, samples = sim.probes()
+    U, F, B = sim.state()
+    sim.close()
+    assert s2["pc_state"] == 2 and s2["pc_fallbacks"] == 0
+    assert s2["cg_not_converged"] == 0 and s2["cg_max_relres"] <= b.cg_tol
+    assert s2["cg_iters_total"] <= 0.8 * s1["cg_iters_total"] * (b.n_steps - 30) / 30
+    r = pyoracle.run(b)
+    assert rel(samples, r["samples"]) <= 1e-4
+    for got, ref in ((U, r["U"]), (F, r["F"]), (B, r["B"])):
+        assert rel(got, ref) <= 1e-4
+
+
+@pytest.mark.gpu
 def test_training_is_what_gains(tmp_path):
     """On the 48^3 Kuhn workload the operator the network starts from does
     not beat Jacobi: with training switched off (learning rate 1e-12, so the

@@ -358,8 +358,11 @@ void upload_sell(Sell& d, const tf_csr& c, bool values) {
     if (values) upload(d.val, val);
     else upload(d.sg, sg);
     // 16-bit columns when every slice spans fewer than 65536 columns
+    // ($TETFIT_SELL_COL32 set to a non-empty value other than 0 keeps int32
+    // columns, to exercise that layout on small meshes)
     std::vector<int32_t> cb(std::max(d.nslices, 1), 0);
-    bool fits = true;
+    const char* force32 = std::getenv("TETFIT_SELL_COL32");
+    bool fits = !(force32 && *force32 && std::strcmp(force32, "0") != 0);
     for (int32_t s = 0; s < d.nslices && fits; ++s) {
         int32_t lo = INT32_MAX, hi = 0;
         for (int64_t o = sptr[s]; o < sptr[s + 1]; ++o) { lo = std::min(lo, col[o]); hi = std::max(hi, col[o]); }
