@@ -376,7 +376,7 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=48, help="Kuhn cubes per side")
-    ap.add_argument("--cpu-steps", type=int, default=8)
+    ap.add_argument("--cpu-steps", type=int, default=60, help="oracle steps of the CPU baseline (about 10-15 s at 48^3)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-dga", action="store_true")
     ap.add_argument("--workload", choices=["gfdtd", "contrast", "batch", "fit", "fitprec", "copy"], default="gfdtd",
