@@ -11,6 +11,7 @@ import pytest
 
 from tetfit_amd import Model, configs
 
+import pyoracle
 from test_gpu_parity import check_run
 
 pytestmark = pytest.mark.gpu
@@ -94,7 +95,6 @@ def test_gfdtd_lossy_losses_column(workdir):
     """tf_read_losses against the checker's Losses vector: the first n
     values, one per step, as poynting.dat pairs them (D:1573-1576)."""
     import numpy as np
-    import pyoracle
     from tetfit_amd.device import Simulation
     from test_tet_host import _lossy
     p = configs.kuhn_cavity(str(workdir), n=5, method="gfdtd", n_steps=40, n_probes=2, block=2, name="gf_loss")
@@ -107,3 +107,45 @@ def test_gfdtd_lossy_losses_column(workdir):
     ref = pyoracle.run(b)["losses"][:b.n_steps]
     assert len(got) == b.n_steps and np.abs(ref).max() > 0
     np.testing.assert_allclose(got, ref, rtol=0, atol=1e-10 * np.abs(ref).max())
+
+
+def test_gfdtd_e_source_cos(workdir):
+    """An e source whose carrier is not zero at t = 0 (cos, with the amp^2
+    quirk): the pass before the loop (D:1434-1453) sets U on its Dirichlet
+    edges at t = 0, which step 1 reads through E U_old."""
+    p = configs.kuhn_cavity(str(workdir), n=5, method="gfdtd", n_steps=40, n_probes=4, block=2, name="gf_ecos")
+    edit(p, lambda s: set_source1(s, type="e", carrier="cos", amplitude="1.5"))
+    st, r = check_run(Model(p).build(1))
+    assert st["cg_not_converged"] == 0
+
+
+@pytest.mark.parametrize("method", ["gfdtd", "dga"])
+def test_written_state(workdir, method):
+    """tf_write_state then a run from step 1 against the checker started from
+    the same fields: gfdtd warm-starts its first solve from the written U on
+    the free edges and the pass before the loop resets the Dirichlet edges
+    (random values written there too); dga from a state that is zero on its
+    class-4 edges."""
+    import numpy as np
+    from tetfit_amd.device import Simulation
+    b = Model(configs.kuhn_cavity(str(workdir), n=5, method=method, n_steps=40, n_probes=4, block=2,
+                                  name=f"ws_{method}")).build(1)
+    rng = np.random.default_rng(11)
+    U0 = rng.standard_normal(b.n_edges)
+    B0 = 1e-8 * rng.standard_normal(b.n_faces)
+    F0 = None
+    if method == "dga":
+        U0[b.dga_ops()["edge_class"] == 4] = 0.0
+        F0 = 1e-2 * rng.standard_normal(b.n_faces)
+    print(f"case: written state {method} E={b.n_edges}", flush=True)
+    sim = Simulation(b, "fp64")
+    sim.set_state(U=U0, F=F0, B=B0)
+    sim.run()
+    t_g, s_g = sim.probes()
+    U, F, B = sim.state()
+    sim.close()
+    r = pyoracle.run(b, U0=U0, F0=F0, B0=B0)
+    from test_gpu_parity import per_probe_err, rel_l2
+    assert per_probe_err(s_g, r["samples"]) <= 1e-10
+    for got, ref in ((U, r["U"]), (B, r["B"])):
+        assert rel_l2(got, ref) <= 1e-10

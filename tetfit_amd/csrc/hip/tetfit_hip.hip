@@ -1665,6 +1665,22 @@ void run_gfdtd(tf_ctx* c, int64_t first, int64_t n, const tf_cg_opts* cg, tf_sta
         return snaps;
     };
 
+    // The pass before the loop (D:1434-1453), once, when the run starts at
+    // step 1: U on the Dirichlet edges takes the boundary values and the e
+    // sources at t = 0 (carrier row 0 is step first - 1 = 0). The fb and
+    // Poynting slot it also writes are rewritten by step 1.
+    if (first == 1) {
+        if (c->n_inj)
+            gf_inject_kernel<T><<<inj_blocks, kBlock, 0, c->stream>>>(
+                c->n_inj, c->inj_edge.as<int32_t>(), c->inj_kind.as<int8_t>(), c->inj_tb.as<int32_t>(),
+                c->inj_te.as<int32_t>(), c->term_src.as<int32_t>(), c->term_w.as<double>(), c->carr.as<double>(),
+                nullptr, c->fb.as<T>(), c->dir_new.as<T>(), c->U.as<T>(), c->poy.as<double>(), nullptr);
+        if (c->n_dir > 0)
+            gf_scatter_kernel<T><<<blocks_for(c->n_dir), kBlock, 0, c->stream>>>(
+                0, c->free_edge.as<int32_t>(), c->x.as<T>(), c->U.as<T>(), c->n_dir, c->dir_edge.as<int32_t>(),
+                c->dir_new.as<T>(), nullptr);
+        c->ud_zero = c->dir_zero_static;
+    }
     GfPre cur;
     cur.learned = predict_learned(c, want_learned) && L.state == LearnedPc::kActive;
     pre(0, cur, true, nullptr);
