@@ -4,6 +4,7 @@
 // Reference loops replaced: fdtd D:1840-1933, dga D:810-939,
 // gfdtd D:1459-1565 (D: = src/DGA/Discretization.hpp), with the CG of
 // src/DGA/ConjugateGradientSolver.hpp:52-103 run on the device.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -467,6 +468,35 @@ void ev_end(tf_ctx* c) {
     if (!c->profiling) return;
     TF_HIP(hipEventRecord(c->recs.back().b, c->stream));
 }
+// Profiled launch whose event pair is bound to the kernel's own dispatch
+// (hipExtLaunchKernelGGL records start and stop with the dispatch), so the
+// time excludes the marker packets that a separate event record adds
+// (about 2 us per launch here); plain launch when not profiling.
+template <typename K, typename... Args>
+void launch_timed(tf_ctx* c, const char* name, K kernel, int grid, Args... args) {
+    if (!c->profiling) {
+        kernel<<<grid, kBlock, 0, c->stream>>>(args...);
+        return;
+    }
+    if (c->ev_used + 2 > c->ev_pool.size())
+        for (int i = 0; i < 256; ++i) {
+            hipEvent_t e;
+            TF_HIP(hipEventCreate(&e));
+            c->ev_pool.push_back(e);
+        }
+    tf_ctx::Rec r{name, c->ev_pool[c->ev_used], c->ev_pool[c->ev_used + 1]};
+    c->ev_used += 2;
+    c->recs.push_back(r);
+    hipExtLaunchKernelGGL(kernel, dim3(grid), dim3(kBlock), 0, c->stream, r.a, r.b, 0, args...);
+}
+#define TF_LAUNCH_TIMED(c, name, nb, kern, T, grid, ...)                                       \
+    do {                                                                                    \
+        switch (nb) {                                                                       \
+            case 4: launch_timed(c, name, kern<T, 4>, grid, __VA_ARGS__); break;               \
+            case 16: launch_timed(c, name, kern<T, 16>, grid, __VA_ARGS__); break;             \
+            default: launch_timed(c, name, kern<T, 8>, grid, __VA_ARGS__); break;              \
+        }                                                                                   \
+    } while (0)
 void ev_flush(tf_ctx* c) {
     if (!c->profiling || c->recs.empty()) return;
     TF_HIP(hipStreamSynchronize(c->stream));
@@ -961,19 +991,15 @@ void run_fit(tf_ctx* c, int64_t first, int64_t n, tf_stats* st) {
     }
     // one step: E phase (with sources and the Poynting partials), H phase
     auto step = [&](const double* cs, double* poy) {
-        ev_begin(c, "fit_e_kernel");
-        fit_e_kernel<T><<<main_blocks + sp_blocks, kBlock, 0, c->stream>>>(
+        launch_timed(c, "fit_e_kernel", fit_e_kernel<T>, main_blocks + sp_blocks, 
             c->E, main_blocks, c->fe_sg.as<uint8_t>(), c->fe_ca.as<T>(), c->fe_cb.as<T>(), c->fe_f4.as<int4>(),
             c->Fv.as<T>(), c->U.as<T>(), c->n_sp, c->sp_edge.as<int32_t>(), c->sp_f4.as<int4>(),
             c->sp_sg.as<uint8_t>(), c->sp_ca.as<double>(), c->sp_cb.as<double>(), c->sp_kind.as<int8_t>(),
             c->sp_tb.as<int32_t>(), c->sp_te.as<int32_t>(), c->term_src.as<int32_t>(), c->term_w.as<double>(), cs,
             sp_blocks ? poy : nullptr);
-        ev_end(c);
-        ev_begin(c, "fit_h_kernel");
-        fit_h_kernel<T><<<blocks_for(c->F), kBlock, 0, c->stream>>>(c->F, c->fh_sg.as<uint8_t>(), c->fh_cm.as<T>(),
-                                                                     c->fh_cn.as<T>(), c->fh_e4.as<int4>(),
-                                                                     c->U.as<T>(), c->Fv.as<T>(), c->B.as<T>(), c->dt);
-        ev_end(c);
+        launch_timed(c, "fit_h_kernel", fit_h_kernel<T>, blocks_for(c->F), c->F, c->fh_sg.as<uint8_t>(),
+                     c->fh_cm.as<T>(), c->fh_cn.as<T>(), c->fh_e4.as<int4>(), c->U.as<T>(), c->Fv.as<T>(),
+                     c->B.as<T>(), c->dt);
     };
     const int64_t g = graph_steps(c) ? run_graph_chunks(c, n, sp_blocks, [&](int, const double* cs, double* poy) {
         step(cs, poy);
@@ -1028,40 +1054,27 @@ void run_dga(tf_ctx* c, int64_t first, int64_t n, tf_stats* st) {
     // Poynting, curl_u = C U with B update, F -= dt N curl_u; poy holds the
     // step's two Poynting partial blocks
     auto step = [&](const double* cs, double* poy) {
-        ev_begin(c, "sell_ct_kernel");
-        TF_LAUNCH_NB(c->tune[TK_CT].unroll, sell_ct_kernel, T, gb_ct, c->stream, c->sCt.signs(), c->Fv.as<T>(),
+        TF_LAUNCH_TIMED(c, "sell_ct_kernel", c->tune[TK_CT].unroll, sell_ct_kernel, T, gb_ct, c->sCt.signs(), c->Fv.as<T>(),
                      c->curl_f.as<T>());
-        ev_end(c);
         if (c->n_inj) {
-            ev_begin(c, "dga_inject_kernel");
-            dga_inject_kernel<T><<<inj_blocks, kBlock, 0, c->stream>>>(
+            launch_timed(c, "dga_inject_kernel", dga_inject_kernel<T>, inj_blocks, 
                 c->n_inj, c->inj_edge.as<int32_t>(), c->inj_kind.as<int8_t>(), c->inj_tb.as<int32_t>(),
                 c->inj_te.as<int32_t>(), c->term_src.as<int32_t>(), c->term_w.as<double>(), cs, c->curl_f.as<T>(),
                 c->U.as<T>(), poy);
-            ev_end(c);
         }
-        ev_begin(c, "sell_h_kernel");
-        TF_LAUNCH_NB(c->tune[TK_H].unroll, sell_h_kernel, T, gb_h, c->stream, c->H.vals<T>(), c->h_edge.as<int32_t>(),
+        TF_LAUNCH_TIMED(c, "sell_h_kernel", c->tune[TK_H].unroll, sell_h_kernel, T, gb_h, c->H.vals<T>(), c->h_edge.as<int32_t>(),
                      c->curl_f.as<T>(), c->U.as<T>(), c->dt, hdt);
-        ev_end(c);
         if (c->n_inj) {
-            ev_begin(c, "dga_poynting_kernel");
-            dga_poynting_kernel<T><<<inj_blocks, kBlock, 0, c->stream>>>(
+            launch_timed(c, "dga_poynting_kernel", dga_poynting_kernel<T>, inj_blocks, 
                 c->n_inj, c->inj_edge.as<int32_t>(), c->inj_kind.as<int8_t>(), c->inj_tb.as<int32_t>(),
                 c->inj_te.as<int32_t>(), c->term_src.as<int32_t>(), c->term_w.as<double>(), cs, c->U.as<T>(),
                 poy + inj_blocks);
-            ev_end(c);
         }
-        ev_begin(c, "dga_curl_kernel");
-        dga_curl_kernel<T><<<blocks_for(c->F), kBlock, 0, c->stream>>>(c->F, c->c_col.as<int32_t>(),
-                                                                        c->c_fixed_signs ? nullptr : c->c_sg.as<int8_t>(), c->U.as<T>(),
-                                                                        c->curl_u.as<T>(), c->B.as<T>(), c->dt, fdt,
-                                                                        nullptr);
-        ev_end(c);
-        ev_begin(c, "sell_n_kernel");
-        TF_LAUNCH_NB(c->tune[TK_N].unroll, sell_n_kernel, T, gb_n, c->stream, c->N.vals<T>(), c->curl_u.as<T>(),
+        launch_timed(c, "dga_curl_kernel", dga_curl_kernel<T>, blocks_for(c->F), c->F, c->c_col.as<int32_t>(),
+                     c->c_fixed_signs ? nullptr : c->c_sg.as<int8_t>(), c->U.as<T>(), c->curl_u.as<T>(),
+                     c->B.as<T>(), c->dt, fdt, (const CgState*)nullptr);
+        TF_LAUNCH_TIMED(c, "sell_n_kernel", c->tune[TK_N].unroll, sell_n_kernel, T, gb_n, c->N.vals<T>(), c->curl_u.as<T>(),
                      c->Fv.as<T>(), c->dt, 0, fdt, nullptr);
-        ev_end(c);
     };
     const int64_t g = graph_steps(c) ? run_graph_chunks(c, n, 2 * inj_blocks, [&](int, const double* cs, double* poy) {
         step(cs, poy);
@@ -1160,18 +1173,15 @@ void cg_start(tf_ctx* c, CgSolve& cs, bool with_rhs, const CgState* guard) {
     T* r0 = cs.learned ? &v.rs[0]->r : v.rj;
     const int r0_stride = cs.learned ? 2 : 1;
     if (with_rhs) {
-        ev_begin(c, "gf_rhs_init_kernel");
-        TF_LAUNCH_NB(8, gf_rhs_init_kernel, T, v.nb_b, c->stream, c->Kf.vals<T>(), c->R1f.vals<T>(), c->Ctf.signs(),
-                     cs.dir_terms ? 1 : 0, c->free_edge.as<int32_t>(), c->U.as<T>(), c->dir_new.as<T>(),
-                     c->Fv.as<T>(), c->fb.as<T>(), c->r0.as<T>(), c->dt, c->pre.as<T>(), r0, r0_stride, v.u,
-                     v.part_rr, v.part_gj[0], dst, cs.learned ? 0 : 1, guard);
+        TF_LAUNCH_TIMED(c, "gf_rhs_init_kernel", 8, gf_rhs_init_kernel, T, v.nb_b, c->Kf.vals<T>(),
+                        c->R1f.vals<T>(), c->Ctf.signs(), cs.dir_terms ? 1 : 0, c->free_edge.as<int32_t>(),
+                        c->U.as<T>(), c->dir_new.as<T>(), c->Fv.as<T>(), c->fb.as<T>(), c->r0.as<T>(), c->dt,
+                        c->pre.as<T>(), r0, r0_stride, v.u, v.part_rr, v.part_gj[0], dst, cs.learned ? 0 : 1,
+                        guard);
     } else {
-        ev_begin(c, "cg_restart_kernel");
-        cg_restart_kernel<T><<<v.nb_b, kBlock, 0, c->stream>>>(v.nf, c->pre.as<T>(), c->r0.as<T>(), r0, r0_stride,
-                                                               v.u, v.part_rr, v.part_gj[0], dst,
-                                                               cs.learned ? 0 : 1);
+        launch_timed(c, "cg_restart_kernel", cg_restart_kernel<T>, v.nb_b, v.nf, c->pre.as<T>(), c->r0.as<T>(), r0,
+                     r0_stride, v.u, v.part_rr, v.part_gj[0], dst, cs.learned ? 0 : 1);
     }
-    ev_end(c);
     cs.it = 0;
     cs.tags.clear();
 }
@@ -1199,30 +1209,23 @@ void cg_passes(tf_ctx* c, CgSolve& cs, int n) {
     const int stop = std::min(cs.it + n, cs.init.maxiter + 1);
     for (int& it = cs.it; it < stop; ++it) {
         if (cs.learned) {
-            ev_begin(c, "cg_pg_kernel");
-            TF_LAUNCH_NB(c->tune[TK_G].unroll, cg_pg_kernel, T, v.nb_g, c->stream, it, v.G, v.gpk,
-                         v.rs[it == 0 ? 0 : (it - 1) & 1], v.rs[it & 1], v.p, v.x, v.u, v.part_del, v.nb_w,
-                         v.part_gl, v.part_rl, dst);
-            ev_end(c);
+            TF_LAUNCH_TIMED(c, "cg_pg_kernel", c->tune[TK_G].unroll, cg_pg_kernel, T, v.nb_g, it, v.G, v.gpk,
+                            (const RS<T>*)v.rs[it == 0 ? 0 : (it - 1) & 1], v.rs[it & 1], (const T*)v.p, v.x,
+                            v.u, (const double*)v.part_del, v.nb_w, v.part_gl, v.part_rl, dst);
             tag(it, 1);
-            ev_begin(c, "cg_wu_kernel");
-            TF_LAUNCH_NB(c->tune[TK_WU].unroll, cg_wu_kernel, T, v.nb_w, c->stream, it, v.A, v.u, v.p,
-                         v.rs[it & 1], v.part_rl, v.part_gl, v.nb_g, v.part_del, dst);
-            ev_end(c);
+            TF_LAUNCH_TIMED(c, "cg_wu_kernel", c->tune[TK_WU].unroll, cg_wu_kernel, T, v.nb_w, it, v.A,
+                            (const T*)v.u, v.p, v.rs[it & 1], (const double*)v.part_rl, (const double*)v.part_gl,
+                            v.nb_g, v.part_del, dst);
             tag(it, 0);
             continue;  // at it == maxiter the stop test of cg_wu ends the solve
         }
-        ev_begin(c, "cg_w_kernel");
-        TF_LAUNCH_NB(c->tune[TK_W].unroll, cg_w_kernel, T, v.nb_w, c->stream, it, v.A, v.u, v.w, 1, v.part_rr,
-                     v.nb_b, v.part_del, dst);
-        ev_end(c);
+        TF_LAUNCH_TIMED(c, "cg_w_kernel", c->tune[TK_W].unroll, cg_w_kernel, T, v.nb_w, it, v.A, (const T*)v.u,
+                        v.w, 1, (const double*)v.part_rr, v.nb_b, v.part_del, dst);
         tag(it, 0);
         if (it == cs.init.maxiter) { ++it; break; }  // the pass at maxiter only runs the stop test
-        ev_begin(c, "cg_upd_kernel");
-        cg_upd_kernel<T><<<v.nb_b, kBlock, 0, c->stream>>>(it, v.nf, c->pre.as<T>(), v.x, v.rj, v.u, v.w, v.p, v.sv,
-                                                           v.part_gj[it & 1], v.nb_b, v.part_del, v.nb_w,
-                                                           v.part_rr, v.part_gj[(it + 1) & 1], dst);
-        ev_end(c);
+        launch_timed(c, "cg_upd_kernel", cg_upd_kernel<T>, v.nb_b, it, v.nf, (const T*)c->pre.as<T>(), v.x, v.rj,
+                     v.u, (const T*)v.w, v.p, v.sv, (const double*)v.part_gj[it & 1], v.nb_b,
+                     (const double*)v.part_del, v.nb_w, v.part_rr, v.part_gj[(it + 1) & 1], dst);
         tag(it, 0);
         cg_capture<T>(c, cs, v, it + 1);
     }
@@ -1612,31 +1615,23 @@ void run_gfdtd(tf_ctx* c, int64_t first, int64_t n, const tf_cg_opts* cg, tf_sta
         const double t = (double)i * c->dt;
         const double* cs = c->carr.as<double>() + (k + 1) * ns;
         const double* cp = c->carr.as<double>() + k * ns;
-        ev_begin(c, "dga_curl_kernel");
-        dga_curl_kernel<T><<<blocks_for(c->F), kBlock, 0, c->stream>>>(
-            c->F, c->c_col.as<int32_t>(), c->c_fixed_signs ? nullptr : c->c_sg.as<int8_t>(), c->U.as<T>(),
-            c->curl_u.as<T>(), c->B.as<T>(), c->dt, nullptr, guard);
-        ev_end(c);
-        ev_begin(c, "sell_n_kernel");
-        TF_LAUNCH_NB(c->tune[TK_N].unroll, sell_n_kernel, T, gb_n, c->stream, c->N.vals<T>(), c->B.as<T>(),
+        launch_timed(c, "dga_curl_kernel", dga_curl_kernel<T>, blocks_for(c->F), c->F, c->c_col.as<int32_t>(),
+                     c->c_fixed_signs ? nullptr : c->c_sg.as<int8_t>(), c->U.as<T>(), c->curl_u.as<T>(),
+                     c->B.as<T>(), c->dt, (const double*)nullptr, guard);
+        TF_LAUNCH_TIMED(c, "sell_n_kernel", c->tune[TK_N].unroll, sell_n_kernel, T, gb_n, c->N.vals<T>(), c->B.as<T>(),
                      c->Fv.as<T>(), c->dt, 1, nullptr, guard);
-        ev_end(c);
         if (fresh) g.sample_slot = allow_print(c, t - c->dt) ? sample_reserve(c, t - c->dt) : -1;
         sample_launch<T>(c, g.sample_slot, guard);
         if (lossy) {
             // U_old.SigMat U_old of the losses (D:1519)
-            ev_begin(c, "quad_form_kernel");
-            quad_form_kernel<T, 8><<<loss_blocks, kBlock, 0, c->stream>>>(
+            launch_timed(c, "quad_form_kernel", quad_form_kernel<T, 8>, loss_blocks, 
                 c->Sig.vals<T>(), c->U.as<T>(), c->loss_part.as<double>() + k * loss_blocks, guard);
-            ev_end(c);
         }
         if (c->n_inj) {
-            ev_begin(c, "gf_inject_kernel");
-            gf_inject_kernel<T><<<inj_blocks, kBlock, 0, c->stream>>>(
+            launch_timed(c, "gf_inject_kernel", gf_inject_kernel<T>, inj_blocks, 
                 c->n_inj, c->inj_edge.as<int32_t>(), c->inj_kind.as<int8_t>(), c->inj_tb.as<int32_t>(),
                 c->inj_te.as<int32_t>(), c->term_src.as<int32_t>(), c->term_w.as<double>(), cs, i > 1 ? cp : nullptr,
                 c->fb.as<T>(), c->dir_new.as<T>(), c->U.as<T>(), c->poy.as<double>() + k * inj_blocks, guard);
-            ev_end(c);
         }
         CgSolve s0;
         s0.init = init;
@@ -1646,11 +1641,9 @@ void run_gfdtd(tf_ctx* c, int64_t first, int64_t n, const tf_cg_opts* cg, tf_sta
         cg_start<T>(c, s0, true, guard);
     };
     auto scatter = [&](const CgState* guard) {
-        ev_begin(c, "gf_scatter_kernel");
-        gf_scatter_kernel<T><<<blocks_for(nf + c->n_dir), kBlock, 0, c->stream>>>(
-            nf, c->free_edge.as<int32_t>(), c->x.as<T>(), c->U.as<T>(), c->n_dir, c->dir_edge.as<int32_t>(),
-            c->dir_new.as<T>(), guard);
-        ev_end(c);
+        launch_timed(c, "gf_scatter_kernel", gf_scatter_kernel<T>, blocks_for(nf + c->n_dir), nf,
+                     c->free_edge.as<int32_t>(), c->x.as<T>(), c->U.as<T>(), c->n_dir, c->dir_edge.as<int32_t>(),
+                     c->dir_new.as<T>(), guard);
     };
     // collection captures of the solve of the current step
     auto collection_snaps = [&]() {
