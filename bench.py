@@ -43,9 +43,13 @@ def build_workload(d, method, n, steps, contrast=False):
     """The 48^3 workload (block 8, seed 2, eps_r U[1,10]); contrast=True is the
     high-contrast variant (6^3-cube blocks, seed 5, eps_r log-uniform [1,80])."""
     from tetfit_amd import Model, configs
+    log(f"[bench] building {method} {n}^3{' (contrast)' if contrast else ''}")
     if contrast:
+        # dga: dt from the host's stability estimate within the same build
+        # (one host build: the explicit operator assembly at 96^3 takes minutes)
         path = configs.kuhn_cavity(d, n=n, method=method, n_steps=steps, n_probes=8, block=6, seed=5,
-                                   eps_range=(1.0, 80.0), log_eps=True, name=f"kuhn{n}_{method}_contrast")
+                                   eps_range=(1.0, 80.0), log_eps=True, name=f"kuhn{n}_{method}_contrast",
+                                   dt="courant" if method == "dga" else None)
     else:
         path = configs.kuhn_cavity(d, n=n, method=method, n_steps=steps, n_probes=8, block=8, seed=2,
                                    dt=DT48.get(method) if n == 48 else None)
@@ -70,6 +74,21 @@ def algorithmic_bytes(info, kernel, vb):
         return nnz * (vb + cb) + per_slice + 5 * vb * n
     if kernel == "cg_pg_kernel":     # u = G r' (bf16 G): r, s gathered, x r/w, p read, r', u written
         return nnz * (2 + cb) + per_slice + 7 * vb * n
+    raise KeyError(kernel)
+
+
+def dga_bytes(info, kernel, vb, n_edges, n_faces):
+    """Algorithmic bytes per launch of the explicit dga SELL kernels, same
+    rules: `info` is the storage of the kernel's operator (sell_info "h" or
+    "n"). sell_h: U += dt H curl_f (curl_f gathered over all edges, a row ->
+    edge index, U read and written per row); sell_n: F -= dt N curl_u (curl_u
+    gathered, F read and written)."""
+    n, nnz, cb = info["rows"], info["nnz"], info["col_bytes"]
+    per_slice = 12 * info["slices"]
+    if kernel == "sell_h_kernel":
+        return nnz * (vb + cb) + per_slice + vb * n_edges + 4 * n + 2 * vb * n
+    if kernel == "sell_n_kernel":
+        return nnz * (vb + cb) + per_slice + vb * n_faces + 2 * vb * n
     raise KeyError(kernel)
 
 
@@ -135,6 +154,8 @@ def time_path(b, precision, steps, warmup, precond="jacobi", label="", learn_ste
     out["tuning"] = tuned
     if b.method == "gfdtd":
         out["A"] = sim.sell_info("a")
+    if b.method == "dga":
+        out["H"], out["N"] = sim.sell_info("h"), sim.sell_info("n")
     if learn is not None:
         learn.update(state_after=st["pc_state"], fallbacks=st["pc_fallbacks"])
         out["learned"] = learn
@@ -432,8 +453,8 @@ def main():
                             learn_steps=args.learn_steps)
         head, headline_label = (fast, "gfdtd fp32 learned") if fast is not none else (plain, "gfdtd fp64 jacobi")
         dga = None
-        if not args.no_dga and not contrast:
-            bd = build_workload(d, "dga", args.n, total)
+        if not args.no_dga:
+            bd = build_workload(d, "dga", args.n, total, contrast)
             dga = time_path(bd, "fp64", args.steps, args.warmup, "jacobi", "dga fp64 (explicit)")
             dga32 = time_path(bd, "fp32", args.steps, args.warmup, "jacobi", "dga fp32 (explicit)")
         vb = 4 if head is fast else 8
@@ -481,6 +502,19 @@ def main():
             line["dga_explicit"] = {"fp64_edge_dof_per_s": dga["edge_dof_per_s"], "fp64_ms_per_step": dga["ms_per_step"],
                                     "fp32_edge_dof_per_s": dga32["edge_dof_per_s"],
                                     "kernels_fp64": dga["kernels"]}
+            droof = []
+            for k, which in (("sell_h_kernel", "H"), ("sell_n_kernel", "N")):
+                for r, vb_d in ((dga, 8), (dga32, 4)):
+                    t = r["kernels"].get(k)
+                    if not t:
+                        continue
+                    nb = dga_bytes(r[which], k, vb_d, bd.n_edges, bd.n_faces)
+                    ach = nb / (t["ms_avg"] * 1e-3) / 1e9
+                    droof.append(dict(kernel=k, dtype="f64" if vb_d == 8 else "f32", bound="hbm",
+                                      achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                                      frac=round(ach / HBM_PEAK_GBS, 4), bytes_per_launch=int(nb),
+                                      ms_per_launch=t["ms_avg"]))
+            line["dga_explicit"]["roofline_kernels"] = droof
         line["kernels"] = head["kernels"]
         print(json.dumps(line), flush=True)
 
