@@ -207,15 +207,17 @@ def batch_workload(d, count=256, steps=2000, seed=4, workers=16):
     return builds
 
 
-def fit_bytes(b, kernel, vb):
+def fit_bytes(b, kernel, vb, grid_order=True):
     """Algorithmic bytes per launch of the FIT kernels: E phase reads U, ca,
-    cb, the packed faces (16 B) and sign byte per edge, writes U, gathers F
-    once; H phase reads B, cm, cn, packed edges and signs per face, writes B
-    and F, gathers U once."""
+    cb and the per-edge code, writes U, gathers F once; H phase reads B, cm,
+    cn and the per-face code, writes B and F, gathers U once. The code is
+    2 bytes (slot order and signs) in grid order, 17 bytes (packed faces or
+    edges and a sign byte) in the index-table layout."""
     E, F = b.n_edges, b.n_faces
+    cb = 2 if grid_order else 17
     if kernel == "fit_e_kernel":
-        return E * (4 * vb + 17) + F * vb
-    return F * (5 * vb + 17) + E * vb
+        return E * (4 * vb + cb) + F * vb
+    return F * (5 * vb + cb) + E * vb
 
 
 def bench_fit(args):
@@ -231,6 +233,7 @@ def bench_fit(args):
         res = {}
         for prec in ("fp64", "fp32"):
             sim = Simulation(b, prec, output=False)
+            grid_order = sim.fit_grid_order()
             sim.run(n=args.warmup)
             st = sim.run(n=steps)
             sim.set_profiling(True)
@@ -245,7 +248,7 @@ def bench_fit(args):
         for kname in ("fit_e_kernel", "fit_h_kernel"):
             k = res["fp64"]["kernels"].get(kname)
             if k:
-                nb = fit_bytes(b, kname, 8)
+                nb = fit_bytes(b, kname, 8, grid_order)
                 ach = nb / (k["ms_avg"] * 1e-3) / 1e9
                 roof[kname] = dict(bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                                    frac=round(ach / HBM_PEAK_GBS, 4), traffic=measured_traffic(kname, 8, nb),

@@ -200,6 +200,13 @@ tf_status tf_set_tuning(tf_ctx* ctx, const int32_t* blocks, const int32_t* unrol
  * offset, 4: int32). No reference counterpart. */
 tf_status tf_sell_info(tf_ctx* ctx, int32_t which, int64_t* info);
 
+/* FIT storage of a context: *grid_order = 1 when fields and coefficients sit
+ * at the grid positions of their edges and faces (the problem carries
+ * edge_sid / face_sid and every common edge and tbc face has exactly its
+ * grid neighbours; the kernels then need no face / edge index tables), 0 for
+ * the index-table layout. For byte accounting. No reference counterpart. */
+tf_status tf_fit_layout(tf_ctx* ctx, int32_t* grid_order);
+
 /* ---- batches ----------------------------------------------------------
  * Many independent dga simulations (uneven meshes, materials, sources and
  * time steps) stepped together: the problems are concatenated into one

@@ -69,6 +69,17 @@ typedef struct {
      * Wh = 1/2 B.(nu_vec B); ep_vec = eps_bar/L (D:5596), nu_vec = nu_bar/A (D:5507) */
     const double*  ep_vec;
     const double*  nu_vec;
+    /* structured position of every edge and face on the nx x ny x nz cell
+     * grid (or NULL): x edges (i<nx, j<=ny, k<=nz) at ((k(ny+1)+j)nx+i), then
+     * y edges (i<=nx, j<ny, k<=nz) at ((k ny+j)(nx+1)+i), then z edges
+     * (i<=nx, j<=ny, k<nz) at ((k(ny+1)+j)(nx+1)+i); x faces (i<=nx, j<ny,
+     * k<nz) at ((k ny+j)(nx+1)+i), then y faces (i<nx, j<=ny, k<nz) at
+     * ((k(ny+1)+j)nx+i), then z faces (i<nx, j<ny, k<=nz) at ((k ny+j)nx+i).
+     * Lets the device store the fields in grid order and find an edge's
+     * faces (and a face's edges) from its position instead of index tables. */
+    const int32_t* edge_sid;
+    const int32_t* face_sid;
+    int32_t        cells[3];
 } tf_fit_ops;
 
 /* DGA operators (methods dga and gfdtd). */

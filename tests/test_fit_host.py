@@ -239,3 +239,42 @@ def test_random_media_and_lossy_coefficients(tmp_path):
     np.testing.assert_allclose(f["M_q"][lossy], (ebar[lossy] - 0.5 * b.dt * sbar[lossy]) / h, rtol=1e-13)
     # the draws really differ per cell
     assert np.unique(np.round(f["M_q"][~lossy] * h / (EPS0 * h * h), 12)).size > 10
+
+
+def test_grid_positions(tmp_path):
+    """edge_sid / face_sid place every edge and face at its grid position
+    (the layout documented in tetfit_problem.h), computed here from the node
+    coordinates: a permutation of the edges and of the faces, on a
+    non-cubic grid."""
+    p = configs.fit_cavity(str(tmp_path), n=5, step=0.1, n_steps=4, n_probes=1, name="gp")
+    s = open(p).read().replace("ygrid {0,0.1,0.525}", "ygrid {0,0.1,0.725}").replace("zgrid {0,0.1,0.525}",
+                                                                                    "zgrid {0,0.1,0.325}")
+    open(p, "w").write(s)
+    b = Model(p).build(1)
+    o, t = b.fit_ops(), b.fit_tables()
+    nx, ny, nz = o["cells"]
+    assert (nx, ny, nz) == (5, 7, 3)
+    pts, g = t["pts"], t["g_nodes"]
+    h = t["L"]
+    lo = pts.min(axis=0)
+    a0, a1 = pts[g[:, 0]], pts[g[:, 1]]
+    axis = np.argmax(np.abs(a1 - a0), axis=1)
+    ijk = np.rint((np.minimum(a0, a1) - lo) / h).astype(np.int64)
+    i, j, k = ijk.T
+    nEx, nEy = nx * (ny + 1) * (nz + 1), (nx + 1) * ny * (nz + 1)
+    want = np.where(axis == 0, (k * (ny + 1) + j) * nx + i,
+                    np.where(axis == 1, nEx + (k * ny + j) * (nx + 1) + i,
+                             nEx + nEy + (k * (ny + 1) + j) * (nx + 1) + i))
+    np.testing.assert_array_equal(o["edge_sid"], want)
+    assert sorted(o["edge_sid"]) == list(range(b.n_edges))
+    # faces: the lowest corner of the face's four edges and its normal axis
+    ce = o["c_edge"]
+    fl = np.minimum.reduce([np.minimum(pts[g[ce[:, q], 0]], pts[g[ce[:, q], 1]]) for q in range(4)])
+    fh = np.maximum.reduce([np.maximum(pts[g[ce[:, q], 0]], pts[g[ce[:, q], 1]]) for q in range(4)])
+    normal = np.argmin(fh - fl, axis=1)
+    i, j, k = np.rint((fl - lo) / h).astype(np.int64).T
+    nFx, nFy = (nx + 1) * ny * nz, nx * (ny + 1) * nz
+    want = np.where(normal == 0, (k * ny + j) * (nx + 1) + i,
+                    np.where(normal == 1, nFx + (k * (ny + 1) + j) * nx + i, nFx + nFy + (k * ny + j) * nx + i))
+    np.testing.assert_array_equal(o["face_sid"], want)
+    assert sorted(o["face_sid"]) == list(range(b.n_faces))

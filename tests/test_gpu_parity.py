@@ -240,3 +240,43 @@ def test_int32_columns_parity(workdir, method, monkeypatch):
     assert sim.sell_info("n")["col_bytes"] == 4
     sim.close()
     check_run(b)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_fit_grid_order_matches_indexed(workdir, prec, monkeypatch):
+    """The FIT step in grid order (faces of an edge and edges of a face found
+    from their grid positions) against the index-table layout on a non-cubic
+    grid with an h source, a j dipole and an e source: probes, Poynting
+    values, final fields and energy bitwise equal (both sum each edge's four
+    faces in the reference's order) and the energy to rounding; the grid
+    order holds less memory."""
+    import re
+    p = configs.fit_cavity(str(workdir), n=10, step=0.1, n_steps=80, n_probes=5, name=f"fgo_{prec}")
+    s = open(p).read().replace("ygrid {0,0.1,1.025}", "ygrid {0,0.1,0.825}")
+    blocks = ("DEFINE source 2\n    SET type j\n    SET location {0.55,0.45,0.35}\n    SET direction z\n"
+              "    SET carrier ricker\n    SET frequency 3e8\n    SET delay 5e-9\nEND source 2\n"
+              "DEFINE source 3\n    SET type e\n    SET surface 4\n    SET direction y\n    SET carrier gaussian\n"
+              "    SET width 5e-9\n    SET delay 1e-8\nEND source 3\n")
+    s = s.replace("DEFINE output 1", blocks + "DEFINE output 1", 1)
+    s = s.replace("DEFINE simulation 1\n", "DEFINE simulation 1\n    SET source 2\n    SET source 3\n", 1)
+    s = re.sub(r"SET surfaces \{[^}]*\}", "SET surfaces {1,3,5,6}", s, count=1)
+    open(p, "w").write(s)
+    b = Model(p).build(1)
+    out = {}
+    for indexed in (False, True):
+        monkeypatch.setenv("TETFIT_FIT_INDEXED", "1" if indexed else "0")
+        print(f"case: fit {prec} indexed={indexed} E={b.n_edges}", flush=True)
+        sim = Simulation(b, prec)
+        nbytes = sim.device_bytes()
+        sim.run()
+        t, smp = sim.probes()
+        U, F, B = sim.state()
+        out[indexed] = (nbytes, t, smp, U, F, B, sim.poynting(), sim.energy())
+        sim.close()
+    g, ix = out[False], out[True]
+    assert g[0] < ix[0]
+    for a, c in zip(g[1:7], ix[1:7]):
+        np.testing.assert_array_equal(np.asarray(a), np.asarray(c))
+    # the energy sums its terms in device order, which differs between layouts
+    np.testing.assert_allclose(g[7], ix[7], rtol=1e-13)
+    assert np.abs(g[2]).max() > 0

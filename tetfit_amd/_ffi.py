@@ -31,7 +31,8 @@ class tf_fit_ops(C.Structure):
                 ("uncommon_edges", i32p), ("n_uncommon", C.c_int64),
                 ("bc_edges", i32p), ("n_bc", C.c_int64), ("bc_value", f64p),
                 ("src_edges", i32p), ("n_src", C.c_int64),
-                ("tbc_faces", i32p), ("n_tbc", C.c_int64), ("ep_vec", f64p), ("nu_vec", f64p)]
+                ("tbc_faces", i32p), ("n_tbc", C.c_int64), ("ep_vec", f64p), ("nu_vec", f64p),
+                ("edge_sid", i32p), ("face_sid", i32p), ("cells", C.c_int32 * 3)]
 
 
 class tf_dga_ops(C.Structure):
@@ -197,6 +198,7 @@ def hip_lib():
     L.tf_autotune.argtypes = [vp]
     L.tf_get_tuning.argtypes = [vp, i32p, i32p, C.c_int32, i32p]
     L.tf_sell_info.argtypes = [vp, C.c_int32, i64p]
+    L.tf_fit_layout.argtypes = [vp, i32p]
     L.tf_set_tuning.argtypes = [vp, i32p, i32p, C.c_int32]
     L.tf_set_graphs.argtypes = [vp, C.c_int32]
     for n in ["tf_create", "tf_set_sources", "tf_set_probes", "tf_set_output", "tf_run",
@@ -204,7 +206,7 @@ def hip_lib():
               "tf_energy", "tf_kernel_times", "tf_set_profiling", "tf_device_bytes", "tf_destroy",
               "tf_mlp_grad", "tf_create_batch", "tf_batch_set_sources", "tf_batch_set_probes",
               "tf_batch_set_output", "tf_batch_read_probes", "tf_batch_read_state", "tf_autotune",
-              "tf_get_tuning", "tf_sell_info", "tf_set_tuning", "tf_set_graphs"]:
+              "tf_get_tuning", "tf_sell_info", "tf_set_tuning", "tf_set_graphs", "tf_fit_layout"]:
         getattr(L, n).restype = C.c_int32
     _hip = L
     return L

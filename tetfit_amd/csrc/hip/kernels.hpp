@@ -54,6 +54,115 @@ __device__ __forceinline__ bool cg_guard_ok(const CgState* g) {
 
 // ------------------------------------------------------------------ FIT
 //
+// Grid order (tf_fit_ops edge_sid / face_sid): with the fields stored at
+// the grid positions of their edges and faces, the four faces around an
+// interior edge and the four edges of a face follow from the position.
+// Slots: x edge (i,j,k): y faces (i,j,k-1), (i,j,k), z faces (i,j-1,k),
+// (i,j,k); y edge: x faces (i,j,k-1), (i,j,k), z faces (i-1,j,k), (i,j,k);
+// z edge: x faces (i,j-1,k), (i,j,k), y faces (i-1,j,k), (i,j,k). x face
+// (i,j,k): y edges (i,j,k), (i,j,k+1), z edges (i,j,k), (i,j+1,k); y face:
+// x edges (i,j,k), (i,j,k+1), z edges (i,j,k), (i+1,j,k); z face: x edges
+// (i,j,k), (i,j+1,k), y edges (i,j,k), (i+1,j,k). Slots of boundary
+// entities may lie outside the grid; those are never read (code 0).
+// Division of n < 2^31 by a fixed divisor d >= 1 with a multiply-high and a
+// shift (Granlund and Montgomery, "Division by invariant integers using
+// multiplication", PLDI 1994): l = ceil(log2 d), m = 2^32 (2^l - d) / d + 1,
+// n / d = (mulhi(m, n) + n) >> l; mulhi(m, n) < n, so the sum stays below
+// 2^32.
+struct FastDiv {
+    uint32_t d, m, l;
+};
+inline FastDiv make_fastdiv(uint32_t d) {
+    uint32_t l = 0;
+    while (((uint64_t)1 << l) < d) ++l;
+    const uint64_t m = ((((uint64_t)1 << l) - d) << 32) / d + 1;
+    return FastDiv{d, (uint32_t)m, l};
+}
+__host__ __device__ inline uint32_t fastdiv(const FastDiv& f, uint32_t n) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const uint32_t t = __umulhi(f.m, n);
+#else
+    const uint32_t t = (uint32_t)(((uint64_t)f.m * n) >> 32);
+#endif
+    return (t + n) >> f.l;
+}
+
+struct FitGrid {
+    int32_t nx, ny, nz;
+    FastDiv dx, dx1, dy, dy1;  // by nx, nx + 1, ny, ny + 1
+};
+inline FitGrid make_fit_grid(int32_t nx, int32_t ny, int32_t nz) {
+    return FitGrid{nx, ny, nz, make_fastdiv(nx), make_fastdiv(nx + 1), make_fastdiv(ny), make_fastdiv(ny + 1)};
+}
+// n = q d + r with the fixed divisor of f
+__host__ __device__ inline void fit_divmod(const FastDiv& f, uint32_t n, uint32_t& q, uint32_t& r) {
+    q = fastdiv(f, n);
+    r = n - q * f.d;
+}
+// 32-bit index arithmetic: grid order is used only when edge and face
+// counts are below 2^31 (fit_grid_codes).
+__host__ __device__ inline void fit_edge_slots(const FitGrid g, uint32_t s, int32_t f[4]) {
+    const uint32_t nx = g.nx, ny = g.ny, nz = g.nz;
+    const uint32_t nEx = nx * (ny + 1) * (nz + 1), nEy = (nx + 1) * ny * (nz + 1);
+    const uint32_t nFx = (nx + 1) * ny * nz, nFy = nx * (ny + 1) * nz;
+    auto xf = [&](uint32_t i, uint32_t j, uint32_t k) { return (int32_t)((k * ny + j) * (nx + 1) + i); };
+    auto yf = [&](uint32_t i, uint32_t j, uint32_t k) { return (int32_t)(nFx + (k * (ny + 1) + j) * nx + i); };
+    auto zf = [&](uint32_t i, uint32_t j, uint32_t k) { return (int32_t)(nFx + nFy + (k * ny + j) * nx + i); };
+    // out-of-grid slots (boundary edges) wrap around in unsigned arithmetic;
+    // they are computed but never read
+    uint32_t i, j, k, t;
+    if (s < nEx) {
+        fit_divmod(g.dx, s, t, i);
+        fit_divmod(g.dy1, t, k, j);
+        f[0] = yf(i, j, k - 1); f[1] = yf(i, j, k); f[2] = zf(i, j - 1, k); f[3] = zf(i, j, k);
+    } else if (s < nEx + nEy) {
+        fit_divmod(g.dx1, s - nEx, t, i);
+        fit_divmod(g.dy, t, k, j);
+        f[0] = xf(i, j, k - 1); f[1] = xf(i, j, k); f[2] = zf(i - 1, j, k); f[3] = zf(i, j, k);
+    } else {
+        fit_divmod(g.dx1, s - nEx - nEy, t, i);
+        fit_divmod(g.dy1, t, k, j);
+        f[0] = xf(i, j - 1, k); f[1] = xf(i, j, k); f[2] = yf(i - 1, j, k); f[3] = yf(i, j, k);
+    }
+}
+__host__ __device__ inline void fit_face_slots(const FitGrid g, uint32_t s, int32_t e[4]) {
+    const uint32_t nx = g.nx, ny = g.ny, nz = g.nz;
+    const uint32_t nEx = nx * (ny + 1) * (nz + 1), nEy = (nx + 1) * ny * (nz + 1);
+    const uint32_t nFx = (nx + 1) * ny * nz, nFy = nx * (ny + 1) * nz;
+    auto xe = [&](uint32_t i, uint32_t j, uint32_t k) { return (int32_t)((k * (ny + 1) + j) * nx + i); };
+    auto ye = [&](uint32_t i, uint32_t j, uint32_t k) { return (int32_t)(nEx + (k * ny + j) * (nx + 1) + i); };
+    auto ze = [&](uint32_t i, uint32_t j, uint32_t k) {
+        return (int32_t)(nEx + nEy + (k * (ny + 1) + j) * (nx + 1) + i);
+    };
+    uint32_t i, j, k, t;
+    if (s < nFx) {
+        fit_divmod(g.dx1, s, t, i);
+        fit_divmod(g.dy, t, k, j);
+        e[0] = ye(i, j, k); e[1] = ye(i, j, k + 1); e[2] = ze(i, j, k); e[3] = ze(i, j + 1, k);
+    } else if (s < nFx + nFy) {
+        fit_divmod(g.dx, s - nFx, t, i);
+        fit_divmod(g.dy1, t, k, j);
+        e[0] = xe(i, j, k); e[1] = xe(i, j, k + 1); e[2] = ze(i, j, k); e[3] = ze(i + 1, j, k);
+    } else {
+        fit_divmod(g.dx, s - nFx - nFy, t, i);
+        fit_divmod(g.dy, t, k, j);
+        e[0] = xe(i, j, k); e[1] = xe(i, j + 1, k); e[2] = ye(i, j, k); e[3] = ye(i + 1, j, k);
+    }
+}
+
+// Sum over the four positions p of sign_p * v[slot_p] in position order,
+// with code = slot map (2 bits per position) << 8 | sign code (2 bits per
+// position: 1 -> +1, 2 -> -1, 0 -> 0): the reference's F0 - F1 + F2 - F3
+// order, whatever slot each of its faces sits in.
+__device__ __forceinline__ double slot_sum(uint32_t code, const double v[4]) {
+    const uint32_t m = code >> 8;
+    double c = sgn2(code & 3) * v[m & 3];
+    c += sgn2((code >> 2) & 3) * v[(m >> 2) & 3];
+    c += sgn2((code >> 4) & 3) * v[(m >> 4) & 3];
+    c += sgn2((code >> 6) & 3) * v[(m >> 6) & 3];
+    return c;
+}
+
 // E phase (D:1850-1895). Main blocks cover every edge in index order; edges
 // whose sign byte is 0 (pec / source / non-common) are left to the special
 // blocks that follow. U = ca*U_old + cb*(sum_k s_k F[f_k])
@@ -62,10 +171,14 @@ __device__ __forceinline__ bool cg_guard_ok(const CgState* g) {
 //   kind 0 (h):  U = ca*U_old + cb*(sum s_k F + src)     (ca = M_h M_q, cb = M_h dt)
 //   kind 1 (e):  U = src
 //   kind 2 (j):  U = ca*U_old + cb*(sum s_k F - src)     (common coefficients)
+// With code16 (grid order) the main blocks take each edge's faces from its
+// position and its sign and order code from code16 (slot_sum); otherwise
+// from the packed face quadruple f4 and the sign byte sg.
 template <typename T>
 __global__ void __launch_bounds__(kBlock) fit_e_kernel(
     int32_t n_edges, int32_t main_blocks, const uint8_t* __restrict__ sg, const T* __restrict__ ca,
-    const T* __restrict__ cb, const int4* __restrict__ f4, const T* __restrict__ Fv, T* __restrict__ U,
+    const T* __restrict__ cb, const int4* __restrict__ f4, const uint16_t* __restrict__ code16, FitGrid grid,
+    const T* __restrict__ Fv, T* __restrict__ U,
     int32_t n_sp, const int32_t* __restrict__ sp_edge, const int4* __restrict__ sp_f4,
     const uint8_t* __restrict__ sp_sg, const double* __restrict__ sp_ca, const double* __restrict__ sp_cb,
     const int8_t* __restrict__ sp_kind, const int32_t* __restrict__ sp_tb, const int32_t* __restrict__ sp_te,
@@ -75,6 +188,15 @@ __global__ void __launch_bounds__(kBlock) fit_e_kernel(
     if (lb < main_blocks) {
         const int32_t e = lb * kBlock + threadIdx.x;
         if (e >= n_edges) return;
+        if (code16) {
+            const uint32_t code = code16[e];
+            if (code == 0) return;
+            int32_t f[4];
+            fit_edge_slots(grid, (uint32_t)e, f);
+            const double v[4] = {(double)Fv[f[0]], (double)Fv[f[1]], (double)Fv[f[2]], (double)Fv[f[3]]};
+            U[e] = (T)((double)ca[e] * (double)U[e] + (double)cb[e] * slot_sum(code, v));
+            return;
+        }
         const uint32_t s = sg[e];
         if (s == 0) return;
         const int4 f = f4[e];
@@ -115,13 +237,26 @@ __global__ void __launch_bounds__(kBlock) fit_e_kernel(
 
 // H phase over tbc faces (D:1908-1915): B = cm*F_old - dt*sum s_k U[e_k];
 // F = cn*B with cm = M_mu*fpml, cn = fpml*M_nu, s = curl*(+,-,+,-).
+// code16 / grid as in fit_e_kernel.
 template <typename T>
 __global__ void __launch_bounds__(kBlock) fit_h_kernel(int32_t n_faces, const uint8_t* __restrict__ sg,
                                                        const T* __restrict__ cm, const T* __restrict__ cn,
-                                                       const int4* __restrict__ e4, const T* __restrict__ U,
+                                                       const int4* __restrict__ e4, const uint16_t* __restrict__ code16,
+                                                       FitGrid grid, const T* __restrict__ U,
                                                        T* __restrict__ Fv, T* __restrict__ B, double dt) {
     const int32_t f = xcd_block() * kBlock + threadIdx.x;
     if (f >= n_faces) return;
+    if (code16) {
+        const uint32_t code = code16[f];
+        if (code == 0) return;
+        int32_t e[4];
+        fit_face_slots(grid, (uint32_t)f, e);
+        const double v[4] = {(double)U[e[0]], (double)U[e[1]], (double)U[e[2]], (double)U[e[3]]};
+        const double b = (double)cm[f] * (double)Fv[f] - dt * slot_sum(code, v);
+        B[f] = (T)b;
+        Fv[f] = (T)((double)cn[f] * b);
+        return;
+    }
     const uint32_t s = sg[f];
     if (s == 0) return;
     const int4 e = e4[f];

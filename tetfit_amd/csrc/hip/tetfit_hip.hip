@@ -156,6 +156,13 @@ struct tf_ctx {
     std::vector<uint8_t> fit_sp_sg_u;
     std::vector<double> fit_sp_ca_u, fit_sp_cb_u;
     std::vector<int4> h_fe_f4;
+    // FIT grid order: fields, coefficients and codes stored at the grid
+    // positions of edges and faces (fit_edge_slots / fit_face_slots)
+    bool fit_grid = false;
+    FitGrid grid{};
+    std::vector<int32_t> h_esid, h_fsid;  // reference edge / face -> grid position
+    std::vector<uint16_t> h_fe_code;      // per grid edge: slot map << 8 | signs (0: not in the main pass)
+    DevBuf fe_code, fh_code;
     std::vector<double> h_ca_common, h_cb_common;
     DevBuf sp_edge, sp_f4, sp_sg, sp_ca, sp_cb, sp_kind, sp_tb, sp_te;
     int32_t n_sp = 0;
@@ -269,7 +276,7 @@ tf_ctx::~tf_ctx() {
 
 size_t tf_ctx::device_bytes() const {
     size_t s = 0;
-    const DevBuf* all[] = {&U, &Fv, &B, &fit_ep, &fit_nu, &fe_sg, &fe_ca, &fe_cb, &fe_f4, &fh_sg, &fh_cm, &fh_cn, &fh_e4,
+    const DevBuf* all[] = {&U, &Fv, &B, &fit_ep, &fit_nu, &fe_sg, &fe_ca, &fe_cb, &fe_f4, &fh_sg, &fh_cm, &fh_cn, &fh_e4, &fe_code, &fh_code,
                            &sp_edge, &sp_f4, &sp_sg, &sp_ca, &sp_cb, &sp_kind, &sp_tb, &sp_te, &c_col, &c_sg,
                            &h_edge, &curl_f, &curl_u, &free_edge, &dir_edge, &dir_new, &fb, &pre, &x, &r, &rs0, &rs1, &z,
                            &p0, &p1, &q, &r0, &cg_part, &cg_state, &inj_edge, &inj_kind, &inj_tb, &inj_te,
@@ -444,6 +451,10 @@ void zero_field(DevBuf& b, int64_t n, hipStream_t s) {
     TF_HIP(hipMemsetAsync(b.p, 0, b.bytes, s));
 }
 
+// Device position of a reference edge / face (grid order or identity).
+int32_t dev_edge(const tf_ctx* c, int32_t e) { return c->fit_grid ? c->h_esid[e] : e; }
+int32_t dev_face(const tf_ctx* c, int32_t f) { return c->fit_grid ? c->h_fsid[f] : f; }
+
 uint8_t code4(const int s[4]) {
     uint8_t c = 0;
     for (int k = 0; k < 4; ++k) c |= (uint8_t)((s[k] > 0 ? 1 : (s[k] < 0 ? 3 : 0)) << (2 * k));
@@ -513,13 +524,100 @@ void ev_flush(tf_ctx* c) {
 
 // ---------------------------------------------------------------- create
 
+// Grid order for a FIT problem that carries grid positions: the slot map of
+// every common edge and tbc face (which slot holds the reference's face /
+// edge at each position, fit_edge_slots / fit_face_slots). False (the
+// index-table layout is used) if any of them does not have exactly its
+// slots' entities, or $TETFIT_FIT_INDEXED is set to a non-empty value other
+// than 0.
+bool fit_grid_codes(tf_ctx* c, const tf_fit_ops& f, std::vector<uint16_t>& ecode, std::vector<uint16_t>& fcode) {
+    const char* forced = std::getenv("TETFIT_FIT_INDEXED");
+    if (forced && *forced && std::strcmp(forced, "0") != 0) return false;
+    if (!f.edge_sid || !f.face_sid) return false;
+    const int32_t E = c->E, F = c->F;
+    if (f.cells[0] <= 0 || f.cells[1] <= 0 || f.cells[2] <= 0) return false;
+    const FitGrid g = make_fit_grid(f.cells[0], f.cells[1], f.cells[2]);
+    // the divisions by invariant integers, checked against plain division
+    for (const FastDiv* fd : {&g.dx, &g.dx1, &g.dy, &g.dy1})
+        for (uint32_t n : {0u, 1u, fd->d - 1, fd->d, fd->d + 1, (uint32_t)E - 1, (uint32_t)F - 1, 0x7FFFFFFFu})
+            if (fastdiv(*fd, n) != n / fd->d) throw std::runtime_error("fast division self-check failed");
+    const int64_t nx = g.nx, ny = g.ny, nz = g.nz;
+    if (nx <= 0 || ny <= 0 || nz <= 0) return false;
+    if (nx * (ny + 1) * (nz + 1) + (nx + 1) * ny * (nz + 1) + (nx + 1) * (ny + 1) * nz != E) return false;
+    if ((int64_t)E >= ((int64_t)1 << 31) || (int64_t)F >= ((int64_t)1 << 31)) return false;
+    if ((nx + 1) * ny * nz + nx * (ny + 1) * nz + nx * ny * (nz + 1) != F) return false;
+    std::vector<int32_t> esid(f.edge_sid, f.edge_sid + E), fsid(f.face_sid, f.face_sid + F);
+    {
+        std::vector<uint8_t> seen_e(E, 0), seen_f(F, 0);
+        for (int32_t v : esid) { if (v < 0 || v >= E || seen_e[v]) return false; seen_e[v] = 1; }
+        for (int32_t v : fsid) { if (v < 0 || v >= F || seen_f[v]) return false; seen_f[v] = 1; }
+    }
+    auto map_of = [](const int32_t slot[4], const int64_t ref[4], uint32_t& m) {
+        m = 0;
+        for (int q = 0; q < 4; ++q) {
+            int hit = -1;
+            for (int k = 0; k < 4; ++k)
+                if (slot[k] == ref[q]) { if (hit >= 0) return false; hit = k; }
+            if (hit < 0) return false;
+            m |= (uint32_t)hit << (2 * q);
+        }
+        return true;
+    };
+    ecode.assign(E, 0);
+    for (int64_t q = 0; q < f.n_common; ++q) {
+        const int32_t e = f.common_edges[q];
+        if (f.ct_ptr[e + 1] - f.ct_ptr[e] != 4) return false;
+        int32_t slot[4];
+        int64_t ref[4];
+        fit_edge_slots(g, (uint32_t)esid[e], slot);
+        for (int k = 0; k < 4; ++k) ref[k] = fsid[f.ct_face[f.ct_ptr[e] + k]];
+        uint32_t m;
+        if (!map_of(slot, ref, m)) return false;
+        const int d = f.dual_curl[e];
+        const int sg[4] = {d, -d, d, -d};
+        ecode[esid[e]] = (uint16_t)(m << 8 | code4(sg));
+    }
+    fcode.assign(F, 0);
+    for (int64_t q = 0; q < f.n_tbc; ++q) {
+        const int32_t fi = f.tbc_faces[q];
+        int32_t slot[4];
+        int64_t ref[4];
+        fit_face_slots(g, (uint32_t)fsid[fi], slot);
+        for (int k = 0; k < 4; ++k) ref[k] = esid[f.c_edge[4 * (int64_t)fi + k]];
+        uint32_t m;
+        if (!map_of(slot, ref, m)) return false;
+        const int d = f.curl[fi];
+        const int sg[4] = {d, -d, d, -d};
+        fcode[fsid[fi]] = (uint16_t)(m << 8 | code4(sg));
+    }
+    c->grid = g;
+    c->h_esid = std::move(esid);
+    c->h_fsid = std::move(fsid);
+    return true;
+}
+
 template <class T>
 void create_fit(tf_ctx* c, const tf_problem* p) {
     const tf_fit_ops& f = p->fit;
     const int32_t E = c->E, F = c->F;
+    std::vector<uint16_t> ecode, fcode;
+    c->fit_grid = fit_grid_codes(c, f, ecode, fcode);
+    // per-entity arrays in device order
+    auto by_edge = [&](auto v) {
+        if (!c->fit_grid) return v;
+        auto o = v;
+        for (int32_t e = 0; e < E; ++e) o[c->h_esid[e]] = v[e];
+        return o;
+    };
+    auto by_face = [&](auto v) {
+        if (!c->fit_grid) return v;
+        auto o = v;
+        for (int32_t q = 0; q < F; ++q) o[c->h_fsid[q]] = v[q];
+        return o;
+    };
     if (f.ep_vec && f.nu_vec) {
-        upload(c->fit_ep, std::vector<double>(f.ep_vec, f.ep_vec + E));
-        upload(c->fit_nu, std::vector<double>(f.nu_vec, f.nu_vec + F));
+        upload(c->fit_ep, by_edge(std::vector<double>(f.ep_vec, f.ep_vec + E)));
+        upload(c->fit_nu, by_face(std::vector<double>(f.nu_vec, f.nu_vec + F)));
     }
     const double dt = p->dt;
     std::vector<uint8_t> sg(E, 0);
@@ -545,10 +643,17 @@ void create_fit(tf_ctx* c, const tf_problem* p) {
     }
     c->h_fe_sg = sg;
     c->h_fe_f4 = f4;
-    upload(c->fe_sg, sg);
-    upload(c->fe_ca, ca);
-    upload(c->fe_cb, cb);
-    upload(c->fe_f4, f4);
+    if (c->fit_grid) {
+        c->h_fe_code = ecode;
+        upload(c->fe_code, ecode);
+        upload(c->fe_ca, by_edge(ca));
+        upload(c->fe_cb, by_edge(cb));
+    } else {
+        upload(c->fe_sg, sg);
+        upload(c->fe_ca, ca);
+        upload(c->fe_cb, cb);
+        upload(c->fe_f4, f4);
+    }
     // special edges: uncommon (h source) and src edges (e / j)
     c->fit_uncommon.assign(f.uncommon_edges, f.uncommon_edges + f.n_uncommon);
     c->fit_src.assign(f.src_edges, f.src_edges + f.n_src);
@@ -583,10 +688,16 @@ void create_fit(tf_ctx* c, const tf_problem* p) {
         cm[fi] = (T)(f.M_mu[fi] * f.fpml[fi]);
         cn[fi] = (T)(f.fpml[fi] * f.M_nu[fi]);
     }
-    upload(c->fh_sg, hs);
-    upload(c->fh_cm, cm);
-    upload(c->fh_cn, cn);
-    upload(c->fh_e4, e4);
+    if (c->fit_grid) {
+        upload(c->fh_code, fcode);
+        upload(c->fh_cm, by_face(cm));
+        upload(c->fh_cn, by_face(cn));
+    } else {
+        upload(c->fh_sg, hs);
+        upload(c->fh_cm, cm);
+        upload(c->fh_cn, cn);
+        upload(c->fh_e4, e4);
+    }
 }
 
 // K = (E - dt/2 SigMat)_f - [A 0]: row k is the row of free edge fe[k] of
@@ -761,6 +872,14 @@ void build_fit_special(tf_ctx* c, const std::vector<std::vector<std::pair<int32_
     std::vector<double> ca, cb, tw;
     std::vector<int8_t> kind;
     std::vector<uint8_t> main_sg = c->h_fe_sg;
+    std::vector<uint16_t> main_code = c->h_fe_code;
+    auto off_main = [&](int32_t e) {  // the special item updates it
+        if (c->fit_grid) main_code[c->h_esid[e]] = 0;
+        else main_sg[e] = 0;
+    };
+    auto dev_f4 = [&](int4 q) {
+        return c->fit_grid ? int4{dev_face(c, q.x), dev_face(c, q.y), dev_face(c, q.z), dev_face(c, q.w)} : q;
+    };
     auto add_terms = [&](int32_t e) {
         tb.push_back((int32_t)tsrc.size());
         auto it = by_edge_terms.size() > (size_t)e ? &by_edge_terms[e] : nullptr;
@@ -770,8 +889,8 @@ void build_fit_special(tf_ctx* c, const std::vector<std::vector<std::pair<int32_
     };
     for (size_t i = 0; i < c->fit_uncommon.size(); ++i) {
         const int32_t e = c->fit_uncommon[i];
-        edge.push_back(e);
-        f4.push_back(c->fit_sp_f4_u[i]);
+        edge.push_back(dev_edge(c, e));
+        f4.push_back(dev_f4(c->fit_sp_f4_u[i]));
         sg.push_back(c->fit_sp_sg_u[i]);
         ca.push_back(c->fit_sp_ca_u[i]);
         cb.push_back(c->fit_sp_cb_u[i]);
@@ -780,22 +899,21 @@ void build_fit_special(tf_ctx* c, const std::vector<std::vector<std::pair<int32_
     }
     for (int32_t e : c->fit_src) {
         const int fld = edge_field.size() > (size_t)e ? edge_field[e] : TF_FIELD_E;
-        edge.push_back(e);
+        edge.push_back(dev_edge(c, e));
         if (fld == TF_FIELD_J) {
-            f4.push_back(c->h_fe_f4[e]);
+            f4.push_back(dev_f4(c->h_fe_f4[e]));
             sg.push_back(c->h_fe_sg[e]);
             ca.push_back(c->h_ca_common[e]);
             cb.push_back(c->h_cb_common[e]);
             kind.push_back(2);
-            main_sg[e] = 0;  // the special item updates it
         } else {
             f4.push_back(int4{0, 0, 0, 0});
             sg.push_back(0);
             ca.push_back(0);
             cb.push_back(0);
             kind.push_back(1);
-            main_sg[e] = 0;
         }
+        off_main(e);
         add_terms(e);
     }
     c->n_sp = (int32_t)edge.size();
@@ -810,7 +928,8 @@ void build_fit_special(tf_ctx* c, const std::vector<std::vector<std::pair<int32_
     if (tsrc.empty()) { tsrc.push_back(0); tw.push_back(0); }
     upload(c->term_src, tsrc);
     upload(c->term_w, tw);
-    upload(c->fe_sg, main_sg);
+    if (c->fit_grid) upload(c->fe_code, main_code);
+    else upload(c->fe_sg, main_sg);
 }
 
 void build_tet_inject(tf_ctx* c, const std::vector<std::vector<std::pair<int32_t, double>>>& by_edge_terms,
@@ -993,12 +1112,13 @@ void run_fit(tf_ctx* c, int64_t first, int64_t n, tf_stats* st) {
     auto step = [&](const double* cs, double* poy) {
         launch_timed(c, "fit_e_kernel", fit_e_kernel<T>, main_blocks + sp_blocks, 
             c->E, main_blocks, c->fe_sg.as<uint8_t>(), c->fe_ca.as<T>(), c->fe_cb.as<T>(), c->fe_f4.as<int4>(),
-            c->Fv.as<T>(), c->U.as<T>(), c->n_sp, c->sp_edge.as<int32_t>(), c->sp_f4.as<int4>(),
+            c->fit_grid ? c->fe_code.as<uint16_t>() : nullptr, c->grid, c->Fv.as<T>(), c->U.as<T>(), c->n_sp, c->sp_edge.as<int32_t>(), c->sp_f4.as<int4>(),
             c->sp_sg.as<uint8_t>(), c->sp_ca.as<double>(), c->sp_cb.as<double>(), c->sp_kind.as<int8_t>(),
             c->sp_tb.as<int32_t>(), c->sp_te.as<int32_t>(), c->term_src.as<int32_t>(), c->term_w.as<double>(), cs,
             sp_blocks ? poy : nullptr);
         launch_timed(c, "fit_h_kernel", fit_h_kernel<T>, blocks_for(c->F), c->F, c->fh_sg.as<uint8_t>(),
-                     c->fh_cm.as<T>(), c->fh_cn.as<T>(), c->fh_e4.as<int4>(), c->U.as<T>(), c->Fv.as<T>(),
+                     c->fh_cm.as<T>(), c->fh_cn.as<T>(), c->fh_e4.as<int4>(),
+                     c->fit_grid ? c->fh_code.as<uint16_t>() : nullptr, c->grid, c->U.as<T>(), c->Fv.as<T>(),
                      c->B.as<T>(), c->dt);
     };
     const int64_t g = graph_steps(c) ? run_graph_chunks(c, n, sp_blocks, [&](int, const double* cs, double* poy) {
@@ -2219,12 +2339,12 @@ tf_status tf_set_probes(tf_ctx* c, const tf_probe* probes, int32_t n) {
             d.nh = p.nh;
             for (int k = 0; k < p.ne; ++k) {
                 if (p.e_idx[k] < 0 || p.e_idx[k] >= c->E) throw std::runtime_error("probe edge out of range");
-                d.e_idx[k] = p.e_idx[k];
+                d.e_idx[k] = c->method == TF_METHOD_FDTD ? dev_edge(c, p.e_idx[k]) : p.e_idx[k];
                 for (int q = 0; q < 3; ++q) d.e_w[k][q] = p.e_w[k][q];
             }
             for (int k = 0; k < p.nh; ++k) {
                 if (p.h_idx[k] < 0 || p.h_idx[k] >= c->F) throw std::runtime_error("probe face out of range");
-                d.h_idx[k] = p.h_idx[k];
+                d.h_idx[k] = c->method == TF_METHOD_FDTD ? dev_face(c, p.h_idx[k]) : p.h_idx[k];
                 for (int q = 0; q < 3; ++q) d.h_w[k][q] = p.h_w[k][q];
             }
         }
@@ -2294,9 +2414,22 @@ tf_status tf_read_state(tf_ctx* c, double* U, double* F, double* B) {
                 for (int64_t i = 0; i < n; ++i) dst[i] = tmp[i];
             }
         };
-        rd(c->U, U, c->E);
-        rd(c->Fv, F, c->F);
-        rd(c->B, B, c->F);
+        // grid order (FIT): device position d of reference entity i
+        auto rd_perm = [&](const DevBuf& b, double* dst, int64_t n, const std::vector<int32_t>& pos) {
+            if (!dst) return;
+            std::vector<double> tmp(n);
+            rd(b, tmp.data(), n);
+            for (int64_t i = 0; i < n; ++i) dst[i] = tmp[pos[i]];
+        };
+        if (c->fit_grid) {
+            rd_perm(c->U, U, c->E, c->h_esid);
+            rd_perm(c->Fv, F, c->F, c->h_fsid);
+            rd_perm(c->B, B, c->F, c->h_fsid);
+        } else {
+            rd(c->U, U, c->E);
+            rd(c->Fv, F, c->F);
+            rd(c->B, B, c->F);
+        }
     });
 }
 
@@ -2313,9 +2446,21 @@ tf_status tf_write_state(tf_ctx* c, const double* U, const double* F, const doub
                 TF_HIP(hipMemcpy(b.p, tmp.data(), n * 4, hipMemcpyHostToDevice));
             }
         };
-        wr(c->U, U, c->E);
-        wr(c->Fv, F, c->F);
-        wr(c->B, B, c->F);
+        if (c->fit_grid) {
+            auto wr_perm = [&](DevBuf& b, const double* src, int64_t n, const std::vector<int32_t>& pos) {
+                if (!src) return;
+                std::vector<double> tmp(n);
+                for (int64_t i = 0; i < n; ++i) tmp[pos[i]] = src[i];
+                wr(b, tmp.data(), n);
+            };
+            wr_perm(c->U, U, c->E, c->h_esid);
+            wr_perm(c->Fv, F, c->F, c->h_fsid);
+            wr_perm(c->B, B, c->F, c->h_fsid);
+        } else {
+            wr(c->U, U, c->E);
+            wr(c->Fv, F, c->F);
+            wr(c->B, B, c->F);
+        }
         if (U) {
             c->ud_zero = true;
             for (int32_t e : c->h_dir_edge)
@@ -2506,6 +2651,12 @@ tf_status tf_autotune(tf_ctx* c) {
         if (c->prec == TF_PREC_FP64) autotune<double>(c);
         else autotune<float>(c);
     });
+}
+
+tf_status tf_fit_layout(tf_ctx* c, int32_t* grid_order) {
+    if (!c || !grid_order) { g_err = "null argument"; return TF_ERR_ARG; }
+    *grid_order = c->fit_grid ? 1 : 0;
+    return TF_OK;
 }
 
 tf_status tf_sell_info(tf_ctx* c, int32_t which, int64_t* info) {

@@ -56,6 +56,8 @@ struct Build {
     std::vector<int32_t> common_edges, uncommon_edges, bc_edges, src_edges, tbc_faces;
     std::vector<double> bc_value;
     std::vector<int32_t> e_cluster, d_faces, g_nodes;  // 12/cube, 6/cube, 2/edge
+    std::vector<int32_t> edge_sid, face_sid;           // grid positions (tf_fit_ops)
+    int32_t cells[3] = {0, 0, 0};
     std::vector<double> pts;                            // 3/node
     std::vector<int32_t> boundary_face;
     int32_t n_cubes = 0;

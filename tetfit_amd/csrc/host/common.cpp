@@ -220,6 +220,11 @@ void Build::finalize_views() {
         f.n_tbc = (int64_t)tbc_faces.size();
         f.ep_vec = M_e.data();
         f.nu_vec = Nu_vec.data();
+        if (!edge_sid.empty() && !face_sid.empty()) {
+            f.edge_sid = edge_sid.data();
+            f.face_sid = face_sid.data();
+            for (int a = 0; a < 3; ++a) f.cells[a] = cells[a];
+        }
     } else {
         tf_dga_ops& d = prob.dga;
         d = tf_dga_ops{};

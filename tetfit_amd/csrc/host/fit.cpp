@@ -108,6 +108,22 @@ void build_fit(const Model& m, const SimulationDef& sim, Build& b) {
     std::vector<uint8_t> Dt_n(totF, 0);
     std::vector<int32_t>& G = b.g_nodes;  // 2 per edge
     G.assign(totE * 2, -1);
+    // grid positions of edges and faces (tf_fit_ops edge_sid / face_sid)
+    b.cells[0] = (int32_t)Nx; b.cells[1] = (int32_t)Ny; b.cells[2] = (int32_t)Nz;
+    b.edge_sid.assign(totE, -1);
+    b.face_sid.assign(totF, -1);
+    const int64_t nEx = (int64_t)Nx * (Ny + 1) * (Nz + 1), nEy = (int64_t)(Nx + 1) * Ny * (Nz + 1);
+    const int64_t nFx = (int64_t)(Nx + 1) * Ny * Nz, nFy = (int64_t)Nx * (Ny + 1) * Nz;
+    auto edge_pos = [&](int a, int64_t i, int64_t j, int64_t k) -> int32_t {
+        if (a == 0) return (int32_t)((k * (Ny + 1) + j) * Nx + i);
+        if (a == 1) return (int32_t)(nEx + (k * Ny + j) * (Nx + 1) + i);
+        return (int32_t)(nEx + nEy + (k * (Ny + 1) + j) * (Nx + 1) + i);
+    };
+    auto face_pos = [&](int a, int64_t i, int64_t j, int64_t k) -> int32_t {
+        if (a == 0) return (int32_t)((k * Ny + j) * (Nx + 1) + i);
+        if (a == 1) return (int32_t)(nFx + (k * (Ny + 1) + j) * Nx + i);
+        return (int32_t)(nFx + nFy + (k * Ny + j) * Nx + i);
+    };
     std::vector<uint8_t> edge_has_G(totE, 0), face_has_C(totF, 0);
     b.c_edge.assign(totF * 4, 0);
     std::vector<int32_t> ct_face(totE * 4, 0);
@@ -177,6 +193,14 @@ void build_fit(const Model& m, const SimulationDef& sim, Build& b) {
                 for (int f = 0; f < 6; ++f) D.push_back(dcube[f]);
                 for (int e = 0; e < 12; ++e) E_cluster.push_back(ecube[e]);
                 for (int q = 0; q < 8; ++q) P_cluster.push_back(pcube[q]);
+                for (int e = 0; e < 12; ++e) {
+                    const int q0 = kEdgeNodes[e][0];  // the edge's lower node
+                    b.edge_sid[ecube[e]] = edge_pos(kEdgeAxis[e], i + (q0 & 1), j + (q0 >> 1 & 1), k + (q0 >> 2 & 1));
+                }
+                for (int f = 0; f < 6; ++f) {
+                    const int a = kFaceAxis[f], o = f >= 3 ? 1 : 0;
+                    b.face_sid[dcube[f]] = face_pos(a, i + (a == 0 ? o : 0), j + (a == 1 ? o : 0), k + (a == 2 ? o : 0));
+                }
 
                 for (int f = 0; f < 6; ++f) {
                     int32_t fid = dcube[f];

@@ -166,6 +166,12 @@ class Simulation:
             bl[k], un[k] = int(b), int(u)
         _check(hip_lib().tf_set_tuning(self._h, bl, un, len(names)))
 
+    def fit_grid_order(self):
+        """tf_fit_layout: True when the FIT fields sit in grid order."""
+        v = C.c_int32()
+        _check(hip_lib().tf_fit_layout(self._h, C.byref(v)))
+        return bool(v.value)
+
     def sell_info(self, which):
         """tf_sell_info: storage of a device operator ("ct", "h", "n", "a",
         "k" (gfdtd residual operator E_f - [A 0]), "rhs1", "ctf") as

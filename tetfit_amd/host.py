@@ -179,7 +179,9 @@ class Build:
             bc_edges=_arr(f.bc_edges, f.n_bc, np.int32), bc_value=_arr(f.bc_value, f.n_bc, np.float64),
             src_edges=_arr(f.src_edges, f.n_src, np.int32),
             tbc_faces=_arr(f.tbc_faces, f.n_tbc, np.int32),
-            ep_vec=_arr(f.ep_vec, E, np.float64), nu_vec=_arr(f.nu_vec, F, np.float64))
+            ep_vec=_arr(f.ep_vec, E, np.float64), nu_vec=_arr(f.nu_vec, F, np.float64),
+            edge_sid=_arr(f.edge_sid, E, np.int32) if f.edge_sid else None,
+            face_sid=_arr(f.face_sid, F, np.int32) if f.face_sid else None, cells=tuple(f.cells))
 
     def fit_tables(self):
         L = host_lib()
