@@ -57,12 +57,12 @@ constexpr int kXcd = 8;  // MI355X: 8 XCDs, each with its own L2
 // [xq + r, (x+1)q + r), with q = grid / 8, r = grid % 8 (a bijection on
 // [0, grid)). Kernels that gather neighbours index rows by the logical block,
 // so each L2 caches the part of the gathered vector near its own rows.
-__device__ __forceinline__ int xcd_block() {
-    const int g = (int)gridDim.x, b = (int)blockIdx.x;
+__device__ __forceinline__ int xcd_block_of(int b, int g) {
     const int x = b % kXcd, i = b / kXcd;
     const int q = g / kXcd, r = g % kXcd;
     return x * q + (x < r ? x : r) + i;
 }
+__device__ __forceinline__ int xcd_block() { return xcd_block_of((int)blockIdx.x, (int)gridDim.x); }
 
 inline int blocks_for(int64_t n, int per_block = kBlock) {
     int64_t b = (n + per_block - 1) / per_block;

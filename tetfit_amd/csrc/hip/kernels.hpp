@@ -9,6 +9,8 @@
 
 #include "common.hpp"
 
+#include <algorithm>
+
 namespace tfd {
 
 __device__ __forceinline__ double sgn2(uint32_t code) {
@@ -99,6 +101,51 @@ __host__ __device__ inline void fit_divmod(const FastDiv& f, uint32_t n, uint32_
     q = fastdiv(f, n);
     r = n - q * f.d;
 }
+// Items of each direction in grid order: x, y, z edges (faces = false) or
+// x, y, z faces (faces = true).
+__host__ __device__ inline void fit_counts(const FitGrid g, bool faces, uint32_t n[3]) {
+    const uint32_t nx = g.nx, ny = g.ny, nz = g.nz;
+    if (faces) {
+        n[0] = (nx + 1) * ny * nz; n[1] = nx * (ny + 1) * nz; n[2] = nx * ny * (nz + 1);
+    } else {
+        n[0] = nx * (ny + 1) * (nz + 1); n[1] = (nx + 1) * ny * (nz + 1); n[2] = (nx + 1) * (ny + 1) * nz;
+    }
+}
+// Block mapping for grid order. The items of one direction are stored k-plane
+// by k-plane, so the XCD that takes the x-th eighth of the blocks of each of
+// the three directions works on one k-slab of the grid, and the neighbours its
+// items gather (faces of an edge, edges of a face) stay in its own L2; with a
+// single contiguous range per XCD, each gathered plane would be read through
+// the L2s of two XCDs. Workgroup b runs on XCD b % 8 and takes the (b / 8)-th
+// block of that XCD's list (its x, then y, then z blocks). The grid has
+// 8 * (longest list) workgroups (slab_grid); returns the item, or -1 for a
+// workgroup past its XCD's list or a lane past its direction's end.
+__host__ __device__ inline uint32_t slab_len(uint32_t n, uint32_t x) {
+    const uint32_t nb = (n + kBlock - 1) / kBlock;
+    return (x + 1) * nb / kXcd - x * nb / kXcd;
+}
+inline int slab_grid(const uint32_t n[3]) {
+    uint32_t m = 1;
+    for (uint32_t x = 0; x < (uint32_t)kXcd; ++x)
+        m = std::max(m, slab_len(n[0], x) + slab_len(n[1], x) + slab_len(n[2], x));
+    return (int)(m * kXcd);
+}
+__device__ __forceinline__ int32_t slab_item(int b, const uint32_t n[3]) {
+    const uint32_t x = (uint32_t)b % kXcd;
+    uint32_t i = (uint32_t)b / kXcd, base = 0;
+#pragma unroll
+    for (int t = 0; t < 3; ++t) {
+        const uint32_t nb = (n[t] + kBlock - 1) / kBlock, lo = x * nb / kXcd, len = (x + 1) * nb / kXcd - lo;
+        if (i < len) {
+            const uint32_t r = (lo + i) * kBlock + threadIdx.x;
+            return r < n[t] ? (int32_t)(base + r) : -1;
+        }
+        i -= len;
+        base += n[t];
+    }
+    return -1;
+}
+
 // 32-bit index arithmetic: grid order is used only when edge and face
 // counts are below 2^31 (fit_grid_codes).
 __host__ __device__ inline void fit_edge_slots(const FitGrid g, uint32_t s, int32_t f[4]) {
@@ -163,20 +210,23 @@ __device__ __forceinline__ double slot_sum(uint32_t code, const double v[4]) {
     return c;
 }
 
-// E phase (D:1850-1895). Main blocks cover every edge in index order; edges
-// whose sign byte is 0 (pec / source / non-common) are left to the special
-// blocks that follow. U = ca*U_old + cb*(sum_k s_k F[f_k])
+// E phase (D:1850-1895). The first sp_blocks workgroups take the special
+// items, so that their longer dependent chains (source value, Poynting sum)
+// start with the launch instead of trailing it; the rest cover every edge in
+// index order (XCD-mapped among themselves). Edges whose sign byte is 0
+// (pec / source / non-common) are left to the special items. U = ca*U_old + cb*(sum_k s_k F[f_k])
 //   common:   ca = upml^2 M_h M_q, cb = upml^3 M_h dt, s = dual_curl*(+,-,+,-)
 // Special items (uncommon h-source edges, e/j source edges):
 //   kind 0 (h):  U = ca*U_old + cb*(sum s_k F + src)     (ca = M_h M_q, cb = M_h dt)
 //   kind 1 (e):  U = src
 //   kind 2 (j):  U = ca*U_old + cb*(sum s_k F - src)     (common coefficients)
-// With code16 (grid order) the main blocks take each edge's faces from its
-// position and its sign and order code from code16 (slot_sum); otherwise
-// from the packed face quadruple f4 and the sign byte sg.
+// With code16 (grid order) the main blocks take their edges by slab_item,
+// each edge's faces from its position and its sign and order code from code16
+// (slot_sum); otherwise edges by index order over XCD ranges, faces from the
+// packed face quadruple f4 and the sign byte sg.
 template <typename T>
 __global__ void __launch_bounds__(kBlock) fit_e_kernel(
-    int32_t n_edges, int32_t main_blocks, const uint8_t* __restrict__ sg, const T* __restrict__ ca,
+    int32_t n_edges, int32_t sp_blocks, const uint8_t* __restrict__ sg, const T* __restrict__ ca,
     const T* __restrict__ cb, const int4* __restrict__ f4, const uint16_t* __restrict__ code16, FitGrid grid,
     const T* __restrict__ Fv, T* __restrict__ U,
     int32_t n_sp, const int32_t* __restrict__ sp_edge, const int4* __restrict__ sp_f4,
@@ -184,11 +234,13 @@ __global__ void __launch_bounds__(kBlock) fit_e_kernel(
     const int8_t* __restrict__ sp_kind, const int32_t* __restrict__ sp_tb, const int32_t* __restrict__ sp_te,
     const int32_t* __restrict__ term_src, const double* __restrict__ term_w,
     const double* __restrict__ carr_step, double* __restrict__ poy_part) {
-    const int lb = xcd_block();
-    if (lb < main_blocks) {
-        const int32_t e = lb * kBlock + threadIdx.x;
-        if (e >= n_edges) return;
+    const int hb = (int)blockIdx.x;
+    if (hb >= sp_blocks) {
         if (code16) {
+            uint32_t cnt[3];
+            fit_counts(grid, false, cnt);
+            const int32_t e = slab_item(hb - sp_blocks, cnt);
+            if (e < 0) return;
             const uint32_t code = code16[e];
             if (code == 0) return;
             int32_t f[4];
@@ -197,6 +249,8 @@ __global__ void __launch_bounds__(kBlock) fit_e_kernel(
             U[e] = (T)((double)ca[e] * (double)U[e] + (double)cb[e] * slot_sum(code, v));
             return;
         }
+        const int32_t e = xcd_block_of(hb - sp_blocks, (int)gridDim.x - sp_blocks) * kBlock + threadIdx.x;
+        if (e >= n_edges) return;
         const uint32_t s = sg[e];
         if (s == 0) return;
         const int4 f = f4[e];
@@ -206,7 +260,7 @@ __global__ void __launch_bounds__(kBlock) fit_e_kernel(
         return;
     }
     __shared__ double red[kBlock / kWave];
-    const int32_t i = (lb - main_blocks) * kBlock + threadIdx.x;
+    const int32_t i = hb * kBlock + threadIdx.x;
     double poy = 0;
     if (i < n_sp) {
         const int32_t e = sp_edge[i];
@@ -232,7 +286,7 @@ __global__ void __launch_bounds__(kBlock) fit_e_kernel(
         if (k == 0) poy = 0.5 * (uold + (double)(T)unew) * src;
     }
     const double s = block_sum(poy, red);
-    if (threadIdx.x == 0) poy_part[lb - main_blocks] = s;
+    if (threadIdx.x == 0) poy_part[hb] = s;
 }
 
 // H phase over tbc faces (D:1908-1915): B = cm*F_old - dt*sum s_k U[e_k];
@@ -244,9 +298,11 @@ __global__ void __launch_bounds__(kBlock) fit_h_kernel(int32_t n_faces, const ui
                                                        const int4* __restrict__ e4, const uint16_t* __restrict__ code16,
                                                        FitGrid grid, const T* __restrict__ U,
                                                        T* __restrict__ Fv, T* __restrict__ B, double dt) {
-    const int32_t f = xcd_block() * kBlock + threadIdx.x;
-    if (f >= n_faces) return;
     if (code16) {
+        uint32_t cnt[3];
+        fit_counts(grid, true, cnt);
+        const int32_t f = slab_item((int)blockIdx.x, cnt);
+        if (f < 0) return;
         const uint32_t code = code16[f];
         if (code == 0) return;
         int32_t e[4];
@@ -257,6 +313,8 @@ __global__ void __launch_bounds__(kBlock) fit_h_kernel(int32_t n_faces, const ui
         Fv[f] = (T)((double)cn[f] * b);
         return;
     }
+    const int32_t f = xcd_block() * kBlock + threadIdx.x;
+    if (f >= n_faces) return;
     const uint32_t s = sg[f];
     if (s == 0) return;
     const int4 e = e4[f];

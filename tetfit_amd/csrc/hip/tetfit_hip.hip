@@ -1099,7 +1099,11 @@ int64_t run_graph_chunks(tf_ctx* c, int64_t m, int per_step, Step step) {
 
 template <class T>
 void run_fit(tf_ctx* c, int64_t first, int64_t n, tf_stats* st) {
-    const int main_blocks = blocks_for(c->E);
+    uint32_t ecnt[3], fcnt[3];
+    fit_counts(c->grid, false, ecnt);
+    fit_counts(c->grid, true, fcnt);
+    const int main_blocks = c->fit_grid ? slab_grid(ecnt) : blocks_for(c->E);
+    const int h_blocks = c->fit_grid ? slab_grid(fcnt) : blocks_for(c->F);
     const int sp_blocks = c->n_sp ? blocks_for(c->n_sp) : 0;
     const size_t ns = std::max<size_t>(c->src.size(), 1);
     if (sp_blocks) {
@@ -1110,13 +1114,13 @@ void run_fit(tf_ctx* c, int64_t first, int64_t n, tf_stats* st) {
     }
     // one step: E phase (with sources and the Poynting partials), H phase
     auto step = [&](const double* cs, double* poy) {
-        launch_timed(c, "fit_e_kernel", fit_e_kernel<T>, main_blocks + sp_blocks, 
-            c->E, main_blocks, c->fe_sg.as<uint8_t>(), c->fe_ca.as<T>(), c->fe_cb.as<T>(), c->fe_f4.as<int4>(),
+        launch_timed(c, "fit_e_kernel", fit_e_kernel<T>, main_blocks + sp_blocks,
+            c->E, sp_blocks, c->fe_sg.as<uint8_t>(), c->fe_ca.as<T>(), c->fe_cb.as<T>(), c->fe_f4.as<int4>(),
             c->fit_grid ? c->fe_code.as<uint16_t>() : nullptr, c->grid, c->Fv.as<T>(), c->U.as<T>(), c->n_sp, c->sp_edge.as<int32_t>(), c->sp_f4.as<int4>(),
             c->sp_sg.as<uint8_t>(), c->sp_ca.as<double>(), c->sp_cb.as<double>(), c->sp_kind.as<int8_t>(),
             c->sp_tb.as<int32_t>(), c->sp_te.as<int32_t>(), c->term_src.as<int32_t>(), c->term_w.as<double>(), cs,
             sp_blocks ? poy : nullptr);
-        launch_timed(c, "fit_h_kernel", fit_h_kernel<T>, blocks_for(c->F), c->F, c->fh_sg.as<uint8_t>(),
+        launch_timed(c, "fit_h_kernel", fit_h_kernel<T>, h_blocks, c->F, c->fh_sg.as<uint8_t>(),
                      c->fh_cm.as<T>(), c->fh_cn.as<T>(), c->fh_e4.as<int4>(),
                      c->fit_grid ? c->fh_code.as<uint16_t>() : nullptr, c->grid, c->U.as<T>(), c->Fv.as<T>(),
                      c->B.as<T>(), c->dt);
