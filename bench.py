@@ -60,14 +60,21 @@ def build_workload(d, method, n, steps, contrast=False):
     return b
 
 
+def slice_bytes(info):
+    """Per-slice header bytes the SELL kernels read: the 8-byte slice
+    pointer and 4 bytes per column base (one, four, or none with int32
+    columns)."""
+    return (8 + 4 * info["col_bases"]) * info["slices"]
+
+
 def algorithmic_bytes(info, kernel, vb):
     """Bytes a CG kernel must move per launch: the operator's non-zeros once
     (values + stored column index; padding entries are not counted), the
-    slice pointer and column base of each 64-row slice, and each vector once
+    slice pointer and column bases of each 64-row slice, and each vector once
     per access (gathered vectors once). `info` is the device storage of A
     (Simulation.sell_info("a")), vb the value bytes of the vectors."""
     n, nnz, cb = info["rows"], info["nnz"], info["col_bytes"]
-    per_slice = 12 * info["slices"]
+    per_slice = slice_bytes(info)
     if kernel == "cg_w_kernel":      # w = A u: u gathered, w written
         return nnz * (vb + cb) + per_slice + 2 * vb * n
     if kernel == "cg_wu_kernel":     # w = A u, p and s updated: u gathered, p, s read and written
@@ -84,7 +91,7 @@ def dga_bytes(info, kernel, vb, n_edges, n_faces):
     edge index, U read and written per row); sell_n: F -= dt N curl_u (curl_u
     gathered, F read and written)."""
     n, nnz, cb = info["rows"], info["nnz"], info["col_bytes"]
-    per_slice = 12 * info["slices"]
+    per_slice = slice_bytes(info)
     if kernel == "sell_h_kernel":
         return nnz * (vb + cb) + per_slice + vb * n_edges + 4 * n + 2 * vb * n
     if kernel == "sell_n_kernel":

@@ -195,9 +195,11 @@ tf_status tf_set_tuning(tf_ctx* ctx, const int32_t* blocks, const int32_t* unrol
  * 1 H, 2 N, 3 A (gfdtd free rows), 4 K = E_f - [A 0] (gfdtd free rows: the
  * couplings left once the warm start cancels, used for the initial residual),
  * 5 RHSmat1, 6 Ct (gfdtd free rows);
- * info[0..4] = rows, 64-row slices, non-zeros, stored entries (with
- * padding), bytes per stored column index (2: per-slice base + 16-bit
- * offset, 4: int32). No reference counterpart. */
+ * info[0..5] = rows, 64-row slices, non-zeros, stored entries (with
+ * padding), bytes per stored column index (2: 16-bit code, 4: int32),
+ * column bases per slice (1: the code is the offset from the slice's base;
+ * 4: the code is a 2-bit window and a 14-bit offset from that window's
+ * base; 0 with int32 columns). No reference counterpart. */
 tf_status tf_sell_info(tf_ctx* ctx, int32_t which, int64_t* info);
 
 /* FIT storage of a context: *grid_order = 1 when fields and coefficients sit

@@ -228,16 +228,20 @@ def test_batch_graph_steps_match_eager(workdir):
             assert np.array_equal(a, e)
 
 
+@pytest.mark.parametrize("layout", ["col32", "win"])
 @pytest.mark.parametrize("method", ["dga", "gfdtd"])
-def test_int32_columns_parity(workdir, method, monkeypatch):
-    """Every SELL operator stored with int32 column indices (the layout for
-    meshes whose slices span more than 65,536 columns) gives the same fp64
-    parity."""
-    monkeypatch.setenv("TETFIT_SELL_COL32", "1")
+def test_column_layouts_parity(workdir, method, layout, monkeypatch):
+    """Every SELL operator stored with int32 column indices, or with 16-bit
+    codes over four column windows per slice (the layouts for meshes whose
+    slices span more than 65,536 columns), gives the same fp64 parity."""
+    monkeypatch.setenv("TETFIT_SELL_COL32" if layout == "col32" else "TETFIT_SELL_WIN", "1")
     b = Model(configs.kuhn_cavity(str(workdir), n=5, method=method, n_steps=60, n_probes=4, block=2,
-                                  name=f"c32_{method}")).build(1)
+                                  name=f"c{layout}_{method}")).build(1)
+    print(f"case: {method} {layout} columns E={b.n_edges}", flush=True)
     sim = Simulation(b, "fp64")
-    assert sim.sell_info("n")["col_bytes"] == 4
+    for op in ("n", "a") if method == "gfdtd" else ("n", "h", "ct"):
+        info = sim.sell_info(op)
+        assert (info["col_bytes"], info["col_bases"]) == ((4, 0) if layout == "col32" else (2, 4)), (op, info)
     sim.close()
     check_run(b)
 

@@ -22,15 +22,44 @@ constexpr int kSlice = 64;
 
 // Device view of a SELL-64 matrix with values of type V (int8 for signed
 // incidence, uint16 for bf16 bits). Exactly one of col / col16 is set.
+// 16-bit column codes come in two forms: with cwin = 0 the code is the
+// offset from the slice's one base cbase[slice] (slices spanning at most
+// 65,536 columns); with cwin = 1 the slice has four bases cbase[4 slice + w]
+// and the code is w << 14 | offset (each of up to four windows spans at most
+// 16,384 columns). Lattice-like meshes numbered plane by plane need the
+// second form once a row's neighbours in the planes below and above lie more
+// than 65,536 apart (Kuhn meshes from about 70^3): their columns still fall
+// into three narrow clusters.
 template <typename V>
 struct SellView {
     int32_t nrows, nslices;
     const int64_t* sptr;
     const int32_t* col;     // int32 columns, or null
-    const uint16_t* col16;  // uint16 offsets from cbase[slice], or null
+    const uint16_t* col16;  // 16-bit column codes, or null
     const int32_t* cbase;
     const V* val;
+    int32_t cwin;
 };
+
+// Column of a 16-bit code in the windowed form, with the slice's four bases
+// in scalar registers. The base is picked with masks rather than a select
+// over the four, which the compiler may lower to a dynamically indexed array
+// (placed in LDS, enlarging every workgroup's allocation).
+struct ColWin {
+    int32_t b0, x01, b2, x23;  // b0, b0 ^ b1, b2, b2 ^ b3
+};
+template <typename V>
+__device__ __forceinline__ ColWin col_windows(const SellView<V>& m, int s) {
+    const int32_t* b = m.cbase + 4 * s;
+    const int32_t b0 = __builtin_amdgcn_readfirstlane(b[0]), b1 = __builtin_amdgcn_readfirstlane(b[1]);
+    const int32_t b2 = __builtin_amdgcn_readfirstlane(b[2]), b3 = __builtin_amdgcn_readfirstlane(b[3]);
+    return ColWin{b0, b0 ^ b1, b2, b2 ^ b3};
+}
+__device__ __forceinline__ int32_t win_col(const ColWin& w, uint32_t code) {
+    const int32_t m1 = -(int32_t)((code >> 14) & 1u), m2 = -(int32_t)(code >> 15);
+    const int32_t lo = w.b0 ^ (w.x01 & m1), hi = w.b2 ^ (w.x23 & m1);
+    return (lo ^ ((lo ^ hi) & m2)) + (int32_t)(code & 0x3FFFu);
+}
 
 __device__ __forceinline__ float bf16_to_float(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
 
@@ -56,7 +85,7 @@ __device__ __forceinline__ double sell_value<double, uint16_t>(uint16_t v) {
 // traffic in flight; the tail batch is clamped to the row's last entry and
 // predicated. Accumulation stays in entry order, so the result does not
 // depend on U or on the launch shape.
-template <typename A, int U, typename V, class G>
+template <typename A, int U, bool W, typename V, class G>
 __device__ __forceinline__ double sell_dot(const SellView<V>& m, int s, int lane, G gather) {
     s = __builtin_amdgcn_readfirstlane(s);
     const int64_t base = m.sptr[s];
@@ -83,7 +112,11 @@ __device__ __forceinline__ double sell_dot(const SellView<V>& m, int s, int lane
         for (; k0 + U <= w; k0 += U) batch(k0, col_at, true);
         if (k0 < w) batch(k0, col_at, false);
     };
-    if (m.col16) {
+    if (W && m.col16 && m.cwin) {
+        const uint16_t* c = m.col16 + base;
+        const ColWin cw = col_windows(m, s);
+        rows([&](int o) { return win_col(cw, c[o]); });
+    } else if (m.col16) {
         const uint16_t* c = m.col16 + base;
         const int32_t cb = __builtin_amdgcn_readfirstlane(m.cbase[s]);
         rows([&](int o) { return cb + (int32_t)c[o]; });
@@ -116,13 +149,13 @@ __device__ __forceinline__ SliceRange block_slices(int nslices) {
     for (SliceRange sr_ = block_slices((m).nslices); sr_.s < sr_.end; sr_.s += sr_.step)
 
 // curl_f = Ct F over edges (dga, D:878-879).
-template <typename T, int NB>
+template <typename T, int NB, bool W = false>
 __global__ void __launch_bounds__(kBlock) sell_ct_kernel(SellView<int8_t> m, const T* __restrict__ x,
                                                          T* __restrict__ y) {
     const int lane = lane_id();
     TF_FOR_SLICES(m) {
         const int s = sr_.s;
-        const double v = sell_dot<T, NB>(m, s, lane, [&](int32_t j) { return x[j]; });
+        const double v = sell_dot<T, NB, W>(m, s, lane, [&](int32_t j) { return x[j]; });
         const int32_t r = s * kSlice + lane;
         if (r < m.nrows) y[r] = (T)v;
     }
@@ -130,14 +163,14 @@ __global__ void __launch_bounds__(kBlock) sell_ct_kernel(SellView<int8_t> m, con
 
 // U[edge(r)] += dt * (H curl_f)[r] (dga, D:889); row_dt (batches) gives
 // each row its own simulation's dt.
-template <typename T, int NB>
+template <typename T, int NB, bool W = false>
 __global__ void __launch_bounds__(kBlock) sell_h_kernel(SellView<T> m, const int32_t* __restrict__ row_edge,
                                                         const T* __restrict__ x, T* __restrict__ U, double dt,
                                                         const double* __restrict__ row_dt) {
     const int lane = lane_id();
     TF_FOR_SLICES(m) {
         const int s = sr_.s;
-        const double v = sell_dot<T, NB>(m, s, lane, [&](int32_t j) { return x[j]; });
+        const double v = sell_dot<T, NB, W>(m, s, lane, [&](int32_t j) { return x[j]; });
         const int32_t r = s * kSlice + lane;
         if (r < m.nrows) {
             const int32_t e = row_edge[r];
@@ -148,7 +181,7 @@ __global__ void __launch_bounds__(kBlock) sell_h_kernel(SellView<T> m, const int
 
 // mode 0: F -= dt * N x (dga, D:917); mode 1: F = N x (gfdtd, D:1468);
 // row_dt as in sell_h_kernel; guard as in cg_guard_ok.
-template <typename T, int NB>
+template <typename T, int NB, bool W = false>
 __global__ void __launch_bounds__(kBlock) sell_n_kernel(SellView<T> m, const T* __restrict__ x, T* __restrict__ Fv,
                                                         double dt, int mode, const double* __restrict__ row_dt,
                                                         const CgState* guard) {
@@ -156,7 +189,7 @@ __global__ void __launch_bounds__(kBlock) sell_n_kernel(SellView<T> m, const T* 
     const int lane = lane_id();
     TF_FOR_SLICES(m) {
         const int s = sr_.s;
-        const double v = sell_dot<T, NB>(m, s, lane, [&](int32_t j) { return x[j]; });
+        const double v = sell_dot<T, NB, W>(m, s, lane, [&](int32_t j) { return x[j]; });
         const int32_t r = s * kSlice + lane;
         if (r < m.nrows) Fv[r] = mode == 0 ? (T)((double)Fv[r] - (row_dt ? row_dt[r] : dt) * v) : (T)v;
     }
@@ -166,7 +199,7 @@ __global__ void __launch_bounds__(kBlock) sell_n_kernel(SellView<T> m, const T* 
 // in fp64: twice the discrete energy x.(M x) / 2 of a symmetric mass matrix
 // (We = 1/2 U.E U, Wh = 1/2 B.N B, D:2773-2775), or the Joule term
 // U.SigMat U of gfdtd's losses (D:1519, D:1544); guard as in cg_guard_ok.
-template <typename T, int NB>
+template <typename T, int NB, bool W = false>
 __global__ void __launch_bounds__(kBlock) quad_form_kernel(SellView<T> m, const T* __restrict__ x,
                                                            double* __restrict__ part, const CgState* guard) {
     if (!cg_guard_ok(guard)) return;
@@ -175,7 +208,7 @@ __global__ void __launch_bounds__(kBlock) quad_form_kernel(SellView<T> m, const 
     double acc = 0;
     TF_FOR_SLICES(m) {
         const int s = sr_.s;
-        const double mx = sell_dot<double, NB>(m, s, lane, [&](int32_t j) { return x[j]; });
+        const double mx = sell_dot<double, NB, W>(m, s, lane, [&](int32_t j) { return x[j]; });
         const int32_t r = s * kSlice + lane;
         if (r < m.nrows) acc += (double)x[r] * mx;
     }
@@ -323,7 +356,7 @@ __global__ void __launch_bounds__(kBlock) cg_restart_kernel(int32_t n, const T* 
 
 // w = A u, partial w.u; with test_stop the pass starts with the stop test on
 // the partials part_rr of the previous update.
-template <typename T, int NB>
+template <typename T, int NB, bool W = false>
 __global__ void __launch_bounds__(kBlock) cg_w_kernel(int32_t it, SellView<T> m, const T* __restrict__ u,
                                                       T* __restrict__ w, int test_stop,
                                                       const double* __restrict__ part_rr, int32_t nb_rr,
@@ -343,7 +376,7 @@ __global__ void __launch_bounds__(kBlock) cg_w_kernel(int32_t it, SellView<T> m,
     double wu = 0, none = 0;
     TF_FOR_SLICES(m) {
         const int s = sr_.s;
-        const double wv = sell_dot<T, NB>(m, s, lane, [&](int32_t j) { return u[j]; });
+        const double wv = sell_dot<T, NB, W>(m, s, lane, [&](int32_t j) { return u[j]; });
         const int32_t i = s * kSlice + lane;
         if (i < m.nrows) {
             const T wi = (T)wv;
@@ -417,10 +450,10 @@ __global__ void __launch_bounds__(kBlock) cg_upd_kernel(int32_t it, int32_t n, c
 
 // Both products of one row over the gathered (r, s) pairs: gr = (G r)_i and
 // gs = (G s)_i, so that u = G (r - alpha s) = gr - alpha gs needs alpha only
-// after the gathers. G is stored packed, 32 bits per entry (16-bit column
-// offset << 16 | bf16 value), when A has 16-bit columns (pk), otherwise as
+// after the gathers. G is stored packed, 32 bits per entry (A's 16-bit
+// column code << 16 | bf16 value), when A has 16-bit columns (pk), otherwise as
 // bf16 values beside A's column indices (m.val).
-template <typename T, int U>
+template <typename T, int U, bool W>
 __device__ __forceinline__ void sell_dot_rs(const SellView<uint16_t>& m, const uint32_t* __restrict__ pk, int s,
                                             int lane, const RS<T>* __restrict__ rs, T& gr, T& gs) {
     s = __builtin_amdgcn_readfirstlane(s);
@@ -447,13 +480,29 @@ __device__ __forceinline__ void sell_dot_rs(const SellView<uint16_t>& m, const u
         for (; k0 + U <= w; k0 += U) batch(k0, entry, true);
         if (k0 < w) batch(k0, entry, false);
     };
-    if (pk) {
+    if (W && pk && m.cwin) {
+        const uint32_t* q = pk + base;
+        const ColWin cw = col_windows(m, s);
+        rows([&](int o, int32_t& c, T& v) {
+            const uint32_t e = q[o];
+            c = win_col(cw, e >> 16);
+            v = (T)bf16_to_float((uint16_t)(e & 0xFFFFu));
+        });
+    } else if (pk) {
         const uint32_t* q = pk + base;
         const int32_t cb = __builtin_amdgcn_readfirstlane(m.cbase[s]);
         rows([&](int o, int32_t& c, T& v) {
             const uint32_t e = q[o];
             c = cb + (int32_t)(e >> 16);
             v = (T)bf16_to_float((uint16_t)(e & 0xFFFFu));
+        });
+    } else if (W && m.col16 && m.cwin) {
+        const uint16_t* c16 = m.col16 + base;
+        const uint16_t* vv = m.val + base;
+        const ColWin cw = col_windows(m, s);
+        rows([&](int o, int32_t& c, T& v) {
+            c = win_col(cw, c16[o]);
+            v = (T)bf16_to_float(vv[o]);
         });
     } else if (m.col16) {
         const uint16_t* c16 = m.col16 + base;
@@ -484,7 +533,7 @@ __device__ __forceinline__ void sell_dot_rs(const SellView<uint16_t>& m, const u
 // products of the wave's first slice do not need alpha and are issued
 // before the partial sums are reduced. G: bf16 values on A's pattern
 // (sell_dot_rs).
-template <typename T, int NB>
+template <typename T, int NB, bool W = false>
 __global__ void __launch_bounds__(kBlock) cg_pg_kernel(int32_t it, SellView<uint16_t> g,
                                                        const uint32_t* __restrict__ gpk,
                                                        const RS<T>* __restrict__ rs_in, RS<T>* __restrict__ rs_out,
@@ -502,7 +551,7 @@ __global__ void __launch_bounds__(kBlock) cg_pg_kernel(int32_t it, SellView<uint
     SliceRange sr = block_slices(g.nslices);
     const bool first = sr.s < sr.end;
     T gr0 = 0, gs0 = 0;
-    if (first) sell_dot_rs<T, NB>(g, gpk, sr.s, lane, rs_in, gr0, gs0);
+    if (first) sell_dot_rs<T, NB, W>(g, gpk, sr.s, lane, rs_in, gr0, gs0);
     double alpha = 0;
     if (it > 0) {
         double del = thread_sum(pd), unused = 0;
@@ -533,7 +582,7 @@ __global__ void __launch_bounds__(kBlock) cg_pg_kernel(int32_t it, SellView<uint
     }
     for (; sr.s < sr.end; sr.s += sr.step) {
         T gr, gs;
-        sell_dot_rs<T, NB>(g, gpk, sr.s, lane, rs_in, gr, gs);
+        sell_dot_rs<T, NB, W>(g, gpk, sr.s, lane, rs_in, gr, gs);
         row(sr.s, gr, gs);
     }
     block_sum2_all(ru, rr, red);
@@ -547,7 +596,7 @@ __global__ void __launch_bounds__(kBlock) cg_pg_kernel(int32_t it, SellView<uint
 // and beta_it (stored in slot it&1), w = A u, p = u + beta p and
 // s_it = w + beta s_{it-1} on the owned rows (rs_cur[i] holds (r_it,
 // s_{it-1}) as cg_pg wrote it; s_it replaces its s), partial w.u (delta_it).
-template <typename T, int NB>
+template <typename T, int NB, bool W = false>
 __global__ void __launch_bounds__(kBlock) cg_wu_kernel(int32_t it, SellView<T> m, const T* __restrict__ u,
                                                        T* __restrict__ p, RS<T>* __restrict__ rs_cur,
                                                        const double* __restrict__ part_rr,
@@ -571,7 +620,7 @@ __global__ void __launch_bounds__(kBlock) cg_wu_kernel(int32_t it, SellView<T> m
     double wu = 0, none = 0;
     TF_FOR_SLICES(m) {
         const int sl = sr_.s;
-        const double wv = sell_dot<T, NB>(m, sl, lane, [&](int32_t j) { return u[j]; });
+        const double wv = sell_dot<T, NB, W>(m, sl, lane, [&](int32_t j) { return u[j]; });
         const int32_t i = sl * kSlice + lane;
         if (i < m.nrows) {
             const T wi = (T)wv;
@@ -604,7 +653,7 @@ __global__ void __launch_bounds__(kBlock) cg_wu_kernel(int32_t it, SellView<T> m
 // products are skipped. r0 is kept in `r0save`
 // for a restart; with jacobi also u = pre r and partial r.u. guard as in
 // cg_guard_ok. SrcFld is U on Dirichlet edges at the new time (`dir_new`).
-template <typename T, int NB>
+template <typename T, int NB, bool W = false>
 __global__ void __launch_bounds__(kBlock) gf_rhs_init_kernel(
     SellView<T> kop, SellView<T> r1, SellView<int8_t> ct, int dir_terms, const int32_t* __restrict__ free_edge,
     const T* __restrict__ U, const T* __restrict__ dir_new, const T* __restrict__ Fv, const T* __restrict__ fb,
@@ -618,12 +667,12 @@ __global__ void __launch_bounds__(kBlock) gf_rhs_init_kernel(
     TF_FOR_SLICES(ct) {
         const int s = sr_.s;
         const double ku = dir_terms && kop.sptr[s + 1] > kop.sptr[s]
-                              ? sell_dot<T, 8>(kop, s, lane, [&](int32_t j) { return U[j]; })
+                              ? sell_dot<T, 8, W>(kop, s, lane, [&](int32_t j) { return U[j]; })
                               : 0.0;
         const double rs = dir_terms && r1.sptr[s + 1] > r1.sptr[s]
-                              ? sell_dot<T, 8>(r1, s, lane, [&](int32_t j) { return dir_new[j]; })
+                              ? sell_dot<T, 8, W>(r1, s, lane, [&](int32_t j) { return dir_new[j]; })
                               : 0.0;
-        const double cf = sell_dot<T, NB>(ct, s, lane, [&](int32_t j) { return Fv[j]; });
+        const double cf = sell_dot<T, NB, W>(ct, s, lane, [&](int32_t j) { return Fv[j]; });
         const int32_t k = s * kSlice + lane;
         if (k < ct.nrows) {
             const T rk = (T)((ku - rs) + dt * cf + dt * (double)fb[free_edge[k]]);

@@ -85,11 +85,12 @@ struct Sell {
     DevBuf sptr, col, val, sg, col16, cbase;
     int32_t nrows = 0, nslices = 0;
     int64_t nnz = 0, entries = 0;
-    bool c16 = false;  // 16-bit column offsets (col16 + cbase) in the kernels' view
+    bool c16 = false;   // 16-bit column codes (col16 + cbase) in the kernels' view
+    bool cwin = false;  // ... in the four-window form (SellView::cwin)
     template <class V>
     SellView<V> view(const DevBuf& v) const {
         return SellView<V>{nrows, nslices, sptr.as<int64_t>(), c16 ? nullptr : col.as<int32_t>(),
-                           c16 ? col16.as<uint16_t>() : nullptr, cbase.as<int32_t>(), v.as<V>()};
+                           c16 ? col16.as<uint16_t>() : nullptr, cbase.as<int32_t>(), v.as<V>(), cwin ? 1 : 0};
     }
     template <class T>
     SellView<T> vals() const { return view<T>(val); }
@@ -107,13 +108,23 @@ struct KTune {
     int unroll = 8;
 };
 
-#define TF_LAUNCH_NB(nb, kern, T, grid, stream, ...)                               \
-    do {                                                                          \
-        switch (nb) {                                                             \
-            case 4: kern<T, 4><<<(grid), kBlock, 0, (stream)>>>(__VA_ARGS__); break;   \
-            case 16: kern<T, 16><<<(grid), kBlock, 0, (stream)>>>(__VA_ARGS__); break; \
-            default: kern<T, 8><<<(grid), kBlock, 0, (stream)>>>(__VA_ARGS__); break;  \
-        }                                                                         \
+// SELL kernels take the gather batch NB and W (compile the decode of
+// windowed 16-bit columns in, set when some operator of the context uses them).
+#define TF_LAUNCH_NB(w, nb, kern, T, grid, stream, ...)                                      \
+    do {                                                                                    \
+        if (w) {                                                                            \
+            switch (nb) {                                                                   \
+                case 4: kern<T, 4, true><<<(grid), kBlock, 0, (stream)>>>(__VA_ARGS__); break;   \
+                case 16: kern<T, 16, true><<<(grid), kBlock, 0, (stream)>>>(__VA_ARGS__); break; \
+                default: kern<T, 8, true><<<(grid), kBlock, 0, (stream)>>>(__VA_ARGS__); break;  \
+            }                                                                               \
+        } else {                                                                            \
+            switch (nb) {                                                                   \
+                case 4: kern<T, 4><<<(grid), kBlock, 0, (stream)>>>(__VA_ARGS__); break;         \
+                case 16: kern<T, 16><<<(grid), kBlock, 0, (stream)>>>(__VA_ARGS__); break;       \
+                default: kern<T, 8><<<(grid), kBlock, 0, (stream)>>>(__VA_ARGS__); break;        \
+            }                                                                               \
+        }                                                                                   \
     } while (0)
 
 // State of the learned preconditioner of a gfdtd context.
@@ -159,6 +170,7 @@ struct tf_ctx {
     // FIT grid order: fields, coefficients and codes stored at the grid
     // positions of edges and faces (fit_edge_slots / fit_face_slots)
     bool fit_grid = false;
+    bool cwin = false;  // some SELL operator uses windowed 16-bit columns: launch the SELL kernels with W
     FitGrid grid{};
     std::vector<int32_t> h_esid, h_fsid;  // reference edge / face -> grid position
     std::vector<uint16_t> h_fe_code;      // per grid edge: slot map << 8 | signs (0: not in the main pass)
@@ -365,12 +377,19 @@ void upload_sell(Sell& d, const tf_csr& c, bool values) {
     upload(d.col, col);
     if (values) upload(d.val, val);
     else upload(d.sg, sg);
-    // 16-bit columns when every slice spans fewer than 65536 columns
-    // ($TETFIT_SELL_COL32 set to a non-empty value other than 0 keeps int32
-    // columns, to exercise that layout on small meshes)
+    // 16-bit column codes: one base per slice when every slice spans at most
+    // 65,536 columns, else four windows of 16,384 columns per slice when
+    // every slice's columns fit in four, else int32 columns.
+    // $TETFIT_SELL_COL32 (non-empty, not 0) keeps int32 columns and
+    // $TETFIT_SELL_WIN (likewise) skips the one-base form, to exercise those
+    // layouts on small meshes.
+    auto env_on = [](const char* name) {
+        const char* v = std::getenv(name);
+        return v && *v && std::strcmp(v, "0") != 0;
+    };
+    const bool force32 = env_on("TETFIT_SELL_COL32");
     std::vector<int32_t> cb(std::max(d.nslices, 1), 0);
-    const char* force32 = std::getenv("TETFIT_SELL_COL32");
-    bool fits = !(force32 && *force32 && std::strcmp(force32, "0") != 0);
+    bool fits = !force32 && !env_on("TETFIT_SELL_WIN");
     for (int32_t s = 0; s < d.nslices && fits; ++s) {
         int32_t lo = INT32_MAX, hi = 0;
         for (int64_t o = sptr[s]; o < sptr[s + 1]; ++o) { lo = std::min(lo, col[o]); hi = std::max(hi, col[o]); }
@@ -378,13 +397,47 @@ void upload_sell(Sell& d, const tf_csr& c, bool values) {
         cb[s] = lo;
         fits = (int64_t)hi - lo <= 65535;
     }
-    d.c16 = fits;
+    std::vector<uint16_t> c16;
     if (fits) {
-        std::vector<uint16_t> c16(std::max<int64_t>(d.entries, 1), 0);
+        c16.assign(std::max<int64_t>(d.entries, 1), 0);
         for (int32_t s = 0; s < d.nslices; ++s)
             for (int64_t o = sptr[s]; o < sptr[s + 1]; ++o) c16[o] = (uint16_t)(col[o] - cb[s]);
-        upload(d.col16, c16);
+    } else if (!force32) {
+        // windows: greedy over the slice's sorted columns, each window opening
+        // at the smallest column not yet covered
+        cb.assign(4 * std::max(d.nslices, 1), 0);
+        std::vector<uint8_t> nwin(std::max(d.nslices, 1), 0);
+        fits = true;
+        std::vector<int32_t> cs;
+        for (int32_t s = 0; s < d.nslices && fits; ++s) {
+            cs.assign(col.begin() + sptr[s], col.begin() + sptr[s + 1]);
+            std::sort(cs.begin(), cs.end());
+            int nw = 0;
+            for (size_t k = 0; k < cs.size();) {
+                if (nw == 4) { fits = false; break; }
+                const int32_t b = cs[k];
+                cb[4 * s + nw++] = b;
+                while (k < cs.size() && (int64_t)cs[k] - b <= 16383) ++k;
+            }
+            nwin[s] = (uint8_t)nw;
+        }
+        if (fits) {
+            c16.assign(std::max<int64_t>(d.entries, 1), 0);
+            for (int32_t s = 0; s < d.nslices; ++s)
+                for (int64_t o = sptr[s]; o < sptr[s + 1]; ++o) {
+                    // the last opened window whose base is <= the column; the
+                    // windows are disjoint and cover every column of the slice
+                    int w = nwin[s] - 1;
+                    while (w > 0 && cb[4 * s + w] > col[o]) --w;
+                    const int64_t off = (int64_t)col[o] - cb[4 * s + w];
+                    if (off < 0 || off > 16383) throw std::logic_error("SELL column window");
+                    c16[o] = (uint16_t)(w << 14 | off);
+                }
+            d.cwin = true;
+        }
     }
+    d.c16 = fits;
+    if (fits) upload(d.col16, c16);
     upload(d.cbase, cb);
 }
 
@@ -502,10 +555,18 @@ void launch_timed(tf_ctx* c, const char* name, K kernel, int grid, Args... args)
 }
 #define TF_LAUNCH_TIMED(c, name, nb, kern, T, grid, ...)                                       \
     do {                                                                                    \
-        switch (nb) {                                                                       \
-            case 4: launch_timed(c, name, kern<T, 4>, grid, __VA_ARGS__); break;               \
-            case 16: launch_timed(c, name, kern<T, 16>, grid, __VA_ARGS__); break;             \
-            default: launch_timed(c, name, kern<T, 8>, grid, __VA_ARGS__); break;              \
+        if ((c)->cwin) {                                                                    \
+            switch (nb) {                                                                   \
+                case 4: launch_timed(c, name, kern<T, 4, true>, grid, __VA_ARGS__); break;     \
+                case 16: launch_timed(c, name, kern<T, 16, true>, grid, __VA_ARGS__); break;   \
+                default: launch_timed(c, name, kern<T, 8, true>, grid, __VA_ARGS__); break;    \
+            }                                                                               \
+        } else {                                                                            \
+            switch (nb) {                                                                   \
+                case 4: launch_timed(c, name, kern<T, 4>, grid, __VA_ARGS__); break;           \
+                case 16: launch_timed(c, name, kern<T, 16>, grid, __VA_ARGS__); break;         \
+                default: launch_timed(c, name, kern<T, 8>, grid, __VA_ARGS__); break;          \
+            }                                                                               \
         }                                                                                   \
     } while (0)
 void ev_flush(tf_ctx* c) {
@@ -854,6 +915,9 @@ void create_tet(tf_ctx* c, const tf_problem* p) {
         TF_HIP(hipHostMalloc((void**)&c->h_state, 2 * sizeof(CgState), hipHostMallocDefault));
         std::memset((void*)c->h_state, 0, 2 * sizeof(CgState));
     }
+    c->cwin = false;
+    for (const Sell* m : {&c->sCt, &c->H, &c->N, &c->A, &c->Kf, &c->R1f, &c->Ctf, &c->Efull, &c->Sig})
+        c->cwin = c->cwin || m->cwin;
     // initial U: fixed (class-4) values
     std::vector<T> u0(E, 0);
     for (int32_t e = 0; e < E; ++e)
@@ -1748,7 +1812,7 @@ void run_gfdtd(tf_ctx* c, int64_t first, int64_t n, const tf_cg_opts* cg, tf_sta
         sample_launch<T>(c, g.sample_slot, guard);
         if (lossy) {
             // U_old.SigMat U_old of the losses (D:1519)
-            launch_timed(c, "quad_form_kernel", quad_form_kernel<T, 8>, loss_blocks, 
+            TF_LAUNCH_TIMED(c, "quad_form_kernel", 8, quad_form_kernel, T, loss_blocks,
                 c->Sig.vals<T>(), c->U.as<T>(), c->loss_part.as<double>() + k * loss_blocks, guard);
         }
         if (c->n_inj) {
@@ -2188,19 +2252,19 @@ void autotune(tf_ctx* c) {
     hipStream_t s = c->stream;
     if (c->method == TF_METHOD_DGA) {
         c->tune[TK_CT] = tune_one(c, TK_CT, c->sCt, kSpmvCap, [&](int b, int u) {
-            TF_LAUNCH_NB(u, sell_ct_kernel, T, b, s, c->sCt.signs(), c->Fv.as<T>(), c->curl_f.as<T>());
+            TF_LAUNCH_NB(c->cwin, u, sell_ct_kernel, T, b, s, c->sCt.signs(), c->Fv.as<T>(), c->curl_f.as<T>());
         });
         c->tune[TK_H] = tune_one(c, TK_H, c->H, kSpmvCap, [&](int b, int u) {
-            TF_LAUNCH_NB(u, sell_h_kernel, T, b, s, c->H.vals<T>(), c->h_edge.as<int32_t>(), c->curl_f.as<T>(),
+            TF_LAUNCH_NB(c->cwin, u, sell_h_kernel, T, b, s, c->H.vals<T>(), c->h_edge.as<int32_t>(), c->curl_f.as<T>(),
                          c->U.as<T>(), 0.0, nullptr);
         });
         c->tune[TK_N] = tune_one(c, TK_N, c->N, kSpmvCap, [&](int b, int u) {
-            TF_LAUNCH_NB(u, sell_n_kernel, T, b, s, c->N.vals<T>(), c->curl_u.as<T>(), c->Fv.as<T>(), 0.0, 0, nullptr,
+            TF_LAUNCH_NB(c->cwin, u, sell_n_kernel, T, b, s, c->N.vals<T>(), c->curl_u.as<T>(), c->Fv.as<T>(), 0.0, 0, nullptr,
                          nullptr);
         });
     } else if (c->method == TF_METHOD_GFDTD) {
         c->tune[TK_N] = tune_one(c, TK_N, c->N, kSpmvCap, [&](int b, int u) {
-            TF_LAUNCH_NB(u, sell_n_kernel, T, b, s, c->N.vals<T>(), c->B.as<T>(), c->Fv.as<T>(), c->dt, 1, nullptr,
+            TF_LAUNCH_NB(c->cwin, u, sell_n_kernel, T, b, s, c->N.vals<T>(), c->B.as<T>(), c->Fv.as<T>(), c->dt, 1, nullptr,
                          nullptr);
         });
         // Tuning launches run pass 1 (the full work of a pass) on a scratch
@@ -2224,7 +2288,7 @@ void autotune(tf_ctx* c) {
         const int nbb = std::max(1, std::min(blocks_for(nf), kCgPartB));  // cg_upd's grid (CgView)
         // Jacobi pass: cg_w (stop test on cg_upd's partials) + the cg_upd that sums its b partials
         c->tune[TK_W] = tune_one(c, TK_W, c->A, kCgPartA, [&](int b, int u) {
-            TF_LAUNCH_NB(u, cg_w_kernel, T, b, s, 1, c->A.vals<T>(), c->z.as<T>(), c->q.as<T>(), 1, one, nbb, part,
+            TF_LAUNCH_NB(c->cwin, u, cg_w_kernel, T, b, s, 1, c->A.vals<T>(), c->z.as<T>(), c->q.as<T>(), 1, one, nbb, part,
                          st);
             cg_upd_kernel<T><<<nbb, kBlock, 0, s>>>(1, nf, c->pre.as<T>(), c->q.as<T>(), c->r.as<T>(), c->z.as<T>(),
                                                     c->q.as<T>(), c->p0.as<T>(), c->p1.as<T>(), one, nbb, one, b,
@@ -2234,11 +2298,11 @@ void autotune(tf_ctx* c) {
             const SellView<uint16_t> gv = c->A.view<uint16_t>(c->lp.gval);
             const uint32_t* gpk = c->lp.gpk.p ? c->lp.gpk.as<uint32_t>() : nullptr;
             auto pg = [&](int b, int u, int nparts) {
-                TF_LAUNCH_NB(u, cg_pg_kernel, T, b, s, 1, gv, gpk, c->rs0.as<RS<T>>(), c->rs1.as<RS<T>>(),
+                TF_LAUNCH_NB(c->cwin, u, cg_pg_kernel, T, b, s, 1, gv, gpk, c->rs0.as<RS<T>>(), c->rs1.as<RS<T>>(),
                              c->p0.as<T>(), c->q.as<T>(), c->z.as<T>(), one, nparts, part, part, st);
             };
             auto wu = [&](int b, int u, int nparts) {
-                TF_LAUNCH_NB(u, cg_wu_kernel, T, b, s, 1, c->A.vals<T>(), c->z.as<T>(), c->p0.as<T>(),
+                TF_LAUNCH_NB(c->cwin, u, cg_wu_kernel, T, b, s, 1, c->A.vals<T>(), c->z.as<T>(), c->p0.as<T>(),
                              c->rs1.as<RS<T>>(), one, one, nparts, part, st);
             };
             // cg_pg (summing cg_wu's partials) + the cg_wu that sums its b partials, then
@@ -2545,15 +2609,15 @@ tf_status tf_energy(tf_ctx* c, double* we, double* wh) {
             ge = sell_blocks(c->Efull, nb);
             gn = sell_blocks(c->N, nb);
             if (c->prec == TF_PREC_FP64) {
-                quad_form_kernel<double, 8><<<ge, kBlock, 0, c->stream>>>(c->Efull.vals<double>(), c->U.as<double>(), pp,
-                                                                    nullptr);
-                quad_form_kernel<double, 8><<<gn, kBlock, 0, c->stream>>>(c->N.vals<double>(), c->B.as<double>(), pp + nb,
-                                                                    nullptr);
+                TF_LAUNCH_NB(c->cwin, 8, quad_form_kernel, double, ge, c->stream, c->Efull.vals<double>(),
+                             c->U.as<double>(), pp, nullptr);
+                TF_LAUNCH_NB(c->cwin, 8, quad_form_kernel, double, gn, c->stream, c->N.vals<double>(),
+                             c->B.as<double>(), pp + nb, nullptr);
             } else {
-                quad_form_kernel<float, 8><<<ge, kBlock, 0, c->stream>>>(c->Efull.vals<float>(), c->U.as<float>(), pp,
-                                                                   nullptr);
-                quad_form_kernel<float, 8><<<gn, kBlock, 0, c->stream>>>(c->N.vals<float>(), c->B.as<float>(), pp + nb,
-                                                                   nullptr);
+                TF_LAUNCH_NB(c->cwin, 8, quad_form_kernel, float, ge, c->stream, c->Efull.vals<float>(),
+                             c->U.as<float>(), pp, nullptr);
+                TF_LAUNCH_NB(c->cwin, 8, quad_form_kernel, float, gn, c->stream, c->N.vals<float>(),
+                             c->B.as<float>(), pp + nb, nullptr);
             }
         } else if (c->prec == TF_PREC_FP64) {
             weighted_sq_kernel<double><<<nb, kBlock, 0, c->stream>>>(c->E, c->U.as<double>(), c->fit_ep.as<double>(), pp);
@@ -2672,6 +2736,7 @@ tf_status tf_sell_info(tf_ctx* c, int32_t which, int64_t* info) {
     info[2] = s.nnz;
     info[3] = s.entries;
     info[4] = s.c16 ? 2 : 4;
+    info[5] = s.c16 ? (s.cwin ? 4 : 1) : 0;
     return TF_OK;
 }
 

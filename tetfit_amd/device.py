@@ -175,11 +175,13 @@ class Simulation:
     def sell_info(self, which):
         """tf_sell_info: storage of a device operator ("ct", "h", "n", "a",
         "k" (gfdtd residual operator E_f - [A 0]), "rhs1", "ctf") as
-        dict(rows, slices, nnz, entries, col_bytes)."""
+        dict(rows, slices, nnz, entries, col_bytes, col_bases) (col_bases:
+        column bases per slice, 1 or 4 with 16-bit codes, 0 with int32)."""
         k = ["ct", "h", "n", "a", "k", "rhs1", "ctf"].index(which)
-        info = (C.c_int64 * 5)()
+        info = (C.c_int64 * 6)()
         _check(hip_lib().tf_sell_info(self._h, k, info))
-        return dict(rows=info[0], slices=info[1], nnz=info[2], entries=info[3], col_bytes=info[4])
+        return dict(rows=info[0], slices=info[1], nnz=info[2], entries=info[3], col_bytes=info[4],
+                    col_bases=info[5])
 
 
 class BatchSimulation:
