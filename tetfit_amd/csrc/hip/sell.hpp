@@ -63,6 +63,20 @@ __device__ __forceinline__ int32_t win_col(const ColWin& w, uint32_t code) {
 
 __device__ __forceinline__ float bf16_to_float(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
 
+// Loads of the operator's values and column codes, which each SpMV reads
+// exactly once. With NT (the wide-mesh instantiation) they are non-temporal,
+// so that the caches evict them first and keep the gathered vector, which
+// every row touches again about two mesh planes of rows later: on a mesh
+// whose operators exceed the 256 MB Infinity Cache (96^3) the matrix bytes
+// streamed in between also exceed an XCD's 4 MB L2, and this cuts the
+// gathers' re-fetches. On a smaller mesh the operators stay resident in the
+// Infinity Cache from one pass to the next, which non-temporal loads give up.
+template <bool NT, typename X>
+__device__ __forceinline__ X stream_load(const X* p) {
+    if constexpr (NT) return __builtin_nontemporal_load(p);
+    else return *p;
+}
+
 template <typename A, typename V>
 __device__ __forceinline__ A sell_value(V v) {
     return (A)v;
@@ -84,7 +98,9 @@ __device__ __forceinline__ double sell_value<double, uint16_t>(uint16_t v) {
 // loads, then U independent gathers) so that each lane keeps enough memory
 // traffic in flight; the tail batch is clamped to the row's last entry and
 // predicated. Accumulation stays in entry order, so the result does not
-// depend on U or on the launch shape.
+// depend on U or on the launch shape. W selects the wide-mesh instantiation:
+// the windowed 16-bit column decode compiled in and non-temporal operator
+// loads (stream_load); the kernels below take W from their launch.
 template <typename A, int U, bool W, typename V, class G>
 __device__ __forceinline__ double sell_dot(const SellView<V>& m, int s, int lane, G gather) {
     s = __builtin_amdgcn_readfirstlane(s);
@@ -99,7 +115,7 @@ __device__ __forceinline__ double sell_dot(const SellView<V>& m, int s, int lane
         for (int u = 0; u < U; ++u) {
             const int kk = full ? k0 + u : min(k0 + u, w - 1);  // clamped: loads stay inside the slice
             ci[u] = col_at(kk * kSlice + lane);
-            vi[u] = sell_value<A, V>(v[kk * kSlice + lane]);
+            vi[u] = sell_value<A, V>(stream_load<W>(v + kk * kSlice + lane));
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) gi[u] = (A)gather(ci[u]);
@@ -115,14 +131,14 @@ __device__ __forceinline__ double sell_dot(const SellView<V>& m, int s, int lane
     if (W && m.col16 && m.cwin) {
         const uint16_t* c = m.col16 + base;
         const ColWin cw = col_windows(m, s);
-        rows([&](int o) { return win_col(cw, c[o]); });
+        rows([&](int o) { return win_col(cw, stream_load<W>(c + o)); });
     } else if (m.col16) {
         const uint16_t* c = m.col16 + base;
         const int32_t cb = __builtin_amdgcn_readfirstlane(m.cbase[s]);
-        rows([&](int o) { return cb + (int32_t)c[o]; });
+        rows([&](int o) { return cb + (int32_t)stream_load<W>(c + o); });
     } else {
         const int32_t* c = m.col + base;
-        rows([&](int o) { return c[o]; });
+        rows([&](int o) { return stream_load<W>(c + o); });
     }
     return (double)acc;
 }
@@ -484,7 +500,7 @@ __device__ __forceinline__ void sell_dot_rs(const SellView<uint16_t>& m, const u
         const uint32_t* q = pk + base;
         const ColWin cw = col_windows(m, s);
         rows([&](int o, int32_t& c, T& v) {
-            const uint32_t e = q[o];
+            const uint32_t e = stream_load<W>(q + o);
             c = win_col(cw, e >> 16);
             v = (T)bf16_to_float((uint16_t)(e & 0xFFFFu));
         });
@@ -492,7 +508,7 @@ __device__ __forceinline__ void sell_dot_rs(const SellView<uint16_t>& m, const u
         const uint32_t* q = pk + base;
         const int32_t cb = __builtin_amdgcn_readfirstlane(m.cbase[s]);
         rows([&](int o, int32_t& c, T& v) {
-            const uint32_t e = q[o];
+            const uint32_t e = stream_load<W>(q + o);
             c = cb + (int32_t)(e >> 16);
             v = (T)bf16_to_float((uint16_t)(e & 0xFFFFu));
         });
@@ -501,23 +517,23 @@ __device__ __forceinline__ void sell_dot_rs(const SellView<uint16_t>& m, const u
         const uint16_t* vv = m.val + base;
         const ColWin cw = col_windows(m, s);
         rows([&](int o, int32_t& c, T& v) {
-            c = win_col(cw, c16[o]);
-            v = (T)bf16_to_float(vv[o]);
+            c = win_col(cw, stream_load<W>(c16 + o));
+            v = (T)bf16_to_float(stream_load<W>(vv + o));
         });
     } else if (m.col16) {
         const uint16_t* c16 = m.col16 + base;
         const uint16_t* vv = m.val + base;
         const int32_t cb = __builtin_amdgcn_readfirstlane(m.cbase[s]);
         rows([&](int o, int32_t& c, T& v) {
-            c = cb + (int32_t)c16[o];
-            v = (T)bf16_to_float(vv[o]);
+            c = cb + (int32_t)stream_load<W>(c16 + o);
+            v = (T)bf16_to_float(stream_load<W>(vv + o));
         });
     } else {
         const int32_t* c32 = m.col + base;
         const uint16_t* vv = m.val + base;
         rows([&](int o, int32_t& c, T& v) {
-            c = c32[o];
-            v = (T)bf16_to_float(vv[o]);
+            c = stream_load<W>(c32 + o);
+            v = (T)bf16_to_float(stream_load<W>(vv + o));
         });
     }
     gr = ar;

@@ -108,8 +108,9 @@ struct KTune {
     int unroll = 8;
 };
 
-// SELL kernels take the gather batch NB and W (compile the decode of
-// windowed 16-bit columns in, set when some operator of the context uses them).
+// SELL kernels take the gather batch NB and W, the wide-mesh instantiation
+// (windowed 16-bit column decode, non-temporal operator loads), launched for
+// contexts with tf_ctx::wide set.
 #define TF_LAUNCH_NB(w, nb, kern, T, grid, stream, ...)                                      \
     do {                                                                                    \
         if (w) {                                                                            \
@@ -170,7 +171,10 @@ struct tf_ctx {
     // FIT grid order: fields, coefficients and codes stored at the grid
     // positions of edges and faces (fit_edge_slots / fit_face_slots)
     bool fit_grid = false;
-    bool cwin = false;  // some SELL operator uses windowed 16-bit columns: launch the SELL kernels with W
+    // launch the SELL kernels' wide-mesh instantiation (W): set when some
+    // operator uses windowed 16-bit columns, or when the operators stepped
+    // each step exceed the 256 MB Infinity Cache
+    bool wide = false;
     FitGrid grid{};
     std::vector<int32_t> h_esid, h_fsid;  // reference edge / face -> grid position
     std::vector<uint16_t> h_fe_code;      // per grid edge: slot map << 8 | signs (0: not in the main pass)
@@ -555,7 +559,7 @@ void launch_timed(tf_ctx* c, const char* name, K kernel, int grid, Args... args)
 }
 #define TF_LAUNCH_TIMED(c, name, nb, kern, T, grid, ...)                                       \
     do {                                                                                    \
-        if ((c)->cwin) {                                                                    \
+        if ((c)->wide) {                                                                    \
             switch (nb) {                                                                   \
                 case 4: launch_timed(c, name, kern<T, 4, true>, grid, __VA_ARGS__); break;     \
                 case 16: launch_timed(c, name, kern<T, 16, true>, grid, __VA_ARGS__); break;   \
@@ -915,9 +919,20 @@ void create_tet(tf_ctx* c, const tf_problem* p) {
         TF_HIP(hipHostMalloc((void**)&c->h_state, 2 * sizeof(CgState), hipHostMallocDefault));
         std::memset((void*)c->h_state, 0, 2 * sizeof(CgState));
     }
-    c->cwin = false;
+    // wide-mesh kernels when some operator needs windowed columns, or when
+    // the operators a step streams repeatedly cannot stay in the 256 MB
+    // Infinity Cache: dga reads H, N and Ct once per step, one step after
+    // another; gfdtd re-reads A (and the learned G, 4 bytes per A entry) in
+    // every CG pass of a step
+    auto bytes = [](const Sell& m) {
+        return m.sptr.bytes + (m.c16 ? m.col16.bytes : m.col.bytes) + m.val.bytes + m.sg.bytes;
+    };
+    c->wide = false;
     for (const Sell* m : {&c->sCt, &c->H, &c->N, &c->A, &c->Kf, &c->R1f, &c->Ctf, &c->Efull, &c->Sig})
-        c->cwin = c->cwin || m->cwin;
+        c->wide = c->wide || m->cwin;
+    const size_t streamed = p->method == TF_METHOD_DGA ? bytes(c->sCt) + bytes(c->H) + bytes(c->N)
+                                                       : bytes(c->A) + 4 * (size_t)c->A.entries;
+    c->wide = c->wide || streamed > ((size_t)256 << 20);
     // initial U: fixed (class-4) values
     std::vector<T> u0(E, 0);
     for (int32_t e = 0; e < E; ++e)
@@ -2252,19 +2267,19 @@ void autotune(tf_ctx* c) {
     hipStream_t s = c->stream;
     if (c->method == TF_METHOD_DGA) {
         c->tune[TK_CT] = tune_one(c, TK_CT, c->sCt, kSpmvCap, [&](int b, int u) {
-            TF_LAUNCH_NB(c->cwin, u, sell_ct_kernel, T, b, s, c->sCt.signs(), c->Fv.as<T>(), c->curl_f.as<T>());
+            TF_LAUNCH_NB(c->wide, u, sell_ct_kernel, T, b, s, c->sCt.signs(), c->Fv.as<T>(), c->curl_f.as<T>());
         });
         c->tune[TK_H] = tune_one(c, TK_H, c->H, kSpmvCap, [&](int b, int u) {
-            TF_LAUNCH_NB(c->cwin, u, sell_h_kernel, T, b, s, c->H.vals<T>(), c->h_edge.as<int32_t>(), c->curl_f.as<T>(),
+            TF_LAUNCH_NB(c->wide, u, sell_h_kernel, T, b, s, c->H.vals<T>(), c->h_edge.as<int32_t>(), c->curl_f.as<T>(),
                          c->U.as<T>(), 0.0, nullptr);
         });
         c->tune[TK_N] = tune_one(c, TK_N, c->N, kSpmvCap, [&](int b, int u) {
-            TF_LAUNCH_NB(c->cwin, u, sell_n_kernel, T, b, s, c->N.vals<T>(), c->curl_u.as<T>(), c->Fv.as<T>(), 0.0, 0, nullptr,
+            TF_LAUNCH_NB(c->wide, u, sell_n_kernel, T, b, s, c->N.vals<T>(), c->curl_u.as<T>(), c->Fv.as<T>(), 0.0, 0, nullptr,
                          nullptr);
         });
     } else if (c->method == TF_METHOD_GFDTD) {
         c->tune[TK_N] = tune_one(c, TK_N, c->N, kSpmvCap, [&](int b, int u) {
-            TF_LAUNCH_NB(c->cwin, u, sell_n_kernel, T, b, s, c->N.vals<T>(), c->B.as<T>(), c->Fv.as<T>(), c->dt, 1, nullptr,
+            TF_LAUNCH_NB(c->wide, u, sell_n_kernel, T, b, s, c->N.vals<T>(), c->B.as<T>(), c->Fv.as<T>(), c->dt, 1, nullptr,
                          nullptr);
         });
         // Tuning launches run pass 1 (the full work of a pass) on a scratch
@@ -2288,7 +2303,7 @@ void autotune(tf_ctx* c) {
         const int nbb = std::max(1, std::min(blocks_for(nf), kCgPartB));  // cg_upd's grid (CgView)
         // Jacobi pass: cg_w (stop test on cg_upd's partials) + the cg_upd that sums its b partials
         c->tune[TK_W] = tune_one(c, TK_W, c->A, kCgPartA, [&](int b, int u) {
-            TF_LAUNCH_NB(c->cwin, u, cg_w_kernel, T, b, s, 1, c->A.vals<T>(), c->z.as<T>(), c->q.as<T>(), 1, one, nbb, part,
+            TF_LAUNCH_NB(c->wide, u, cg_w_kernel, T, b, s, 1, c->A.vals<T>(), c->z.as<T>(), c->q.as<T>(), 1, one, nbb, part,
                          st);
             cg_upd_kernel<T><<<nbb, kBlock, 0, s>>>(1, nf, c->pre.as<T>(), c->q.as<T>(), c->r.as<T>(), c->z.as<T>(),
                                                     c->q.as<T>(), c->p0.as<T>(), c->p1.as<T>(), one, nbb, one, b,
@@ -2298,11 +2313,11 @@ void autotune(tf_ctx* c) {
             const SellView<uint16_t> gv = c->A.view<uint16_t>(c->lp.gval);
             const uint32_t* gpk = c->lp.gpk.p ? c->lp.gpk.as<uint32_t>() : nullptr;
             auto pg = [&](int b, int u, int nparts) {
-                TF_LAUNCH_NB(c->cwin, u, cg_pg_kernel, T, b, s, 1, gv, gpk, c->rs0.as<RS<T>>(), c->rs1.as<RS<T>>(),
+                TF_LAUNCH_NB(c->wide, u, cg_pg_kernel, T, b, s, 1, gv, gpk, c->rs0.as<RS<T>>(), c->rs1.as<RS<T>>(),
                              c->p0.as<T>(), c->q.as<T>(), c->z.as<T>(), one, nparts, part, part, st);
             };
             auto wu = [&](int b, int u, int nparts) {
-                TF_LAUNCH_NB(c->cwin, u, cg_wu_kernel, T, b, s, 1, c->A.vals<T>(), c->z.as<T>(), c->p0.as<T>(),
+                TF_LAUNCH_NB(c->wide, u, cg_wu_kernel, T, b, s, 1, c->A.vals<T>(), c->z.as<T>(), c->p0.as<T>(),
                              c->rs1.as<RS<T>>(), one, one, nparts, part, st);
             };
             // cg_pg (summing cg_wu's partials) + the cg_wu that sums its b partials, then
@@ -2609,14 +2624,14 @@ tf_status tf_energy(tf_ctx* c, double* we, double* wh) {
             ge = sell_blocks(c->Efull, nb);
             gn = sell_blocks(c->N, nb);
             if (c->prec == TF_PREC_FP64) {
-                TF_LAUNCH_NB(c->cwin, 8, quad_form_kernel, double, ge, c->stream, c->Efull.vals<double>(),
+                TF_LAUNCH_NB(c->wide, 8, quad_form_kernel, double, ge, c->stream, c->Efull.vals<double>(),
                              c->U.as<double>(), pp, nullptr);
-                TF_LAUNCH_NB(c->cwin, 8, quad_form_kernel, double, gn, c->stream, c->N.vals<double>(),
+                TF_LAUNCH_NB(c->wide, 8, quad_form_kernel, double, gn, c->stream, c->N.vals<double>(),
                              c->B.as<double>(), pp + nb, nullptr);
             } else {
-                TF_LAUNCH_NB(c->cwin, 8, quad_form_kernel, float, ge, c->stream, c->Efull.vals<float>(),
+                TF_LAUNCH_NB(c->wide, 8, quad_form_kernel, float, ge, c->stream, c->Efull.vals<float>(),
                              c->U.as<float>(), pp, nullptr);
-                TF_LAUNCH_NB(c->cwin, 8, quad_form_kernel, float, gn, c->stream, c->N.vals<float>(),
+                TF_LAUNCH_NB(c->wide, 8, quad_form_kernel, float, gn, c->stream, c->N.vals<float>(),
                              c->B.as<float>(), pp + nb, nullptr);
             }
         } else if (c->prec == TF_PREC_FP64) {
