@@ -523,7 +523,7 @@ def main():
                     droof.append(dict(kernel=k, dtype="f64" if vb_d == 8 else "f32", bound="hbm",
                                       achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                                       frac=round(ach / HBM_PEAK_GBS, 4), bytes_per_launch=int(nb),
-                                      ms_per_launch=t["ms_avg"]))
+                                      ms_per_launch=t["ms_avg"], traffic=measured_traffic(k, vb_d, nb)))
             line["dga_explicit"]["roofline_kernels"] = droof
         line["kernels"] = head["kernels"]
         print(json.dumps(line), flush=True)
