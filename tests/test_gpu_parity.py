@@ -228,20 +228,24 @@ def test_batch_graph_steps_match_eager(workdir):
             assert np.array_equal(a, e)
 
 
-@pytest.mark.parametrize("layout", ["col32", "win"])
+@pytest.mark.parametrize("layout", ["col32", "win", "wide"])
 @pytest.mark.parametrize("method", ["dga", "gfdtd"])
 def test_column_layouts_parity(workdir, method, layout, monkeypatch):
     """Every SELL operator stored with int32 column indices, or with 16-bit
     codes over four column windows per slice (the layouts for meshes whose
-    slices span more than 65,536 columns), gives the same fp64 parity."""
-    monkeypatch.setenv("TETFIT_SELL_COL32" if layout == "col32" else "TETFIT_SELL_WIN", "1")
+    slices span more than 65,536 columns), gives the same fp64 parity; so do
+    the wide-mesh kernels (non-temporal operator loads) on one-base 16-bit
+    columns, the combination of a large mesh numbered compactly."""
+    env = {"col32": "TETFIT_SELL_COL32", "win": "TETFIT_SELL_WIN", "wide": "TETFIT_SELL_WIDE"}[layout]
+    monkeypatch.setenv(env, "1")
     b = Model(configs.kuhn_cavity(str(workdir), n=5, method=method, n_steps=60, n_probes=4, block=2,
                                   name=f"c{layout}_{method}")).build(1)
     print(f"case: {method} {layout} columns E={b.n_edges}", flush=True)
     sim = Simulation(b, "fp64")
     for op in ("n", "a") if method == "gfdtd" else ("n", "h", "ct"):
         info = sim.sell_info(op)
-        assert (info["col_bytes"], info["col_bases"]) == ((4, 0) if layout == "col32" else (2, 4)), (op, info)
+        want = {"col32": (4, 0), "win": (2, 4), "wide": (2, 1)}[layout]
+        assert (info["col_bytes"], info["col_bases"]) == want, (op, info)
     sim.close()
     check_run(b)
 

@@ -933,6 +933,10 @@ void create_tet(tf_ctx* c, const tf_problem* p) {
     const size_t streamed = p->method == TF_METHOD_DGA ? bytes(c->sCt) + bytes(c->H) + bytes(c->N)
                                                        : bytes(c->A) + 4 * (size_t)c->A.entries;
     c->wide = c->wide || streamed > ((size_t)256 << 20);
+    // $TETFIT_SELL_WIDE (non-empty, not 0) selects the wide-mesh kernels on
+    // any mesh, to exercise them with one-base columns on small meshes
+    if (const char* v = std::getenv("TETFIT_SELL_WIDE"))
+        if (*v && std::strcmp(v, "0") != 0) c->wide = true;
     // initial U: fixed (class-4) values
     std::vector<T> u0(E, 0);
     for (int32_t e = 0; e < E; ++e)
