@@ -193,19 +193,21 @@ def test_learned_preconditioner_gain(contrast_cavity, prec, bar):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("layout", ["col32", "win"])
+@pytest.mark.parametrize("layout", ["col32", "win", "wide"])
 def test_learned_column_layouts(contrast_cavity, layout, monkeypatch):
     """The same learned run with A (and G beside it) stored with int32
     column indices (the unpacked G path of cg_pg_kernel and the int32 path of
-    the SELL kernels), or with 16-bit codes over four column windows per
-    slice (packed G decoded through the windows)."""
+    the SELL kernels), with 16-bit codes over four column windows per slice
+    (packed G decoded through the windows), or with one-base 16-bit columns
+    in the wide-mesh kernels (non-temporal operator loads)."""
     from tetfit_amd.device import Simulation
-    monkeypatch.setenv("TETFIT_SELL_COL32" if layout == "col32" else "TETFIT_SELL_WIN", "1")
+    env = {"col32": "TETFIT_SELL_COL32", "win": "TETFIT_SELL_WIN", "wide": "TETFIT_SELL_WIDE"}[layout]
+    monkeypatch.setenv(env, "1")
     b = contrast_cavity
     print(f"case: learned fp32 {layout} columns E={b.n_edges} steps={b.n_steps}", flush=True)
     sim = Simulation(b, "fp32")
     info = sim.sell_info("a")
-    assert (info["col_bytes"], info["col_bases"]) == ((4, 0) if layout == "col32" else (2, 4))
+    assert (info["col_bytes"], info["col_bases"]) == {"col32": (4, 0), "win": (2, 4), "wide": (2, 1)}[layout]
     s1 = sim.run(n=30, precond="learned", learn_steps=30, train_iters=80)
     s2 = sim.run(n=b.n_steps - 30, precond="learned")
     _, samples = sim.probes()
