@@ -39,6 +39,7 @@ struct SellView {
     const int32_t* cbase;
     const V* val;
     int32_t cwin;
+    int32_t band;  // waves take each XCD's slices round-robin (block_slices)
 };
 
 // Column of a 16-bit code in the windowed form, with the slice's four bases
@@ -147,22 +148,35 @@ __device__ __forceinline__ int wave_id() { return (blockIdx.x * blockDim.x + thr
 __device__ __forceinline__ int wave_count() { return gridDim.x * blockDim.x / kWave; }
 __device__ __forceinline__ int lane_id() { return threadIdx.x & (kWave - 1); }
 
-// Slices [s0, s1) of this workgroup: the logical blocks (xcd_block) split
-// the slices into contiguous chunks, so each XCD streams one contiguous
-// part of the matrix and its L2 holds the part of the gathered vector that
-// those rows touch. The waves of a block take the chunk's slices in turn.
+// Slices of this wave. Each XCD takes one contiguous eighth of the slices,
+// so its L2 holds the part of the gathered vector that those rows touch.
+// Default: the XCD's part is split into one contiguous chunk per workgroup
+// (logical blocks, xcd_block) and the waves of a workgroup take the chunk's
+// slices in turn. With band set (batched dga: many small block-diagonal
+// instances), the waves of all the XCD's workgroups take its slices
+// round-robin, so at any moment they work on one band of consecutive
+// slices; with fewer than 8 workgroups the parts are as many as the
+// workgroups. Either way every slice goes to exactly one wave.
 struct SliceRange {
     int s, end, step;
 };
-__device__ __forceinline__ SliceRange block_slices(int nslices) {
+__device__ __forceinline__ SliceRange block_slices(int nslices, int band) {
+    const int wpb = (int)(blockDim.x / kWave), w = (int)(threadIdx.x / kWave);
+    if (band) {
+        const int g = (int)gridDim.x, b = (int)blockIdx.x;
+        const int nx = g < kXcd ? g : kXcd, x = b % nx, i = b / nx;
+        const int bx = g / nx + (x < g % nx ? 1 : 0);  // workgroups of part x
+        const int x0 = (int)((int64_t)nslices * x / nx), x1 = (int)((int64_t)nslices * (x + 1) / nx);
+        return SliceRange{x0 + i * wpb + w, x1, bx * wpb};
+    }
     const int lb = xcd_block();
     const int s0 = (int)((int64_t)nslices * lb / gridDim.x);
     const int s1 = (int)((int64_t)nslices * (lb + 1) / gridDim.x);
-    return SliceRange{s0 + (int)(threadIdx.x / kWave), s1, (int)(blockDim.x / kWave)};
+    return SliceRange{s0 + w, s1, wpb};
 }
 
 #define TF_FOR_SLICES(m)                                                               \
-    for (SliceRange sr_ = block_slices((m).nslices); sr_.s < sr_.end; sr_.s += sr_.step)
+    for (SliceRange sr_ = block_slices((m).nslices, (m).band); sr_.s < sr_.end; sr_.s += sr_.step)
 
 // curl_f = Ct F over edges (dga, D:878-879).
 template <typename T, int NB, bool W = false>
@@ -564,7 +578,7 @@ __global__ void __launch_bounds__(kBlock) cg_pg_kernel(int32_t it, SellView<uint
     const double gam = st->gam[k], beta = st->beta[k], alpha_pp = st->alpha[k ^ 1];
     if (done) return;
     const int lane = lane_id();
-    SliceRange sr = block_slices(g.nslices);
+    SliceRange sr = block_slices(g.nslices, g.band);
     const bool first = sr.s < sr.end;
     T gr0 = 0, gs0 = 0;
     if (first) sell_dot_rs<T, NB, W>(g, gpk, sr.s, lane, rs_in, gr0, gs0);

@@ -87,10 +87,12 @@ struct Sell {
     int64_t nnz = 0, entries = 0;
     bool c16 = false;   // 16-bit column codes (col16 + cbase) in the kernels' view
     bool cwin = false;  // ... in the four-window form (SellView::cwin)
+    bool band = false;  // banded slice order (SellView::band), batched dga
     template <class V>
     SellView<V> view(const DevBuf& v) const {
         return SellView<V>{nrows, nslices, sptr.as<int64_t>(), c16 ? nullptr : col.as<int32_t>(),
-                           c16 ? col16.as<uint16_t>() : nullptr, cbase.as<int32_t>(), v.as<V>(), cwin ? 1 : 0};
+                           c16 ? col16.as<uint16_t>() : nullptr, cbase.as<int32_t>(), v.as<V>(), cwin ? 1 : 0,
+                           band ? 1 : 0};
     }
     template <class T>
     SellView<T> vals() const { return view<T>(val); }
@@ -2368,6 +2370,9 @@ tf_status tf_create_batch(const tf_problem* const* probs, int32_t n, int32_t pre
         MergedDga m;
         merge_dga(probs, n, m);
         auto c = make_ctx(&m.p, prec);
+        // the instances' operators are block-diagonal: banded slice order
+        // (measured 2-3% faster on 256 instances of 8^3-32^3)
+        for (Sell* s : {&c->sCt, &c->H, &c->N}) s->band = true;
         auto bt = std::make_unique<tf_ctx::Batch>();
         bt->n = n;
         bt->eoff = m.eoff;
