@@ -130,20 +130,41 @@ inline int slab_grid(const uint32_t n[3]) {
         m = std::max(m, slab_len(n[0], x) + slab_len(n[1], x) + slab_len(n[2], x));
     return (int)(m * kXcd);
 }
+// The XCD's list interleaves the three directions block by block (x, y, z,
+// x, y, z, ...; the longer directions' last few blocks follow), so that all
+// three advance through the slab's planes together and a plane of gathered
+// neighbours, read by two directions, is fetched once while it is in L2.
 __device__ __forceinline__ int32_t slab_item(int b, const uint32_t n[3]) {
-    const uint32_t x = (uint32_t)b % kXcd;
-    uint32_t i = (uint32_t)b / kXcd, base = 0;
+    const uint32_t x = (uint32_t)b % kXcd, i = (uint32_t)b / kXcd;
+    uint32_t lo[3], len[3];
 #pragma unroll
     for (int t = 0; t < 3; ++t) {
-        const uint32_t nb = (n[t] + kBlock - 1) / kBlock, lo = x * nb / kXcd, len = (x + 1) * nb / kXcd - lo;
-        if (i < len) {
-            const uint32_t r = (lo + i) * kBlock + threadIdx.x;
-            return r < n[t] ? (int32_t)(base + r) : -1;
-        }
-        i -= len;
-        base += n[t];
+        const uint32_t nb = (n[t] + kBlock - 1) / kBlock;
+        lo[t] = x * nb / kXcd;
+        len[t] = (x + 1) * nb / kXcd - lo[t];
     }
-    return -1;
+    const uint32_t m = min(len[0], min(len[1], len[2]));
+    int t = -1;
+    uint32_t j = 0;
+    if (i < 3 * m) {
+        t = (int)(i % 3);
+        j = i / 3;
+    } else {
+        uint32_t k = i - 3 * m;
+#pragma unroll
+        for (int u = 0; u < 3; ++u) {
+            if (t < 0 && k < len[u] - m) {
+                t = u;
+                j = m + k;
+            } else if (t < 0) {
+                k -= len[u] - m;
+            }
+        }
+        if (t < 0) return -1;
+    }
+    const uint32_t base = t == 0 ? 0u : (t == 1 ? n[0] : n[0] + n[1]);
+    const uint32_t r = (lo[t] + j) * kBlock + threadIdx.x;
+    return r < n[t] ? (int32_t)(base + r) : -1;
 }
 
 // 32-bit index arithmetic: grid order is used only when edge and face
