@@ -293,7 +293,10 @@ def bench_fitprec(args):
     chunk = 100
     with tempfile.TemporaryDirectory() as d:
         t = time.time()
-        b = Model(configs.fit_random_media(d, n=n, n_steps=steps, n_probes=16)).build(1)
+        log(f"[bench] writing the fdtd random media {n}^3 input")
+        path = configs.fit_random_media(d, n=n, n_steps=steps, n_probes=16)
+        log(f"[bench] input written ({time.time() - t:.1f}s); meshing and assembling on the host")
+        b = Model(path).build(1)
         log(f"[bench] built fdtd random media {n}^3: E={b.n_edges} F={b.n_faces} dt={b.dt:.4g} "
             f"({time.time() - t:.1f}s host set-up)")
         out = {}
