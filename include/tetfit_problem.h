@@ -131,7 +131,13 @@ typedef enum {
 typedef enum {
     TF_FIELD_H = 0,            /* boundary magnetic source (Fb / curl_fb)    */
     TF_FIELD_E = 1,            /* imposed edge voltage U(e)                   */
-    TF_FIELD_J = 2             /* impressed current I(e)                      */
+    TF_FIELD_J = 2,            /* impressed current I(e)                      */
+    TF_FIELD_D = 3             /* any other type on an edge (d): the loops of */
+                               /* D:827-842 and D:1485-1500 route such an     */
+                               /* edge to ComputeEfieldSource, which sums "e" */
+                               /* sources only (D:3928) and so gives 0: dga   */
+                               /* resets the edge's U to 0 before the H       */
+                               /* update, gfdtd overwrites it with the solve  */
 } tf_field_kind;
 
 /* Time part of one DEFINE source (Source::Compute's carrier, Source.hpp:316-327). */

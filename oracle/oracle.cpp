@@ -374,8 +374,14 @@ void run_dga(const tf_problem* p, const SourceTable& st, const tf_probe_geom* pg
         if (d.edge_class[e] == 4) U_d[e] = d.fixed_value[e];  // ComputeEdgeBC(ee,0)
     }
     // initial e-sources at t = 0 (D:762-790)
-    for (int64_t k = 0; k < (int64_t)st.se.size(); ++k)
-        if (st.se[k].field == TF_FIELD_E && d.edge_class[st.se[k].edge] == 4) U_d[st.se[k].edge] = e_source(st, k, 0.0);
+    // before the loop every source edge, h edges included, takes
+    // ComputeEfieldSource(ee, 0), the sum of its "e" sources (D:746-765)
+    for (int64_t k = 0; k < (int64_t)st.se.size(); ++k) {
+        const int32_t e = st.se[k].edge;
+        if (st.se[k].field == TF_FIELD_J) continue;  // rejected for dga (A-7)
+        if (d.edge_class[e] == 4) U_d[e] = e_source(st, k, 0.0);
+        else if (d.h_row[e] >= 0) U_a[d.h_row[e]] = e_source(st, k, 0.0);
+    }
     auto assemble_U = [&]() {  // U = M*U_frac
         for (int32_t e = 0; e < E; ++e) {
             if (d.h_row[e] >= 0) U[e] = U_a[d.h_row[e]];
@@ -400,7 +406,9 @@ void run_dga(const tf_problem* p, const SourceTable& st, const tf_probe_geom* pg
         for (int64_t k = 0; k < (int64_t)st.se.size(); ++k) {
             const tf_source_edge& s = st.se[k];
             if (s.field == TF_FIELD_H) curl_b[s.edge] = h_source(st, k, t - 0.5 * dt);  // Ctb*Fb
-            else if (s.field == TF_FIELD_E) U_d[s.edge] = e_source(st, k, t);
+            else if (s.field == TF_FIELD_E && d.edge_class[s.edge] == 4) U_d[s.edge] = e_source(st, k, t);
+            else if (s.field != TF_FIELD_J && d.h_row[s.edge] >= 0)
+                U_a[d.h_row[s.edge]] = e_source(st, k, t);  // a d edge: U_a = 0 before the H update (D:835-848)
         }
         // curl_f = C^T F + curl_b - I
         csr_mv(d.Ct, F, cf);
@@ -515,7 +523,7 @@ void run_gfdtd(const tf_problem* p, const SourceTable& st, const tf_probe_geom* 
             } else if (s.field == TF_FIELD_J) {
                 if (h_too) I[s.edge] = j_source(st, k, t);  // D:1491-1492
                 SrcFld[s.edge] = U[s.edge];
-            } else if (s.field == TF_FIELD_E) {
+            } else {  // e, and d edges (whose "e" sum is 0; free, so the solve overwrites U)
                 U[s.edge] = e_source(st, k, t);
                 SrcFld[s.edge] = U[s.edge];
             }
@@ -594,6 +602,14 @@ void run_gfdtd(const tf_problem* p, const SourceTable& st, const tf_probe_geom* 
 extern "C" {
 
 const char* oracle_last_error(void) { return g_err.c_str(); }
+
+void oracle_export(int32_t fdtd, const tf_probe_geom* pg, int32_t n_probes, int32_t n_edges, int32_t n_faces,
+                   const double* U, const double* B, double* six) {
+    std::vector<double> u(U, U + n_edges), b(B, B + n_faces), out;
+    if (fdtd) export_fit(u, b, pg, n_probes, out);
+    else export_tet(u, b, pg, n_probes, out);
+    std::memcpy(six, out.data(), sizeof(double) * out.size());
+}
 
 void oracle_source_compute(const tf_source_desc* s, const double p[3], double t, double out3[3]) {
     V3 v = source_compute(*s, p, t);

@@ -80,6 +80,12 @@ int32_t oracle_cg_solve(int32_t n, const int64_t* rowptr, const int32_t* col, co
 
 const char* oracle_last_error(void);
 
+/* One probe sample of the state (U, B): fdtd=1 GetFitElectricField /
+ * GetFitMagneticField (D:3555-3588), else GetWhitneyElectricField /
+ * GetMagneticField (D:3768-3897); six[p*6 + c], c = Ex,Ey,Ez,Hx,Hy,Hz. */
+void oracle_export(int32_t fdtd, const tf_probe_geom* pg, int32_t n_probes, int32_t n_edges, int32_t n_faces,
+                   const double* U, const double* B, double* six);
+
 /* Preconditioner MLP (16 -> 64 -> 64 -> 1, ReLU) forward and backward in
  * fp64: y[i] and grad = sum_i gy[i] dy[i]/dparams. */
 void oracle_mlp(const float* params, const float* x, const double* gy, int64_t n, double* y, double* grad);

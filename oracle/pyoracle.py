@@ -44,6 +44,9 @@ def lib():
     L.oracle_cg_solve.argtypes = [C.c_int32, C.POINTER(C.c_int64), C.POINTER(C.c_int32), dp, dp, dp,
                                   C.c_double, C.c_int32, dp, dp]
     L.oracle_cg_solve.restype = C.c_int32
+    L.oracle_export.argtypes = [C.c_int32, C.POINTER(_ffi.tf_probe_geom), C.c_int32, C.c_int32, C.c_int32, dp, dp,
+                                dp]
+    L.oracle_export.restype = None
     fp = C.POINTER(C.c_float)
     L.oracle_mlp.argtypes = [fp, fp, dp, C.c_int64, dp, dp]
     L.oracle_mlp.restype = None
@@ -138,3 +141,20 @@ def mlp(params, x, gy=None):
     g = np.zeros(P.size)
     L.oracle_mlp(P.ctypes.data_as(fp), X.ctypes.data_as(fp), _dp(G), n, _dp(y), _dp(g))
     return y, g
+
+
+def export(build, U, B):
+    """Probe samples [probe, 6] of the state (U, B) by the checker's
+    GetFit*Field / GetWhitneyElectricField / GetMagneticField."""
+    from tetfit_amd import _ffi
+    L = lib()
+    H = _ffi.host_lib()
+    pg = C.POINTER(_ffi.tf_probe_geom)()
+    npg = C.c_int32()
+    H.tf_build_probe_geom(build._h, C.byref(pg), C.byref(npg))
+    u = np.ascontiguousarray(U, dtype=np.float64)
+    b = np.ascontiguousarray(B, dtype=np.float64)
+    six = np.zeros(max(npg.value, 1) * 6)
+    L.oracle_export(1 if build.method == "fdtd" else 0, pg, npg.value, build.n_edges, build.n_faces, _dp(u), _dp(b),
+                    _dp(six))
+    return six[:npg.value * 6].reshape(npg.value, 6)

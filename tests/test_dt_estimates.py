@@ -1,0 +1,84 @@
+"""The host's stability estimates (ComputeFDTDTimeStep D:7036-7119,
+ComputeDGATimeStep D:7121-7174, ComputeFEMTimeStep D:6928-7034, whose
+Spectra eigen-solves cannot be reproduced) against the exact largest
+eigenvalue of the same operators: a closed form for the FIT curl-curl on a
+uniform grid, dense eigenvalues for the tetrahedral operators. The host
+reports dt = 2/sqrt(lambda_est); lambda_est = 4/dt^2 must not fall below the
+true lambda_max (that would give an unstable dt) and exceeds it by at most
+the 2% pad on the power iteration."""
+import re
+
+import numpy as np
+import pytest
+import scipy.linalg as sl
+import scipy.sparse as sp
+
+from tetfit_amd import Model, configs
+
+EPS0 = 8.854187817e-12
+MU0 = 4e-7 * np.pi
+
+
+def csr(m):
+    v = m["val"] if m["val"] is not None else m["sgn"].astype(float)
+    return sp.csr_matrix((v, m["col"], m["rowptr"]), shape=(m["nrows"], m["ncols"]))
+
+
+def test_fit_curl_curl_closed_form(workdir):
+    """Vacuum box of 5 x 4 x 6 cells with steps 0.1, 0.07, 0.13. The FIT
+    operator diag(L/eps) Ct diag(nu/A) C on all edges has the Yee spectrum
+    4c^2 sum_i sin^2(m_i pi / (2 N_i)) / d_i^2 with m_i up to N_i, so
+    lambda_max = 4c^2 sum_i 1/d_i^2; on the edges left free by PEC walls
+    (the common edges) m_i stops at N_i - 1."""
+    p = configs.fit_cavity(str(workdir), n=5, step=0.1, n_steps=2, n_probes=1, name="dtfit", dt=1e-13)
+    s = open(p).read()
+    s = re.sub(r"ygrid \{[^}]*\}", "ygrid {0,0.07,0.2900001}", s)
+    s = re.sub(r"zgrid \{[^}]*\}", "zgrid {0,0.13,0.7900001}", s)
+    s = re.sub(r"SET probe \{[^}]*\}", "SET probe {0.25,0.15,0.3}", s)
+    s = re.sub(r"SET size \{[^}]*\}", "SET size {1,1,1}", s)
+    open(p, "w").write(s)
+    b = Model(p).build(1)
+    f = b.fit_ops()
+    assert f["cells"] == (5, 4, 6)
+    E, F = b.n_edges, b.n_faces
+    C = sp.csr_matrix((np.tile([1.0, -1.0, 1.0, -1.0], F) * np.repeat(f["curl"], 4),
+                       (np.repeat(np.arange(F), 4), f["c_edge"].ravel())), shape=(F, E))
+    Ct = sp.csr_matrix((f["ct_sgn"].astype(float), f["ct_face"], f["ct_ptr"]), shape=(E, F))
+    assert abs(Ct - C.T).max() == 0
+    op = (sp.diags(1 / f["ep_vec"]) @ Ct @ sp.diags(f["nu_vec"]) @ C).toarray()
+    c2 = 1 / (EPS0 * MU0)
+    d = np.array([0.1, 0.07, 0.13])
+    Nc = np.array([5, 4, 6])
+    lam_all = 4 * c2 * np.sum(1 / d ** 2)
+    lam_pec = 4 * c2 * np.sum(np.sin((Nc - 1) * np.pi / (2 * Nc)) ** 2 / d ** 2)
+    ev = np.linalg.eigvals(op).real
+    assert abs(ev.max() / lam_all - 1) < 1e-9
+    ce = f["common_edges"]
+    assert abs(np.linalg.eigvals(op[np.ix_(ce, ce)]).real.max() / lam_pec - 1) < 1e-9
+    est = 4 / b.stable_dt ** 2
+    assert 1.0 <= est / lam_all <= 1.021
+
+
+@pytest.mark.parametrize("method", ["dga", "gfdtd"])
+@pytest.mark.parametrize("n,seed", [(3, 1), (4, 2)])
+def test_tet_dt_estimate_bounds_lambda_max(workdir, method, n, seed):
+    """dga: lambda_max of H Ct N C on the H rows; gfdtd: the largest
+    generalized eigenvalue of (Ct N C, A) on the free edges, both from dense
+    eigen-solves on jittered Kuhn meshes with eps_r drawn per block."""
+    p = configs.kuhn_cavity(str(workdir), n=n, method=method, n_steps=2, n_probes=1, block=2, seed=seed,
+                            dt=1e-13, name=f"dt_{method}{n}")
+    b = Model(p).build(1)
+    o = b.dga_ops()
+    C, Ct, N = csr(o["C"]), csr(o["Ct"]), csr(o["N"])
+    if method == "dga":
+        H = csr(o["H"])
+        hr = o["h_row"]
+        rows = np.where(hr >= 0)[0]
+        P = sp.csr_matrix((np.ones(len(rows)), (rows, hr[rows])), shape=(b.n_edges, H.shape[0]))
+        lam = np.linalg.eigvals((H @ Ct @ N @ C @ P).toarray()).real.max()
+    else:
+        fe = o["free_edges"]
+        K = (Ct @ N @ C).toarray()[np.ix_(fe, fe)]
+        lam = sl.eigh(K, csr(o["A"]).toarray(), eigvals_only=True).max()
+    est = 4 / b.stable_dt ** 2
+    assert 1.0 <= est / lam <= 1.021

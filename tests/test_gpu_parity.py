@@ -250,19 +250,27 @@ def test_column_layouts_parity(workdir, method, layout, monkeypatch):
     check_run(b)
 
 
-@pytest.mark.parametrize("prec", ["fp64", "fp32"])
-def test_fit_grid_order_matches_indexed(workdir, prec, monkeypatch):
+@pytest.mark.parametrize("prec,n", [("fp64", 10), ("fp32", 10), ("fp64", 40)])
+def test_fit_grid_order_matches_indexed(workdir, prec, n, monkeypatch):
     """The FIT step in grid order (faces of an edge and edges of a face found
     from their grid positions) against the index-table layout on a non-cubic
     grid with an h source, a j dipole and an e source: probes, Poynting
     values, final fields and energy bitwise equal (both sum each edge's four
     faces in the reference's order) and the energy to rounding; the grid
-    order holds less memory."""
+    order holds less memory. The n x 8 x n grids: at n = 10 the x, y and z
+    edges (and faces) each fill 4 blocks of 256, so every XCD's slab has
+    equal direction lengths; at n = 40 they fill 58, 53 and 58 blocks, so
+    the XCDs' longer directions append their extra blocks after the
+    interleaved part."""
     import re
-    p = configs.fit_cavity(str(workdir), n=10, step=0.1, n_steps=80, n_probes=5, name=f"fgo_{prec}")
-    s = open(p).read().replace("ygrid {0,0.1,1.025}", "ygrid {0,0.1,0.825}")
-    blocks = ("DEFINE source 2\n    SET type j\n    SET location {0.55,0.45,0.35}\n    SET direction z\n"
-              "    SET carrier ricker\n    SET frequency 3e8\n    SET delay 5e-9\nEND source 2\n"
+    step = 1.0 / n
+    p = configs.fit_cavity(str(workdir), n=n, step=step, n_steps=80, n_probes=5, name=f"fgo_{prec}_{n}")
+    s = open(p).read()
+    s = re.sub(r"ygrid \{0,([^,]*),([^}]*)\}", lambda m: "ygrid {0,%s,%r}" % (m.group(1), 8.25 * step), s)
+    s = re.sub(r"SET probe \{([^,]*),([^,]*),([^}]*)\}",
+               lambda m: "SET probe {%s,%r,%s}" % (m.group(1), 4.5 * step, m.group(3)), s)
+    blocks = ("DEFINE source 2\n    SET type j\n    SET location {0.55,%r,0.35}\n    SET direction z\n"
+              "    SET carrier ricker\n    SET frequency 3e8\n    SET delay 5e-9\nEND source 2\n" % (4.5 * step) +
               "DEFINE source 3\n    SET type e\n    SET surface 4\n    SET direction y\n    SET carrier gaussian\n"
               "    SET width 5e-9\n    SET delay 1e-8\nEND source 3\n")
     s = s.replace("DEFINE output 1", blocks + "DEFINE output 1", 1)

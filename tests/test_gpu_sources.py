@@ -149,3 +149,16 @@ def test_written_state(workdir, method):
     assert per_probe_err(s_g, r["samples"]) <= 1e-10
     for got, ref in ((U, r["U"]), (B, r["B"])):
         assert rel_l2(got, ref) <= 1e-10
+
+
+@pytest.mark.parametrize("method", ["dga", "gfdtd"])
+def test_d_source_wall(workdir, method):
+    """A d source on the x-max wall (not PEC) beside the h source on x-min:
+    dga resets U on its edges to ComputeEfieldSource's 0 before each H update
+    (D:835-848), gfdtd leaves them to the solve (D:1497-1499, D:1540)."""
+    from test_source_routing import d_wall_case
+    b = Model(d_wall_case(workdir, method, f"gpu_dwall_{method}")).build(1)
+    print(f"case: d source wall {method} E={b.n_edges}", flush=True)
+    st, r = check_run(b)
+    if method == "gfdtd":
+        assert st["cg_not_converged"] == 0

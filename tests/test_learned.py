@@ -253,3 +253,29 @@ def test_training_is_what_gains(tmp_path):
     assert u["pc_loss"] >= 0.99 * u["pc_loss0"]  # nothing learned
     assert u["pc_state"] == 3  # switched off
     assert u["pc_fallbacks"] >= 1 or u["pc_learned_iters"] > jac * u["pc_learned_solves"]
+
+
+@pytest.mark.gpu
+def test_learned_quiet_steps(tmp_path):
+    """Steps whose right-hand side is exactly zero (a gaussian carrier whose
+    delay keeps it at 0.0 for the whole run): every solve starts from a zero
+    residual and stops at once, as the reference's 0/0 > tol test does
+    (ConjugateGradientSolver.hpp:72). On the learned path such a solve counts
+    as converged: no Jacobi redo, no missed tolerance, fields stay zero."""
+    from tetfit_amd.device import Simulation
+    p = configs.kuhn_cavity(str(tmp_path), n=5, method="gfdtd", n_steps=40, n_probes=2, block=2, dt=2e-11,
+                            name="quiet")
+    s = open(p).read().replace("SET carrier sin", "SET carrier gaussian\n    SET width 1e-9\n    SET delay 1e-6")
+    open(p, "w").write(s)
+    b = Model(p).build(1)
+    print(f"case: learned quiet steps E={b.n_edges}", flush=True)
+    sim = Simulation(b, "fp32")
+    s1 = sim.run(n=10, precond="learned", learn_steps=10, train_iters=20)
+    s2 = sim.run(n=b.n_steps - 10, precond="learned")
+    U, F, B = sim.state()
+    sim.close()
+    print(s1, s2, flush=True)
+    assert s2["pc_fallbacks"] == 0
+    assert s1["cg_not_converged"] == 0 and s2["cg_not_converged"] == 0
+    assert s1["cg_iters_total"] == 0 and s2["cg_iters_total"] == 0
+    assert not np.any(U) and not np.any(F) and not np.any(B)

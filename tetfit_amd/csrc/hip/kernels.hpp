@@ -383,14 +383,18 @@ __global__ void __launch_bounds__(kBlock) dga_inject_kernel(
     int32_t n_inj, const int32_t* __restrict__ inj_edge, const int8_t* __restrict__ inj_kind,
     const int32_t* __restrict__ inj_tb, const int32_t* __restrict__ inj_te, const int32_t* __restrict__ term_src,
     const double* __restrict__ term_w, const double* __restrict__ carr_step, T* __restrict__ curl_f,
-    T* __restrict__ U, double* __restrict__ poy_part) {
+    T* __restrict__ U, double* __restrict__ poy_part, int init) {
     __shared__ double red[kBlock / kWave];
     const int32_t i = blockIdx.x * kBlock + threadIdx.x;
     double poy = 0;
     if (i < n_inj) {
         const int32_t e = inj_edge[i];
         const double v = source_value(inj_tb[i], inj_te[i], term_src, term_w, carr_step);
-        if (inj_kind[i] == 0) {
+        if (init && inj_kind[i] == 0) {
+            // before the loop every source edge takes its e-source sum at
+            // t = 0 (D:746-765), which is 0 on an h edge
+            U[e] = (T)0;
+        } else if (inj_kind[i] == 0) {
             curl_f[e] = (T)((double)curl_f[e] + v);
             poy = 0.5 * (double)U[e] * v;
         } else {

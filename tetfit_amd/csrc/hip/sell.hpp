@@ -326,11 +326,14 @@ constexpr int kRed2 = 2 * (kBlock / kWave) + 2;
 // Stop decision of pass `it` from the residual norm RR = ||r_it||^2 (the
 // reference's test, ConjugateGradientSolver.hpp:68-74): stop unless
 // ||r|| / ||r0|| > tol and it < maxiter. Every block decides the same;
-// block 0 records it.
+// block 0 records it. A zero initial residual (a quiet step: the warm start
+// already solves the system) stops at once as in the reference, whose 0/0
+// fails the `> tol` test; its ratio is recorded as 0 so that a learned solve
+// counts as converged.
 __device__ __forceinline__ bool cg_decide_stop(int32_t it, double RR, double rr0_prev, double tol, int32_t maxiter,
                                                CgState* st) {
     const double rr0 = it == 0 ? RR : rr0_prev;
-    const double ratio = std::sqrt(RR) / std::sqrt(rr0);
+    const double ratio = rr0 == 0.0 ? 0.0 : std::sqrt(RR) / std::sqrt(rr0);
     const bool stop = !(ratio > tol && it < maxiter);
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         if (it == 0) st->rr0 = RR;

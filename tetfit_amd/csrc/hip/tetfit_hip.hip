@@ -1024,6 +1024,10 @@ void build_tet_inject(tf_ctx* c, const std::vector<std::vector<std::pair<int32_t
     std::vector<double> tw;
     for (int32_t e = 0; e < c->E; ++e) {
         if (by_edge_terms[e].empty()) continue;
+        // a d edge (neither h, j nor e) is set to its e-source sum, 0, by the
+        // dga loop before the H update (D:835-842); gfdtd overwrites it with
+        // the solve, so nothing is injected there
+        if (edge_field[e] == TF_FIELD_D && c->method == TF_METHOD_GFDTD) continue;
         edge.push_back(e);
         kind.push_back(edge_field[e] == TF_FIELD_H ? 0 : edge_field[e] == TF_FIELD_J ? 2 : 1);
         if (edge_field[e] == TF_FIELD_J && c->method != TF_METHOD_GFDTD)
@@ -1250,7 +1254,7 @@ void run_dga(tf_ctx* c, int64_t first, int64_t n, tf_stats* st) {
             dga_inject_kernel<T><<<inj_blocks, kBlock, 0, c->stream>>>(
                 c->n_inj, c->inj_edge.as<int32_t>(), c->inj_kind.as<int8_t>(), c->inj_tb.as<int32_t>(),
                 c->inj_te.as<int32_t>(), c->term_src.as<int32_t>(), c->term_w.as<double>(), cs0, c->curl_f.as<T>(),
-                c->U.as<T>(), c->poy.as<double>());
+                c->U.as<T>(), c->poy.as<double>(), 1);
         }
         if (want_sample(c, 0, 0.0)) sample<T>(c, 0.0);
     }
@@ -1269,7 +1273,7 @@ void run_dga(tf_ctx* c, int64_t first, int64_t n, tf_stats* st) {
             launch_timed(c, "dga_inject_kernel", dga_inject_kernel<T>, inj_blocks, 
                 c->n_inj, c->inj_edge.as<int32_t>(), c->inj_kind.as<int8_t>(), c->inj_tb.as<int32_t>(),
                 c->inj_te.as<int32_t>(), c->term_src.as<int32_t>(), c->term_w.as<double>(), cs, c->curl_f.as<T>(),
-                c->U.as<T>(), poy);
+                c->U.as<T>(), poy, 0);
         }
         TF_LAUNCH_TIMED(c, "sell_h_kernel", c->tune[TK_H].unroll, sell_h_kernel, T, gb_h, c->H.vals<T>(), c->h_edge.as<int32_t>(),
                      c->curl_f.as<T>(), c->U.as<T>(), c->dt, hdt);

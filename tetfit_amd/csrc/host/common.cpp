@@ -112,7 +112,8 @@ static int carrier_kind(const std::string& c) {
 static int field_kind(const std::string& st) {
     if (st == "e") return TF_FIELD_E;
     if (st == "j") return TF_FIELD_J;
-    return TF_FIELD_H;
+    if (st == "h") return TF_FIELD_H;
+    return TF_FIELD_D;
 }
 
 tf_source make_source(const SourceDef& s) {

@@ -555,7 +555,9 @@ void build_fit(const Model& m, const SimulationDef& sim, Build& b) {
         tf_source_edge d{};
         d.edge = (int32_t)e;
         const SourceDef& s0 = sources[edge_src[e].front()];
-        d.field = s0.st == "e" ? TF_FIELD_E : (s0.st == "j" ? TF_FIELD_J : TF_FIELD_H);
+        d.field = make_source(s0).field;  // only e, h (D:5431-5455) and j dipoles reach edge_src here
+        if (edge_src[e].size() > 4)
+            throw Error("more than 4 sources on one edge (edge " + std::to_string(e) + ")");
         Vec3 eb = edge_bar(e);
         Vec3 ev = P(G[(size_t)e * 2 + 1]) - P(G[(size_t)e * 2]);
         Vec3 vec = d.field == TF_FIELD_H ? bnd_dual(e) : ev;
@@ -563,9 +565,8 @@ void build_fit(const Model& m, const SimulationDef& sim, Build& b) {
         d.vec[0] = vec.x; d.vec[1] = vec.y; d.vec[2] = vec.z;
         for (uint32_t sl : edge_src[e]) {
             const SourceDef& s = sources[sl];
-            int want = s.st == "e" ? TF_FIELD_E : (s.st == "j" ? TF_FIELD_J : TF_FIELD_H);
-            if (d.n_src < 4) d.src[d.n_src++] = src_index[sl];
-            if (want != d.field) continue;  // the loop only evaluates sources of the edge's kind
+            d.src[d.n_src++] = src_index[sl];
+            if (make_source(s).field != d.field) continue;  // the loop only evaluates sources of the edge's kind
             Vec3 sp = SourceEval::space(s, eb);
             double w = dot(sp, vec);
             if (d.field == TF_FIELD_J) w = (1.0 / norm(ev)) * w;

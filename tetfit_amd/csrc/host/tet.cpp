@@ -1019,8 +1019,16 @@ void build_tet(const Model& m, const SimulationDef& sim, Build& b) {
         const SourceDef& s0 = sources[edge_src[e].front()];
         tf_source_edge d{};
         d.edge = (int32_t)e;
-        d.field = s0.st == "e" ? TF_FIELD_E : (s0.st == "j" ? TF_FIELD_J : TF_FIELD_H);
+        d.field = make_source(s0).field;
         if (d.field == TF_FIELD_H && bnd_edges[e] < 0) continue;  // gfdtd checks bnd_edges >= 0 (D:1488)
+        // an edge whose first source is neither h nor j goes to
+        // ComputeEfieldSource (D:827-842, D:1497-1499), which sums the "e"
+        // sources only (D:3928): for a d edge that is U = 0. gfdtd overwrites
+        // such a free edge with the solve (no effect); dga resets its U_a to
+        // 0 before the H update, carried here by zero-weight terms
+        if (d.field == TF_FIELD_D && !is_dga) continue;
+        if (edge_src[e].size() > 4)
+            throw Error("more than 4 sources on one edge (edge " + std::to_string(e) + ")");
         Vec3 ev = P(en[2 * (size_t)e + 1]) - P(en[2 * (size_t)e]);
         Vec3 vec = d.field == TF_FIELD_H ? bnd_dual[bnd_edges[e]] : ev;
         const Vec3& eb = edge_bars[e];
@@ -1028,12 +1036,14 @@ void build_tet(const Model& m, const SimulationDef& sim, Build& b) {
         d.vec[0] = vec.x; d.vec[1] = vec.y; d.vec[2] = vec.z;
         for (uint32_t sl : edge_src[e]) {
             const SourceDef& s = sources[sl];
-            int want = s.st == "e" ? TF_FIELD_E : (s.st == "j" ? TF_FIELD_J : TF_FIELD_H);
-            if (d.n_src < 4) d.src[d.n_src++] = src_index[sl];
-            if (want != d.field) continue;
-            Vec3 sp = SourceEval::space(s, eb);
-            double w = dot(sp, vec);
-            if (d.field == TF_FIELD_J) w = (1.0 / norm(ev)) * w;
+            d.src[d.n_src++] = src_index[sl];
+            if (make_source(s).field != d.field) continue;
+            double w = 0.0;
+            if (d.field != TF_FIELD_D) {
+                Vec3 sp = SourceEval::space(s, eb);
+                w = dot(sp, vec);
+                if (d.field == TF_FIELD_J) w = (1.0 / norm(ev)) * w;
+            }
             b.terms.push_back(tf_source_term{(int32_t)e, src_index[sl], w});
         }
         b.src_edge_desc.push_back(d);
