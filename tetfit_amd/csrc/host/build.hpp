@@ -13,6 +13,15 @@
 
 namespace tfh {
 
+// Wall time of the set-up phases, printed to stderr when TETFIT_HOST_TIMING
+// is set (for finding where host set-up time goes).
+struct PhaseClock {
+    bool on;
+    double t0, t;
+    PhaseClock();
+    void mark(const char* phase);
+};
+
 // Owning storage behind a tf_csr.
 struct CsrStore {
     int32_t nrows = 0, ncols = 0;
@@ -101,6 +110,10 @@ tf_source make_source(const SourceDef& s);
 // Power-iteration estimate of lambda_max for a symmetric-positive operator
 // given through `apply(x, y)` on vectors of size n (replaces Spectra's
 // GenEigsSolver call in ComputeFDTDTimeStep / ComputeDGATimeStep).
+// Dot product over fixed blocks of 32,768 entries summed in block order:
+// the same result for any number of threads.
+double par_dot(const double* a, const double* b, size_t n);
+
 double power_lambda_max(size_t n,
                         const std::function<void(const std::vector<double>&, std::vector<double>&)>& apply,
                         int iters);

@@ -1,12 +1,28 @@
 // common.cpp - CSR assembly helpers, source evaluation split, dt power
 // iteration and output schedule shared by the FIT and tetrahedral builders.
 #include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <cmath>
 #include <numeric>
 
 #include "build.hpp"
 
 namespace tfh {
+
+static double wall_seconds() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+PhaseClock::PhaseClock() : on(std::getenv("TETFIT_HOST_TIMING") != nullptr), t0(wall_seconds()), t(t0) {}
+
+void PhaseClock::mark(const char* phase) {
+    if (!on) return;
+    const double now = wall_seconds();
+    std::fprintf(stderr, "[host] %-28s %8.3f s (total %8.3f s)\n", phase, now - t, now - t0);
+    t = now;
+}
 
 tf_csr CsrStore::view() const {
     tf_csr c{};
@@ -151,6 +167,22 @@ tf_source_desc describe_source(const SourceDef& s) {
 
 // ---------------------------------------------------------------- dt
 
+double par_dot(const double* a, const double* b, size_t n) {
+    constexpr size_t kBlk = 32768;
+    const int64_t nb = (int64_t)((n + kBlk - 1) / kBlk);
+    std::vector<double> part(nb);
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < nb; ++i) {
+        const size_t lo = (size_t)i * kBlk, hi = std::min(n, lo + kBlk);
+        double s = 0;
+        for (size_t j = lo; j < hi; ++j) s += a[j] * b[j];
+        part[i] = s;
+    }
+    double s = 0;
+    for (double v : part) s += v;
+    return s;
+}
+
 double power_lambda_max(size_t n,
                         const std::function<void(const std::vector<double>&, std::vector<double>&)>& apply,
                         int iters) {
@@ -163,11 +195,12 @@ double power_lambda_max(size_t n,
     }
     double lam = 0;
     for (int it = 0; it < iters; ++it) {
-        double nx = std::sqrt(std::inner_product(x.begin(), x.end(), x.begin(), 0.0));
+        double nx = std::sqrt(par_dot(x.data(), x.data(), n));
         if (nx == 0) return 0;
-        for (auto& v : x) v /= nx;
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < (int64_t)n; ++i) x[i] /= nx;
         apply(x, y);
-        lam = std::inner_product(x.begin(), x.end(), y.begin(), 0.0);
+        lam = par_dot(x.data(), y.data(), n);
         std::swap(x, y);
     }
     return lam;

@@ -359,11 +359,13 @@ void build_fit(const Model& m, const SimulationDef& sim, Build& b) {
         for (uint32_t e = 0; e < totE; ++e) Hv[e] = edge_len[e] / avg_eps[e];
         std::vector<double> Fw(totF);
         auto apply = [&](const std::vector<double>& X, std::vector<double>& Y) {
-            for (uint32_t f = 0; f < totF; ++f) {
+#pragma omp parallel for schedule(static)
+            for (int64_t f = 0; f < (int64_t)totF; ++f) {
                 const int32_t* c = &b.c_edge[(size_t)f * 4];
                 Fw[f] = Nv[f] * b.curl[f] * (X[c[0]] - X[c[1]] + X[c[2]] - X[c[3]]);
             }
-            for (uint32_t e = 0; e < totE; ++e) {
+#pragma omp parallel for schedule(static)
+            for (int64_t e = 0; e < (int64_t)totE; ++e) {
                 double s = 0;
                 for (int64_t q = b.ct_ptr[e]; q < b.ct_ptr[e + 1]; ++q) s += b.ct_sgn[q] * Fw[b.ct_face[q]];
                 Y[e] = Hv[e] * s;

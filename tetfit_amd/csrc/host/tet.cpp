@@ -136,6 +136,7 @@ void build_tet(const Model& m, const SimulationDef& sim, Build& b) {
     const bool is_dga = sim.method == "dga";
     const double scale = msh.scale;
 
+    PhaseClock clk;
     Reader rd(msh.file);
     // ---- points
     const uint64_t np = rd.uint();
@@ -144,6 +145,7 @@ void build_tet(const Model& m, const SimulationDef& sim, Build& b) {
         double x = rd.num(), y = rd.num(), z = rd.num();
         pts[i] = Vec3(x, y, z) * scale;
     }
+    clk.mark("read points");
     // ---- tetrahedra (node-sorted, label 7 dropped, sorted by node tuple)
     const uint64_t nt_file = rd.uint();
     struct TetIn { uint32_t n[4]; uint32_t mat; };
@@ -207,6 +209,7 @@ void build_tet(const Model& m, const SimulationDef& sim, Build& b) {
             for (int k = 0; k < 4; ++k) av[pos[b.vols[(size_t)t * 4 + k]]++] = t;
     }
 
+    clk.mark("tets");
     // ---- faces: sort + unique (unique() D:10173-10219)
     std::vector<uint32_t> flab((size_t)T * 4);
     std::sort(tri.begin(), tri.end(), [](const Tri& x, const Tri& y) {
@@ -253,6 +256,7 @@ void build_tet(const Model& m, const SimulationDef& sim, Build& b) {
     }
     std::vector<uint32_t>().swap(flab);
 
+    clk.mark("faces");
     // ---- edges: sort + unique with etn / nte / dual_star_offsets (D:10221-10299)
     std::vector<Ed> ed((size_t)F * 3);
     for (uint32_t f = 0; f < F; ++f) {
@@ -313,6 +317,7 @@ void build_tet(const Model& m, const SimulationDef& sim, Build& b) {
         return (P(n[0]) + P(n[1]) + P(n[2]) + P(n[3])) / 4.0;
     };
 
+    clk.mark("edges");
     // ---- C, boundary faces, bnd edges and Ctb, face classes (D:6320-6546)
     b.boundary_face.assign(F, 0);
     std::vector<int32_t> bnd_edges(E, -1);
@@ -411,6 +416,7 @@ void build_tet(const Model& m, const SimulationDef& sim, Build& b) {
         }
     }
 
+    clk.mark("boundary, sources");
     // ---- edge classes (D:6770-6905)
     b.edge_class.assign(E, 0);
     b.h_row.assign(E, -1);
@@ -496,6 +502,7 @@ void build_tet(const Model& m, const SimulationDef& sim, Build& b) {
     }
     b.bnd_edge = bnd_edges;
 
+    clk.mark("classes, C");
     // ---- local geometry per tet
     // local edge order after sort_unique + swap(1,2) + swap(1,3) (D:10314-10316):
     // 0:(n0n1) 1:(n1n2) 2:(n0n2) 3:(n0n3) 4:(n1n3) 5:(n2n3)
@@ -557,6 +564,7 @@ void build_tet(const Model& m, const SimulationDef& sim, Build& b) {
         M[4] = M[1]; M[8] = M[2]; M[12] = M[3]; M[9] = M[6]; M[13] = M[7]; M[14] = M[11];
     };
 
+    clk.mark("tet edges");
     // ---- N = sum over tets of local_M^-1 (rows n_index = face id for class-1 faces)
     {
         std::vector<int32_t> pr, pc;
@@ -578,6 +586,7 @@ void build_tet(const Model& m, const SimulationDef& sim, Build& b) {
         }
     }
 
+    clk.mark("N");
     // Corner data of tet t at its local node k (D:8913-9078): local edges and
     // the face area vector / sign factor that define the dual basis f_i.
     // k=0: edges (0,2,3) faces (1,2,3) s=-1 ; k=1: (0,1,4) faces (0,2,3) s=(+,-,-)
@@ -687,28 +696,33 @@ void build_tet(const Model& m, const SimulationDef& sim, Build& b) {
                 }
             }
         }
+    clk.mark("H (node stars)");
         // ---- dt: power iteration on H * Ct * N * C (ComputeDGATimeStep D:7121)
         {
             std::vector<double> U(E), cu(F), Fw(F), cf(E);
             auto apply = [&](const std::vector<double>& x, std::vector<double>& y) {
                 // x, y live on H rows (class-1 edges)
-                std::fill(U.begin(), U.end(), 0.0);
-                for (uint32_t e = 0; e < E; ++e) if (b.h_row[e] >= 0) U[e] = x[b.h_row[e]];
-                for (uint32_t f = 0; f < F; ++f) {
+#pragma omp parallel for schedule(static)
+                for (int64_t e = 0; e < (int64_t)E; ++e) U[e] = b.h_row[e] >= 0 ? x[b.h_row[e]] : 0.0;
+#pragma omp parallel for schedule(static)
+                for (int64_t f = 0; f < (int64_t)F; ++f) {
                     double s = 0;
                     for (int q = 0; q < 3; ++q) s += b.C.sgn[(size_t)f * 3 + q] * U[b.C.col[(size_t)f * 3 + q]];
                     cu[f] = s;
                 }
-                for (uint32_t f = 0; f < F; ++f) {
+#pragma omp parallel for schedule(static)
+                for (int64_t f = 0; f < (int64_t)F; ++f) {
                     double s = 0;
                     for (int64_t q = b.N.rowptr[f]; q < b.N.rowptr[f + 1]; ++q) s += b.N.val[q] * cu[b.N.col[q]];
                     Fw[f] = s;
                 }
-                for (uint32_t e = 0; e < E; ++e) {
+#pragma omp parallel for schedule(static)
+                for (int64_t e = 0; e < (int64_t)E; ++e) {
                     double s = 0;
                     for (int64_t q = b.Ct.rowptr[e]; q < b.Ct.rowptr[e + 1]; ++q) s += b.Ct.sgn[q] * Fw[b.Ct.col[q]];
                     cf[e] = s;
                 }
+#pragma omp parallel for schedule(static)
                 for (int32_t r = 0; r < H_size; ++r) {
                     double s = 0;
                     for (int64_t q = b.H.rowptr[r]; q < b.H.rowptr[r + 1]; ++q) s += b.H.val[q] * cf[b.H.col[q]];
@@ -767,6 +781,7 @@ void build_tet(const Model& m, const SimulationDef& sim, Build& b) {
                 }
             }
         }
+    clk.mark("E (corner terms)");
         // free (non-Dirichlet) edges: compressed_dirichlet (D:8145-8171)
         std::vector<int32_t> is_dir(E, -1);
         for (uint32_t i = 0; i < E; ++i) {
@@ -798,6 +813,7 @@ void build_tet(const Model& m, const SimulationDef& sim, Build& b) {
             }
             b.RHS1.rowptr[r + 1] = (int64_t)b.RHS1.col.size();
         }
+    clk.mark("A, RHSmat1");
         // ---- dt: generalized power iteration on (Ct N C, A) (ComputeFEMTimeStep D:6928)
         {
             std::vector<double> U(E), cu(F), Fw(F), cf(E), r(nfree), z(nfree), pp(nfree), qq(nfree), y(nfree);
@@ -806,18 +822,23 @@ void build_tet(const Model& m, const SimulationDef& sim, Build& b) {
                 for (int64_t q = b.A.rowptr[i]; q < b.A.rowptr[i + 1]; ++q)
                     if (b.A.col[q] == i && b.A.val[q] != 0) dinv[i] = 1.0 / b.A.val[q];
             auto Kx = [&](const std::vector<double>& x, std::vector<double>& out) {
-                std::fill(U.begin(), U.end(), 0.0);
+#pragma omp parallel for schedule(static)
+                for (int64_t e = 0; e < (int64_t)E; ++e) U[e] = 0.0;
+#pragma omp parallel for schedule(static)
                 for (int32_t i = 0; i < nfree; ++i) U[b.free_edges[i]] = x[i];
-                for (uint32_t f = 0; f < F; ++f) {
+#pragma omp parallel for schedule(static)
+                for (int64_t f = 0; f < (int64_t)F; ++f) {
                     double s = 0;
                     for (int q = 0; q < 3; ++q) s += b.C.sgn[(size_t)f * 3 + q] * U[b.C.col[(size_t)f * 3 + q]];
                     cu[f] = s;
                 }
-                for (uint32_t f = 0; f < F; ++f) {
+#pragma omp parallel for schedule(static)
+                for (int64_t f = 0; f < (int64_t)F; ++f) {
                     double s = 0;
                     for (int64_t q = b.N.rowptr[f]; q < b.N.rowptr[f + 1]; ++q) s += b.N.val[q] * cu[b.N.col[q]];
                     Fw[f] = s;
                 }
+#pragma omp parallel for schedule(static)
                 for (int32_t i = 0; i < nfree; ++i) {
                     int32_t e = b.free_edges[i];
                     double s = 0;
@@ -826,27 +847,34 @@ void build_tet(const Model& m, const SimulationDef& sim, Build& b) {
                 }
             };
             auto Ax = [&](const std::vector<double>& x, std::vector<double>& out) {
+#pragma omp parallel for schedule(static)
                 for (int32_t i = 0; i < nfree; ++i) {
                     double s = 0;
                     for (int64_t q = b.A.rowptr[i]; q < b.A.rowptr[i + 1]; ++q) s += b.A.val[q] * x[b.A.col[q]];
                     out[i] = s;
                 }
             };
+            auto dotv = [&](const std::vector<double>& a, const std::vector<double>& c) {
+                return par_dot(a.data(), c.data(), a.size());
+            };
             auto solveA = [&](const std::vector<double>& rhs, std::vector<double>& x) {
                 std::fill(x.begin(), x.end(), 0.0);
                 r = rhs;
-                double r0 = std::sqrt(std::inner_product(r.begin(), r.end(), r.begin(), 0.0)), rho = 0, rho_p;
+                double r0 = std::sqrt(dotv(r, r)), rho = 0, rho_p;
                 if (r0 == 0) return;
                 for (int it = 0; it < 200; ++it) {
+#pragma omp parallel for schedule(static)
                     for (int32_t i = 0; i < nfree; ++i) z[i] = dinv[i] * r[i];
                     rho_p = rho;
-                    rho = std::inner_product(r.begin(), r.end(), z.begin(), 0.0);
-                    if (it == 0) pp = z;
-                    else for (int32_t i = 0; i < nfree; ++i) pp[i] = z[i] + rho / rho_p * pp[i];
+                    rho = dotv(r, z);
+                    const double beta = it == 0 ? 0.0 : rho / rho_p;
+#pragma omp parallel for schedule(static)
+                    for (int32_t i = 0; i < nfree; ++i) pp[i] = it == 0 ? z[i] : z[i] + beta * pp[i];
                     Ax(pp, qq);
-                    double a = rho / std::inner_product(pp.begin(), pp.end(), qq.begin(), 0.0);
+                    double a = rho / dotv(pp, qq);
+#pragma omp parallel for schedule(static)
                     for (int32_t i = 0; i < nfree; ++i) { x[i] += a * pp[i]; r[i] -= a * qq[i]; }
-                    if (std::sqrt(std::inner_product(r.begin(), r.end(), r.begin(), 0.0)) / r0 < 1e-6) break;
+                    if (std::sqrt(dotv(r, r)) / r0 < 1e-6) break;
                 }
             };
             std::vector<double> kx(nfree), w(nfree);
@@ -861,13 +889,13 @@ void build_tet(const Model& m, const SimulationDef& sim, Build& b) {
             double lam = 0;
             for (int it = 0; it < 25; ++it) {
                 apply(x, y);
-                double n2 = std::sqrt(std::inner_product(y.begin(), y.end(), y.begin(), 0.0));
+                double n2 = std::sqrt(dotv(y, y));
                 if (n2 == 0) break;
+#pragma omp parallel for schedule(static)
                 for (int32_t i = 0; i < nfree; ++i) x[i] = y[i] / n2;
                 Kx(x, kx);
                 Ax(x, w);
-                lam = std::inner_product(x.begin(), x.end(), kx.begin(), 0.0) /
-                      std::inner_product(x.begin(), x.end(), w.begin(), 0.0);
+                lam = dotv(x, kx) / dotv(x, w);
             }
             lam *= 1.02;
             b.stable_dt = lam > 0 ? 2.0 / std::sqrt(lam) : 0;
@@ -897,6 +925,7 @@ void build_tet(const Model& m, const SimulationDef& sim, Build& b) {
         }
     }
 
+    clk.mark("dt estimate, coefficients");
     // ---- probes (FindProbe D:3645) with Whitney E and face H weights
     uint32_t root = 0;
     auto det4 = [](const Vec3 r[4]) {
@@ -1007,6 +1036,7 @@ void build_tet(const Model& m, const SimulationDef& sim, Build& b) {
         }
     }
 
+    clk.mark("probes");
     // ---- sources on edges: h (bnd dual vector), e (edge vector)
     std::map<uint32_t, int32_t> src_index;
     for (uint32_t sl : sim.sources) {
@@ -1063,6 +1093,7 @@ void build_tet(const Model& m, const SimulationDef& sim, Build& b) {
     int64_t i = 1;
     while ((double)i * t_step <= sim.duration) ++i;
     b.n_steps = i - 1;
+    clk.mark("sources");
 }
 
 }  // namespace tfh
