@@ -1,18 +1,25 @@
 """Benchmark of the TetFIT time-stepping hot path on one MI355X.
 
-Workload (one GPU): the 48^3-cube Kuhn tetrahedral cavity (795,024 edges,
-1,340,928 faces, 663,552 tets; vertices jittered +-0.2h with seed 2; eps_r
-piecewise constant per 8^3 block, uniform in [1,10]; mu_r = 1; PEC on 5
-walls; wave-profile h source on x-min at 2e8 Hz), stepped with the
-CG-solved DGA leapfrog (method gfdtd, CG tolerance 1e-8, maxiter 100) and,
-beside it, with the explicit DGA leapfrog (method dga).
+Default workload (the metric's configuration): the 96^3-cube Kuhn
+tetrahedral cavity (6,276,384 edges; vertices jittered +-0.2h with seed 5;
+eps_r piecewise constant per 6^3-cube block, log-uniform in [1,80]; mu_r =
+1; PEC on 5 walls; wave-profile h source on x-min at 2e8 Hz), stepped with
+the CG-solved DGA leapfrog (method gfdtd, CG tolerance 1e-8, maxiter 100)
+for 3000 steps from a zero field, once on the plain FP64 path (Jacobi CG)
+and once on the learned+FP32 path (fp32 fields and operators, fp64 CG
+reductions, the learned sparse-approximate-inverse preconditioner trained on
+the device from the first 200 solves).
 
 A step is one leapfrog step over the whole mesh. value = edge-DoF updates/s
-= n_edges * steps / seconds of the timed region (HIP events on the
-context's stream, inputs resident in HBM). The headline path is
-"learned+FP32": fp32 fields and operators with fp64 CG reductions and the
-learned sparse-approximate-inverse preconditioner; "plain FP64" (Jacobi CG) is reported beside
-it. CG iterations/step are the mean ConjugateGradientSolver::iterations().
+of the learned+FP32 path over the whole job = n_edges * steps / wall seconds,
+its one-time cost (preconditioner set-up, the 200 collection solves with
+Jacobi CG, training) included; inputs are resident in HBM when the job
+starts. Beside it: the plain FP64 rate over the same job, both steady-state
+rates, CG iterations/step, the fp32 field error against the plain FP64 run,
+the fp64 residual of the learned path's last solve, kernel rooflines, the
+CPU baseline on the same mesh and the 48^3 legs. Other workloads: --workload
+gfdtd (48^3 steady-state legs), contrast (96^3 steady-state legs with the
+explicit dga path), batch, fit, fitprec, copy.
 
 Prints one JSON line. --gpus N with N > 1 exits: this build drives one GPU.
 """
@@ -31,8 +38,9 @@ import numpy as np  # noqa: E402
 HBM_PEAK_GBS = 8000.0
 METRIC = "edge-DoF updates/s + CG iters/step: learned+FP32 vs plain FP64, 1 GPU"
 # dt pinned with SET step: 0.9 x the host stability estimate of each method on
-# this mesh (power iteration; see tetfit_amd/configs.py)
+# these meshes (power iteration; see tetfit_amd/configs.py)
 DT48 = {"dga": 2.281e-11, "gfdtd": 4.777e-11}
+DT96_CONTRAST = {"gfdtd": 2.264e-11}
 
 
 def log(*a):
@@ -47,9 +55,9 @@ def build_workload(d, method, n, steps, contrast=False):
     if contrast:
         # dga: dt from the host's stability estimate within the same build
         # (one host build: the explicit operator assembly at 96^3 takes minutes)
+        dt = "courant" if method == "dga" else (DT96_CONTRAST.get(method) if n == 96 else None)
         path = configs.kuhn_cavity(d, n=n, method=method, n_steps=steps, n_probes=8, block=6, seed=5,
-                                   eps_range=(1.0, 80.0), log_eps=True, name=f"kuhn{n}_{method}_contrast",
-                                   dt="courant" if method == "dga" else None)
+                                   eps_range=(1.0, 80.0), log_eps=True, name=f"kuhn{n}_{method}_contrast", dt=dt)
     else:
         path = configs.kuhn_cavity(d, n=n, method=method, n_steps=steps, n_probes=8, block=8, seed=2,
                                    dt=DT48.get(method) if n == 48 else None)
@@ -225,6 +233,215 @@ def fit_bytes(b, kernel, vb, grid_order=True):
     if kernel == "fit_e_kernel":
         return E * (4 * vb + cb) + F * vb
     return F * (5 * vb + cb) + E * vb
+
+
+def run_chunks(sim, steps, chunk, label, precond="jacobi", learn_steps=200, stop_before_last=False):
+    """Advances `steps` steps in tf_run calls of at most `chunk` steps (one
+    progress line each), wall-clocked around all calls (each returns after
+    its stream has drained). The learned path's first call covers exactly the
+    collection solves, so the preconditioner set-up, collection and training
+    fall inside the timed region; the launch shapes of the learned passes are
+    then fetched (autotune, a process-cache hit after the warm-up context).
+    With stop_before_last the last step is left for the caller. Returns
+    (seconds, stats of the calls, seconds of the last full chunk, its steps)."""
+    out = dict(cg_iters_total=0, cg_not_converged=0, cg_max_relres=0.0, pc=None, chunks=[])
+    todo = steps - (1 if stop_before_last else 0)
+    t0 = time.perf_counter()
+    done = 0
+    while done < todo:
+        k = min(chunk, todo - done)
+        if precond == "learned" and done == 0:
+            k = min(learn_steps, todo)
+        tc = time.perf_counter()
+        st = sim.run(n=k, precond=precond, learn_steps=learn_steps)
+        if precond == "learned" and done == 0:
+            out["pc"] = st
+            sim.autotune()
+        dtc = time.perf_counter() - tc
+        done += k
+        out["cg_iters_total"] += st["cg_iters_total"]
+        out["cg_not_converged"] += st["cg_not_converged"]
+        out["cg_max_relres"] = max(out["cg_max_relres"], st["cg_max_relres"])
+        out["chunks"].append((k, dtc, st["cg_iters_total"]))
+        log(f"[bench] {label}: {done}/{steps} steps, {1e3 * dtc / k:.3f} ms/step over the last {k}"
+            + (f", CG {st['cg_iters_total'] / k:.2f} it/step" if st["cg_iters_total"] else ""))
+    return time.perf_counter() - t0, out
+
+
+def true_residual(b, sim64, state_before, first_step, x_path):
+    """fp64 residual of the last solve of a run: the state before that step is
+    written into the fp64 context, which redoes the step with CG at tol 1e-13
+    (x*); then, with the host-assembled fp64 A on the free edges,
+    ||A (x* - x)|| / ||A (x* - x_old)|| = ||b - A x|| / ||b - A x_old||, the
+    reference's stopping ratio evaluated in fp64 for the path's solution x."""
+    import scipy.sparse as sp
+    U0, F0, B0 = state_before
+    sim64.set_state(U0, F0, B0)
+    st = sim64.run(n=1, first=first_step, cg_tol=1e-13, cg_maxiter=5000)
+    Ustar, _, _ = sim64.state()
+    ops = b.dga_ops()
+    A = ops["A"]
+    free = ops["free_edges"]
+    Am = sp.csr_matrix((A["val"], A["col"], A["rowptr"]), shape=(A["nrows"], A["ncols"]))
+    r = Am @ (Ustar[free] - x_path[free])
+    r0 = Am @ (Ustar[free] - U0[free])
+    return dict(fp64_relres=float(np.linalg.norm(r) / max(np.linalg.norm(r0), 1e-300)),
+                reference_solve_relres=st["cg_max_relres"], reference_solve_iters=st["cg_iters_total"])
+
+
+def rel_l2(a, r):
+    return float(np.linalg.norm(a - r) / max(np.linalg.norm(r), 1e-300))
+
+
+def bench_headline(args):
+    """The metric on its configuration: the 96^3 Kuhn cavity with eps_r
+    log-uniform in [1, 80] per 6^3 block (seed 5), method gfdtd, CG tol 1e-8,
+    3000 steps. Plain FP64 (Jacobi CG) and learned+FP32 (200 collection
+    solves, on-device training, then the learned preconditioner) each run the
+    whole job from a zero field; value is the learned path's end-to-end
+    edge-DoF/s over all steps, set-up of the preconditioner, collection and
+    training included. A warm-up context of each path runs first (and fills
+    the autotuner's process cache) and is discarded."""
+    from tetfit_amd.device import Simulation
+    n = args.n if args.n != 48 else 96
+    # the job is the configuration's 3000 steps (the learned path's one-time
+    # cost is amortised over it); a longer --steps lengthens it
+    steps = max(args.steps, 3000) if args.steps_given else 3000
+    chunk = 500
+    with tempfile.TemporaryDirectory() as d:
+        bg = build_workload(d, "gfdtd", n, steps + 60, contrast=True)
+        E = bg.n_edges
+        # warm-up contexts: tuning cache, code paths, first-touch allocations
+        for prec, pc in (("fp64", "jacobi"), ("fp32", "learned")):
+            sim = Simulation(bg, prec, output=False)
+            sim.autotune()
+            if pc == "learned":
+                sim.run(n=args.learn_steps, precond="learned", learn_steps=args.learn_steps)
+                sim.autotune()
+            sim.run(n=max(args.warmup, 1), precond=pc)
+            sim.close()
+            log(f"[bench] warm-up {prec} {pc} done")
+        # plain FP64, whole job
+        p64 = Simulation(bg, "fp64")
+        p64.autotune()
+        sec64, r64 = run_chunks(p64, steps, chunk, "plain fp64 (Jacobi)")
+        U64, F64, B64 = p64.state()
+        _, s64 = p64.probes()
+        # learned + FP32, whole job (the last step taken apart for the residual check)
+        l32 = Simulation(bg, "fp32")
+        l32.autotune()
+        sec32a, r32 = run_chunks(l32, steps, chunk, "learned fp32", precond="learned",
+                                 learn_steps=args.learn_steps, stop_before_last=True)
+        before = l32.state()
+        t = time.perf_counter()
+        st_last = l32.run(n=1, precond="learned")
+        sec32 = sec32a + (time.perf_counter() - t)
+        r32["cg_iters_total"] += st_last["cg_iters_total"]
+        r32["cg_max_relres"] = max(r32["cg_max_relres"], st_last["cg_max_relres"])
+        r32["cg_not_converged"] += st_last["cg_not_converged"]
+        U32, F32, B32 = l32.state()
+        _, s32 = l32.probes()
+        pc = r32["pc"]
+        # steady state: the learned path's chunks after the collection call
+        ss = [c for c in r32["chunks"][1:]]
+        ss_steps = sum(c[0] for c in ss)
+        ss_sec = sum(c[1] for c in ss)
+        ss_it = sum(c[2] for c in ss)
+        p_steps = sum(c[0] for c in r64["chunks"][1:]) or steps
+        p_sec = sum(c[1] for c in r64["chunks"][1:]) or sec64
+        # per-kernel times (HIP events on the context's stream) over 30 more steps
+        first_prof = l32.next_step
+        l32.set_profiling(True)
+        l32.run(n=30, precond="learned")
+        kern32 = {k: {"ms_avg": v[0], "launches": v[1]} for k, v in l32.kernel_times().items()}
+        info_a32 = l32.sell_info("a")
+        l32.close()
+        log("[bench] fp64 check of the learned path's last solve")
+        resid = true_residual(bg, p64, before, bg.first_step + steps - 1, U32)
+        p64.set_profiling(True)
+        p64.run(n=10)
+        kern64 = {k: {"ms_avg": v[0], "launches": v[1]} for k, v in p64.kernel_times().items()}
+        info_a64 = p64.sell_info("a")
+        p64.close()
+        del first_prof
+        nprobe = s64.shape[1]
+        per_probe = [max(rel_l2(s32[:, p, :3], s64[:, p, :3]), rel_l2(s32[:, p, 3:], s64[:, p, 3:]))
+                     for p in range(nprobe)]
+        fp32_error = {"vs": "plain FP64 GPU path, same steps", "steps": steps,
+                      "U": rel_l2(U32, U64), "F": rel_l2(F32, F64), "B": rel_l2(B32, B64),
+                      "per_probe_max": max(per_probe), "per_probe": per_probe,
+                      "last_solve_fp64_relres": resid["fp64_relres"],
+                      "last_solve_device_relres": st_last["cg_max_relres"],
+                      "reference_solve": {"tol": 1e-13, "relres": resid["reference_solve_relres"],
+                                          "iters": resid["reference_solve_iters"]}}
+        log(f"[bench] fp32 learned vs fp64 plain: U {fp32_error['U']:.3e} F {fp32_error['F']:.3e} "
+            f"B {fp32_error['B']:.3e}, probes <= {fp32_error['per_probe_max']:.3e}; last solve fp64 "
+            f"relres {resid['fp64_relres']:.3e}")
+        value = E * steps / sec32
+        plain_rate = E * steps / sec64
+        roofs = roofline_of(info_a32, kern32, ["cg_wu_kernel", "cg_pg_kernel"], 4)
+        plain_roofs = roofline_of(info_a64, kern64, ["cg_w_kernel"], 8)
+        amortized = {
+            "steps": steps, "learned_fp32_seconds": sec32, "plain_fp64_seconds": sec64,
+            "learned_fp32_edge_dof_per_s": value, "plain_fp64_edge_dof_per_s": plain_rate,
+            "speedup": value / plain_rate,
+            "learned_one_time": {"collection_steps": args.learn_steps,
+                                 "first_call_seconds": r32["chunks"][0][1],
+                                 "train_seconds": pc["pc_train_seconds"],
+                                 "collection_seconds": r32["chunks"][0][1] - pc["pc_train_seconds"],
+                                 "jacobi_iters_per_step": pc["pc_jacobi_mean_iters"],
+                                 "loss0": pc["pc_loss0"], "loss": pc["pc_loss"], "pairs": pc["pc_pairs"]},
+            "learned_state_end": st_last.get("pc_state"), "learned_fallbacks": st_last.get("pc_fallbacks"),
+        }
+        steady = {
+            "learned_fp32": {"steps": ss_steps, "ms_per_step": 1e3 * ss_sec / max(ss_steps, 1),
+                             "edge_dof_per_s": E * ss_steps / ss_sec if ss_sec else None,
+                             "cg_iters_per_step": ss_it / max(ss_steps, 1)},
+            "plain_fp64": {"steps": p_steps, "ms_per_step": 1e3 * p_sec / p_steps,
+                           "edge_dof_per_s": E * p_steps / p_sec,
+                           "cg_iters_per_step": sum(c[2] for c in r64["chunks"][1:]) / p_steps
+                           if len(r64["chunks"]) > 1 else r64["cg_iters_total"] / steps},
+        }
+        steady["speedup"] = (steady["learned_fp32"]["edge_dof_per_s"] / steady["plain_fp64"]["edge_dof_per_s"]
+                             if steady["learned_fp32"]["edge_dof_per_s"] else None)
+        cpu = None
+        if not args.no_cpu:
+            t = time.time()
+            cpu = cpu_baseline(bg, args.cpu_steps)
+            log(f"[bench] cpu baseline {cpu['value'] / 1e6:.2f} M edge-DoF/s ({time.time() - t:.1f}s)")
+        legs48 = None
+        if not args.no_48:
+            b48 = build_workload(d, "gfdtd", 48, 400 + args.learn_steps, False)
+            legs48 = {}
+            for key, prec, pcn in (("plain_fp64", "fp64", "jacobi"), ("learned_fp32", "fp32", "learned")):
+                r = time_path(b48, prec, 200, 20, pcn, f"48^3 gfdtd {prec} {pcn}", learn_steps=args.learn_steps)
+                legs48[key] = {"edge_dof_per_s": r["edge_dof_per_s"], "ms_per_step": r["ms_per_step"],
+                               "cg_iters_per_step": r["cg_iters_per_step"]}
+                if r.get("learned"):
+                    legs48[key]["learned"] = r["learned"]
+            legs48["workload"] = "Kuhn 48^3, eps_r U[1,10] per 8^3 block (seed 2), gfdtd tol 1e-8, 200 steps steady state"
+            legs48["speedup"] = legs48["learned_fp32"]["edge_dof_per_s"] / legs48["plain_fp64"]["edge_dof_per_s"]
+        line = {
+            "metric": METRIC, "value": value, "unit": "edge-DoF/s", "n_gpus": 1, "steps": steps,
+            "requested_steps": args.steps if args.steps_given else None,
+            "warmup": args.warmup, "ms_per_step": 1e3 * sec32 / steps, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": f"Kuhn {n}^3 jittered tet cavity (seed 5), eps_r log-uniform [1,80] per 6^3 "
+                                   f"block, method gfdtd (CG tol 1e-8), {steps} steps end to end",
+                       "path": "learned+FP32 (200 collection solves, training and set-up inside the timed job)",
+                       "n_edges": E, "n_faces": bg.n_faces, "parallelism": "single GPU"},
+            "cg_iters_per_step": r32["cg_iters_total"] / steps,
+            "plain_fp64": {"edge_dof_per_s": plain_rate, "ms_per_step": 1e3 * sec64 / steps,
+                           "cg_iters_per_step": r64["cg_iters_total"] / steps,
+                           "cg_not_converged": r64["cg_not_converged"], "cg_max_relres": r64["cg_max_relres"]},
+            "speedup_vs_plain_fp64": value / plain_rate,
+            "amortized": amortized, "steady_state": steady, "fp32_error": fp32_error,
+            "learned_cg": {"cg_not_converged": r32["cg_not_converged"], "cg_max_relres": r32["cg_max_relres"]},
+            "roofline": roofs[0] if roofs else None, "roofline_kernels": roofs,
+            "plain_fp64_roofline": plain_roofs[0] if plain_roofs else None,
+            "A_storage": info_a32, "cpu_baseline": cpu, "legs_48": legs48, "kernels": kern32,
+        }
+        print(json.dumps(line), flush=True)
 
 
 def bench_fit(args):
@@ -410,20 +627,26 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=48, help="Kuhn cubes per side")
-    ap.add_argument("--cpu-steps", type=int, default=60, help="oracle steps of the CPU baseline (about 10-15 s at 48^3)")
+    ap.add_argument("--cpu-steps", type=int, default=None,
+                    help="oracle steps of the CPU baseline (default 10 at 96^3, 60 at 48^3: 10-30 s of CPU work)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-dga", action="store_true")
-    ap.add_argument("--workload", choices=["gfdtd", "contrast", "batch", "fit", "fitprec", "copy"], default="gfdtd",
-                    help="gfdtd: the 48^3 headline; contrast: gfdtd on the high-contrast Kuhn mesh (--n, default "
-                         "96); batch: batched dga instances; fit: FIT 64^3; fitprec: FIT random-media fp32 vs "
-                         "fp64 study (--n, default 256); copy: a 1 GiB device copy (the reference kernel for "
-                         "HBM counter scaling). Every workload but gfdtd prints its own JSON line.")
+    ap.add_argument("--no-48", action="store_true", help="headline: skip the 48^3 legs")
+    ap.add_argument("--workload", choices=["headline", "gfdtd", "contrast", "batch", "fit", "fitprec", "copy"],
+                    default="headline",
+                    help="headline: the metric's configuration end to end (96^3 contrast mesh, 3000 steps, "
+                         "learned+FP32 vs plain FP64, plus 48^3 legs); gfdtd: 48^3 steady-state legs; contrast: "
+                         "96^3 steady-state legs with the explicit dga path (--n); batch: batched dga instances; "
+                         "fit: FIT 64^3; fitprec: FIT random-media fp32 vs fp64 study (--n, default 256); copy: "
+                         "a 1 GiB device copy (the reference kernel for HBM counter scaling).")
     ap.add_argument("--batch", type=int, default=256, help="instances of the batch workload")
     ap.add_argument("--paths", default="plain64,jacobi32,learned32,learned64",
                     help="gfdtd paths to run (profiling runs pick one)")
     ap.add_argument("--learn-steps", type=int, default=200, help="collection solves of the learned paths")
     args = ap.parse_args()
     args.steps_given = any(a.startswith("--steps") for a in sys.argv[1:])
+    if args.cpu_steps is None:
+        args.cpu_steps = 10 if args.workload == "headline" else 60
     if args.gpus != 1:
         print(f"bench.py drives exactly one GPU (asked for {args.gpus}); multi-GPU runs are not part of this build",
               file=sys.stderr)
@@ -432,6 +655,9 @@ def main():
     if device_count() < 1:
         print("bench.py needs a HIP device", file=sys.stderr)
         sys.exit(1)
+    if args.workload == "headline":
+        bench_headline(args)
+        return
     if args.workload == "batch":
         bench_batch(args)
         return

@@ -28,7 +28,9 @@ extern "C" {
 typedef enum {
     TF_METHOD_FDTD  = 0,   /* "method fdtd"  on a cartesian mesher grid       */
     TF_METHOD_DGA   = 1,   /* "method dga"   explicit, local-inverse Hodge     */
-    TF_METHOD_GFDTD = 2    /* "method gfdtd" consistent eps mass, CG solve    */
+    TF_METHOD_GFDTD = 2,   /* "method gfdtd" consistent eps mass, CG solve    */
+    TF_METHOD_FEM   = 3    /* "method fem" Whitney edge elements, second order */
+                           /* in time, CG solve (D:1580-1790)                 */
 } tf_method;
 
 /* Compressed sparse rows. rowptr has nrows+1 entries (int64 so that row
@@ -82,6 +84,33 @@ typedef struct {
     int32_t        cells[3];
 } tf_fit_ops;
 
+/* Fractional unknowns of method dga with lossy (sigma) or magnetic-loss
+ * (chi) media, ConstructCodecasaMaterialMatrices D:9180-9481 and the loop
+ * D:883-920. Edge classes 1..4 and face classes 1..3 split the unknowns into
+ *   U_frac = [U_a (h_size) | U_b (p_size) | U_c (q_size) | U_d (b_size)]
+ *   F_frac = [F_a (n_size) | F_b (r_size) | F_c (s_size)]
+ * and each step does
+ *   U_frac <- Ru U_frac + dt Ku curl_f,   U = Mu U_frac,
+ *   F_frac <- Rf F_frac - dt Kf curl_u,   F = Tf F_frac,
+ * with Ku = [H; Mp; Mq; 0], Ru = [I; diag P_p; Q; I], Kf = [N; Tr; Ts],
+ * Rf = [I; diag R_r; S] (all rows of these are in that stacked order). The
+ * boundary values and e sources of an edge are written to U_frac entry
+ * slot[edge] (U_d for class 4, U_a for class 1; -1: no entry) before the
+ * update. enabled = 0 (and everything else empty) for lossless media, which
+ * step with H and N directly. */
+typedef struct {
+    int32_t        enabled;
+    int32_t        h_size, p_size, q_size, b_size;
+    int32_t        n_size, r_size, s_size;
+    tf_csr         Ku;            /* nU x n_edges, nU = h+p+q+b               */
+    tf_csr         Ru;            /* nU x nU                                  */
+    tf_csr         Mu;            /* n_edges x nU, entries 1 (1 or 2 per row)  */
+    tf_csr         Kf;            /* nF x n_faces, nF = n+r+s                 */
+    tf_csr         Rf;            /* nF x nF                                  */
+    tf_csr         Tf;            /* n_faces x nF, entries 1 (1 or 2 per row)  */
+    const int32_t* slot;          /* per edge                                 */
+} tf_dga_frac;
+
 /* DGA operators (methods dga and gfdtd). */
 typedef struct {
     tf_csr         C;             /* faces x edges, signs                     */
@@ -95,15 +124,20 @@ typedef struct {
     const int32_t* h_row;         /* per edge: h_index for class 1, else -1   */
     /* both */
     tf_csr         N;             /* n_faces x n_faces                        */
-    /* gfdtd */
-    tf_csr         E;             /* n_edges x n_edges eps mass               */
-    tf_csr         A;             /* n_free x n_free system matrix            */
+    /* gfdtd and fem */
+    tf_csr         E;             /* n_edges x n_edges eps mass (fem: the     */
+                                  /* Whitney element mass, D:7917-8058)       */
+    tf_csr         A;             /* n_free x n_free system matrix (fem:      */
+                                  /* E/dt^2 + SigMat/(2 dt), D:8086-8112)     */
     tf_csr         RHS1;          /* RHSmat1: n_edges x n_edges               */
     const int32_t* free_edges;    /* compressed_dirichlet                      */
     int32_t        n_free;
     tf_csr         Sig;           /* SigMat: n_edges x n_edges conductivity    */
                                   /* mass on E's pattern (gfdtd); nnz 0 when   */
                                   /* no tet is lossy                           */
+    /* dga with lossy media: the fractional operators (H is then empty and N
+     * is the lossless all-face N of the energy diagnostic) */
+    tf_dga_frac    frac;
 } tf_dga_ops;
 
 typedef struct {

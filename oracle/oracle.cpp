@@ -437,6 +437,121 @@ void run_dga(const tf_problem* p, const SourceTable& st, const tf_probe_geom* pg
     if (out->B) std::memcpy(out->B, B.data(), sizeof(double) * Fn);
 }
 
+// ---------------------------------------------------------------- dga, lossy
+// D:538-939 with edge classes 2/3 and face classes 2/3: the fractional
+// unknowns U_frac = [U_a | U_b | U_c | U_d], F_frac = [F_a | F_b | F_c]
+// (the stacked operators of tf_dga_frac: Ku = [H; Mp; Mq], Ru = [I; diag
+// P_p; Q; I], Kf = [N; Tr; Ts], Rf = [I; diag R_r; S]). Deviation: the
+// reference maps Old_F_c onto the old-U buffer (D:586), reading the wrong
+// (and possibly out-of-range) entries; F_c here reads the old F_c.
+void run_dga_frac(const tf_problem* p, const SourceTable& st, const tf_probe_geom* pg, int32_t np, Printer pr,
+                  int64_t first, int64_t n, const double* U0, const double* F0, const double* B0, oracle_out* out) {
+    const tf_dga_ops& d = p->dga;
+    const tf_dga_frac& fr = d.frac;
+    const int32_t E = p->n_edges, Fn = p->n_faces;
+    const int32_t h = fr.h_size, pp = fr.p_size, q = fr.q_size;
+    const int32_t nU = fr.Ku.nrows, nF = fr.Kf.nrows;
+    const int32_t nn = fr.n_size, nr = fr.r_size;
+    const double dt = p->dt;
+    std::vector<double> U(E, 0.0), F(Fn, 0.0), B(Fn, 0.0), I(E, 0.0), Uf(nU, 0.0), Ff(nF, 0.0);
+    // a written state: each edge's (face's) value split evenly over its
+    // fractional entries
+    if (U0) {
+        U.assign(U0, U0 + E);
+        for (int32_t e = 0; e < E; ++e) {
+            const int64_t a = fr.Mu.rowptr[e], c = fr.Mu.rowptr[e + 1];
+            for (int64_t k = a; k < c; ++k) Uf[fr.Mu.col[k]] = U[e] / (double)(c - a);
+        }
+    }
+    if (F0) {
+        F.assign(F0, F0 + Fn);
+        for (int32_t f = 0; f < Fn; ++f) {
+            const int64_t a = fr.Tf.rowptr[f], c = fr.Tf.rowptr[f + 1];
+            for (int64_t k = a; k < c; ++k) Ff[fr.Tf.col[k]] = F[f] / (double)(c - a);
+        }
+    }
+    if (B0) B.assign(B0, B0 + Fn);
+    // before the loop (D:726-790): boundary edges take ComputeEdgeBC(ee,0),
+    // every source edge ComputeEfieldSource(ee,0) in each of its fractional
+    // entries (half of it in U_c)
+    for (int32_t e = 0; e < E; ++e)
+        if (d.edge_class[e] == 4 && st.by_edge[e] < 0) Uf[fr.slot[e]] = d.fixed_value[e];
+    auto put_all = [&](int32_t e, double v) {
+        for (int64_t k = fr.Mu.rowptr[e]; k < fr.Mu.rowptr[e + 1]; ++k) {
+            const int32_t c = fr.Mu.col[k];
+            Uf[c] = (c >= h + pp && c < h + pp + q) ? 0.5 * v : v;
+        }
+    };
+    for (int64_t k = 0; k < (int64_t)st.se.size(); ++k) {
+        if (st.se[k].field == TF_FIELD_J) continue;  // rejected for dga (A-7)
+        put_all(st.se[k].edge, e_source(st, k, 0.0));
+    }
+    auto mv_add = [](const tf_csr& M, const std::vector<double>& x, int32_t r) {
+        double s = 0;
+        for (int64_t k = M.rowptr[r]; k < M.rowptr[r + 1]; ++k) s += M.val[k] * x[M.col[k]];
+        return s;
+    };
+    auto gather = [](const tf_csr& M, const std::vector<double>& x, std::vector<double>& y) {
+        for (int32_t r = 0; r < M.nrows; ++r) {
+            double s = 0;
+            for (int64_t k = M.rowptr[r]; k < M.rowptr[r + 1]; ++k) s += x[M.col[k]];
+            y[r] = s;
+        }
+    };
+    gather(fr.Mu, Uf, U);  // U = M*U_frac
+    Sink sink{out, np};
+    std::vector<double> six, U_old, Uf_old, Ff_old, curl_b(E, 0.0), cf, kc, cu, kf;
+    if (first == 1 && pr.allow(0.0)) {
+        export_tet(U, B, pg, np, six);
+        sink.put(0.0, six);
+    }
+    double loop_s = 0;
+    for (int64_t i = first; i < first + n; ++i) {
+        const double t0 = now_s();
+        const double t = (double)i * dt;
+        std::fill(curl_b.begin(), curl_b.end(), 0.0);
+        U_old = U;
+        Uf_old = Uf;
+        for (int32_t e = 0; e < E; ++e)
+            if (d.edge_class[e] == 4 && st.by_edge[e] < 0) Uf[fr.slot[e]] = t < 0 ? 0.0 : d.fixed_value[e];
+        for (int64_t k = 0; k < (int64_t)st.se.size(); ++k) {
+            const tf_source_edge& s = st.se[k];
+            if (s.field == TF_FIELD_H) curl_b[s.edge] = h_source(st, k, t - 0.5 * dt);
+            else if (s.field != TF_FIELD_J) put_all(s.edge, e_source(st, k, t));
+        }
+        csr_mv(d.Ct, F, cf);
+        for (int32_t e = 0; e < E; ++e) cf[e] = cf[e] + curl_b[e] - I[e];
+        csr_mv(fr.Ku, cf, kc);
+        // U_a = U_a + dt*(H curl_f); U_b = P_p .* Old_U_b + dt*(Mp curl_f);
+        // U_c = Q Old_U_c + dt*(Mq curl_f)  (D:888-890)
+        for (int32_t r = 0; r < h; ++r) Uf[r] = Uf[r] + dt * kc[r];
+        for (int32_t r = h; r < h + pp + q; ++r) Uf[r] = mv_add(fr.Ru, Uf_old, r) + dt * kc[r];
+        gather(fr.Mu, Uf, U);
+        double poy = 0;
+        for (int32_t e = 0; e < E; ++e) poy += (U_old[e] + U[e]) * curl_b[e];
+        sink.poy(0.5 * poy);
+        Ff_old = Ff;
+        csr_mv(d.C, U, cu);
+        for (int32_t f = 0; f < Fn; ++f) B[f] = B[f] - dt * cu[f];
+        // F_a -= (dt N) curl_u; F_b = R_r .* Old_F_b - (dt Tr) curl_u;
+        // F_c = S Old_F_c - (dt Ts) curl_u  (D:916-919)
+        csr_mv(fr.Kf, cu, kf, dt);
+        for (int32_t r = 0; r < nn; ++r) Ff[r] = Ff[r] - kf[r];
+        for (int32_t r = nn; r < nF; ++r) Ff[r] = mv_add(fr.Rf, Ff_old, r) - kf[r];
+        (void)nr;
+        gather(fr.Tf, Ff, F);
+        loop_s += now_s() - t0;
+        if (pr.allow(t)) {
+            export_tet(U, B, pg, np, six);
+            sink.put(t, six);
+        }
+    }
+    out->seconds = loop_s;
+    if (out->U) std::memcpy(out->U, U.data(), sizeof(double) * E);
+    if (out->F) std::memcpy(out->F, F.data(), sizeof(double) * Fn);
+    if (out->B) std::memcpy(out->B, B.data(), sizeof(double) * Fn);
+}
+
 // ConjugateGradientSolver, ConjugateGradientSolver.hpp:52-128
 struct Cg {
     int32_t n;
@@ -597,6 +712,118 @@ void run_gfdtd(const tf_problem* p, const SourceTable& st, const tf_probe_geom* 
     if (out->B) std::memcpy(out->B, B.data(), sizeof(double) * Fn);
 }
 
+// ---------------------------------------------------------------- fem
+// D:1580-1790: Whitney edge elements, second order in time. Per step i
+// (t = i dt): curl_u = C U, B -= dt curl_u, export at t - dt; sources at t;
+//   Psi = 2/dt^2 E U_old - Ct N C U_old + 1/dt Ctb (Fb - Fb_old)
+//         - 1/dt^2 E U_older + 0.5/dt SigMat U_older - RHSmat1 SrcFld - I/dt,
+// A solution = Psi on the free edges (warm start: the previous solution,
+// zero before the first solve), U_older = U_old, U_old = U.
+void run_fem(const tf_problem* p, const SourceTable& st, const tf_probe_geom* pg, int32_t np, Printer pr,
+             int64_t first, int64_t n, double tol, int32_t maxiter, const double* U0, const double* B0,
+             oracle_out* out) {
+    const tf_dga_ops& d = p->dga;
+    const int32_t E = p->n_edges, Fn = p->n_faces, nf = d.n_free;
+    const double dt = p->dt;
+    std::vector<double> U(E, 0.0), B(Fn, 0.0), I(E, 0.0), SrcFld(E, 0.0), Fb(E, 0.0), Fb_old(E, 0.0);
+    if (U0) U.assign(U0, U0 + E);
+    if (B0) B.assign(B0, B0 + Fn);
+    std::vector<uint8_t> is_free(E, 0);
+    for (int32_t k = 0; k < nf; ++k) is_free[d.free_edges[k]] = 1;
+    std::vector<double> U_old = U, U_older = U_old;
+    // the pass before the loop (D:1660-1680): no h sources
+    for (int32_t e = 0; e < E; ++e)
+        if (!is_free[e] && st.by_edge[e] < 0) { U[e] = d.fixed_value[e]; SrcFld[e] = U[e]; }
+    auto src_pass = [&](double t, bool h_too) {
+        for (int64_t k = 0; k < (int64_t)st.se.size(); ++k) {
+            const tf_source_edge& s = st.se[k];
+            if (s.field == TF_FIELD_H) {
+                if (h_too) Fb[s.edge] = h_source(st, k, t - 0.5 * dt);
+            } else if (s.field == TF_FIELD_J) {
+                I[s.edge] = j_source(st, k, t);
+                SrcFld[s.edge] = U[s.edge];
+            } else {
+                U[s.edge] = e_source(st, k, t);
+                SrcFld[s.edge] = U[s.edge];
+            }
+        }
+    };
+    src_pass(0.0, false);
+    U_older = U_old;
+    U_old = U;
+    Cg cg{nf, d.A.rowptr, d.A.col, d.A.val, {}, tol, maxiter};
+    cg.compute();
+    std::vector<double> solution(nf, 0.0), rhs(nf, 0.0), cu, nc, cc, eu, eo, so, r1, sx;
+    Sink sink{out, np};
+    std::vector<double> six;
+    auto quad_sig = [&](const std::vector<double>& x) {
+        if (d.Sig.nnz == 0) return 0.0;
+        csr_mv(d.Sig, x, sx);
+        double q = 0;
+        for (int32_t e = 0; e < E; ++e) q += x[e] * sx[e];
+        return q;
+    };
+    auto loss = [&](double v) {
+        if (out->losses && out->n_losses < out->cap_losses) out->losses[out->n_losses] = v;
+        out->n_losses++;
+    };
+    const double s2 = 1 / (dt * dt);
+    double loop_s = 0;
+    for (int64_t i = first; i < first + n; ++i) {
+        double t0 = now_s();
+        const double t = (double)i * dt;
+        csr_mv(d.C, U, cu);
+        for (int32_t f = 0; f < Fn; ++f) B[f] -= dt * cu[f];
+        loop_s += now_s() - t0;
+        if (pr.allow(t - dt)) {
+            export_tet(U, B, pg, np, six);
+            sink.put(t - dt, six);
+        }
+        t0 = now_s();
+        Fb_old = Fb;
+        for (int32_t e = 0; e < E; ++e)
+            if (!is_free[e] && st.by_edge[e] < 0) { U[e] = t < 0 ? 0.0 : d.fixed_value[e]; SrcFld[e] = U[e]; }
+        src_pass(t, true);
+        csr_mv(d.C, U_old, cu);
+        csr_mv(d.N, cu, nc);
+        csr_mv(d.Ct, nc, cc);
+        csr_mv(d.E, U_old, eu);
+        csr_mv(d.E, U_older, eo);
+        const bool sig = d.Sig.nnz > 0;
+        if (sig) csr_mv(d.Sig, U_older, so);
+        csr_mv(d.RHS1, SrcFld, r1);
+        double poy = 0;
+        for (int32_t e = 0; e < E; ++e) poy += U_old[e] * (0.5 * (Fb[e] + Fb_old[e]));
+        sink.poy(poy);
+        loss(0.5 * quad_sig(U_old));
+        for (int32_t k = 0; k < nf; ++k) {
+            const int32_t e = d.free_edges[k];
+            double psi = 2.0 * eu[e] * s2 - cc[e];
+            psi = psi + (1 / dt) * (Fb[e] - Fb_old[e]);
+            psi = psi - eo[e] * s2;
+            if (sig) psi = psi + (0.5 / dt) * so[e];
+            psi = psi - r1[e];
+            psi = psi - (1 / dt) * I[e];
+            rhs[k] = psi;
+        }
+        solution = cg.solve(rhs, solution);
+        out->cg_iters_total += cg.iter;
+        if (cg.iter > out->cg_iters_max) out->cg_iters_max = cg.iter;
+        if (std::isfinite(cg.final_error) && cg.final_error > out->cg_max_relres) out->cg_max_relres = cg.final_error;
+        if (cg.iter >= maxiter && cg.final_error > tol) out->cg_not_converged++;
+        for (int32_t k = 0; k < nf; ++k) U[d.free_edges[k]] = solution[k];
+        if (i > 1) loss(quad_sig(U));
+        U_older = U_old;
+        U_old = U;
+        loop_s += now_s() - t0;
+    }
+    if (first + n > 1) loss(quad_sig(U));
+    out->seconds = loop_s;
+    if (out->U) std::memcpy(out->U, U.data(), sizeof(double) * E);
+    if (out->F) std::fill(out->F, out->F + Fn, 0.0);
+    if (out->B) std::memcpy(out->B, B.data(), sizeof(double) * Fn);
+}
+
 }  // namespace
 
 extern "C" {
@@ -650,8 +877,12 @@ int32_t oracle_run(const tf_problem* p, const tf_source_desc* sd, int32_t n_sd, 
         Printer pr{out_enabled != 0, out_delay, out_period};
         if (p->method == TF_METHOD_FDTD)
             run_fdtd(p, st, pg, n_probes, pr, first, n, faithful != 0, U0, F0, B0, out);
+        else if (p->method == TF_METHOD_DGA && p->dga.frac.enabled)
+            run_dga_frac(p, st, pg, n_probes, pr, first, n, U0, F0, B0, out);
         else if (p->method == TF_METHOD_DGA)
             run_dga(p, st, pg, n_probes, pr, first, n, U0, F0, B0, out);
+        else if (p->method == TF_METHOD_FEM)
+            run_fem(p, st, pg, n_probes, pr, first, n, cg_tol, cg_maxiter, U0, B0, out);
         else
             run_gfdtd(p, st, pg, n_probes, pr, first, n, cg_tol, cg_maxiter, U0, F0, B0, out);
         return 0;

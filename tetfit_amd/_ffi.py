@@ -35,12 +35,19 @@ class tf_fit_ops(C.Structure):
                 ("edge_sid", i32p), ("face_sid", i32p), ("cells", C.c_int32 * 3)]
 
 
+class tf_dga_frac(C.Structure):
+    _fields_ = [("enabled", C.c_int32), ("h_size", C.c_int32), ("p_size", C.c_int32), ("q_size", C.c_int32),
+                ("b_size", C.c_int32), ("n_size", C.c_int32), ("r_size", C.c_int32), ("s_size", C.c_int32),
+                ("Ku", tf_csr), ("Ru", tf_csr), ("Mu", tf_csr), ("Kf", tf_csr), ("Rf", tf_csr), ("Tf", tf_csr),
+                ("slot", i32p)]
+
+
 class tf_dga_ops(C.Structure):
     _fields_ = [("C", tf_csr), ("Ct", tf_csr), ("bnd_edge", i32p), ("n_bnd", C.c_int32),
                 ("edge_class", i8p), ("fixed_value", f64p),
                 ("H", tf_csr), ("h_row", i32p), ("N", tf_csr),
                 ("E", tf_csr), ("A", tf_csr), ("RHS1", tf_csr),
-                ("free_edges", i32p), ("n_free", C.c_int32), ("Sig", tf_csr)]
+                ("free_edges", i32p), ("n_free", C.c_int32), ("Sig", tf_csr), ("frac", tf_dga_frac)]
 
 
 class tf_problem(C.Structure):

@@ -139,9 +139,13 @@ def fit_random_media(directory, n=512, n_steps=4000, n_probes=16, eps_range=(1.0
 
 def kuhn_cavity(directory, n=48, method="dga", n_steps=1000, n_probes=8, jitter=0.2, seed=2,
                 block=8, eps_range=(1.0, 10.0), log_eps=False, freq=2e8, tol=1e-8, dt=None,
-                name=None, dt_safety=0.9, pec_surfaces="1,3,4,5,6", source_surface=2):
+                name=None, dt_safety=0.9, pec_surfaces="1,3,4,5,6", source_surface=2,
+                sigma=0.0, sigma_frac=0.0, chi=0.0, chi_frac=0.0, mu_range=None, extra_source=""):
     """Kuhn mesh of the unit cube, eps_r per block drawn from eps_range
-    (uniform, or log-uniform) with numpy default_rng(seed). Returns the .fdtd
+    (uniform, or log-uniform) with numpy default_rng(seed). Lossy media: a
+    fraction sigma_frac of the blocks (drawn after eps) get conductivity
+    sigma S/m, a fraction chi_frac magnetic loss chi; mu_range draws mu_r per
+    block (last). Returns the .fdtd
     path. When dt is None the host's stability estimate is computed once and
     dt = dt_safety x estimate (rounded to 4 digits) is pinned with SET step.
     dt="courant" writes no step and `SET courant dt_safety` instead (one host
@@ -162,6 +166,9 @@ def kuhn_cavity(directory, n=48, method="dga", n_steps=1000, n_probes=8, jitter=
         eps = np.exp(rng.uniform(math.log(lo), math.log(hi), size=len(labels)))
     else:
         eps = rng.uniform(lo, hi, size=len(labels))
+    sig = np.where(rng.uniform(size=len(labels)) < sigma_frac, sigma, 0.0) if sigma_frac > 0 else np.zeros(len(labels))
+    chis = np.where(rng.uniform(size=len(labels)) < chi_frac, chi, 0.0) if chi_frac > 0 else np.zeros(len(labels))
+    mus = rng.uniform(mu_range[0], mu_range[1], size=len(labels)) if mu_range else np.ones(len(labels))
     h = 1.0 / n
     pts = [((i + 0.37) / n_probes, 0.5 + 0.11 * h, 0.5 + 0.07 * h) for i in range(n_probes)]
 
@@ -171,11 +178,18 @@ def kuhn_cavity(directory, n=48, method="dga", n_steps=1000, n_probes=8, jitter=
             f.write(f"# Kuhn {n}^3 cavity, jitter {jitter}h, seed {seed}, method {method}\n")
             f.write("DEFINE mesh 1\n    SET type tetrahedral\n    SET mesher netgen\n"
                     "    SET file %s\nEND mesh 1\n" % os.path.basename(mesh))
-            for l, e in zip(labels, eps):
-                f.write("DEFINE material %d\n    SET epsilon %r\n    SET mu 1\nEND material %d\n" % (l, float(e), l))
+            for l, e, sg, ch, mu in zip(labels, eps, sig, chis, mus):
+                f.write("DEFINE material %d\n    SET epsilon %r\n    SET mu %s\n"
+                        % (l, float(e), "1" if mu == 1.0 else repr(float(mu))))
+                if sg:
+                    f.write("    SET sigma %r\n" % float(sg))
+                if ch:
+                    f.write("    SET chi %r\n" % float(ch))
+                f.write("END material %d\n" % l)
             f.write("DEFINE bc 1\n    SET type pec\n    SET surfaces {%s}\nEND bc 1\n" % pec_surfaces)
             f.write(SOURCE_BLOCK.format(carrier="sin", freq=repr(freq), extra="")
                     .replace("SET surface 2", "SET surface %d" % source_surface))
+            f.write(extra_source)
             f.write("DEFINE output 1\n    SET mode probepoint\n    SET name %s\n    SET period 0\n" % name)
             f.write(_probe_lines(pts))
             f.write("END output 1\n")

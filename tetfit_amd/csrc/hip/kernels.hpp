@@ -423,6 +423,55 @@ __global__ void __launch_bounds__(kBlock) dga_poynting_kernel(
     if (threadIdx.x == 0) poy_part[blockIdx.x] = s;
 }
 
+// ---- lossy dga: fractional unknowns (D:883-920, tf_dga_frac)
+// Sources of a lossy dga step. h edges as in dga_inject_kernel (curl_f and
+// the first Poynting partial); the other kinds write their value into the
+// edge's U_frac entry slot[e] (U_d of a class-4 edge, U_a of a class-1
+// edge; none for classes 2/3, whose entries the update rebuilds from their
+// old values, D:888-890). With init (the pass before the loop, D:726-790)
+// every source edge writes its e-source value at t = 0 (0 unless an e
+// source) into all of its fractional entries, half of it into U_c entries
+// ([q0, q1)); mu_a / mu_b are the entries of each edge's Mu row.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) frac_inject_kernel(
+    int32_t n_inj, const int32_t* __restrict__ inj_edge, const int8_t* __restrict__ inj_kind,
+    const int32_t* __restrict__ inj_tb, const int32_t* __restrict__ inj_te, const int32_t* __restrict__ term_src,
+    const double* __restrict__ term_w, const double* __restrict__ carr_step, T* __restrict__ curl_f,
+    const T* __restrict__ U, T* __restrict__ Uf, const int32_t* __restrict__ slot, const int32_t* __restrict__ mu_a,
+    const int32_t* __restrict__ mu_b, int32_t q0, int32_t q1, double* __restrict__ poy_part, int init) {
+    __shared__ double red[kBlock / kWave];
+    const int32_t i = blockIdx.x * kBlock + threadIdx.x;
+    double poy = 0;
+    if (i < n_inj) {
+        const int32_t e = inj_edge[i];
+        const double v = source_value(inj_tb[i], inj_te[i], term_src, term_w, carr_step);
+        if (init) {
+            const double u = inj_kind[i] == 0 ? 0.0 : v;
+            for (int32_t c : {mu_a[e], mu_b[e]})
+                if (c >= 0) Uf[c] = (T)(c >= q0 && c < q1 ? 0.5 * u : u);
+        } else if (inj_kind[i] == 0) {
+            curl_f[e] = (T)((double)curl_f[e] + v);
+            poy = 0.5 * (double)U[e] * v;
+        } else if (slot[e] >= 0) {
+            Uf[slot[e]] = (T)v;
+        }
+    }
+    const double s = block_sum(poy, red);
+    if (threadIdx.x == 0) poy_part[blockIdx.x] = s;
+}
+
+// out[i] = X[a[i]] (+ X[b[i]] when b[i] >= 0): U = Mu U_frac, F = Tf F_frac
+// (rows of one or two unit entries).
+template <typename T>
+__global__ void __launch_bounds__(kBlock) frac_gather_kernel(int32_t n, const int32_t* __restrict__ a,
+                                                             const int32_t* __restrict__ b, const T* __restrict__ X,
+                                                             T* __restrict__ out) {
+    const int32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const int32_t j = b[i];
+    out[i] = j >= 0 ? (T)((double)X[a[i]] + (double)X[j]) : X[a[i]];
+}
+
 // curl_u = C U (3 signed edges per face), B -= dt*curl_u (D:902-910).
 template <typename T>
 __global__ void __launch_bounds__(kBlock) dga_curl_kernel(int32_t n_faces, const int32_t* __restrict__ c_col,
@@ -459,30 +508,65 @@ __device__ __forceinline__ double sum_partials(const double* __restrict__ part, 
 }
 
 // U[free[k]] = x[k]; Dirichlet edges take their new values.
+// U_older (fem, or null): the old U of every written edge is kept first
+// (U_older = U_old, D:1764).
 template <typename T>
 __global__ void gf_scatter_kernel(int32_t n_free, const int32_t* __restrict__ free_edge, const T* __restrict__ x,
                                   T* __restrict__ U, int32_t n_dir, const int32_t* __restrict__ dir_edge,
-                                  const T* __restrict__ dir_new, const CgState* guard) {
+                                  const T* __restrict__ dir_new, const CgState* guard, T* __restrict__ U_older) {
     if (!cg_guard_ok(guard)) return;
     const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n_free) U[free_edge[i]] = x[i];
-    else if (i - n_free < n_dir) {
+    if (i < n_free) {
+        const int32_t e = free_edge[i];
+        if (U_older) U_older[e] = U[e];
+        U[e] = x[i];
+    } else if (i - n_free < n_dir) {
         const int32_t e = dir_edge[i - n_free];
+        if (U_older) U_older[e] = U[e];
         U[e] = dir_new[e];
+    }
+}
+
+// fem: the vectors the step's initial residual gathers (U = U_old here, Src
+// = the Dirichlet values at the new time): free edges W = W2 = U - U_older;
+// Dirichlet edges W = 2 U - U_older - Src, W2 = Src - U_older, so that with
+// x0 = U_old on the free edges
+//   Psi_f - A x0 = 1/dt^2 (E W)_f - 0.5/dt (SigMat W2)_f - (Ct N C U_old)_f
+//                  + 1/dt (Ctb (Fb - Fb_old) - I)_f
+// without the cancelling 2/dt^2 E U_old - A x0 of the reference's Psi
+// (D:1731-1733). guard as in cg_guard_ok.
+template <typename T>
+__global__ void fem_w_kernel(int32_t n_free, const int32_t* __restrict__ free_edge, int32_t n_dir,
+                             const int32_t* __restrict__ dir_edge, const T* __restrict__ U,
+                             const T* __restrict__ U_older, const T* __restrict__ src, T* __restrict__ W,
+                             T* __restrict__ W2, const CgState* guard) {
+    if (!cg_guard_ok(guard)) return;
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n_free) {
+        const int32_t e = free_edge[i];
+        const T v = (T)((double)U[e] - (double)U_older[e]);
+        W[e] = v;
+        W2[e] = v;
+    } else if (i - n_free < n_dir) {
+        const int32_t e = dir_edge[i - n_free];
+        const double u = U[e], uo = U_older[e], sv = src[e];
+        W[e] = (T)((u - uo) + (u - sv));
+        W2[e] = (T)(sv - uo);
     }
 }
 
 // gfdtd sources at step i (D:1478-1502): fb, the edge-indexed forcing
 // Ctb Fb - I of Psi (h sources at t - dt/2: Fb; j dipoles at t: -I), and
 // dir_new (e sources); the Poynting term U_old . 0.5 Ctb (Fb + Fb_old)
-// (D:1518) with fb_old from the previous step's carriers.
+// (D:1518) with fb_old from the previous step's carriers. With delta (fem,
+// D:1701-1733) an h edge's forcing is Fb - Fb_old.
 template <typename T>
 __global__ void __launch_bounds__(kBlock) gf_inject_kernel(
     int32_t n_inj, const int32_t* __restrict__ inj_edge, const int8_t* __restrict__ inj_kind,
     const int32_t* __restrict__ inj_tb, const int32_t* __restrict__ inj_te, const int32_t* __restrict__ term_src,
     const double* __restrict__ term_w, const double* __restrict__ carr_step, const double* __restrict__ carr_prev,
     T* __restrict__ fb, T* __restrict__ dir_new, const T* __restrict__ U, double* __restrict__ poy_part,
-    const CgState* guard) {
+    const CgState* guard, int delta) {
     if (!cg_guard_ok(guard)) return;
     __shared__ double red[kBlock / kWave];
     const int32_t i = blockIdx.x * kBlock + threadIdx.x;
@@ -491,8 +575,8 @@ __global__ void __launch_bounds__(kBlock) gf_inject_kernel(
         const int32_t e = inj_edge[i];
         const double v = source_value(inj_tb[i], inj_te[i], term_src, term_w, carr_step);
         if (inj_kind[i] == 0) {
-            fb[e] = (T)v;
             const double vo = carr_prev ? source_value(inj_tb[i], inj_te[i], term_src, term_w, carr_prev) : 0.0;
+            fb[e] = (T)(delta ? v - vo : v);
             poy = (double)U[e] * (0.5 * (v + vo));
         } else if (inj_kind[i] == 2) {
             fb[e] = (T)(-v);  // current I at t (D:1491-1492): -dt I joins dt Ctb Fb in Psi

@@ -22,15 +22,20 @@ constexpr int kBank = 64;  // samples in the bank = lanes of a wave
 
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
+// Training window: bank row e is matrix row w0 + e (e < nw); a column j
+// enters as bank row j - w0 and only when that lies in [0, nw) (the
+// principal submatrix of the window's rows).
+//
 // out[e][s] = sum_j M_ej in[j][s] (- sub[e][s]); M on A's SELL pattern with
 // fp32 values mval (0 on padding). One wave per row, lane = sample.
-__global__ void __launch_bounds__(kBlock) bank_spmm_kernel(int32_t n, const int64_t* __restrict__ sptr,
+__global__ void __launch_bounds__(kBlock) bank_spmm_kernel(int32_t nw, int32_t w0, const int64_t* __restrict__ sptr,
                                                            const int32_t* __restrict__ col,
                                                            const float* __restrict__ mval,
                                                            const float* __restrict__ in,
                                                            const float* __restrict__ sub, float* __restrict__ out) {
     const int lane = lane_id();
-    for (int row = uniform(wave_id()); row < n; row += wave_count()) {
+    for (int e = uniform(wave_id()); e < nw; e += wave_count()) {
+        const int row = w0 + e;
         const int s = row / kSlice, r = row % kSlice;
         const int64_t bs = sptr[s];
         const int w = (int)((sptr[s + 1] - bs) / kSlice);
@@ -42,27 +47,29 @@ __global__ void __launch_bounds__(kBlock) bank_spmm_kernel(int32_t n, const int6
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 const int64_t pos = bs + (int64_t)(k + u) * kSlice + r;
-                j[u] = col[pos];
+                j[u] = col[pos] - w0;
                 v[u] = mval[pos];
             }
 #pragma unroll
-            for (int u = 0; u < 4; ++u) x[u] = in[(int64_t)j[u] * kBank + lane];
+            for (int u = 0; u < 4; ++u) x[u] = (uint32_t)j[u] < (uint32_t)nw ? in[(int64_t)j[u] * kBank + lane] : 0.f;
 #pragma unroll
             for (int u = 0; u < 4; ++u) acc += v[u] * x[u];
         }
         for (; k < w; ++k) {
             const int64_t pos = bs + (int64_t)k * kSlice + r;
-            acc += mval[pos] * in[(int64_t)col[pos] * kBank + lane];
+            const int32_t j = col[pos] - w0;
+            if ((uint32_t)j < (uint32_t)nw) acc += mval[pos] * in[(int64_t)j * kBank + lane];
         }
-        const int64_t o = (int64_t)row * kBank + lane;
+        const int64_t o = (int64_t)e * kBank + lane;
         if (sub) acc -= sub[o];
         out[o] = acc;
     }
 }
 
-// Per SELL position (e, j): gpos = coef * sum_s w_s AE[e][s] R[j][s]; loss
-// partial per block: sum_e sum_s w_s E[e][s] AE[e][s].
-__global__ void __launch_bounds__(kBlock) bank_grad_kernel(int32_t n, const int64_t* __restrict__ sptr,
+// Per SELL position (e, j) of the window's rows: gpos = coef * sum_s w_s
+// AE[e][s] R[j][s] (0 for a column outside the window); loss partial per
+// block: sum_e sum_s w_s E[e][s] AE[e][s].
+__global__ void __launch_bounds__(kBlock) bank_grad_kernel(int32_t nw, int32_t w0, const int64_t* __restrict__ sptr,
                                                            const int32_t* __restrict__ col,
                                                            const float* __restrict__ R, const float* __restrict__ E,
                                                            const float* __restrict__ AE,
@@ -72,16 +79,18 @@ __global__ void __launch_bounds__(kBlock) bank_grad_kernel(int32_t n, const int6
     const int lane = lane_id();
     const float ws = wts[lane];
     double loss = 0;
-    for (int row = uniform(wave_id()); row < n; row += wave_count()) {
+    for (int e = uniform(wave_id()); e < nw; e += wave_count()) {
+        const int row = w0 + e;
         const int s = row / kSlice, r = row % kSlice;
         const int64_t bs = sptr[s];
         const int w = (int)((sptr[s + 1] - bs) / kSlice);
-        const int64_t o = (int64_t)row * kBank + lane;
+        const int64_t o = (int64_t)e * kBank + lane;
         const float ae = AE[o] * ws;
         loss += (double)ae * (double)E[o];
         for (int k = 0; k < w; ++k) {
             const int64_t pos = bs + (int64_t)k * kSlice + r;
-            float g = ae * R[(int64_t)col[pos] * kBank + lane];
+            const int32_t j = col[pos] - w0;
+            float g = (uint32_t)j < (uint32_t)nw ? ae * R[(int64_t)j * kBank + lane] : 0.f;
 #pragma unroll
             for (int off = 32; off > 0; off >>= 1) g += __shfl_xor(g, off, 64);
             if (lane == 0) gpos[pos] = coef * g;
@@ -128,26 +137,24 @@ __global__ void expand_kernel(int64_t entries, const int32_t* __restrict__ pos2p
     if (gpk) gpk[pos] = ((uint32_t)c16[pos] << 16) | b;
 }
 
-// Bank column `slot` <- r, x (the CG iterate being captured).
+// Bank column `slot` <- x (the CG iterate x_k being captured), the
+// training window's rows (x points at its first row).
 template <typename T>
-__global__ void bank_capture_kernel(int32_t n, int slot, const T* __restrict__ r, const T* __restrict__ x,
-                                    float* __restrict__ Rb, float* __restrict__ Cb) {
+__global__ void bank_capture_kernel(int32_t nw, int slot, const T* __restrict__ x, float* __restrict__ Cb) {
     const int32_t e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= n) return;
-    Rb[(int64_t)e * kBank + slot] = (float)r[e];
+    if (e >= nw) return;
     Cb[(int64_t)e * kBank + slot] = (float)x[e];
 }
 
-// After the solve: c~ = D^1/2 (x_final - x_k), r~ = D^-1/2 r_k.
+// After the solve: c~ = D^1/2 (x_final - x_k) (xf and isd point at the
+// window's first row). The residual r~ = Â c~ is formed at training time.
 template <typename T>
-__global__ void bank_finalize_kernel(int32_t n, int slot, const T* __restrict__ xf, const float* __restrict__ isd,
-                                     float* __restrict__ Rb, float* __restrict__ Cb) {
+__global__ void bank_finalize_kernel(int32_t nw, int slot, const T* __restrict__ xf, const float* __restrict__ isd,
+                                     float* __restrict__ Cb) {
     const int32_t e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= n) return;
+    if (e >= nw) return;
     const int64_t o = (int64_t)e * kBank + slot;
-    const double d = (double)isd[e];
-    Cb[o] = (float)(((double)xf[e] - (double)Cb[o]) / d);
-    Rb[o] = (float)((double)Rb[o] * d);
+    Cb[o] = (float)(((double)xf[e] - (double)Cb[o]) / (double)isd[e]);
 }
 
 // Uniform(-sqrt3, sqrt3) probe entries (unit variance) from a counter hash,

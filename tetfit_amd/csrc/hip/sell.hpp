@@ -209,6 +209,24 @@ __global__ void __launch_bounds__(kBlock) sell_h_kernel(SellView<T> m, const int
     }
 }
 
+// Lossy dga, one fractional row r per lane (D:888-890, D:916-919):
+// Y[r] = (R X)[r] + sdt (K v)[r], K the row of H / Mp / Mq (N / Tr / Ts)
+// applied to curl_f (curl_u), R the row's old entries (the identity, P_p
+// or R_r, a node-star Q or tet S block) applied to the old U_frac (F_frac);
+// sdt = dt for U, -dt for F. K and R have the same rows.
+template <typename T, int NB, bool W = false>
+__global__ void __launch_bounds__(kBlock) sell_frac_kernel(SellView<T> K, SellView<T> R, const T* __restrict__ v,
+                                                           const T* __restrict__ X, T* __restrict__ Y, double sdt) {
+    const int lane = lane_id();
+    TF_FOR_SLICES(K) {
+        const int s = sr_.s;
+        const double kv = sell_dot<T, NB, W>(K, s, lane, [&](int32_t j) { return v[j]; });
+        const double rx = sell_dot<T, NB, W>(R, s, lane, [&](int32_t j) { return X[j]; });
+        const int32_t r = s * kSlice + lane;
+        if (r < K.nrows) Y[r] = (T)(rx + sdt * kv);
+    }
+}
+
 // mode 0: F -= dt * N x (dga, D:917); mode 1: F = N x (gfdtd, D:1468);
 // row_dt as in sell_h_kernel; guard as in cg_guard_ok.
 template <typename T, int NB, bool W = false>
@@ -686,11 +704,20 @@ __global__ void __launch_bounds__(kBlock) cg_wu_kernel(int32_t it, SellView<T> m
 // products are skipped. r0 is kept in `r0save`
 // for a restart; with jacobi also u = pre r and partial r.u. guard as in
 // cg_guard_ok. SrcFld is U on Dirichlet edges at the new time (`dir_new`).
+//
+// The same kernel forms fem's residual (fem_w_kernel) with other operators,
+// vectors and coefficients: r0 = (ck (kop xk) + cr (r1 xr)) + cc (ct F) + cf
+// fb; gfdtd: xk = U, xr = dir_new, ck = 1, cr = -1, cc = cf = dt; fem: kop =
+// E_f, r1 = SigMat_f, xk = W, xr = W2, ck = 1/dt^2, cr = -0.5/dt, cc = -1,
+// cf = 1/dt, F = N C U_old.
+struct RhsCoef {
+    double ck, cr, cc, cf;
+};
 template <typename T, int NB, bool W = false>
 __global__ void __launch_bounds__(kBlock) gf_rhs_init_kernel(
     SellView<T> kop, SellView<T> r1, SellView<int8_t> ct, int dir_terms, const int32_t* __restrict__ free_edge,
     const T* __restrict__ U, const T* __restrict__ dir_new, const T* __restrict__ Fv, const T* __restrict__ fb,
-    T* __restrict__ r0save, double dt, const T* __restrict__ pre, T* __restrict__ r, int r_stride,
+    T* __restrict__ r0save, RhsCoef co, const T* __restrict__ pre, T* __restrict__ r, int r_stride,
     T* __restrict__ u, double* __restrict__ part_rr, double* __restrict__ part_gam, CgState* st, int jacobi,
     const CgState* guard) {
     if (!cg_guard_ok(guard)) return;
@@ -708,7 +735,7 @@ __global__ void __launch_bounds__(kBlock) gf_rhs_init_kernel(
         const double cf = sell_dot<T, NB, W>(ct, s, lane, [&](int32_t j) { return Fv[j]; });
         const int32_t k = s * kSlice + lane;
         if (k < ct.nrows) {
-            const T rk = (T)((ku - rs) + dt * cf + dt * (double)fb[free_edge[k]]);
+            const T rk = (T)((co.ck * ku + co.cr * rs) + co.cc * cf + co.cf * (double)fb[free_edge[k]]);
             r0save[k] = rk;
             r[(int64_t)k * r_stride] = rk;
             rr += (double)rk * (double)rk;

@@ -9,14 +9,17 @@
 
 #include <algorithm>
 #include <array>
+#include <atomic>
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <cmath>
 #include <cstring>
+#include <future>
 #include <map>
 #include <memory>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "common.hpp"
@@ -130,6 +133,43 @@ struct KTune {
         }                                                                                   \
     } while (0)
 
+// Runs f(lo, hi) over [0, n) in chunks of `chunk` entries on up to 16 host
+// threads (chunk boundaries do not depend on the thread count). f must not
+// throw.
+template <class Fn>
+void par_chunks(int64_t n, int64_t chunk, const Fn& f) {
+    const int64_t nc = (n + chunk - 1) / chunk;
+    const int64_t hw = std::max(1u, std::thread::hardware_concurrency());
+    const int nt = (int)std::min<int64_t>(nc, std::min<int64_t>(hw, 16));
+    if (nt <= 1) {
+        for (int64_t k = 0; k < nc; ++k) f(k * chunk, std::min(n, (k + 1) * chunk));
+        return;
+    }
+    std::atomic<int64_t> next{0};
+    std::vector<std::thread> th;
+    for (int t = 0; t < nt; ++t)
+        th.emplace_back([&] {
+            for (int64_t k; (k = next++) < nc;) f(k * chunk, std::min(n, (k + 1) * chunk));
+        });
+    for (auto& t : th) t.join();
+}
+
+// Rows of the free-row order the learned preconditioner is trained on: all
+// of them up to this many, else a window of this many in the middle of the
+// order (a slab of the mesh); the loss is over the principal submatrix of
+// those rows, with every sample's residual formed from its correction.
+constexpr int32_t kTrainRows = 1 << 20;
+
+// Host tables of the learned preconditioner (learned_tables()).
+struct LearnedTables {
+    int32_t n_pairs = 0, n_pad = 0;
+    int64_t pair_w0 = 0, pair_w1 = 0;  // pairs whose first row lies in the training window
+    std::vector<uint16_t> X;           // standardised bf16 features [pair][16]
+    std::vector<float> base, scale, ahat;
+    std::vector<int64_t> pa, pb;
+    std::vector<int32_t> pos2pair;
+};
+
 // State of the learned preconditioner of a gfdtd context.
 struct LearnedPc {
     enum { kOff = 0, kCollecting = 1, kActive = 2, kDisabled = 3 };
@@ -138,6 +178,9 @@ struct LearnedPc {
     int learn_steps = 200, train_iters = 150;
     double lr = 3e-3;
     int32_t n_pairs = 0, n_pad = 0;
+    int32_t w0 = 0, nw = 0;            // training window: free rows [w0, w0 + nw)
+    int64_t pair_w0 = 0, pair_w1 = 0;  // pairs whose first row lies in the window
+    std::future<std::unique_ptr<LearnedTables>> tables;  // host tables, built in the background
     DevBuf X, base, scale, pa, pb, pos2pair, y, ghat, gval, gpk, ahat, isd;
     DevBuf P, Pbest, m, v, part, gy, gpos, lpart, cpart;
     DevBuf Rb, Cb, T1, T2;
@@ -190,6 +233,19 @@ struct tf_ctx {
     Sell sCt, H, N, A, Kf, R1f, Ctf;  // Kf: E_f - [A 0] of the gfdtd residual (gf_rhs_init_kernel)
     Sell Efull;  // eps mass E of the energy diagnostic (all edges), tet methods
     Sell Sig;    // gfdtd SigMat (all edges) of the losses, lossy media only
+    // dga with lossy media (tf_dga_frac): stacked fractional operators, the
+    // Mu / Tf rows as (first, second or -1) entries, the source slot of each
+    // edge, and U_frac / F_frac ping-pong buffers (fcur: the current one)
+    bool frac = false;
+    Sell fKu, fRu, fKf, fRf;
+    // fem: U of the step before last (U_older) and the residual's gathered
+    // vectors W, W2 (fem_w_kernel); Kf, R1f then hold E and SigMat on the free rows
+    bool fem = false;
+    DevBuf U_older, fw, fw2;
+    DevBuf mu_a, mu_b, tf_a, tf_b, fslot, uf[2], ff[2];
+    int fcur = 0;
+    int32_t nU = 0, nF = 0, q0 = 0, q1 = 0;
+    std::vector<int32_t> h_mu_a, h_mu_b, h_tf_a, h_tf_b;
     DevBuf loss_part;            // per step and block: U_old.SigMat U_old partials of a run
     int64_t loss_cap = 0;
     std::vector<double> loss_q;  // per step i of the simulation: U_old(i).SigMat U_old(i)
@@ -298,11 +354,12 @@ size_t tf_ctx::device_bytes() const {
                            &sp_edge, &sp_f4, &sp_sg, &sp_ca, &sp_cb, &sp_kind, &sp_tb, &sp_te, &c_col, &c_sg,
                            &h_edge, &curl_f, &curl_u, &free_edge, &dir_edge, &dir_new, &fb, &pre, &x, &r, &rs0, &rs1, &z,
                            &p0, &p1, &q, &r0, &cg_part, &cg_state, &inj_edge, &inj_kind, &inj_tb, &inj_te,
-                           &term_src, &term_w, &carr, &probes, &samples, &poy, &row_dt_h, &row_dt_f};
+                           &term_src, &term_w, &carr, &probes, &samples, &poy, &row_dt_h, &row_dt_f,
+                           &mu_a, &mu_b, &tf_a, &tf_b, &fslot, &uf[0], &uf[1], &ff[0], &ff[1], &U_older, &fw, &fw2};
     for (auto b : all) s += b->bytes;
     const Csr32* cs[] = {&Ct};
     for (auto c : cs) s += c->ptr.bytes + c->col.bytes + c->val.bytes + c->sg.bytes;
-    const Sell* ss[] = {&sCt, &H, &N, &A, &Kf, &R1f, &Ctf, &Efull, &Sig};
+    const Sell* ss[] = {&sCt, &H, &N, &A, &Kf, &R1f, &Ctf, &Efull, &Sig, &fKu, &fRu, &fKf, &fRf};
     for (auto c : ss) s += c->sptr.bytes + c->col.bytes + c->val.bytes + c->sg.bytes + c->col16.bytes + c->cbase.bytes;
     for (auto b : lp.bufs()) s += b->bytes;
     return s;
@@ -838,11 +895,58 @@ void create_tet(tf_ctx* c, const tf_problem* p) {
     }
     upload_sell<T>(c->N, d.N, true);
     if (d.E.nrows == E && d.E.nnz > 0) upload_sell<T>(c->Efull, d.E, true);
-    if (p->method == TF_METHOD_GFDTD && d.Sig.nnz > 0) upload_sell<T>(c->Sig, d.Sig, true);
+    if ((p->method == TF_METHOD_GFDTD || p->method == TF_METHOD_FEM) && d.Sig.nnz > 0) upload_sell<T>(c->Sig, d.Sig, true);
     c->h_fixed_value.assign(d.fixed_value, d.fixed_value + E);
     zero_field<T>(c->curl_u, F, c->stream);
     zero_field<T>(c->curl_f, E, c->stream);
-    if (p->method == TF_METHOD_DGA) {
+    if (p->method == TF_METHOD_DGA && d.frac.enabled) {
+        // lossy media: the fractional operators in their own row order
+        const tf_dga_frac& fr = d.frac;
+        c->frac = true;
+        c->nU = fr.Ku.nrows;
+        c->nF = fr.Kf.nrows;
+        c->q0 = fr.h_size + fr.p_size;
+        c->q1 = c->q0 + fr.q_size;
+        if (fr.Ru.nrows != c->nU || fr.Rf.nrows != c->nF || fr.Mu.nrows != E || fr.Tf.nrows != F)
+            throw std::runtime_error("inconsistent fractional operators");
+        upload_sell<T>(c->fKu, fr.Ku, true);
+        upload_sell<T>(c->fRu, fr.Ru, true);
+        upload_sell<T>(c->fKf, fr.Kf, true);
+        upload_sell<T>(c->fRf, fr.Rf, true);
+        auto two = [](const tf_csr& m, int32_t ncols, std::vector<int32_t>& a, std::vector<int32_t>& b) {
+            a.assign(m.nrows, -1);
+            b.assign(m.nrows, -1);
+            for (int32_t r = 0; r < m.nrows; ++r) {
+                const int64_t k0 = m.rowptr[r], k1 = m.rowptr[r + 1];
+                if (k1 - k0 < 1 || k1 - k0 > 2) throw std::runtime_error("a Mu / Tf row needs one or two entries");
+                for (int64_t k = k0; k < k1; ++k) {
+                    if (m.col[k] < 0 || m.col[k] >= ncols || m.val[k] != 1.0)
+                        throw std::runtime_error("Mu / Tf entries must be unit entries in range");
+                }
+                a[r] = m.col[k0];
+                if (k1 - k0 == 2) b[r] = m.col[k0 + 1];
+            }
+        };
+        two(fr.Mu, c->nU, c->h_mu_a, c->h_mu_b);
+        two(fr.Tf, c->nF, c->h_tf_a, c->h_tf_b);
+        upload(c->mu_a, c->h_mu_a);
+        upload(c->mu_b, c->h_mu_b);
+        upload(c->tf_a, c->h_tf_a);
+        upload(c->tf_b, c->h_tf_b);
+        std::vector<int32_t> sl(fr.slot, fr.slot + E);
+        for (int32_t v : sl)
+            if (v < -1 || v >= c->nU) throw std::runtime_error("fractional slot out of range");
+        upload(c->fslot, sl);
+        // U_d holds the class-4 values (ComputeEdgeBC at t = 0, D:762-773)
+        std::vector<T> u0(std::max(c->nU, 1), (T)0);
+        for (int32_t e = 0; e < E; ++e)
+            if (d.edge_class[e] == 4 && sl[e] >= 0) u0[sl[e]] = (T)c->h_fixed_value[e];
+        for (int k = 0; k < 2; ++k) {
+            upload(c->uf[k], u0);
+            zero_field<T>(c->ff[k], std::max(c->nF, 1), c->stream);
+        }
+        c->fcur = 0;
+    } else if (p->method == TF_METHOD_DGA) {
         // H rows in SELL-C-sigma order; h_edge maps a SELL row to its edge
         c->h_perm = sigma_order(d.H);
         CsrBuf hs;
@@ -875,7 +979,30 @@ void create_tet(tf_ctx* c, const tf_problem* p) {
         c->hA_sptr = sell_layout(a.v);
         std::vector<int32_t> fe(d.n_free);
         for (int32_t k = 0; k < d.n_free; ++k) fe[k] = d.free_edges[perm[k]];
-        {
+        if (p->method == TF_METHOD_FEM) {
+            // fem residual: E and SigMat on the free rows (fem_w_kernel)
+            c->fem = true;
+            CsrBuf m;
+            select_rows(d.E, fe.data(), d.n_free, nullptr, m);
+            upload_sell<T>(c->Kf, m.v, true);
+            c->k_dir_only = false;
+            if (d.Sig.nnz > 0) {
+                select_rows(d.Sig, fe.data(), d.n_free, nullptr, m);
+            } else {
+                m = CsrBuf{};
+                m.rp.assign(d.n_free + 1, 0);
+                m.v = tf_csr{};
+                m.v.nrows = d.n_free;
+                m.v.ncols = E;
+                m.v.rowptr = m.rp.data();
+                m.v.col = m.col.data();
+                m.v.val = m.val.data();
+            }
+            upload_sell<T>(c->R1f, m.v, true);
+            select_rows(d.Ct, fe.data(), d.n_free, nullptr, m);
+            upload_sell<T>(c->Ctf, m.v, false);
+            for (DevBuf* b : {&c->U_older, &c->fw, &c->fw2}) zero_field<T>(*b, E, c->stream);
+        } else {
             CsrBuf m;
             residual_operator(d, p->dt, perm, fe, m);
             upload_sell<T>(c->Kf, m.v, true);
@@ -930,10 +1057,13 @@ void create_tet(tf_ctx* c, const tf_problem* p) {
         return m.sptr.bytes + (m.c16 ? m.col16.bytes : m.col.bytes) + m.val.bytes + m.sg.bytes;
     };
     c->wide = false;
-    for (const Sell* m : {&c->sCt, &c->H, &c->N, &c->A, &c->Kf, &c->R1f, &c->Ctf, &c->Efull, &c->Sig})
+    for (const Sell* m : {&c->sCt, &c->H, &c->N, &c->A, &c->Kf, &c->R1f, &c->Ctf, &c->Efull, &c->Sig, &c->fKu,
+                          &c->fRu, &c->fKf, &c->fRf})
         c->wide = c->wide || m->cwin;
-    const size_t streamed = p->method == TF_METHOD_DGA ? bytes(c->sCt) + bytes(c->H) + bytes(c->N)
-                                                       : bytes(c->A) + 4 * (size_t)c->A.entries;
+    const size_t streamed = p->method == TF_METHOD_DGA
+                                ? bytes(c->sCt) + bytes(c->H) + bytes(c->N) + bytes(c->fKu) + bytes(c->fRu) +
+                                      bytes(c->fKf) + bytes(c->fRf)
+                                : bytes(c->A) + 4 * (size_t)c->A.entries;
     c->wide = c->wide || streamed > ((size_t)256 << 20);
     // $TETFIT_SELL_WIDE (non-empty, not 0) selects the wide-mesh kernels on
     // any mesh, to exercise them with one-base columns on small meshes
@@ -1027,10 +1157,11 @@ void build_tet_inject(tf_ctx* c, const std::vector<std::vector<std::pair<int32_t
         // a d edge (neither h, j nor e) is set to its e-source sum, 0, by the
         // dga loop before the H update (D:835-842); gfdtd overwrites it with
         // the solve, so nothing is injected there
-        if (edge_field[e] == TF_FIELD_D && c->method == TF_METHOD_GFDTD) continue;
+        const bool solved = c->method == TF_METHOD_GFDTD || c->method == TF_METHOD_FEM;
+        if (edge_field[e] == TF_FIELD_D && solved) continue;
         edge.push_back(e);
         kind.push_back(edge_field[e] == TF_FIELD_H ? 0 : edge_field[e] == TF_FIELD_J ? 2 : 1);
-        if (edge_field[e] == TF_FIELD_J && c->method != TF_METHOD_GFDTD)
+        if (edge_field[e] == TF_FIELD_J && !solved)
             throw std::runtime_error("j sources are not supported for dga");
         tb.push_back((int32_t)tsrc.size());
         for (auto& pr : by_edge_terms[e]) { tsrc.push_back(pr.first); tw.push_back(pr.second); }
@@ -1239,6 +1370,86 @@ void run_fit(tf_ctx* c, int64_t first, int64_t n, tf_stats* st) {
     st->steps_done += n;
 }
 
+// Lossy dga (D:810-939 with edge classes 2/3 and face classes 2/3): one
+// step is curl_f = Ct F with the h sources, the boundary values and e
+// sources into U_frac, U_frac' = Ru U_frac + dt Ku curl_f, U = Mu U_frac',
+// the Poynting partials, curl_u = C U with the B update, F_frac' = Rf F_frac
+// - dt Kf curl_u, F = Tf F_frac'; the two U_frac (F_frac) buffers swap
+// roles every step. Launched from the host step by step (no graph).
+template <class T>
+void run_dga_frac(tf_ctx* c, int64_t first, int64_t n, tf_stats* st) {
+    const int inj_blocks = c->n_inj ? blocks_for(c->n_inj) : 0;
+    const size_t ns = std::max<size_t>(c->src.size(), 1);
+    if (inj_blocks && c->poy_cap < 2 * n * inj_blocks) {
+        c->poy.alloc(2 * n * inj_blocks * sizeof(double));
+        c->poy_cap = 2 * n * inj_blocks;
+    }
+    auto inject = [&](const double* cs, double* poy, int init) {
+        if (!c->n_inj) return;
+        launch_timed(c, "frac_inject_kernel", frac_inject_kernel<T>, inj_blocks, c->n_inj, c->inj_edge.as<int32_t>(),
+                     c->inj_kind.as<int8_t>(), c->inj_tb.as<int32_t>(), c->inj_te.as<int32_t>(),
+                     c->term_src.as<int32_t>(), c->term_w.as<double>(), cs, c->curl_f.as<T>(),
+                     (const T*)c->U.as<T>(), c->uf[c->fcur].as<T>(), c->fslot.as<int32_t>(),
+                     c->mu_a.as<int32_t>(), c->mu_b.as<int32_t>(), c->q0, c->q1, poy, init);
+    };
+    auto gather_u = [&]() {
+        launch_timed(c, "frac_gather_kernel", frac_gather_kernel<T>, blocks_for(c->E), c->E, c->mu_a.as<int32_t>(),
+                     c->mu_b.as<int32_t>(), (const T*)c->uf[c->fcur].as<T>(), c->U.as<T>());
+    };
+    if (!c->initial_done && first == 1) {
+        // the pass before the loop (D:726-790), then the export at t = 0
+        inject(c->carr.as<double>(), c->poy.as<double>(), 1);
+        gather_u();
+        if (want_sample(c, 0, 0.0)) sample<T>(c, 0.0);
+    }
+    c->initial_done = true;
+    const int gb_ct = tuned_blocks(c, TK_CT, c->sCt, kSpmvCap);
+    const int gb_u = sell_blocks(c->fKu, kSpmvCap), gb_f = sell_blocks(c->fKf, kSpmvCap);
+    for (int64_t k = 0; k < n; ++k) {
+        const int64_t i = first + k;
+        const double t = (double)i * c->dt;
+        const double* cs = c->carr.as<double>() + (k + 1) * ns;
+        double* poy = c->poy.as<double>() + 2 * k * inj_blocks;
+        TF_LAUNCH_TIMED(c, "sell_ct_kernel", c->tune[TK_CT].unroll, sell_ct_kernel, T, gb_ct, c->sCt.signs(),
+                        c->Fv.as<T>(), c->curl_f.as<T>());
+        inject(cs, poy, 0);
+        const int nx = c->fcur ^ 1;
+        if (c->nU > 0)
+            TF_LAUNCH_TIMED(c, "sell_frac_kernel", 8, sell_frac_kernel, T, gb_u, c->fKu.vals<T>(), c->fRu.vals<T>(),
+                            (const T*)c->curl_f.as<T>(), (const T*)c->uf[c->fcur].as<T>(), c->uf[nx].as<T>(), c->dt);
+        launch_timed(c, "frac_gather_kernel", frac_gather_kernel<T>, blocks_for(c->E), c->E, c->mu_a.as<int32_t>(),
+                     c->mu_b.as<int32_t>(), (const T*)c->uf[nx].as<T>(), c->U.as<T>());
+        if (c->n_inj)
+            launch_timed(c, "dga_poynting_kernel", dga_poynting_kernel<T>, inj_blocks, c->n_inj,
+                         c->inj_edge.as<int32_t>(), c->inj_kind.as<int8_t>(), c->inj_tb.as<int32_t>(),
+                         c->inj_te.as<int32_t>(), c->term_src.as<int32_t>(), c->term_w.as<double>(), cs,
+                         (const T*)c->U.as<T>(), poy + inj_blocks);
+        launch_timed(c, "dga_curl_kernel", dga_curl_kernel<T>, blocks_for(c->F), c->F, c->c_col.as<int32_t>(),
+                     c->c_fixed_signs ? nullptr : c->c_sg.as<int8_t>(), (const T*)c->U.as<T>(), c->curl_u.as<T>(),
+                     c->B.as<T>(), c->dt, (const double*)nullptr, (const CgState*)nullptr);
+        if (c->nF > 0)
+            TF_LAUNCH_TIMED(c, "sell_frac_kernel", 8, sell_frac_kernel, T, gb_f, c->fKf.vals<T>(), c->fRf.vals<T>(),
+                            (const T*)c->curl_u.as<T>(), (const T*)c->ff[c->fcur].as<T>(), c->ff[nx].as<T>(), -c->dt);
+        launch_timed(c, "frac_gather_kernel", frac_gather_kernel<T>, blocks_for(c->F), c->F, c->tf_a.as<int32_t>(),
+                     c->tf_b.as<int32_t>(), (const T*)c->ff[nx].as<T>(), c->Fv.as<T>());
+        c->fcur = nx;
+        if (want_sample(c, i, t)) sample<T>(c, t);
+        if (c->profiling && (k & 63) == 63) ev_flush(c);
+    }
+    TF_HIP(hipGetLastError());
+    std::vector<double> part((size_t)2 * n * std::max(inj_blocks, 1), 0.0);
+    if (inj_blocks)
+        TF_HIP(hipMemcpyAsync(part.data(), c->poy.p, 2 * n * inj_blocks * sizeof(double), hipMemcpyDeviceToHost,
+                              c->stream));
+    TF_HIP(hipStreamSynchronize(c->stream));
+    for (int64_t k = 0; k < n; ++k) {
+        double sum = 0;
+        for (int b = 0; b < 2 * inj_blocks; ++b) sum += part[2 * k * inj_blocks + b];
+        c->poynting.push_back(sum);
+    }
+    st->steps_done += n;
+}
+
 template <class T>
 void run_dga(tf_ctx* c, int64_t first, int64_t n, tf_stats* st) {
     const int inj_blocks = c->n_inj ? blocks_for(c->n_inj) : 0;
@@ -1246,6 +1457,10 @@ void run_dga(tf_ctx* c, int64_t first, int64_t n, tf_stats* st) {
     if (inj_blocks && c->poy_cap < 2 * n * inj_blocks) {
         c->poy.alloc(2 * n * inj_blocks * sizeof(double));
         c->poy_cap = 2 * n * inj_blocks;
+    }
+    if (c->frac) {
+        run_dga_frac<T>(c, first, n, st);
+        return;
     }
     if (!c->initial_done && first == 1) {
         // initial export at t = 0 (D:805-806), after the t = 0 e-source values
@@ -1386,11 +1601,16 @@ void cg_start(tf_ctx* c, CgSolve& cs, bool with_rhs, const CgState* guard) {
     T* r0 = cs.learned ? &v.rs[0]->r : v.rj;
     const int r0_stride = cs.learned ? 2 : 1;
     if (with_rhs) {
+        // gfdtd: r0 = (K U - RHS1 SrcFld) + dt Ct F + dt fb; fem: r0 = (E W /
+        // dt^2 - 0.5/dt SigMat W2) - Ct (N C U_old) + fb / dt
+        const double dt = c->dt;
+        const RhsCoef co = c->fem ? RhsCoef{1 / (dt * dt), -0.5 / dt, -1.0, 1 / dt} : RhsCoef{1.0, -1.0, dt, dt};
         TF_LAUNCH_TIMED(c, "gf_rhs_init_kernel", 8, gf_rhs_init_kernel, T, v.nb_b, c->Kf.vals<T>(),
-                        c->R1f.vals<T>(), c->Ctf.signs(), cs.dir_terms ? 1 : 0, c->free_edge.as<int32_t>(),
-                        c->U.as<T>(), c->dir_new.as<T>(), c->Fv.as<T>(), c->fb.as<T>(), c->r0.as<T>(), c->dt,
-                        c->pre.as<T>(), r0, r0_stride, v.u, v.part_rr, v.part_gj[0], dst, cs.learned ? 0 : 1,
-                        guard);
+                        c->R1f.vals<T>(), c->Ctf.signs(), (cs.dir_terms || c->fem) ? 1 : 0,
+                        c->free_edge.as<int32_t>(), c->fem ? c->fw.as<T>() : c->U.as<T>(),
+                        c->fem ? c->fw2.as<T>() : c->dir_new.as<T>(), c->Fv.as<T>(), c->fb.as<T>(),
+                        c->r0.as<T>(), co, c->pre.as<T>(), r0, r0_stride, v.u, v.part_rr, v.part_gj[0], dst,
+                        cs.learned ? 0 : 1, guard);
     } else {
         launch_timed(c, "cg_restart_kernel", cg_restart_kernel<T>, v.nb_b, v.nf, c->pre.as<T>(), c->r0.as<T>(), r0,
                      r0_stride, v.u, v.part_rr, v.part_gj[0], dst, cs.learned ? 0 : 1);
@@ -1404,9 +1624,9 @@ void cg_capture(tf_ctx* c, const CgSolve& cs, const CgView<T>& v, int k) {
     if (!cs.snaps) return;
     for (auto& sn : *cs.snaps)
         if (sn.first == k)
-            bank_capture_kernel<T><<<blocks_for(v.nf), kBlock, 0, c->stream>>>(v.nf, sn.second, v.rj, v.x,
-                                                                                 c->lp.Rb.as<float>(),
-                                                                                 c->lp.Cb.as<float>());
+            bank_capture_kernel<T><<<blocks_for(c->lp.nw), kBlock, 0, c->stream>>>(c->lp.nw, sn.second,
+                                                                                     v.x + c->lp.w0,
+                                                                                     c->lp.Cb.as<float>());
 }
 
 // Launches passes [cs.it, min(cs.it + n, maxiter + 1)) and then an async copy
@@ -1496,7 +1716,202 @@ uint16_t bf16_bits(float f) {
 constexpr int kCgSlots = 48;                      // bank slots for captured pairs
 constexpr int kCaptureIters[4] = {0, 3, 7, 12};   // CG iterations captured per chosen step
 
-// Pair list, features and parameter set-up (host, once per context).
+// Host tables of the learned preconditioner: pair list, standardised
+// features, SELL position -> pair map and Â on A's SELL pattern. Built on
+// host threads while the context's first (collection) solves run on the
+// device; nothing here touches the GPU.
+std::unique_ptr<LearnedTables> learned_tables(const tf_ctx* c, std::vector<double> is, int32_t w0, int32_t w1) {
+    auto T = std::make_unique<LearnedTables>();
+    const int32_t n = c->n_free;
+    const auto& rp = c->hA_ptr;
+    const auto& col = c->hA_col;
+    const auto& val = c->hA_val;
+    const int64_t nnz = (int64_t)col.size();
+    std::atomic<int> bad{0};  // 1 repeated column, 2 unsymmetric pattern
+    // rows keep their CSR entry order (that of the SELL layout); ord lists
+    // each row's positions by increasing column for lookups and merges
+    std::vector<int64_t> ord(nnz);
+    par_chunks(n, 4096, [&](int64_t lo, int64_t hi) {
+        for (int64_t e = lo; e < hi; ++e) {
+            for (int64_t q = rp[e]; q < rp[e + 1]; ++q) ord[q] = q;
+            std::sort(ord.begin() + rp[e], ord.begin() + rp[e + 1],
+                      [&](int64_t a, int64_t b) { return col[a] < col[b]; });
+            for (int64_t q = rp[e] + 1; q < rp[e + 1]; ++q)
+                if (col[ord[q]] == col[ord[q - 1]]) bad = 1;
+        }
+    });
+    if (bad) throw std::runtime_error("A has a repeated column in a row");
+    auto find = [&](int32_t row, int32_t cl) -> int64_t {
+        const int64_t* b = ord.data() + rp[row];
+        const int64_t* en = ord.data() + rp[row + 1];
+        const int64_t* it = std::lower_bound(b, en, cl, [&](int64_t q, int32_t v) { return col[q] < v; });
+        if (it == en || col[*it] != cl) { bad = 2; return rp[row]; }
+        return *it;
+    };
+    auto sell_pos = [&](int32_t row, int64_t q) {
+        return c->hA_sptr[row / kSlice] + (q - rp[row]) * kSlice + row % kSlice;
+    };
+    // pairs: upper triangle incl. diagonal, in CSR order (row-major prefix)
+    std::vector<int64_t> pfirst((size_t)n + 1, 0);
+    par_chunks(n, 65536, [&](int64_t lo, int64_t hi) {
+        for (int64_t e = lo; e < hi; ++e) {
+            int64_t k = 0;
+            for (int64_t q = rp[e]; q < rp[e + 1]; ++q) k += col[q] >= e;
+            pfirst[e + 1] = k;
+        }
+    });
+    for (int32_t e = 0; e < n; ++e) pfirst[e + 1] += pfirst[e];
+    const int64_t np64 = pfirst[n];
+    if (np64 >= INT32_MAX) throw std::runtime_error("A has too many entry pairs for the learned preconditioner");
+    const int32_t np = (int32_t)np64;
+    std::vector<int32_t> pair_of(nnz, -1), pe(np), pj(np);
+    par_chunks(n, 65536, [&](int64_t lo, int64_t hi) {
+        for (int64_t e = lo; e < hi; ++e) {
+            int64_t k = pfirst[e];
+            for (int64_t q = rp[e]; q < rp[e + 1]; ++q)
+                if (col[q] >= e) {
+                    pair_of[q] = (int32_t)k;
+                    pe[k] = (int32_t)e;
+                    pj[k] = col[q];
+                    ++k;
+                }
+        }
+    });
+    par_chunks(n, 65536, [&](int64_t lo, int64_t hi) {
+        for (int64_t e = lo; e < hi; ++e)
+            for (int64_t q = rp[e]; q < rp[e + 1]; ++q)
+                if (col[q] < e) pair_of[q] = pair_of[find(col[q], (int32_t)e)];
+    });
+    if (bad) throw std::runtime_error("A pattern is not symmetric");
+    T->n_pairs = np;
+    T->n_pad = (np + 15) / 16 * 16;
+    T->pair_w0 = pfirst[w0];
+    T->pair_w1 = pfirst[w1];
+    // row statistics of Â (off-diagonal part); means over fixed row chunks
+    // summed in chunk order (independent of the thread count)
+    std::vector<double> rho(n, 0), sig(n, 0), ssum(n, 0), nnzr(n, 0), ld(n, 0);
+    constexpr int64_t kRowChunk = 65536;
+    const int64_t nrc = (n + kRowChunk - 1) / kRowChunk;
+    std::vector<double> cld(nrc, 0), cnz(nrc, 0);
+    par_chunks(n, kRowChunk, [&](int64_t lo, int64_t hi) {
+        double a_ld = 0, a_nz = 0;
+        for (int64_t e = lo; e < hi; ++e) {
+            double d = 0;
+            for (int64_t q = rp[e]; q < rp[e + 1]; ++q) {
+                if (col[q] == e) { d = val[q]; continue; }
+                const double a = val[q] * is[e] * is[col[q]];
+                rho[e] += std::fabs(a);
+                sig[e] += a * a;
+                ssum[e] += a;
+            }
+            nnzr[e] = (double)(rp[e + 1] - rp[e]);
+            ld[e] = std::log(d);
+            a_ld += ld[e];
+            a_nz += nnzr[e];
+        }
+        cld[lo / kRowChunk] = a_ld;
+        cnz[lo / kRowChunk] = a_nz;
+    });
+    double ldm = 0, nm = 0;
+    for (int64_t k = 0; k < nrc; ++k) { ldm += cld[k]; nm += cnz[k]; }
+    ldm /= n;
+    nm /= n;
+    par_chunks(n, kRowChunk, [&](int64_t lo, int64_t hi) {
+        for (int64_t e = lo; e < hi; ++e) { ld[e] -= ldm; nnzr[e] /= nm; }
+    });
+    // features (16 per pair, symmetric in e, j), then standardised
+    std::vector<float> F((size_t)np * kMlpIn);
+    T->base.resize(np);
+    T->scale.resize(np);
+    T->pa.resize(np);
+    T->pb.resize(np);
+    constexpr int64_t kPairChunk = 65536;
+    const int64_t npc = (np + kPairChunk - 1) / kPairChunk;
+    std::vector<double> fsum((size_t)npc * kMlpIn, 0.0);
+    par_chunks(np, kPairChunk, [&](int64_t lo, int64_t hi) {
+        double* acc = &fsum[(size_t)(lo / kPairChunk) * kMlpIn];
+        for (int64_t p = lo; p < hi; ++p) {
+            const int32_t e = pe[p], j = pj[p];
+            const bool dg = e == j;
+            const int64_t qe = find(e, j);
+            const double ah = val[qe] * is[e] * is[j];
+            // (Â^2)_ej: merge of rows e and j
+            double a2 = 0;
+            for (int64_t x = rp[e], y = rp[j]; x < rp[e + 1] && y < rp[j + 1];) {
+                const int64_t qx = ord[x], qy = ord[y];
+                if (col[qx] < col[qy]) ++x;
+                else if (col[qy] < col[qx]) ++y;
+                else {
+                    a2 += val[qx] * is[e] * is[col[qx]] * val[qy] * is[j] * is[col[qy]];
+                    ++x;
+                    ++y;
+                }
+            }
+            const double id = dg ? 1.0 : 0.0;
+            float* f = &F[(size_t)p * kMlpIn];
+            const double mean2[5][2] = {{0.5 * (ld[e] + ld[j]), std::fabs(ld[e] - ld[j])},
+                                        {0.5 * (rho[e] + rho[j]), std::fabs(rho[e] - rho[j])},
+                                        {0.5 * (sig[e] + sig[j]), std::fabs(sig[e] - sig[j])},
+                                        {0.5 * (ssum[e] + ssum[j]), std::fabs(ssum[e] - ssum[j])},
+                                        {0.5 * (nnzr[e] + nnzr[j]), std::fabs(nnzr[e] - nnzr[j])}};
+            f[0] = (float)id;
+            f[1] = (float)(dg ? 0.0 : ah);
+            f[2] = (float)a2;
+            f[3] = (float)(dg ? 0.0 : std::fabs(ah));
+            for (int k = 0; k < 5; ++k) { f[4 + 2 * k] = (float)mean2[k][0]; f[5 + 2 * k] = (float)mean2[k][1]; }
+            f[14] = (float)(id * a2);
+            f[15] = (float)((1 - id) * a2);
+            for (int k = 0; k < kMlpIn; ++k) acc[k] += f[k];
+            T->base[p] = (float)(dg ? 1.0 : -ah);
+            T->scale[p] = (float)(is[e] * is[j]);
+            T->pa[p] = sell_pos(e, qe);
+            T->pb[p] = dg ? -1 : sell_pos(j, find(j, e));
+        }
+    });
+    if (bad) throw std::runtime_error("A pattern is not symmetric");
+    double mu[kMlpIn] = {}, sd[kMlpIn] = {};
+    for (int64_t c2 = 0; c2 < npc; ++c2)
+        for (int k = 0; k < kMlpIn; ++k) mu[k] += fsum[(size_t)c2 * kMlpIn + k];
+    for (int k = 0; k < kMlpIn; ++k) mu[k] /= np;
+    std::fill(fsum.begin(), fsum.end(), 0.0);
+    par_chunks(np, kPairChunk, [&](int64_t lo, int64_t hi) {
+        double* acc = &fsum[(size_t)(lo / kPairChunk) * kMlpIn];
+        for (int64_t p = lo; p < hi; ++p)
+            for (int k = 0; k < kMlpIn; ++k) {
+                const double t = F[(size_t)p * kMlpIn + k] - mu[k];
+                acc[k] += t * t;
+            }
+    });
+    for (int64_t c2 = 0; c2 < npc; ++c2)
+        for (int k = 0; k < kMlpIn; ++k) sd[k] += fsum[(size_t)c2 * kMlpIn + k];
+    for (int k = 0; k < kMlpIn; ++k) {
+        sd[k] = std::sqrt(sd[k] / np);
+        if (!(sd[k] > 1e-12)) sd[k] = 1;
+    }
+    T->X.assign((size_t)T->n_pad * kMlpIn, 0);
+    par_chunks(np, kPairChunk, [&](int64_t lo, int64_t hi) {
+        for (int64_t p = lo; p < hi; ++p)
+            for (int k = 0; k < kMlpIn; ++k)
+                T->X[(size_t)p * kMlpIn + k] = bf16_bits((float)((F[(size_t)p * kMlpIn + k] - mu[k]) / sd[k]));
+    });
+    // SELL position -> pair, and Â on the SELL pattern
+    const int64_t entries = c->A.entries;
+    T->pos2pair.assign(entries, -1);
+    T->ahat.assign(entries, 0.f);
+    par_chunks(n, 65536, [&](int64_t lo, int64_t hi) {
+        for (int64_t e = lo; e < hi; ++e)
+            for (int64_t q = rp[e]; q < rp[e + 1]; ++q) {
+                const int64_t o = sell_pos((int32_t)e, q);
+                T->pos2pair[o] = pair_of[q];
+                T->ahat[o] = (float)(val[q] * is[e] * is[col[q]]);
+            }
+    });
+    return T;
+}
+
+// Parameter and bank set-up (once per context). The host tables of
+// learned_tables() are built on host threads in the background and
+// uploaded by learned_train() once the collection solves are done.
 void learned_setup(tf_ctx* c, const tf_cg_opts* cg) {
     LearnedPc& L = c->lp;
     const int32_t n = c->n_free;
@@ -1504,151 +1919,25 @@ void learned_setup(tf_ctx* c, const tf_cg_opts* cg) {
     const auto& rp = c->hA_ptr;
     const auto& col = c->hA_col;
     const auto& val = c->hA_val;
-    std::vector<double> d(n, 0.0);
-    for (int32_t e = 0; e < n; ++e) {
-        for (int64_t q = rp[e]; q < rp[e + 1]; ++q)
-            if (col[q] == e) d[e] = val[q];
-        if (!(d[e] > 0)) throw std::runtime_error("A has a non-positive diagonal entry");
-    }
-    std::vector<double> is(n);
-    for (int32_t e = 0; e < n; ++e) is[e] = 1.0 / std::sqrt(d[e]);
-    // rows keep their CSR entry order (that of the SELL layout); ord lists
-    // each row's positions by increasing column for lookups and merges
-    std::vector<int64_t> ord(col.size());
-    for (int32_t e = 0; e < n; ++e) {
-        for (int64_t q = rp[e]; q < rp[e + 1]; ++q) ord[q] = q;
-        std::sort(ord.begin() + rp[e], ord.begin() + rp[e + 1], [&](int64_t a, int64_t b) { return col[a] < col[b]; });
-        for (int64_t q = rp[e] + 1; q < rp[e + 1]; ++q)
-            if (col[ord[q]] == col[ord[q - 1]]) throw std::runtime_error("A has a repeated column in a row");
-    }
-    auto find = [&](int32_t row, int32_t cl) -> int64_t {
-        const int64_t* b = ord.data() + rp[row];
-        const int64_t* en = ord.data() + rp[row + 1];
-        const int64_t* it = std::lower_bound(b, en, cl, [&](int64_t q, int32_t v) { return col[q] < v; });
-        if (it == en || col[*it] != cl) throw std::runtime_error("A pattern is not symmetric");
-        return *it;
-    };
-    auto sell_pos = [&](int32_t row, int64_t q) {
-        return c->hA_sptr[row / kSlice] + (q - rp[row]) * kSlice + row % kSlice;
-    };
-    // pairs: upper triangle incl. diagonal, in CSR order
-    std::vector<int32_t> pair_of(col.size(), -1), pe, pj;
-    for (int32_t e = 0; e < n; ++e)
-        for (int64_t q = rp[e]; q < rp[e + 1]; ++q)
-            if (col[q] >= e) {
-                pair_of[q] = (int32_t)pe.size();
-                pe.push_back(e);
-                pj.push_back(col[q]);
-            }
-    for (int32_t e = 0; e < n; ++e)
-        for (int64_t q = rp[e]; q < rp[e + 1]; ++q)
-            if (col[q] < e) pair_of[q] = pair_of[find(col[q], e)];
-    const int32_t np = (int32_t)pe.size();
-    L.n_pairs = np;
-    L.n_pad = (np + 15) / 16 * 16;
-    // row statistics of Â (off-diagonal part)
-    std::vector<double> rho(n, 0), sig(n, 0), ssum(n, 0), nnz(n, 0), ld(n, 0);
-    double ldm = 0, nm = 0;
-    for (int32_t e = 0; e < n; ++e) {
-        for (int64_t q = rp[e]; q < rp[e + 1]; ++q) {
-            if (col[q] == e) continue;
-            const double a = val[q] * is[e] * is[col[q]];
-            rho[e] += std::fabs(a);
-            sig[e] += a * a;
-            ssum[e] += a;
+    std::vector<double> is(n, 0.0);
+    std::atomic<int> bad{0};
+    par_chunks(n, 65536, [&](int64_t lo, int64_t hi) {
+        for (int64_t e = lo; e < hi; ++e) {
+            double d = 0;
+            for (int64_t q = rp[e]; q < rp[e + 1]; ++q)
+                if (col[q] == e) d = val[q];
+            if (!(d > 0)) bad = 1;
+            is[e] = 1.0 / std::sqrt(d);
         }
-        nnz[e] = (double)(rp[e + 1] - rp[e]);
-        ld[e] = std::log(d[e]);
-        ldm += ld[e];
-        nm += nnz[e];
-    }
-    ldm /= n;
-    nm /= n;
-    for (int32_t e = 0; e < n; ++e) { ld[e] -= ldm; nnz[e] /= nm; }
-    // features (16 per pair, symmetric in e, j), then standardised
-    std::vector<float> F((size_t)np * kMlpIn);
-    std::vector<float> base(np), scale(np);
-    std::vector<int64_t> pa(np), pb(np);
-    for (int32_t p = 0; p < np; ++p) {
-        const int32_t e = pe[p], j = pj[p];
-        const bool dg = e == j;
-        const int64_t qe = find(e, j);
-        const double ah = val[qe] * is[e] * is[j];
-        // (Â^2)_ej: merge of rows e and j
-        double a2 = 0;
-        for (int64_t x = rp[e], y = rp[j]; x < rp[e + 1] && y < rp[j + 1];) {
-            const int64_t qx = ord[x], qy = ord[y];
-            if (col[qx] < col[qy]) ++x;
-            else if (col[qy] < col[qx]) ++y;
-            else {
-                a2 += val[qx] * is[e] * is[col[qx]] * val[qy] * is[j] * is[col[qy]];
-                ++x;
-                ++y;
-            }
-        }
-        const double id = dg ? 1.0 : 0.0;
-        float* f = &F[(size_t)p * kMlpIn];
-        const double mean2[5][2] = {{0.5 * (ld[e] + ld[j]), std::fabs(ld[e] - ld[j])},
-                                    {0.5 * (rho[e] + rho[j]), std::fabs(rho[e] - rho[j])},
-                                    {0.5 * (sig[e] + sig[j]), std::fabs(sig[e] - sig[j])},
-                                    {0.5 * (ssum[e] + ssum[j]), std::fabs(ssum[e] - ssum[j])},
-                                    {0.5 * (nnz[e] + nnz[j]), std::fabs(nnz[e] - nnz[j])}};
-        f[0] = (float)id;
-        f[1] = (float)(dg ? 0.0 : ah);
-        f[2] = (float)a2;
-        f[3] = (float)(dg ? 0.0 : std::fabs(ah));
-        for (int k = 0; k < 5; ++k) { f[4 + 2 * k] = (float)mean2[k][0]; f[5 + 2 * k] = (float)mean2[k][1]; }
-        f[14] = (float)(id * a2);
-        f[15] = (float)((1 - id) * a2);
-        base[p] = (float)(dg ? 1.0 : -ah);
-        scale[p] = (float)(is[e] * is[j]);
-        pa[p] = sell_pos(e, qe);
-        pb[p] = dg ? -1 : sell_pos(j, find(j, e));
-    }
-    std::vector<uint16_t> X((size_t)L.n_pad * kMlpIn, 0);
-    for (int k = 0; k < kMlpIn; ++k) {
-        double mu = 0, m2 = 0;
-        for (int32_t p = 0; p < np; ++p) mu += F[(size_t)p * kMlpIn + k];
-        mu /= np;
-        for (int32_t p = 0; p < np; ++p) {
-            const double t = F[(size_t)p * kMlpIn + k] - mu;
-            m2 += t * t;
-        }
-        double sd = std::sqrt(m2 / np);
-        if (!(sd > 1e-12)) sd = 1;
-        for (int32_t p = 0; p < np; ++p)
-            X[(size_t)p * kMlpIn + k] = bf16_bits((float)((F[(size_t)p * kMlpIn + k] - mu) / sd));
-    }
-    // SELL position -> pair, and Â on the SELL pattern
-    const int64_t entries = c->A.entries;
-    std::vector<int32_t> pos2pair(entries, -1);
-    std::vector<float> ahat(entries, 0.f);
-    for (int32_t e = 0; e < n; ++e)
-        for (int64_t q = rp[e]; q < rp[e + 1]; ++q) {
-            const int64_t o = sell_pos(e, q);
-            pos2pair[o] = pair_of[q];
-            ahat[o] = (float)(val[q] * is[e] * is[col[q]]);
-        }
+    });
+    if (bad) throw std::runtime_error("A has a non-positive diagonal entry");
+    // training window: rows [w0, w0 + nw) of the free-row order (whole
+    // 64-row slices; the middle of the mesh when it has more rows)
+    L.nw = n <= kTrainRows ? n : kTrainRows;
+    L.w0 = n <= kTrainRows ? 0 : (n - L.nw) / 2 / kSlice * kSlice;
     std::vector<float> isd(n);
     for (int32_t e = 0; e < n; ++e) isd[e] = (float)is[e];
-    upload(L.X, X);
-    upload(L.base, base);
-    upload(L.scale, scale);
-    upload(L.pa, pa);
-    upload(L.pb, pb);
-    upload(L.pos2pair, pos2pair);
-    upload(L.ahat, ahat);
     upload(L.isd, isd);
-    L.y.alloc((size_t)L.n_pad * 4);
-    TF_HIP(hipMemsetAsync(L.y.p, 0, L.y.bytes, c->stream));
-    L.gy.alloc((size_t)L.n_pad * 4);
-    TF_HIP(hipMemsetAsync(L.gy.p, 0, L.gy.bytes, c->stream));
-    L.ghat.alloc(entries * 4);
-    // bf16 values: packed with A's 16-bit column offsets (cg_pg_kernel reads
-    // one word per entry) when A has them, else beside A's int32 columns
-    if (c->A.c16) L.gpk.alloc(entries * 4);
-    else L.gval.alloc(entries * 2);
-    L.gpos.alloc(entries * 4);
     // parameters: He-uniform W1, W2 from a fixed LCG, zero biases, zero output layer
     std::vector<float> P(kMlpParams, 0.f);
     uint64_t st = 0x243F6A8885A308D3ULL;
@@ -1667,7 +1956,7 @@ void learned_setup(tf_ctx* c, const tf_cg_opts* cg) {
     L.part.alloc((size_t)nbb * kMlpParams * 4);
     L.lpart.alloc(kSpmvCap * sizeof(double));
     L.cpart.alloc((size_t)kSpmvCap * kBank * sizeof(double));
-    const size_t bank = (size_t)n * kBank * 4;
+    const size_t bank = (size_t)L.nw * kBank * 4;
     for (DevBuf* b : {&L.Rb, &L.Cb, &L.T1, &L.T2}) {
         b->alloc(bank);
         TF_HIP(hipMemsetAsync(b->p, 0, bank, c->stream));
@@ -1679,32 +1968,72 @@ void learned_setup(tf_ctx* c, const tf_cg_opts* cg) {
     L.state = LearnedPc::kCollecting;
     L.steps = 0;
     TF_HIP(hipStreamSynchronize(c->stream));
+    const int32_t w0 = L.w0, w1 = L.w0 + L.nw;
+    L.tables = std::async(std::launch::async, [c, is = std::move(is), w0, w1]() mutable {
+        return learned_tables(c, std::move(is), w0, w1);
+    });
+}
+
+// Waits for the host tables and uploads them (before the training).
+void learned_upload_tables(tf_ctx* c) {
+    LearnedPc& L = c->lp;
+    std::unique_ptr<LearnedTables> T = L.tables.get();
+    L.n_pairs = T->n_pairs;
+    L.n_pad = T->n_pad;
+    L.pair_w0 = T->pair_w0;
+    L.pair_w1 = T->pair_w1;
+    upload(L.X, T->X);
+    upload(L.base, T->base);
+    upload(L.scale, T->scale);
+    upload(L.pa, T->pa);
+    upload(L.pb, T->pb);
+    upload(L.pos2pair, T->pos2pair);
+    upload(L.ahat, T->ahat);
+    T.reset();
+    L.y.alloc((size_t)L.n_pad * 4);
+    TF_HIP(hipMemsetAsync(L.y.p, 0, L.y.bytes, c->stream));
+    L.gy.alloc((size_t)L.n_pad * 4);
+    TF_HIP(hipMemsetAsync(L.gy.p, 0, L.gy.bytes, c->stream));
+    const int64_t entries = c->A.entries;
+    L.ghat.alloc(entries * 4);
+    // positions outside the training window keep a zero gradient
+    L.gpos.alloc(entries * 4);
+    TF_HIP(hipMemsetAsync(L.gpos.p, 0, L.gpos.bytes, c->stream));
+    // bf16 values: packed with A's 16-bit column offsets (cg_pg_kernel reads
+    // one word per entry) when A has them, else beside A's int32 columns
+    if (c->A.c16) L.gpk.alloc(entries * 4);
+    else L.gval.alloc(entries * 2);
 }
 
 int row_blocks(int32_t n) { return std::max(1, std::min(kSpmvCap, (n + kBlock / kWave - 1) / (kBlock / kWave))); }
 
 // Trains the MLP on the bank (Adam, train_iters steps), keeps the parameters
-// of the lowest loss and writes G's values for CG.
+// of the lowest loss and writes G's values for CG. The loss is over the
+// training window's rows: Â and Ĝ restricted to the window (columns outside
+// it dropped), each sample's residual r~ = Â c~ formed from its correction,
+// so only the pairs of the window's rows run through the MLP while training.
 void learned_train(tf_ctx* c) {
     LearnedPc& L = c->lp;
-    const int32_t n = c->n_free;
     const auto t0 = std::chrono::steady_clock::now();
+    learned_upload_tables(c);
+    const int32_t nw = L.nw, w0 = L.w0;
     hipStream_t s = c->stream;
     const int64_t* sptr = c->A.sptr.as<int64_t>();
     const int32_t* col = c->A.col.as<int32_t>();
-    const int rb = row_blocks(n);
-    const int64_t nb_elems = (int64_t)n * kBank;
-    // free slots become random probes: c~ = unit-variance noise, r~ = Â c~
+    const int rb = row_blocks(nw);
+    const int64_t nb_elems = (int64_t)nw * kBank;
+    // free slots become random probes c~ = unit-variance noise; every slot's
+    // residual is r~ = Â c~ on the window
     uint64_t mask = 0;
     for (int k = 0; k < kBank; ++k)
         if (L.slot[k] == 0) { mask |= 1ULL << k; L.slot[k] = 2; }
     const int eb = (int)std::min<int64_t>((nb_elems + kBlock - 1) / kBlock, INT32_MAX);
-    if (mask) bank_random_kernel<<<eb, kBlock, 0, s>>>(n, mask, 0x5EEDULL, L.Cb.as<float>());
-    bank_spmm_kernel<<<rb, kBlock, 0, s>>>(n, sptr, col, L.ahat.as<float>(), L.Cb.as<float>(), nullptr,
+    if (mask) bank_random_kernel<<<eb, kBlock, 0, s>>>(nw, mask, 0x5EEDULL, L.Cb.as<float>());
+    bank_spmm_kernel<<<rb, kBlock, 0, s>>>(nw, w0, sptr, col, L.ahat.as<float>(), L.Cb.as<float>(), nullptr,
                                            L.T1.as<float>());
-    if (mask) bank_copy_cols_kernel<<<eb, kBlock, 0, s>>>(n, mask, L.T1.as<float>(), L.Rb.as<float>());
+    bank_copy_cols_kernel<<<eb, kBlock, 0, s>>>(nw, ~0ULL, L.T1.as<float>(), L.Rb.as<float>());
     // weights: w_s = 1 / (S c~_s^T Â c~_s)
-    bank_coldot_kernel<<<rb, kBlock, 0, s>>>(n, L.Cb.as<float>(), L.T1.as<float>(), L.cpart.as<double>());
+    bank_coldot_kernel<<<rb, kBlock, 0, s>>>(nw, L.Cb.as<float>(), L.T1.as<float>(), L.cpart.as<double>());
     std::vector<double> cp((size_t)rb * kBank);
     TF_HIP(hipMemcpyAsync(cp.data(), L.cpart.p, cp.size() * 8, hipMemcpyDeviceToHost, s));
     TF_HIP(hipStreamSynchronize(s));
@@ -1716,25 +2045,35 @@ void learned_train(tf_ctx* c) {
     }
     DevBuf wts;
     upload(wts, w);
+    // the window's pairs (whole 16-pair tiles) and SELL positions
+    const int64_t tile0 = L.pair_w0 / 16, tile1 = (L.pair_w1 + 15) / 16;
+    const int64_t wtiles = std::max<int64_t>(tile1 - tile0, 0);
     const int64_t tiles = L.n_pad / 16;
-    const int nbb = (int)std::min<int64_t>(512, std::max<int64_t>(1, (tiles + 3) / 4));
-    const int fb = (int)std::min<int64_t>(4096, std::max<int64_t>(1, (tiles + 3) / 4));
+    auto grid_of = [](int64_t t, int64_t cap) { return (int)std::min<int64_t>(cap, std::max<int64_t>(1, (t + 3) / 4)); };
+    const int nbb = grid_of(wtiles, 512);
+    const int64_t pos0 = c->hA_sptr[w0 / kSlice];
+    const int64_t pos1 = c->hA_sptr[(w0 + nw + kSlice - 1) / kSlice];
     const int64_t entries = c->A.entries;
-    const int xb = (int)((entries + kBlock - 1) / kBlock);
-    const int pb = blocks_for(L.n_pairs);
+    const int xbw = (int)std::max<int64_t>(1, (pos1 - pos0 + kBlock - 1) / kBlock);
+    const int64_t wpairs = L.pair_w1 - L.pair_w0;
+    const int pb = blocks_for(std::max<int64_t>(wpairs, 1));
+    const short* Xw = L.X.as<short>() + tile0 * 16 * kMlpIn;
+    float* yw = L.y.as<float>() + tile0 * 16;
+    float* gyw = L.gy.as<float>() + tile0 * 16;
     std::vector<double> lsum(rb);
     double best = INFINITY;
     for (int it = 0; it <= L.train_iters; ++it) {
-        mlp_forward_kernel<<<fb, kBlock, 0, s>>>(tiles, L.P.as<float>(), L.X.as<short>(), L.y.as<float>());
-        expand_kernel<<<xb, kBlock, 0, s>>>(entries, L.pos2pair.as<int32_t>(), L.base.as<float>(),
-                                            L.scale.as<float>(), L.y.as<float>(), L.ghat.as<float>(), nullptr,
-                                            nullptr, nullptr);
-        bank_spmm_kernel<<<rb, kBlock, 0, s>>>(n, sptr, col, L.ghat.as<float>(), L.Rb.as<float>(),
+        if (wtiles) mlp_forward_kernel<<<grid_of(wtiles, 4096), kBlock, 0, s>>>(wtiles, L.P.as<float>(), Xw, yw);
+        expand_kernel<<<xbw, kBlock, 0, s>>>(pos1 - pos0, L.pos2pair.as<int32_t>() + pos0, L.base.as<float>(),
+                                             L.scale.as<float>(), L.y.as<float>(), L.ghat.as<float>() + pos0,
+                                             nullptr, nullptr, nullptr);
+        bank_spmm_kernel<<<rb, kBlock, 0, s>>>(nw, w0, sptr, col, L.ghat.as<float>(), L.Rb.as<float>(),
                                                L.Cb.as<float>(), L.T1.as<float>());
-        bank_spmm_kernel<<<rb, kBlock, 0, s>>>(n, sptr, col, L.ahat.as<float>(), L.T1.as<float>(), nullptr,
+        bank_spmm_kernel<<<rb, kBlock, 0, s>>>(nw, w0, sptr, col, L.ahat.as<float>(), L.T1.as<float>(), nullptr,
                                                L.T2.as<float>());
-        bank_grad_kernel<<<rb, kBlock, 0, s>>>(n, sptr, col, L.Rb.as<float>(), L.T1.as<float>(), L.T2.as<float>(),
-                                               wts.as<float>(), 2.f, L.gpos.as<float>(), L.lpart.as<double>());
+        bank_grad_kernel<<<rb, kBlock, 0, s>>>(nw, w0, sptr, col, L.Rb.as<float>(), L.T1.as<float>(),
+                                               L.T2.as<float>(), wts.as<float>(), 2.f, L.gpos.as<float>(),
+                                               L.lpart.as<double>());
         TF_HIP(hipMemcpyAsync(lsum.data(), L.lpart.p, rb * 8, hipMemcpyDeviceToHost, s));
         TF_HIP(hipStreamSynchronize(s));
         double loss = 0;
@@ -1745,22 +2084,29 @@ void learned_train(tf_ctx* c) {
             TF_HIP(hipMemcpyAsync(L.Pbest.p, L.P.p, kMlpParams * 4, hipMemcpyDeviceToDevice, s));
         }
         if (it == L.train_iters) break;
-        pair_grad_kernel<<<pb, kBlock, 0, s>>>(L.n_pairs, L.pa.as<int64_t>(), L.pb.as<int64_t>(),
-                                               L.gpos.as<float>(), L.gy.as<float>());
-        mlp_backward_kernel<<<nbb, kBlock, 0, s>>>(tiles, L.P.as<float>(), L.X.as<short>(), L.gy.as<float>(),
-                                                   L.part.as<float>());
+        if (wpairs > 0)
+            pair_grad_kernel<<<pb, kBlock, 0, s>>>((int32_t)wpairs, L.pa.as<int64_t>() + L.pair_w0,
+                                                   L.pb.as<int64_t>() + L.pair_w0, L.gpos.as<float>(),
+                                                   L.gy.as<float>() + L.pair_w0);
+        if (wtiles)
+            mlp_backward_kernel<<<nbb, kBlock, 0, s>>>(wtiles, L.P.as<float>(), Xw, gyw, L.part.as<float>());
         mlp_adam_kernel<<<blocks_for(kMlpParams), kBlock, 0, s>>>(nbb, L.part.as<float>(), L.P.as<float>(),
                                                                   L.m.as<float>(), L.v.as<float>(), nullptr,
                                                                   (float)L.lr, 0.9f, 0.999f, 1e-8f, it + 1, 1.f);
     }
     L.loss = best;
-    // frozen G from the best parameters
-    mlp_forward_kernel<<<fb, kBlock, 0, s>>>(tiles, L.Pbest.as<float>(), L.X.as<short>(), L.y.as<float>());
+    // frozen G from the best parameters, over every pair
+    mlp_forward_kernel<<<grid_of(tiles, 4096), kBlock, 0, s>>>(tiles, L.Pbest.as<float>(), L.X.as<short>(),
+                                                               L.y.as<float>());
+    const int xb = (int)((entries + kBlock - 1) / kBlock);
     expand_kernel<<<xb, kBlock, 0, s>>>(entries, L.pos2pair.as<int32_t>(), L.base.as<float>(), L.scale.as<float>(),
                                         L.y.as<float>(), nullptr, c->A.c16 ? nullptr : L.gval.as<uint16_t>(),
                                         c->A.col16.as<uint16_t>(), c->A.c16 ? L.gpk.as<uint32_t>() : nullptr);
     TF_HIP(hipGetLastError());
     TF_HIP(hipStreamSynchronize(s));
+    // the training-only buffers are not needed once G is frozen
+    for (DevBuf* b : {&L.Rb, &L.Cb, &L.T1, &L.T2, &L.ghat, &L.gpos, &L.X, &L.ahat, &L.pos2pair})
+        b->release();
     L.train_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     L.state = LearnedPc::kActive;
 }
@@ -1831,8 +2177,9 @@ void run_gfdtd(tf_ctx* c, int64_t first, int64_t n, const tf_cg_opts* cg, tf_sta
         launch_timed(c, "dga_curl_kernel", dga_curl_kernel<T>, blocks_for(c->F), c->F, c->c_col.as<int32_t>(),
                      c->c_fixed_signs ? nullptr : c->c_sg.as<int8_t>(), c->U.as<T>(), c->curl_u.as<T>(),
                      c->B.as<T>(), c->dt, (const double*)nullptr, guard);
-        TF_LAUNCH_TIMED(c, "sell_n_kernel", c->tune[TK_N].unroll, sell_n_kernel, T, gb_n, c->N.vals<T>(), c->B.as<T>(),
-                     c->Fv.as<T>(), c->dt, 1, nullptr, guard);
+        // gfdtd: F = N B (D:1468); fem: N C U_old for the curl-curl term (D:1729)
+        TF_LAUNCH_TIMED(c, "sell_n_kernel", c->tune[TK_N].unroll, sell_n_kernel, T, gb_n, c->N.vals<T>(),
+                        c->fem ? c->curl_u.as<T>() : c->B.as<T>(), c->Fv.as<T>(), c->dt, 1, nullptr, guard);
         if (fresh) g.sample_slot = allow_print(c, t - c->dt) ? sample_reserve(c, t - c->dt) : -1;
         sample_launch<T>(c, g.sample_slot, guard);
         if (lossy) {
@@ -1844,8 +2191,13 @@ void run_gfdtd(tf_ctx* c, int64_t first, int64_t n, const tf_cg_opts* cg, tf_sta
             launch_timed(c, "gf_inject_kernel", gf_inject_kernel<T>, inj_blocks, 
                 c->n_inj, c->inj_edge.as<int32_t>(), c->inj_kind.as<int8_t>(), c->inj_tb.as<int32_t>(),
                 c->inj_te.as<int32_t>(), c->term_src.as<int32_t>(), c->term_w.as<double>(), cs, i > 1 ? cp : nullptr,
-                c->fb.as<T>(), c->dir_new.as<T>(), c->U.as<T>(), c->poy.as<double>() + k * inj_blocks, guard);
+                c->fb.as<T>(), c->dir_new.as<T>(), c->U.as<T>(), c->poy.as<double>() + k * inj_blocks, guard,
+                c->fem ? 1 : 0);
         }
+        if (c->fem)
+            launch_timed(c, "fem_w_kernel", fem_w_kernel<T>, blocks_for(nf + c->n_dir), nf, c->free_edge.as<int32_t>(),
+                         c->n_dir, c->dir_edge.as<int32_t>(), (const T*)c->U.as<T>(), (const T*)c->U_older.as<T>(),
+                         (const T*)c->dir_new.as<T>(), c->fw.as<T>(), c->fw2.as<T>(), guard);
         CgSolve s0;
         s0.init = init;
         s0.learned = g.learned;
@@ -1856,7 +2208,7 @@ void run_gfdtd(tf_ctx* c, int64_t first, int64_t n, const tf_cg_opts* cg, tf_sta
     auto scatter = [&](const CgState* guard) {
         launch_timed(c, "gf_scatter_kernel", gf_scatter_kernel<T>, blocks_for(nf + c->n_dir), nf,
                      c->free_edge.as<int32_t>(), c->x.as<T>(), c->U.as<T>(), c->n_dir, c->dir_edge.as<int32_t>(),
-                     c->dir_new.as<T>(), guard);
+                     c->dir_new.as<T>(), guard, c->fem ? c->U_older.as<T>() : nullptr);
     };
     // collection captures of the solve of the current step
     auto collection_snaps = [&]() {
@@ -1880,11 +2232,12 @@ void run_gfdtd(tf_ctx* c, int64_t first, int64_t n, const tf_cg_opts* cg, tf_sta
             gf_inject_kernel<T><<<inj_blocks, kBlock, 0, c->stream>>>(
                 c->n_inj, c->inj_edge.as<int32_t>(), c->inj_kind.as<int8_t>(), c->inj_tb.as<int32_t>(),
                 c->inj_te.as<int32_t>(), c->term_src.as<int32_t>(), c->term_w.as<double>(), c->carr.as<double>(),
-                nullptr, c->fb.as<T>(), c->dir_new.as<T>(), c->U.as<T>(), c->poy.as<double>(), nullptr);
+                nullptr, c->fb.as<T>(), c->dir_new.as<T>(), c->U.as<T>(), c->poy.as<double>(), nullptr, 0);
+        // fem keeps U before this pass as U_older (D:1682-1683)
         if (c->n_dir > 0)
             gf_scatter_kernel<T><<<blocks_for(c->n_dir), kBlock, 0, c->stream>>>(
                 0, c->free_edge.as<int32_t>(), c->x.as<T>(), c->U.as<T>(), c->n_dir, c->dir_edge.as<int32_t>(),
-                c->dir_new.as<T>(), nullptr);
+                c->dir_new.as<T>(), nullptr, c->fem ? c->U_older.as<T>() : nullptr);
         c->ud_zero = c->dir_zero_static;
     }
     GfPre cur;
@@ -1956,8 +2309,8 @@ void run_gfdtd(tf_ctx* c, int64_t first, int64_t n, const tf_cg_opts* cg, tf_sta
             if (ok)
                 for (auto& sn : snaps)
                     if (sn.first < h.iter) {
-                        bank_finalize_kernel<T><<<blocks_for(nf), kBlock, 0, c->stream>>>(
-                            nf, sn.second, c->x.as<T>(), L.isd.as<float>(), L.Rb.as<float>(), L.Cb.as<float>());
+                        bank_finalize_kernel<T><<<blocks_for(L.nw), kBlock, 0, c->stream>>>(
+                            L.nw, sn.second, c->x.as<T>() + L.w0, L.isd.as<float>() + L.w0, L.Cb.as<float>());
                         L.slot[sn.second] = 1;
                     }
             if (++L.steps >= L.learn_steps) learned_train(c);
@@ -2032,7 +2385,7 @@ tf_status guarded(F f) {
 }
 
 std::unique_ptr<tf_ctx> make_ctx(const tf_problem* p, int32_t prec) {
-    if (p->method < 0 || p->method > 2) throw std::runtime_error("unknown method");
+    if (p->method < 0 || p->method > 3) throw std::runtime_error("unknown method");
     if (prec != TF_PREC_FP64 && prec != TF_PREC_FP32) throw std::runtime_error("unknown precision");
     if (p->n_edges <= 0 || p->n_faces <= 0) throw std::runtime_error("empty problem");
     int ndev = 0;
@@ -2124,6 +2477,7 @@ void merge_dga(const tf_problem* const* probs, int32_t n, MergedDga& m) {
         const tf_problem* p = probs[k];
         if (!p) throw std::runtime_error("null problem in batch");
         if (p->method != TF_METHOD_DGA) throw std::runtime_error("batches hold method dga problems");
+        if (p->dga.frac.enabled) throw std::runtime_error("batches hold lossless dga problems only");
         if (p->n_edges <= 0 || p->n_faces <= 0) throw std::runtime_error("empty problem in batch");
         if (!(p->dt > 0)) throw std::runtime_error("batch instance without a time step");
         const tf_dga_ops& d = p->dga;
@@ -2279,6 +2633,7 @@ void autotune(tf_ctx* c) {
         c->tune[TK_CT] = tune_one(c, TK_CT, c->sCt, kSpmvCap, [&](int b, int u) {
             TF_LAUNCH_NB(c->wide, u, sell_ct_kernel, T, b, s, c->sCt.signs(), c->Fv.as<T>(), c->curl_f.as<T>());
         });
+        if (c->frac) return;  // lossy media: only sell_ct of the tuned kernels runs
         c->tune[TK_H] = tune_one(c, TK_H, c->H, kSpmvCap, [&](int b, int u) {
             TF_LAUNCH_NB(c->wide, u, sell_h_kernel, T, b, s, c->H.vals<T>(), c->h_edge.as<int32_t>(), c->curl_f.as<T>(),
                          c->U.as<T>(), 0.0, nullptr);
@@ -2287,7 +2642,7 @@ void autotune(tf_ctx* c) {
             TF_LAUNCH_NB(c->wide, u, sell_n_kernel, T, b, s, c->N.vals<T>(), c->curl_u.as<T>(), c->Fv.as<T>(), 0.0, 0, nullptr,
                          nullptr);
         });
-    } else if (c->method == TF_METHOD_GFDTD) {
+    } else if (c->method == TF_METHOD_GFDTD || c->method == TF_METHOD_FEM) {
         c->tune[TK_N] = tune_one(c, TK_N, c->N, kSpmvCap, [&](int b, int u) {
             TF_LAUNCH_NB(c->wide, u, sell_n_kernel, T, b, s, c->N.vals<T>(), c->B.as<T>(), c->Fv.as<T>(), c->dt, 1, nullptr,
                          nullptr);
@@ -2523,7 +2878,12 @@ tf_status tf_read_state(tf_ctx* c, double* U, double* F, double* B) {
             rd_perm(c->B, B, c->F, c->h_fsid);
         } else {
             rd(c->U, U, c->E);
-            rd(c->Fv, F, c->F);
+            // fem has no F field (its buffer holds N C U_old of the last step)
+            if (c->fem) {
+                if (F) std::fill(F, F + c->F, 0.0);
+            } else {
+                rd(c->Fv, F, c->F);
+            }
             rd(c->B, B, c->F);
         }
     });
@@ -2562,9 +2922,28 @@ tf_status tf_write_state(tf_ctx* c, const double* U, const double* F, const doub
             for (int32_t e : c->h_dir_edge)
                 if (U[e] != 0) c->ud_zero = false;
         }
-        // gfdtd: the next solve's warm start is the written field on the free
-        // edges, as after every step (the solution vector is U's free part)
-        if (U && c->method == TF_METHOD_GFDTD && c->n_free > 0) {
+        // lossy dga: each edge's (face's) value split evenly over its
+        // fractional entries, the entries no edge (face) reads set to zero
+        if (c->frac) {
+            auto split = [&](DevBuf& dst, int32_t nx, const double* src, int32_t n, const std::vector<int32_t>& a,
+                             const std::vector<int32_t>& b) {
+                if (!src) return;
+                std::vector<double> v(std::max(nx, 1), 0.0);
+                for (int32_t i = 0; i < n; ++i) {
+                    const double w = b[i] >= 0 ? 0.5 * src[i] : src[i];
+                    v[a[i]] = w;
+                    if (b[i] >= 0) v[b[i]] = w;
+                }
+                wr(dst, v.data(), (int64_t)v.size());
+            };
+            split(c->uf[c->fcur], c->nU, U, c->E, c->h_mu_a, c->h_mu_b);
+            split(c->ff[c->fcur], c->nF, F, c->F, c->h_tf_a, c->h_tf_b);
+        }
+        // gfdtd and fem: the next solve's warm start is the written field on
+        // the free edges, as after every step (the solution vector is U's
+        // free part); fem also takes it as the step before (U_older)
+        if (U && c->fem) TF_HIP(hipMemcpy(c->U_older.p, c->U.p, c->U.bytes, hipMemcpyDeviceToDevice));
+        if (U && (c->method == TF_METHOD_GFDTD || c->method == TF_METHOD_FEM) && c->n_free > 0) {
             if (c->prec == TF_PREC_FP64)
                 gather_free_kernel<double><<<blocks_for(c->n_free), kBlock, 0, c->stream>>>(
                     c->n_free, c->free_edge.as<int32_t>(), c->U.as<double>(), c->x.as<double>());
@@ -2607,7 +2986,7 @@ tf_status tf_read_losses(tf_ctx* c, double* vals, int64_t cap, int64_t* n) {
     // dga here; the FIT Joule weights exist only in l2norm mode, D:5593-5597)
     *n = (int64_t)c->poynting.size();
     std::vector<double> col;
-    if (c->method == TF_METHOD_GFDTD) {
+    if (c->method == TF_METHOD_GFDTD || c->method == TF_METHOD_FEM) {  // fem: the same pushes, D:1748-1768
         const int64_t ns = (int64_t)c->loss_q.size();  // q[i-1] = U_old(i).SigMat U_old(i)
         for (int64_t i = 1; i <= ns && (int64_t)col.size() < *n; ++i) {
             col.push_back(0.5 * c->loss_q[i - 1]);

@@ -74,6 +74,12 @@ struct Build {
 
     // ---- DGA tables
     CsrStore C, Ct, H, N, E, A, RHS1, Sig;
+    // lossy dga (tf_dga_frac): stacked fractional operators, the U_frac entry
+    // each edge's boundary / source value goes to, and the block sizes
+    // h, p, q, b, n, r, s
+    CsrStore frac_Ku, frac_Ru, frac_Kf, frac_Rf, frac_M, frac_T;
+    std::vector<int32_t> frac_slot;
+    int32_t frac_sizes[7] = {0, 0, 0, 0, 0, 0, 0};
     std::vector<int32_t> bnd_edge, h_row, free_edges;
     std::vector<int8_t> edge_class;
     std::vector<double> fixed_value;

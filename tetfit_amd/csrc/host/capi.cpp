@@ -79,9 +79,9 @@ int32_t tf_build_simulation(const tf_model* m, int32_t label, tf_build** out) {
         auto b = std::make_unique<tf_build>();
         b->b.method = sim.method;
         if (sim.method == "fdtd") tfh::build_fit(m->m, sim, b->b);
-        else if (sim.method == "dga" || sim.method == "gfdtd") tfh::build_tet(m->m, sim, b->b);
-        else throw tfh::Error("method '" + sim.method + "' is outside this build's hot path (fdtd, dga, gfdtd)");
-        if (sim.method == "gfdtd" && sim.solver == "agmg")
+        else if (sim.method == "dga" || sim.method == "gfdtd" || sim.method == "fem") tfh::build_tet(m->m, sim, b->b);
+        else throw tfh::Error("method '" + sim.method + "' is outside this build's hot path (fdtd, dga, gfdtd, fem)");
+        if ((sim.method == "gfdtd" || sim.method == "fem") && sim.solver == "agmg")
             throw tfh::Error("solver agmg needs the Fortran AGMG library, not available; use solver cg");
         b->b.finalize_views();
         *out = b.release();

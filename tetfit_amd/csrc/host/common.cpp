@@ -277,6 +277,24 @@ void Build::finalize_views() {
         d.free_edges = free_edges.empty() ? nullptr : free_edges.data();
         d.n_free = (int32_t)free_edges.size();
         d.Sig = Sig.view();
+        if (!frac_slot.empty()) {
+            tf_dga_frac& fr = d.frac;
+            fr.enabled = 1;
+            fr.h_size = frac_sizes[0];
+            fr.p_size = frac_sizes[1];
+            fr.q_size = frac_sizes[2];
+            fr.b_size = frac_sizes[3];
+            fr.n_size = frac_sizes[4];
+            fr.r_size = frac_sizes[5];
+            fr.s_size = frac_sizes[6];
+            fr.Ku = frac_Ku.view();
+            fr.Ru = frac_Ru.view();
+            fr.Mu = frac_M.view();
+            fr.Kf = frac_Kf.view();
+            fr.Rf = frac_Rf.view();
+            fr.Tf = frac_T.view();
+            fr.slot = frac_slot.data();
+        }
     }
 }
 

@@ -57,6 +57,37 @@ struct Tri { uint32_t a, b, c, slot; };
 struct Ed { uint32_t a, b, slot; };
 
 // Cholesky solve of an SPD matrix (n x n, row-major) for its inverse.
+// Inverse of a general n x n matrix (row-major) by Gauss-Jordan elimination
+// with partial pivoting; false when singular.
+bool general_inverse(const std::vector<double>& A0, int n, std::vector<double>& inv) {
+    std::vector<double> A(A0);
+    inv.assign((size_t)n * n, 0.0);
+    for (int i = 0; i < n; ++i) inv[(size_t)i * n + i] = 1.0;
+    for (int c = 0; c < n; ++c) {
+        int piv = c;
+        for (int r = c + 1; r < n; ++r)
+            if (std::fabs(A[(size_t)r * n + c]) > std::fabs(A[(size_t)piv * n + c])) piv = r;
+        if (!(A[(size_t)piv * n + c] != 0)) return false;
+        if (piv != c)
+            for (int k = 0; k < n; ++k) {
+                std::swap(A[(size_t)c * n + k], A[(size_t)piv * n + k]);
+                std::swap(inv[(size_t)c * n + k], inv[(size_t)piv * n + k]);
+            }
+        const double d = A[(size_t)c * n + c];
+        for (int k = 0; k < n; ++k) { A[(size_t)c * n + k] /= d; inv[(size_t)c * n + k] /= d; }
+        for (int r = 0; r < n; ++r) {
+            if (r == c) continue;
+            const double f = A[(size_t)r * n + c];
+            if (f == 0) continue;
+            for (int k = 0; k < n; ++k) {
+                A[(size_t)r * n + k] -= f * A[(size_t)c * n + k];
+                inv[(size_t)r * n + k] -= f * inv[(size_t)c * n + k];
+            }
+        }
+    }
+    return true;
+}
+
 bool spd_inverse(const std::vector<double>& A, int n, std::vector<double>& inv) {
     std::vector<double> L(n * n, 0.0);
     for (int j = 0; j < n; ++j) {
@@ -134,6 +165,7 @@ void build_tet(const Model& m, const SimulationDef& sim, Build& b) {
     std::map<uint32_t, BoundaryCondition> bcs = m.bcs;
     std::map<uint32_t, SourceDef> sources = m.sources;
     const bool is_dga = sim.method == "dga";
+    const bool is_fem = sim.method == "fem";
     const double scale = msh.scale;
 
     PhaseClock clk;
@@ -422,6 +454,18 @@ void build_tet(const Model& m, const SimulationDef& sim, Build& b) {
     b.h_row.assign(E, -1);
     b.fixed_value.assign(E, 0.0);
     int32_t H_size = 0;
+    // fractional indices of the lossy dga unknowns (D:6806-6893): p_index of
+    // class-2 edges, boundary_index of class-4 edges, and the offset of each
+    // fractured node star's block of U_c (dual_is_fractured), all handed out
+    // in edge order
+    std::vector<int32_t> p_index(E, -1), bnd_index(E, -1), star_off(np, -1);
+    int32_t P_size = 0, Q_size = 0, B_size = 0;
+    auto fracture_star = [&](uint32_t node) {
+        if (star_off[node] < 0) {
+            star_off[node] = Q_size;
+            Q_size += (int32_t)(nte_ptr[node + 1] - nte_ptr[node]);
+        }
+    };
     for (uint32_t k = 0; k < E; ++k) {
         if (!edge_src[k].empty()) {
             std::vector<uint32_t> keep;
@@ -445,13 +489,19 @@ void build_tet(const Model& m, const SimulationDef& sim, Build& b) {
                 if (sources[s].st == "e") { b.edge_class[k] = 4; treated = true; break; }
         }
         if (!treated) {
-            if (rec0 > 2 || rec1 > 2 || rec0 + rec1 == 1 || rec0 + rec1 == 3 || dvp_sig[p0] != dvp_sig[p1])
+            if (rec0 > 2 || rec1 > 2 || rec0 + rec1 == 1 || rec0 + rec1 == 3 || dvp_sig[p0] != dvp_sig[p1]) {
                 b.edge_class[k] = 3;
-            else if (rec0 == 2 || rec1 == 2 || rec0 == 0 || rec1 == 0) {
+                fracture_star(p0);
+                fracture_star(p1);
+            } else if (rec0 == 2 || rec1 == 2 || rec0 == 0 || rec1 == 0) {
                 b.edge_class[k] = 1;
                 b.h_row[k] = H_size++;
-            } else
+            } else {
                 b.edge_class[k] = 2;
+                p_index[k] = P_size++;
+            }
+        } else {
+            bnd_index[k] = B_size++;
         }
         if (b.edge_class[k] == 4 && edge_bcs[k] != 0) {
             // ComputeEdgeBC: BCs[BndToBC[edge_bcs[e]]].GetValue() (D:3899-3905)
@@ -459,14 +509,22 @@ void build_tet(const Model& m, const SimulationDef& sim, Build& b) {
             b.fixed_value[k] = bcs[l].val;
         }
     }
-    if (is_dga) {
-        for (uint32_t k = 0; k < E; ++k)
-            if (b.edge_class[k] == 2 || b.edge_class[k] == 3)
-                throw Error("method dga with lossy media (edge classes 2/3) is not supported by this build");
-        for (uint32_t f = 0; f < F; ++f)
-            if (face_class[f] != 1)
-                throw Error("method dga with magnetic losses (face classes 2/3) is not supported by this build");
+    // face indices of the fractional magnetic unknowns (D:6501-6541): n_index
+    // (class 1), r_index (class 2) and the offset of each fractured tet's
+    // block of F_c (primal_is_fractured), handed out in face order
+    std::vector<int32_t> n_index(F, -1), r_index(F, -1), tet_off(T, -1);
+    int32_t N_size = 0, R_size = 0, S_size = 0;
+    for (uint32_t f = 0; f < F; ++f) {
+        if (face_class[f] == 1) n_index[f] = N_size++;
+        else if (face_class[f] == 2) r_index[f] = R_size++;
+        else
+            for (int q = 0; q < ftv_n[f]; ++q) {
+                const int32_t t = ftv_t[(size_t)f * 2 + q];
+                if (tet_off[t] < 0) { tet_off[t] = S_size; S_size += 4; }
+            }
     }
+    // lossy or magnetic-loss media: dga steps the fractional unknowns
+    const bool frac = is_dga && (P_size > 0 || Q_size > 0 || R_size > 0 || S_size > 0);
 
     // ---- C (faces x edges) and its transpose
     {
@@ -545,10 +603,10 @@ void build_tet(const Model& m, const SimulationDef& sim, Build& b) {
         out[3] = cross(ev0, ev2) / 2.0;
     };
     // local_M of the Codecasa / gfdtd mu block (D:8757-8776)
-    auto local_M = [&](uint32_t t, double M[16]) {
+    // with w = chi instead of mu: local_Z of the magnetic losses (D:8778-8801)
+    auto local_W = [&](uint32_t t, const Mat3& mu, double M[16]) {
         Vec3 fv[6];
         dual_area_vectors(t, fv);
-        const Mat3& mu = materials[b.vol_material[t]].mu;
         auto d = [&](int i, int j) { return dot(fv[i], mu * fv[j]); };
         double coeff = 36.0 / std::fabs(b.cell_volumes[t]);
         M[0] = (d(0, 0) + d(2, 2) + d(3, 3)) * coeff;
@@ -563,6 +621,7 @@ void build_tet(const Model& m, const SimulationDef& sim, Build& b) {
         M[11] = (-d(2, 3) - d(1, 4)) * coeff;
         M[4] = M[1]; M[8] = M[2]; M[12] = M[3]; M[9] = M[6]; M[13] = M[7]; M[14] = M[11];
     };
+    auto local_M = [&](uint32_t t, double M[16]) { local_W(t, materials[b.vol_material[t]].mu, M); };
 
     clk.mark("tet edges");
     // ---- N = sum over tets of local_M^-1 (rows n_index = face id for class-1 faces)
@@ -601,16 +660,27 @@ void build_tet(const Model& m, const SimulationDef& sim, Build& b) {
 
     if (is_dga) {
         // ---- node-star eps blocks, local inverses and H (D:8819-9121, 9263-9371)
-        // H row pattern of a class-1 edge (a,b): star(a) U star(b) without class-4 edges.
+        // Rows of H: the class-1 edges. With lossy media (fractional
+        // unknowns) this H serves the dt estimate only, and every
+        // non-Dirichlet edge is a row: the lossless operator of the same
+        // mesh (ComputeDGATimeStep works on the lossless inverses,
+        // D:9088-9141); the stepped operators are assembled once dt is known.
+        std::vector<int32_t> est_row = b.h_row;
+        int32_t est_size = H_size;
+        if (frac) {
+            est_size = 0;
+            for (uint32_t e = 0; e < E; ++e) est_row[e] = b.edge_class[e] != 4 ? est_size++ : -1;
+        }
+        // H row pattern of a row edge (a,b): star(a) U star(b) without class-4 edges.
         std::vector<int32_t> pr, pc;
         {
             CsrStore& H = b.H;
-            H.nrows = H_size;
+            H.nrows = est_size;
             H.ncols = (int32_t)E;
-            H.rowptr.assign(H_size + 1, 0);
+            H.rowptr.assign(est_size + 1, 0);
             std::vector<int32_t> row;
             for (uint32_t e = 0; e < E; ++e) {
-                if (b.h_row[e] < 0) continue;
+                if (est_row[e] < 0) continue;
                 row.clear();
                 for (int s = 0; s < 2; ++s) {
                     uint32_t n = en[2 * (size_t)e + s];
@@ -620,7 +690,7 @@ void build_tet(const Model& m, const SimulationDef& sim, Build& b) {
                 std::sort(row.begin(), row.end());
                 row.erase(std::unique(row.begin(), row.end()), row.end());
                 H.col.insert(H.col.end(), row.begin(), row.end());
-                H.rowptr[b.h_row[e] + 1] = (int64_t)H.col.size();
+                H.rowptr[est_row[e] + 1] = (int64_t)H.col.size();
             }
             H.val.assign(H.col.size(), 0.0);
         }
@@ -687,12 +757,12 @@ void build_tet(const Model& m, const SimulationDef& sim, Build& b) {
                 if (b.edge_class[gi[j]] == 4)
                     for (uint32_t c = 0; c < deg; ++c) LH[(size_t)j * deg + c] = LH[(size_t)c * deg + j] = 0;
             for (uint32_t jj = 0; jj < deg; ++jj) {
-                if (b.edge_class[gi[jj]] != 1) continue;
+                if (est_row[gi[jj]] < 0) continue;
                 for (uint32_t kk = jj; kk < deg; ++kk) {
                     if (b.edge_class[gi[kk]] == 4 || LH[(size_t)jj * deg + kk] == 0) continue;
                     const double v = LH[(size_t)jj * deg + kk];
-                    csr_at(b.H, b.h_row[gi[jj]], (int32_t)gi[kk]) += v;
-                    if (jj != kk) csr_at(b.H, b.h_row[gi[kk]], (int32_t)gi[jj]) += v;
+                    csr_at(b.H, est_row[gi[jj]], (int32_t)gi[kk]) += v;
+                    if (jj != kk) csr_at(b.H, est_row[gi[kk]], (int32_t)gi[jj]) += v;
                 }
             }
         }
@@ -701,9 +771,9 @@ void build_tet(const Model& m, const SimulationDef& sim, Build& b) {
         {
             std::vector<double> U(E), cu(F), Fw(F), cf(E);
             auto apply = [&](const std::vector<double>& x, std::vector<double>& y) {
-                // x, y live on H rows (class-1 edges)
+                // x, y live on H rows
 #pragma omp parallel for schedule(static)
-                for (int64_t e = 0; e < (int64_t)E; ++e) U[e] = b.h_row[e] >= 0 ? x[b.h_row[e]] : 0.0;
+                for (int64_t e = 0; e < (int64_t)E; ++e) U[e] = est_row[e] >= 0 ? x[est_row[e]] : 0.0;
 #pragma omp parallel for schedule(static)
                 for (int64_t f = 0; f < (int64_t)F; ++f) {
                     double s = 0;
@@ -723,16 +793,94 @@ void build_tet(const Model& m, const SimulationDef& sim, Build& b) {
                     cf[e] = s;
                 }
 #pragma omp parallel for schedule(static)
-                for (int32_t r = 0; r < H_size; ++r) {
+                for (int32_t r = 0; r < est_size; ++r) {
                     double s = 0;
                     for (int64_t q = b.H.rowptr[r]; q < b.H.rowptr[r + 1]; ++q) s += b.H.val[q] * cf[b.H.col[q]];
                     y[r] = s;
                 }
             };
-            double lam = power_lambda_max((size_t)H_size, apply, 60) * 1.02;
+            double lam = power_lambda_max((size_t)est_size, apply, 60) * 1.02;
             b.stable_dt = lam > 0 ? 2.0 / std::sqrt(lam) : 0;
         }
-    } else {
+    } else if (is_fem) {
+        // ---- fem: Whitney element mass E and SigMat (ConstructFEMaterialMatrices
+        // D:7917-8058). Per tet, the Whitney functions of its six edges at
+        // the four nodes (weight 1/40) and at the barycentres of the four
+        // faces (weight 9/40) of the degree-3 rule, |V| sum w_k . eps w_j,
+        // pushed for j <= k (and mirrored) in the reference's order.
+        Triplets Etr, Str;
+        bool lossy = false;
+        for (uint32_t t = 0; t < T; ++t) {
+            const Material& mt = materials[b.vol_material[t]];
+            if (mt.chi.det() != 0)
+                throw Error("method fem with magnetic losses (chi) is not supported: the reference scales its "
+                            "local Z by a time step it has not computed yet (D:7798, D:8074)");
+            const bool lossy_t = mt.sigma.det() != 0;
+            lossy = lossy || lossy_t;
+            const int32_t* n = &b.vols[(size_t)t * 4];
+            // barycentric coordinates: lambda_i(p) = [p 1] . wn[:, i]
+            double M4[16], wn[16];
+            for (int i = 0; i < 4; ++i) {
+                const Vec3& q = P(n[i]);
+                M4[i * 4 + 0] = q.x; M4[i * 4 + 1] = q.y; M4[i * 4 + 2] = q.z; M4[i * 4 + 3] = 1.0;
+            }
+            {
+                std::vector<double> A(M4, M4 + 16), inv;
+                if (!general_inverse(A, 4, inv)) throw Error("degenerate tetrahedron in the fem assembly");
+                for (int z = 0; z < 16; ++z) wn[z] = inv[z];
+            }
+            Vec3 g[4];
+            for (int i = 0; i < 4; ++i) g[i] = Vec3(wn[0 * 4 + i], wn[1 * 4 + i], wn[2 * 4 + i]);
+            auto whitney = [&](const Vec3& q, Vec3 w[6]) {
+                double l[4];
+                for (int i = 0; i < 4; ++i) l[i] = q.x * wn[i] + q.y * wn[4 + i] + q.z * wn[8 + i] + wn[12 + i];
+                const int le[6][2] = {{0, 1}, {1, 2}, {0, 2}, {0, 3}, {1, 3}, {2, 3}};
+                for (int k = 0; k < 6; ++k) w[k] = l[le[k][0]] * g[le[k][1]] - l[le[k][1]] * g[le[k][0]];
+            };
+            double sn[6][6] = {}, sf[6][6] = {}, sns[6][6] = {}, sfs[6][6] = {};
+            for (int h = 0; h < 4; ++h) {
+                Vec3 w1[6], w2[6];
+                whitney(P(n[h]), w1);
+                whitney(face_bars[b.vtf[(size_t)t * 4 + h]], w2);
+                for (int j = 0; j < 6; ++j)
+                    for (int k = j; k < 6; ++k) {
+                        sn[j][k] = sn[j][k] + dot(w1[k], mt.epsilon * w1[j]);
+                        sf[j][k] = sf[j][k] + dot(w2[k], mt.epsilon * w2[j]);
+                        sns[j][k] = sns[j][k] + dot(w1[k], mt.sigma * w1[j]);
+                        sfs[j][k] = sfs[j][k] + dot(w2[k], mt.sigma * w2[j]);
+                    }
+            }
+            const int32_t* te = &tet_edges[(size_t)t * 6];
+            const double vol = std::fabs(b.cell_volumes[t]);
+            for (int j = 0; j < 6; ++j)
+                for (int k = j; k < 6; ++k) {
+                    const double val = vol * (sn[j][k] + sf[j][k] * 9.0) / 40.0;
+                    if (val != 0) {
+                        Etr.add(te[j], te[k], val);
+                        if (k != j) Etr.add(te[k], te[j], val);
+                    }
+                    if (lossy_t) {
+                        const double sv = vol * (sns[j][k] + sfs[j][k] * 9.0) / 40.0;
+                        if (sv != 0) {
+                            Str.add(te[j], te[k], sv);
+                            if (k != j) Str.add(te[k], te[j], sv);
+                        }
+                    }
+                }
+        }
+        b.E = Etr.to_csr((int32_t)E, (int32_t)E);
+        if (lossy) {
+            // SigMat on E's pattern (its entries are a subset)
+            CsrStore S = Str.to_csr((int32_t)E, (int32_t)E);
+            b.Sig = b.E;
+            std::fill(b.Sig.val.begin(), b.Sig.val.end(), 0.0);
+            for (uint32_t r = 0; r < E; ++r)
+                for (int64_t q = S.rowptr[r]; q < S.rowptr[r + 1]; ++q) csr_at(b.Sig, (int32_t)r, S.col[q]) = S.val[q];
+        }
+    clk.mark("E, SigMat (Whitney elements)");
+    }
+    if (!is_dga) {
+        if (!is_fem) {
         // ---- gfdtd: E from the tet-corner dual basis (D:8379-8420), A, RHSmat1
         std::vector<int32_t> pr, pc;
         pr.reserve((size_t)T * 36);
@@ -782,7 +930,9 @@ void build_tet(const Model& m, const SimulationDef& sim, Build& b) {
             }
         }
     clk.mark("E (corner terms)");
-        // free (non-Dirichlet) edges: compressed_dirichlet (D:8145-8171)
+        }
+        // free (non-Dirichlet) edges: compressed_dirichlet (D:8145-8171;
+        // fem: D:7703-7729, the same rule)
         std::vector<int32_t> is_dir(E, -1);
         for (uint32_t i = 0; i < E; ++i) {
             bool no_bc = edge_bcs[i] == 0 || bc_type(edge_bcs[i]) == "none";
@@ -908,17 +1058,296 @@ void build_tet(const Model& m, const SimulationDef& sim, Build& b) {
         else t_step = sim_dt_request;
     }
     b.prob.dt = t_step;
-    if (!is_dga && !b.Sig.rowptr.empty()) {
-        // SysMat = E + (dt/2) SigMat split into A (free x free) and RHSmat1
-        // (free rows, Dirichlet columns), D:8646-8668; SigMat has E's pattern
-        const double h = 0.5 * t_step;
+    if (frac) {
+        // ---- lossy / magnetic-loss dga: the fractional operators once dt is
+        // known (ConstructCodecasaMaterialMatrices, D:9180-9481). Triplets are
+        // pushed in the reference's order and summed (overwritten for M and
+        // T) in that order.
+        const double hdt = 0.5 * t_step;
+        // tets: local_N = (M + dt/2 Z)^-1 and local_RS = local_N (M - dt/2 Z)
+        // where chi is invertible, else M^-1 and the identity (D:9180-9257)
+        Triplets Ntr, Trtr, Str, Tstr, Ttr;
+        std::vector<double> Rr(R_size, 0.0);
+        {
+            std::vector<double> A(16), LN;
+            double Mb[16], Zb[16], RS[16];
+            for (uint32_t t = 0; t < T; ++t) {
+                const Mat3& chi = materials[b.vol_material[t]].chi;
+                local_M(t, Mb);
+                const bool lossy_t = chi.det() != 0;
+                if (lossy_t) {
+                    local_W(t, chi, Zb);
+                    for (int z = 0; z < 16; ++z) A[z] = Mb[z] + hdt * Zb[z];
+                } else {
+                    A.assign(Mb, Mb + 16);
+                }
+                if (!spd_inverse(A, 4, LN)) throw Error("local mu block not positive definite (degenerate tet)");
+                for (int i = 0; i < 4; ++i)
+                    for (int j = 0; j < 4; ++j) {
+                        double v = 0;
+                        if (lossy_t)
+                            for (int q = 0; q < 4; ++q) v += LN[i * 4 + q] * (Mb[q * 4 + j] - hdt * Zb[q * 4 + j]);
+                        else
+                            v = i == j ? 1.0 : 0.0;
+                        RS[i * 4 + j] = v;
+                    }
+                const int32_t* fid = &b.vtf[(size_t)t * 4];  // abs(vtf_list[t]): f4, f3, f2, f1
+                const int32_t off = tet_off[t];
+                for (int jj = 0; jj < 4; ++jj) {
+                    const int32_t j = fid[jj];
+                    if (face_class[j] == 1) {
+                        for (int kk = 0; kk < 4; ++kk)
+                            if (LN[jj * 4 + kk] != 0) Ntr.add(n_index[j], fid[kk], LN[jj * 4 + kk]);
+                        Ttr.add(j, n_index[j], 1.0);
+                    } else if (face_class[j] == 2) {
+                        Rr[r_index[j]] = RS[jj * 4 + jj];
+                        for (int kk = 0; kk < 4; ++kk)
+                            if (LN[jj * 4 + kk] != 0) Trtr.add(r_index[j], fid[kk], LN[jj * 4 + kk]);
+                        Ttr.add(j, N_size + r_index[j], 1.0);
+                    } else {
+                        if (off < 0) throw Error("internal: class-3 face of an unfractured tet");
+                        Ttr.add(j, N_size + R_size + off + jj, 1.0);
+                    }
+                    if (off >= 0)
+                        for (int kk = jj; kk < 4; ++kk) {
+                            const double rs = RS[jj * 4 + kk], ln = LN[jj * 4 + kk];
+                            if (rs != 0) {
+                                Str.add(off + jj, off + kk, rs);
+                                if (jj != kk) Str.add(off + kk, off + jj, rs);
+                            }
+                            if (ln != 0) {
+                                Tstr.add(off + jj, fid[kk], ln);
+                                if (jj != kk) Tstr.add(off + kk, fid[jj], ln);
+                            }
+                        }
+                }
+            }
+        }
+        // node stars: local_E and local_S (D:8913-9078), class-4 rows and
+        // columns set to the identity (E) and zero (S), local_H = (E +
+        // dt/2 S)^-1 and local_PQ = local_H (E - dt/2 S) at stars touching a
+        // tet with invertible sigma, else E^-1 and the identity; class-4 rows
+        // and columns of local_H zeroed, of local_PQ zeroed with a unit
+        // diagonal (D:9262-9333). The mirrored H and Mp entries go to the row
+        // of the partner edge's h_index / p_index, which is 0 when the partner
+        // belongs to another class (reproduced; D:9363-9366, D:9382-9385).
+        Triplets Htr, Mptr, Mqtr, Qtr, Mtr;
+        std::vector<double> Pp(P_size, 0.0);
+        {
+            std::vector<double> LE, LS, A, LH, LPQ;
+            Vec3 fa[4];
+            const int32_t Uf_h = H_size, Uf_q = H_size + P_size, Uf_b = H_size + P_size + Q_size;
+            auto hidx = [&](uint32_t e) { return b.edge_class[e] == 1 ? b.h_row[e] : 0; };
+            auto pidx = [&](uint32_t e) { return b.edge_class[e] == 2 ? p_index[e] : 0; };
+            for (uint32_t nid = 0; nid < np; ++nid) {
+                const uint32_t deg = nte_ptr[nid + 1] - nte_ptr[nid];
+                if (deg == 0) continue;
+                const uint32_t* gi = &nte[nte_ptr[nid]];
+                LE.assign((size_t)deg * deg, 0.0);
+                LS.assign((size_t)deg * deg, 0.0);
+                bool sigma_node = false;
+                auto loc = [&](int32_t e) { return (uint32_t)(std::lower_bound(gi, gi + deg, (uint32_t)e) - gi); };
+                for (uint32_t q = av_cnt[nid]; q < av_cnt[nid + 1]; ++q) {
+                    const uint32_t t = av[q];
+                    const int32_t* n = &b.vols[(size_t)t * 4];
+                    int k = 0;
+                    while (n[k] != (int32_t)nid) ++k;
+                    primal_area_vectors(t, fa);
+                    const double vol = std::fabs(b.cell_volumes[t]);
+                    const Mat3& eps = materials[b.vol_material[t]].epsilon;
+                    const Mat3& sig = materials[b.vol_material[t]].sigma;
+                    const bool lossy_t = sig.det() != 0;
+                    if (lossy_t) sigma_node = true;
+                    Vec3 f[3];
+                    uint32_t le[3];
+                    for (int i = 0; i < 3; ++i) {
+                        int face = kCornerFaces[k][i];
+                        double sg = vtf_sgn[(size_t)t * 4 + face];
+                        f[i] = fa[face] * (kCornerSign[k][i] * 2.0 / 3.0 / vol * sg);
+                        le[i] = loc(tet_edges[(size_t)t * 6 + kCornerEdges[k][i]]);
+                    }
+                    const int pairs[3][2] = {{0, 1}, {0, 2}, {1, 2}};
+                    for (std::vector<double>* M : {&LE, &LS}) {
+                        const bool s_part = M == &LS;
+                        if (s_part && !lossy_t) continue;
+                        const Mat3& w = s_part ? sig : eps;
+                        std::vector<double>& L = *M;
+                        for (int i = 0; i < 3; ++i) L[(size_t)le[i] * deg + le[i]] += dot(f[i], w * f[i]) * vol / 4;
+                        for (auto& pq : pairs) {
+                            double v = dot(f[pq[0]], w * f[pq[1]]) * vol / 4;
+                            double& a = L[(size_t)le[pq[0]] * deg + le[pq[1]]];
+                            a = a + v;
+                            L[(size_t)le[pq[1]] * deg + le[pq[0]]] = a;
+                        }
+                    }
+                }
+                for (uint32_t j = 0; j < deg; ++j)
+                    if (b.edge_class[gi[j]] == 4) {
+                        for (uint32_t c = 0; c < deg; ++c) {
+                            LE[(size_t)j * deg + c] = LE[(size_t)c * deg + j] = 0;
+                            if (sigma_node) LS[(size_t)j * deg + c] = LS[(size_t)c * deg + j] = 0;
+                        }
+                        LE[(size_t)j * deg + j] = 1;
+                    }
+                A = LE;
+                if (sigma_node)
+                    for (size_t z = 0; z < A.size(); ++z) A[z] += hdt * LS[z];
+                if (!spd_inverse(A, (int)deg, LH)) throw Error("node-star eps block not positive definite");
+                LPQ.assign((size_t)deg * deg, 0.0);
+                if (sigma_node) {
+                    for (uint32_t i = 0; i < deg; ++i)
+                        for (uint32_t j = 0; j < deg; ++j) {
+                            double v = 0;
+                            for (uint32_t q = 0; q < deg; ++q)
+                                v += LH[(size_t)i * deg + q] * (LE[(size_t)q * deg + j] - hdt * LS[(size_t)q * deg + j]);
+                            LPQ[(size_t)i * deg + j] = v;
+                        }
+                } else {
+                    for (uint32_t i = 0; i < deg; ++i) LPQ[(size_t)i * deg + i] = 1.0;
+                }
+                for (uint32_t j = 0; j < deg; ++j)
+                    if (b.edge_class[gi[j]] == 4)
+                        for (uint32_t c = 0; c < deg; ++c) {
+                            LH[(size_t)j * deg + c] = LH[(size_t)c * deg + j] = 0;
+                            LPQ[(size_t)j * deg + c] = LPQ[(size_t)c * deg + j] = 0;
+                            LPQ[(size_t)j * deg + j] = 1;
+                        }
+                const int32_t off = star_off[nid];
+                for (uint32_t jj = 0; jj < deg; ++jj) {
+                    const uint32_t j = gi[jj];
+                    const int cls = b.edge_class[j];
+                    if (cls == 1 || cls == 2) {
+                        Triplets& tr = cls == 1 ? Htr : Mptr;
+                        const int32_t row = cls == 1 ? b.h_row[j] : p_index[j];
+                        const int32_t nrows = cls == 1 ? H_size : P_size;
+                        if (cls == 2) Pp[p_index[j]] = LPQ[(size_t)jj * deg + jj];
+                        for (uint32_t kk = jj; kk < deg; ++kk) {
+                            const double v = LH[(size_t)jj * deg + kk];
+                            if (b.edge_class[gi[kk]] == 4 || v == 0) continue;
+                            tr.add(row, (int32_t)gi[kk], v);
+                            if (jj != kk && nrows > 0) tr.add(cls == 1 ? hidx(gi[kk]) : pidx(gi[kk]), (int32_t)j, v);
+                        }
+                        Mtr.add((int32_t)j, cls == 1 ? row : Uf_h + row, 1.0);
+                    } else if (cls == 3) {
+                        if (off < 0) throw Error("internal: class-3 edge of an unfractured node star");
+                        Mtr.add((int32_t)j, Uf_q + off + (int32_t)jj, 1.0);
+                    } else {
+                        Mtr.add((int32_t)j, Uf_b + bnd_index[j], 1.0);
+                    }
+                    if (off >= 0 && cls != 4)
+                        for (uint32_t kk = 0; kk < deg; ++kk) {
+                            if (b.edge_class[gi[kk]] == 4) continue;
+                            const double pq = LPQ[(size_t)jj * deg + kk], h = LH[(size_t)jj * deg + kk];
+                            if (pq != 0) Qtr.add(off + (int32_t)jj, off + (int32_t)kk, pq);
+                            if (h != 0) Mqtr.add(off + (int32_t)jj, (int32_t)gi[kk], h);
+                        }
+                }
+            }
+        }
+        // Stacked for the stepping (rows in fractional order):
+        //   Ku = [H; Mp; Mq; 0 (boundary rows)], Ru = [I; diag P_p; Q; I]
+        //   Kf = [N; Tr; Ts],                   Rf = [I; diag R_r; S]
+        // so that U_frac <- Ru U_frac_old + dt Ku curl_f and F_frac <- Rf
+        // F_frac_old - dt Kf curl_u, U = M U_frac, F = T F_frac.
+        const int32_t nU = H_size + P_size + Q_size + B_size, nF = N_size + R_size + S_size;
+        auto stack = [](std::initializer_list<const CsrStore*> parts, int32_t ncols, int32_t nrows_total) {
+            CsrStore m;
+            m.nrows = nrows_total;
+            m.ncols = ncols;
+            m.rowptr.assign(1, 0);
+            for (const CsrStore* p : parts) {
+                for (int32_t r = 0; r < p->nrows; ++r) {
+                    for (int64_t q = p->rowptr[r]; q < p->rowptr[r + 1]; ++q) {
+                        m.col.push_back(p->col[q]);
+                        m.val.push_back(p->val[q]);
+                    }
+                    m.rowptr.push_back((int64_t)m.col.size());
+                }
+            }
+            while ((int32_t)m.rowptr.size() < nrows_total + 1) m.rowptr.push_back((int64_t)m.col.size());
+            return m;
+        };
+        CsrStore Hm = Htr.to_csr(H_size, (int32_t)E), Mp = Mptr.to_csr(P_size, (int32_t)E),
+                 Mq = Mqtr.to_csr(Q_size, (int32_t)E);
+        b.frac_Ku = stack({&Hm, &Mp, &Mq}, (int32_t)E, nU);
+        CsrStore Qm = Qtr.to_csr(Q_size, Q_size);
+        {
+            CsrStore& R = b.frac_Ru;
+            R.nrows = R.ncols = nU;
+            R.rowptr.assign(1, 0);
+            for (int32_t r = 0; r < nU; ++r) {
+                if (r < H_size || r >= H_size + P_size + Q_size) {
+                    R.col.push_back(r);
+                    R.val.push_back(1.0);
+                } else if (r < H_size + P_size) {
+                    R.col.push_back(r);
+                    R.val.push_back(Pp[r - H_size]);
+                } else {
+                    const int32_t qr = r - H_size - P_size;
+                    for (int64_t q = Qm.rowptr[qr]; q < Qm.rowptr[qr + 1]; ++q) {
+                        R.col.push_back(H_size + P_size + Qm.col[q]);
+                        R.val.push_back(Qm.val[q]);
+                    }
+                }
+                R.rowptr.push_back((int64_t)R.col.size());
+            }
+        }
+        CsrStore Nm = Ntr.to_csr(N_size, (int32_t)F), Tr = Trtr.to_csr(R_size, (int32_t)F),
+                 Ts = Tstr.to_csr(S_size, (int32_t)F);
+        b.frac_Kf = stack({&Nm, &Tr, &Ts}, (int32_t)F, nF);
+        CsrStore Sm = Str.to_csr(S_size, S_size);
+        {
+            CsrStore& R = b.frac_Rf;
+            R.nrows = R.ncols = nF;
+            R.rowptr.assign(1, 0);
+            for (int32_t r = 0; r < nF; ++r) {
+                if (r < N_size) {
+                    R.col.push_back(r);
+                    R.val.push_back(1.0);
+                } else if (r < N_size + R_size) {
+                    R.col.push_back(r);
+                    R.val.push_back(Rr[r - N_size]);
+                } else {
+                    const int32_t sr = r - N_size - R_size;
+                    for (int64_t q = Sm.rowptr[sr]; q < Sm.rowptr[sr + 1]; ++q) {
+                        R.col.push_back(N_size + R_size + Sm.col[q]);
+                        R.val.push_back(Sm.val[q]);
+                    }
+                }
+                R.rowptr.push_back((int64_t)R.col.size());
+            }
+        }
+        b.frac_M = Mtr.to_csr((int32_t)E, nU, true);
+        b.frac_T = Ttr.to_csr((int32_t)F, nF, true);
+        // the U_frac entry a boundary value or source writes each step: U_d
+        // for class-4 edges, U_a for class-1 edges (a d source resets it,
+        // D:843-849); U_b and U_c are overwritten from their old values by
+        // the update, so writes to them have no effect (D:888-890)
+        b.frac_slot.assign(E, -1);
+        for (uint32_t e = 0; e < E; ++e) {
+            if (b.edge_class[e] == 1) b.frac_slot[e] = b.h_row[e];
+            else if (b.edge_class[e] == 4) b.frac_slot[e] = H_size + P_size + Q_size + bnd_index[e];
+        }
+        b.frac_sizes[0] = H_size; b.frac_sizes[1] = P_size; b.frac_sizes[2] = Q_size; b.frac_sizes[3] = B_size;
+        b.frac_sizes[4] = N_size; b.frac_sizes[5] = R_size; b.frac_sizes[6] = S_size;
+        // the lossless H above was the dt estimate's; the stepping uses Ku
+        b.H = CsrStore();
+    }
+    if (!is_dga && (is_fem || !b.Sig.rowptr.empty())) {
+        // gfdtd: SysMat = E + (dt/2) SigMat (D:8646-8668); fem: SysMat =
+        // E (1/dt^2) + (0.5/dt) SigMat (D:8086-8090); split into A (free x
+        // free) and RHSmat1 (free rows, Dirichlet columns); SigMat has E's
+        // pattern
+        const double h = 0.5 * t_step, inv2 = 1 / (t_step * t_step), sh = 0.5 / t_step;
+        const bool sig = !b.Sig.rowptr.empty();
         std::vector<uint8_t> is_free(E, 0);
         for (int32_t e : b.free_edges) is_free[e] = 1;
         size_t qa = 0, q1 = 0;
         for (uint32_t r = 0; r < E; ++r) {
             if (!is_free[r]) continue;
             for (int64_t q = b.E.rowptr[r]; q < b.E.rowptr[r + 1]; ++q) {
-                const double v = b.E.val[q] + h * b.Sig.val[q];
+                const double v = is_fem ? (sig ? b.E.val[q] * inv2 + sh * b.Sig.val[q] : b.E.val[q] * inv2)
+                                        : b.E.val[q] + h * b.Sig.val[q];
                 if (is_free[b.E.col[q]]) b.A.val[qa++] = v;
                 else b.RHS1.val[q1++] = v;
             }
@@ -1084,7 +1513,7 @@ void build_tet(const Model& m, const SimulationDef& sim, Build& b) {
     b.pts.resize(np * 3);
     for (uint64_t i = 0; i < np; ++i) { b.pts[i * 3] = pts[i].x; b.pts[i * 3 + 1] = pts[i].y; b.pts[i * 3 + 2] = pts[i].z; }
     b.n_pts = (int32_t)np;
-    b.prob.method = is_dga ? TF_METHOD_DGA : TF_METHOD_GFDTD;
+    b.prob.method = is_dga ? TF_METHOD_DGA : is_fem ? TF_METHOD_FEM : TF_METHOD_GFDTD;
     b.prob.n_faces = (int32_t)F;
     b.prob.n_vols = (int32_t)T;
     b.prob.duration = sim.duration;

@@ -69,7 +69,7 @@ class Build:
         self.label = label
         self.problem_ptr = L.tf_build_problem(h)
         p = self.problem_ptr.contents
-        self.method = {0: "fdtd", 1: "dga", 2: "gfdtd"}[p.method]
+        self.method = {0: "fdtd", 1: "dga", 2: "gfdtd", 3: "fem"}[p.method]
         self.n_edges, self.n_faces, self.n_vols = p.n_edges, p.n_faces, p.n_vols
         self.dt, self.duration = p.dt, p.duration
         self.n_steps = L.tf_build_n_steps(h)
@@ -219,7 +219,16 @@ class Build:
                     RHS1=_csr(d.RHS1), bnd_edge=_arr(d.bnd_edge, E, np.int32), n_bnd=d.n_bnd,
                     edge_class=_arr(d.edge_class, E, np.int8), fixed_value=_arr(d.fixed_value, E, np.float64),
                     h_row=_arr(d.h_row, E, np.int32) if d.h_row else None,
-                    free_edges=_arr(d.free_edges, d.n_free, np.int32))
+                    free_edges=_arr(d.free_edges, d.n_free, np.int32), Sig=_csr(d.Sig), frac=self.frac_ops())
+
+    def frac_ops(self):
+        """Fractional operators of a lossy dga problem (tf_dga_frac), or None."""
+        fr = self.problem_ptr.contents.dga.frac
+        if not fr.enabled:
+            return None
+        sizes = {k: getattr(fr, k) for k in ("h_size", "p_size", "q_size", "b_size", "n_size", "r_size", "s_size")}
+        return dict(sizes, Ku=_csr(fr.Ku), Ru=_csr(fr.Ru), Mu=_csr(fr.Mu), Kf=_csr(fr.Kf), Rf=_csr(fr.Rf),
+                    Tf=_csr(fr.Tf), slot=_arr(fr.slot, self.n_edges, np.int32))
 
 
 def write_kuhn_mesh(path, n, jitter=0.2, seed=0, block=8):
