@@ -179,9 +179,10 @@ tf_status tf_destroy(tf_ctx* ctx);
 /* Autotuner: times the context's SELL kernels (Ct, H, N for dga; N, the
  * Jacobi pass's A product and, once the learned preconditioner is active,
  * its two passes (G and A products) for gfdtd) over grid sizes and gather
- * batch depths and keeps the fastest for later tf_run calls. Fields are
- * identical for every choice (each row is summed by one lane in entry
- * order). Choices are cached per (kernel, precision, rows, entries) in the
+ * batch depths and keeps the fastest for later tf_run calls. Every row is
+ * summed by one lane in entry order for each choice, so explicit fields are
+ * identical; the CG's dot products are summed per workgroup, so a different
+ * grid changes their rounding (gfdtd / fem fields agree to rounding). Choices are cached per (kernel, precision, rows, entries) in the
  * process and in the file named by $TETFIT_TUNE_CACHE if set. No reference
  * counterpart. tf_get_tuning reports the choice per kernel (order:
  * sell_ct, sell_h, sell_n, cg_w, cg_pg, cg_wu; blocks 0 = default). */

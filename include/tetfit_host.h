@@ -68,10 +68,13 @@ typedef struct {
     int32_t type;          /* tf_field_kind                                   */
     int32_t carrier;       /* tf_carrier_kind                                 */
     int32_t profile;       /* 0 dc/const, 1 wave, 2 gaussian                  */
-    int32_t dir;           /* 0 x, 1 y, 2 z (cartesian directions only)       */
-    int32_t bfunc[3];      /* 0 none, 1 sin, 2 cos                            */
+    int32_t dir;           /* 0 x, 1 y, 2 z, 3 rho, 4 theta, 5 phi (spherical),*/
+                           /* 6 r, 7 psi (cylindrical)                        */
+    int32_t bfunc[3];      /* 0 none, 1 sin, 2 cos, 3 J_n, 4 Y_n, 5 j_l, 6 y_l, */
+                           /* 7 P_l (Source.hpp:362-417)                      */
     double  amp, freq, delay, width, impedance, radius, decay;
     double  kvec[3], center[3];
+    int32_t border[3];     /* order n / l of the Bessel / Legendre modes       */
 } tf_source_desc;
 
 typedef struct {

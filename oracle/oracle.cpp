@@ -2,6 +2,8 @@
 //
 // Single-threaded fp64 restatement of the reference loops; every function
 // names the reference lines it follows (D: = src/DGA/Discretization.hpp).
+// the C++ special functions (cyl_bessel_j & co.) under -std=c++14
+#define __STDCPP_WANT_MATH_SPEC_FUNCS__ 1
 #include "oracle.h"
 
 #include <chrono>
@@ -44,6 +46,38 @@ V3 source_compute(const tf_source_desc& s, const double p[3], double t) {
         ret *= std::pow(std::pow(p[0] - s.center[0], 2) + std::pow(p[1] - s.center[1], 2) +
                             std::pow(p[2] - s.center[2], 2),
                         -0.5 * s.decay);
+    const double* c = s.center;
+    // coordinates of the profile: cartesian, spherical (rho, theta, phi,
+    // Source.hpp:334-349) or cylindrical (r, psi, z; the centre's angle
+    // branch takes the sign of the point's x, Source.hpp:430-457)
+    double q1[3] = {p[0], p[1], p[2]}, q2[3] = {c[0], c[1], c[2]};
+    const bool sph = s.dir >= 3 && s.dir <= 5, cyl = s.dir >= 6;
+    if (sph) {
+        q1[0] = q1[1] = q1[2] = q2[0] = q2[1] = q2[2] = 0;
+        if (p[0] != 0 || p[1] != 0 || p[2] != 0) {
+            q1[0] = std::sqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
+            q1[1] = std::acos(p[2] / q1[0]);
+            q1[2] = std::atan2(p[1], p[0]);
+        }
+        if (c[0] != 0 || c[1] != 0 || c[2] != 0) {
+            q2[0] = std::sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+            q2[1] = std::acos(c[2] / q2[0]);
+            q2[2] = std::atan2(c[1], c[0]);
+        }
+    } else if (cyl) {
+        q1[0] = q1[1] = q2[0] = q2[1] = 0;
+        if (p[0] != 0 || p[1] != 0) {
+            q1[0] = std::sqrt(p[0] * p[0] + p[1] * p[1]);
+            if (p[0] >= 0) q1[1] = std::asin(p[1] / q1[0]);
+            else q1[1] = -std::asin(p[1] / q1[0]) + PI_L;
+        }
+        if (c[0] != 0 || c[1] != 0 || c[2] != 0) {
+            q2[0] = std::sqrt(c[0] * c[0] + c[1] * c[1]);
+            if (c[0] == 0 && c[1] == 0) q2[1] = 0;
+            else if (p[0] >= 0) q2[1] = std::asin(c[1] / q2[0]);
+            else q2[1] = -std::asin(c[1] / q2[0]) + PI_L;
+        }
+    }
     if (s.profile == 2) {
         double e = -((std::pow(p[0] - s.center[0], 2) + std::pow(p[1] - s.center[1], 2) +
                       std::pow(p[2] - s.center[2], 2)) /
@@ -51,14 +85,39 @@ V3 source_compute(const tf_source_desc& s, const double p[3], double t) {
         ret *= std::exp(e);
     } else if (s.profile == 1) {
         for (int j = 0; j < 3; ++j) {
-            if (s.bfunc[j] == 1) ret *= std::sin(2 * PI_L * s.kvec[j] * (long double)(p[j] - s.center[j]));
-            else if (s.bfunc[j] == 2) ret *= std::cos(2 * PI_L * s.kvec[j] * (long double)(p[j] - s.center[j]));
+            const double x = q1[j] - q2[j];
+            const int n = s.border[j];
+            if (s.bfunc[j] == 1) ret *= std::sin(2 * PI_L * s.kvec[j] * (long double)x);
+            else if (s.bfunc[j] == 2) ret *= std::cos(2 * PI_L * s.kvec[j] * (long double)x);
+            else if (s.bfunc[j] >= 3 && (sph || cyl)) {
+                // GSL's Jn, Yn, jl, yl, Pl (Source.hpp:362-404); x < 0 by parity
+                const double a = s.kvec[j] * x;
+                const double par = (n % 2 == 0) ? 1.0 : -1.0;
+                switch (s.bfunc[j]) {
+                    case 3: ret *= a < 0 ? par * std::cyl_bessel_j((double)n, -a) : std::cyl_bessel_j((double)n, a); break;
+                    case 4: ret *= std::cyl_neumann((double)n, a); break;
+                    case 5: ret *= a < 0 ? par * std::sph_bessel((unsigned)n, -a) : std::sph_bessel((unsigned)n, a); break;
+                    case 6: ret *= a < 0 ? -par * std::sph_neumann((unsigned)n, -a) : std::sph_neumann((unsigned)n, a); break;
+                    default: ret *= std::legendre((unsigned)n, std::cos(a)); break;
+                }
+            }
         }
     }
     V3 v;
-    if (s.dir == 0) v.x = ret;
-    else if (s.dir == 1) v.y = ret;
-    else v.z = ret;
+    switch (s.dir) {
+        case 0: v.x = ret; break;
+        case 1: v.y = ret; break;
+        case 3: v.x = std::sin(q1[1]) * std::cos(q1[2]) * ret; v.y = std::sin(q1[1]) * std::sin(q1[2]) * ret;
+                v.z = std::cos(q1[1]) * ret; break;
+        case 4: v.x = q1[0] * std::cos(q1[1]) * std::cos(q1[2]) * ret;
+                v.y = q1[0] * std::cos(q1[1]) * std::sin(q1[2]) * ret;
+                v.z = -q1[0] * std::sin(q1[2]) * ret; break;  // sin(phi), as Source.hpp:426 has it
+        case 5: v.x = -q1[0] * std::sin(q1[1]) * std::sin(q1[2]) * ret;
+                v.y = q1[0] * std::sin(q1[1]) * std::cos(q1[2]) * ret; break;
+        case 6: v.x = std::cos(q1[1]) * ret; v.y = std::sin(q1[1]) * ret; break;
+        case 7: v.x = -q1[0] * std::sin(q1[1]) * ret; v.y = q1[0] * std::cos(q1[1]) * ret; break;
+        default: v.z = ret; break;
+    }
     return v;
 }
 

@@ -59,12 +59,14 @@ def test_fit_curl_curl_closed_form(workdir):
     assert 1.0 <= est / lam_all <= 1.021
 
 
-@pytest.mark.parametrize("method", ["dga", "gfdtd"])
+@pytest.mark.parametrize("method", ["dga", "gfdtd", "fem"])
 @pytest.mark.parametrize("n,seed", [(3, 1), (4, 2)])
 def test_tet_dt_estimate_bounds_lambda_max(workdir, method, n, seed):
     """dga: lambda_max of H Ct N C on the H rows; gfdtd: the largest
-    generalized eigenvalue of (Ct N C, A) on the free edges, both from dense
-    eigen-solves on jittered Kuhn meshes with eps_r drawn per block."""
+    generalized eigenvalue of (Ct N C, A) on the free edges; fem: the same
+    with the Whitney element mass E in place of A (ComputeFEMTimeStep,
+    D:6928-7034); all from dense eigen-solves on jittered Kuhn meshes with
+    eps_r drawn per block."""
     p = configs.kuhn_cavity(str(workdir), n=n, method=method, n_steps=2, n_probes=1, block=2, seed=seed,
                             dt=1e-13, name=f"dt_{method}{n}")
     b = Model(p).build(1)
@@ -79,6 +81,7 @@ def test_tet_dt_estimate_bounds_lambda_max(workdir, method, n, seed):
     else:
         fe = o["free_edges"]
         K = (Ct @ N @ C).toarray()[np.ix_(fe, fe)]
-        lam = sl.eigh(K, csr(o["A"]).toarray(), eigvals_only=True).max()
+        M = csr(o["A"]).toarray() if method == "gfdtd" else csr(o["E"]).toarray()[np.ix_(fe, fe)]
+        lam = sl.eigh(K, M, eigvals_only=True).max()
     est = 4 / b.stable_dt ** 2
     assert 1.0 <= est / lam <= 1.021

@@ -77,7 +77,8 @@ class tf_source_desc(C.Structure):
                 ("dir", C.c_int32), ("bfunc", C.c_int32 * 3),
                 ("amp", C.c_double), ("freq", C.c_double), ("delay", C.c_double),
                 ("width", C.c_double), ("impedance", C.c_double), ("radius", C.c_double),
-                ("decay", C.c_double), ("kvec", C.c_double * 3), ("center", C.c_double * 3)]
+                ("decay", C.c_double), ("kvec", C.c_double * 3), ("center", C.c_double * 3),
+                ("border", C.c_int32 * 3)]
 
 
 class tf_source_edge(C.Structure):

@@ -45,9 +45,10 @@ def _duration(n_steps, dt, first_step):
 
 def fit_cavity(directory, n=64, step=0.0156, n_steps=2000, n_probes=32, dt=None,
                freq=2e8, carrier="sin", name="fit", eps_r=1.0, sigma=0.0, extra_source="",
-               pec_surfaces="1,3,4,5,6", source_surface=2):
+               pec_surfaces="1,3,4,5,6", source_surface=2, pml=None):
     """FIT box [0, n*step]^3 (xgrid {0,step,n*step+tiny}), one box solid of
-    material 1. Returns the .fdtd path."""
+    material 1; pml = {"+x": width, ...} adds pml boundary conditions. Returns
+    the .fdtd path."""
     os.makedirs(directory, exist_ok=True)
     stop = 1.0 if abs(n * step - 0.9984) < 1e-9 else n * step + 0.25 * step
     solids = os.path.join(directory, f"{name}_solids.txt")
@@ -77,6 +78,9 @@ def fit_cavity(directory, n=64, step=0.0156, n_steps=2000, n_probes=32, dt=None,
         f.write("END mesh 1\n")
         f.write(mat)
         f.write("DEFINE bc 1\n    SET type pec\n    SET surfaces {%s}\nEND bc 1\n" % pec_surfaces)
+        for k, (side, w) in enumerate(sorted((pml or {}).items())):
+            f.write("DEFINE bc %d\n    SET type pml\n    SET direction %s\n    SET width %r\nEND bc %d\n"
+                    % (90 + k, side, w, 90 + k))
         f.write(SOURCE_BLOCK.format(carrier=carrier, freq=repr(freq), extra=extra_source)
                 .replace("SET surface 2", "SET surface %d" % source_surface))
         f.write("DEFINE output 1\n    SET mode probepoint\n    SET name %s\n    SET period 0\n" % name)

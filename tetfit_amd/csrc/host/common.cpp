@@ -89,26 +89,88 @@ double SourceEval::carrier(const SourceDef& s, double t) {
     return ret;
 }
 
+// One mode factor of the wave profile (Source.hpp:362-417): sin / cos of
+// 2 pi k x in long double like the reference (its PI is long double), the
+// GSL Bessel and Legendre forms through the C++17 special functions
+// (J_n and y_l of negative arguments by parity, as GSL evaluates them).
+static long double mode_factor(const std::string& f, double k, double x) {
+    if (f == "sin") return std::sin(2 * PI_L * k * (long double)x);
+    if (f == "cos") return std::cos(2 * PI_L * k * (long double)x);
+    if (f.empty()) return 1.0;
+    const char c = f[0];
+    if (c != 'J' && c != 'Y' && c != 'j' && c != 'y' && c != 'P') return 1.0;
+    const int l = std::stoi(f.substr(1));
+    const double a = k * x;
+    const double sgn = (l % 2 == 0) ? 1.0 : -1.0;
+    switch (c) {
+        case 'J': return a < 0 ? sgn * std::cyl_bessel_j((double)l, -a) : std::cyl_bessel_j((double)l, a);
+        case 'Y': return std::cyl_neumann((double)l, a);
+        case 'j': return a < 0 ? sgn * std::sph_bessel((unsigned)l, -a) : std::sph_bessel((unsigned)l, a);
+        case 'y': return a < 0 ? -sgn * std::sph_neumann((unsigned)l, -a) : std::sph_neumann((unsigned)l, a);
+        default: return std::legendre((unsigned)l, std::cos(a));
+    }
+}
+
 Vec3 SourceEval::space(const SourceDef& s, const Vec3& p) {
     double ret = 1.0;
     if (s.decay != 0)
         ret *= std::pow(std::pow(p[0] - s.center[0], 2) + std::pow(p[1] - s.center[1], 2) +
                             std::pow(p[2] - s.center[2], 2),
                         -0.5 * s.decay);
-    if (s.dir != "x" && s.dir != "y" && s.dir != "z")
-        throw Error("source direction '" + s.dir + "' (curvilinear) is not supported by this build");
-    if (s.prof == "gaussian") {
+    auto gauss = [&]() {
         double e = -((std::pow(p[0] - s.center[0], 2) + std::pow(p[1] - s.center[1], 2) +
                       std::pow(p[2] - s.center[2], 2)) /
                      std::pow(s.radius, 2));
         ret *= std::exp(e);
-    } else if (s.prof == "wave") {
+    };
+    const std::array<double, 3>& c = s.center;
+    if (s.dir == "rho" || s.dir == "theta" || s.dir == "phi") {
+        // spherical coordinates of the point and the centre (Source.hpp:334-349)
+        double r1[3] = {0, 0, 0}, r2[3] = {0, 0, 0};
+        if (p[0] != 0 || p[1] != 0 || p[2] != 0) {
+            r1[0] = std::sqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
+            r1[1] = std::acos(p[2] / r1[0]);
+            r1[2] = std::atan2(p[1], p[0]);
+        }
+        if (c[0] != 0 || c[1] != 0 || c[2] != 0) {
+            r2[0] = std::sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+            r2[1] = std::acos(c[2] / r2[0]);
+            r2[2] = std::atan2(c[1], c[0]);
+        }
+        if (s.prof == "gaussian") gauss();
+        else if (s.prof == "wave")
+            for (int j = 0; j < 3; ++j) ret *= mode_factor(s.bfuncs[j], s.kvec[j], r1[j] - r2[j]);
+        const double st = std::sin(r1[1]), ct = std::cos(r1[1]), sp = std::sin(r1[2]), cp = std::cos(r1[2]);
+        if (s.dir == "rho") return Vec3(st * cp * ret, st * sp * ret, ct * ret);
+        // the reference's theta vector has -r sin(phi) as its z component (Source.hpp:426)
+        if (s.dir == "theta") return Vec3(r1[0] * ct * cp * ret, r1[0] * ct * sp * ret, -r1[0] * sp * ret);
+        return Vec3(-r1[0] * st * sp * ret, r1[0] * st * cp * ret, 0);
+    }
+    if (s.dir == "r" || s.dir == "psi") {
+        // cylindrical coordinates (Source.hpp:430-457); the centre's angle
+        // branch follows the sign of the point's x, as in the reference
+        double r1[3] = {0, 0, p[2]}, r2[3] = {0, 0, c[2]};
+        if (p[0] != 0 || p[1] != 0) {
+            r1[0] = std::sqrt(p[0] * p[0] + p[1] * p[1]);
+            r1[1] = p[0] >= 0 ? std::asin(p[1] / r1[0]) : (double)(-std::asin(p[1] / r1[0]) + PI_L);
+        }
+        if (c[0] != 0 || c[1] != 0 || c[2] != 0) {
+            r2[0] = std::sqrt(c[0] * c[0] + c[1] * c[1]);
+            if (c[0] == 0 && c[1] == 0) r2[1] = 0;
+            else r2[1] = p[0] >= 0 ? std::asin(c[1] / r2[0]) : (double)(-std::asin(c[1] / r2[0]) + PI_L);
+        }
+        if (s.prof == "gaussian") gauss();
+        else if (s.prof == "wave")
+            for (int j = 0; j < 3; ++j) ret *= mode_factor(s.bfuncs[j], s.kvec[j], r1[j] - r2[j]);
+        if (s.dir == "r") return Vec3(std::cos(r1[1]) * ret, std::sin(r1[1]) * ret, 0);
+        return Vec3(-r1[0] * std::sin(r1[1]) * ret, r1[0] * std::cos(r1[1]) * ret, 0);
+    }
+    if (s.prof == "gaussian") gauss();
+    else if (s.prof == "wave") {
         for (int j = 0; j < 3; ++j) {
             const std::string& f = s.bfuncs[j];
             if (f == "sin") ret *= std::sin(2 * PI_L * s.kvec[j] * (long double)(p[j] - s.center[j]));
             else if (f == "cos") ret *= std::cos(2 * PI_L * s.kvec[j] * (long double)(p[j] - s.center[j]));
-            else if (!f.empty() && (f[0] == 'J' || f[0] == 'Y' || f[0] == 'j' || f[0] == 'y' || f[0] == 'P'))
-                throw Error("source mode '" + f + "' needs Bessel/Legendre functions (GSL), not supported");
         }
     }
     if (s.dir == "x") return Vec3(ret, 0, 0);
@@ -149,9 +211,20 @@ tf_source_desc describe_source(const SourceDef& s) {
     d.type = field_kind(s.st);
     d.carrier = carrier_kind(s.carrier);
     d.profile = s.prof == "wave" ? 1 : (s.prof == "gaussian" ? 2 : 0);
-    d.dir = s.dir == "x" ? 0 : (s.dir == "y" ? 1 : 2);
+    static const char* const kDirs[] = {"x", "y", "z", "rho", "theta", "phi", "r", "psi"};
+    d.dir = 2;
+    for (int k = 0; k < 8; ++k)
+        if (s.dir == kDirs[k]) d.dir = k;
     for (int j = 0; j < 3; ++j) {
-        d.bfunc[j] = s.bfuncs[j] == "sin" ? 1 : (s.bfuncs[j] == "cos" ? 2 : 0);
+        const std::string& f = s.bfuncs[j];
+        d.bfunc[j] = 0;
+        d.border[j] = 0;
+        if (f == "sin") d.bfunc[j] = 1;
+        else if (f == "cos") d.bfunc[j] = 2;
+        else if (!f.empty() && std::string("JYjyP").find(f[0]) != std::string::npos) {
+            d.bfunc[j] = 3 + (int32_t)std::string("JYjyP").find(f[0]);
+            d.border[j] = std::stoi(f.substr(1));
+        }
         d.kvec[j] = s.kvec[j];
         d.center[j] = s.center[j];
     }

@@ -52,9 +52,59 @@ void build_fit(const Model& m, const SimulationDef& sim, Build& b) {
     auto mit = m.meshes.find(sim.mesh);
     if (mit == m.meshes.end()) throw Error("simulation refers to an undefined mesh");
     const MeshDef& msh = mit->second;
-    for (auto& kv : m.bcs)
-        if (kv.second.type == "pml") throw Error("PML boundary conditions are not supported by this build");
     std::map<uint32_t, SolidDef> solids = read_solids_file(msh.file);
+    // ---- PML (D:4414-4556): a bc of type pml adds floor(width scale / L)
+    // cells on its side of the grid; the inner box [xmin, xmax] keeps its
+    // place, and every solid gets an extension solid (new labels after the
+    // last, taken in reverse label order) grown by the widths
+    // (Solid::ConstructPMLExtension, Solid.hpp:297-354), written to text at
+    // the stream's default 6 significant digits and read back. Only the +x,
+    // +y, +z sides are supported: the reference's mesher links a cube to its
+    // lower neighbours only above the inner box's lower corner (D:4621-4633),
+    // so layers on the -x, -y, -z sides (and the rho / r / t forms, which
+    // grow every side, or index past a 2-entry width table) come out
+    // disconnected; cylinders with PML trip an assertion on that table.
+    double pml_width[3] = {0, 0, 0};  // +x, +y, +z
+    bool pml_present = false;
+    for (auto& kv : m.bcs) {
+        const BoundaryCondition& bc = kv.second;
+        if (bc.type != "pml") continue;
+        pml_present = true;
+        if (!bc.surfs.empty()) throw Error("a pml boundary condition takes a direction and a width, not surfaces");
+        if (bc.dir == "+x") pml_width[0] = std::fabs(bc.thickness);
+        else if (bc.dir == "+y") pml_width[1] = std::fabs(bc.thickness);
+        else if (bc.dir == "+z") pml_width[2] = std::fabs(bc.thickness);
+        else
+            throw Error("pml direction '" + bc.dir + "' is not supported (only +x, +y, +z: the reference's mesh "
+                        "is disconnected in layers on the lower sides, D:4621-4633)");
+    }
+    if (pml_present) {
+        auto g6 = [](double v) {
+            char buf[64];
+            std::snprintf(buf, sizeof buf, "%g", v);
+            return std::stod(buf);
+        };
+        uint32_t label = solids.empty() ? 0 : solids.rbegin()->first;
+        std::vector<std::pair<uint32_t, SolidDef>> ext;
+        for (auto it = solids.rbegin(); it != solids.rend(); ++it) {
+            SolidDef s = it->second;
+            if (s.type == "cylinder")
+                throw Error("cylinder solids with PML are not supported (the reference asserts on a 2-entry "
+                            "width table, D:4416 with Solid.hpp:335)");
+            if (s.type == "box") {
+                for (int a = 0; a < 3; ++a) {
+                    s.corner[a] = g6(s.corner[a]);
+                    s.size[a] = g6(s.size[a] + pml_width[a]);
+                }
+            } else if (s.type == "sphere") {
+                for (int a = 0; a < 3; ++a) s.center[a] = g6(s.center[a]);
+                const double r = g6(std::sqrt(s.squareradius));
+                s.squareradius = r * r;
+            }
+            ext.push_back({++label, s});
+        }
+        for (auto& e : ext) solids[e.first] = e.second;
+    }
     std::map<uint32_t, Material> materials = m.materials;  // operator[] may insert defaults
     std::map<uint32_t, BoundaryCondition> bcs = m.bcs;
     std::map<uint32_t, SourceDef> sources = m.sources;
@@ -80,9 +130,12 @@ void build_fit(const Model& m, const SimulationDef& sim, Build& b) {
     const double xmax = scale * msh.xmax, ymax = scale * msh.ymax, zmax = scale * msh.zmax;
     double Lx = scale * msh.xstep, Ly = scale * msh.ystep, Lz = scale * msh.zstep;
     for (auto& r : m.refinements) { Lx /= r.second.x; Ly /= r.second.y; Lz /= r.second.z; }
-    const uint32_t Nx = (uint32_t)(std::fabs(xmax - xmin) / Lx);
-    const uint32_t Ny = (uint32_t)(std::fabs(ymax - ymin) / Ly);
-    const uint32_t Nz = (uint32_t)(std::fabs(zmax - zmin) / Lz);
+    const uint32_t Npml[3] = {(uint32_t)(pml_width[0] * scale / Lx), (uint32_t)(pml_width[1] * scale / Ly),
+                              (uint32_t)(pml_width[2] * scale / Lz)};
+    const uint32_t Nx = (uint32_t)(std::fabs(xmax - xmin) / Lx) + Npml[0];
+    const uint32_t Ny = (uint32_t)(std::fabs(ymax - ymin) / Ly) + Npml[1];
+    const uint32_t Nz = (uint32_t)(std::fabs(zmax - zmin) / Lz) + Npml[2];
+    const double true_max[3] = {xmax + double(Npml[0]) * Lx, ymax + double(Npml[1]) * Ly, zmax + double(Npml[2]) * Lz};
     if (Nx == 0 || Ny == 0 || Nz == 0) throw Error("cartesian grid has no cells");
     const uint64_t totE = (uint64_t)Nx * (Ny + 1) * (Nz + 1) + (uint64_t)(Nx + 1) * Ny * (Nz + 1) +
                           (uint64_t)(Nx + 1) * (Ny + 1) * Nz;
@@ -445,6 +498,25 @@ void build_fit(const Model& m, const SimulationDef& sim, Build& b) {
         for (int q = 0; q < 8; ++q) bc = bc + P(P_cluster[(size_t)v * 8 + q]);
         return bc / 8.0;
     };
+    // PML factors (D:5519-5557 faces, D:5609-5647 edges): beyond the inner
+    // box on axis a, cf *= exp(-(d / width_a)^3 * 4 / 2 / 150 / pi / L_a) with
+    // d the distance past the box and width_a = true_max - max, evaluated
+    // in long double like the reference (its PI is long double)
+    if (pml_present) {
+        const long double PI_L = 3.141592653589793238L;
+        const double inner[3] = {xmax, ymax, zmax};
+        auto pml = [&](const Vec3& c) {
+            double cf = 1;
+            for (int a = 0; a < 3; ++a)
+                if (Npml[a] > 0 && c[a] > inner[a]) {
+                    const double d = std::fabs(inner[a] - c[a]) / std::fabs(inner[a] - true_max[a]);
+                    cf *= std::exp(std::pow(d, 3) * 4 / 2 / 150 / PI_L / len[a] * -1.0L);
+                }
+            return cf;
+        };
+        for (uint32_t f = 0; f < totF; ++f) b.fpml[f] = pml(face_bar(f));
+        for (uint32_t e = 0; e < totE; ++e) b.upml[e] = pml(edge_bar(e));
+    }
 
     // ---- probes (FindFitProbe, D:3590-3643) and j dipoles (D:447-492)
     set_output(m, sim, b);

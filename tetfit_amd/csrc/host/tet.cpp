@@ -7,6 +7,7 @@
 // ConstructGFDTDMaterialMatrices (D:8132-8692, method gfdtd), FindProbe
 // (D:3645-3747) and the Whitney / face probe weights (D:3768-3897).
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -14,6 +15,7 @@
 #include <fstream>
 #include <numeric>
 #include <sstream>
+#include <type_traits>
 
 #include "build.hpp"
 
@@ -21,21 +23,26 @@ namespace tfh {
 
 namespace {
 
+std::vector<char> read_file(const std::string& path) {
+    std::ifstream f(path, std::ios::binary);
+    if (!f.is_open()) throw Error("cannot open mesh file " + path);
+    f.seekg(0, std::ios::end);
+    size_t n = (size_t)f.tellg();
+    f.seekg(0);
+    std::vector<char> buf(n + 1);
+    f.read(buf.data(), (std::streamsize)n);
+    buf[n] = 0;
+    return buf;
+}
+
 struct Reader {
     std::vector<char> buf;
     const char* p = nullptr;
     const char* end = nullptr;
-    explicit Reader(const std::string& path) {
-        std::ifstream f(path, std::ios::binary);
-        if (!f.is_open()) throw Error("cannot open mesh file " + path);
-        f.seekg(0, std::ios::end);
-        size_t n = (size_t)f.tellg();
-        f.seekg(0);
-        buf.resize(n + 1);
-        f.read(buf.data(), (std::streamsize)n);
-        buf[n] = 0;
+    explicit Reader(std::vector<char> text) : buf(std::move(text)) {
+        if (buf.empty() || buf.back() != 0) buf.push_back(0);
         p = buf.data();
-        end = p + n;
+        end = p + buf.size() - 1;
     }
     double num() {
         char* e;
@@ -52,6 +59,86 @@ struct Reader {
         return v;
     }
 };
+
+// ConvertFromGMSH (D:5692-5866): a gmsh 2.2 ASCII mesh turned into the
+// netgen text the reader takes, line for line as the reference writes its
+// temporary .mesh file: after the 4 header lines the node count and the
+// nodes (id x y z, written back as "x y z" at the stream's default 6
+// significant digits), 2 more lines, the element count and the elements
+// (number, type, tag count, tags, nodes; read_element_line, Utilities.hpp:
+// 159-223 takes the first two numbers after the type as tags, the first of
+// them the label): tetrahedra (type 4) as "label n0 n1 n2 n3" with the
+// nodes sorted and the tetrahedra sorted by node tuple, triangles (type 2)
+// as "label n0 n1 n2" in file order.
+std::vector<char> convert_gmsh(const std::string& path) {
+    std::vector<char> in = read_file(path);
+    const char* p = in.data();
+    auto next_line = [&]() {
+        while (*p && *p != '\n') ++p;
+        if (*p) ++p;
+    };
+    auto num = [&](auto& v) {
+        char* e;
+        if constexpr (std::is_floating_point<std::decay_t<decltype(v)>>::value) v = std::strtod(p, &e);
+        else v = (std::decay_t<decltype(v)>)std::strtoll(p, &e, 10);
+        if (e == p) throw Error("malformed gmsh mesh file " + path);
+        p = e;
+    };
+    for (int i = 0; i < 4; ++i) next_line();
+    uint32_t npts = 0;
+    num(npts);
+    std::string out = std::to_string(npts) + "\n";
+    next_line();
+    char buf[128];
+    for (uint32_t i = 0; i < npts; ++i) {
+        double id, x, y, z;
+        num(id); num(x); num(y); num(z);
+        std::snprintf(buf, sizeof buf, "%g %g %g\n", x, y, z);
+        out += buf;
+        next_line();
+    }
+    for (int i = 0; i < 2; ++i) next_line();
+    uint32_t nel = 0;
+    num(nel);
+    struct TetL { uint32_t n[4]; uint32_t lab; };
+    std::vector<TetL> tets;
+    std::vector<std::array<uint32_t, 4>> tris;  // label, n0, n1, n2
+    for (uint32_t i = 0; i < nel; ++i) {
+        next_line();
+        int64_t number, type;
+        num(number); num(type);
+        if (type == 4 || type == 2) {
+            int64_t ntags, tag1, tag2;
+            num(ntags); num(tag1); num(tag2);
+            const int nn = type == 4 ? 4 : 3;
+            int64_t v[4];
+            for (int k = 0; k < nn; ++k) num(v[k]);
+            if (type == 4) {
+                TetL t;
+                for (int k = 0; k < 4; ++k) t.n[k] = (uint32_t)v[k];
+                std::sort(t.n, t.n + 4);
+                t.lab = (uint32_t)tag1;
+                tets.push_back(t);
+            } else {
+                tris.push_back({(uint32_t)tag1, (uint32_t)v[0], (uint32_t)v[1], (uint32_t)v[2]});
+            }
+        }
+    }
+    std::stable_sort(tets.begin(), tets.end(), [](const TetL& a, const TetL& c) {
+        return std::lexicographical_compare(a.n, a.n + 4, c.n, c.n + 4);
+    });
+    out += std::to_string(tets.size()) + "\n";
+    for (auto& t : tets) {
+        std::snprintf(buf, sizeof buf, "%u %u %u %u %u\n", t.lab, t.n[0], t.n[1], t.n[2], t.n[3]);
+        out += buf;
+    }
+    out += std::to_string(tris.size()) + "\n";
+    for (auto& t : tris) {
+        std::snprintf(buf, sizeof buf, "%u %u %u %u\n", t[0], t[1], t[2], t[3]);
+        out += buf;
+    }
+    return std::vector<char>(out.begin(), out.end());
+}
 
 struct Tri { uint32_t a, b, c, slot; };
 struct Ed { uint32_t a, b, slot; };
@@ -158,7 +245,7 @@ void build_tet(const Model& m, const SimulationDef& sim, Build& b) {
     auto mit = m.meshes.find(sim.mesh);
     if (mit == m.meshes.end()) throw Error("simulation refers to an undefined mesh");
     const MeshDef& msh = mit->second;
-    if (msh.mesher == "gmsh") throw Error("gmsh meshes are not supported by this build (write netgen neutral)");
+
     for (auto& kv : m.bcs)
         if (kv.second.type == "pml") throw Error("PML boundary conditions are not supported by this build");
     std::map<uint32_t, Material> materials = m.materials;
@@ -169,7 +256,7 @@ void build_tet(const Model& m, const SimulationDef& sim, Build& b) {
     const double scale = msh.scale;
 
     PhaseClock clk;
-    Reader rd(msh.file);
+    Reader rd(msh.mesher == "gmsh" ? convert_gmsh(msh.file) : read_file(msh.file));
     // ---- points
     const uint64_t np = rd.uint();
     std::vector<Vec3> pts(np);
