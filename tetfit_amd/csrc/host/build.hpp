@@ -43,6 +43,21 @@ struct Triplets {
     CsrStore to_csr(int32_t nrows, int32_t ncols, bool overwrite = false) const;
 };
 
+// One (column, value) contribution to a row under assembly.
+struct RowEntry {
+    int32_t col;
+    double val;
+};
+
+// A CSR matrix assembled row by row on all host threads. `row(r, out)`
+// appends the contributions of row r in summation order; equal columns are
+// summed in that order from +0.0, so the result is bitwise the one of a
+// sequential loop that adds the same contributions in the same per-entry
+// order (zero contributions still enter the pattern). Exceptions thrown by
+// `row` are rethrown on the calling thread.
+CsrStore assemble_rows(int32_t nrows, int32_t ncols,
+                       const std::function<void(int32_t, std::vector<RowEntry>&)>& row);
+
 struct Build {
     tf_problem prob{};
     std::string method;

@@ -1,10 +1,12 @@
 // common.cpp - CSR assembly helpers, source evaluation split, dt power
 // iteration and output schedule shared by the FIT and tetrahedral builders.
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <cmath>
+#include <mutex>
 #include <numeric>
 
 #include "build.hpp"
@@ -236,6 +238,72 @@ tf_source_desc describe_source(const SourceDef& s) {
     d.radius = s.radius;
     d.decay = s.decay;
     return d;
+}
+
+// ---------------------------------------------------------------- rows
+
+CsrStore assemble_rows(int32_t nrows, int32_t ncols,
+                       const std::function<void(int32_t, std::vector<RowEntry>&)>& row) {
+    constexpr int32_t kBlk = 4096;
+    const int32_t nb = (nrows + kBlk - 1) / kBlk;
+    std::vector<std::vector<int32_t>> bcol(nb);
+    std::vector<std::vector<double>> bval(nb);
+    CsrStore m;
+    m.nrows = nrows;
+    m.ncols = ncols;
+    m.rowptr.assign((size_t)nrows + 1, 0);
+    std::atomic<bool> failed{false};
+    std::string what;
+    std::mutex mu;
+#pragma omp parallel
+    {
+        std::vector<RowEntry> ent;
+#pragma omp for schedule(dynamic, 1)
+        for (int32_t blk = 0; blk < nb; ++blk) {
+            if (failed.load(std::memory_order_relaxed)) continue;
+            try {
+                const int32_t r0 = blk * kBlk, r1 = std::min(nrows, r0 + kBlk);
+                for (int32_t r = r0; r < r1; ++r) {
+                    ent.clear();
+                    row(r, ent);
+                    // stable insertion sort by column: equal columns keep their order
+                    for (size_t i = 1; i < ent.size(); ++i) {
+                        RowEntry x = ent[i];
+                        size_t j = i;
+                        while (j > 0 && ent[j - 1].col > x.col) { ent[j] = ent[j - 1]; --j; }
+                        ent[j] = x;
+                    }
+                    int64_t cnt = 0;
+                    for (size_t i = 0; i < ent.size();) {
+                        double s = 0.0;
+                        size_t j = i;
+                        for (; j < ent.size() && ent[j].col == ent[i].col; ++j) s += ent[j].val;
+                        bcol[blk].push_back(ent[i].col);
+                        bval[blk].push_back(s);
+                        ++cnt;
+                        i = j;
+                    }
+                    m.rowptr[(size_t)r + 1] = cnt;
+                }
+            } catch (const std::exception& e) {
+                std::lock_guard<std::mutex> g(mu);
+                if (!failed.exchange(true)) what = e.what();
+            }
+        }
+    }
+    if (failed) throw Error(what);
+    for (int32_t r = 0; r < nrows; ++r) m.rowptr[(size_t)r + 1] += m.rowptr[r];
+    m.col.resize((size_t)m.rowptr[nrows]);
+    m.val.resize((size_t)m.rowptr[nrows]);
+#pragma omp parallel for schedule(static)
+    for (int32_t blk = 0; blk < nb; ++blk) {
+        const int64_t at = m.rowptr[(size_t)blk * kBlk];
+        std::copy(bcol[blk].begin(), bcol[blk].end(), m.col.begin() + at);
+        std::copy(bval[blk].begin(), bval[blk].end(), m.val.begin() + at);
+        std::vector<int32_t>().swap(bcol[blk]);
+        std::vector<double>().swap(bval[blk]);
+    }
+    return m;
 }
 
 // ---------------------------------------------------------------- dt

@@ -8,6 +8,7 @@
 // (D:3645-3747) and the Whitney / face probe weights (D:3768-3897).
 #include <algorithm>
 #include <array>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -205,31 +206,6 @@ bool spd_inverse(const std::vector<double>& A, int n, std::vector<double>& inv) 
         for (int i = 0; i < n; ++i) inv[i * n + c] = x[i];
     }
     return true;
-}
-
-// CSR pattern from (row, col) pairs: rows sorted, duplicates removed.
-CsrStore pattern_from_pairs(int32_t nrows, int32_t ncols, const std::vector<int32_t>& pr,
-                            const std::vector<int32_t>& pc) {
-    CsrStore m;
-    m.nrows = nrows;
-    m.ncols = ncols;
-    std::vector<int64_t> cnt(nrows + 1, 0);
-    for (int32_t r : pr) cnt[r + 1]++;
-    for (int32_t i = 0; i < nrows; ++i) cnt[i + 1] += cnt[i];
-    std::vector<int32_t> cols(pr.size());
-    std::vector<int64_t> pos(cnt.begin(), cnt.end() - 1);
-    for (size_t k = 0; k < pr.size(); ++k) cols[pos[pr[k]]++] = pc[k];
-    m.rowptr.assign(nrows + 1, 0);
-    m.col.reserve(pr.size() / 2);
-    for (int32_t i = 0; i < nrows; ++i) {
-        auto b = cols.begin() + cnt[i], e = cols.begin() + cnt[i + 1];
-        std::sort(b, e);
-        auto u = std::unique(b, e);
-        m.col.insert(m.col.end(), b, u);
-        m.rowptr[i + 1] = (int64_t)m.col.size();
-    }
-    m.val.assign(m.col.size(), 0.0);
-    return m;
 }
 
 inline double& csr_at(CsrStore& m, int32_t r, int32_t c) {
@@ -708,28 +684,39 @@ void build_tet(const Model& m, const SimulationDef& sim, Build& b) {
         M[11] = (-d(2, 3) - d(1, 4)) * coeff;
         M[4] = M[1]; M[8] = M[2]; M[12] = M[3]; M[9] = M[6]; M[13] = M[7]; M[14] = M[11];
     };
-    auto local_M = [&](uint32_t t, double M[16]) { local_W(t, materials[b.vol_material[t]].mu, M); };
+    // (map::at: the lambdas below also run on host threads)
+    auto local_M = [&](uint32_t t, double M[16]) { local_W(t, materials.at(b.vol_material[t]).mu, M); };
 
     clk.mark("tet edges");
-    // ---- N = sum over tets of local_M^-1 (rows n_index = face id for class-1 faces)
+    // ---- N = sum over tets of local_M^-1 (rows n_index = face id for class-1
+    // faces). The per-tet inverses on all threads, then each face row sums
+    // its (at most two) tets' rows in increasing tet order: the order of the
+    // sequential loop over tets.
     {
-        std::vector<int32_t> pr, pc;
-        pr.reserve((size_t)T * 16);
-        pc.reserve((size_t)T * 16);
-        for (uint32_t t = 0; t < T; ++t)
-            for (int j = 0; j < 4; ++j)
-                for (int k = 0; k < 4; ++k) { pr.push_back(b.vtf[(size_t)t * 4 + j]); pc.push_back(b.vtf[(size_t)t * 4 + k]); }
-        b.N = pattern_from_pairs((int32_t)F, (int32_t)F, pr, pc);
-        std::vector<double> Mv(16), inv;
-        double M[16];
-        for (uint32_t t = 0; t < T; ++t) {
-            local_M(t, M);
-            Mv.assign(M, M + 16);
-            if (!spd_inverse(Mv, 4, inv)) throw Error("local mu block not positive definite (degenerate tet)");
-            for (int j = 0; j < 4; ++j)
-                for (int k = 0; k < 4; ++k)
-                    if (inv[j * 4 + k] != 0) csr_at(b.N, b.vtf[(size_t)t * 4 + j], b.vtf[(size_t)t * 4 + k]) += inv[j * 4 + k];
+        std::vector<double> ninv((size_t)T * 16);
+        std::atomic<bool> bad{false};
+#pragma omp parallel
+        {
+            std::vector<double> Mv(16), inv;
+            double M[16];
+#pragma omp for schedule(static)
+            for (int64_t t = 0; t < (int64_t)T; ++t) {
+                local_M((uint32_t)t, M);
+                Mv.assign(M, M + 16);
+                if (!spd_inverse(Mv, 4, inv)) { bad = true; continue; }
+                std::copy(inv.begin(), inv.end(), ninv.begin() + t * 16);
+            }
         }
+        if (bad) throw Error("local mu block not positive definite (degenerate tet)");
+        b.N = assemble_rows((int32_t)F, (int32_t)F, [&](int32_t f, std::vector<RowEntry>& out) {
+            for (int q = 0; q < ftv_n[f]; ++q) {
+                const int32_t t = ftv_t[(size_t)f * 2 + q];
+                const int32_t* tf = &b.vtf[(size_t)t * 4];
+                int j = 0;
+                while (tf[j] != f) ++j;
+                for (int k = 0; k < 4; ++k) out.push_back({tf[k], ninv[(size_t)t * 16 + j * 4 + k]});
+            }
+        });
     }
 
     clk.mark("N");
@@ -758,101 +745,124 @@ void build_tet(const Model& m, const SimulationDef& sim, Build& b) {
             est_size = 0;
             for (uint32_t e = 0; e < E; ++e) est_row[e] = b.edge_class[e] != 4 ? est_size++ : -1;
         }
-        // H row pattern of a row edge (a,b): star(a) U star(b) without class-4 edges.
-        std::vector<int32_t> pr, pc;
-        {
-            CsrStore& H = b.H;
-            H.nrows = est_size;
-            H.ncols = (int32_t)E;
-            H.rowptr.assign(est_size + 1, 0);
-            std::vector<int32_t> row;
-            for (uint32_t e = 0; e < E; ++e) {
-                if (est_row[e] < 0) continue;
-                row.clear();
-                for (int s = 0; s < 2; ++s) {
-                    uint32_t n = en[2 * (size_t)e + s];
-                    for (uint32_t q = nte_ptr[n]; q < nte_ptr[n + 1]; ++q)
-                        if (b.edge_class[nte[q]] != 4) row.push_back((int32_t)nte[q]);
-                }
-                std::sort(row.begin(), row.end());
-                row.erase(std::unique(row.begin(), row.end()), row.end());
-                H.col.insert(H.col.end(), row.begin(), row.end());
-                H.rowptr[est_row[e] + 1] = (int64_t)H.col.size();
+        // Node-star corner term of tet t at its local node k: the dual basis
+        // f_i of the corner's three edges (D:8913-9078).
+        auto star_corner = [&](uint32_t t, int k, Vec3 f[3]) {
+            Vec3 fa[4];
+            primal_area_vectors(t, fa);
+            const double vol = std::fabs(b.cell_volumes[t]);
+            for (int i = 0; i < 3; ++i) {
+                int face = kCornerFaces[k][i];
+                double sg = vtf_sgn[(size_t)t * 4 + face];
+                f[i] = fa[face] * (kCornerSign[k][i] * 2.0 / 3.0 / vol * sg);
             }
-            H.val.assign(H.col.size(), 0.0);
-        }
+            return vol;
+        };
         // E: the node-star eps mass blocks summed without the class-4
         // identity rows (E_fracs assembled into E_trip, D:9435-9445), the
-        // matrix of the energy diagnostic We = 1/2 U.E U (D:2773)
-        {
-            std::vector<int32_t> er, ec;
-            er.reserve((size_t)T * 36);
-            ec.reserve((size_t)T * 36);
-            for (uint32_t t = 0; t < T; ++t)
-                for (int k = 0; k < 4; ++k)
-                    for (int i = 0; i < 3; ++i)
-                        for (int j = 0; j < 3; ++j) {
-                            er.push_back(tet_edges[(size_t)t * 6 + kCornerEdges[k][i]]);
-                            ec.push_back(tet_edges[(size_t)t * 6 + kCornerEdges[k][j]]);
-                        }
-            b.E = pattern_from_pairs((int32_t)E, (int32_t)E, er, ec);
-        }
-        std::vector<double> LE, LH;
-        Vec3 fa[4];
+        // matrix of the energy diagnostic We = 1/2 U.E U (D:2773). Row e =
+        // (a, b) collects the corners at a, then at b, of the tets holding e
+        // (each in associated-volume order): per entry the order of the
+        // reference's loop over node stars.
+        b.E = assemble_rows((int32_t)E, (int32_t)E, [&](int32_t row, std::vector<RowEntry>& out) {
+            for (int s = 0; s < 2; ++s) {
+                const uint32_t nid = en[2 * (size_t)row + s], other = en[2 * (size_t)row + 1 - s];
+                for (uint32_t q = av_cnt[nid]; q < av_cnt[nid + 1]; ++q) {
+                    const uint32_t t = av[q];
+                    const int32_t* n = &b.vols[(size_t)t * 4];
+                    if (n[0] != (int32_t)other && n[1] != (int32_t)other && n[2] != (int32_t)other &&
+                        n[3] != (int32_t)other)
+                        continue;
+                    int k = 0;
+                    while (n[k] != (int32_t)nid) ++k;
+                    Vec3 f[3];
+                    const double vol = star_corner(t, k, f);
+                    const Mat3& eps = materials.at(b.vol_material[t]).epsilon;
+                    int32_t e[3];
+                    int i0 = 0;
+                    for (int i = 0; i < 3; ++i) {
+                        e[i] = tet_edges[(size_t)t * 6 + kCornerEdges[k][i]];
+                        if (e[i] == row) i0 = i;
+                    }
+                    for (int j = 0; j < 3; ++j) out.push_back({e[j], dot(f[i0], eps * f[j]) * vol / 4});
+                }
+            }
+        });
+        // Node-star blocks LE, their inverses LH (class-4 rows/cols set to the
+        // identity for the inversion and zeroed after, D:9270-9333), one star
+        // per task; each kept at lh_off[node] (deg x deg)
+        std::vector<int64_t> lh_off(np + 1, 0);
         for (uint32_t nid = 0; nid < np; ++nid) {
-            const uint32_t deg = nte_ptr[nid + 1] - nte_ptr[nid];
-            if (deg == 0) continue;
-            const uint32_t* gi = &nte[nte_ptr[nid]];
-            LE.assign((size_t)deg * deg, 0.0);
-            auto loc = [&](int32_t e) { return (uint32_t)(std::lower_bound(gi, gi + deg, (uint32_t)e) - gi); };
-            for (uint32_t q = av_cnt[nid]; q < av_cnt[nid + 1]; ++q) {
-                const uint32_t t = av[q];
-                const int32_t* n = &b.vols[(size_t)t * 4];
-                int k = 0;
-                while (n[k] != (int32_t)nid) ++k;
-                primal_area_vectors(t, fa);
-                const double vol = std::fabs(b.cell_volumes[t]);
-                const Mat3& eps = materials[b.vol_material[t]].epsilon;
-                Vec3 f[3];
-                uint32_t le[3];
-                for (int i = 0; i < 3; ++i) {
-                    int face = kCornerFaces[k][i];
-                    double sg = vtf_sgn[(size_t)t * 4 + face];
-                    f[i] = fa[face] * (kCornerSign[k][i] * 2.0 / 3.0 / vol * sg);
-                    le[i] = loc(tet_edges[(size_t)t * 6 + kCornerEdges[k][i]]);
+            const int64_t deg = nte_ptr[nid + 1] - nte_ptr[nid];
+            lh_off[nid + 1] = lh_off[nid] + deg * deg;
+        }
+        std::vector<double> LHall((size_t)lh_off[np]);
+        std::atomic<bool> bad{false};
+#pragma omp parallel
+        {
+            std::vector<double> LE, LH;
+#pragma omp for schedule(dynamic, 256)
+            for (int64_t nid = 0; nid < (int64_t)np; ++nid) {
+                const uint32_t deg = nte_ptr[nid + 1] - nte_ptr[nid];
+                if (deg == 0) continue;
+                const uint32_t* gi = &nte[nte_ptr[nid]];
+                LE.assign((size_t)deg * deg, 0.0);
+                auto loc = [&](int32_t e) { return (uint32_t)(std::lower_bound(gi, gi + deg, (uint32_t)e) - gi); };
+                for (uint32_t q = av_cnt[nid]; q < av_cnt[nid + 1]; ++q) {
+                    const uint32_t t = av[q];
+                    const int32_t* n = &b.vols[(size_t)t * 4];
+                    int k = 0;
+                    while (n[k] != (int32_t)nid) ++k;
+                    Vec3 f[3];
+                    const double vol = star_corner(t, k, f);
+                    const Mat3& eps = materials.at(b.vol_material[t]).epsilon;
+                    uint32_t le[3];
+                    for (int i = 0; i < 3; ++i) le[i] = loc(tet_edges[(size_t)t * 6 + kCornerEdges[k][i]]);
+                    for (int i = 0; i < 3; ++i) LE[(size_t)le[i] * deg + le[i]] += dot(f[i], eps * f[i]) * vol / 4;
+                    const int pairs[3][2] = {{0, 1}, {0, 2}, {1, 2}};
+                    for (auto& pq : pairs) {
+                        double v = dot(f[pq[0]], eps * f[pq[1]]) * vol / 4;
+                        double& a = LE[(size_t)le[pq[0]] * deg + le[pq[1]]];
+                        a = a + v;
+                        LE[(size_t)le[pq[1]] * deg + le[pq[0]]] = a;
+                    }
                 }
-                for (int i = 0; i < 3; ++i) LE[(size_t)le[i] * deg + le[i]] += dot(f[i], eps * f[i]) * vol / 4;
-                const int pairs[3][2] = {{0, 1}, {0, 2}, {1, 2}};
-                for (auto& pq : pairs) {
-                    double v = dot(f[pq[0]], eps * f[pq[1]]) * vol / 4;
-                    double& a = LE[(size_t)le[pq[0]] * deg + le[pq[1]]];
-                    a = a + v;
-                    LE[(size_t)le[pq[1]] * deg + le[pq[0]]] = a;
-                }
-                for (int i = 0; i < 3; ++i)
-                    for (int j = 0; j < 3; ++j)
-                        csr_at(b.E, (int32_t)gi[le[i]], (int32_t)gi[le[j]]) += dot(f[i], eps * f[j]) * vol / 4;
-            }
-            // class-4 rows/cols -> identity, invert, zero them again (D:9270-9333)
-            for (uint32_t j = 0; j < deg; ++j)
-                if (b.edge_class[gi[j]] == 4) {
-                    for (uint32_t c = 0; c < deg; ++c) LE[(size_t)j * deg + c] = LE[(size_t)c * deg + j] = 0;
-                    LE[(size_t)j * deg + j] = 1;
-                }
-            if (!spd_inverse(LE, (int)deg, LH)) throw Error("node-star eps block not positive definite");
-            for (uint32_t j = 0; j < deg; ++j)
-                if (b.edge_class[gi[j]] == 4)
-                    for (uint32_t c = 0; c < deg; ++c) LH[(size_t)j * deg + c] = LH[(size_t)c * deg + j] = 0;
-            for (uint32_t jj = 0; jj < deg; ++jj) {
-                if (est_row[gi[jj]] < 0) continue;
-                for (uint32_t kk = jj; kk < deg; ++kk) {
-                    if (b.edge_class[gi[kk]] == 4 || LH[(size_t)jj * deg + kk] == 0) continue;
-                    const double v = LH[(size_t)jj * deg + kk];
-                    csr_at(b.H, est_row[gi[jj]], (int32_t)gi[kk]) += v;
-                    if (jj != kk) csr_at(b.H, est_row[gi[kk]], (int32_t)gi[jj]) += v;
-                }
+                for (uint32_t j = 0; j < deg; ++j)
+                    if (b.edge_class[gi[j]] == 4) {
+                        for (uint32_t c = 0; c < deg; ++c) LE[(size_t)j * deg + c] = LE[(size_t)c * deg + j] = 0;
+                        LE[(size_t)j * deg + j] = 1;
+                    }
+                if (!spd_inverse(LE, (int)deg, LH)) { bad = true; continue; }
+                for (uint32_t j = 0; j < deg; ++j)
+                    if (b.edge_class[gi[j]] == 4)
+                        for (uint32_t c = 0; c < deg; ++c) LH[(size_t)j * deg + c] = LH[(size_t)c * deg + j] = 0;
+                std::copy(LH.begin(), LH.end(), LHall.begin() + lh_off[nid]);
             }
         }
+        if (bad) throw Error("node-star eps block not positive definite");
+        // H (rows: est_row; D:9345-9371 pushes each star's upper triangle and
+        // its mirror, so entry (j, k) of a star reads LH[min(j,k), max(j,k)]).
+        // Row e = (a, b) sums the stars of a, then b, over the non-class-4
+        // edges of each: the row pattern star(a) U star(b) without class 4.
+        std::vector<int32_t> est_edge(est_size);
+        for (uint32_t e = 0; e < E; ++e)
+            if (est_row[e] >= 0) est_edge[est_row[e]] = (int32_t)e;
+        b.H = assemble_rows(est_size, (int32_t)E, [&](int32_t r, std::vector<RowEntry>& out) {
+            const int32_t row = est_edge[r];
+            for (int s = 0; s < 2; ++s) {
+                const uint32_t nid = en[2 * (size_t)row + s];
+                const uint32_t deg = nte_ptr[nid + 1] - nte_ptr[nid];
+                const uint32_t* gi = &nte[nte_ptr[nid]];
+                const uint32_t jj = (uint32_t)(std::lower_bound(gi, gi + deg, (uint32_t)row) - gi);
+                const double* LH = &LHall[lh_off[nid]];
+                for (uint32_t kk = 0; kk < deg; ++kk) {
+                    if (b.edge_class[gi[kk]] == 4) continue;
+                    const uint32_t lo = std::min(jj, kk), hi = std::max(jj, kk);
+                    out.push_back({(int32_t)gi[kk], LH[(size_t)lo * deg + hi]});
+                }
+            }
+        });
+        std::vector<double>().swap(LHall);
     clk.mark("H (node stars)");
         // ---- dt: power iteration on H * Ct * N * C (ComputeDGATimeStep D:7121)
         {
@@ -968,54 +978,55 @@ void build_tet(const Model& m, const SimulationDef& sim, Build& b) {
     }
     if (!is_dga) {
         if (!is_fem) {
-        // ---- gfdtd: E from the tet-corner dual basis (D:8379-8420), A, RHSmat1
-        std::vector<int32_t> pr, pc;
-        pr.reserve((size_t)T * 36);
-        pc.reserve((size_t)T * 36);
-        for (uint32_t t = 0; t < T; ++t)
-            for (int k = 0; k < 4; ++k)
-                for (int i = 0; i < 3; ++i)
-                    for (int j = 0; j < 3; ++j) {
-                        pr.push_back(tet_edges[(size_t)t * 6 + kCornerEdges[k][i]]);
-                        pc.push_back(tet_edges[(size_t)t * 6 + kCornerEdges[k][j]]);
-                    }
-        b.E = pattern_from_pairs((int32_t)E, (int32_t)E, pr, pc);
-        std::vector<int32_t>().swap(pr);
-        std::vector<int32_t>().swap(pc);
+        // ---- gfdtd: E from the tet-corner dual basis (D:8379-8420), A, RHSmat1.
+        // Row e = (a, b) collects the corners at a, then at b, of the tets
+        // holding e (each in associated-volume order): per entry the order of
+        // the reference's loop over nodes and their tets. The corner's 3x3
+        // block is pushed in the order of D:8420-8429, which touches each
+        // entry once.
         // SigMat: the same corner terms with sigma, from the tets whose sigma
-        // tensor is invertible (D:8600-8612); kept on E's pattern
+        // tensor is invertible (D:8600-8612); kept on E's pattern (the other
+        // corners enter it as zeros)
         bool lossy = false;
         for (uint32_t t = 0; t < T && !lossy; ++t) lossy = materials[b.vol_material[t]].sigma.det() != 0;
-        if (lossy) {
-            b.Sig = b.E;
-            std::fill(b.Sig.val.begin(), b.Sig.val.end(), 0.0);
-        }
-        for (uint32_t nid = 0; nid < np; ++nid) {
-            for (uint32_t q = av_cnt[nid]; q < av_cnt[nid + 1]; ++q) {
-                const uint32_t t = av[q];
-                const int32_t* n = &b.vols[(size_t)t * 4];
-                int k = 0;
-                while (n[k] != (int32_t)nid) ++k;
-                const double omega = std::fabs(b.cell_volumes[t]) / double(4);
-                const Mat3& eps = materials[b.vol_material[t]].epsilon;
-                int32_t e[3];
-                for (int i = 0; i < 3; ++i) e[i] = tet_edges[(size_t)t * 6 + kCornerEdges[k][i]];
-                Vec3 a1 = edge_bars[e[0]] - P(nid), a2 = edge_bars[e[1]] - P(nid), a3 = edge_bars[e[2]] - P(nid);
-                double nc = std::pow(dot(a1, cross(a2, a3)), -1);
-                Vec3 f[3] = {nc * cross(a2, a3), nc * cross(a3, a1), nc * cross(a1, a2)};
-                Mat3 oe;
-                for (int z = 0; z < 9; ++z) oe.m[z] = omega * eps.m[z];
-                // push order of D:8420-8429: (1,1),(2,2),(3,3),(1,2),(2,1),(1,3),(3,1),(2,3),(3,2)
-                const int order[9][2] = {{0, 0}, {1, 1}, {2, 2}, {0, 1}, {1, 0}, {0, 2}, {2, 0}, {1, 2}, {2, 1}};
-                for (auto& o : order) csr_at(b.E, e[o[0]], e[o[1]]) += dot(f[o[0]], oe * f[o[1]]);
-                const Mat3& sig = materials[b.vol_material[t]].sigma;
-                if (lossy && sig.det() != 0) {
-                    Mat3 os;
-                    for (int z = 0; z < 9; ++z) os.m[z] = omega * sig.m[z];
-                    for (auto& o : order) csr_at(b.Sig, e[o[0]], e[o[1]]) += dot(f[o[0]], os * f[o[1]]);
+        auto corner_rows = [&](bool sigma) {
+            return [&, sigma](int32_t row, std::vector<RowEntry>& out) {
+                for (int s = 0; s < 2; ++s) {
+                    const uint32_t nid = en[2 * (size_t)row + s], other = en[2 * (size_t)row + 1 - s];
+                    for (uint32_t q = av_cnt[nid]; q < av_cnt[nid + 1]; ++q) {
+                        const uint32_t t = av[q];
+                        const int32_t* n = &b.vols[(size_t)t * 4];
+                        if (n[0] != (int32_t)other && n[1] != (int32_t)other && n[2] != (int32_t)other &&
+                            n[3] != (int32_t)other)
+                            continue;
+                        int k = 0;
+                        while (n[k] != (int32_t)nid) ++k;
+                        const Material& mt = materials.at(b.vol_material[t]);
+                        const bool use = !sigma || mt.sigma.det() != 0;
+                        int32_t e[3];
+                        int i0 = 0;
+                        for (int i = 0; i < 3; ++i) {
+                            e[i] = tet_edges[(size_t)t * 6 + kCornerEdges[k][i]];
+                            if (e[i] == row) i0 = i;
+                        }
+                        if (!use) {
+                            for (int j = 0; j < 3; ++j) out.push_back({e[j], 0.0});
+                            continue;
+                        }
+                        const double omega = std::fabs(b.cell_volumes[t]) / double(4);
+                        const Mat3& w = sigma ? mt.sigma : mt.epsilon;
+                        Vec3 a1 = edge_bars[e[0]] - P(nid), a2 = edge_bars[e[1]] - P(nid), a3 = edge_bars[e[2]] - P(nid);
+                        double nc = std::pow(dot(a1, cross(a2, a3)), -1);
+                        Vec3 f[3] = {nc * cross(a2, a3), nc * cross(a3, a1), nc * cross(a1, a2)};
+                        Mat3 ow;
+                        for (int z = 0; z < 9; ++z) ow.m[z] = omega * w.m[z];
+                        for (int j = 0; j < 3; ++j) out.push_back({e[j], dot(f[i0], ow * f[j])});
+                    }
                 }
-            }
-        }
+            };
+        };
+        b.E = assemble_rows((int32_t)E, (int32_t)E, corner_rows(false));
+        if (lossy) b.Sig = assemble_rows((int32_t)E, (int32_t)E, corner_rows(true));
     clk.mark("E (corner terms)");
         }
         // free (non-Dirichlet) edges: compressed_dirichlet (D:8145-8171;
