@@ -178,8 +178,10 @@ tf_status tf_destroy(tf_ctx* ctx);
 
 /* Autotuner: times the context's SELL kernels (Ct, H, N for dga; N, the
  * Jacobi pass's A product and, once the learned preconditioner is active,
- * its two passes (G and A products) for gfdtd) over grid sizes and gather
- * batch depths and keeps the fastest for later tf_run calls. Every row is
+ * its two passes (G and A products) for gfdtd) over grid sizes, gather
+ * batch depths and slice-to-wave mappings (band 0: one contiguous chunk of
+ * slices per workgroup; 1: an XCD's waves take its slices round-robin) and
+ * keeps the fastest for later tf_run calls. Every row is
  * summed by one lane in entry order for each choice, so explicit fields are
  * identical; the CG's dot products are summed per workgroup, so a different
  * grid changes their rounding (gfdtd / fem fields agree to rounding). Choices are cached per (kernel, precision, rows, entries) in the
@@ -187,10 +189,12 @@ tf_status tf_destroy(tf_ctx* ctx);
  * counterpart. tf_get_tuning reports the choice per kernel (order:
  * sell_ct, sell_h, sell_n, cg_w, cg_pg, cg_wu; blocks 0 = default). */
 tf_status tf_autotune(tf_ctx* ctx);
-tf_status tf_get_tuning(tf_ctx* ctx, int32_t* blocks, int32_t* unroll, int32_t cap, int32_t* n);
+tf_status tf_get_tuning(tf_ctx* ctx, int32_t* blocks, int32_t* unroll, int32_t* band, int32_t cap, int32_t* n);
 /* Sets the launch shape of the first n kernels in tf_get_tuning's order
- * (blocks 0 = default grid; unroll 4, 8 or 16), e.g. to replay a choice. */
-tf_status tf_set_tuning(tf_ctx* ctx, const int32_t* blocks, const int32_t* unroll, int32_t n);
+ * (blocks 0 = default grid; unroll 4, 8 or 16; band 0 or 1, null = all 0),
+ * e.g. to replay a choice. */
+tf_status tf_set_tuning(tf_ctx* ctx, const int32_t* blocks, const int32_t* unroll, const int32_t* band,
+                        int32_t n);
 
 /* Storage of one device operator, for byte accounting: which = 0 Ct (dga),
  * 1 H, 2 N, 3 A (gfdtd free rows), 4 K = E_f - [A 0] (gfdtd free rows: the

@@ -204,10 +204,10 @@ def hip_lib():
     L.tf_batch_read_probes.argtypes = [vp, C.c_int32, f64p, f64p, C.c_int64, i64p]
     L.tf_batch_read_state.argtypes = [vp, C.c_int32, f64p, f64p, f64p]
     L.tf_autotune.argtypes = [vp]
-    L.tf_get_tuning.argtypes = [vp, i32p, i32p, C.c_int32, i32p]
+    L.tf_get_tuning.argtypes = [vp, i32p, i32p, i32p, C.c_int32, i32p]
     L.tf_sell_info.argtypes = [vp, C.c_int32, i64p]
     L.tf_fit_layout.argtypes = [vp, i32p]
-    L.tf_set_tuning.argtypes = [vp, i32p, i32p, C.c_int32]
+    L.tf_set_tuning.argtypes = [vp, i32p, i32p, i32p, C.c_int32]
     L.tf_set_graphs.argtypes = [vp, C.c_int32]
     for n in ["tf_create", "tf_set_sources", "tf_set_probes", "tf_set_output", "tf_run",
               "tf_read_state", "tf_write_state", "tf_read_probes", "tf_read_poynting", "tf_read_losses",

@@ -103,14 +103,17 @@ struct Sell {
 };
 
 // Launch shape of the tuned SELL kernels: grid size (0 = one slice per
-// wave) and entries per gather batch (4, 8 or 16). Results do not depend on
-// either (each row is summed by one lane in column order).
+// wave), entries per gather batch (4, 8 or 16) and the slice-to-wave mapping
+// (band 0: one contiguous chunk per workgroup; 1: the waves of an XCD take
+// its slices round-robin, block_slices). Results do not depend on any of
+// them (each row is summed by one lane in column order).
 enum TuneKernel { TK_CT = 0, TK_H, TK_N, TK_W, TK_G, TK_WU, TK_COUNT };
 const char* const kTuneName[TK_COUNT] = {"sell_ct_kernel", "sell_h_kernel", "sell_n_kernel", "cg_w_kernel",
                                          "cg_pg_kernel", "cg_wu_kernel"};
 struct KTune {
     int blocks = 0;
     int unroll = 8;
+    int band = 0;
 };
 
 // SELL kernels take the gather batch NB and W, the wide-mesh instantiation
@@ -554,6 +557,19 @@ void select_rows(const tf_csr& m, const int32_t* rows, int32_t n, const int32_t*
 int sell_blocks(const Sell& m, int cap) {
     const int need = (m.nslices + kBlock / kWave - 1) / (kBlock / kWave);
     return std::max(1, std::min(need, cap));
+}
+
+// An operator's view with the slice mapping of launch shape t (a matrix
+// banded by construction, batched dga, stays banded); tuned_view: with the
+// context's tuned shape of kernel k.
+template <class V>
+SellView<V> with_band(SellView<V> v, const KTune& t) {
+    v.band = v.band || t.band ? 1 : 0;
+    return v;
+}
+template <class V>
+SellView<V> tuned_view(const tf_ctx* c, int k, SellView<V> v) {
+    return with_band(v, c->tune[k]);
 }
 
 int tuned_blocks(const tf_ctx* c, int k, const Sell& m, int cap) {
@@ -1410,8 +1426,8 @@ void run_dga_frac(tf_ctx* c, int64_t first, int64_t n, tf_stats* st) {
         const double t = (double)i * c->dt;
         const double* cs = c->carr.as<double>() + (k + 1) * ns;
         double* poy = c->poy.as<double>() + 2 * k * inj_blocks;
-        TF_LAUNCH_TIMED(c, "sell_ct_kernel", c->tune[TK_CT].unroll, sell_ct_kernel, T, gb_ct, c->sCt.signs(),
-                        c->Fv.as<T>(), c->curl_f.as<T>());
+        TF_LAUNCH_TIMED(c, "sell_ct_kernel", c->tune[TK_CT].unroll, sell_ct_kernel, T, gb_ct,
+                        tuned_view(c, TK_CT, c->sCt.signs()), c->Fv.as<T>(), c->curl_f.as<T>());
         inject(cs, poy, 0);
         const int nx = c->fcur ^ 1;
         if (c->nU > 0)
@@ -1482,16 +1498,17 @@ void run_dga(tf_ctx* c, int64_t first, int64_t n, tf_stats* st) {
     // Poynting, curl_u = C U with B update, F -= dt N curl_u; poy holds the
     // step's two Poynting partial blocks
     auto step = [&](const double* cs, double* poy) {
-        TF_LAUNCH_TIMED(c, "sell_ct_kernel", c->tune[TK_CT].unroll, sell_ct_kernel, T, gb_ct, c->sCt.signs(), c->Fv.as<T>(),
-                     c->curl_f.as<T>());
+        TF_LAUNCH_TIMED(c, "sell_ct_kernel", c->tune[TK_CT].unroll, sell_ct_kernel, T, gb_ct,
+                        tuned_view(c, TK_CT, c->sCt.signs()), c->Fv.as<T>(), c->curl_f.as<T>());
         if (c->n_inj) {
             launch_timed(c, "dga_inject_kernel", dga_inject_kernel<T>, inj_blocks, 
                 c->n_inj, c->inj_edge.as<int32_t>(), c->inj_kind.as<int8_t>(), c->inj_tb.as<int32_t>(),
                 c->inj_te.as<int32_t>(), c->term_src.as<int32_t>(), c->term_w.as<double>(), cs, c->curl_f.as<T>(),
                 c->U.as<T>(), poy, 0);
         }
-        TF_LAUNCH_TIMED(c, "sell_h_kernel", c->tune[TK_H].unroll, sell_h_kernel, T, gb_h, c->H.vals<T>(), c->h_edge.as<int32_t>(),
-                     c->curl_f.as<T>(), c->U.as<T>(), c->dt, hdt);
+        TF_LAUNCH_TIMED(c, "sell_h_kernel", c->tune[TK_H].unroll, sell_h_kernel, T, gb_h,
+                        tuned_view(c, TK_H, c->H.vals<T>()), c->h_edge.as<int32_t>(), c->curl_f.as<T>(), c->U.as<T>(),
+                        c->dt, hdt);
         if (c->n_inj) {
             launch_timed(c, "dga_poynting_kernel", dga_poynting_kernel<T>, inj_blocks, 
                 c->n_inj, c->inj_edge.as<int32_t>(), c->inj_kind.as<int8_t>(), c->inj_tb.as<int32_t>(),
@@ -1501,8 +1518,8 @@ void run_dga(tf_ctx* c, int64_t first, int64_t n, tf_stats* st) {
         launch_timed(c, "dga_curl_kernel", dga_curl_kernel<T>, blocks_for(c->F), c->F, c->c_col.as<int32_t>(),
                      c->c_fixed_signs ? nullptr : c->c_sg.as<int8_t>(), c->U.as<T>(), c->curl_u.as<T>(),
                      c->B.as<T>(), c->dt, fdt, (const CgState*)nullptr);
-        TF_LAUNCH_TIMED(c, "sell_n_kernel", c->tune[TK_N].unroll, sell_n_kernel, T, gb_n, c->N.vals<T>(), c->curl_u.as<T>(),
-                     c->Fv.as<T>(), c->dt, 0, fdt, nullptr);
+        TF_LAUNCH_TIMED(c, "sell_n_kernel", c->tune[TK_N].unroll, sell_n_kernel, T, gb_n,
+                        tuned_view(c, TK_N, c->N.vals<T>()), c->curl_u.as<T>(), c->Fv.as<T>(), c->dt, 0, fdt, nullptr);
     };
     const int64_t g = graph_steps(c) ? run_graph_chunks(c, n, 2 * inj_blocks, [&](int, const double* cs, double* poy) {
         step(cs, poy);
@@ -1579,8 +1596,8 @@ struct CgView {
         rj = c->r.as<T>();
         rs[0] = c->rs0.as<RS<T>>();
         rs[1] = c->rs1.as<RS<T>>();
-        A = c->A.vals<T>();
-        G = c->A.view<uint16_t>(c->lp.gval);
+        A = tuned_view(c, learned ? TK_WU : TK_W, c->A.vals<T>());
+        G = tuned_view(c, TK_G, c->A.view<uint16_t>(c->lp.gval));
         gpk = c->lp.gpk.p ? c->lp.gpk.as<uint32_t>() : nullptr;
     }
 };
@@ -2178,8 +2195,8 @@ void run_gfdtd(tf_ctx* c, int64_t first, int64_t n, const tf_cg_opts* cg, tf_sta
                      c->c_fixed_signs ? nullptr : c->c_sg.as<int8_t>(), c->U.as<T>(), c->curl_u.as<T>(),
                      c->B.as<T>(), c->dt, (const double*)nullptr, guard);
         // gfdtd: F = N B (D:1468); fem: N C U_old for the curl-curl term (D:1729)
-        TF_LAUNCH_TIMED(c, "sell_n_kernel", c->tune[TK_N].unroll, sell_n_kernel, T, gb_n, c->N.vals<T>(),
-                        c->fem ? c->curl_u.as<T>() : c->B.as<T>(), c->Fv.as<T>(), c->dt, 1, nullptr, guard);
+        TF_LAUNCH_TIMED(c, "sell_n_kernel", c->tune[TK_N].unroll, sell_n_kernel, T, gb_n,
+                        tuned_view(c, TK_N, c->N.vals<T>()), c->fem ? c->curl_u.as<T>() : c->B.as<T>(), c->Fv.as<T>(), c->dt, 1, nullptr, guard);
         if (fresh) g.sample_slot = allow_print(c, t - c->dt) ? sample_reserve(c, t - c->dt) : -1;
         sample_launch<T>(c, g.sample_slot, guard);
         if (lossy) {
@@ -2570,8 +2587,8 @@ void tune_cache_load() {
     FILE* f = std::fopen(path, "r");
     if (!f) return;
     char key[256];
-    int b = 0, u = 0;
-    while (std::fscanf(f, "%255s %d %d", key, &b, &u) == 3) g_tune_cache[key] = KTune{b, u};
+    int b = 0, u = 0, band = 0;
+    while (std::fscanf(f, "%255s %d %d %d", key, &b, &u, &band) == 4) g_tune_cache[key] = KTune{b, u, band};
     std::fclose(f);
 }
 
@@ -2581,7 +2598,7 @@ void tune_cache_store(const std::string& key, const KTune& t) {
     if (!path) return;
     FILE* f = std::fopen(path, "a");
     if (!f) return;
-    std::fprintf(f, "%s %d %d\n", key.c_str(), t.blocks, t.unroll);
+    std::fprintf(f, "%s %d %d %d\n", key.c_str(), t.blocks, t.unroll, t.band);
     std::fclose(f);
 }
 
@@ -2600,20 +2617,22 @@ KTune tune_one(tf_ctx* c, int k, const Sell& m, int cap, L launch) {
     hipEvent_t e0, e1;
     TF_HIP(hipEventCreate(&e0));
     TF_HIP(hipEventCreate(&e1));
-    KTune best{sell_blocks(m, cap), 8};
+    KTune best{sell_blocks(m, cap), 8, 0};
     float best_ms = INFINITY;
     for (int b : cand)
-        for (int u : {4, 8, 16}) {
-            launch(b, u);
-            launch(b, u);
-            TF_HIP(hipEventRecord(e0, c->stream));
-            for (int r = 0; r < 5; ++r) launch(b, u);
-            TF_HIP(hipEventRecord(e1, c->stream));
-            TF_HIP(hipEventSynchronize(e1));
-            float ms = 0;
-            TF_HIP(hipEventElapsedTime(&ms, e0, e1));
-            if (ms < best_ms) { best_ms = ms; best = KTune{b, u}; }
-        }
+        for (int u : {4, 8, 16})
+            for (int band : {0, 1}) {
+                const KTune t{b, u, band};
+                launch(t);
+                launch(t);
+                TF_HIP(hipEventRecord(e0, c->stream));
+                for (int r = 0; r < 5; ++r) launch(t);
+                TF_HIP(hipEventRecord(e1, c->stream));
+                TF_HIP(hipEventSynchronize(e1));
+                float ms = 0;
+                TF_HIP(hipEventElapsedTime(&ms, e0, e1));
+                if (ms < best_ms) { best_ms = ms; best = t; }
+            }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     TF_HIP(hipGetLastError());
@@ -2630,26 +2649,29 @@ template <class T>
 void autotune(tf_ctx* c) {
     hipStream_t s = c->stream;
     if (c->method == TF_METHOD_DGA) {
-        c->tune[TK_CT] = tune_one(c, TK_CT, c->sCt, kSpmvCap, [&](int b, int u) {
-            TF_LAUNCH_NB(c->wide, u, sell_ct_kernel, T, b, s, c->sCt.signs(), c->Fv.as<T>(), c->curl_f.as<T>());
+        c->tune[TK_CT] = tune_one(c, TK_CT, c->sCt, kSpmvCap, [&](const KTune& t) {
+            TF_LAUNCH_NB(c->wide, t.unroll, sell_ct_kernel, T, t.blocks, s, with_band(c->sCt.signs(), t),
+                         c->Fv.as<T>(), c->curl_f.as<T>());
         });
         if (c->frac) return;  // lossy media: only sell_ct of the tuned kernels runs
-        c->tune[TK_H] = tune_one(c, TK_H, c->H, kSpmvCap, [&](int b, int u) {
-            TF_LAUNCH_NB(c->wide, u, sell_h_kernel, T, b, s, c->H.vals<T>(), c->h_edge.as<int32_t>(), c->curl_f.as<T>(),
-                         c->U.as<T>(), 0.0, nullptr);
+        c->tune[TK_H] = tune_one(c, TK_H, c->H, kSpmvCap, [&](const KTune& t) {
+            TF_LAUNCH_NB(c->wide, t.unroll, sell_h_kernel, T, t.blocks, s, with_band(c->H.vals<T>(), t),
+                         c->h_edge.as<int32_t>(), c->curl_f.as<T>(), c->U.as<T>(), 0.0, nullptr);
         });
-        c->tune[TK_N] = tune_one(c, TK_N, c->N, kSpmvCap, [&](int b, int u) {
-            TF_LAUNCH_NB(c->wide, u, sell_n_kernel, T, b, s, c->N.vals<T>(), c->curl_u.as<T>(), c->Fv.as<T>(), 0.0, 0, nullptr,
-                         nullptr);
+        c->tune[TK_N] = tune_one(c, TK_N, c->N, kSpmvCap, [&](const KTune& t) {
+            TF_LAUNCH_NB(c->wide, t.unroll, sell_n_kernel, T, t.blocks, s, with_band(c->N.vals<T>(), t),
+                         c->curl_u.as<T>(), c->Fv.as<T>(), 0.0, 0, nullptr, nullptr);
         });
     } else if (c->method == TF_METHOD_GFDTD || c->method == TF_METHOD_FEM) {
-        c->tune[TK_N] = tune_one(c, TK_N, c->N, kSpmvCap, [&](int b, int u) {
-            TF_LAUNCH_NB(c->wide, u, sell_n_kernel, T, b, s, c->N.vals<T>(), c->B.as<T>(), c->Fv.as<T>(), c->dt, 1, nullptr,
-                         nullptr);
+        c->tune[TK_N] = tune_one(c, TK_N, c->N, kSpmvCap, [&](const KTune& t) {
+            TF_LAUNCH_NB(c->wide, t.unroll, sell_n_kernel, T, t.blocks, s, with_band(c->N.vals<T>(), t), c->B.as<T>(),
+                         c->Fv.as<T>(), c->dt, 1, nullptr, nullptr);
         });
         // Tuning launches run pass 1 (the full work of a pass) on a scratch
-        // CG state that never stops (tol 0, no iteration cap) with unit
-        // partials. Their outputs go to vectors every solve rewrites before
+        // CG state that never stops (tol 0, no iteration cap, rr0 = 1: the
+        // ratio test sees a non-zero residual) with unit partials; that it
+        // did not stop is checked after the last launch, since timings of
+        // launches past a stop would be of empty passes. Their outputs go to vectors every solve rewrites before
         // use: u (z), w (q, also standing in for x), r, p (p0), s (p1) and
         // the (r, s) pairs (rs0, rs1). A pass kernel's grid sets how many
         // partials the next kernel sums, so each candidate is timed together
@@ -2657,6 +2679,7 @@ void autotune(tf_ctx* c) {
         CgState sc{};
         sc.tol = 0.0;
         sc.maxiter = INT32_MAX;
+        sc.rr0 = 1.0;
         sc.gam[0] = sc.gam[1] = sc.alpha[0] = sc.alpha[1] = 1.0;
         DevBuf scratch, ones;
         upload(scratch, std::vector<CgState>{sc});
@@ -2667,39 +2690,44 @@ void autotune(tf_ctx* c) {
         const int32_t nf = c->n_free;
         const int nbb = std::max(1, std::min(blocks_for(nf), kCgPartB));  // cg_upd's grid (CgView)
         // Jacobi pass: cg_w (stop test on cg_upd's partials) + the cg_upd that sums its b partials
-        c->tune[TK_W] = tune_one(c, TK_W, c->A, kCgPartA, [&](int b, int u) {
-            TF_LAUNCH_NB(c->wide, u, cg_w_kernel, T, b, s, 1, c->A.vals<T>(), c->z.as<T>(), c->q.as<T>(), 1, one, nbb, part,
-                         st);
+        c->tune[TK_W] = tune_one(c, TK_W, c->A, kCgPartA, [&](const KTune& t) {
+            TF_LAUNCH_NB(c->wide, t.unroll, cg_w_kernel, T, t.blocks, s, 1, with_band(c->A.vals<T>(), t), c->z.as<T>(),
+                         c->q.as<T>(), 1, one, nbb, part, st);
             cg_upd_kernel<T><<<nbb, kBlock, 0, s>>>(1, nf, c->pre.as<T>(), c->q.as<T>(), c->r.as<T>(), c->z.as<T>(),
-                                                    c->q.as<T>(), c->p0.as<T>(), c->p1.as<T>(), one, nbb, one, b,
-                                                    part, part, st);
+                                                    c->q.as<T>(), c->p0.as<T>(), c->p1.as<T>(), one, nbb, one,
+                                                    t.blocks, part, part, st);
         });
         if (c->lp.state == LearnedPc::kActive) {
             const SellView<uint16_t> gv = c->A.view<uint16_t>(c->lp.gval);
             const uint32_t* gpk = c->lp.gpk.p ? c->lp.gpk.as<uint32_t>() : nullptr;
-            auto pg = [&](int b, int u, int nparts) {
-                TF_LAUNCH_NB(c->wide, u, cg_pg_kernel, T, b, s, 1, gv, gpk, c->rs0.as<RS<T>>(), c->rs1.as<RS<T>>(),
-                             c->p0.as<T>(), c->q.as<T>(), c->z.as<T>(), one, nparts, part, part, st);
+            auto pg = [&](const KTune& t, int nparts) {
+                TF_LAUNCH_NB(c->wide, t.unroll, cg_pg_kernel, T, t.blocks, s, 1, with_band(gv, t), gpk,
+                             c->rs0.as<RS<T>>(), c->rs1.as<RS<T>>(), c->p0.as<T>(), c->q.as<T>(), c->z.as<T>(), one,
+                             nparts, part, part, st);
             };
-            auto wu = [&](int b, int u, int nparts) {
-                TF_LAUNCH_NB(c->wide, u, cg_wu_kernel, T, b, s, 1, c->A.vals<T>(), c->z.as<T>(), c->p0.as<T>(),
-                             c->rs1.as<RS<T>>(), one, one, nparts, part, st);
+            auto wu = [&](const KTune& t, int nparts) {
+                TF_LAUNCH_NB(c->wide, t.unroll, cg_wu_kernel, T, t.blocks, s, 1, with_band(c->A.vals<T>(), t),
+                             c->z.as<T>(), c->p0.as<T>(), c->rs1.as<RS<T>>(), one, one, nparts, part, st);
             };
             // cg_pg (summing cg_wu's partials) + the cg_wu that sums its b partials, then
             // cg_wu (summing the chosen cg_pg's partials) + the cg_pg that sums its b
-            const int wb0 = tuned_blocks(c, TK_WU, c->A, kCgPartA);
-            const int wu0 = c->tune[TK_WU].unroll > 0 ? c->tune[TK_WU].unroll : 8;
-            c->tune[TK_G] = tune_one(c, TK_G, c->A, kCgPartC, [&](int b, int u) {
-                pg(b, u, wb0);
-                wu(wb0, wu0, b);
+            KTune W0 = c->tune[TK_WU];
+            W0.blocks = tuned_blocks(c, TK_WU, c->A, kCgPartA);
+            c->tune[TK_G] = tune_one(c, TK_G, c->A, kCgPartC, [&](const KTune& t) {
+                pg(t, W0.blocks);
+                wu(W0, t.blocks);
             });
-            const KTune G = c->tune[TK_G];
-            c->tune[TK_WU] = tune_one(c, TK_WU, c->A, kCgPartA, [&](int b, int u) {
-                wu(b, u, G.blocks);
-                pg(G.blocks, G.unroll, b);
+            KTune G = c->tune[TK_G];
+            G.blocks = tuned_blocks(c, TK_G, c->A, kCgPartC);
+            c->tune[TK_WU] = tune_one(c, TK_WU, c->A, kCgPartA, [&](const KTune& t) {
+                wu(t, G.blocks);
+                pg(G, t.blocks);
             });
         }
+        CgState back{};
+        TF_HIP(hipMemcpyAsync(&back, st, sizeof(CgState), hipMemcpyDeviceToHost, s));
         TF_HIP(hipStreamSynchronize(s));
+        if (back.done) throw std::runtime_error("autotune: the scratch CG state stopped during tuning");
     }
 }
 
@@ -3153,25 +3181,26 @@ tf_status tf_set_graphs(tf_ctx* c, int32_t on) {
     return TF_OK;
 }
 
-tf_status tf_set_tuning(tf_ctx* c, const int32_t* blocks, const int32_t* unroll, int32_t n) {
+tf_status tf_set_tuning(tf_ctx* c, const int32_t* blocks, const int32_t* unroll, const int32_t* band, int32_t n) {
     if (!c || !blocks || !unroll || n < 0 || n > TK_COUNT) { g_err = "bad argument"; return TF_ERR_ARG; }
-    drop_graph(c);
-    for (int k = 0; k < n; ++k) {
-        if (blocks[k] < 0 || (unroll[k] != 4 && unroll[k] != 8 && unroll[k] != 16)) {
-            g_err = "tuning: blocks >= 0 (0 = default), unroll 4, 8 or 16";
+    for (int k = 0; k < n; ++k)
+        if (blocks[k] < 0 || (unroll[k] != 4 && unroll[k] != 8 && unroll[k] != 16) ||
+            (band && band[k] != 0 && band[k] != 1)) {
+            g_err = "tuning: blocks >= 0 (0 = default), unroll 4, 8 or 16, band 0 or 1";
             return TF_ERR_ARG;
         }
-        c->tune[k] = KTune{blocks[k], unroll[k]};
-    }
+    drop_graph(c);
+    for (int k = 0; k < n; ++k) c->tune[k] = KTune{blocks[k], unroll[k], band ? band[k] : 0};
     return TF_OK;
 }
 
-tf_status tf_get_tuning(tf_ctx* c, int32_t* blocks, int32_t* unroll, int32_t cap, int32_t* n) {
+tf_status tf_get_tuning(tf_ctx* c, int32_t* blocks, int32_t* unroll, int32_t* band, int32_t cap, int32_t* n) {
     if (!c || !n) { g_err = "null argument"; return TF_ERR_ARG; }
     *n = TK_COUNT;
     for (int k = 0; k < std::min<int32_t>(cap, TK_COUNT); ++k) {
         if (blocks) blocks[k] = c->tune[k].blocks;
         if (unroll) unroll[k] = c->tune[k].unroll;
+        if (band) band[k] = c->tune[k].band;
     }
     return TF_OK;
 }

@@ -142,29 +142,34 @@ class Simulation:
         _check(hip_lib().tf_device_bytes(self._h, C.byref(b)))
         return b.value
 
+    TUNED = ["sell_ct_kernel", "sell_h_kernel", "sell_n_kernel", "cg_w_kernel", "cg_pg_kernel", "cg_wu_kernel"]
+
+    def tuning(self):
+        """tf_get_tuning: {kernel: (blocks, unroll, band)} (blocks 0 = default grid)."""
+        bl, un, bd, n = (C.c_int32 * 8)(), (C.c_int32 * 8)(), (C.c_int32 * 8)(), C.c_int32()
+        _check(hip_lib().tf_get_tuning(self._h, bl, un, bd, 8, C.byref(n)))
+        return {self.TUNED[k]: (bl[k], un[k], bd[k]) for k in range(min(n.value, len(self.TUNED)))}
+
     def autotune(self):
-        """tf_autotune; returns {kernel: (blocks, unroll)}."""
-        L = hip_lib()
-        _check(L.tf_autotune(self._h))
-        bl, un, n = (C.c_int32 * 8)(), (C.c_int32 * 8)(), C.c_int32()
-        _check(L.tf_get_tuning(self._h, bl, un, 8, C.byref(n)))
-        names = ["sell_ct_kernel", "sell_h_kernel", "sell_n_kernel", "cg_w_kernel", "cg_pg_kernel", "cg_wu_kernel"]
-        return {names[k]: (bl[k], un[k]) for k in range(min(n.value, len(names)))}
+        """tf_autotune; returns {kernel: (blocks, unroll, band)}."""
+        _check(hip_lib().tf_autotune(self._h))
+        return self.tuning()
 
     def set_graphs(self, on):
         """tf_set_graphs: step fdtd / dga in HIP graphs when nothing is sampled (default on)."""
         _check(hip_lib().tf_set_graphs(self._h, int(bool(on))))
 
     def set_tuning(self, choice):
-        """tf_set_tuning from {kernel: (blocks, unroll)} (kernels as in autotune())."""
-        names = ["sell_ct_kernel", "sell_h_kernel", "sell_n_kernel", "cg_w_kernel", "cg_pg_kernel", "cg_wu_kernel"]
-        cur = {}
-        bl, un, n = (C.c_int32 * 8)(), (C.c_int32 * 8)(), C.c_int32()
-        _check(hip_lib().tf_get_tuning(self._h, bl, un, 8, C.byref(n)))
-        for k in range(len(names)):
-            b, u = choice.get(names[k], (bl[k], un[k]))
-            bl[k], un[k] = int(b), int(u)
-        _check(hip_lib().tf_set_tuning(self._h, bl, un, len(names)))
+        """tf_set_tuning from {kernel: (blocks, unroll[, band])} (kernels as in
+        autotune(); kernels not named keep their shape)."""
+        cur = self.tuning()
+        bl, un, bd = (C.c_int32 * 8)(), (C.c_int32 * 8)(), (C.c_int32 * 8)()
+        for k, name in enumerate(self.TUNED):
+            t = tuple(choice.get(name, cur[name]))
+            if len(t) == 2:
+                t = t + (cur[name][2],)
+            bl[k], un[k], bd[k] = (int(v) for v in t)
+        _check(hip_lib().tf_set_tuning(self._h, bl, un, bd, len(self.TUNED)))
 
     def fit_grid_order(self):
         """tf_fit_layout: True when the FIT fields sit in grid order."""
@@ -250,6 +255,11 @@ class BatchSimulation:
 
     def kernel_times(self):
         return Simulation.kernel_times(self)
+
+    TUNED = Simulation.TUNED
+
+    def tuning(self):
+        return Simulation.tuning(self)
 
     def autotune(self):
         return Simulation.autotune(self)
