@@ -243,6 +243,21 @@ tf_status tf_batch_read_state(tf_ctx* ctx, int32_t inst, double* U, double* F, d
 tf_status tf_mlp_grad(const float* params, const float* x, const float* gy,
                       int64_t n, float* y, float* grad);
 
+/* tf_mlp_grad without any device allocation, for callers that cannot
+ * allocate (code under HIP stream capture, serving loops, harnesses that
+ * forbid hipMalloc). No reference counterpart. All pointers are device
+ * memory owned by the caller: params (TF_MLP_PARAMS floats), x (n x
+ * TF_MLP_IN bf16 bit patterns), gy (n), y (n, out), grad (TF_MLP_PARAMS,
+ * out); work happens on `stream` (a hipStream_t, null = the null stream)
+ * and is not waited for. y equals tf_mlp_grad's bit for bit; grad sums the
+ * same per-row terms over another partition (fp32 rounding differs). The
+ * work buffers live in one fixed device array per GPU (about 2.7 MB, kept
+ * until the process ends) whatever n is; n above 2^31 rows is rejected.
+ * Calls that overlap on one GPU (other streams, other threads) share that
+ * array and race; calls in stream order do not. */
+tf_status tf_mlp_grad_noalloc(const float* params, const uint16_t* x, const float* gy,
+                              int64_t n, float* y, float* grad, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -209,12 +209,15 @@ def hip_lib():
     L.tf_fit_layout.argtypes = [vp, i32p]
     L.tf_set_tuning.argtypes = [vp, i32p, i32p, i32p, C.c_int32]
     L.tf_set_graphs.argtypes = [vp, C.c_int32]
+    # device pointers (integers such as torch's data_ptr()) and a hipStream_t
+    L.tf_mlp_grad_noalloc.argtypes = [vp, vp, vp, C.c_int64, vp, vp, vp]
     for n in ["tf_create", "tf_set_sources", "tf_set_probes", "tf_set_output", "tf_run",
               "tf_read_state", "tf_write_state", "tf_read_probes", "tf_read_poynting", "tf_read_losses",
               "tf_energy", "tf_kernel_times", "tf_set_profiling", "tf_device_bytes", "tf_destroy",
               "tf_mlp_grad", "tf_create_batch", "tf_batch_set_sources", "tf_batch_set_probes",
               "tf_batch_set_output", "tf_batch_read_probes", "tf_batch_read_state", "tf_autotune",
-              "tf_get_tuning", "tf_sell_info", "tf_set_tuning", "tf_set_graphs", "tf_fit_layout"]:
+              "tf_get_tuning", "tf_sell_info", "tf_set_tuning", "tf_set_graphs", "tf_fit_layout",
+              "tf_mlp_grad_noalloc"]:
         getattr(L, n).restype = C.c_int32
     _hip = L
     return L

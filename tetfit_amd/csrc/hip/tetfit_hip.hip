@@ -3300,3 +3300,5 @@ tf_status tf_batch_read_state(tf_ctx* c, int32_t inst, double* U, double* F, dou
 }
 
 }  // extern "C"
+
+#include "noalloc.hpp"
