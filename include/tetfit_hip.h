@@ -155,6 +155,19 @@ tf_status tf_read_losses(tf_ctx* ctx, double* vals, int64_t cap, int64_t* n);
  * (batches). */
 tf_status tf_energy(tf_ctx* ctx, double* we, double* wh);
 
+/* Energy time series (the discrete-energy part of the l2norm output,
+ * ExportFields D:2250-2284 / D:2773-2775, ExportFitFields D:3468-3470):
+ * with energy output on, every output sample (the probe schedule of
+ * tf_set_output, AllowPrint) also records We and Wh of the state the probes
+ * see, with the reductions of tf_energy (the same values bit for bit).
+ * Switching it resets the sample series. TF_ERR_STATE for contexts without
+ * the energy matrices. The Poynting flux and losses columns come from
+ * tf_read_poynting / tf_read_losses (one value per step). */
+tf_status tf_set_energy_output(tf_ctx* ctx, int32_t on);
+/* times[k] and vals[2k], vals[2k+1] = (We, Wh) of sample k, k < min(cap, n);
+ * n = samples so far. TF_ERR_STATE when energy output is off. */
+tf_status tf_read_energy(tf_ctx* ctx, double* times, double* vals, int64_t cap, int64_t* n);
+
 /* Average duration of each kernel launched by the last tf_run, measured with
  * HIP events on the context's stream: names[i] (NUL-separated, packed into
  * `names` of capacity cap_names), ms_avg[i], launches[i]. */

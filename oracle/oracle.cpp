@@ -197,14 +197,48 @@ struct Printer {
     }
 };
 
+// 1/2 x.(M x) over a CSR matrix, rows in order (the energy diagnostic).
+double half_quad(const tf_csr& m, const std::vector<double>& x) {
+    double s = 0;
+    for (int32_t r = 0; r < m.nrows; ++r) {
+        double mx = 0;
+        for (int64_t q = m.rowptr[r]; q < m.rowptr[r + 1]; ++q) mx += m.val[q] * x[m.col[q]];
+        s += x[r] * mx;
+    }
+    return 0.5 * s;
+}
+
+double half_weighted(const double* w, const std::vector<double>& x) {
+    double s = 0;
+    for (size_t i = 0; i < x.size(); ++i) s += w[i] * x[i] * x[i];
+    return 0.5 * s;
+}
+
+// We, Wh of a state: ExportFields l2norm (D:2773-2775) for the tet methods,
+// ExportFitFields (D:3468-3470) for fdtd.
+void energies(const tf_problem* p, const std::vector<double>& U, const std::vector<double>& B, double& we,
+              double& wh) {
+    if (p->method == TF_METHOD_FDTD) {
+        we = half_weighted(p->fit.ep_vec, U);
+        wh = half_weighted(p->fit.nu_vec, B);
+    } else {
+        we = half_quad(p->dga.E, U);
+        wh = half_quad(p->dga.N, B);
+    }
+}
+
 struct Sink {
     oracle_out* o;
     int32_t np;
-    void put(double t, const std::vector<double>& six) {
+    const tf_problem* p;
+    // one output sample: the probes' six values and, when asked for, the
+    // energies of the same state (U, B)
+    void put(double t, const std::vector<double>& six, const std::vector<double>& U, const std::vector<double>& B) {
         if (o->n_samples < o->cap_samples) {
             if (o->times) o->times[o->n_samples] = t;
             if (o->vals)
                 std::memcpy(o->vals + o->n_samples * np * 6, six.data(), sizeof(double) * six.size());
+            if (o->energy) energies(p, U, B, o->energy[2 * o->n_samples], o->energy[2 * o->n_samples + 1]);
         }
         o->n_samples++;
     }
@@ -336,7 +370,7 @@ void run_fdtd(const tf_problem* p, const SourceTable& st, const tf_probe_geom* p
             for (int64_t q = f.ct_ptr[e]; q < f.ct_ptr[e + 1]; ++q)
                 Ct_vec[e].push_back(Sgn{(uint32_t)f.ct_face[q], f.ct_sgn[q]});
     }
-    Sink sink{out, np};
+    Sink sink{out, np, p};
     std::vector<double> six;
     double loop_s = 0;
     for (int64_t i = first; i < first + n; ++i) {
@@ -394,7 +428,7 @@ void run_fdtd(const tf_problem* p, const SourceTable& st, const tf_probe_geom* p
         loop_s += now_s() - t0;
         if (pr.allow(t)) {
             export_fit(U, B, pg, np, six);
-            sink.put(t, six);
+            sink.put(t, six, U, B);
         }
     }
     out->seconds = loop_s;
@@ -448,11 +482,11 @@ void run_dga(const tf_problem* p, const SourceTable& st, const tf_probe_geom* pg
         }
     };
     assemble_U();
-    Sink sink{out, np};
+    Sink sink{out, np, p};
     std::vector<double> six, U_old, curl_b(E, 0.0), cf, hc, cu, nc;
     if (first == 1 && pr.allow(0.0)) {  // initial export at t = 0 (D:805-806, A-6)
         export_tet(U, B, pg, np, six);
-        sink.put(0.0, six);
+        sink.put(0.0, six, U, B);
     }
     double loop_s = 0;
     for (int64_t i = first; i < first + n; ++i) {
@@ -487,7 +521,7 @@ void run_dga(const tf_problem* p, const SourceTable& st, const tf_probe_geom* pg
         loop_s += now_s() - t0;
         if (pr.allow(t)) {
             export_tet(U, B, pg, np, six);
-            sink.put(t, six);
+            sink.put(t, six, U, B);
         }
     }
     out->seconds = loop_s;
@@ -558,11 +592,11 @@ void run_dga_frac(const tf_problem* p, const SourceTable& st, const tf_probe_geo
         }
     };
     gather(fr.Mu, Uf, U);  // U = M*U_frac
-    Sink sink{out, np};
+    Sink sink{out, np, p};
     std::vector<double> six, U_old, Uf_old, Ff_old, curl_b(E, 0.0), cf, kc, cu, kf;
     if (first == 1 && pr.allow(0.0)) {
         export_tet(U, B, pg, np, six);
-        sink.put(0.0, six);
+        sink.put(0.0, six, U, B);
     }
     double loop_s = 0;
     for (int64_t i = first; i < first + n; ++i) {
@@ -602,7 +636,7 @@ void run_dga_frac(const tf_problem* p, const SourceTable& st, const tf_probe_geo
         loop_s += now_s() - t0;
         if (pr.allow(t)) {
             export_tet(U, B, pg, np, six);
-            sink.put(t, six);
+            sink.put(t, six, U, B);
         }
     }
     out->seconds = loop_s;
@@ -709,7 +743,7 @@ void run_gfdtd(const tf_problem* p, const SourceTable& st, const tf_probe_geom* 
     cg.compute();
     std::vector<double> solution(nf, 0.0), rhs(nf, 0.0), cu, eu, r1, ctf, su;
     for (int32_t k = 0; k < nf; ++k) solution[k] = U[d.free_edges[k]];
-    Sink sink{out, np};
+    Sink sink{out, np, p};
     std::vector<double> six, sx;
     // U.(SigMat U) and the Losses vector (zero when every tet is lossless)
     auto quad_sig = [&](const std::vector<double>& x) {
@@ -733,7 +767,7 @@ void run_gfdtd(const tf_problem* p, const SourceTable& st, const tf_probe_geom* 
         loop_s += now_s() - t0;
         if (pr.allow(t - dt)) {
             export_tet(U, B, pg, np, six);
-            sink.put(t - dt, six);
+            sink.put(t - dt, six, U, B);
         }
         t0 = now_s();
         Fb_old = Fb;
@@ -813,7 +847,7 @@ void run_fem(const tf_problem* p, const SourceTable& st, const tf_probe_geom* pg
     Cg cg{nf, d.A.rowptr, d.A.col, d.A.val, {}, tol, maxiter};
     cg.compute();
     std::vector<double> solution(nf, 0.0), rhs(nf, 0.0), cu, nc, cc, eu, eo, so, r1, sx;
-    Sink sink{out, np};
+    Sink sink{out, np, p};
     std::vector<double> six;
     auto quad_sig = [&](const std::vector<double>& x) {
         if (d.Sig.nnz == 0) return 0.0;
@@ -836,7 +870,7 @@ void run_fem(const tf_problem* p, const SourceTable& st, const tf_probe_geom* pg
         loop_s += now_s() - t0;
         if (pr.allow(t - dt)) {
             export_tet(U, B, pg, np, six);
-            sink.put(t - dt, six);
+            sink.put(t - dt, six, U, B);
         }
         t0 = now_s();
         Fb_old = Fb;
@@ -895,6 +929,11 @@ void oracle_export(int32_t fdtd, const tf_probe_geom* pg, int32_t n_probes, int3
     if (fdtd) export_fit(u, b, pg, n_probes, out);
     else export_tet(u, b, pg, n_probes, out);
     std::memcpy(six, out.data(), sizeof(double) * out.size());
+}
+
+void oracle_energy(const tf_problem* p, const double* U, const double* B, double* we, double* wh) {
+    std::vector<double> u(U, U + p->n_edges), b(B, B + p->n_faces);
+    energies(p, u, b, *we, *wh);
 }
 
 void oracle_source_compute(const tf_source_desc* s, const double p[3], double t, double out3[3]) {

@@ -57,6 +57,11 @@ typedef struct {
     double* losses;
     int64_t cap_losses;
     int64_t n_losses;
+    /* energies at each probe sample (may be NULL): energy[2k], energy[2k+1]
+     * = We, Wh of the state the sample's probes read (ExportFields l2norm
+     * D:2773-2775: 1/2 U.(E U), 1/2 B.(N B); ExportFitFields D:3468-3470:
+     * 1/2 U.(ep_vec U), 1/2 B.(nu_vec B)); cap = cap_samples */
+    double* energy;
 } oracle_out;
 
 /* Runs steps [first, first+n) of the problem's method. faithful=1 keeps the
@@ -69,6 +74,9 @@ int32_t oracle_run(const tf_problem* p, const tf_source_desc* sd, int32_t n_sd, 
                    double out_delay, double out_period, int64_t first, int64_t n, int32_t faithful,
                    double cg_tol, int32_t cg_maxiter, const double* U0, const double* F0, const double* B0,
                    oracle_out* out);
+
+/* We, Wh of the state (U, B) as the energy series records them. */
+void oracle_energy(const tf_problem* p, const double* U, const double* B, double* we, double* wh);
 
 /* Source::Compute(p, t) for one source (3-vector into out3). */
 void oracle_source_compute(const tf_source_desc* s, const double p[3], double t, double out3[3]);

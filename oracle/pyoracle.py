@@ -21,7 +21,8 @@ class oracle_out(C.Structure):
                 ("U", C.POINTER(C.c_double)), ("F", C.POINTER(C.c_double)), ("B", C.POINTER(C.c_double)),
                 ("cg_iters_total", C.c_int64), ("cg_iters_max", C.c_int32), ("cg_not_converged", C.c_int32),
                 ("cg_max_relres", C.c_double), ("seconds", C.c_double),
-                ("losses", C.POINTER(C.c_double)), ("cap_losses", C.c_int64), ("n_losses", C.c_int64)]
+                ("losses", C.POINTER(C.c_double)), ("cap_losses", C.c_int64), ("n_losses", C.c_int64),
+                ("energy", C.POINTER(C.c_double))]
 
 
 def lib():
@@ -39,6 +40,8 @@ def lib():
                              C.c_double, C.c_int32, dp, dp, dp, C.POINTER(oracle_out)]
     L.oracle_run.restype = C.c_int32
     L.oracle_last_error.restype = C.c_char_p
+    L.oracle_energy.argtypes = [C.POINTER(_ffi.tf_problem), dp, dp, dp, dp]
+    L.oracle_energy.restype = None
     L.oracle_source_compute.argtypes = [C.POINTER(_ffi.tf_source_desc), dp, C.c_double, dp]
     L.oracle_source_compute.restype = None
     L.oracle_cg_solve.argtypes = [C.c_int32, C.POINTER(C.c_int64), C.POINTER(C.c_int32), dp, dp, dp,
@@ -59,10 +62,11 @@ def _dp(a):
 
 
 def run(build, first=None, n=None, faithful=True, cg_tol=None, cg_maxiter=100, U0=None, F0=None, B0=None,
-        output=True, max_samples=None):
+        output=True, max_samples=None, energy=False):
     """Runs the oracle on a tetfit_amd.host.Build; returns a dict with
     times, samples [k, probe, 6], poynting, losses (gfdtd's Losses vector in
-    push order), U, F, B and CG statistics."""
+    push order), U, F, B and CG statistics; with energy, also We and Wh at
+    every sample."""
     from tetfit_amd import _ffi
     L = lib()
     first = build.first_step if first is None else first
@@ -90,11 +94,13 @@ def run(build, first=None, n=None, faithful=True, cg_tol=None, cg_maxiter=100, U
     o.poynting, o.cap_poynting = _dp(poy), len(poy)
     o.losses, o.cap_losses = _dp(loss), len(loss)
     o.U, o.F, o.B = _dp(U), _dp(F), _dp(B)
+    en = np.zeros(2 * cap) if energy else None
+    o.energy = _dp(en)
     tol = build.cg_tol if cg_tol is None else cg_tol
     u0 = None if U0 is None else np.ascontiguousarray(U0, dtype=np.float64)
     f0 = None if F0 is None else np.ascontiguousarray(F0, dtype=np.float64)
     b0 = None if B0 is None else np.ascontiguousarray(B0, dtype=np.float64)
-    rc = L.oracle_run(build.problem_ptr, sd, nsd, se, nse, pg, np_, 1 if (output and build.output_mode in (1, 2)) else 0,
+    rc = L.oracle_run(build.problem_ptr, sd, nsd, se, nse, pg, np_, 1 if (output and build.output_mode in (1, 2, 3)) else 0,
                       build.output_delay, build.output_period, int(first), int(n), 1 if faithful else 0,
                       float(tol), int(cg_maxiter), _dp(u0), _dp(f0), _dp(b0), C.byref(o))
     if rc != 0:
@@ -103,7 +109,18 @@ def run(build, first=None, n=None, faithful=True, cg_tol=None, cg_maxiter=100, U
     return dict(times=times[:ns], samples=vals[:ns * np_ * 6].reshape(ns, np_, 6), n_samples=o.n_samples,
                 poynting=poy[:min(o.n_poynting, len(poy))], losses=loss[:min(o.n_losses, len(loss))], U=U, F=F, B=B,
                 cg_iters_total=o.cg_iters_total, cg_iters_max=o.cg_iters_max,
-                cg_not_converged=o.cg_not_converged, cg_max_relres=o.cg_max_relres, seconds=o.seconds)
+                cg_not_converged=o.cg_not_converged, cg_max_relres=o.cg_max_relres, seconds=o.seconds,
+                We=None if en is None else en[0:2 * ns:2].copy(), Wh=None if en is None else en[1:2 * ns:2].copy())
+
+
+def energy(build, U, B):
+    """(We, Wh) of a state as the oracle's energy series records it."""
+    L = lib()
+    we, wh = C.c_double(), C.c_double()
+    u = np.ascontiguousarray(U, dtype=np.float64)
+    b = np.ascontiguousarray(B, dtype=np.float64)
+    L.oracle_energy(build.problem_ptr, _dp(u), _dp(b), C.byref(we), C.byref(wh))
+    return we.value, wh.value
 
 
 def source_compute(desc_struct, p, t):

@@ -313,3 +313,36 @@ def test_fit_pml_parity(workdir):
                                  pml={"+x": 0.35, "+z": 0.25})).build(1)
     assert np.any(b.fit_ops()["upml"] < 1)
     check_run(b)
+
+
+@pytest.mark.parametrize("prec,precond", [("fp64", "jacobi"), ("fp32", "learned")])
+def test_slice_stream_bitwise(workdir, prec, precond, monkeypatch):
+    """The CG kernels stream windowed-column slices through a two-stage
+    software pipeline (next batch's entries loaded while the current batch's
+    gathers are in flight); one-base columns take the unpipelined slice loop
+    of the same wide-mesh kernels. Both sum every row in entry order, so a
+    gfdtd run (Jacobi fp64, learned fp32) gives bitwise equal fields and
+    iteration counts either way."""
+    from tetfit_amd.device import Simulation
+    b = Model(configs.kuhn_cavity(str(workdir), n=10, method="gfdtd", n_steps=60, n_probes=4, block=2, seed=5,
+                                  eps_range=(1.0, 80.0), log_eps=True, name="stream_bw")).build(1)
+    out = {}
+    for layout, env in (("win", "TETFIT_SELL_WIN"), ("wide", "TETFIT_SELL_WIDE")):
+        monkeypatch.delenv("TETFIT_SELL_WIN", raising=False)
+        monkeypatch.delenv("TETFIT_SELL_WIDE", raising=False)
+        monkeypatch.setenv(env, "1")
+        print(f"case: slice stream {prec} {precond} {layout} E={b.n_edges}", flush=True)
+        sim = Simulation(b, prec)
+        info = sim.sell_info("a")
+        assert info["col_bases"] == (4 if layout == "win" else 1)
+        if precond == "learned":
+            sim.run(n=30, precond="learned", learn_steps=30, train_iters=40)
+            st = sim.run(n=30, precond="learned")
+            assert st["pc_state"] == 2
+        else:
+            st = sim.run()
+        out[layout] = (st["cg_iters_total"],) + sim.state()
+        sim.close()
+    assert out["win"][0] == out["wide"][0]
+    for a, c in zip(out["win"][1:], out["wide"][1:]):
+        assert np.array_equal(a, c)

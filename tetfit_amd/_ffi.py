@@ -190,6 +190,8 @@ def hip_lib():
     L.tf_read_poynting.argtypes = [vp, f64p, C.c_int64, i64p]
     L.tf_read_losses.argtypes = [vp, f64p, C.c_int64, i64p]
     L.tf_energy.argtypes = [vp, f64p, f64p]
+    L.tf_set_energy_output.argtypes = [vp, C.c_int32]
+    L.tf_read_energy.argtypes = [vp, f64p, f64p, C.c_int64, i64p]
     L.tf_kernel_times.argtypes = [vp, C.c_char_p, C.c_int32, f64p, i64p, C.c_int32, i32p]
     L.tf_set_profiling.argtypes = [vp, C.c_int32]
     L.tf_device_bytes.argtypes = [vp, i64p]
@@ -213,7 +215,7 @@ def hip_lib():
     L.tf_mlp_grad_noalloc.argtypes = [vp, vp, vp, C.c_int64, vp, vp, vp]
     for n in ["tf_create", "tf_set_sources", "tf_set_probes", "tf_set_output", "tf_run",
               "tf_read_state", "tf_write_state", "tf_read_probes", "tf_read_poynting", "tf_read_losses",
-              "tf_energy", "tf_kernel_times", "tf_set_profiling", "tf_device_bytes", "tf_destroy",
+              "tf_energy", "tf_set_energy_output", "tf_read_energy", "tf_kernel_times", "tf_set_profiling", "tf_device_bytes", "tf_destroy",
               "tf_mlp_grad", "tf_create_batch", "tf_batch_set_sources", "tf_batch_set_probes",
               "tf_batch_set_output", "tf_batch_read_probes", "tf_batch_read_state", "tf_autotune",
               "tf_get_tuning", "tf_sell_info", "tf_set_tuning", "tf_set_graphs", "tf_fit_layout",

@@ -9,6 +9,12 @@
 //   poynting.dat  (t, Poynting flux, losses per step, D:945-950, D:1573-1578,
 //       D:1936-1941; losses from tf_read_losses: gfdtd's SigMat terms, 0 for
 //       dga (lossless here) and fdtd (Joule weights only in l2norm mode))
+//   <name><sim:05d>_energy_electric.dat, _energy_magnetic.dat  (l2norm and
+//       maxerror modes, D:2250-2282: t, W, W_analytic, W_difference, and the
+//       trapezoid time integrals of the three, D:2795-2818; H at t + dt/2).
+//       The analytic comparison is not built: the analytic field is taken as
+//       zero, so W_analytic and its integral are 0 and the difference
+//       columns repeat the numeric ones.
 //
 // usage: tetfit input.fdtd [--fp32] [--learned] [--tol X]
 //   --fp32     fp32 fields and operators (fp64 reductions)
@@ -52,6 +58,32 @@ void write_line(std::ofstream& os, const double c[3], double t, const double v[3
     os << std::setw(15) << c[0] << " " << std::setw(15) << c[1] << " " << std::setw(15) << c[2] << " "
        << std::setw(15) << t << " " << std::setw(15) << v[0] << " " << std::setw(15) << v[1] << " "
        << std::setw(15) << v[2] << std::endl;
+}
+
+// The two energy files of D:2250-2282 from the sampled (We, Wh) series;
+// the time integrals are the reference's trapezoid sums over the sample
+// times (D:2795-2808), 0 at the first sample.
+void write_energy(const std::string& name, int32_t sim, const std::vector<double>& t,
+                  const std::vector<double>& vals, double dt) {
+    std::ostringstream se, sh;
+    se << name << std::setw(5) << std::setfill('0') << sim << "_energy_electric.dat";
+    sh << name << std::setw(5) << std::setfill('0') << sim << "_energy_magnetic.dat";
+    std::ofstream os(se.str()), osh(sh.str());
+    double cum_e = 0, cum_h = 0;
+    for (size_t k = 0; k < t.size(); ++k) {
+        const double we = vals[2 * k], wh = vals[2 * k + 1];
+        if (k > 0) {
+            cum_e += 0.5 * (t[k] - t[k - 1]) * (vals[2 * k - 2] + we);
+            cum_h += 0.5 * (t[k] - t[k - 1]) * (vals[2 * k - 1] + wh);
+        }
+        // columns: t, W, W_analytic (0), W_difference (= W), and their integrals
+        os << std::setw(15) << t[k] << " " << std::setw(15) << we << " " << std::setw(15) << 0.0 << " "
+           << std::setw(15) << we << " " << std::setw(15) << cum_e << " " << std::setw(15) << 0.0 << " "
+           << std::setw(15) << cum_e << std::endl;
+        osh << std::setw(15) << t[k] + 0.5 * dt << " " << std::setw(15) << wh << " " << std::setw(15) << 0.0 << " "
+            << std::setw(15) << wh << " " << std::setw(15) << cum_h << " " << std::setw(15) << 0.0 << " "
+            << std::setw(15) << cum_h << std::endl;
+    }
 }
 
 void write_probes(const std::string& name, int32_t sim, int32_t frame, const double* points, int32_t np,
@@ -141,8 +173,9 @@ int main(int argc, char** argv) {
         tf_build_probes(b, &probes, &points, &n_probes);
         check(tf_set_probes(ctx, probes, n_probes), "tf_set_probes");
         const int32_t mode = tf_build_output_mode(b);
-        if (mode == 1 || mode == 2) check(tf_set_output(ctx, tf_build_output_delay(b), tf_build_output_period(b)),
-                                          "tf_set_output");
+        if (mode == 1 || mode == 2 || mode == 3)
+            check(tf_set_output(ctx, tf_build_output_delay(b), tf_build_output_period(b)), "tf_set_output");
+        if (mode == 2 || mode == 3) check(tf_set_energy_output(ctx, 1), "tf_set_energy_output");
         tf_cg_opts cg{};
         cg.tol = tol_override > 0 ? tol_override : tf_build_cg_tol(b);
         cg.maxiter = 100;  // D:1388
@@ -176,6 +209,13 @@ int main(int argc, char** argv) {
             check(tf_read_probes(ctx, times.data(), vals.data(), ns, &ns), "tf_read_probes");
             write_probes(tf_build_output_name(b), label, tf_build_output_frame(b), points, n_probes, times, vals,
                          prob->dt);
+        }
+        if (mode == 2 || mode == 3) {
+            int64_t ns = 0;
+            check(tf_read_energy(ctx, nullptr, nullptr, 0, &ns), "tf_read_energy");
+            std::vector<double> times(ns), vals(2 * (size_t)ns);
+            check(tf_read_energy(ctx, times.data(), vals.data(), ns, &ns), "tf_read_energy");
+            write_energy(tf_build_output_name(b), label, times, vals, prob->dt);
         }
         // poynting.dat: step k at (k + 1/2) dt (fdtd, dga) or (k + 1) dt (gfdtd)
         {

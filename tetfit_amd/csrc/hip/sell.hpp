@@ -178,6 +178,116 @@ __device__ __forceinline__ SliceRange block_slices(int nslices, int band) {
 #define TF_FOR_SLICES(m)                                                               \
     for (SliceRange sr_ = block_slices((m).nslices, (m).band); sr_.s < sr_.end; sr_.s += sr_.step)
 
+// ---- software-pipelined slice stream (windowed 16-bit columns) ------------
+// A wave's slices as a stream of batches of U entries: (s, 0), (s, U), ...
+// up to the slice width, then the next slice. While the gathers of batch b
+// are in flight, the entries of batch b+1 are loaded, so each lane keeps two
+// batches of loads outstanding instead of waiting out the entry load and
+// then the gather of every slice in turn. Vector memory counters retire in
+// issue order, so the order of issue is: gathers of b, the row loads of b's
+// slice when b is its last batch, entries of b+1; the wait before b's sums
+// then leaves b+1's entries in flight. Entries are summed in entry order, so
+// the results equal those of the unpipelined slice loop bit for bit.
+struct SliceMeta {
+    int64_t base;  // first entry of the slice
+    int w;         // width (entries per row)
+    ColWin cw;     // column windows
+};
+// A wave-uniform load through the constant address space, which the
+// compiler issues as a scalar memory load: slice pointers and column bases
+// are never written by the kernels, and a vector load of them would retire
+// in order with (and so wait out) the batch loads in flight.
+template <typename X>
+__device__ __forceinline__ X scalar_load(const X* p) {
+    return *(const __attribute__((address_space(4))) X*)p;
+}
+template <typename V>
+__device__ __forceinline__ SliceMeta slice_meta(const SellView<V>& m, int s) {
+    s = __builtin_amdgcn_readfirstlane(s);
+    const int64_t base = scalar_load(m.sptr + s);
+    const int w = (int)((scalar_load(m.sptr + s + 1) - base) / kSlice);
+    const int32_t* b = m.cbase + 4 * s;
+    const int32_t b0 = scalar_load(b), b1 = scalar_load(b + 1), b2 = scalar_load(b + 2), b3 = scalar_load(b + 3);
+    return SliceMeta{base, w, ColWin{b0, b0 ^ b1, b2, b2 ^ b3}};
+}
+
+// E: one entry as loaded (column code and value); load(pos) loads the entry
+// at SELL position pos; gather(e, cw) fetches the gathered operand of e;
+// add(acc, e, g) accumulates; row_pre(s) issues the loads of slice s's row
+// vectors and returns them; row_post(s, acc, rin) finishes slice s.
+template <int U, typename E, typename Acc, typename V>
+struct SellStream {
+    const SellView<V>& m;
+    SliceRange sr;
+    int lane;
+    SliceMeta cur;
+    int k0 = 0;
+    E e[U];
+
+    __device__ __forceinline__ SellStream(const SellView<V>& mv, SliceRange r, int l) : m(mv), sr(r), lane(l) {}
+
+    template <class Load>
+    __device__ __forceinline__ void fetch(const SliceMeta& mm, int kk0, E (&ee)[U], Load load) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int kk = min(kk0 + u, mm.w - 1);  // clamped: loads stay inside the slice
+            ee[u] = load(mm.base + (int64_t)kk * kSlice + lane);
+        }
+    }
+
+    // Issues the first batch's loads (before a reduction the caller waits on).
+    template <class Load>
+    __device__ __forceinline__ void start(Load load) {
+        if (sr.s >= sr.end) return;
+        cur = slice_meta(m, sr.s);
+        k0 = 0;
+        fetch(cur, 0, e, load);
+    }
+
+    // Every slice must have width >= 1 (the host checks it for the matrices
+    // streamed this way). The loop body issues the same loads on every pass,
+    // whatever the batch's position in its slice (the row loads of a
+    // slice's earlier batches and the last pass's re-fetch of its own batch
+    // are spare loads of cached lines), so the compiler's wait before the
+    // sums can leave exactly the next batch's U loads in flight.
+    template <class Load, class Gather, class Add, class RowPre, class RowPost>
+    __device__ __forceinline__ void run(Load load, Gather gather, Add add, RowPre row_pre, RowPost row_post) {
+        int s = sr.s;
+        if (s >= sr.end) return;
+        Acc acc{};
+        for (;;) {
+            decltype(gather(e[0], cur.cw)) g[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) g[u] = gather(e[u], cur.cw);
+            const auto rin = row_pre(s);
+            const bool last = k0 + U >= cur.w;
+            const int sn = last ? s + sr.step : s;
+            const bool more = sn < sr.end;
+            SliceMeta mn = cur;
+            int kn = k0;
+            if (more) {
+                kn = last ? 0 : k0 + U;
+                if (last) mn = slice_meta(m, sn);
+            }
+            E en[U];
+            fetch(mn, kn, en, load);
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                if (k0 + u < cur.w) add(acc, e[u], g[u]);
+            if (last) {
+                row_post(s, acc, rin);
+                acc = Acc{};
+            }
+            if (!more) break;
+            s = sn;
+            k0 = kn;
+            cur = mn;
+#pragma unroll
+            for (int u = 0; u < U; ++u) e[u] = en[u];
+        }
+    }
+};
+
 // curl_f = Ct F over edges (dga, D:878-879).
 template <typename T, int NB, bool W = false>
 __global__ void __launch_bounds__(kBlock) sell_ct_kernel(SellView<int8_t> m, const T* __restrict__ x,
@@ -262,6 +372,20 @@ __global__ void __launch_bounds__(kBlock) quad_form_kernel(SellView<T> m, const 
     }
     acc = block_sum(acc, red);
     if (threadIdx.x == 0) part[blockIdx.x] = acc;
+}
+
+// Energies of one output sample from the partials of the two reductions:
+// out = (1/2 sum pe[0..ge), 1/2 sum pn[0..gn)), each sum taken in partial
+// order on one thread (the order in which tf_energy sums them on the host).
+__global__ void __launch_bounds__(64) energy_sum_kernel(const double* __restrict__ pe, int ge,
+                                                        const double* __restrict__ pn, int gn,
+                                                        double* __restrict__ out, const CgState* guard) {
+    if (!cg_guard_ok(guard) || threadIdx.x != 0) return;
+    double a = 0, b = 0;
+    for (int k = 0; k < ge; ++k) a += pe[k];
+    for (int k = 0; k < gn; ++k) b += pn[k];
+    out[0] = 0.5 * a;
+    out[1] = 0.5 * b;
 }
 
 // ---- CG (ConjugateGradientSolver.hpp:52-103) ------------------------------
@@ -425,6 +549,28 @@ __global__ void __launch_bounds__(kBlock) cg_w_kernel(int32_t it, SellView<T> m,
     }
     const int lane = lane_id();
     double wu = 0, none = 0;
+    if constexpr (W) {
+        if (m.col16 && m.cwin) {
+            struct Ent { uint16_t c; T v; };
+            SellStream<NB, Ent, T, T> ss(m, block_slices(m.nslices, m.band), lane);
+            auto load = [&](int64_t pos) { return Ent{stream_load<W>(m.col16 + pos), stream_load<W>(m.val + pos)}; };
+            ss.start(load);
+            ss.run(
+                load, [&](const Ent& e, const ColWin& cw) { return u[win_col(cw, e.c)]; },
+                [&](T& a, const Ent& e, T gv) { a += e.v * gv; },
+                [&](int sl) { return u[min(sl * kSlice + lane, m.nrows - 1)]; },
+                [&](int sl, T wi, T ui) {
+                    const int32_t i = sl * kSlice + lane;
+                    if (i < m.nrows) {
+                        w[i] = wi;
+                        wu += (double)wi * (double)ui;
+                    }
+                });
+            block_sum2_all(wu, none, red);
+            if (threadIdx.x == 0) part_del[blockIdx.x] = wu;
+            return;
+        }
+    }
     TF_FOR_SLICES(m) {
         const int s = sr_.s;
         const double wv = sell_dot<T, NB, W>(m, s, lane, [&](int32_t j) { return u[j]; });
@@ -600,17 +746,65 @@ __global__ void __launch_bounds__(kBlock) cg_pg_kernel(int32_t it, SellView<uint
     if (done) return;
     const int lane = lane_id();
     SliceRange sr = block_slices(g.nslices, g.band);
+    double alpha = 0;
+    auto reduce_alpha = [&] {
+        if (it > 0) {
+            double del = thread_sum(pd), unused = 0;
+            block_sum2_all(del, unused, red);
+            alpha = it == 1 ? gam / del : gam / (del - beta * gam / alpha_pp);
+            if (threadIdx.x == 0 && blockIdx.x == 0) st->alpha[k] = alpha;
+        }
+    };
+    double ru = 0, rr = 0;
+    if constexpr (W) {
+        if (gpk && g.cwin) {
+            // the slice stream: packed entries (A's column code << 16 | bf16 G value)
+            struct Acc { T r, s; };
+            struct RowIn { RS<T> o; T pv, xv; };
+            SellStream<NB, uint32_t, Acc, uint16_t> ss(g, sr, lane);
+            auto load = [&](int64_t pos) { return stream_load<W>(gpk + pos); };
+            ss.start(load);  // in flight during the reduction
+            reduce_alpha();
+            ss.run(
+                load, [&](uint32_t e, const ColWin& cw) { return rs_in[win_col(cw, e >> 16)]; },
+                [&](Acc& a, uint32_t e, const RS<T>& gv) {
+                    const T v = (T)bf16_to_float((uint16_t)(e & 0xFFFFu));
+                    a.r += v * gv.r;
+                    a.s += v * gv.s;
+                },
+                [&](int sl) {
+                    // unconditional loads (a clamped row past the end): no
+                    // branch whose join would wait on them
+                    const int32_t i = min(sl * kSlice + lane, g.nrows - 1);
+                    return RowIn{rs_in[i], p[i], x[i]};
+                },
+                [&](int sl, const Acc& a, const RowIn& q) {
+                    const int32_t i = sl * kSlice + lane;
+                    if (i < g.nrows) {
+                        T ri = q.o.r, ui = a.r;
+                        if (it > 0) {
+                            ri = cg_r_next(q.o.r, q.o.s, alpha);
+                            rs_out[i] = RS<T>{ri, q.o.s};
+                            x[i] = (T)((double)q.xv + alpha * (double)q.pv);
+                            ui = (T)((double)a.r - alpha * (double)a.s);
+                        }
+                        u[i] = ui;
+                        ru += (double)ri * (double)ui;
+                        rr += (double)ri * (double)ri;
+                    }
+                });
+            block_sum2_all(ru, rr, red);
+            if (threadIdx.x == 0) {
+                part_gam[blockIdx.x] = ru;
+                part_rr[blockIdx.x] = rr;
+            }
+            return;
+        }
+    }
     const bool first = sr.s < sr.end;
     T gr0 = 0, gs0 = 0;
     if (first) sell_dot_rs<T, NB, W>(g, gpk, sr.s, lane, rs_in, gr0, gs0);
-    double alpha = 0;
-    if (it > 0) {
-        double del = thread_sum(pd), unused = 0;
-        block_sum2_all(del, unused, red);
-        alpha = it == 1 ? gam / del : gam / (del - beta * gam / alpha_pp);
-        if (threadIdx.x == 0 && blockIdx.x == 0) st->alpha[k] = alpha;
-    }
-    double ru = 0, rr = 0;
+    reduce_alpha();
     auto row = [&](int sl, T gr, T gs) {
         const int32_t i = sl * kSlice + lane;
         if (i < g.nrows) {
@@ -669,6 +863,38 @@ __global__ void __launch_bounds__(kBlock) cg_wu_kernel(int32_t it, SellView<T> m
     }
     const int lane = lane_id();
     double wu = 0, none = 0;
+    if constexpr (W) {
+        if (m.col16 && m.cwin) {
+            struct Ent { uint16_t c; T v; };
+            struct RowIn { T ui, pv; RS<T> o; };
+            SellStream<NB, Ent, T, T> ss(m, block_slices(m.nslices, m.band), lane);
+            auto load = [&](int64_t pos) { return Ent{stream_load<W>(m.col16 + pos), stream_load<W>(m.val + pos)}; };
+            ss.start(load);
+            ss.run(
+                load, [&](const Ent& e, const ColWin& cw) { return u[win_col(cw, e.c)]; },
+                [&](T& a, const Ent& e, T gv) { a += e.v * gv; },
+                [&](int sl) {
+                    const int32_t i = min(sl * kSlice + lane, m.nrows - 1);
+                    return RowIn{u[i], p[i], rs_cur[i]};
+                },
+                [&](int sl, T wi, const RowIn& q) {
+                    const int32_t i = sl * kSlice + lane;
+                    if (i < m.nrows) {
+                        if (it == 0) {
+                            p[i] = q.ui;
+                            rs_cur[i] = RS<T>{q.o.r, wi};
+                        } else {
+                            p[i] = (T)((double)q.ui + beta * (double)q.pv);
+                            rs_cur[i] = RS<T>{q.o.r, (T)((double)wi + beta * (double)q.o.s)};
+                        }
+                        wu += (double)wi * (double)q.ui;
+                    }
+                });
+            block_sum2_all(wu, none, red);
+            if (threadIdx.x == 0) part_del[blockIdx.x] = wu;
+            return;
+        }
+    }
     TF_FOR_SLICES(m) {
         const int sl = sr_.s;
         const double wv = sell_dot<T, NB, W>(m, sl, lane, [&](int32_t j) { return u[j]; });

@@ -91,6 +91,7 @@ struct Sell {
     bool c16 = false;   // 16-bit column codes (col16 + cbase) in the kernels' view
     bool cwin = false;  // ... in the four-window form (SellView::cwin)
     bool band = false;  // banded slice order (SellView::band), batched dga
+    int64_t min_width = 0;  // narrowest slice (the CG kernels' slice streams need >= 1)
     template <class V>
     SellView<V> view(const DevBuf& v) const {
         return SellView<V>{nrows, nslices, sptr.as<int64_t>(), c16 ? nullptr : col.as<int32_t>(),
@@ -302,9 +303,11 @@ struct tf_ctx {
     DevBuf term_src, term_w, carr;
     int64_t carr_cap = 0;
 
-    // probes and output schedule
+    // probes and output schedule; with energy_out the samples also record
+    // (We, Wh) in eser (2 per sample, epart: the reductions' partials)
     int32_t n_probes = 0;
-    DevBuf probes, samples;
+    DevBuf probes, samples, eser, epart;
+    bool energy_out = false;
     int64_t sample_cap = 0, n_samples = 0;
     std::vector<double> sample_times;
     double next_print = 0, period = 0;
@@ -357,7 +360,7 @@ size_t tf_ctx::device_bytes() const {
                            &sp_edge, &sp_f4, &sp_sg, &sp_ca, &sp_cb, &sp_kind, &sp_tb, &sp_te, &c_col, &c_sg,
                            &h_edge, &curl_f, &curl_u, &free_edge, &dir_edge, &dir_new, &fb, &pre, &x, &r, &rs0, &rs1, &z,
                            &p0, &p1, &q, &r0, &cg_part, &cg_state, &inj_edge, &inj_kind, &inj_tb, &inj_te,
-                           &term_src, &term_w, &carr, &probes, &samples, &poy, &row_dt_h, &row_dt_f,
+                           &term_src, &term_w, &carr, &probes, &samples, &eser, &epart, &poy, &row_dt_h, &row_dt_f,
                            &mu_a, &mu_b, &tf_a, &tf_b, &fslot, &uf[0], &uf[1], &ff[0], &ff[1], &U_older, &fw, &fw2};
     for (auto b : all) s += b->bytes;
     const Csr32* cs[] = {&Ct};
@@ -412,6 +415,8 @@ void upload_sell(Sell& d, const tf_csr& c, bool values) {
     d.nslices = (c.nrows + kSlice - 1) / kSlice;
     const std::vector<int64_t> sptr = sell_layout(c);
     d.entries = sptr[d.nslices];
+    d.min_width = d.nslices ? INT64_MAX : 0;
+    for (int32_t s = 0; s < d.nslices; ++s) d.min_width = std::min(d.min_width, (sptr[s + 1] - sptr[s]) / kSlice);
     std::vector<int32_t> col(d.entries, 0);
     std::vector<T> val(values ? d.entries : 0, (T)0);
     std::vector<int8_t> sg(values ? 0 : d.entries, 0);
@@ -989,6 +994,9 @@ void create_tet(tf_ctx* c, const tf_problem* p) {
         CsrBuf a;
         select_rows(d.A, perm.data(), d.n_free, inv.data(), a);
         upload_sell<T>(c->A, a.v, true);
+        // the CG kernels stream A's (and G's) slices assuming width >= 1:
+        // every free row holds its diagonal
+        if (c->A.nslices > 0 && c->A.min_width < 1) throw std::runtime_error("A has a 64-row slice without entries");
         c->hA_ptr = a.rp;
         c->hA_col = a.col;
         c->hA_val = a.val;
@@ -1218,35 +1226,74 @@ bool want_sample(tf_ctx* c, int64_t i, double t) {
     return allow_print(c, t);
 }
 
-// Host bookkeeping of one probe sample at time t: its slot in the sample
-// buffer (grown when full), or -1 without probes.
+// Host bookkeeping of one output sample at time t: its slot in the sample
+// buffers (probe values, energies; grown when full), or -1 when the sample
+// records nothing on the device (no probes, no energy output).
 int64_t sample_reserve(tf_ctx* c, double t) {
     c->sample_times.push_back(t);
     const int64_t slot = c->n_samples++;
-    if (c->n_probes == 0) return -1;
+    if (c->n_probes == 0 && !c->energy_out) return -1;
     if (c->n_samples > c->sample_cap) {
-        int64_t cap = std::max<int64_t>(1024, c->sample_cap * 2);
-        DevBuf nb;
-        nb.alloc(cap * c->n_probes * 6 * sizeof(double));
-        if (slot)
-            TF_HIP(hipMemcpyAsync(nb.p, c->samples.p, slot * c->n_probes * 6 * sizeof(double),
-                                  hipMemcpyDeviceToDevice, c->stream));
-        TF_HIP(hipStreamSynchronize(c->stream));
-        std::swap(nb.p, c->samples.p);
-        std::swap(nb.bytes, c->samples.bytes);
+        const int64_t cap = std::max<int64_t>(1024, c->sample_cap * 2);
+        auto grow = [&](DevBuf& buf, int64_t per) {
+            DevBuf nb;
+            nb.alloc(cap * per * sizeof(double));
+            if (slot && buf.p)
+                TF_HIP(hipMemcpyAsync(nb.p, buf.p, slot * per * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+            TF_HIP(hipStreamSynchronize(c->stream));
+            std::swap(nb.p, buf.p);
+            std::swap(nb.bytes, buf.bytes);
+        };
+        if (c->n_probes > 0) grow(c->samples, (int64_t)c->n_probes * 6);
+        if (c->energy_out) grow(c->eser, 2);
         c->sample_cap = cap;
     }
     return slot;
 }
 
+// Partials per energy reduction (tf_energy uses the same grids, so a sample
+// equals tf_energy of the same state bit for bit).
+constexpr int kEnergyParts = 1024;
+
+// Launches the two energy reductions of the current state into epart and
+// returns their partial counts (ge, gn): dga / gfdtd / fem We = 1/2 U.E U,
+// Wh = 1/2 B.N B (D:2773-2775), fdtd with the ep_vec / nu_vec weights
+// (D:3468-3470).
+template <class T>
+void energy_partials(tf_ctx* c, const CgState* guard, int& ge, int& gn) {
+    double* pp = c->epart.as<double>();
+    if (c->method != TF_METHOD_FDTD) {
+        ge = sell_blocks(c->Efull, kEnergyParts);
+        gn = sell_blocks(c->N, kEnergyParts);
+        TF_LAUNCH_NB(c->wide, 8, quad_form_kernel, T, ge, c->stream, c->Efull.vals<T>(), c->U.as<T>(), pp, guard);
+        TF_LAUNCH_NB(c->wide, 8, quad_form_kernel, T, gn, c->stream, c->N.vals<T>(), c->B.as<T>(),
+                     pp + kEnergyParts, guard);
+    } else {
+        ge = gn = kEnergyParts;
+        weighted_sq_kernel<T><<<kEnergyParts, kBlock, 0, c->stream>>>(c->E, c->U.as<T>(), c->fit_ep.as<double>(), pp);
+        weighted_sq_kernel<T><<<kEnergyParts, kBlock, 0, c->stream>>>(c->F, c->B.as<T>(), c->fit_nu.as<double>(),
+                                                                      pp + kEnergyParts);
+    }
+}
+
 template <class T>
 void sample_launch(tf_ctx* c, int64_t slot, const CgState* guard) {
     if (slot < 0) return;
-    ev_begin(c, "probe_kernel");
-    probe_kernel<T><<<blocks_for(c->n_probes * 6, 64), 64, 0, c->stream>>>(
-        c->n_probes, c->probes.as<ProbeDev>(), c->U.as<T>(), c->B.as<T>(),
-        c->samples.as<double>() + slot * c->n_probes * 6, guard);
-    ev_end(c);
+    if (c->n_probes > 0) {
+        ev_begin(c, "probe_kernel");
+        probe_kernel<T><<<blocks_for(c->n_probes * 6, 64), 64, 0, c->stream>>>(
+            c->n_probes, c->probes.as<ProbeDev>(), c->U.as<T>(), c->B.as<T>(),
+            c->samples.as<double>() + slot * c->n_probes * 6, guard);
+        ev_end(c);
+    }
+    if (c->energy_out) {
+        ev_begin(c, "energy_kernels");
+        int ge = 0, gn = 0;
+        energy_partials<T>(c, guard, ge, gn);
+        energy_sum_kernel<<<1, 64, 0, c->stream>>>(c->epart.as<double>(), ge, c->epart.as<double>() + kEnergyParts,
+                                                   gn, c->eser.as<double>() + 2 * slot, guard);
+        ev_end(c);
+    }
 }
 
 template <class T>
@@ -1987,7 +2034,12 @@ void learned_setup(tf_ctx* c, const tf_cg_opts* cg) {
     TF_HIP(hipStreamSynchronize(c->stream));
     const int32_t w0 = L.w0, w1 = L.w0 + L.nw;
     L.tables = std::async(std::launch::async, [c, is = std::move(is), w0, w1]() mutable {
-        return learned_tables(c, std::move(is), w0, w1);
+        const auto t0 = std::chrono::steady_clock::now();
+        auto T = learned_tables(c, std::move(is), w0, w1);
+        if (std::getenv("TETFIT_HOST_TIMING"))
+            std::fprintf(stderr, "[learned] host tables built in %.3f s (%d pairs)\n",
+                         std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(), T->n_pairs);
+        return T;
     });
 }
 
@@ -2032,7 +2084,17 @@ int row_blocks(int32_t n) { return std::max(1, std::min(kSpmvCap, (n + kBlock / 
 void learned_train(tf_ctx* c) {
     LearnedPc& L = c->lp;
     const auto t0 = std::chrono::steady_clock::now();
+    // phase times on stderr with TETFIT_HOST_TIMING set (diagnostics)
+    const bool timing = std::getenv("TETFIT_HOST_TIMING") != nullptr;
+    auto since = [](std::chrono::steady_clock::time_point a) {
+        return std::chrono::duration<double>(std::chrono::steady_clock::now() - a).count();
+    };
+    L.tables.wait();
+    const double t_wait = since(t0);
     learned_upload_tables(c);
+    TF_HIP(hipStreamSynchronize(c->stream));
+    const double t_upload = since(t0) - t_wait;
+    const auto t1 = std::chrono::steady_clock::now();
     const int32_t nw = L.nw, w0 = L.w0;
     hipStream_t s = c->stream;
     const int64_t* sptr = c->A.sptr.as<int64_t>();
@@ -2112,6 +2174,8 @@ void learned_train(tf_ctx* c) {
                                                                   (float)L.lr, 0.9f, 0.999f, 1e-8f, it + 1, 1.f);
     }
     L.loss = best;
+    const double t_loop = since(t1);
+    const auto t2 = std::chrono::steady_clock::now();
     // frozen G from the best parameters, over every pair
     mlp_forward_kernel<<<grid_of(tiles, 4096), kBlock, 0, s>>>(tiles, L.Pbest.as<float>(), L.X.as<short>(),
                                                                L.y.as<float>());
@@ -2125,6 +2189,11 @@ void learned_train(tf_ctx* c) {
     for (DevBuf* b : {&L.Rb, &L.Cb, &L.T1, &L.T2, &L.ghat, &L.gpos, &L.X, &L.ahat, &L.pos2pair})
         b->release();
     L.train_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (timing)
+        std::fprintf(stderr,
+                     "[learned] tables wait %.3f s, upload %.3f s, training %.3f s (%d iterations, %d window rows), "
+                     "G %.3f s\n",
+                     t_wait, t_upload, t_loop, L.train_iters, L.nw, since(t2));
     L.state = LearnedPc::kActive;
 }
 
@@ -2567,7 +2636,7 @@ void batch_sources(tf_ctx* c) {
 //
 // For each SELL kernel of the context: candidate grid sizes (one slice per
 // wave, 1/2, 1/4, 1/8 of that, and 256..8192 blocks) x gather batches
-// {4, 8, 16}; each candidate is launched twice to warm up and timed over
+// {4, 8, 16} x slice mappings {chunk, band}; each candidate is launched twice to warm up and timed over
 // five launches with HIP events; the fastest is kept. Results are cached per
 // (kernel, precision, rows, entries) in the process and, when the
 // environment names a file in TETFIT_TUNE_CACHE, in that file.
@@ -2610,7 +2679,10 @@ KTune tune_one(tf_ctx* c, int k, const Sell& m, int cap, L launch) {
     if (it != g_tune_cache.end()) return it->second;
     const int need = sell_blocks(m, INT32_MAX);
     std::vector<int> cand;
-    for (int b : {need, (need + 1) / 2, (need + 3) / 4, (need + 7) / 8, 256, 512, 1024, 2048, 4096, 8192})
+    // whole multiples of the 256 CUs up to 8 workgroups per CU (kernels whose
+    // registers allow fewer resident waves run best at those), then larger
+    for (int b : {need, (need + 1) / 2, (need + 3) / 4, (need + 7) / 8, 256, 512, 768, 1024, 1280, 1536, 1792, 2048,
+                  4096, 8192})
         if (b >= 1 && b <= std::min(need, cap)) cand.push_back(b);
     std::sort(cand.begin(), cand.end());
     cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
@@ -3027,50 +3099,63 @@ tf_status tf_read_losses(tf_ctx* c, double* vals, int64_t cap, int64_t* n) {
     return TF_OK;
 }
 
+// Energy matrices of the context: E and N (tet methods), the ep_vec / nu_vec
+// weights (fdtd); batches of dga problems carry none.
+static bool has_energy(const tf_ctx* c) {
+    return c->method == TF_METHOD_FDTD ? c->fit_ep.p != nullptr : c->Efull.nrows == c->E;
+}
+
 tf_status tf_energy(tf_ctx* c, double* we, double* wh) {
     if (!c) { g_err = "null argument"; return TF_ERR_ARG; }
-    if (c->method == TF_METHOD_FDTD ? !c->fit_ep.p : c->Efull.nrows != c->E) {
+    if (!has_energy(c)) {
         g_err = "no energy matrices in this context (batches of dga problems carry none)";
         return TF_ERR_STATE;
     }
     return guarded([&] {
-        const int nb = 1024;
-        DevBuf part;
-        part.alloc(2 * nb * sizeof(double));
-        double* pp = part.as<double>();
-        int ge = nb, gn = nb;  // partials written by each reduction
-        if (c->method != TF_METHOD_FDTD) {
-            // We = 1/2 U.E U, Wh = 1/2 B.N B (D:2773-2775)
-            ge = sell_blocks(c->Efull, nb);
-            gn = sell_blocks(c->N, nb);
-            if (c->prec == TF_PREC_FP64) {
-                TF_LAUNCH_NB(c->wide, 8, quad_form_kernel, double, ge, c->stream, c->Efull.vals<double>(),
-                             c->U.as<double>(), pp, nullptr);
-                TF_LAUNCH_NB(c->wide, 8, quad_form_kernel, double, gn, c->stream, c->N.vals<double>(),
-                             c->B.as<double>(), pp + nb, nullptr);
-            } else {
-                TF_LAUNCH_NB(c->wide, 8, quad_form_kernel, float, ge, c->stream, c->Efull.vals<float>(),
-                             c->U.as<float>(), pp, nullptr);
-                TF_LAUNCH_NB(c->wide, 8, quad_form_kernel, float, gn, c->stream, c->N.vals<float>(),
-                             c->B.as<float>(), pp + nb, nullptr);
-            }
-        } else if (c->prec == TF_PREC_FP64) {
-            weighted_sq_kernel<double><<<nb, kBlock, 0, c->stream>>>(c->E, c->U.as<double>(), c->fit_ep.as<double>(), pp);
-            weighted_sq_kernel<double><<<nb, kBlock, 0, c->stream>>>(c->F, c->B.as<double>(), c->fit_nu.as<double>(),
-                                                                     pp + nb);
-        } else {
-            weighted_sq_kernel<float><<<nb, kBlock, 0, c->stream>>>(c->E, c->U.as<float>(), c->fit_ep.as<double>(), pp);
-            weighted_sq_kernel<float><<<nb, kBlock, 0, c->stream>>>(c->F, c->B.as<float>(), c->fit_nu.as<double>(),
-                                                                    pp + nb);
-        }
-        std::vector<double> h(2 * nb);
-        TF_HIP(hipMemcpyAsync(h.data(), pp, h.size() * 8, hipMemcpyDeviceToHost, c->stream));
+        if (!c->epart.p) c->epart.alloc(2 * kEnergyParts * sizeof(double));
+        int ge = 0, gn = 0;
+        if (c->prec == TF_PREC_FP64) energy_partials<double>(c, nullptr, ge, gn);
+        else energy_partials<float>(c, nullptr, ge, gn);
+        std::vector<double> h(2 * kEnergyParts);
+        TF_HIP(hipMemcpyAsync(h.data(), c->epart.p, h.size() * 8, hipMemcpyDeviceToHost, c->stream));
         TF_HIP(hipStreamSynchronize(c->stream));
         double a = 0, b = 0;
         for (int k = 0; k < ge; ++k) a += h[k];
-        for (int k = 0; k < gn; ++k) b += h[nb + k];
+        for (int k = 0; k < gn; ++k) b += h[kEnergyParts + k];
         if (we) *we = 0.5 * a;
         if (wh) *wh = 0.5 * b;
+    });
+}
+
+tf_status tf_set_energy_output(tf_ctx* c, int32_t on) {
+    if (!c) { g_err = "null argument"; return TF_ERR_ARG; }
+    if (on && !has_energy(c)) {
+        g_err = "no energy matrices in this context (batches of dga problems carry none)";
+        return TF_ERR_STATE;
+    }
+    return guarded([&] {
+        drop_graph(c);
+        if (on && !c->epart.p) c->epart.alloc(2 * kEnergyParts * sizeof(double));
+        c->energy_out = on != 0;
+        // a new series: samples taken so far are dropped
+        c->n_samples = 0;
+        c->sample_times.clear();
+        c->sample_cap = 0;
+        c->samples.release();
+        c->eser.release();
+    });
+}
+
+tf_status tf_read_energy(tf_ctx* c, double* times, double* vals, int64_t cap, int64_t* n) {
+    if (!c || !n) { g_err = "null argument"; return TF_ERR_ARG; }
+    if (!c->energy_out) { g_err = "energy output is off (tf_set_energy_output)"; return TF_ERR_STATE; }
+    return guarded([&] {
+        TF_HIP(hipStreamSynchronize(c->stream));
+        *n = c->n_samples;
+        const int64_t m = std::min(cap, c->n_samples);
+        if (times)
+            for (int64_t k = 0; k < m; ++k) times[k] = c->sample_times[k];
+        if (vals && m > 0) TF_HIP(hipMemcpy(vals, c->eser.p, m * 2 * sizeof(double), hipMemcpyDeviceToHost));
     });
 }
 

@@ -28,7 +28,10 @@ def device_count():
 class Simulation:
     """Device context for one assembled simulation."""
 
-    def __init__(self, build, precision="fp64", output=True):
+    def __init__(self, build, precision="fp64", output=True, energy=None):
+        """energy: record (We, Wh) at every output sample (tf_set_energy_output);
+        None = when the simulation's output mode is l2norm or maxerror (the
+        modes in which the reference writes its energy files)."""
         L = hip_lib()
         self.build = build
         self.precision = precision
@@ -40,8 +43,12 @@ class Simulation:
         p, _, npr = build.probes_raw()
         self.n_probes = npr
         _check(L.tf_set_probes(h, p, npr))
-        if output and build.output_mode in (1, 2):
+        if output and build.output_mode in (1, 2, 3):
             _check(L.tf_set_output(h, build.output_delay, build.output_period))
+        if energy is None:
+            energy = output and build.output_mode in (2, 3)
+        if energy:
+            _check(L.tf_set_energy_output(h, 1))
         self.next_step = build.first_step
 
     def close(self):
@@ -106,6 +113,18 @@ class Simulation:
         we, wh = C.c_double(), C.c_double()
         _check(hip_lib().tf_energy(self._h, C.byref(we), C.byref(wh)))
         return we.value, wh.value
+
+    def energy_series(self):
+        """tf_read_energy: (times, We, Wh) of the output samples so far."""
+        L = hip_lib()
+        n = C.c_int64()
+        _check(L.tf_read_energy(self._h, None, None, 0, C.byref(n)))
+        k = n.value
+        times = np.zeros(max(k, 1))
+        vals = np.zeros(max(2 * k, 2))
+        dp = C.POINTER(C.c_double)
+        _check(L.tf_read_energy(self._h, times.ctypes.data_as(dp), vals.ctypes.data_as(dp), k, C.byref(n)))
+        return times[:k], vals[0:2 * k:2].copy(), vals[1:2 * k:2].copy()
 
     def poynting(self):
         L = hip_lib()
