@@ -346,3 +346,76 @@ def test_slice_stream_bitwise(workdir, prec, precond, monkeypatch):
     assert out["win"][0] == out["wide"][0]
     for a, c in zip(out["win"][1:], out["wide"][1:]):
         assert np.array_equal(a, c)
+
+
+@pytest.fixture(scope="module")
+def contrast10(tmp_path_factory):
+    d = tmp_path_factory.mktemp("extrap")
+    return Model(configs.kuhn_cavity(str(d), n=10, method="gfdtd", n_steps=80, n_probes=4, block=2, seed=5,
+                                     eps_range=(1.0, 80.0), log_eps=True, name="extrap10")).build(1)
+
+
+@pytest.mark.parametrize("guess", [1, 2])
+def test_extrapolated_start(contrast10, guess):
+    """gfdtd solves started from the linear / quadratic extrapolation of the
+    previous steps (fp64, Jacobi): every solve still meets the reference's
+    test ||r|| <= tol ||b - A U_old||, now with fewer CG iterations in total;
+    fields and probes agree with warm-started runs and the oracle to the
+    tolerance's accumulated effect; the last solve's fp64 residual, measured
+    against a tight reference solve from the same state with the host's A,
+    is within tol of the warm-start residual."""
+    import scipy.sparse as sp
+    b = contrast10
+    print(f"case: extrapolated start order {guess} E={b.n_edges}", flush=True)
+    sa = Simulation(b, "fp64")
+    st_a = sa.run()
+    Ua = sa.state()[0]
+    sa.close()
+    sb = Simulation(b, "fp64")
+    st_b1 = sb.run(n=b.n_steps - 1, guess=guess)
+    S = sb.state()
+    st_b2 = sb.run(n=1, guess=guess)
+    Ub = sb.state()[0]
+    _, samples = sb.probes()
+    sb.close()
+    tol = b.cg_tol
+    iters_a = st_a["cg_iters_total"]
+    iters_b = st_b1["cg_iters_total"] + st_b2["cg_iters_total"]
+    print("iterations warm start", iters_a, "extrapolated", iters_b, flush=True)
+    assert iters_b < iters_a
+    for st in (st_b1, st_b2):
+        assert st["cg_not_converged"] == 0 and st["cg_max_relres"] <= tol
+    assert rel_l2(Ub, Ua) <= 1e-6
+    r = pyoracle.run(b)
+    assert rel_l2(samples, r["samples"]) <= 1e-6
+    # the last solve against a tight solve from the same state
+    ref = Simulation(b, "fp64", output=False)
+    ref.set_state(*S)
+    ref.run(n=1, first=b.first_step + b.n_steps - 1, cg_tol=1e-13, cg_maxiter=5000)
+    Ustar = ref.state()[0]
+    ref.close()
+    ops = b.dga_ops()
+    A, free = ops["A"], ops["free_edges"]
+    Am = sp.csr_matrix((A["val"], A["col"], A["rowptr"]), shape=(A["nrows"], A["ncols"]))
+    ratio = np.linalg.norm(Am @ (Ustar[free] - Ub[free])) / np.linalg.norm(Am @ (Ustar[free] - S[0][free]))
+    assert ratio <= 1.05 * tol, ratio
+
+
+def test_extrapolated_start_learned(contrast10):
+    """The learned fp32 path with quadratic-extrapolated starts: G trained
+    and kept (no fallbacks), every solve converged, fields against the
+    oracle to the fp32 bar."""
+    b = contrast10
+    print(f"case: extrapolated start learned fp32 E={b.n_edges}", flush=True)
+    sim = Simulation(b, "fp32")
+    s1 = sim.run(n=30, precond="learned", learn_steps=30, train_iters=60, guess=2)
+    s2 = sim.run(n=b.n_steps - 30, precond="learned", guess=2)
+    _, samples = sim.probes()
+    U, F, B = sim.state()
+    sim.close()
+    assert s2["pc_state"] == 2 and s2["pc_fallbacks"] == 0
+    assert s1["cg_not_converged"] == 0 and s2["cg_not_converged"] == 0 and s2["cg_max_relres"] <= b.cg_tol
+    r = pyoracle.run(b)
+    assert rel_l2(samples, r["samples"]) <= 1e-4
+    for got, want in ((U, r["U"]), (F, r["F"]), (B, r["B"])):
+        assert rel_l2(got, want) <= 1e-4

@@ -94,7 +94,8 @@ class tf_probe_geom(C.Structure):
 
 class tf_cg_opts(C.Structure):
     _fields_ = [("tol", C.c_double), ("maxiter", C.c_int32), ("precond", C.c_int32),
-                ("learn_steps", C.c_int32), ("train_iters", C.c_int32), ("lr", C.c_double)]
+                ("learn_steps", C.c_int32), ("train_iters", C.c_int32), ("lr", C.c_double),
+                ("guess", C.c_int32)]
 
 
 class tf_stats(C.Structure):

@@ -42,6 +42,7 @@ struct CgState {
     int32_t iter;      // iterations done when the solve stopped
     int32_t done;
     int32_t need_conv;  // the solve must meet tol (learned solves; Jacobi ones may stop at maxiter)
+    int32_t rr0_given;  // rr0 holds the warm-start residual of an extrapolated start (cg_set_rr0_kernel)
 };
 
 // Guard of the kernels a gfdtd step enqueues ahead of the host's check of
@@ -52,6 +53,31 @@ struct CgState {
 // once the solve is complete. A null guard always passes.
 __device__ __forceinline__ bool cg_guard_ok(const CgState* g) {
     return !g || (g->done && (!g->need_conv || g->relres <= g->tol));
+}
+
+// Extrapolated start of a gfdtd solve (an option of the fast path; the
+// stop test stays relative to the warm-start residual b - A U_old). On the
+// free rows x holds U_n (the warm start), h1 = U_{n-1}, h2 = U_{n-2}:
+// order 1: x0 = 2 U_n - U_{n-1}; order 2: x0 = 3 U_n - 3 U_{n-1} + U_{n-2};
+// order 0: x unchanged. d = x0 - U_n as the rounded x0 gives it (exact when
+// x0 and U_n lie within a factor of two, Sterbenz), for r0 - A d; the
+// history then shifts by one step. Guard as in cg_guard_ok.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) cg_extrap_kernel(int32_t n, T* __restrict__ x, T* __restrict__ h1,
+                                                           T* __restrict__ h2, T* __restrict__ d, int order,
+                                                           const CgState* guard) {
+    if (!cg_guard_ok(guard)) return;
+    for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const T xi = x[i], a = h1[i], b = h2[i];
+        const double x0 = order == 2 ? 3.0 * (double)xi - 3.0 * (double)a + (double)b
+                          : order == 1 ? 2.0 * (double)xi - (double)a
+                                       : (double)xi;
+        const T x0t = (T)x0;
+        d[i] = (T)((double)x0t - (double)xi);
+        x[i] = x0t;
+        h2[i] = a;
+        h1[i] = xi;
+    }
 }
 
 // ------------------------------------------------------------------ FIT

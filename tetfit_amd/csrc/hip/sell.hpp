@@ -471,14 +471,18 @@ constexpr int kRed2 = 2 * (kBlock / kWave) + 2;
 // block 0 records it. A zero initial residual (a quiet step: the warm start
 // already solves the system) stops at once as in the reference, whose 0/0
 // fails the `> tol` test; its ratio is recorded as 0 so that a learned solve
-// counts as converged.
-__device__ __forceinline__ bool cg_decide_stop(int32_t it, double RR, double rr0_prev, double tol, int32_t maxiter,
-                                               CgState* st) {
-    const double rr0 = it == 0 ? RR : rr0_prev;
+// counts as converged. After an extrapolated start (rr0_given) r0 is the
+// warm-start residual b - A U_old set before pass 0, so the test is the
+// reference's for the warm start while the iterates start elsewhere; a zero
+// warm-start residual there falls back to the start's own ||r_0||.
+__device__ __forceinline__ bool cg_decide_stop(int32_t it, double RR, double rr0_prev, int32_t rr0_given,
+                                               double tol, int32_t maxiter, CgState* st) {
+    const bool own = it == 0 && !(rr0_given && rr0_prev > 0.0);
+    const double rr0 = own ? RR : rr0_prev;
     const double ratio = rr0 == 0.0 ? 0.0 : std::sqrt(RR) / std::sqrt(rr0);
     const bool stop = !(ratio > tol && it < maxiter);
     if (threadIdx.x == 0 && blockIdx.x == 0) {
-        if (it == 0) st->rr0 = RR;
+        if (own) st->rr0 = RR;
         st->relres = ratio;
         if (stop) { st->done = 1; st->iter = it; }
     }
@@ -540,12 +544,12 @@ __global__ void __launch_bounds__(kBlock) cg_w_kernel(int32_t it, SellView<T> m,
     const Partials prr = load_partials(part_rr, test_stop ? nb_rr : 0);
     const int done = st->done;
     const double rr0 = st->rr0, tol = st->tol;
-    const int32_t maxiter = st->maxiter;
+    const int32_t maxiter = st->maxiter, given = st->rr0_given;
     if (done) return;
     if (test_stop) {
         double RR = thread_sum(prr), unused = 0;
         block_sum2_all(RR, unused, red);
-        if (cg_decide_stop(it, RR, rr0, tol, maxiter, st)) return;
+        if (cg_decide_stop(it, RR, rr0, given, tol, maxiter, st)) return;
     }
     const int lane = lane_id();
     double wu = 0, none = 0;
@@ -851,11 +855,11 @@ __global__ void __launch_bounds__(kBlock) cg_wu_kernel(int32_t it, SellView<T> m
     const Partials prr = load_partials(part_rr, nb_pg), pga = load_partials(part_gam, nb_pg);
     const int done = st->done;
     const double rr0 = st->rr0, tol = st->tol, gam_prev = st->gam[(it - 1) & 1];
-    const int32_t maxiter = st->maxiter;
+    const int32_t maxiter = st->maxiter, given = st->rr0_given;
     if (done) return;
     double RR = thread_sum(prr), gam = thread_sum(pga);
     block_sum2_all(RR, gam, red);
-    if (cg_decide_stop(it, RR, rr0, tol, maxiter, st)) return;
+    if (cg_decide_stop(it, RR, rr0, given, tol, maxiter, st)) return;
     const double beta = it == 0 ? 0.0 : gam / gam_prev;
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         st->gam[it & 1] = gam;
@@ -945,11 +949,11 @@ __global__ void __launch_bounds__(kBlock) gf_rhs_init_kernel(
     const T* __restrict__ U, const T* __restrict__ dir_new, const T* __restrict__ Fv, const T* __restrict__ fb,
     T* __restrict__ r0save, RhsCoef co, const T* __restrict__ pre, T* __restrict__ r, int r_stride,
     T* __restrict__ u, double* __restrict__ part_rr, double* __restrict__ part_gam, CgState* st, int jacobi,
-    const CgState* guard) {
+    const CgState* guard, SellView<T> am, const T* __restrict__ d, double* __restrict__ part_rr0) {
     if (!cg_guard_ok(guard)) return;
     __shared__ double red[kRed2];
     const int lane = lane_id();
-    double rr = 0, ru = 0;
+    double rr = 0, ru = 0, rr0 = 0;
     TF_FOR_SLICES(ct) {
         const int s = sr_.s;
         const double ku = dir_terms && kop.sptr[s + 1] > kop.sptr[s]
@@ -959,10 +963,17 @@ __global__ void __launch_bounds__(kBlock) gf_rhs_init_kernel(
                               ? sell_dot<T, 8, W>(r1, s, lane, [&](int32_t j) { return dir_new[j]; })
                               : 0.0;
         const double cf = sell_dot<T, NB, W>(ct, s, lane, [&](int32_t j) { return Fv[j]; });
+        // extrapolated start x0 = U_old + d: r = r0 - A d (r0 = b - A U_old)
+        const double ad = d ? sell_dot<T, NB, W>(am, s, lane, [&](int32_t j) { return d[j]; }) : 0.0;
         const int32_t k = s * kSlice + lane;
         if (k < ct.nrows) {
-            const T rk = (T)((co.ck * ku + co.cr * rs) + co.cc * cf + co.cf * (double)fb[free_edge[k]]);
-            r0save[k] = rk;
+            const T rk0 = (T)((co.ck * ku + co.cr * rs) + co.cc * cf + co.cf * (double)fb[free_edge[k]]);
+            r0save[k] = rk0;
+            T rk = rk0;
+            if (d) {
+                rr0 += (double)rk0 * (double)rk0;
+                rk = (T)((double)rk0 - ad);
+            }
             r[(int64_t)k * r_stride] = rk;
             rr += (double)rk * (double)rk;
             if (jacobi) {
@@ -973,11 +984,28 @@ __global__ void __launch_bounds__(kBlock) gf_rhs_init_kernel(
         }
     }
     block_sum2_all(rr, ru, red);
+    if (d) {
+        double none = 0;
+        block_sum2_all(rr0, none, red);
+    }
     if (threadIdx.x == 0) {
         part_rr[blockIdx.x] = rr;
         if (jacobi) part_gam[blockIdx.x] = ru;
+        if (d) part_rr0[blockIdx.x] = rr0;
         if (blockIdx.x == 0) { st->done = 0; st->iter = 0; }
     }
+}
+
+// rr0 of an extrapolated start: ||b - A U_old||^2 from gf_rhs_init's
+// partials, summed in order on one thread, marked as given for the stop
+// test of pass 0 (cg_decide_stop); guard as in cg_guard_ok.
+__global__ void __launch_bounds__(kBlock) cg_set_rr0_kernel(const double* __restrict__ part, int nb, CgState* st,
+                                                            const CgState* guard) {
+    if (!cg_guard_ok(guard) || threadIdx.x != 0) return;
+    double s = 0;
+    for (int k = 0; k < nb; ++k) s += part[k];
+    st->rr0 = s;
+    st->rr0_given = 1;
 }
 
 }  // namespace tfd

@@ -257,6 +257,12 @@ struct tf_ctx {
     DevBuf c_col, c_sg, h_edge, curl_f, curl_u;
     DevBuf free_edge, dir_edge, dir_new, fb, pre, x, r, z, p0, p1, q, r0, cg_part, cg_state;
     DevBuf rs0, rs1;  // (r, s) pairs of the learned passes, even / odd pass
+    // extrapolated starts (tf_cg_opts.guess): U_{n-1}, U_{n-2} on the free
+    // rows, the start's shift d, gf_rhs_init's partials of ||b - A U_old||^2;
+    // hist = steps of history held, valid for the solve of step hist_next
+    DevBuf hx1, hx2, dx, rr0part;
+    int hist = 0;
+    int64_t hist_next = -1;
     std::vector<int32_t> h_perm;  // SELL row r of H holds row h_perm[r] of the problem's H
     int32_t n_free = 0, n_dir = 0;
     // gfdtd: every Dirichlet value of every step is zero (pec values, no e
@@ -359,7 +365,7 @@ size_t tf_ctx::device_bytes() const {
     const DevBuf* all[] = {&U, &Fv, &B, &fit_ep, &fit_nu, &fe_sg, &fe_ca, &fe_cb, &fe_f4, &fh_sg, &fh_cm, &fh_cn, &fh_e4, &fe_code, &fh_code,
                            &sp_edge, &sp_f4, &sp_sg, &sp_ca, &sp_cb, &sp_kind, &sp_tb, &sp_te, &c_col, &c_sg,
                            &h_edge, &curl_f, &curl_u, &free_edge, &dir_edge, &dir_new, &fb, &pre, &x, &r, &rs0, &rs1, &z,
-                           &p0, &p1, &q, &r0, &cg_part, &cg_state, &inj_edge, &inj_kind, &inj_tb, &inj_te,
+                           &p0, &p1, &q, &r0, &hx1, &hx2, &dx, &rr0part, &cg_part, &cg_state, &inj_edge, &inj_kind, &inj_tb, &inj_te,
                            &term_src, &term_w, &carr, &probes, &samples, &eser, &epart, &poy, &row_dt_h, &row_dt_f,
                            &mu_a, &mu_b, &tf_a, &tf_b, &fslot, &uf[0], &uf[1], &ff[0], &ff[1], &U_older, &fw, &fw2};
     for (auto b : all) s += b->bytes;
@@ -1653,8 +1659,12 @@ struct CgView {
 // solve: with_rhs, the step's initial residual (gf_rhs_init_kernel, guarded,
 // which also keeps it in r0); otherwise a restart from r0 (cg_restart_kernel)
 // with x back at the step's warm start.
+// extrap (with_rhs only): -1 warm start; 0..2 the order of the extrapolated
+// start (0: history shift only); launch_extrap false reuses the shift d of
+// an earlier start of the same solve (x already holds the start).
 template <class T>
-void cg_start(tf_ctx* c, CgSolve& cs, bool with_rhs, const CgState* guard) {
+void cg_start(tf_ctx* c, CgSolve& cs, bool with_rhs, const CgState* guard, int extrap = -1,
+              bool launch_extrap = true) {
     const CgView<T> v(c, cs.learned);
     CgState* dst = c->cg_state.as<CgState>() + cs.slot;
     cs.init.need_conv = cs.learned ? 1 : 0;
@@ -1669,12 +1679,19 @@ void cg_start(tf_ctx* c, CgSolve& cs, bool with_rhs, const CgState* guard) {
         // dt^2 - 0.5/dt SigMat W2) - Ct (N C U_old) + fb / dt
         const double dt = c->dt;
         const RhsCoef co = c->fem ? RhsCoef{1 / (dt * dt), -0.5 / dt, -1.0, 1 / dt} : RhsCoef{1.0, -1.0, dt, dt};
+        if (extrap >= 0 && launch_extrap)
+            launch_timed(c, "cg_extrap_kernel", cg_extrap_kernel<T>, blocks_for(v.nf), v.nf, v.x, c->hx1.as<T>(),
+                         c->hx2.as<T>(), c->dx.as<T>(), extrap, guard);
+        const bool shifted = extrap > 0;
         TF_LAUNCH_TIMED(c, "gf_rhs_init_kernel", 8, gf_rhs_init_kernel, T, v.nb_b, c->Kf.vals<T>(),
                         c->R1f.vals<T>(), c->Ctf.signs(), (cs.dir_terms || c->fem) ? 1 : 0,
                         c->free_edge.as<int32_t>(), c->fem ? c->fw.as<T>() : c->U.as<T>(),
                         c->fem ? c->fw2.as<T>() : c->dir_new.as<T>(), c->Fv.as<T>(), c->fb.as<T>(),
                         c->r0.as<T>(), co, c->pre.as<T>(), r0, r0_stride, v.u, v.part_rr, v.part_gj[0], dst,
-                        cs.learned ? 0 : 1, guard);
+                        cs.learned ? 0 : 1, guard, v.A, shifted ? (const T*)c->dx.as<T>() : nullptr,
+                        c->rr0part.as<double>());
+        if (shifted)
+            launch_timed(c, "cg_set_rr0_kernel", cg_set_rr0_kernel, 1, c->rr0part.as<double>(), v.nb_b, dst, guard);
     } else {
         launch_timed(c, "cg_restart_kernel", cg_restart_kernel<T>, v.nb_b, v.nf, c->pre.as<T>(), c->r0.as<T>(), r0,
                      r0_stride, v.u, v.part_rr, v.part_gj[0], dst, cs.learned ? 0 : 1);
@@ -2213,6 +2230,7 @@ void learned_train(tf_ctx* c) {
 struct GfPre {
     int64_t sample_slot = -1;  // probe sample slot of this step (-1: none)
     bool learned = false;      // preconditioner the step's gf_rhs_init was launched for
+    int extrap = -1;           // start of the step's solve (cg_start's extrap)
 };
 
 // The learned preconditioner state the next solve will see: active, or
@@ -2245,6 +2263,24 @@ void run_gfdtd(tf_ctx* c, int64_t first, int64_t n, const tf_cg_opts* cg, tf_sta
     const bool want_learned = cg && cg->precond == TF_PRECOND_LEARNED;
     if (want_learned && c->lp.state == LearnedPc::kOff) learned_setup(c, cg);
     if (!c->solve_ev) TF_HIP(hipEventCreateWithFlags(&c->solve_ev, hipEventDisableTiming));
+    // extrapolated starts: history buffers on first use
+    const int guess = cg && !c->fem ? cg->guess : 0;
+    if (guess && !c->dx.p) {
+        const size_t nb = (size_t)std::max(nf, 1) * sizeof(T);
+        for (DevBuf* b : {&c->hx1, &c->hx2, &c->dx}) b->alloc(nb);
+        c->rr0part.alloc(kCgPartB * sizeof(double));
+        c->hist = 0;
+    }
+    // the start of step i's solve: the history counts the solves of steps
+    // i-1, i-2, ... just before it (a jump in steps or a state write clears it)
+    auto start_of = [&](int64_t i) {
+        if (!guess) return -1;
+        if (c->hist_next != i) c->hist = 0;
+        const int order = std::min(guess, c->hist);
+        c->hist = std::min(c->hist + 1, 2);
+        c->hist_next = i + 1;
+        return order;
+    };
     LearnedPc& L = c->lp;
     const int gb_n = tuned_blocks(c, TK_N, c->N, kSpmvCap);
     CgState* states = c->cg_state.as<CgState>();
@@ -2289,7 +2325,8 @@ void run_gfdtd(tf_ctx* c, int64_t first, int64_t n, const tf_cg_opts* cg, tf_sta
         s0.learned = g.learned;
         s0.slot = (int)(k & 1);
         s0.dir_terms = dir_terms(k);
-        cg_start<T>(c, s0, true, guard);
+        if (fresh) g.extrap = start_of(i);
+        cg_start<T>(c, s0, true, guard, g.extrap);
     };
     auto scatter = [&](const CgState* guard) {
         launch_timed(c, "gf_scatter_kernel", gf_scatter_kernel<T>, blocks_for(nf + c->n_dir), nf,
@@ -2343,7 +2380,7 @@ void run_gfdtd(tf_ctx* c, int64_t first, int64_t n, const tf_cg_opts* cg, tf_sta
         if (learned != cur.learned) {
             cur.learned = learned;
             sol.dir_terms = dir_terms(k);
-            cg_start<T>(c, sol, true, nullptr);
+            cg_start<T>(c, sol, true, nullptr, cur.extrap, false);
         } else {
             sol.init.need_conv = learned ? 1 : 0;
             sol.it = 0;
@@ -2925,7 +2962,8 @@ tf_status tf_run(tf_ctx* c, int64_t first, int64_t n, const tf_cg_opts* cg, tf_s
         if (n == 0) return;
         if (cg && cg->precond != TF_PRECOND_JACOBI && cg->precond != TF_PRECOND_LEARNED)
             throw std::runtime_error("unknown preconditioner");
-        if (cg && (cg->maxiter < 0 || !(cg->tol >= 0))) throw std::runtime_error("bad CG options");
+        if (cg && (cg->maxiter < 0 || !(cg->tol >= 0) || cg->guess < 0 || cg->guess > 2))
+            throw std::runtime_error("bad CG options");
         ensure_carr(c, first, n);
         c->ktimes.clear();
         hipEvent_t e0, e1;
@@ -2993,6 +3031,8 @@ tf_status tf_write_state(tf_ctx* c, const double* U, const double* F, const doub
     if (!c) { g_err = "null argument"; return TF_ERR_ARG; }
     return guarded([&] {
         TF_HIP(hipStreamSynchronize(c->stream));
+        c->hist = 0;  // extrapolated starts begin again from the written state
+        c->hist_next = -1;
         auto wr = [&](DevBuf& b, const double* src, int64_t n) {
             if (!src) return;
             if (c->prec == TF_PREC_FP64) {

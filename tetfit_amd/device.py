@@ -60,15 +60,17 @@ class Simulation:
         self.close()
 
     def run(self, n=None, first=None, cg_tol=None, cg_maxiter=100, precond="jacobi", learn_steps=0,
-            train_iters=0, lr=0.0):
+            train_iters=0, lr=0.0, guess=0):
         """Advances n steps from `first` (default: where the last run ended).
         precond "learned" trains the preconditioner during the context's first
-        `learn_steps` solves (0: 200) and uses it afterwards."""
+        `learn_steps` solves (0: 200) and uses it afterwards. guess: start of
+        each gfdtd solve, 0 the warm start (the reference), 1 / 2 linear /
+        quadratic extrapolation from the previous steps (same stop test)."""
         L = hip_lib()
         first = self.next_step if first is None else first
         n = self.build.n_steps - (first - self.build.first_step) if n is None else n
         opts = _ffi.tf_cg_opts(self.build.cg_tol if cg_tol is None else cg_tol, int(cg_maxiter), PRECOND[precond],
-                               int(learn_steps), int(train_iters), float(lr))
+                               int(learn_steps), int(train_iters), float(lr), int(guess))
         st = _ffi.tf_stats()
         _check(L.tf_run(self._h, int(first), int(n), C.byref(opts), C.byref(st)))
         self.next_step = first + n
