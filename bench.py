@@ -41,6 +41,10 @@ METRIC = "edge-DoF updates/s + CG iters/step: learned+FP32 vs plain FP64, 1 GPU"
 # these meshes (power iteration; see tetfit_amd/configs.py)
 DT48 = {"dga": 2.281e-11, "gfdtd": 4.777e-11}
 DT96_CONTRAST = {"gfdtd": 2.264e-11}
+# CG start of the fast (learned+FP32) path's solves: quadratic extrapolation
+# 3U_n - 3U_{n-1} + U_{n-2} (tf_cg_opts.guess); the stop test stays the
+# reference's, relative to the warm-start residual. Plain FP64 starts warm.
+FAST_GUESS = 2
 
 
 def log(*a):
@@ -144,12 +148,14 @@ def roofline_of(info, kernels, names, vb):
 def time_path(b, precision, steps, warmup, precond="jacobi", label="", learn_steps=200):
     """Times `steps` steps after `warmup` untimed ones. For the learned
     preconditioner the context first runs its `learn_steps` Jacobi-PCG
-    collection steps and the on-device training (untimed, reported apart)."""
+    collection steps and the on-device training (untimed, reported apart);
+    its solves start from the quadratic extrapolation (FAST_GUESS)."""
     from tetfit_amd.device import Simulation
     sim = Simulation(b, precision, output=False)
     learn = None
+    g = FAST_GUESS if precond == "learned" else 0
     if precond == "learned":
-        ls = sim.run(n=learn_steps, precond="learned", learn_steps=learn_steps)
+        ls = sim.run(n=learn_steps, precond="learned", learn_steps=learn_steps, guess=g)
         learn = dict(learn_steps=learn_steps, collection_seconds=ls["seconds"] - ls["pc_train_seconds"],
                      train_seconds=ls["pc_train_seconds"], pairs=ls["pc_pairs"], loss0=ls["pc_loss0"],
                      loss=ls["pc_loss"], jacobi_iters_per_step=ls["pc_jacobi_mean_iters"], state=ls["pc_state"])
@@ -158,8 +164,8 @@ def time_path(b, precision, steps, warmup, precond="jacobi", label="", learn_ste
     tuned = sim.autotune()
     log(f"[bench] {label}: launch shapes {tuned}")
     if warmup:
-        sim.run(n=warmup, precond=precond)
-    st = sim.run(n=steps, precond=precond)
+        sim.run(n=warmup, precond=precond, guess=g)
+    st = sim.run(n=steps, precond=precond, guess=g)
     sec = st["seconds"]
     out = dict(seconds=sec, ms_per_step=1e3 * sec / steps,
                edge_dof_per_s=b.n_edges * steps / sec,
@@ -176,7 +182,7 @@ def time_path(b, precision, steps, warmup, precond="jacobi", label="", learn_ste
         out["learned"] = learn
     # per-kernel durations with HIP events on the context's stream
     sim.set_profiling(True)
-    sim.run(n=min(steps, 50), precond=precond)
+    sim.run(n=min(steps, 50), precond=precond, guess=g)
     out["kernels"] = {k: {"ms_avg": v[0], "launches": v[1]} for k, v in sim.kernel_times().items()}
     sim.close()
     log(f"[bench] {label}: {out['ms_per_step']:.3f} ms/step, {out['edge_dof_per_s'] / 1e9:.3f} G edge-DoF/s"
@@ -235,7 +241,7 @@ def fit_bytes(b, kernel, vb, grid_order=True):
     return F * (5 * vb + cb) + E * vb
 
 
-def run_chunks(sim, steps, chunk, label, precond="jacobi", learn_steps=200, stop_before_last=False):
+def run_chunks(sim, steps, chunk, label, precond="jacobi", learn_steps=200, stop_before_last=False, guess=0):
     """Advances `steps` steps in tf_run calls of at most `chunk` steps (one
     progress line each), wall-clocked around all calls (each returns after
     its stream has drained). The learned path's first call covers exactly the
@@ -253,7 +259,7 @@ def run_chunks(sim, steps, chunk, label, precond="jacobi", learn_steps=200, stop
         if precond == "learned" and done == 0:
             k = min(learn_steps, todo)
         tc = time.perf_counter()
-        st = sim.run(n=k, precond=precond, learn_steps=learn_steps)
+        st = sim.run(n=k, precond=precond, learn_steps=learn_steps, guess=guess)
         if precond == "learned" and done == 0:
             out["pc"] = st
             out["shapes"] = sim.autotune()
@@ -317,9 +323,9 @@ def bench_headline(args):
             sim = Simulation(bg, prec, output=False)
             sim.autotune()
             if pc == "learned":
-                sim.run(n=args.learn_steps, precond="learned", learn_steps=args.learn_steps)
+                sim.run(n=args.learn_steps, precond="learned", learn_steps=args.learn_steps, guess=FAST_GUESS)
                 sim.autotune()
-            sim.run(n=max(args.warmup, 1), precond=pc)
+            sim.run(n=max(args.warmup, 1), precond=pc, guess=FAST_GUESS if pc == "learned" else 0)
             sim.close()
             log(f"[bench] warm-up {prec} {pc} done")
         # plain FP64, whole job
@@ -331,11 +337,11 @@ def bench_headline(args):
         # learned + FP32, whole job (the last step taken apart for the residual check)
         l32 = Simulation(bg, "fp32")
         l32.autotune()
-        sec32a, r32 = run_chunks(l32, steps, chunk, "learned fp32", precond="learned",
+        sec32a, r32 = run_chunks(l32, steps, chunk, "learned fp32", precond="learned", guess=FAST_GUESS,
                                  learn_steps=args.learn_steps, stop_before_last=True)
         before = l32.state()
         t = time.perf_counter()
-        st_last = l32.run(n=1, precond="learned")
+        st_last = l32.run(n=1, precond="learned", guess=FAST_GUESS)
         sec32 = sec32a + (time.perf_counter() - t)
         r32["cg_iters_total"] += st_last["cg_iters_total"]
         r32["cg_max_relres"] = max(r32["cg_max_relres"], st_last["cg_max_relres"])
@@ -353,7 +359,7 @@ def bench_headline(args):
         # per-kernel times (HIP events on the context's stream) over 30 more steps
         first_prof = l32.next_step
         l32.set_profiling(True)
-        l32.run(n=30, precond="learned")
+        l32.run(n=30, precond="learned", guess=FAST_GUESS)
         kern32 = {k: {"ms_avg": v[0], "launches": v[1]} for k, v in l32.kernel_times().items()}
         info_a32 = l32.sell_info("a")
         l32.close()
@@ -429,7 +435,11 @@ def bench_headline(args):
             "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
             "config": {"workload": f"Kuhn {n}^3 jittered tet cavity (seed 5), eps_r log-uniform [1,80] per 6^3 "
                                    f"block, method gfdtd (CG tol 1e-8), {steps} steps end to end",
-                       "path": "learned+FP32 (200 collection solves, training and set-up inside the timed job)",
+                       "path": "learned+FP32 (200 collection solves, training and set-up inside the timed job; "
+                               "CG starts from the quadratic extrapolation of the previous steps, stop test relative "
+                               "to the warm-start residual as in the reference)",
+                       "cg_start": {"learned_fp32": "quadratic extrapolation (collection solves warm)",
+                                    "plain_fp64": "warm start U_n (the reference)"},
                        "n_edges": E, "n_faces": bg.n_faces, "parallelism": "single GPU"},
             "cg_iters_per_step": r32["cg_iters_total"] / steps,
             "plain_fp64": {"edge_dof_per_s": plain_rate, "ms_per_step": 1e3 * sec64 / steps,

@@ -69,13 +69,14 @@ typedef struct {
     int32_t maxiter;           /* 100 in the reference callers (D:1388)       */
     int32_t precond;           /* tf_precond_kind                             */
     int32_t learn_steps;       /* LEARNED: solves that supply training pairs (0: 200) */
-    int32_t train_iters;       /* LEARNED: Adam steps (0: 150)                */
+    int32_t train_iters;       /* LEARNED: Adam steps (0: 80)                 */
     double  lr;                /* LEARNED: Adam learning rate (0: 3e-3)       */
     int32_t guess;             /* gfdtd start of each solve: 0 the warm start
                                   U_n (the reference); 1 / 2 linear / quadratic
                                   extrapolation 2U_n - U_{n-1} /
                                   3U_n - 3U_{n-1} + U_{n-2} once the history
-                                  holds enough steps. The stop test stays
+                                  holds enough steps (LEARNED: the collection
+                                  solves start warm). The stop test stays
                                   ||r|| <= tol ||b - A U_n|| (the reference's
                                   warm-start residual). No reference
                                   counterpart. */

@@ -214,7 +214,12 @@ __device__ __forceinline__ SliceMeta slice_meta(const SellView<V>& m, int s) {
 // E: one entry as loaded (column code and value); load(pos) loads the entry
 // at SELL position pos; gather(e, cw) fetches the gathered operand of e;
 // add(acc, e, g) accumulates; row_pre(s) issues the loads of slice s's row
-// vectors and returns them; row_post(s, acc, rin) finishes slice s.
+// vectors and returns them; row_post(s, acc, rin) finishes slice s;
+// next(sn), when given, runs once the next slice sn is known, before its
+// entries are loaded (loads of a row indirection one slice ahead).
+struct NoNext {
+    __device__ __forceinline__ void operator()(int) const {}
+};
 template <int U, typename E, typename Acc, typename V>
 struct SellStream {
     const SellView<V>& m;
@@ -250,8 +255,9 @@ struct SellStream {
     // slice's earlier batches and the last pass's re-fetch of its own batch
     // are spare loads of cached lines), so the compiler's wait before the
     // sums can leave exactly the next batch's U loads in flight.
-    template <class Load, class Gather, class Add, class RowPre, class RowPost>
-    __device__ __forceinline__ void run(Load load, Gather gather, Add add, RowPre row_pre, RowPost row_post) {
+    template <class Load, class Gather, class Add, class RowPre, class RowPost, class Next = NoNext>
+    __device__ __forceinline__ void run(Load load, Gather gather, Add add, RowPre row_pre, RowPost row_post,
+                                        Next next = Next{}) {
         int s = sr.s;
         if (s >= sr.end) return;
         Acc acc{};
@@ -267,7 +273,10 @@ struct SellStream {
             int kn = k0;
             if (more) {
                 kn = last ? 0 : k0 + U;
-                if (last) mn = slice_meta(m, sn);
+                if (last) {
+                    mn = slice_meta(m, sn);
+                    next(sn);
+                }
             }
             E en[U];
             fetch(mn, kn, en, load);
@@ -293,6 +302,22 @@ template <typename T, int NB, bool W = false>
 __global__ void __launch_bounds__(kBlock) sell_ct_kernel(SellView<int8_t> m, const T* __restrict__ x,
                                                          T* __restrict__ y) {
     const int lane = lane_id();
+    if constexpr (W) {
+        if (m.col16 && m.cwin) {
+            struct Ent { uint16_t c; int8_t v; };
+            SellStream<NB, Ent, T, int8_t> ss(m, block_slices(m.nslices, m.band), lane);
+            auto load = [&](int64_t pos) { return Ent{stream_load<W>(m.col16 + pos), stream_load<W>(m.val + pos)}; };
+            ss.start(load);
+            ss.run(
+                load, [&](const Ent& e, const ColWin& cw) { return x[win_col(cw, e.c)]; },
+                [&](T& a, const Ent& e, T gv) { a += (T)e.v * gv; }, [&](int) { return 0; },
+                [&](int sl, T v, int) {
+                    const int32_t r = sl * kSlice + lane;
+                    if (r < m.nrows) y[r] = v;
+                });
+            return;
+        }
+    }
     TF_FOR_SLICES(m) {
         const int s = sr_.s;
         const double v = sell_dot<T, NB, W>(m, s, lane, [&](int32_t j) { return x[j]; });
@@ -308,6 +333,28 @@ __global__ void __launch_bounds__(kBlock) sell_h_kernel(SellView<T> m, const int
                                                         const T* __restrict__ x, T* __restrict__ U, double dt,
                                                         const double* __restrict__ row_dt) {
     const int lane = lane_id();
+    if constexpr (W) {
+        if (m.col16 && m.cwin) {
+            struct Ent { uint16_t c; T v; };
+            const SliceRange sr = block_slices(m.nslices, m.band);
+            // edge of the lane's row, one slice ahead (clamped row past the end)
+            auto edge_of = [&](int sl) { return row_edge[min(sl * kSlice + lane, m.nrows - 1)]; };
+            int32_t e_cur = sr.s < sr.end ? edge_of(sr.s) : 0, e_nxt = 0;
+            SellStream<NB, Ent, T, T> ss(m, sr, lane);
+            auto load = [&](int64_t pos) { return Ent{stream_load<W>(m.col16 + pos), stream_load<W>(m.val + pos)}; };
+            ss.start(load);
+            ss.run(
+                load, [&](const Ent& e, const ColWin& cw) { return x[win_col(cw, e.c)]; },
+                [&](T& a, const Ent& e, T gv) { a += e.v * gv; }, [&](int) { return U[e_cur]; },
+                [&](int sl, T a, T uold) {
+                    const int32_t r = sl * kSlice + lane;
+                    if (r < m.nrows) U[e_cur] = (T)((double)uold + (row_dt ? row_dt[r] : dt) * (double)a);
+                    e_cur = e_nxt;
+                },
+                [&](int sn) { e_nxt = edge_of(sn); });
+            return;
+        }
+    }
     TF_FOR_SLICES(m) {
         const int s = sr_.s;
         const double v = sell_dot<T, NB, W>(m, s, lane, [&](int32_t j) { return x[j]; });
@@ -345,6 +392,9 @@ __global__ void __launch_bounds__(kBlock) sell_n_kernel(SellView<T> m, const T* 
                                                         const CgState* guard) {
     if (!cg_guard_ok(guard)) return;
     const int lane = lane_id();
+    // (the slice stream measured slower here: N's rows hold about 7 entries,
+    // one batch, and its extra registers cost more occupancy than the
+    // overlap gains)
     TF_FOR_SLICES(m) {
         const int s = sr_.s;
         const double v = sell_dot<T, NB, W>(m, s, lane, [&](int32_t j) { return x[j]; });
@@ -997,15 +1047,20 @@ __global__ void __launch_bounds__(kBlock) gf_rhs_init_kernel(
 }
 
 // rr0 of an extrapolated start: ||b - A U_old||^2 from gf_rhs_init's
-// partials, summed in order on one thread, marked as given for the stop
-// test of pass 0 (cg_decide_stop); guard as in cg_guard_ok.
+// partials (a fixed assignment of partials to threads and a fixed reduction
+// tree: the same sum every time), marked as given for the stop test of
+// pass 0 (cg_decide_stop); guard as in cg_guard_ok.
 __global__ void __launch_bounds__(kBlock) cg_set_rr0_kernel(const double* __restrict__ part, int nb, CgState* st,
                                                             const CgState* guard) {
-    if (!cg_guard_ok(guard) || threadIdx.x != 0) return;
-    double s = 0;
-    for (int k = 0; k < nb; ++k) s += part[k];
-    st->rr0 = s;
-    st->rr0_given = 1;
+    if (!cg_guard_ok(guard)) return;
+    __shared__ double red[kRed2];
+    double s = 0, none = 0;
+    for (int k = threadIdx.x; k < nb; k += blockDim.x) s += part[k];
+    block_sum2_all(s, none, red);
+    if (threadIdx.x == 0) {
+        st->rr0 = s;
+        st->rr0_given = 1;
+    }
 }
 
 }  // namespace tfd

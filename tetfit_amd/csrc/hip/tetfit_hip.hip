@@ -2044,7 +2044,7 @@ void learned_setup(tf_ctx* c, const tf_cg_opts* cg) {
     }
     L.slot.assign(kBank, 0);
     L.learn_steps = cg->learn_steps > 0 ? cg->learn_steps : 200;
-    L.train_iters = cg->train_iters > 0 ? cg->train_iters : 150;
+    L.train_iters = cg->train_iters > 0 ? cg->train_iters : 80;
     L.lr = cg->lr > 0 ? cg->lr : 3e-3;
     L.state = LearnedPc::kCollecting;
     L.steps = 0;
@@ -2272,11 +2272,14 @@ void run_gfdtd(tf_ctx* c, int64_t first, int64_t n, const tf_cg_opts* cg, tf_sta
         c->hist = 0;
     }
     // the start of step i's solve: the history counts the solves of steps
-    // i-1, i-2, ... just before it (a jump in steps or a state write clears it)
-    auto start_of = [&](int64_t i) {
+    // i-1, i-2, ... just before it (a jump in steps or a state write clears
+    // it). The learned preconditioner's collection solves start warm (their
+    // captured pairs are warm-start data) while the history keeps filling.
+    auto start_of = [&](int64_t i, bool learned_solve) {
         if (!guess) return -1;
         if (c->hist_next != i) c->hist = 0;
-        const int order = std::min(guess, c->hist);
+        const bool collecting = want_learned && !learned_solve && c->lp.state == LearnedPc::kCollecting;
+        const int order = collecting ? 0 : std::min(guess, c->hist);
         c->hist = std::min(c->hist + 1, 2);
         c->hist_next = i + 1;
         return order;
@@ -2325,7 +2328,7 @@ void run_gfdtd(tf_ctx* c, int64_t first, int64_t n, const tf_cg_opts* cg, tf_sta
         s0.learned = g.learned;
         s0.slot = (int)(k & 1);
         s0.dir_terms = dir_terms(k);
-        if (fresh) g.extrap = start_of(i);
+        if (fresh) g.extrap = start_of(i, g.learned);
         cg_start<T>(c, s0, true, guard, g.extrap);
     };
     auto scatter = [&](const CgState* guard) {
