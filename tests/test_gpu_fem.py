@@ -47,6 +47,14 @@ def test_fem_parity(workdir):
     check_fem(b)
 
 
+def test_fem_sigma_row_order(workdir, monkeypatch):
+    """The CG's free rows in SELL-C-sigma order instead of the edges'
+    locality rank: same bars (the edge-ordered scatter and W kernels)."""
+    monkeypatch.setenv("TETFIT_ROW_ORDER", "sigma")
+    _, b = fem_case(workdir, n=5, seed=3, name="gfem_sig", n_steps=60, dt=1.5e-11)
+    check_fem(b)
+
+
 def test_fem_lossy_with_j_dipole(workdir):
     """sigma in half of the blocks (the SigMat terms of Psi and the losses)
     and a j dipole beside the h source (-I/dt in Psi)."""

@@ -419,3 +419,16 @@ def test_extrapolated_start_learned(contrast10):
     assert rel_l2(samples, r["samples"]) <= 1e-4
     for got, want in ((U, r["U"]), (F, r["F"]), (B, r["B"])):
         assert rel_l2(got, want) <= 1e-4
+
+
+@pytest.mark.parametrize("order", ["sigma", "locality"])
+def test_cg_row_orders_parity(workdir, order, monkeypatch):
+    """The CG's free rows in either order (the edges' locality rank, the
+    default when the problem carries one, or SELL-C-sigma) meet the fp64
+    parity bars, and the edge-ordered scatter writes back every edge."""
+    monkeypatch.setenv("TETFIT_ROW_ORDER", order)
+    b = Model(configs.kuhn_cavity(str(workdir), n=6, method="gfdtd", n_steps=40, n_probes=4, block=2,
+                                  name=f"ord_{order}")).build(1)
+    assert b.edge_rank() is not None
+    st, _ = check_run(b)
+    assert st["cg_not_converged"] == 0 and st["cg_max_relres"] <= b.cg_tol

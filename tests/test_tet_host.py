@@ -389,3 +389,29 @@ def test_oracle_gfdtd_losses_push_order(tmp_path):
     assert len(r1["losses"]) == 2 and len(r2["losses"]) == 4
     np.testing.assert_allclose(r1["losses"], [0.5 * q(U0), q(r1["U"])], rtol=1e-12)
     np.testing.assert_allclose(r2["losses"], [0.5 * q(U0), 0.5 * q(r1["U"]), q(r2["U"]), q(r2["U"])], rtol=1e-12)
+
+
+def test_edge_locality_rank_recovers_lattice(kuhn):
+    """tf_dga_ops.edge_rank on a jittered Kuhn mesh: a permutation that
+    lists the edges by (z, direction class, y, x) of their lattice base
+    point, i.e. the snapping recovers the lattice (vertices at the jittered
+    positions, h = 1/n found from the vertex sample) and edges of one
+    direction along one x-line get consecutive ranks. The dga problem,
+    whose stepping does not use it, carries none."""
+    n = 4
+    assert kuhn["dga"].edge_rank() is None
+    b = kuhn["gfdtd"]
+    rank = b.edge_rank().copy()
+    assert np.array_equal(np.sort(rank), np.arange(b.n_edges))
+    t = b.tet_tables()
+    q = np.rint(t["pts"] * n).astype(np.int64)
+    assert np.abs(t["pts"] * n - q).max() < 0.25          # jitter 0.2h: the lattice is unique
+    a, c = q[t["edge_nodes"][:, 0]], q[t["edge_nodes"][:, 1]]
+    d = c - a
+    flip = (d[:, 2] < 0) | ((d[:, 2] == 0) & (d[:, 1] < 0)) | ((d[:, 2] == 0) & (d[:, 1] == 0) & (d[:, 0] < 0))
+    d[flip] *= -1
+    base = np.where(flip[:, None], c, a)
+    assert set(map(tuple, d)) == {(1, 0, 0), (0, 1, 0), (0, 0, 1), (1, 1, 0), (1, 0, 1), (0, 1, 1), (1, 1, 1)}
+    cls = (d[:, 0] + 1) + 3 * (d[:, 1] + 1) + 9 * (d[:, 2] + 1)
+    want = np.lexsort((base[:, 0], base[:, 1], cls, base[:, 2]))
+    assert np.array_equal(np.argsort(rank), want)

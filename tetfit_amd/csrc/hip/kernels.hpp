@@ -553,6 +553,22 @@ __global__ void gf_scatter_kernel(int32_t n_free, const int32_t* __restrict__ fr
     }
 }
 
+// The same in edge order: edge e takes x[edge_row[e]], or its new
+// Dirichlet value when edge_row[e] < 0. The free rows are ordered for the
+// gathers of the CG, not by edge, so one thread per edge gathers from x and
+// writes U (and U_older) in whole cache lines.
+template <typename T>
+__global__ void gf_scatter_edges_kernel(int32_t n_edges, const int32_t* __restrict__ edge_row,
+                                        const T* __restrict__ x, T* __restrict__ U, const T* __restrict__ dir_new,
+                                        const CgState* guard, T* __restrict__ U_older) {
+    if (!cg_guard_ok(guard)) return;
+    const int32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n_edges) return;
+    const int32_t r = edge_row[e];
+    if (U_older) U_older[e] = U[e];
+    U[e] = r >= 0 ? x[r] : dir_new[e];
+}
+
 // fem: the vectors the step's initial residual gathers (U = U_old here, Src
 // = the Dirichlet values at the new time): free edges W = W2 = U - U_older;
 // Dirichlet edges W = 2 U - U_older - Src, W2 = Src - U_older, so that with
@@ -561,20 +577,19 @@ __global__ void gf_scatter_kernel(int32_t n_free, const int32_t* __restrict__ fr
 //                  + 1/dt (Ctb (Fb - Fb_old) - I)_f
 // without the cancelling 2/dt^2 E U_old - A x0 of the reference's Psi
 // (D:1731-1733). guard as in cg_guard_ok.
+// One thread per edge; edge_row[e] < 0 marks a Dirichlet edge.
 template <typename T>
-__global__ void fem_w_kernel(int32_t n_free, const int32_t* __restrict__ free_edge, int32_t n_dir,
-                             const int32_t* __restrict__ dir_edge, const T* __restrict__ U,
+__global__ void fem_w_kernel(int32_t n_edges, const int32_t* __restrict__ edge_row, const T* __restrict__ U,
                              const T* __restrict__ U_older, const T* __restrict__ src, T* __restrict__ W,
                              T* __restrict__ W2, const CgState* guard) {
     if (!cg_guard_ok(guard)) return;
-    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n_free) {
-        const int32_t e = free_edge[i];
+    const int32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n_edges) return;
+    if (edge_row[e] >= 0) {
         const T v = (T)((double)U[e] - (double)U_older[e]);
         W[e] = v;
         W2[e] = v;
-    } else if (i - n_free < n_dir) {
-        const int32_t e = dir_edge[i - n_free];
+    } else {
         const double u = U[e], uo = U_older[e], sv = src[e];
         W[e] = (T)((u - uo) + (u - sv));
         W2[e] = (T)(sv - uo);

@@ -256,6 +256,7 @@ struct tf_ctx {
     bool c_fixed_signs = false;
     DevBuf c_col, c_sg, h_edge, curl_f, curl_u;
     DevBuf free_edge, dir_edge, dir_new, fb, pre, x, r, z, p0, p1, q, r0, cg_part, cg_state;
+    DevBuf edge_row;  // per edge: its free row in the CG order, or -1 (Dirichlet)
     DevBuf rs0, rs1;  // (r, s) pairs of the learned passes, even / odd pass
     // extrapolated starts (tf_cg_opts.guess): U_{n-1}, U_{n-2} on the free
     // rows, the start's shift d, gf_rhs_init's partials of ||b - A U_old||^2;
@@ -364,7 +365,7 @@ size_t tf_ctx::device_bytes() const {
     size_t s = 0;
     const DevBuf* all[] = {&U, &Fv, &B, &fit_ep, &fit_nu, &fe_sg, &fe_ca, &fe_cb, &fe_f4, &fh_sg, &fh_cm, &fh_cn, &fh_e4, &fe_code, &fh_code,
                            &sp_edge, &sp_f4, &sp_sg, &sp_ca, &sp_cb, &sp_kind, &sp_tb, &sp_te, &c_col, &c_sg,
-                           &h_edge, &curl_f, &curl_u, &free_edge, &dir_edge, &dir_new, &fb, &pre, &x, &r, &rs0, &rs1, &z,
+                           &h_edge, &curl_f, &curl_u, &free_edge, &edge_row, &dir_edge, &dir_new, &fb, &pre, &x, &r, &rs0, &rs1, &z,
                            &p0, &p1, &q, &r0, &hx1, &hx2, &dx, &rr0part, &cg_part, &cg_state, &inj_edge, &inj_kind, &inj_tb, &inj_te,
                            &term_src, &term_w, &carr, &probes, &samples, &eser, &epart, &poy, &row_dt_h, &row_dt_f,
                            &mu_a, &mu_b, &tf_a, &tf_b, &fslot, &uf[0], &uf[1], &ff[0], &ff[1], &U_older, &fw, &fw2};
@@ -529,6 +530,21 @@ std::vector<int32_t> sigma_order(const tf_csr& m) {
             return m.rowptr[a + 1] - m.rowptr[a] > m.rowptr[b + 1] - m.rowptr[b];
         });
     }
+    return perm;
+}
+
+// Row order of the CG's free rows: by the edges' locality rank when the
+// problem carries one (tf_dga_ops.edge_rank: same-class edges along a grid
+// line consecutive, so a slice's gathers touch few cache lines), else
+// SELL-C-sigma. TETFIT_ROW_ORDER=sigma keeps SELL-C-sigma. perm[new] = old.
+std::vector<int32_t> free_row_order(const tf_dga_ops& d) {
+    const char* env = std::getenv("TETFIT_ROW_ORDER");
+    if (!d.edge_rank || (env && std::strcmp(env, "sigma") == 0)) return sigma_order(d.A);
+    std::vector<int32_t> perm(d.n_free);
+    for (int32_t i = 0; i < d.n_free; ++i) perm[i] = i;
+    std::stable_sort(perm.begin(), perm.end(), [&](int32_t a, int32_t b) {
+        return d.edge_rank[d.free_edges[a]] < d.edge_rank[d.free_edges[b]];
+    });
     return perm;
 }
 
@@ -989,12 +1005,12 @@ void create_tet(tf_ctx* c, const tf_problem* p) {
         }
         upload(c->h_edge, he);
     } else {
-        // The CG runs over the free rows in SELL-C-sigma order of A: row k of
+        // The CG runs over the free rows in the order of free_row_order: row k of
         // every free-row operator (A, and K, RHSmat1 and Ct for the initial
         // residual) is free edge free_edges[perm[k]], and A's columns are
         // relabeled to match, so all CG vectors use that order.
         if (d.A.nrows != d.n_free) throw std::runtime_error("A must have one row per free edge");
-        const std::vector<int32_t> perm = sigma_order(d.A);
+        const std::vector<int32_t> perm = free_row_order(d);
         std::vector<int32_t> inv(d.n_free);
         for (int32_t k = 0; k < d.n_free; ++k) inv[perm[k]] = k;
         CsrBuf a;
@@ -1048,6 +1064,9 @@ void create_tet(tf_ctx* c, const tf_problem* p) {
         }
         c->n_free = d.n_free;
         upload(c->free_edge, fe);
+        std::vector<int32_t> er(E, -1);
+        for (int32_t k = 0; k < d.n_free; ++k) er[fe[k]] = k;
+        upload(c->edge_row, er);
         std::vector<uint8_t> is_free(E, 0);
         for (int32_t e : fe) is_free[e] = 1;
         c->h_dir_edge.clear();
@@ -2320,9 +2339,9 @@ void run_gfdtd(tf_ctx* c, int64_t first, int64_t n, const tf_cg_opts* cg, tf_sta
                 c->fem ? 1 : 0);
         }
         if (c->fem)
-            launch_timed(c, "fem_w_kernel", fem_w_kernel<T>, blocks_for(nf + c->n_dir), nf, c->free_edge.as<int32_t>(),
-                         c->n_dir, c->dir_edge.as<int32_t>(), (const T*)c->U.as<T>(), (const T*)c->U_older.as<T>(),
-                         (const T*)c->dir_new.as<T>(), c->fw.as<T>(), c->fw2.as<T>(), guard);
+            launch_timed(c, "fem_w_kernel", fem_w_kernel<T>, blocks_for(c->E), c->E, c->edge_row.as<int32_t>(),
+                         (const T*)c->U.as<T>(), (const T*)c->U_older.as<T>(), (const T*)c->dir_new.as<T>(),
+                         c->fw.as<T>(), c->fw2.as<T>(), guard);
         CgSolve s0;
         s0.init = init;
         s0.learned = g.learned;
@@ -2332,9 +2351,9 @@ void run_gfdtd(tf_ctx* c, int64_t first, int64_t n, const tf_cg_opts* cg, tf_sta
         cg_start<T>(c, s0, true, guard, g.extrap);
     };
     auto scatter = [&](const CgState* guard) {
-        launch_timed(c, "gf_scatter_kernel", gf_scatter_kernel<T>, blocks_for(nf + c->n_dir), nf,
-                     c->free_edge.as<int32_t>(), c->x.as<T>(), c->U.as<T>(), c->n_dir, c->dir_edge.as<int32_t>(),
-                     c->dir_new.as<T>(), guard, c->fem ? c->U_older.as<T>() : nullptr);
+        launch_timed(c, "gf_scatter_kernel", gf_scatter_edges_kernel<T>, blocks_for(c->E), c->E,
+                     c->edge_row.as<int32_t>(), (const T*)c->x.as<T>(), c->U.as<T>(), (const T*)c->dir_new.as<T>(),
+                     guard, c->fem ? c->U_older.as<T>() : nullptr);
     };
     // collection captures of the solve of the current step
     auto collection_snaps = [&]() {

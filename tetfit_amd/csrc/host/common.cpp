@@ -418,6 +418,7 @@ void Build::finalize_views() {
         d.free_edges = free_edges.empty() ? nullptr : free_edges.data();
         d.n_free = (int32_t)free_edges.size();
         d.Sig = Sig.view();
+        d.edge_rank = edge_rank.empty() ? nullptr : edge_rank.data();
         if (!frac_slot.empty()) {
             tf_dga_frac& fr = d.frac;
             fr.enabled = 1;

@@ -215,6 +215,99 @@ inline double& csr_at(CsrStore& m, int32_t r, int32_t c) {
     return m.val[it - m.col.begin()];
 }
 
+// Locality rank of the edges (tf_dga_ops.edge_rank). The vertices are snapped
+// to a lattice of spacing h per axis (h from the median largest component of
+// the edge vectors, adjusted so the bounding box holds a whole number of
+// steps; on a jittered Kuhn mesh this recovers the lattice exactly). Each edge
+// gets a direction class (its lattice step, sign-normalised) and a base
+// lattice point (the endpoint the step leaves from), and the edges are sorted
+// by (z, band of J y-lines, class, y, x): edges of one class along one x-line
+// are consecutive, so a 64-row slice of same-class rows reaches each
+// neighbour class through consecutive positions. J bounds a (z, band) block
+// to about 8,192 edges, so a row's neighbours in the planes z-1, z, z+1 stay
+// in three narrow position ranges. Returns an empty vector when the keys
+// would not fit 63 bits.
+std::vector<int32_t> edge_locality_rank(const std::vector<Vec3>& pts, const std::vector<uint32_t>& en, size_t E) {
+    std::vector<int32_t> rank;
+    if (E == 0 || pts.empty()) return rank;
+    double lo[3] = {HUGE_VAL, HUGE_VAL, HUGE_VAL}, hi[3] = {-HUGE_VAL, -HUGE_VAL, -HUGE_VAL};
+    for (const Vec3& p : pts) {
+        const double c[3] = {p.x, p.y, p.z};
+        for (int a = 0; a < 3; ++a) { lo[a] = std::min(lo[a], c[a]); hi[a] = std::max(hi[a], c[a]); }
+    }
+    std::vector<double> big;
+    const size_t stride = std::max<size_t>(1, E / 200000);
+    for (size_t e = 0; e < E; e += stride) {
+        const Vec3 d = pts[en[2 * e + 1]] - pts[en[2 * e]];
+        big.push_back(std::max(std::fabs(d.x), std::max(std::fabs(d.y), std::fabs(d.z))));
+    }
+    std::nth_element(big.begin(), big.begin() + big.size() / 2, big.end());
+    const double hmed = big[big.size() / 2];
+    if (!(hmed > 0)) return rank;
+    // steps per axis: of the counts within [0.75, 1.35] x extent/hmed (the
+    // median is biased high by the diagonal edges), the one whose lattice the
+    // vertices sit closest to (sum of squared distances over a sample)
+    double h[3];
+    int64_t cnt[3];
+    const size_t vstride = std::max<size_t>(1, pts.size() / 20000);
+    for (int a = 0; a < 3; ++a) {
+        const double ext = hi[a] - lo[a];
+        const int64_t c0 = std::max<int64_t>(1, std::llround(ext / hmed));
+        int64_t best = c0;
+        double best_r = HUGE_VAL;
+        for (int64_t c = std::max<int64_t>(1, (int64_t)(0.75 * c0)); ext > 0 && c <= (int64_t)(1.35 * c0) + 1; ++c) {
+            const double s = (double)c / ext;
+            double r = 0;
+            for (size_t i = 0; i < pts.size(); i += vstride) {
+                const double u = ((a == 0 ? pts[i].x : a == 1 ? pts[i].y : pts[i].z) - lo[a]) * s;
+                r += (u - std::nearbyint(u)) * (u - std::nearbyint(u));
+            }
+            if (r < best_r) { best_r = r; best = c; }
+        }
+        cnt[a] = best;
+        h[a] = ext > 0 ? ext / (double)cnt[a] : 1.0;
+        cnt[a] += 1;  // lattice points per axis
+    }
+    auto snap = [&](const Vec3& p, int64_t* q) {
+        const double c[3] = {p.x, p.y, p.z};
+        for (int a = 0; a < 3; ++a) q[a] = std::min<int64_t>(cnt[a] - 1, std::max<int64_t>(0, std::llround((c[a] - lo[a]) / h[a])));
+    };
+    std::vector<uint8_t> cls(E);
+    std::vector<int64_t> base(3 * E);
+    bool seen[125] = {};
+    for (size_t e = 0; e < E; ++e) {
+        int64_t qa[3], qb[3], d[3];
+        snap(pts[en[2 * e]], qa);
+        snap(pts[en[2 * e + 1]], qb);
+        for (int a = 0; a < 3; ++a) d[a] = qb[a] - qa[a];
+        // sign-normalise: the last non-zero component (z, then y, then x) positive
+        int s = 1;
+        for (int a = 2; a >= 0; --a)
+            if (d[a] != 0) { s = d[a] > 0 ? 1 : -1; break; }
+        const int64_t* q = s > 0 ? qa : qb;
+        int c = 0, m = 1;
+        for (int a = 0; a < 3; ++a, m *= 5) c += (int)(std::min<int64_t>(2, std::max<int64_t>(-2, s * d[a])) + 2) * m;
+        cls[e] = (uint8_t)c;
+        seen[c] = true;
+        for (int a = 0; a < 3; ++a) base[3 * e + a] = q[a];
+    }
+    int64_t ncls = 0;
+    for (bool v : seen) ncls += v;
+    const int64_t J = std::max<int64_t>(1, 8192 / std::max<int64_t>(1, ncls * cnt[0]));
+    const int64_t nband = (cnt[1] + J - 1) / J;
+    const long double span = (long double)cnt[2] * nband * 125 * cnt[1] * cnt[0];
+    if (span >= 4.6e18L) return rank;
+    std::vector<std::pair<int64_t, int32_t>> key(E);
+    for (size_t e = 0; e < E; ++e) {
+        const int64_t x = base[3 * e], y = base[3 * e + 1], z = base[3 * e + 2];
+        key[e] = {(((z * nband + y / J) * 125 + cls[e]) * cnt[1] + y) * cnt[0] + x, (int32_t)e};
+    }
+    std::sort(key.begin(), key.end());
+    rank.resize(E);
+    for (size_t r = 0; r < E; ++r) rank[key[r].second] = (int32_t)r;
+    return rank;
+}
+
 }  // namespace
 
 void build_tet(const Model& m, const SimulationDef& sim, Build& b) {
@@ -1616,6 +1709,7 @@ void build_tet(const Model& m, const SimulationDef& sim, Build& b) {
         b.src_edge_desc.push_back(d);
     }
 
+    if (!is_dga) b.edge_rank = edge_locality_rank(pts, en, E);
     b.face_nodes.assign(fn.begin(), fn.end());
     b.edge_nodes.assign(en.begin(), en.end());
     b.pts.resize(np * 3);

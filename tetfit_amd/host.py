@@ -221,6 +221,13 @@ class Build:
                     h_row=_arr(d.h_row, E, np.int32) if d.h_row else None,
                     free_edges=_arr(d.free_edges, d.n_free, np.int32), Sig=_csr(d.Sig), frac=self.frac_ops())
 
+    def edge_rank(self):
+        """tf_dga_ops.edge_rank (the locality rank of every edge), or None."""
+        d = self.problem_ptr.contents.dga
+        if self.method == "fdtd" or not d.edge_rank:
+            return None
+        return _arr(d.edge_rank, self.n_edges, np.int32)
+
     def frac_ops(self):
         """Fractional operators of a lossy dga problem (tf_dga_frac), or None."""
         fr = self.problem_ptr.contents.dga.frac
