@@ -138,12 +138,14 @@ typedef struct {
     /* dga with lossy media: the fractional operators (H is then empty and N
      * is the lossless all-face N of the energy diagnostic) */
     tf_dga_frac    frac;
-    /* optional (NULL: none): a locality rank of every edge, used by a stepping
-     * backend only to order its rows (the free rows of A): edges of one
-     * direction class on one grid line get consecutive ranks, so the rows of
-     * a 64-row slice gather neighbouring entries from a few cache lines. Any
-     * permutation is valid; results do not depend on it beyond rounding. */
+    /* optional (NULL: none; given together): locality ranks of every edge
+     * and face, used by a stepping backend only to lay out its rows and
+     * fields: entities of one shape class on one grid line get consecutive
+     * ranks, so the rows of a 64-row slice gather neighbouring entries from a
+     * few cache lines. Any permutations are valid; results do not depend on
+     * them beyond the rounding of reductions. */
     const int32_t* edge_rank;
+    const int32_t* face_rank;
 } tf_dga_ops;
 
 typedef struct {

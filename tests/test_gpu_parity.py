@@ -432,3 +432,31 @@ def test_cg_row_orders_parity(workdir, order, monkeypatch):
     assert b.edge_rank() is not None
     st, _ = check_run(b)
     assert st["cg_not_converged"] == 0 and st["cg_max_relres"] <= b.cg_tol
+
+
+@pytest.mark.parametrize("method", ["dga", "gfdtd", "fem"])
+def test_tet_locality_layout_bitwise(workdir, method, monkeypatch):
+    """Fields and operators laid out in the edges' and faces' locality order
+    (the default for tet problems that carry the ranks) give the fields of
+    the problem's own numbering bit for bit: rows keep their entry order,
+    only labels change. Probe samples are bitwise too; the Poynting sums
+    (partials over source edges in another order) agree to rounding."""
+    if method == "fem":
+        from test_fem import fem_case
+        _, b = fem_case(workdir, n=5, seed=3, name="lay_fem", n_steps=40, dt=1.5e-11)
+    else:
+        b = Model(configs.kuhn_cavity(str(workdir), n=6, method=method, n_steps=60, n_probes=4, block=2,
+                                      name=f"lay_{method}")).build(1)
+    out = {}
+    for order in ("numbering", "locality"):
+        monkeypatch.setenv("TETFIT_TET_ORDER", order)
+        print(f"case: {method} layout {order}", flush=True)
+        sim = Simulation(b, "fp64")
+        sim.run()
+        out[order] = (sim.probes()[1], sim.state(), sim.poynting())
+        sim.close()
+    (s0, st0, p0), (s1, st1, p1) = out["numbering"], out["locality"]
+    assert np.array_equal(s0, s1)
+    for a, e in zip(st0, st1):
+        assert np.array_equal(a, e)
+    np.testing.assert_allclose(p1, p0, rtol=1e-12, atol=1e-14 * max(np.abs(p0).max(), 1e-300))

@@ -396,10 +396,8 @@ def test_edge_locality_rank_recovers_lattice(kuhn):
     lists the edges by (z, direction class, y, x) of their lattice base
     point, i.e. the snapping recovers the lattice (vertices at the jittered
     positions, h = 1/n found from the vertex sample) and edges of one
-    direction along one x-line get consecutive ranks. The dga problem,
-    whose stepping does not use it, carries none."""
+    direction along one x-line get consecutive ranks."""
     n = 4
-    assert kuhn["dga"].edge_rank() is None
     b = kuhn["gfdtd"]
     rank = b.edge_rank().copy()
     assert np.array_equal(np.sort(rank), np.arange(b.n_edges))
@@ -415,3 +413,29 @@ def test_edge_locality_rank_recovers_lattice(kuhn):
     cls = (d[:, 0] + 1) + 3 * (d[:, 1] + 1) + 9 * (d[:, 2] + 1)
     want = np.lexsort((base[:, 0], base[:, 1], cls, base[:, 2]))
     assert np.array_equal(np.argsort(rank), want)
+
+
+def test_face_locality_rank_recovers_lattice(kuhn):
+    """tf_dga_ops.face_rank on the jittered Kuhn mesh: a permutation that
+    lists the faces by (z, shape class, y, x) of their base lattice point
+    (the first snapped vertex in (z, y, x) order); a Kuhn lattice point
+    carries 12 face shapes. Edges and faces are ranked for every tet
+    method."""
+    n = 4
+    for m in ("dga", "gfdtd"):
+        b = kuhn[m]
+        assert b.edge_rank() is not None and b.face_rank() is not None
+        rank = b.face_rank().copy()
+        assert np.array_equal(np.sort(rank), np.arange(b.n_faces))
+        t = b.tet_tables()
+        q = np.rint(t["pts"] * n).astype(np.int64)
+        v = q[t["face_nodes"]]                                   # [F, 3, 3]
+        key = v[:, :, 2] * 10000 + v[:, :, 1] * 100 + v[:, :, 0]
+        srt = np.take_along_axis(v, np.argsort(key, axis=1)[:, :, None], axis=1)
+        base = srt[:, 0]
+        o1, o2 = srt[:, 1] - base, srt[:, 2] - base
+        enc = lambda o: (o[:, 0] + 2) + 5 * (o[:, 1] + 2) + 25 * (o[:, 2] + 2)
+        code = enc(o1) * 125 + enc(o2)
+        assert len(np.unique(code)) == 12
+        want = np.lexsort((base[:, 0], base[:, 1], code, base[:, 2]))
+        assert np.array_equal(np.argsort(rank), want)

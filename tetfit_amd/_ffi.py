@@ -48,7 +48,7 @@ class tf_dga_ops(C.Structure):
                 ("H", tf_csr), ("h_row", i32p), ("N", tf_csr),
                 ("E", tf_csr), ("A", tf_csr), ("RHS1", tf_csr),
                 ("free_edges", i32p), ("n_free", C.c_int32), ("Sig", tf_csr), ("frac", tf_dga_frac),
-                ("edge_rank", i32p)]
+                ("edge_rank", i32p), ("face_rank", i32p)]
 
 
 class tf_problem(C.Structure):

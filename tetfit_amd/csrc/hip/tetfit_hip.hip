@@ -220,12 +220,13 @@ struct tf_ctx {
     // FIT grid order: fields, coefficients and codes stored at the grid
     // positions of edges and faces (fit_edge_slots / fit_face_slots)
     bool fit_grid = false;
+    bool tet_relabel = false;  // tet fields and operators in the locality ranks' order (h_esid, h_fsid)
     // launch the SELL kernels' wide-mesh instantiation (W): set when some
     // operator uses windowed 16-bit columns, or when the operators stepped
     // each step exceed the 256 MB Infinity Cache
     bool wide = false;
     FitGrid grid{};
-    std::vector<int32_t> h_esid, h_fsid;  // reference edge / face -> grid position
+    std::vector<int32_t> h_esid, h_fsid;  // reference edge / face -> device position (FIT grid order, tet relabel)
     std::vector<uint16_t> h_fe_code;      // per grid edge: slot map << 8 | signs (0: not in the main pass)
     DevBuf fe_code, fh_code;
     std::vector<double> h_ca_common, h_cb_common;
@@ -559,9 +560,15 @@ struct CsrBuf {
 };
 void select_rows(const tf_csr& m, const int32_t* rows, int32_t n, const int32_t* cmap, CsrBuf& o) {
     o.rp.assign(1, 0);
+    o.rp.reserve((size_t)n + 1);
     o.col.clear();
     o.val.clear();
     o.sg.clear();
+    int64_t total = 0;
+    for (int32_t k = 0; k < n; ++k) total += m.rowptr[rows[k] + 1] - m.rowptr[rows[k]];
+    o.col.reserve(total);
+    if (m.val) o.val.reserve(total);
+    if (m.sgn) o.sg.reserve(total);
     for (int32_t k = 0; k < n; ++k) {
         const int32_t e = rows[k];
         for (int64_t q = m.rowptr[e]; q < m.rowptr[e + 1]; ++q) {
@@ -579,6 +586,101 @@ void select_rows(const tf_csr& m, const int32_t* rows, int32_t n, const int32_t*
     o.v.col = o.col.data();
     o.v.val = m.val ? o.val.data() : nullptr;
     o.v.sgn = m.sgn ? o.sg.data() : nullptr;
+}
+
+// A tet problem renumbered by its locality ranks (tf_dga_ops.edge_rank,
+// face_rank): edge e of the problem is edge ep[e] here and face f is face
+// fp[f]. Rows keep their entries in the problem's order (only the labels
+// change), so every row sum runs in the same order and each field entry is
+// bitwise the one of the problem's numbering. H's rows follow their edges'
+// new labels (h_row relabeled to match); A keeps its rows, the free edges
+// their order in A (with new labels). Not for lossy dga (its fractional
+// unknowns keep the problem's order) or when TETFIT_TET_ORDER=numbering.
+struct TetRelabel {
+    CsrBuf C, Ct, H, N, E, RHS1, Sig;
+    std::vector<int32_t> ep, fp, bnd_edge, h_row, free_edges, iota_e, iota_f;
+    std::vector<int8_t> edge_class;
+    std::vector<double> fixed_value;
+    tf_problem p{};
+};
+
+bool relabel_tet(const tf_problem* src, TetRelabel& r) {
+    const tf_dga_ops& d = src->dga;
+    const int32_t E = src->n_edges, F = src->n_faces;
+    const char* env = std::getenv("TETFIT_TET_ORDER");
+    if (src->method == TF_METHOD_FDTD || !d.edge_rank || !d.face_rank || d.frac.enabled ||
+        (env && std::strcmp(env, "numbering") == 0))
+        return false;
+    auto inverse = [](const int32_t* rank, int32_t n, std::vector<int32_t>& fwd, std::vector<int32_t>& inv) {
+        fwd.assign(rank, rank + n);
+        inv.assign(n, -1);
+        for (int32_t i = 0; i < n; ++i) {
+            if (rank[i] < 0 || rank[i] >= n || inv[rank[i]] >= 0) throw std::runtime_error("locality rank is not a permutation");
+            inv[rank[i]] = i;
+        }
+    };
+    std::vector<int32_t> einv, finv;
+    inverse(d.edge_rank, E, r.ep, einv);
+    inverse(d.face_rank, F, r.fp, finv);
+    const int32_t* ep = r.ep.data();
+    const int32_t* fp = r.fp.data();
+    if (d.C.nrows != F || d.Ct.nrows != E || d.N.nrows != F) throw std::runtime_error("operator sizes");
+    select_rows(d.C, finv.data(), F, ep, r.C);
+    select_rows(d.Ct, einv.data(), E, fp, r.Ct);
+    select_rows(d.N, finv.data(), F, fp, r.N);
+    auto full = [&](const tf_csr& m, CsrBuf& o) -> tf_csr {
+        if (m.nrows != E) return m;  // absent (0 rows)
+        select_rows(m, einv.data(), E, ep, o);
+        return o.v;
+    };
+    tf_problem& p = r.p;
+    p = *src;
+    tf_dga_ops& o = p.dga;
+    o.C = r.C.v;
+    o.Ct = r.Ct.v;
+    o.N = r.N.v;
+    o.E = full(d.E, r.E);
+    o.RHS1 = full(d.RHS1, r.RHS1);
+    o.Sig = full(d.Sig, r.Sig);
+    r.bnd_edge.assign(E, -1);
+    r.edge_class.assign(E, 0);
+    r.fixed_value.assign(E, 0.0);
+    for (int32_t e = 0; e < E; ++e) {
+        if (d.bnd_edge) r.bnd_edge[ep[e]] = d.bnd_edge[e];
+        r.edge_class[ep[e]] = d.edge_class[e];
+        r.fixed_value[ep[e]] = d.fixed_value[e];
+    }
+    o.bnd_edge = d.bnd_edge ? r.bnd_edge.data() : nullptr;
+    o.edge_class = r.edge_class.data();
+    o.fixed_value = r.fixed_value.data();
+    if (d.h_row && d.H.nrows > 0) {
+        std::vector<int32_t> hrows;
+        hrows.reserve(d.H.nrows);
+        r.h_row.assign(E, -1);
+        for (int32_t e2 = 0; e2 < E; ++e2) {
+            const int32_t h = d.h_row[einv[e2]];
+            if (h < 0) continue;
+            if (h >= d.H.nrows) throw std::runtime_error("h_row out of range");
+            r.h_row[e2] = (int32_t)hrows.size();
+            hrows.push_back(h);
+        }
+        if ((int32_t)hrows.size() != d.H.nrows) throw std::runtime_error("H rows and class-1 edges differ");
+        select_rows(d.H, hrows.data(), d.H.nrows, ep, r.H);
+        o.H = r.H.v;
+        o.h_row = r.h_row.data();
+    }
+    if (d.n_free > 0) {
+        r.free_edges.resize(d.n_free);
+        for (int32_t k = 0; k < d.n_free; ++k) r.free_edges[k] = ep[d.free_edges[k]];
+        o.free_edges = r.free_edges.data();
+    }
+    r.iota_e.resize(E);
+    r.iota_f.resize(F);
+    for (int32_t e = 0; e < E; ++e) r.iota_e[e] = e;
+    for (int32_t f = 0; f < F; ++f) r.iota_f[f] = f;
+    o.edge_rank = r.iota_e.data();
+    o.face_rank = r.iota_f.data();
+    return true;
 }
 
 int sell_blocks(const Sell& m, int cap) {
@@ -611,8 +713,8 @@ void zero_field(DevBuf& b, int64_t n, hipStream_t s) {
 }
 
 // Device position of a reference edge / face (grid order or identity).
-int32_t dev_edge(const tf_ctx* c, int32_t e) { return c->fit_grid ? c->h_esid[e] : e; }
-int32_t dev_face(const tf_ctx* c, int32_t f) { return c->fit_grid ? c->h_fsid[f] : f; }
+int32_t dev_edge(const tf_ctx* c, int32_t e) { return c->fit_grid || c->tet_relabel ? c->h_esid[e] : e; }
+int32_t dev_face(const tf_ctx* c, int32_t f) { return c->fit_grid || c->tet_relabel ? c->h_fsid[f] : f; }
 
 uint8_t code4(const int s[4]) {
     uint8_t c = 0;
@@ -991,7 +1093,13 @@ void create_tet(tf_ctx* c, const tf_problem* p) {
         c->fcur = 0;
     } else if (p->method == TF_METHOD_DGA) {
         // H rows in SELL-C-sigma order; h_edge maps a SELL row to its edge
-        c->h_perm = sigma_order(d.H);
+        // (a relabeled problem's H rows are already in its edges' locality order)
+        if (c->tet_relabel) {
+            c->h_perm.resize(d.H.nrows);
+            for (int32_t r = 0; r < d.H.nrows; ++r) c->h_perm[r] = r;
+        } else {
+            c->h_perm = sigma_order(d.H);
+        }
         CsrBuf hs;
         select_rows(d.H, c->h_perm.data(), d.H.nrows, nullptr, hs);
         upload_sell<T>(c->H, hs.v, true);
@@ -2559,8 +2667,16 @@ std::unique_ptr<tf_ctx> make_ctx(const tf_problem* p, int32_t prec) {
         if (prec == TF_PREC_FP64) build_fit_special<double>(c.get(), none, fld);
         else build_fit_special<float>(c.get(), none, fld);
     } else {
-        if (prec == TF_PREC_FP64) create_tet<double>(c.get(), p);
-        else create_tet<float>(c.get(), p);
+        TetRelabel rl;
+        const tf_problem* q = p;
+        if (relabel_tet(p, rl)) {
+            q = &rl.p;
+            c->tet_relabel = true;
+            c->h_esid = rl.ep;
+            c->h_fsid = rl.fp;
+        }
+        if (prec == TF_PREC_FP64) create_tet<double>(c.get(), q);
+        else create_tet<float>(c.get(), q);
         std::vector<std::vector<std::pair<int32_t, double>>> none(c->E);
         std::vector<int> fld(c->E, TF_FIELD_H);
         build_tet_inject(c.get(), none, fld);
@@ -2921,8 +3037,10 @@ tf_status tf_set_sources(tf_ctx* c, const tf_source* src, int32_t n_src, const t
             const tf_source_term& t = terms[i];
             if (t.edge < 0 || t.edge >= c->E || t.source < 0 || t.source >= n_src)
                 throw std::runtime_error("source term out of range");
-            by_edge[t.edge].push_back({t.source, t.weight});
-            if (fld[t.edge] < 0) fld[t.edge] = src[t.source].field;
+            // FIT maps to grid positions in build_fit_special; tets here
+            const int32_t e = c->method == TF_METHOD_FDTD ? t.edge : dev_edge(c, t.edge);
+            by_edge[e].push_back({t.source, t.weight});
+            if (fld[e] < 0) fld[e] = src[t.source].field;
         }
         for (auto& f : fld)
             if (f < 0) f = TF_FIELD_E;
@@ -2949,12 +3067,12 @@ tf_status tf_set_probes(tf_ctx* c, const tf_probe* probes, int32_t n) {
             d.nh = p.nh;
             for (int k = 0; k < p.ne; ++k) {
                 if (p.e_idx[k] < 0 || p.e_idx[k] >= c->E) throw std::runtime_error("probe edge out of range");
-                d.e_idx[k] = c->method == TF_METHOD_FDTD ? dev_edge(c, p.e_idx[k]) : p.e_idx[k];
+                d.e_idx[k] = dev_edge(c, p.e_idx[k]);
                 for (int q = 0; q < 3; ++q) d.e_w[k][q] = p.e_w[k][q];
             }
             for (int k = 0; k < p.nh; ++k) {
                 if (p.h_idx[k] < 0 || p.h_idx[k] >= c->F) throw std::runtime_error("probe face out of range");
-                d.h_idx[k] = c->method == TF_METHOD_FDTD ? dev_face(c, p.h_idx[k]) : p.h_idx[k];
+                d.h_idx[k] = dev_face(c, p.h_idx[k]);
                 for (int q = 0; q < 3; ++q) d.h_w[k][q] = p.h_w[k][q];
             }
         }
@@ -3025,27 +3143,24 @@ tf_status tf_read_state(tf_ctx* c, double* U, double* F, double* B) {
                 for (int64_t i = 0; i < n; ++i) dst[i] = tmp[i];
             }
         };
-        // grid order (FIT): device position d of reference entity i
+        // FIT grid order or tet locality order: entity i sits at device position pos[i]
         auto rd_perm = [&](const DevBuf& b, double* dst, int64_t n, const std::vector<int32_t>& pos) {
             if (!dst) return;
             std::vector<double> tmp(n);
             rd(b, tmp.data(), n);
             for (int64_t i = 0; i < n; ++i) dst[i] = tmp[pos[i]];
         };
-        if (c->fit_grid) {
+        if (c->fit_grid || c->tet_relabel) {
             rd_perm(c->U, U, c->E, c->h_esid);
-            rd_perm(c->Fv, F, c->F, c->h_fsid);
+            if (!c->fem) rd_perm(c->Fv, F, c->F, c->h_fsid);
             rd_perm(c->B, B, c->F, c->h_fsid);
         } else {
             rd(c->U, U, c->E);
-            // fem has no F field (its buffer holds N C U_old of the last step)
-            if (c->fem) {
-                if (F) std::fill(F, F + c->F, 0.0);
-            } else {
-                rd(c->Fv, F, c->F);
-            }
+            if (!c->fem) rd(c->Fv, F, c->F);
             rd(c->B, B, c->F);
         }
+        // fem has no F field (its buffer holds N C U_old of the last step)
+        if (c->fem && F) std::fill(F, F + c->F, 0.0);
     });
 }
 
@@ -3064,7 +3179,7 @@ tf_status tf_write_state(tf_ctx* c, const double* U, const double* F, const doub
                 TF_HIP(hipMemcpy(b.p, tmp.data(), n * 4, hipMemcpyHostToDevice));
             }
         };
-        if (c->fit_grid) {
+        if (c->fit_grid || c->tet_relabel) {
             auto wr_perm = [&](DevBuf& b, const double* src, int64_t n, const std::vector<int32_t>& pos) {
                 if (!src) return;
                 std::vector<double> tmp(n);

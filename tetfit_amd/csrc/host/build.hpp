@@ -96,7 +96,7 @@ struct Build {
     std::vector<int32_t> frac_slot;
     int32_t frac_sizes[7] = {0, 0, 0, 0, 0, 0, 0};
     std::vector<int32_t> bnd_edge, h_row, free_edges;
-    std::vector<int32_t> edge_rank;  // locality rank of every edge (tf_dga_ops.edge_rank)
+    std::vector<int32_t> edge_rank, face_rank;  // locality ranks (tf_dga_ops.edge_rank, face_rank)
     std::vector<int8_t> edge_class;
     std::vector<double> fixed_value;
     int32_t n_bnd = 0;

@@ -419,6 +419,7 @@ void Build::finalize_views() {
         d.n_free = (int32_t)free_edges.size();
         d.Sig = Sig.view();
         d.edge_rank = edge_rank.empty() ? nullptr : edge_rank.data();
+        d.face_rank = face_rank.empty() ? nullptr : face_rank.data();
         if (!frac_slot.empty()) {
             tf_dga_frac& fr = d.frac;
             fr.enabled = 1;

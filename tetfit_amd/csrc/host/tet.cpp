@@ -16,6 +16,7 @@
 #include <fstream>
 #include <numeric>
 #include <sstream>
+#include <tuple>
 #include <type_traits>
 
 #include "build.hpp"
@@ -215,25 +216,37 @@ inline double& csr_at(CsrStore& m, int32_t r, int32_t c) {
     return m.val[it - m.col.begin()];
 }
 
-// Locality rank of the edges (tf_dga_ops.edge_rank). The vertices are snapped
-// to a lattice of spacing h per axis (h from the median largest component of
-// the edge vectors, adjusted so the bounding box holds a whole number of
-// steps; on a jittered Kuhn mesh this recovers the lattice exactly). Each edge
-// gets a direction class (its lattice step, sign-normalised) and a base
-// lattice point (the endpoint the step leaves from), and the edges are sorted
-// by (z, band of J y-lines, class, y, x): edges of one class along one x-line
-// are consecutive, so a 64-row slice of same-class rows reaches each
-// neighbour class through consecutive positions. J bounds a (z, band) block
-// to about 8,192 edges, so a row's neighbours in the planes z-1, z, z+1 stay
-// in three narrow position ranges. Returns an empty vector when the keys
-// would not fit 63 bits.
-std::vector<int32_t> edge_locality_rank(const std::vector<Vec3>& pts, const std::vector<uint32_t>& en, size_t E) {
-    std::vector<int32_t> rank;
-    if (E == 0 || pts.empty()) return rank;
-    double lo[3] = {HUGE_VAL, HUGE_VAL, HUGE_VAL}, hi[3] = {-HUGE_VAL, -HUGE_VAL, -HUGE_VAL};
+// Locality ranks of the edges and faces (tf_dga_ops.edge_rank, face_rank).
+// The vertices are snapped to a lattice of spacing h per axis (h from the
+// median largest component of the edge vectors, adjusted so the bounding box
+// holds a whole number of steps that the vertices sit closest to; on a
+// jittered Kuhn mesh this recovers the lattice exactly). An edge or face
+// gets a base lattice point (the first of its snapped vertices in (z, y, x)
+// order) and a shape class (the other vertices' lattice offsets from it),
+// and the entities are sorted by (z, band of J y-lines, class, y, x):
+// entities of one class along one x-line are consecutive, so a 64-row slice
+// of same-class rows reaches each neighbour class through consecutive
+// positions. J bounds a (z, band) block to about 8,192 entities, so a row's
+// neighbours in the planes z-1, z, z+1 stay in three narrow position ranges.
+struct Lattice {
+    double lo[3] = {0, 0, 0}, h[3] = {1, 1, 1};
+    int64_t cnt[3] = {1, 1, 1};  // lattice points per axis
+    bool ok = false;
+    void snap(const Vec3& p, int64_t* q) const {
+        const double c[3] = {p.x, p.y, p.z};
+        for (int a = 0; a < 3; ++a)
+            q[a] = std::min<int64_t>(cnt[a] - 1, std::max<int64_t>(0, std::llround((c[a] - lo[a]) / h[a])));
+    }
+};
+
+Lattice mesh_lattice(const std::vector<Vec3>& pts, const std::vector<uint32_t>& en, size_t E) {
+    Lattice L;
+    if (E == 0 || pts.empty()) return L;
+    double hi[3] = {-HUGE_VAL, -HUGE_VAL, -HUGE_VAL};
+    for (int a = 0; a < 3; ++a) L.lo[a] = HUGE_VAL;
     for (const Vec3& p : pts) {
         const double c[3] = {p.x, p.y, p.z};
-        for (int a = 0; a < 3; ++a) { lo[a] = std::min(lo[a], c[a]); hi[a] = std::max(hi[a], c[a]); }
+        for (int a = 0; a < 3; ++a) { L.lo[a] = std::min(L.lo[a], c[a]); hi[a] = std::max(hi[a], c[a]); }
     }
     std::vector<double> big;
     const size_t stride = std::max<size_t>(1, E / 200000);
@@ -243,15 +256,13 @@ std::vector<int32_t> edge_locality_rank(const std::vector<Vec3>& pts, const std:
     }
     std::nth_element(big.begin(), big.begin() + big.size() / 2, big.end());
     const double hmed = big[big.size() / 2];
-    if (!(hmed > 0)) return rank;
+    if (!(hmed > 0)) return L;
     // steps per axis: of the counts within [0.75, 1.35] x extent/hmed (the
     // median is biased high by the diagonal edges), the one whose lattice the
     // vertices sit closest to (sum of squared distances over a sample)
-    double h[3];
-    int64_t cnt[3];
     const size_t vstride = std::max<size_t>(1, pts.size() / 20000);
     for (int a = 0; a < 3; ++a) {
-        const double ext = hi[a] - lo[a];
+        const double ext = hi[a] - L.lo[a];
         const int64_t c0 = std::max<int64_t>(1, std::llround(ext / hmed));
         int64_t best = c0;
         double best_r = HUGE_VAL;
@@ -259,52 +270,64 @@ std::vector<int32_t> edge_locality_rank(const std::vector<Vec3>& pts, const std:
             const double s = (double)c / ext;
             double r = 0;
             for (size_t i = 0; i < pts.size(); i += vstride) {
-                const double u = ((a == 0 ? pts[i].x : a == 1 ? pts[i].y : pts[i].z) - lo[a]) * s;
+                const double u = ((a == 0 ? pts[i].x : a == 1 ? pts[i].y : pts[i].z) - L.lo[a]) * s;
                 r += (u - std::nearbyint(u)) * (u - std::nearbyint(u));
             }
             if (r < best_r) { best_r = r; best = c; }
         }
-        cnt[a] = best;
-        h[a] = ext > 0 ? ext / (double)cnt[a] : 1.0;
-        cnt[a] += 1;  // lattice points per axis
+        L.h[a] = ext > 0 ? ext / (double)best : 1.0;
+        L.cnt[a] = best + 1;
     }
-    auto snap = [&](const Vec3& p, int64_t* q) {
-        const double c[3] = {p.x, p.y, p.z};
-        for (int a = 0; a < 3; ++a) q[a] = std::min<int64_t>(cnt[a] - 1, std::max<int64_t>(0, std::llround((c[a] - lo[a]) / h[a])));
-    };
-    std::vector<uint8_t> cls(E);
-    std::vector<int64_t> base(3 * E);
-    bool seen[125] = {};
-    for (size_t e = 0; e < E; ++e) {
-        int64_t qa[3], qb[3], d[3];
-        snap(pts[en[2 * e]], qa);
-        snap(pts[en[2 * e + 1]], qb);
-        for (int a = 0; a < 3; ++a) d[a] = qb[a] - qa[a];
-        // sign-normalise: the last non-zero component (z, then y, then x) positive
-        int s = 1;
-        for (int a = 2; a >= 0; --a)
-            if (d[a] != 0) { s = d[a] > 0 ? 1 : -1; break; }
-        const int64_t* q = s > 0 ? qa : qb;
+    L.ok = true;
+    return L;
+}
+
+// Rank of n entities of `per` (2: edges, 3: faces) vertices each; empty when
+// the lattice is unusable or the keys would not fit 63 bits.
+std::vector<int32_t> locality_rank(const Lattice& L, const std::vector<Vec3>& pts, const std::vector<uint32_t>& nodes,
+                                   int per, size_t n) {
+    std::vector<int32_t> rank;
+    if (!L.ok || n == 0) return rank;
+    auto enc = [](const int64_t* o) {
         int c = 0, m = 1;
-        for (int a = 0; a < 3; ++a, m *= 5) c += (int)(std::min<int64_t>(2, std::max<int64_t>(-2, s * d[a])) + 2) * m;
-        cls[e] = (uint8_t)c;
-        seen[c] = true;
-        for (int a = 0; a < 3; ++a) base[3 * e + a] = q[a];
+        for (int a = 0; a < 3; ++a, m *= 5) c += (int)(std::min<int64_t>(2, std::max<int64_t>(-2, o[a])) + 2) * m;
+        return c;
+    };
+    std::vector<int32_t> code(n);
+    std::vector<int64_t> base(3 * n);
+    for (size_t i = 0; i < n; ++i) {
+        std::array<std::array<int64_t, 3>, 3> q{};
+        for (int k = 0; k < per; ++k) L.snap(pts[nodes[per * i + k]], q[k].data());
+        // (z, y, x) order of the snapped vertices
+        std::sort(q.begin(), q.begin() + per, [](const std::array<int64_t, 3>& u, const std::array<int64_t, 3>& v) {
+            return std::make_tuple(u[2], u[1], u[0]) < std::make_tuple(v[2], v[1], v[0]);
+        });
+        int c = 0;
+        for (int k = 1; k < per; ++k) {
+            const int64_t o[3] = {q[k][0] - q[0][0], q[k][1] - q[0][1], q[k][2] - q[0][2]};
+            c = c * 125 + enc(o);
+        }
+        code[i] = c;
+        for (int a = 0; a < 3; ++a) base[3 * i + a] = q[0][a];
     }
-    int64_t ncls = 0;
-    for (bool v : seen) ncls += v;
-    const int64_t J = std::max<int64_t>(1, 8192 / std::max<int64_t>(1, ncls * cnt[0]));
-    const int64_t nband = (cnt[1] + J - 1) / J;
-    const long double span = (long double)cnt[2] * nband * 125 * cnt[1] * cnt[0];
+    // classes numbered densely in code order
+    std::vector<int32_t> codes(code);
+    std::sort(codes.begin(), codes.end());
+    codes.erase(std::unique(codes.begin(), codes.end()), codes.end());
+    const int64_t ncls = (int64_t)codes.size();
+    const int64_t J = std::max<int64_t>(1, 8192 / std::max<int64_t>(1, ncls * L.cnt[0]));
+    const int64_t nband = (L.cnt[1] + J - 1) / J;
+    const long double span = (long double)L.cnt[2] * nband * ncls * L.cnt[1] * L.cnt[0];
     if (span >= 4.6e18L) return rank;
-    std::vector<std::pair<int64_t, int32_t>> key(E);
-    for (size_t e = 0; e < E; ++e) {
-        const int64_t x = base[3 * e], y = base[3 * e + 1], z = base[3 * e + 2];
-        key[e] = {(((z * nband + y / J) * 125 + cls[e]) * cnt[1] + y) * cnt[0] + x, (int32_t)e};
+    std::vector<std::pair<int64_t, int32_t>> key(n);
+    for (size_t i = 0; i < n; ++i) {
+        const int64_t x = base[3 * i], y = base[3 * i + 1], z = base[3 * i + 2];
+        const int64_t cls = std::lower_bound(codes.begin(), codes.end(), code[i]) - codes.begin();
+        key[i] = {(((z * nband + y / J) * ncls + cls) * L.cnt[1] + y) * L.cnt[0] + x, (int32_t)i};
     }
     std::sort(key.begin(), key.end());
-    rank.resize(E);
-    for (size_t r = 0; r < E; ++r) rank[key[r].second] = (int32_t)r;
+    rank.resize(n);
+    for (size_t r = 0; r < n; ++r) rank[key[r].second] = (int32_t)r;
     return rank;
 }
 
@@ -1709,7 +1732,12 @@ void build_tet(const Model& m, const SimulationDef& sim, Build& b) {
         b.src_edge_desc.push_back(d);
     }
 
-    if (!is_dga) b.edge_rank = edge_locality_rank(pts, en, E);
+    {
+        const Lattice lat = mesh_lattice(pts, en, E);
+        b.edge_rank = locality_rank(lat, pts, en, 2, E);
+        b.face_rank = locality_rank(lat, pts, fn, 3, F);
+        if (b.edge_rank.empty() || b.face_rank.empty()) { b.edge_rank.clear(); b.face_rank.clear(); }
+    }
     b.face_nodes.assign(fn.begin(), fn.end());
     b.edge_nodes.assign(en.begin(), en.end());
     b.pts.resize(np * 3);

@@ -228,6 +228,13 @@ class Build:
             return None
         return _arr(d.edge_rank, self.n_edges, np.int32)
 
+    def face_rank(self):
+        """tf_dga_ops.face_rank (the locality rank of every face), or None."""
+        d = self.problem_ptr.contents.dga
+        if self.method == "fdtd" or not d.face_rank:
+            return None
+        return _arr(d.face_rank, self.n_faces, np.int32)
+
     def frac_ops(self):
         """Fractional operators of a lossy dga problem (tf_dga_frac), or None."""
         fr = self.problem_ptr.contents.dga.frac
