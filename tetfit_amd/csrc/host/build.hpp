@@ -138,7 +138,7 @@ double par_dot(const double* a, const double* b, size_t n);
 
 double power_lambda_max(size_t n,
                         const std::function<void(const std::vector<double>&, std::vector<double>&)>& apply,
-                        int iters);
+                        int iters, const std::vector<double>* start = nullptr);
 
 void build_fit(const Model& m, const SimulationDef& sim, Build& b);
 void build_tet(const Model& m, const SimulationDef& sim, Build& b);

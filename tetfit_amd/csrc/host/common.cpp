@@ -326,13 +326,18 @@ double par_dot(const double* a, const double* b, size_t n) {
 
 double power_lambda_max(size_t n,
                         const std::function<void(const std::vector<double>&, std::vector<double>&)>& apply,
-                        int iters) {
-    // Deterministic start vector with all modes present.
+                        int iters, const std::vector<double>* start) {
+    // Deterministic start vector with all modes present (or the caller's).
     std::vector<double> x(n), y(n);
-    uint64_t st = 0x9E3779B97F4A7C15ull;
-    for (size_t i = 0; i < n; ++i) {
-        st ^= st << 13; st ^= st >> 7; st ^= st << 17;
-        x[i] = (double)(st >> 11) * (1.0 / 9007199254740992.0) - 0.5;
+    if (start) {
+        if (start->size() != n) throw Error("power iteration: start vector size");
+        x = *start;
+    } else {
+        uint64_t st = 0x9E3779B97F4A7C15ull;
+        for (size_t i = 0; i < n; ++i) {
+            st ^= st << 13; st ^= st >> 7; st ^= st << 17;
+            x[i] = (double)(st >> 11) * (1.0 / 9007199254740992.0) - 0.5;
+        }
     }
     double lam = 0;
     for (int it = 0; it < iters; ++it) {

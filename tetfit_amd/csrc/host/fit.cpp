@@ -49,6 +49,7 @@ const std::vector<Src> kFaceReuse[6] = {{{0, 5}}, {{1, 4}}, {{2, 3}}, {}, {}, {}
 }  // namespace
 
 void build_fit(const Model& m, const SimulationDef& sim, Build& b) {
+    PhaseClock clk;
     auto mit = m.meshes.find(sim.mesh);
     if (mit == m.meshes.end()) throw Error("simulation refers to an undefined mesh");
     const MeshDef& msh = mit->second;
@@ -336,6 +337,7 @@ void build_fit(const Model& m, const SimulationDef& sim, Build& b) {
     if (nf != totF || ne != totE || np != totP) throw Error("cartesian mesher: entity count mismatch");
     b.n_cubes = (int32_t)nv;
 
+    clk.mark("mesher (cubes, numbering, averages)");
     // ---- Ct_vec as CSR
     b.ct_ptr.assign(totE + 1, 0);
     for (uint64_t e = 0; e < totE; ++e) b.ct_ptr[e + 1] = b.ct_ptr[e] + ct_n[e];
@@ -348,6 +350,7 @@ void build_fit(const Model& m, const SimulationDef& sim, Build& b) {
         }
     std::vector<int32_t>().swap(ct_face);
 
+    clk.mark("Ct");
     // ---- edge classes (D:5382-5471)
     std::vector<std::vector<uint32_t>> edge_src(totE);
     std::vector<uint32_t> edge_bcs(totE, 0);
@@ -405,28 +408,164 @@ void build_fit(const Model& m, const SimulationDef& sim, Build& b) {
     for (uint32_t f = 0; f < totF; ++f)
         if (tbc[f]) b.tbc_faces.push_back((int32_t)f);
 
-    // ---- time step: power iteration on the FIT curl-curl (D:7036-7119)
+    clk.mark("edge classes, bc");
+    // ---- time step: power iteration on the FIT curl-curl (D:7036-7119),
+    // Hv Ct Nv C with Nv = avg_ni / area per face and Hv = length / avg_eps
+    // per edge. It runs in grid order: each face's four edges and each
+    // edge's (up to four) faces are found from the grid position, with the
+    // signs C and Ct give them (read off c_edge / curl and ct_sgn), so the
+    // operator is the same entry by entry and only the storage order differs;
+    // the start vector is the same pseudo-random vector, placed at the grid
+    // positions. Power iteration approaches lambda_max from below: pad by 2%.
     {
-        std::vector<double> Nv(totF), Hv(totE);
-        for (uint32_t f = 0; f < totF; ++f) Nv[f] = avg_ni[f] / face_area[f];
-        for (uint32_t e = 0; e < totE; ++e) Hv[e] = edge_len[e] / avg_eps[e];
-        std::vector<double> Fw(totF);
-        auto apply = [&](const std::vector<double>& X, std::vector<double>& Y) {
-#pragma omp parallel for schedule(static)
-            for (int64_t f = 0; f < (int64_t)totF; ++f) {
-                const int32_t* c = &b.c_edge[(size_t)f * 4];
-                Fw[f] = Nv[f] * b.curl[f] * (X[c[0]] - X[c[1]] + X[c[2]] - X[c[3]]);
-            }
-#pragma omp parallel for schedule(static)
-            for (int64_t e = 0; e < (int64_t)totE; ++e) {
-                double s = 0;
-                for (int64_t q = b.ct_ptr[e]; q < b.ct_ptr[e + 1]; ++q) s += b.ct_sgn[q] * Fw[b.ct_face[q]];
-                Y[e] = Hv[e] * s;
+        const int64_t nEz = (int64_t)(Nx + 1) * (Ny + 1) * Nz, nFz = (int64_t)Nx * Ny * (Nz + 1);
+        (void)nEz; (void)nFz;
+        // grid position -> (axis, i, j, k)
+        auto dec_edge = [&](int64_t g, int64_t* q) -> int {
+            if (g < nEx) { q[0] = g % Nx; q[1] = g / Nx % (Ny + 1); q[2] = g / Nx / (Ny + 1); return 0; }
+            if (g < nEx + nEy) { g -= nEx; q[0] = g % (Nx + 1); q[1] = g / (Nx + 1) % Ny; q[2] = g / (Nx + 1) / Ny; return 1; }
+            g -= nEx + nEy; q[0] = g % (Nx + 1); q[1] = g / (Nx + 1) % (Ny + 1); q[2] = g / (Nx + 1) / (Ny + 1); return 2;
+        };
+        auto dec_face = [&](int64_t g, int64_t* q) -> int {
+            if (g < nFx) { q[0] = g % (Nx + 1); q[1] = g / (Nx + 1) % Ny; q[2] = g / (Nx + 1) / Ny; return 0; }
+            if (g < nFx + nFy) { g -= nFx; q[0] = g % Nx; q[1] = g / Nx % (Ny + 1); q[2] = g / Nx / (Ny + 1); return 1; }
+            g -= nFx + nFy; q[0] = g % Nx; q[1] = g / Nx % Ny; q[2] = g / Nx / Ny; return 2;
+        };
+        // the four edges of a face: x-face y(i,j,k) z(i,j+1,k) y(i,j,k+1) z(i,j,k); y-face z(i,j,k)
+        // x(i,j,k+1) z(i+1,j,k) x(i,j,k); z-face x(i,j,k) y(i+1,j,k) x(i,j+1,k) y(i,j,k)
+        auto face_edges = [&](int a, int64_t i, int64_t j, int64_t k, int32_t* e4) {
+            if (a == 0) { e4[0] = edge_pos(1, i, j, k); e4[1] = edge_pos(2, i, j + 1, k); e4[2] = edge_pos(1, i, j, k + 1); e4[3] = edge_pos(2, i, j, k); }
+            else if (a == 1) { e4[0] = edge_pos(2, i, j, k); e4[1] = edge_pos(0, i, j, k + 1); e4[2] = edge_pos(2, i + 1, j, k); e4[3] = edge_pos(0, i, j, k); }
+            else { e4[0] = edge_pos(0, i, j, k); e4[1] = edge_pos(1, i + 1, j, k); e4[2] = edge_pos(0, i, j + 1, k); e4[3] = edge_pos(1, i, j, k); }
+        };
+        // the faces of an edge (-1: outside the grid): x-edge y(i,j,k) y(i,j,k-1) z(i,j,k)
+        // z(i,j-1,k); y-edge z(i,j,k) z(i-1,j,k) x(i,j,k) x(i,j,k-1); z-edge x(i,j,k)
+        // x(i,j-1,k) y(i,j,k) y(i-1,j,k)
+        auto edge_faces = [&](int a, int64_t i, int64_t j, int64_t k, int32_t* f4) {
+            if (a == 0) {
+                f4[0] = k < Nz ? face_pos(1, i, j, k) : -1; f4[1] = k > 0 ? face_pos(1, i, j, k - 1) : -1;
+                f4[2] = j < Ny ? face_pos(2, i, j, k) : -1; f4[3] = j > 0 ? face_pos(2, i, j - 1, k) : -1;
+            } else if (a == 1) {
+                f4[0] = i < Nx ? face_pos(2, i, j, k) : -1; f4[1] = i > 0 ? face_pos(2, i - 1, j, k) : -1;
+                f4[2] = k < Nz ? face_pos(0, i, j, k) : -1; f4[3] = k > 0 ? face_pos(0, i, j, k - 1) : -1;
+            } else {
+                f4[0] = j < Ny ? face_pos(0, i, j, k) : -1; f4[1] = j > 0 ? face_pos(0, i, j - 1, k) : -1;
+                f4[2] = i < Nx ? face_pos(1, i, j, k) : -1; f4[3] = i > 0 ? face_pos(1, i - 1, j, k) : -1;
             }
         };
-        // The operator Hv*Ct*Nv*C is similar to a symmetric one; power
-        // iteration approaches lambda_max from below, so pad by 2%.
-        double lam = power_lambda_max(totE, apply, 60) * 1.02;
+        // per grid face / edge: two bits per slot, 1 = +1, 2 = -1, 0 = no entry
+        std::vector<uint8_t> fsg(totF, 0), esg(totE, 0);
+        std::vector<double> Nv(totF), Hv(totE);
+        bool ok = true;
+#pragma omp parallel for schedule(static) reduction(&& : ok)
+        for (int64_t f = 0; f < (int64_t)totF; ++f) {
+            const int32_t g = b.face_sid[f];
+            int64_t q[3];
+            const int a = dec_face(g, q);
+            int32_t e4[4];
+            face_edges(a, q[0], q[1], q[2], e4);
+            uint8_t code = 0;
+            for (int t = 0; t < 4; ++t) {
+                const int32_t ge = b.edge_sid[b.c_edge[(size_t)f * 4 + t]];
+                const int sgn = b.curl[f] * (t & 1 ? -1 : 1);
+                int slot = -1;
+                for (int u = 0; u < 4; ++u) if (e4[u] == ge) slot = u;
+                if (slot < 0) ok = false;
+                else code |= (uint8_t)((sgn > 0 ? 1 : 2) << (2 * slot));
+            }
+            fsg[g] = code;
+            Nv[g] = avg_ni[f] / face_area[f];
+        }
+#pragma omp parallel for schedule(static) reduction(&& : ok)
+        for (int64_t e = 0; e < (int64_t)totE; ++e) {
+            const int32_t g = b.edge_sid[e];
+            int64_t q[3];
+            const int a = dec_edge(g, q);
+            int32_t f4[4];
+            edge_faces(a, q[0], q[1], q[2], f4);
+            uint8_t code = 0;
+            for (int64_t t = b.ct_ptr[e]; t < b.ct_ptr[e + 1]; ++t) {
+                const int32_t gf = b.face_sid[b.ct_face[t]];
+                int slot = -1;
+                for (int u = 0; u < 4; ++u) if (f4[u] == gf) slot = u;
+                if (slot < 0 || (code >> (2 * slot) & 3)) ok = false;
+                else code |= (uint8_t)((b.ct_sgn[t] > 0 ? 1 : 2) << (2 * slot));
+            }
+            esg[g] = code;
+            Hv[g] = edge_len[e] / avg_eps[e];
+        }
+        if (!ok) throw Error("internal: FIT incidence does not match the grid");
+        // row bases of the grid arrays (entity (i, j, k) of a row at base + i)
+        const int64_t NX = Nx, NY = Ny, NZ = Nz;
+        auto EX = [&](int64_t j, int64_t k) { return (k * (NY + 1) + j) * NX; };
+        auto EY = [&](int64_t j, int64_t k) { return nEx + (k * NY + j) * (NX + 1); };
+        auto EZ = [&](int64_t j, int64_t k) { return nEx + nEy + (k * (NY + 1) + j) * (NX + 1); };
+        auto FX = [&](int64_t j, int64_t k) { return (k * NY + j) * (NX + 1); };
+        auto FY = [&](int64_t j, int64_t k) { return nFx + (k * (NY + 1) + j) * NX; };
+        auto FZ = [&](int64_t j, int64_t k) { return nFx + nFy + (k * NY + j) * NX; };
+        // one row of entities at g0 .. g0 + n: out[g] = w[g] * sum over the
+        // four slots u with a sign code of (+-) in[base[u] + i]
+        auto row = [](const uint8_t* code, const double* w, const double* in, double* out, int64_t g0, int64_t n,
+                      const int64_t* base) {
+            for (int64_t i = 0; i < n; ++i) {
+                const int c = code[g0 + i];
+                double s = 0;
+                for (int u = 0; u < 4; ++u) {
+                    const int cu = c >> (2 * u) & 3;
+                    if (cu) s += (cu == 1 ? 1.0 : -1.0) * in[base[u] + i];
+                }
+                out[g0 + i] = w[g0 + i] * s;
+            }
+        };
+        std::vector<double> Fw(totF);
+        auto apply = [&](const std::vector<double>& X, std::vector<double>& Y) {
+            const double* x = X.data();
+            double* fw = Fw.data();
+#pragma omp parallel for schedule(static)
+            for (int64_t r = 0; r < NZ * NY + (NZ * (NY + 1)) + (NZ + 1) * NY; ++r) {
+                int64_t base[4];
+                if (r < NZ * NY) {  // x faces, rows (j < Ny, k < Nz) of Nx + 1
+                    const int64_t j = r % NY, k = r / NY;
+                    base[0] = EY(j, k); base[1] = EZ(j + 1, k); base[2] = EY(j, k + 1); base[3] = EZ(j, k);
+                    row(fsg.data(), Nv.data(), x, fw, FX(j, k), NX + 1, base);
+                } else if (r < NZ * NY + NZ * (NY + 1)) {  // y faces, rows (j <= Ny, k < Nz) of Nx
+                    const int64_t q = r - NZ * NY, j = q % (NY + 1), k = q / (NY + 1);
+                    base[0] = EZ(j, k); base[1] = EX(j, k + 1); base[2] = EZ(j, k) + 1; base[3] = EX(j, k);
+                    row(fsg.data(), Nv.data(), x, fw, FY(j, k), NX, base);
+                } else {  // z faces, rows (j < Ny, k <= Nz) of Nx
+                    const int64_t q = r - NZ * NY - NZ * (NY + 1), j = q % NY, k = q / NY;
+                    base[0] = EX(j, k); base[1] = EY(j, k) + 1; base[2] = EX(j + 1, k); base[3] = EY(j, k);
+                    row(fsg.data(), Nv.data(), x, fw, FZ(j, k), NX, base);
+                }
+            }
+            double* y = Y.data();
+#pragma omp parallel for schedule(static)
+            for (int64_t r = 0; r < (NZ + 1) * (NY + 1) + (NZ + 1) * NY + NZ * (NY + 1); ++r) {
+                int64_t base[4];
+                if (r < (NZ + 1) * (NY + 1)) {  // x edges, rows (j <= Ny, k <= Nz) of Nx
+                    const int64_t j = r % (NY + 1), k = r / (NY + 1);
+                    base[0] = FY(j, k); base[1] = FY(j, k - 1); base[2] = FZ(j, k); base[3] = FZ(j - 1, k);
+                    row(esg.data(), Hv.data(), fw, y, EX(j, k), NX, base);
+                } else if (r < (NZ + 1) * (NY + 1) + (NZ + 1) * NY) {  // y edges, rows (j < Ny, k <= Nz) of Nx + 1
+                    const int64_t q = r - (NZ + 1) * (NY + 1), j = q % NY, k = q / NY;
+                    base[0] = FZ(j, k); base[1] = FZ(j, k) - 1; base[2] = FX(j, k); base[3] = FX(j, k - 1);
+                    row(esg.data(), Hv.data(), fw, y, EY(j, k), NX + 1, base);
+                } else {  // z edges, rows (j <= Ny, k < Nz) of Nx + 1
+                    const int64_t q = r - (NZ + 1) * (NY + 1) - (NZ + 1) * NY, j = q % (NY + 1), k = q / (NY + 1);
+                    base[0] = FX(j, k); base[1] = FX(j - 1, k); base[2] = FY(j, k); base[3] = FY(j, k) - 1;
+                    row(esg.data(), Hv.data(), fw, y, EZ(j, k), NX + 1, base);
+                }
+            }
+        };
+        std::vector<double> x0(totE);
+        {
+            uint64_t st = 0x9E3779B97F4A7C15ull;
+            for (uint64_t e = 0; e < totE; ++e) {
+                st ^= st << 13; st ^= st >> 7; st ^= st << 17;
+                x0[b.edge_sid[e]] = (double)(st >> 11) * (1.0 / 9007199254740992.0) - 0.5;
+            }
+        }
+        double lam = power_lambda_max(totE, apply, 60, &x0) * 1.02;
         b.stable_dt = lam > 0 ? 2.0 / std::sqrt(lam) : 0;
     }
     double t_step = sim.courant * b.stable_dt;
@@ -436,6 +575,7 @@ void build_fit(const Model& m, const SimulationDef& sim, Build& b) {
     }
     b.prob.dt = t_step;
 
+    clk.mark("dt estimate");
     // ---- coefficients (D:5487-5649). PML factors are 1 without PML.
     b.M_nu.resize(totF); b.M_mu.resize(totF); b.fpml.assign(totF, 1.0); b.Nu_vec.assign(totF, 0.0);
     for (uint32_t f = 0; f < totF; ++f) {
@@ -476,6 +616,7 @@ void build_fit(const Model& m, const SimulationDef& sim, Build& b) {
     }
     b.boundary_face.assign(boundary_face.begin(), boundary_face.end());
 
+    clk.mark("coefficients");
     // ---- geometry helpers
     auto P = [&](int32_t n) { return Vec3(b.pts[(size_t)n * 3], b.pts[(size_t)n * 3 + 1], b.pts[(size_t)n * 3 + 2]); };
     auto edge_bar = [&](uint32_t e) {
@@ -518,6 +659,7 @@ void build_fit(const Model& m, const SimulationDef& sim, Build& b) {
         for (uint32_t e = 0; e < totE; ++e) b.upml[e] = pml(edge_bar(e));
     }
 
+    clk.mark("geometry helpers");
     // ---- probes (FindFitProbe, D:3590-3643) and j dipoles (D:447-492)
     set_output(m, sim, b);
     uint32_t root = 0;
@@ -595,6 +737,7 @@ void build_fit(const Model& m, const SimulationDef& sim, Build& b) {
         }
     }
 
+    clk.mark("probes");
     // ---- sources: per-edge weights. h sources on uncommon edges use the
     // boundary dual edge vector (D:4148-4199); e/j sources the edge vector.
     std::map<uint32_t, int32_t> src_index;
@@ -665,6 +808,7 @@ void build_fit(const Model& m, const SimulationDef& sim, Build& b) {
     int64_t i = 0;
     while ((double)i * t_step <= sim.duration) ++i;
     b.n_steps = i;
+    clk.mark("sources");
 }
 
 }  // namespace tfh
