@@ -498,14 +498,83 @@ __global__ void __launch_bounds__(kBlock) frac_gather_kernel(int32_t n, const in
     out[i] = j >= 0 ? (T)((double)X[a[i]] + (double)X[j]) : X[a[i]];
 }
 
+// Source work taken by workgroups appended to dga_curl_kernel's grid (after
+// the face workgroups in logical order), one injection item per thread:
+//   mode 1 (dga): the second Poynting half of the step, 0.5 U_new v on the
+//     h-source edges;
+//   mode 2 (gfdtd, fem): the step's sources as gf_inject_kernel forms them
+//     (fb, dir_new and the Poynting term from U_old).
+// One fp64 partial per appended workgroup into poy_part; mode 0: none.
+template <typename T>
+struct CurlSources {
+    int32_t n_inj = 0;
+    int32_t mode = 0;
+    const int32_t* edge = nullptr;
+    const int8_t* kind = nullptr;
+    const int32_t* tb = nullptr;
+    const int32_t* te = nullptr;
+    const int32_t* term_src = nullptr;
+    const double* term_w = nullptr;
+    const double* carr_step = nullptr;
+    const double* carr_prev = nullptr;  // mode 2: the previous step's carriers (null at step 1)
+    T* fb = nullptr;
+    T* dir_new = nullptr;
+    int32_t delta = 0;                  // mode 2, fem: an h edge's forcing is Fb - Fb_old
+    double* poy_part = nullptr;
+};
+
+// gfdtd sources of one item (D:1478-1502): returns its Poynting term.
+template <typename T>
+__device__ __forceinline__ double gf_source_item(int32_t i, const int32_t* __restrict__ inj_edge,
+                                                 const int8_t* __restrict__ inj_kind, const int32_t* __restrict__ inj_tb,
+                                                 const int32_t* __restrict__ inj_te,
+                                                 const int32_t* __restrict__ term_src,
+                                                 const double* __restrict__ term_w, const double* __restrict__ carr_step,
+                                                 const double* __restrict__ carr_prev, T* __restrict__ fb,
+                                                 T* __restrict__ dir_new, const T* __restrict__ U, int delta) {
+    const int32_t e = inj_edge[i];
+    const double v = source_value(inj_tb[i], inj_te[i], term_src, term_w, carr_step);
+    if (inj_kind[i] == 0) {
+        const double vo = carr_prev ? source_value(inj_tb[i], inj_te[i], term_src, term_w, carr_prev) : 0.0;
+        fb[e] = (T)(delta ? v - vo : v);
+        return (double)U[e] * (0.5 * (v + vo));
+    }
+    if (inj_kind[i] == 2) {
+        fb[e] = (T)(-v);  // current I at t (D:1491-1492): -dt I joins dt Ctb Fb in Psi
+    } else {
+        dir_new[e] = (T)v;
+    }
+    return 0.0;
+}
+
 // curl_u = C U (3 signed edges per face), B -= dt*curl_u (D:902-910).
 template <typename T>
 __global__ void __launch_bounds__(kBlock) dga_curl_kernel(int32_t n_faces, const int32_t* __restrict__ c_col,
                                                           const int8_t* __restrict__ c_sg, const T* __restrict__ U,
                                                           T* __restrict__ curl_u, T* __restrict__ B, double dt,
-                                                          const double* __restrict__ face_dt, const CgState* guard) {
+                                                          const double* __restrict__ face_dt, const CgState* guard,
+                                                          CurlSources<T> py) {
     if (!cg_guard_ok(guard)) return;
-    const int32_t f = xcd_block() * kBlock + threadIdx.x;
+    const int lb = xcd_block(), face_blocks = (n_faces + kBlock - 1) / kBlock;
+    if (py.mode && lb >= face_blocks) {
+        __shared__ double red[kBlock / kWave];
+        const int32_t i = (lb - face_blocks) * kBlock + threadIdx.x;
+        double poy = 0;
+        if (i < py.n_inj) {
+            if (py.mode == 1) {
+                if (py.kind[i] == 0)
+                    poy = 0.5 * (double)U[py.edge[i]] *
+                          source_value(py.tb[i], py.te[i], py.term_src, py.term_w, py.carr_step);
+            } else {
+                poy = gf_source_item<T>(i, py.edge, py.kind, py.tb, py.te, py.term_src, py.term_w, py.carr_step,
+                                        py.carr_prev, py.fb, py.dir_new, U, py.delta);
+            }
+        }
+        const double s = block_sum(poy, red);
+        if (threadIdx.x == 0) py.poy_part[lb - face_blocks] = s;
+        return;
+    }
+    const int32_t f = lb * kBlock + threadIdx.x;
     if (f >= n_faces) return;
     const int32_t* c = c_col + 3 * (int64_t)f;
     double cu;
@@ -612,19 +681,9 @@ __global__ void __launch_bounds__(kBlock) gf_inject_kernel(
     __shared__ double red[kBlock / kWave];
     const int32_t i = blockIdx.x * kBlock + threadIdx.x;
     double poy = 0;
-    if (i < n_inj) {
-        const int32_t e = inj_edge[i];
-        const double v = source_value(inj_tb[i], inj_te[i], term_src, term_w, carr_step);
-        if (inj_kind[i] == 0) {
-            const double vo = carr_prev ? source_value(inj_tb[i], inj_te[i], term_src, term_w, carr_prev) : 0.0;
-            fb[e] = (T)(delta ? v - vo : v);
-            poy = (double)U[e] * (0.5 * (v + vo));
-        } else if (inj_kind[i] == 2) {
-            fb[e] = (T)(-v);  // current I at t (D:1491-1492): -dt I joins dt Ctb Fb in Psi
-        } else {
-            dir_new[e] = (T)v;
-        }
-    }
+    if (i < n_inj)
+        poy = gf_source_item<T>(i, inj_edge, inj_kind, inj_tb, inj_te, term_src, term_w, carr_step, carr_prev, fb,
+                                dir_new, U, delta);
     const double s = block_sum(poy, red);
     if (threadIdx.x == 0) poy_part[blockIdx.x] = s;
 }

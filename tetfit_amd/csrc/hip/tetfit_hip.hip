@@ -258,6 +258,7 @@ struct tf_ctx {
     DevBuf c_col, c_sg, h_edge, curl_f, curl_u;
     DevBuf free_edge, dir_edge, dir_new, fb, pre, x, r, z, p0, p1, q, r0, cg_part, cg_state;
     DevBuf edge_row;  // per edge: its free row in the CG order, or -1 (Dirichlet)
+    DevBuf ct_src_ptr;  // dga: per Ct slice, its injection items (CtSources.ptr)
     DevBuf rs0, rs1;  // (r, s) pairs of the learned passes, even / odd pass
     // extrapolated starts (tf_cg_opts.guess): U_{n-1}, U_{n-2} on the free
     // rows, the start's shift d, gf_rhs_init's partials of ||b - A U_old||^2;
@@ -366,7 +367,7 @@ size_t tf_ctx::device_bytes() const {
     size_t s = 0;
     const DevBuf* all[] = {&U, &Fv, &B, &fit_ep, &fit_nu, &fe_sg, &fe_ca, &fe_cb, &fe_f4, &fh_sg, &fh_cm, &fh_cn, &fh_e4, &fe_code, &fh_code,
                            &sp_edge, &sp_f4, &sp_sg, &sp_ca, &sp_cb, &sp_kind, &sp_tb, &sp_te, &c_col, &c_sg,
-                           &h_edge, &curl_f, &curl_u, &free_edge, &edge_row, &dir_edge, &dir_new, &fb, &pre, &x, &r, &rs0, &rs1, &z,
+                           &h_edge, &curl_f, &curl_u, &free_edge, &edge_row, &ct_src_ptr, &dir_edge, &dir_new, &fb, &pre, &x, &r, &rs0, &rs1, &z,
                            &p0, &p1, &q, &r0, &hx1, &hx2, &dx, &rr0part, &cg_part, &cg_state, &inj_edge, &inj_kind, &inj_tb, &inj_te,
                            &term_src, &term_w, &carr, &probes, &samples, &eser, &epart, &poy, &row_dt_h, &row_dt_f,
                            &mu_a, &mu_b, &tf_a, &tf_b, &fslot, &uf[0], &uf[1], &ff[0], &ff[1], &U_older, &fw, &fw2};
@@ -1328,6 +1329,13 @@ void build_tet_inject(tf_ctx* c, const std::vector<std::vector<std::pair<int32_t
     c->dir_zero_static = c->k_dir_only && std::find(kind.begin(), kind.end(), (int8_t)1) == kind.end();
     for (int32_t e : c->h_dir_edge)
         if (c->h_fixed_value[e] != 0) c->dir_zero_static = false;
+    if (c->method == TF_METHOD_DGA && !c->frac && c->sCt.nslices > 0) {
+        // items of each Ct slice (rows are edges; items are sorted by edge)
+        std::vector<int32_t> ptr(c->sCt.nslices + 1, 0);
+        for (int32_t e : edge) ptr[e / kSlice + 1]++;
+        for (int32_t q = 0; q < c->sCt.nslices; ++q) ptr[q + 1] += ptr[q];
+        upload(c->ct_src_ptr, ptr);
+    }
     upload(c->inj_edge, edge);
     upload(c->inj_kind, kind);
     upload(c->inj_tb, tb);
@@ -1607,7 +1615,8 @@ void run_dga_frac(tf_ctx* c, int64_t first, int64_t n, tf_stats* st) {
         const double* cs = c->carr.as<double>() + (k + 1) * ns;
         double* poy = c->poy.as<double>() + 2 * k * inj_blocks;
         TF_LAUNCH_TIMED(c, "sell_ct_kernel", c->tune[TK_CT].unroll, sell_ct_kernel, T, gb_ct,
-                        tuned_view(c, TK_CT, c->sCt.signs()), c->Fv.as<T>(), c->curl_f.as<T>());
+                        tuned_view(c, TK_CT, c->sCt.signs()), c->Fv.as<T>(), c->curl_f.as<T>(), CtSources{},
+                        (T*)nullptr);
         inject(cs, poy, 0);
         const int nx = c->fcur ^ 1;
         if (c->nU > 0)
@@ -1622,7 +1631,7 @@ void run_dga_frac(tf_ctx* c, int64_t first, int64_t n, tf_stats* st) {
                          (const T*)c->U.as<T>(), poy + inj_blocks);
         launch_timed(c, "dga_curl_kernel", dga_curl_kernel<T>, blocks_for(c->F), c->F, c->c_col.as<int32_t>(),
                      c->c_fixed_signs ? nullptr : c->c_sg.as<int8_t>(), (const T*)c->U.as<T>(), c->curl_u.as<T>(),
-                     c->B.as<T>(), c->dt, (const double*)nullptr, (const CgState*)nullptr);
+                     c->B.as<T>(), c->dt, (const double*)nullptr, (const CgState*)nullptr, CurlSources<T>{});
         if (c->nF > 0)
             TF_LAUNCH_TIMED(c, "sell_frac_kernel", 8, sell_frac_kernel, T, gb_f, c->fKf.vals<T>(), c->fRf.vals<T>(),
                             (const T*)c->curl_u.as<T>(), (const T*)c->ff[c->fcur].as<T>(), c->ff[nx].as<T>(), -c->dt);
@@ -1646,17 +1655,46 @@ void run_dga_frac(tf_ctx* c, int64_t first, int64_t n, tf_stats* st) {
     st->steps_done += n;
 }
 
+// The context's injection items as dga_curl_kernel's appended source work
+// (mode 1: dga second Poynting half; mode 2: gfdtd / fem sources).
+template <class T>
+CurlSources<T> curl_sources(tf_ctx* c, int mode, const double* carr_step, const double* carr_prev, double* poy) {
+    CurlSources<T> s;
+    s.n_inj = c->n_inj;
+    s.mode = mode;
+    s.edge = c->inj_edge.as<int32_t>();
+    s.kind = c->inj_kind.as<int8_t>();
+    s.tb = c->inj_tb.as<int32_t>();
+    s.te = c->inj_te.as<int32_t>();
+    s.term_src = c->term_src.as<int32_t>();
+    s.term_w = c->term_w.as<double>();
+    s.carr_step = carr_step;
+    s.carr_prev = carr_prev;
+    s.fb = c->fb.p ? c->fb.as<T>() : nullptr;
+    s.dir_new = c->dir_new.p ? c->dir_new.as<T>() : nullptr;
+    s.delta = c->fem ? 1 : 0;
+    s.poy_part = poy;
+    return s;
+}
+
 template <class T>
 void run_dga(tf_ctx* c, int64_t first, int64_t n, tf_stats* st) {
-    const int inj_blocks = c->n_inj ? blocks_for(c->n_inj) : 0;
-    const size_t ns = std::max<size_t>(c->src.size(), 1);
-    if (inj_blocks && c->poy_cap < 2 * n * inj_blocks) {
-        c->poy.alloc(2 * n * inj_blocks * sizeof(double));
-        c->poy_cap = 2 * n * inj_blocks;
-    }
     if (c->frac) {
         run_dga_frac<T>(c, first, n, st);
         return;
+    }
+    const int inj_blocks = c->n_inj ? blocks_for(c->n_inj) : 0;
+    const size_t ns = std::max<size_t>(c->src.size(), 1);
+    const double* hdt = c->batch ? c->row_dt_h.as<double>() : nullptr;
+    const double* fdt = c->batch ? c->row_dt_f.as<double>() : nullptr;
+    const int gb_ct = tuned_blocks(c, TK_CT, c->sCt, kSpmvCap), gb_h = tuned_blocks(c, TK_H, c->H, kSpmvCap);
+    const int gb_n = tuned_blocks(c, TK_N, c->N, kSpmvCap);
+    // Poynting partials of a step: sell_ct's workgroups (first half), then
+    // the injection workgroups appended to dga_curl (second half)
+    const int per_step = c->n_inj ? gb_ct + inj_blocks : 0;
+    if (per_step && c->poy_cap < (int64_t)per_step * n) {
+        c->poy.alloc((size_t)per_step * n * sizeof(double));
+        c->poy_cap = (int64_t)per_step * n;
     }
     if (!c->initial_done && first == 1) {
         // initial export at t = 0 (D:805-806), after the t = 0 e-source values
@@ -1670,56 +1708,49 @@ void run_dga(tf_ctx* c, int64_t first, int64_t n, tf_stats* st) {
         if (want_sample(c, 0, 0.0)) sample<T>(c, 0.0);
     }
     c->initial_done = true;
-    const double* hdt = c->batch ? c->row_dt_h.as<double>() : nullptr;
-    const double* fdt = c->batch ? c->row_dt_f.as<double>() : nullptr;
-    const int gb_ct = tuned_blocks(c, TK_CT, c->sCt, kSpmvCap), gb_h = tuned_blocks(c, TK_H, c->H, kSpmvCap);
-    const int gb_n = tuned_blocks(c, TK_N, c->N, kSpmvCap);
-    // one step (D:810-939): curl_f = Ct F, h sources, U += dt H curl_f,
-    // Poynting, curl_u = C U with B update, F -= dt N curl_u; poy holds the
-    // step's two Poynting partial blocks
+    // one step (D:810-939): curl_f = Ct F with the h sources (and the e
+    // sources into U, the first Poynting half), U += dt H curl_f, curl_u =
+    // C U with the B update (and the second Poynting half), F -= dt N
+    // curl_u; poy holds the step's Poynting partials, per_step of them
     auto step = [&](const double* cs, double* poy) {
-        TF_LAUNCH_TIMED(c, "sell_ct_kernel", c->tune[TK_CT].unroll, sell_ct_kernel, T, gb_ct,
-                        tuned_view(c, TK_CT, c->sCt.signs()), c->Fv.as<T>(), c->curl_f.as<T>());
+        CtSources src{};
+        CurlSources<T> py{};
         if (c->n_inj) {
-            launch_timed(c, "dga_inject_kernel", dga_inject_kernel<T>, inj_blocks, 
-                c->n_inj, c->inj_edge.as<int32_t>(), c->inj_kind.as<int8_t>(), c->inj_tb.as<int32_t>(),
-                c->inj_te.as<int32_t>(), c->term_src.as<int32_t>(), c->term_w.as<double>(), cs, c->curl_f.as<T>(),
-                c->U.as<T>(), poy, 0);
+            src = CtSources{c->ct_src_ptr.as<int32_t>(), c->inj_edge.as<int32_t>(), c->inj_kind.as<int8_t>(),
+                            c->inj_tb.as<int32_t>(), c->inj_te.as<int32_t>(), c->term_src.as<int32_t>(),
+                            c->term_w.as<double>(), cs, poy};
+            py = curl_sources<T>(c, 1, cs, nullptr, poy + gb_ct);
         }
+        TF_LAUNCH_TIMED(c, "sell_ct_kernel", c->tune[TK_CT].unroll, sell_ct_kernel, T, gb_ct,
+                        tuned_view(c, TK_CT, c->sCt.signs()), c->Fv.as<T>(), c->curl_f.as<T>(), src, c->U.as<T>());
         TF_LAUNCH_TIMED(c, "sell_h_kernel", c->tune[TK_H].unroll, sell_h_kernel, T, gb_h,
                         tuned_view(c, TK_H, c->H.vals<T>()), c->h_edge.as<int32_t>(), c->curl_f.as<T>(), c->U.as<T>(),
                         c->dt, hdt);
-        if (c->n_inj) {
-            launch_timed(c, "dga_poynting_kernel", dga_poynting_kernel<T>, inj_blocks, 
-                c->n_inj, c->inj_edge.as<int32_t>(), c->inj_kind.as<int8_t>(), c->inj_tb.as<int32_t>(),
-                c->inj_te.as<int32_t>(), c->term_src.as<int32_t>(), c->term_w.as<double>(), cs, c->U.as<T>(),
-                poy + inj_blocks);
-        }
-        launch_timed(c, "dga_curl_kernel", dga_curl_kernel<T>, blocks_for(c->F), c->F, c->c_col.as<int32_t>(),
-                     c->c_fixed_signs ? nullptr : c->c_sg.as<int8_t>(), c->U.as<T>(), c->curl_u.as<T>(),
-                     c->B.as<T>(), c->dt, fdt, (const CgState*)nullptr);
+        launch_timed(c, "dga_curl_kernel", dga_curl_kernel<T>, blocks_for(c->F) + (c->n_inj ? inj_blocks : 0), c->F,
+                     c->c_col.as<int32_t>(), c->c_fixed_signs ? nullptr : c->c_sg.as<int8_t>(), c->U.as<T>(),
+                     c->curl_u.as<T>(), c->B.as<T>(), c->dt, fdt, (const CgState*)nullptr, py);
         TF_LAUNCH_TIMED(c, "sell_n_kernel", c->tune[TK_N].unroll, sell_n_kernel, T, gb_n,
                         tuned_view(c, TK_N, c->N.vals<T>()), c->curl_u.as<T>(), c->Fv.as<T>(), c->dt, 0, fdt, nullptr);
     };
-    const int64_t g = graph_steps(c) ? run_graph_chunks(c, n, 2 * inj_blocks, [&](int, const double* cs, double* poy) {
+    const int64_t g = graph_steps(c) ? run_graph_chunks(c, n, per_step, [&](int, const double* cs, double* poy) {
         step(cs, poy);
     }) : 0;
     for (int64_t k = g; k < n; ++k) {
         const int64_t i = first + k;
         const double t = (double)i * c->dt;
-        step(c->carr.as<double>() + (k + 1) * ns, c->poy.as<double>() + 2 * k * inj_blocks);
+        step(c->carr.as<double>() + (k + 1) * ns, c->poy.as<double>() + (size_t)per_step * k);
         if (want_sample(c, i, t)) sample<T>(c, t);
         if (c->profiling && (k & 63) == 63) ev_flush(c);
     }
     TF_HIP(hipGetLastError());
-    std::vector<double> part((size_t)2 * n * std::max(inj_blocks, 1), 0.0);
-    if (inj_blocks)
-        TF_HIP(hipMemcpyAsync(part.data(), c->poy.p, 2 * n * inj_blocks * sizeof(double), hipMemcpyDeviceToHost,
+    std::vector<double> part((size_t)std::max(per_step, 1) * n, 0.0);
+    if (per_step)
+        TF_HIP(hipMemcpyAsync(part.data(), c->poy.p, (size_t)per_step * n * sizeof(double), hipMemcpyDeviceToHost,
                               c->stream));
     TF_HIP(hipStreamSynchronize(c->stream));
     for (int64_t k = 0; k < n; ++k) {
         double s = 0;
-        for (int b = 0; b < 2 * inj_blocks; ++b) s += part[2 * k * inj_blocks + b];
+        for (int b = 0; b < per_step; ++b) s += part[(size_t)per_step * k + b];
         c->poynting.push_back(s);
     }
     st->steps_done += n;
@@ -2426,9 +2457,15 @@ void run_gfdtd(tf_ctx* c, int64_t first, int64_t n, const tf_cg_opts* cg, tf_sta
         const double t = (double)i * c->dt;
         const double* cs = c->carr.as<double>() + (k + 1) * ns;
         const double* cp = c->carr.as<double>() + k * ns;
-        launch_timed(c, "dga_curl_kernel", dga_curl_kernel<T>, blocks_for(c->F), c->F, c->c_col.as<int32_t>(),
-                     c->c_fixed_signs ? nullptr : c->c_sg.as<int8_t>(), c->U.as<T>(), c->curl_u.as<T>(),
-                     c->B.as<T>(), c->dt, (const double*)nullptr, guard);
+        // the step's sources (fb, dir_new, Poynting from U_old) ride along in
+        // dga_curl's appended workgroups: nothing reads fb or dir_new before
+        // the solve's initial residual, and U stays U_old until the scatter
+        const CurlSources<T> inj = c->n_inj ? curl_sources<T>(c, 2, cs, i > 1 ? cp : nullptr,
+                                                              c->poy.as<double>() + k * inj_blocks)
+                                            : CurlSources<T>{};
+        launch_timed(c, "dga_curl_kernel", dga_curl_kernel<T>, blocks_for(c->F) + (c->n_inj ? inj_blocks : 0), c->F,
+                     c->c_col.as<int32_t>(), c->c_fixed_signs ? nullptr : c->c_sg.as<int8_t>(), c->U.as<T>(),
+                     c->curl_u.as<T>(), c->B.as<T>(), c->dt, (const double*)nullptr, guard, inj);
         // gfdtd: F = N B (D:1468); fem: N C U_old for the curl-curl term (D:1729)
         TF_LAUNCH_TIMED(c, "sell_n_kernel", c->tune[TK_N].unroll, sell_n_kernel, T, gb_n,
                         tuned_view(c, TK_N, c->N.vals<T>()), c->fem ? c->curl_u.as<T>() : c->B.as<T>(), c->Fv.as<T>(), c->dt, 1, nullptr, guard);
@@ -2438,13 +2475,6 @@ void run_gfdtd(tf_ctx* c, int64_t first, int64_t n, const tf_cg_opts* cg, tf_sta
             // U_old.SigMat U_old of the losses (D:1519)
             TF_LAUNCH_TIMED(c, "quad_form_kernel", 8, quad_form_kernel, T, loss_blocks,
                 c->Sig.vals<T>(), c->U.as<T>(), c->loss_part.as<double>() + k * loss_blocks, guard);
-        }
-        if (c->n_inj) {
-            launch_timed(c, "gf_inject_kernel", gf_inject_kernel<T>, inj_blocks, 
-                c->n_inj, c->inj_edge.as<int32_t>(), c->inj_kind.as<int8_t>(), c->inj_tb.as<int32_t>(),
-                c->inj_te.as<int32_t>(), c->term_src.as<int32_t>(), c->term_w.as<double>(), cs, i > 1 ? cp : nullptr,
-                c->fb.as<T>(), c->dir_new.as<T>(), c->U.as<T>(), c->poy.as<double>() + k * inj_blocks, guard,
-                c->fem ? 1 : 0);
         }
         if (c->fem)
             launch_timed(c, "fem_w_kernel", fem_w_kernel<T>, blocks_for(c->E), c->E, c->edge_row.as<int32_t>(),
@@ -2898,7 +2928,7 @@ void autotune(tf_ctx* c) {
     if (c->method == TF_METHOD_DGA) {
         c->tune[TK_CT] = tune_one(c, TK_CT, c->sCt, kSpmvCap, [&](const KTune& t) {
             TF_LAUNCH_NB(c->wide, t.unroll, sell_ct_kernel, T, t.blocks, s, with_band(c->sCt.signs(), t),
-                         c->Fv.as<T>(), c->curl_f.as<T>());
+                         c->Fv.as<T>(), c->curl_f.as<T>(), CtSources{}, (T*)nullptr);
         });
         if (c->frac) return;  // lossy media: only sell_ct of the tuned kernels runs
         c->tune[TK_H] = tune_one(c, TK_H, c->H, kSpmvCap, [&](const KTune& t) {

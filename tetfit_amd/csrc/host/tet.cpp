@@ -15,6 +15,7 @@
 #include <cstring>
 #include <fstream>
 #include <numeric>
+#include <parallel/algorithm>
 #include <sstream>
 #include <tuple>
 #include <type_traits>
@@ -295,7 +296,8 @@ std::vector<int32_t> locality_rank(const Lattice& L, const std::vector<Vec3>& pt
     };
     std::vector<int32_t> code(n);
     std::vector<int64_t> base(3 * n);
-    for (size_t i = 0; i < n; ++i) {
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < (int64_t)n; ++i) {
         std::array<std::array<int64_t, 3>, 3> q{};
         for (int k = 0; k < per; ++k) L.snap(pts[nodes[per * i + k]], q[k].data());
         // (z, y, x) order of the snapped vertices
@@ -320,14 +322,17 @@ std::vector<int32_t> locality_rank(const Lattice& L, const std::vector<Vec3>& pt
     const long double span = (long double)L.cnt[2] * nband * ncls * L.cnt[1] * L.cnt[0];
     if (span >= 4.6e18L) return rank;
     std::vector<std::pair<int64_t, int32_t>> key(n);
-    for (size_t i = 0; i < n; ++i) {
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < (int64_t)n; ++i) {
         const int64_t x = base[3 * i], y = base[3 * i + 1], z = base[3 * i + 2];
         const int64_t cls = std::lower_bound(codes.begin(), codes.end(), code[i]) - codes.begin();
         key[i] = {(((z * nband + y / J) * ncls + cls) * L.cnt[1] + y) * L.cnt[0] + x, (int32_t)i};
     }
-    std::sort(key.begin(), key.end());
+    // (key, index) pairs are distinct, so any sort gives the same order
+    __gnu_parallel::sort(key.begin(), key.end());
     rank.resize(n);
-    for (size_t r = 0; r < n; ++r) rank[key[r].second] = (int32_t)r;
+#pragma omp parallel for schedule(static)
+    for (int64_t r = 0; r < (int64_t)n; ++r) rank[key[r].second] = (int32_t)r;
     return rank;
 }
 
@@ -423,7 +428,9 @@ void build_tet(const Model& m, const SimulationDef& sim, Build& b) {
     clk.mark("tets");
     // ---- faces: sort + unique (unique() D:10173-10219)
     std::vector<uint32_t> flab((size_t)T * 4);
-    std::sort(tri.begin(), tri.end(), [](const Tri& x, const Tri& y) {
+    // (copies of one face are equal under the comparator and get one label,
+    // so the order among them, which a parallel sort leaves open, is moot)
+    __gnu_parallel::sort(tri.begin(), tri.end(), [](const Tri& x, const Tri& y) {
         if (x.a != y.a) return x.a < y.a;
         if (x.b != y.b) return x.b < y.b;
         return x.c < y.c;
@@ -476,7 +483,7 @@ void build_tet(const Model& m, const SimulationDef& sim, Build& b) {
         ed[f + F] = {p0, p2, f + F};
         ed[f + 2 * (size_t)F] = {p1, p2, f + 2 * F};
     }
-    std::sort(ed.begin(), ed.end(), [](const Ed& x, const Ed& y) {
+    __gnu_parallel::sort(ed.begin(), ed.end(), [](const Ed& x, const Ed& y) {
         return x.a != y.a ? x.a < y.a : x.b < y.b;
     });
     std::vector<uint32_t> elab((size_t)F * 3);
