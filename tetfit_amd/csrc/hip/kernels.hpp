@@ -555,9 +555,12 @@ __global__ void __launch_bounds__(kBlock) dga_curl_kernel(int32_t n_faces, const
                                                           const double* __restrict__ face_dt, const CgState* guard,
                                                           CurlSources<T> py) {
     if (!cg_guard_ok(guard)) return;
-    const int lb = xcd_block(), face_blocks = (n_faces + kBlock - 1) / kBlock;
-    if (py.mode && lb >= face_blocks) {
+    // the appended workgroups are the last ones dispatched, spread over the
+    // XCDs; the face workgroups keep the XCD-aware map over their own count
+    const int face_blocks = (n_faces + kBlock - 1) / kBlock;
+    if (py.mode && (int)blockIdx.x >= face_blocks) {
         __shared__ double red[kBlock / kWave];
+        const int lb = (int)blockIdx.x;
         const int32_t i = (lb - face_blocks) * kBlock + threadIdx.x;
         double poy = 0;
         if (i < py.n_inj) {
@@ -574,7 +577,7 @@ __global__ void __launch_bounds__(kBlock) dga_curl_kernel(int32_t n_faces, const
         if (threadIdx.x == 0) py.poy_part[lb - face_blocks] = s;
         return;
     }
-    const int32_t f = lb * kBlock + threadIdx.x;
+    const int32_t f = xcd_block_of((int)blockIdx.x, py.mode ? face_blocks : (int)gridDim.x) * kBlock + threadIdx.x;
     if (f >= n_faces) return;
     const int32_t* c = c_col + 3 * (int64_t)f;
     double cu;

@@ -297,52 +297,11 @@ struct SellStream {
     }
 };
 
-// Sources of a dga step applied by the rows they sit on (sell_ct_kernel):
-// the injection items of the edges of slice s are items ptr[s] .. ptr[s+1]
-// (sorted by edge). An h item adds its value v to its row's curl_f
-// (Ctb Fb, D:878-882) and contributes the first Poynting half 0.5 U_old v;
-// an e (or d) item sets U[edge] = v (D:835-842): that edge is not a row of
-// H, so no later kernel of the step reads it before dga_curl.
-struct CtSources {
-    const int32_t* ptr;  // per slice (nslices + 1), null: no sources
-    const int32_t* edge;
-    const int8_t* kind;
-    const int32_t* tb;
-    const int32_t* te;
-    const int32_t* term_src;
-    const double* term_w;
-    const double* carr_step;
-    double* poy_part;  // one partial per workgroup
-};
-
-// Row r's value after its sources: curl_f rounded to T first, then v added
-// in fp64 (the rounding the separate injection pass had); poy collects the
-// lane's Poynting terms.
-template <typename T>
-__device__ __forceinline__ T ct_row_sources(const CtSources& src, int sl, int32_t r, T cf, T* __restrict__ U,
-                                            double& poy) {
-    const int a = src.ptr[sl], b = src.ptr[sl + 1];
-    for (int t = a; t < b; ++t) {
-        if (src.edge[t] != r) continue;
-        const double v = source_value(src.tb[t], src.te[t], src.term_src, src.term_w, src.carr_step);
-        if (src.kind[t] == 0) {
-            cf = (T)((double)cf + v);
-            poy += 0.5 * (double)U[r] * v;
-        } else {
-            U[r] = (T)v;
-        }
-    }
-    return cf;
-}
-
-// curl_f = Ct F over edges (dga, D:878-879), with the step's sources when
-// src.ptr is set (rows are edges; U is only touched by source items).
+// curl_f = Ct F over edges (dga, D:878-879).
 template <typename T, int NB, bool W = false>
 __global__ void __launch_bounds__(kBlock) sell_ct_kernel(SellView<int8_t> m, const T* __restrict__ x,
-                                                         T* __restrict__ y, CtSources src, T* __restrict__ U) {
+                                                         T* __restrict__ y) {
     const int lane = lane_id();
-    double poy = 0;
-    bool done = false;
     if constexpr (W) {
         if (m.col16 && m.cwin) {
             struct Ent { uint16_t c; int8_t v; };
@@ -354,26 +313,16 @@ __global__ void __launch_bounds__(kBlock) sell_ct_kernel(SellView<int8_t> m, con
                 [&](T& a, const Ent& e, T gv) { a += (T)e.v * gv; }, [&](int) { return 0; },
                 [&](int sl, T v, int) {
                     const int32_t r = sl * kSlice + lane;
-                    if (src.ptr) v = ct_row_sources(src, sl, r, v, U, poy);
                     if (r < m.nrows) y[r] = v;
                 });
-            done = true;
+            return;
         }
     }
-    if (!done) {
-        TF_FOR_SLICES(m) {
-            const int s = sr_.s;
-            const double v = sell_dot<T, NB, W>(m, s, lane, [&](int32_t j) { return x[j]; });
-            const int32_t r = s * kSlice + lane;
-            T cf = (T)v;
-            if (src.ptr) cf = ct_row_sources(src, s, r, cf, U, poy);
-            if (r < m.nrows) y[r] = cf;
-        }
-    }
-    if (src.poy_part) {
-        __shared__ double red[kBlock / kWave];
-        const double bs = block_sum(poy, red);
-        if (threadIdx.x == 0) src.poy_part[blockIdx.x] = bs;
+    TF_FOR_SLICES(m) {
+        const int s = sr_.s;
+        const double v = sell_dot<T, NB, W>(m, s, lane, [&](int32_t j) { return x[j]; });
+        const int32_t r = s * kSlice + lane;
+        if (r < m.nrows) y[r] = (T)v;
     }
 }
 

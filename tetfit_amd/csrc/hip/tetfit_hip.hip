@@ -258,7 +258,6 @@ struct tf_ctx {
     DevBuf c_col, c_sg, h_edge, curl_f, curl_u;
     DevBuf free_edge, dir_edge, dir_new, fb, pre, x, r, z, p0, p1, q, r0, cg_part, cg_state;
     DevBuf edge_row;  // per edge: its free row in the CG order, or -1 (Dirichlet)
-    DevBuf ct_src_ptr;  // dga: per Ct slice, its injection items (CtSources.ptr)
     DevBuf rs0, rs1;  // (r, s) pairs of the learned passes, even / odd pass
     // extrapolated starts (tf_cg_opts.guess): U_{n-1}, U_{n-2} on the free
     // rows, the start's shift d, gf_rhs_init's partials of ||b - A U_old||^2;
@@ -367,7 +366,7 @@ size_t tf_ctx::device_bytes() const {
     size_t s = 0;
     const DevBuf* all[] = {&U, &Fv, &B, &fit_ep, &fit_nu, &fe_sg, &fe_ca, &fe_cb, &fe_f4, &fh_sg, &fh_cm, &fh_cn, &fh_e4, &fe_code, &fh_code,
                            &sp_edge, &sp_f4, &sp_sg, &sp_ca, &sp_cb, &sp_kind, &sp_tb, &sp_te, &c_col, &c_sg,
-                           &h_edge, &curl_f, &curl_u, &free_edge, &edge_row, &ct_src_ptr, &dir_edge, &dir_new, &fb, &pre, &x, &r, &rs0, &rs1, &z,
+                           &h_edge, &curl_f, &curl_u, &free_edge, &edge_row, &dir_edge, &dir_new, &fb, &pre, &x, &r, &rs0, &rs1, &z,
                            &p0, &p1, &q, &r0, &hx1, &hx2, &dx, &rr0part, &cg_part, &cg_state, &inj_edge, &inj_kind, &inj_tb, &inj_te,
                            &term_src, &term_w, &carr, &probes, &samples, &eser, &epart, &poy, &row_dt_h, &row_dt_f,
                            &mu_a, &mu_b, &tf_a, &tf_b, &fslot, &uf[0], &uf[1], &ff[0], &ff[1], &U_older, &fw, &fw2};
@@ -1329,13 +1328,6 @@ void build_tet_inject(tf_ctx* c, const std::vector<std::vector<std::pair<int32_t
     c->dir_zero_static = c->k_dir_only && std::find(kind.begin(), kind.end(), (int8_t)1) == kind.end();
     for (int32_t e : c->h_dir_edge)
         if (c->h_fixed_value[e] != 0) c->dir_zero_static = false;
-    if (c->method == TF_METHOD_DGA && !c->frac && c->sCt.nslices > 0) {
-        // items of each Ct slice (rows are edges; items are sorted by edge)
-        std::vector<int32_t> ptr(c->sCt.nslices + 1, 0);
-        for (int32_t e : edge) ptr[e / kSlice + 1]++;
-        for (int32_t q = 0; q < c->sCt.nslices; ++q) ptr[q + 1] += ptr[q];
-        upload(c->ct_src_ptr, ptr);
-    }
     upload(c->inj_edge, edge);
     upload(c->inj_kind, kind);
     upload(c->inj_tb, tb);
@@ -1615,8 +1607,7 @@ void run_dga_frac(tf_ctx* c, int64_t first, int64_t n, tf_stats* st) {
         const double* cs = c->carr.as<double>() + (k + 1) * ns;
         double* poy = c->poy.as<double>() + 2 * k * inj_blocks;
         TF_LAUNCH_TIMED(c, "sell_ct_kernel", c->tune[TK_CT].unroll, sell_ct_kernel, T, gb_ct,
-                        tuned_view(c, TK_CT, c->sCt.signs()), c->Fv.as<T>(), c->curl_f.as<T>(), CtSources{},
-                        (T*)nullptr);
+                        tuned_view(c, TK_CT, c->sCt.signs()), c->Fv.as<T>(), c->curl_f.as<T>());
         inject(cs, poy, 0);
         const int nx = c->fcur ^ 1;
         if (c->nU > 0)
@@ -1689,9 +1680,10 @@ void run_dga(tf_ctx* c, int64_t first, int64_t n, tf_stats* st) {
     const double* fdt = c->batch ? c->row_dt_f.as<double>() : nullptr;
     const int gb_ct = tuned_blocks(c, TK_CT, c->sCt, kSpmvCap), gb_h = tuned_blocks(c, TK_H, c->H, kSpmvCap);
     const int gb_n = tuned_blocks(c, TK_N, c->N, kSpmvCap);
-    // Poynting partials of a step: sell_ct's workgroups (first half), then
-    // the injection workgroups appended to dga_curl (second half)
-    const int per_step = c->n_inj ? gb_ct + inj_blocks : 0;
+    // Poynting partials of a step: the injection kernel's workgroups (first
+    // half, 0.5 U_old v), then the workgroups appended to dga_curl (second
+    // half, 0.5 U_new v)
+    const int per_step = c->n_inj ? 2 * inj_blocks : 0;
     if (per_step && c->poy_cap < (int64_t)per_step * n) {
         c->poy.alloc((size_t)per_step * n * sizeof(double));
         c->poy_cap = (int64_t)per_step * n;
@@ -1708,24 +1700,22 @@ void run_dga(tf_ctx* c, int64_t first, int64_t n, tf_stats* st) {
         if (want_sample(c, 0, 0.0)) sample<T>(c, 0.0);
     }
     c->initial_done = true;
-    // one step (D:810-939): curl_f = Ct F with the h sources (and the e
-    // sources into U, the first Poynting half), U += dt H curl_f, curl_u =
-    // C U with the B update (and the second Poynting half), F -= dt N
-    // curl_u; poy holds the step's Poynting partials, per_step of them
+    // one step (D:810-939): curl_f = Ct F, the sources (h into curl_f, e into
+    // U, the first Poynting half), U += dt H curl_f, curl_u = C U with the B
+    // update and, in dga_curl's appended workgroups, the second Poynting half,
+    // F -= dt N curl_u
     auto step = [&](const double* cs, double* poy) {
-        CtSources src{};
-        CurlSources<T> py{};
-        if (c->n_inj) {
-            src = CtSources{c->ct_src_ptr.as<int32_t>(), c->inj_edge.as<int32_t>(), c->inj_kind.as<int8_t>(),
-                            c->inj_tb.as<int32_t>(), c->inj_te.as<int32_t>(), c->term_src.as<int32_t>(),
-                            c->term_w.as<double>(), cs, poy};
-            py = curl_sources<T>(c, 1, cs, nullptr, poy + gb_ct);
-        }
         TF_LAUNCH_TIMED(c, "sell_ct_kernel", c->tune[TK_CT].unroll, sell_ct_kernel, T, gb_ct,
-                        tuned_view(c, TK_CT, c->sCt.signs()), c->Fv.as<T>(), c->curl_f.as<T>(), src, c->U.as<T>());
+                        tuned_view(c, TK_CT, c->sCt.signs()), c->Fv.as<T>(), c->curl_f.as<T>());
+        if (c->n_inj)
+            launch_timed(c, "dga_inject_kernel", dga_inject_kernel<T>, inj_blocks, c->n_inj, c->inj_edge.as<int32_t>(),
+                         c->inj_kind.as<int8_t>(), c->inj_tb.as<int32_t>(), c->inj_te.as<int32_t>(),
+                         c->term_src.as<int32_t>(), c->term_w.as<double>(), cs, c->curl_f.as<T>(), c->U.as<T>(), poy,
+                         0);
         TF_LAUNCH_TIMED(c, "sell_h_kernel", c->tune[TK_H].unroll, sell_h_kernel, T, gb_h,
                         tuned_view(c, TK_H, c->H.vals<T>()), c->h_edge.as<int32_t>(), c->curl_f.as<T>(), c->U.as<T>(),
                         c->dt, hdt);
+        const CurlSources<T> py = c->n_inj ? curl_sources<T>(c, 1, cs, nullptr, poy + inj_blocks) : CurlSources<T>{};
         launch_timed(c, "dga_curl_kernel", dga_curl_kernel<T>, blocks_for(c->F) + (c->n_inj ? inj_blocks : 0), c->F,
                      c->c_col.as<int32_t>(), c->c_fixed_signs ? nullptr : c->c_sg.as<int8_t>(), c->U.as<T>(),
                      c->curl_u.as<T>(), c->B.as<T>(), c->dt, fdt, (const CgState*)nullptr, py);
@@ -2928,7 +2918,7 @@ void autotune(tf_ctx* c) {
     if (c->method == TF_METHOD_DGA) {
         c->tune[TK_CT] = tune_one(c, TK_CT, c->sCt, kSpmvCap, [&](const KTune& t) {
             TF_LAUNCH_NB(c->wide, t.unroll, sell_ct_kernel, T, t.blocks, s, with_band(c->sCt.signs(), t),
-                         c->Fv.as<T>(), c->curl_f.as<T>(), CtSources{}, (T*)nullptr);
+                         c->Fv.as<T>(), c->curl_f.as<T>());
         });
         if (c->frac) return;  // lossy media: only sell_ct of the tuned kernels runs
         c->tune[TK_H] = tune_one(c, TK_H, c->H, kSpmvCap, [&](const KTune& t) {
