@@ -526,12 +526,12 @@ constexpr int kRed2 = 2 * (kBlock / kWave) + 2;
 // reference's for the warm start while the iterates start elsewhere; a zero
 // warm-start residual there falls back to the start's own ||r_0||.
 __device__ __forceinline__ bool cg_decide_stop(int32_t it, double RR, double rr0_prev, int32_t rr0_given,
-                                               double tol, int32_t maxiter, CgState* st) {
+                                               double tol, int32_t maxiter, CgState* st, bool leader) {
     const bool own = it == 0 && !(rr0_given && rr0_prev > 0.0);
     const double rr0 = own ? RR : rr0_prev;
     const double ratio = rr0 == 0.0 ? 0.0 : std::sqrt(RR) / std::sqrt(rr0);
     const bool stop = !(ratio > tol && it < maxiter);
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
+    if (threadIdx.x == 0 && leader) {
         if (own) st->rr0 = RR;
         st->relres = ratio;
         if (stop) { st->done = 1; st->iter = it; }
@@ -552,6 +552,36 @@ struct alignas(2 * sizeof(T)) RS {
 template <typename T>
 __device__ __forceinline__ T cg_r_next(T r, T s, double alpha) {
     return (T)__builtin_fma(-alpha, (double)s, (double)r);
+}
+
+// Batched solves (tf_create_batch of gfdtd instances): independent systems
+// stacked block-diagonally, each instance's rows a whole number of 64-row
+// slices. Workgroup b works on slices [s0[b], s1[b]) of instance inst[b];
+// the per-workgroup partials of instance i are those of workgroups
+// [w0[i], w1[i]) (the same table for every kernel of a solve), so each
+// instance sums only its own partials in a fixed order, and its state is
+// st + 2 i (two slots per instance). inst == null: one solve over the grid.
+struct CgBatch {
+    const int32_t* inst = nullptr;
+    const int32_t* s0 = nullptr;
+    const int32_t* s1 = nullptr;
+    const int32_t* w0 = nullptr;
+    const int32_t* w1 = nullptr;
+};
+struct CgSeg {
+    int p0, n;       // partials of this workgroup's solve
+    CgState* st;
+    bool leader;     // the workgroup that records the solve's scalars
+};
+__device__ __forceinline__ CgSeg cg_seg(const CgBatch& bt, int nb, CgState* st) {
+    if (!bt.inst) return CgSeg{0, nb, st, blockIdx.x == 0};
+    const int i = bt.inst[blockIdx.x], w0 = bt.w0[i];
+    return CgSeg{w0, bt.w1[i] - w0, st + 2 * i, (int)blockIdx.x == w0};
+}
+__device__ __forceinline__ SliceRange cg_slices(int nslices, int band, const CgBatch& bt) {
+    if (!bt.inst) return block_slices(nslices, band);
+    const int wpb = (int)(blockDim.x / kWave), w = (int)(threadIdx.x / kWave);
+    return SliceRange{bt.s0[blockIdx.x] + w, bt.s1[blockIdx.x], wpb};
 }
 
 // Restart of a solve from its saved initial residual: r = r0 (r[i * r_stride]),
@@ -589,24 +619,32 @@ template <typename T, int NB, bool W = false>
 __global__ void __launch_bounds__(kBlock) cg_w_kernel(int32_t it, SellView<T> m, const T* __restrict__ u,
                                                       T* __restrict__ w, int test_stop,
                                                       const double* __restrict__ part_rr, int32_t nb_rr,
-                                                      double* __restrict__ part_del, CgState* st) {
+                                                      double* __restrict__ part_del, CgState* st_in,
+                                                      CgBatch bt) {
     __shared__ double red[kRed2];
-    const Partials prr = load_partials(part_rr, test_stop ? nb_rr : 0);
+    const CgSeg sg = cg_seg(bt, nb_rr, st_in);
+    CgState* st = sg.st;
+    const Partials prr = load_partials(part_rr + sg.p0, test_stop ? sg.n : 0);
     const int done = st->done;
     const double rr0 = st->rr0, tol = st->tol;
     const int32_t maxiter = st->maxiter, given = st->rr0_given;
-    if (done) return;
+    if (done) {
+        // a batched instance that has stopped leaves zero partials for the
+        // kernels of its later passes (they return at its done flag too)
+        return;
+    }
     if (test_stop) {
         double RR = thread_sum(prr), unused = 0;
         block_sum2_all(RR, unused, red);
-        if (cg_decide_stop(it, RR, rr0, given, tol, maxiter, st)) return;
+        if (cg_decide_stop(it, RR, rr0, given, tol, maxiter, st, sg.leader)) return;
     }
     const int lane = lane_id();
     double wu = 0, none = 0;
+    const SliceRange srng = cg_slices(m.nslices, m.band, bt);
     if constexpr (W) {
         if (m.col16 && m.cwin) {
             struct Ent { uint16_t c; T v; };
-            SellStream<NB, Ent, T, T> ss(m, block_slices(m.nslices, m.band), lane);
+            SellStream<NB, Ent, T, T> ss(m, srng, lane);
             auto load = [&](int64_t pos) { return Ent{stream_load<W>(m.col16 + pos), stream_load<W>(m.val + pos)}; };
             ss.start(load);
             ss.run(
@@ -625,7 +663,7 @@ __global__ void __launch_bounds__(kBlock) cg_w_kernel(int32_t it, SellView<T> m,
             return;
         }
     }
-    TF_FOR_SLICES(m) {
+    for (SliceRange sr_ = srng; sr_.s < sr_.end; sr_.s += sr_.step) {
         const int s = sr_.s;
         const double wv = sell_dot<T, NB, W>(m, s, lane, [&](int32_t j) { return u[j]; });
         const int32_t i = s * kSlice + lane;
@@ -649,16 +687,24 @@ __global__ void __launch_bounds__(kBlock) cg_upd_kernel(int32_t it, int32_t n, c
                                                         const double* __restrict__ part_gam, int32_t nb_gam,
                                                         const double* __restrict__ part_del, int32_t nb_del,
                                                         double* __restrict__ part_rr, double* __restrict__ part_gam_next,
-                                                        CgState* st) {
+                                                        CgState* st_in, CgBatch bt) {
     __shared__ double red[kRed2];
-    const Partials pg = load_partials(part_gam, nb_gam), pd = load_partials(part_del, nb_del);
+    const CgSeg sg = cg_seg(bt, nb_gam, st_in);
+    CgState* st = sg.st;
+    // batched: the same workgroup table for both partial arrays
+    const Partials pg = load_partials(part_gam + sg.p0, sg.n),
+                   pd = load_partials(part_del + (bt.inst ? sg.p0 : 0), bt.inst ? sg.n : nb_del);
     const int done = st->done;
     const double gam_prev = st->gam[(it - 1) & 1], alpha_prev = st->alpha[(it - 1) & 1];
     if (done) return;
+    // rows: a grid-stride loop, or (batched) the rows of the workgroup's slices
+    const int32_t rstep = bt.inst ? (int32_t)blockDim.x : (int32_t)(gridDim.x * blockDim.x);
+    const int32_t i0 = bt.inst ? bt.s0[blockIdx.x] * kSlice + (int32_t)threadIdx.x
+                               : (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+    const int32_t iend = bt.inst ? min(n, bt.s1[blockIdx.x] * kSlice) : n;
     // the thread's first element is loaded before the partial sums are reduced
-    const int32_t i0 = blockIdx.x * blockDim.x + threadIdx.x;
     T u0 = 0, w0 = 0, p0 = 0, s0 = 0, x0 = 0, r0 = 0, q0 = 0;
-    if (i0 < n) {
+    if (i0 < iend) {
         u0 = u[i0]; w0 = w[i0]; p0 = p[i0]; s0 = s[i0]; x0 = x[i0]; r0 = r[i0]; q0 = pre[i0];
     }
     double gam = thread_sum(pg), del = thread_sum(pd);
@@ -671,12 +717,12 @@ __global__ void __launch_bounds__(kBlock) cg_upd_kernel(int32_t it, int32_t n, c
         beta = gam / gam_prev;
         alpha = gam / (del - beta * gam / alpha_prev);
     }
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
+    if (threadIdx.x == 0 && sg.leader) {
         st->gam[it & 1] = gam;
         st->alpha[it & 1] = alpha;
     }
     double rr = 0, ru = 0;
-    for (int32_t i = i0; i < n; i += gridDim.x * blockDim.x) {
+    for (int32_t i = i0; i < iend; i += rstep) {
         if (i != i0) {
             u0 = u[i]; w0 = w[i]; p0 = p[i]; s0 = s[i]; x0 = x[i]; r0 = r[i]; q0 = pre[i];
         }
@@ -909,7 +955,7 @@ __global__ void __launch_bounds__(kBlock) cg_wu_kernel(int32_t it, SellView<T> m
     if (done) return;
     double RR = thread_sum(prr), gam = thread_sum(pga);
     block_sum2_all(RR, gam, red);
-    if (cg_decide_stop(it, RR, rr0, given, tol, maxiter, st)) return;
+    if (cg_decide_stop(it, RR, rr0, given, tol, maxiter, st, blockIdx.x == 0)) return;
     const double beta = it == 0 ? 0.0 : gam / gam_prev;
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         st->gam[it & 1] = gam;
@@ -998,13 +1044,15 @@ __global__ void __launch_bounds__(kBlock) gf_rhs_init_kernel(
     SellView<T> kop, SellView<T> r1, SellView<int8_t> ct, int dir_terms, const int32_t* __restrict__ free_edge,
     const T* __restrict__ U, const T* __restrict__ dir_new, const T* __restrict__ Fv, const T* __restrict__ fb,
     T* __restrict__ r0save, RhsCoef co, const T* __restrict__ pre, T* __restrict__ r, int r_stride,
-    T* __restrict__ u, double* __restrict__ part_rr, double* __restrict__ part_gam, CgState* st, int jacobi,
-    const CgState* guard, SellView<T> am, const T* __restrict__ d, double* __restrict__ part_rr0) {
+    T* __restrict__ u, double* __restrict__ part_rr, double* __restrict__ part_gam, CgState* st_in, int jacobi,
+    const CgState* guard, SellView<T> am, const T* __restrict__ d, double* __restrict__ part_rr0,
+    CgBatch bt, const double* __restrict__ row_dt) {
     if (!cg_guard_ok(guard)) return;
     __shared__ double red[kRed2];
     const int lane = lane_id();
+    const CgSeg sg = cg_seg(bt, 0, st_in);
     double rr = 0, ru = 0, rr0 = 0;
-    TF_FOR_SLICES(ct) {
+    for (SliceRange sr_ = cg_slices(ct.nslices, ct.band, bt); sr_.s < sr_.end; sr_.s += sr_.step) {
         const int s = sr_.s;
         const double ku = dir_terms && kop.sptr[s + 1] > kop.sptr[s]
                               ? sell_dot<T, 8, W>(kop, s, lane, [&](int32_t j) { return U[j]; })
@@ -1017,7 +1065,9 @@ __global__ void __launch_bounds__(kBlock) gf_rhs_init_kernel(
         const double ad = d ? sell_dot<T, NB, W>(am, s, lane, [&](int32_t j) { return d[j]; }) : 0.0;
         const int32_t k = s * kSlice + lane;
         if (k < ct.nrows) {
-            const T rk0 = (T)((co.ck * ku + co.cr * rs) + co.cc * cf + co.cf * (double)fb[free_edge[k]]);
+            // batched instances: each row with its instance's dt (gfdtd: cc = cf = dt)
+            const double cc = row_dt ? row_dt[k] : co.cc, cfb = row_dt ? row_dt[k] : co.cf;
+            const T rk0 = (T)((co.ck * ku + co.cr * rs) + cc * cf + cfb * (double)fb[free_edge[k]]);
             r0save[k] = rk0;
             T rk = rk0;
             if (d) {
@@ -1042,7 +1092,7 @@ __global__ void __launch_bounds__(kBlock) gf_rhs_init_kernel(
         part_rr[blockIdx.x] = rr;
         if (jacobi) part_gam[blockIdx.x] = ru;
         if (d) part_rr0[blockIdx.x] = rr0;
-        if (blockIdx.x == 0) { st->done = 0; st->iter = 0; }
+        if (sg.leader) { sg.st->done = 0; sg.st->iter = 0; }
     }
 }
 

@@ -301,9 +301,15 @@ struct tf_ctx {
         std::vector<std::vector<tf_probe>> probes;
         std::vector<int64_t> sample_steps;
         int64_t every = 0;                       // probe sample period in steps (0: off)
+        std::vector<int32_t> ecnt;               // edges of each instance (gfdtd batches pad eoff with
+                                                 // solver-only edges after them)
+        bool gfdtd = false;                      // CG-solved instances (run_gfdtd_batch)
     };
     std::unique_ptr<Batch> batch;
     DevBuf row_dt_h, row_dt_f;                   // per H row / per face dt (batches)
+    DevBuf row_dt_cg;                            // per CG row dt (gfdtd batches)
+    DevBuf bt_inst, bt_s0, bt_s1, bt_w0, bt_w1;  // CG workgroup table of a gfdtd batch (CgBatch)
+    int32_t bt_chunks = 0;
     std::vector<double> src_dt;                  // per source dt of the carrier table
 
     // sources
@@ -369,7 +375,7 @@ size_t tf_ctx::device_bytes() const {
                            &h_edge, &curl_f, &curl_u, &free_edge, &edge_row, &dir_edge, &dir_new, &fb, &pre, &x, &r, &rs0, &rs1, &z,
                            &p0, &p1, &q, &r0, &hx1, &hx2, &dx, &rr0part, &cg_part, &cg_state, &inj_edge, &inj_kind, &inj_tb, &inj_te,
                            &term_src, &term_w, &carr, &probes, &samples, &eser, &epart, &poy, &row_dt_h, &row_dt_f,
-                           &mu_a, &mu_b, &tf_a, &tf_b, &fslot, &uf[0], &uf[1], &ff[0], &ff[1], &U_older, &fw, &fw2};
+                           &mu_a, &mu_b, &tf_a, &tf_b, &fslot, &row_dt_cg, &bt_inst, &bt_s0, &bt_s1, &bt_w0, &bt_w1, &uf[0], &uf[1], &ff[0], &ff[1], &U_older, &fw, &fw2};
     for (auto b : all) s += b->bytes;
     const Csr32* cs[] = {&Ct};
     for (auto c : cs) s += c->ptr.bytes + c->col.bytes + c->val.bytes + c->sg.bytes;
@@ -546,6 +552,22 @@ std::vector<int32_t> free_row_order(const tf_dga_ops& d) {
     std::stable_sort(perm.begin(), perm.end(), [&](int32_t a, int32_t b) {
         return d.edge_rank[d.free_edges[a]] < d.edge_rank[d.free_edges[b]];
     });
+    return perm;
+}
+
+// Row order of a gfdtd batch's CG rows: instance k owns merged rows
+// [row0[k], row0[k+1]) (multiples of 64, so no slice holds two instances);
+// inside an instance, SELL-C-sigma windows that stop at its end.
+std::vector<int32_t> batch_row_order(const tf_dga_ops& d, const std::vector<int32_t>& row0) {
+    std::vector<int32_t> perm(d.n_free);
+    for (int32_t i = 0; i < d.n_free; ++i) perm[i] = i;
+    for (size_t k = 0; k + 1 < row0.size(); ++k)
+        for (int32_t w0 = row0[k]; w0 < row0[k + 1]; w0 += kSigma) {
+            const int32_t w1 = std::min(row0[k + 1], w0 + kSigma);
+            std::stable_sort(perm.begin() + w0, perm.begin() + w1, [&](int32_t a, int32_t b) {
+                return d.A.rowptr[a + 1] - d.A.rowptr[a] > d.A.rowptr[b + 1] - d.A.rowptr[b];
+            });
+        }
     return perm;
 }
 
@@ -1024,7 +1046,7 @@ void residual_operator(const tf_dga_ops& d, double dt, const std::vector<int32_t
 }
 
 template <class T>
-void create_tet(tf_ctx* c, const tf_problem* p) {
+void create_tet(tf_ctx* c, const tf_problem* p, const std::vector<int32_t>* batch_rows) {
     const tf_dga_ops& d = p->dga;
     const int32_t E = c->E, F = c->F;
     if (p->method == TF_METHOD_DGA) upload_sell<T>(c->sCt, d.Ct, false);
@@ -1118,7 +1140,7 @@ void create_tet(tf_ctx* c, const tf_problem* p) {
         // residual) is free edge free_edges[perm[k]], and A's columns are
         // relabeled to match, so all CG vectors use that order.
         if (d.A.nrows != d.n_free) throw std::runtime_error("A must have one row per free edge");
-        const std::vector<int32_t> perm = free_row_order(d);
+        const std::vector<int32_t> perm = batch_rows ? batch_row_order(d, *batch_rows) : free_row_order(d);
         std::vector<int32_t> inv(d.n_free);
         for (int32_t k = 0; k < d.n_free; ++k) inv[perm[k]] = k;
         CsrBuf a;
@@ -1837,7 +1859,7 @@ void cg_start(tf_ctx* c, CgSolve& cs, bool with_rhs, const CgState* guard, int e
                         c->fem ? c->fw2.as<T>() : c->dir_new.as<T>(), c->Fv.as<T>(), c->fb.as<T>(),
                         c->r0.as<T>(), co, c->pre.as<T>(), r0, r0_stride, v.u, v.part_rr, v.part_gj[0], dst,
                         cs.learned ? 0 : 1, guard, v.A, shifted ? (const T*)c->dx.as<T>() : nullptr,
-                        c->rr0part.as<double>());
+                        c->rr0part.as<double>(), CgBatch{}, (const double*)nullptr);
         if (shifted)
             launch_timed(c, "cg_set_rr0_kernel", cg_set_rr0_kernel, 1, c->rr0part.as<double>(), v.nb_b, dst, guard);
     } else {
@@ -1882,12 +1904,12 @@ void cg_passes(tf_ctx* c, CgSolve& cs, int n) {
             continue;  // at it == maxiter the stop test of cg_wu ends the solve
         }
         TF_LAUNCH_TIMED(c, "cg_w_kernel", c->tune[TK_W].unroll, cg_w_kernel, T, v.nb_w, it, v.A, (const T*)v.u,
-                        v.w, 1, (const double*)v.part_rr, v.nb_b, v.part_del, dst);
+                        v.w, 1, (const double*)v.part_rr, v.nb_b, v.part_del, dst, CgBatch{});
         tag(it, 0);
         if (it == cs.init.maxiter) { ++it; break; }  // the pass at maxiter only runs the stop test
         launch_timed(c, "cg_upd_kernel", cg_upd_kernel<T>, v.nb_b, it, v.nf, (const T*)c->pre.as<T>(), v.x, v.rj,
                      v.u, (const T*)v.w, v.p, v.sv, (const double*)v.part_gj[it & 1], v.nb_b,
-                     (const double*)v.part_del, v.nb_w, v.part_rr, v.part_gj[(it + 1) & 1], dst);
+                     (const double*)v.part_del, v.nb_w, v.part_rr, v.part_gj[(it + 1) & 1], dst, CgBatch{});
         tag(it, 0);
         cg_capture<T>(c, cs, v, it + 1);
     }
@@ -2657,7 +2679,8 @@ tf_status guarded(F f) {
     }
 }
 
-std::unique_ptr<tf_ctx> make_ctx(const tf_problem* p, int32_t prec) {
+std::unique_ptr<tf_ctx> make_ctx(const tf_problem* p, int32_t prec,
+                                 const std::vector<int32_t>* batch_rows = nullptr) {
     if (p->method < 0 || p->method > 3) throw std::runtime_error("unknown method");
     if (prec != TF_PREC_FP64 && prec != TF_PREC_FP32) throw std::runtime_error("unknown precision");
     if (p->n_edges <= 0 || p->n_faces <= 0) throw std::runtime_error("empty problem");
@@ -2695,8 +2718,8 @@ std::unique_ptr<tf_ctx> make_ctx(const tf_problem* p, int32_t prec) {
             c->h_esid = rl.ep;
             c->h_fsid = rl.fp;
         }
-        if (prec == TF_PREC_FP64) create_tet<double>(c.get(), q);
-        else create_tet<float>(c.get(), q);
+        if (prec == TF_PREC_FP64) create_tet<double>(c.get(), q, batch_rows);
+        else create_tet<float>(c.get(), q, batch_rows);
         std::vector<std::vector<std::pair<int32_t, double>>> none(c->E);
         std::vector<int> fld(c->E, TF_FIELD_H);
         build_tet_inject(c.get(), none, fld);
@@ -2959,10 +2982,10 @@ void autotune(tf_ctx* c) {
         // Jacobi pass: cg_w (stop test on cg_upd's partials) + the cg_upd that sums its b partials
         c->tune[TK_W] = tune_one(c, TK_W, c->A, kCgPartA, [&](const KTune& t) {
             TF_LAUNCH_NB(c->wide, t.unroll, cg_w_kernel, T, t.blocks, s, 1, with_band(c->A.vals<T>(), t), c->z.as<T>(),
-                         c->q.as<T>(), 1, one, nbb, part, st);
+                         c->q.as<T>(), 1, one, nbb, part, st, CgBatch{});
             cg_upd_kernel<T><<<nbb, kBlock, 0, s>>>(1, nf, c->pre.as<T>(), c->q.as<T>(), c->r.as<T>(), c->z.as<T>(),
                                                     c->q.as<T>(), c->p0.as<T>(), c->p1.as<T>(), one, nbb, one,
-                                                    t.blocks, part, part, st);
+                                                    t.blocks, part, part, st, CgBatch{});
         });
         if (c->lp.state == LearnedPc::kActive) {
             const SellView<uint16_t> gv = c->A.view<uint16_t>(c->lp.gval);
@@ -2998,6 +3021,265 @@ void autotune(tf_ctx* c) {
     }
 }
 
+// Block-diagonal concatenation of n lossless gfdtd problems for a batched
+// CG (run_gfdtd_batch). Instance k's free edges are padded to a multiple of
+// 64 with solver-only edges appended after its own edges: no face touches
+// them (empty C columns and Ct rows), their rows of A and E are the diagonal
+// 1, so their CG rows stay 0 (no right-hand side) and every 64-row slice of
+// the CG belongs to one instance. row0[k] is instance k's first CG row.
+struct MergedGf {
+    std::vector<int64_t> c_ptr, ct_ptr, n_ptr, e_ptr, a_ptr, r1_ptr;
+    std::vector<int32_t> c_col, ct_col, n_col, e_col, a_col, r1_col, bnd_edge, free_edges;
+    std::vector<int8_t> c_sg, ct_sg, edge_class;
+    std::vector<double> n_val, e_val, a_val, r1_val, fixed_value, f_dt, cg_dt;
+    std::vector<int32_t> eoff, foff, ecnt, row0;
+    tf_problem p{};
+};
+
+void merge_gfdtd(const tf_problem* const* probs, int32_t n, MergedGf& m) {
+    m.eoff.assign(1, 0);
+    m.foff.assign(1, 0);
+    m.row0.assign(1, 0);
+    int32_t boff = 0;
+    auto empty_rows = [](std::vector<int64_t>& ptr, int32_t k) {
+        if (ptr.empty()) ptr.push_back(0);
+        for (int32_t r = 0; r < k; ++r) ptr.push_back(ptr.back());
+    };
+    auto unit_rows = [](std::vector<int64_t>& ptr, std::vector<int32_t>& col, std::vector<double>& val,
+                        int32_t first_col, int32_t k) {
+        if (ptr.empty()) ptr.push_back(0);
+        for (int32_t r = 0; r < k; ++r) {
+            col.push_back(first_col + r);
+            val.push_back(1.0);
+            ptr.push_back((int64_t)col.size());
+        }
+    };
+    for (int32_t k = 0; k < n; ++k) {
+        const tf_problem* p = probs[k];
+        if (!p) throw std::runtime_error("null problem in batch");
+        if (p->method != TF_METHOD_GFDTD) throw std::runtime_error("a batch holds one method (dga or gfdtd)");
+        const tf_dga_ops& d = p->dga;
+        if (d.Sig.nnz > 0) throw std::runtime_error("gfdtd batches hold lossless problems only");
+        if (!(p->dt > 0)) throw std::runtime_error("batch instance without a time step");
+        const int32_t E = p->n_edges, F = p->n_faces, nf = d.n_free;
+        if (d.C.nrows != F || d.Ct.nrows != E || d.N.nrows != F || d.A.nrows != nf || d.RHS1.nrows != E ||
+            d.E.nrows != E || !d.free_edges)
+            throw std::runtime_error("inconsistent operators");
+        const int32_t pad = (kSlice - nf % kSlice) % kSlice;
+        const int64_t eo = m.eoff.back(), fo = m.foff.back(), ro = m.row0.back();
+        if (eo + E + pad > INT32_MAX || fo + F > INT32_MAX) throw std::runtime_error("batch too large");
+        append_csr(d.C, (int32_t)eo, m.c_ptr, m.c_col, &m.c_sg, nullptr);
+        append_csr(d.Ct, (int32_t)fo, m.ct_ptr, m.ct_col, &m.ct_sg, nullptr);
+        empty_rows(m.ct_ptr, pad);
+        append_csr(d.N, (int32_t)fo, m.n_ptr, m.n_col, nullptr, &m.n_val);
+        append_csr(d.E, (int32_t)eo, m.e_ptr, m.e_col, nullptr, &m.e_val);
+        unit_rows(m.e_ptr, m.e_col, m.e_val, (int32_t)(eo + E), pad);
+        append_csr(d.RHS1, (int32_t)eo, m.r1_ptr, m.r1_col, nullptr, &m.r1_val);
+        empty_rows(m.r1_ptr, pad);
+        append_csr(d.A, (int32_t)ro, m.a_ptr, m.a_col, nullptr, &m.a_val);
+        unit_rows(m.a_ptr, m.a_col, m.a_val, (int32_t)(ro + nf), pad);
+        for (int32_t i = 0; i < nf; ++i) m.free_edges.push_back((int32_t)(eo + d.free_edges[i]));
+        for (int32_t j = 0; j < pad; ++j) m.free_edges.push_back((int32_t)(eo + E + j));
+        for (int32_t e = 0; e < E + pad; ++e) {
+            const bool own = e < E;
+            m.bnd_edge.push_back(own && d.bnd_edge && d.bnd_edge[e] >= 0 ? d.bnd_edge[e] + boff : -1);
+            m.edge_class.push_back(own ? d.edge_class[e] : (int8_t)1);
+            m.fixed_value.push_back(own ? d.fixed_value[e] : 0.0);
+        }
+        m.f_dt.insert(m.f_dt.end(), (size_t)F, p->dt);
+        m.cg_dt.insert(m.cg_dt.end(), (size_t)(nf + pad), p->dt);
+        boff += d.n_bnd;
+        m.ecnt.push_back(E);
+        m.eoff.push_back((int32_t)(eo + E + pad));
+        m.foff.push_back((int32_t)(fo + F));
+        m.row0.push_back((int32_t)(ro + nf + pad));
+    }
+    const int32_t E = m.eoff.back(), F = m.foff.back(), NF = m.row0.back();
+    tf_problem& p = m.p;
+    p.method = TF_METHOD_GFDTD;
+    p.n_edges = E;
+    p.n_faces = F;
+    p.dt = probs[0]->dt;
+    p.duration = probs[0]->duration;
+    for (int32_t k = 0; k < n; ++k) p.n_vols += probs[k]->n_vols;
+    tf_dga_ops& d = p.dga;
+    d.C = csr_view(F, E, m.c_ptr, m.c_col, &m.c_sg, nullptr);
+    d.Ct = csr_view(E, F, m.ct_ptr, m.ct_col, &m.ct_sg, nullptr);
+    d.N = csr_view(F, F, m.n_ptr, m.n_col, nullptr, &m.n_val);
+    d.E = csr_view(E, E, m.e_ptr, m.e_col, nullptr, &m.e_val);
+    d.A = csr_view(NF, NF, m.a_ptr, m.a_col, nullptr, &m.a_val);
+    d.RHS1 = csr_view(E, E, m.r1_ptr, m.r1_col, nullptr, &m.r1_val);
+    d.free_edges = m.free_edges.data();
+    d.n_free = NF;
+    d.bnd_edge = m.bnd_edge.data();
+    d.n_bnd = boff;
+    d.edge_class = m.edge_class.data();
+    d.fixed_value = m.fixed_value.data();
+}
+
+// CG workgroup table of a gfdtd batch: each instance's slices in chunks of
+// kBatchChunk (whole instances of up to 4 waves' worth of slices per
+// workgroup), the workgroups of one instance consecutive.
+constexpr int kBatchChunk = 8;
+void build_batch_table(tf_ctx* c, const std::vector<int32_t>& row0) {
+    std::vector<int32_t> inst, s0, s1, w0, w1;
+    const int32_t n = (int32_t)row0.size() - 1;
+    for (int32_t k = 0; k < n; ++k) {
+        const int32_t a = row0[k] / kSlice, b = row0[k + 1] / kSlice;
+        w0.push_back((int32_t)inst.size());
+        for (int32_t q = a; q < b; q += kBatchChunk) {
+            inst.push_back(k);
+            s0.push_back(q);
+            s1.push_back(std::min(b, q + kBatchChunk));
+        }
+        if ((int32_t)inst.size() - w0.back() > kMaxPart) throw std::runtime_error("batch instance too large");
+        if (a == b) throw std::runtime_error("batch instance without free edges");
+        w1.push_back((int32_t)inst.size());
+    }
+    c->bt_chunks = (int32_t)inst.size();
+    upload(c->bt_inst, inst);
+    upload(c->bt_s0, s0);
+    upload(c->bt_s1, s1);
+    upload(c->bt_w0, w0);
+    upload(c->bt_w1, w1);
+    // partials per workgroup: w.u, r.r and r.u (two), fp64 (at least the
+    // single-solve layout, which the autotuner's trial launches use)
+    c->cg_part.alloc(sizeof(double) *
+                     std::max<size_t>(4 * (size_t)c->bt_chunks, kCgPartA + 3 * kCgPartB + 2 * kCgPartC));
+    // two state slots per instance (slot 0 used), pinned host copies
+    c->cg_state.alloc(2 * sizeof(CgState) * (size_t)n);
+    TF_HIP(hipMemsetAsync(c->cg_state.p, 0, c->cg_state.bytes, c->stream));
+    if (c->h_state) TF_HIP(hipHostFree(c->h_state));
+    c->h_state = nullptr;
+    TF_HIP(hipHostMalloc((void**)&c->h_state, 2 * sizeof(CgState) * (size_t)n, hipHostMallocDefault));
+    std::memset((void*)c->h_state, 0, 2 * sizeof(CgState) * (size_t)n);
+}
+
+// n steps of a gfdtd batch (D:1459-1565 for every instance): the loop of
+// run_gfdtd with plain Jacobi CG, one solve per instance per step (each on
+// its own scalars and stop test, CgBatch), the host waiting on all of them
+// before the next step.
+template <class T>
+void run_gfdtd_batch(tf_ctx* c, int64_t first, int64_t n, const tf_cg_opts* cg, tf_stats* st) {
+    if (cg && cg->precond != TF_PRECOND_JACOBI) throw std::runtime_error("gfdtd batches solve with Jacobi CG only");
+    if (cg && cg->guess) throw std::runtime_error("gfdtd batches start each solve from U_n (guess 0)");
+    auto& b = *c->batch;
+    const int32_t ni = b.n;
+    const int inj_blocks = c->n_inj ? blocks_for(c->n_inj) : 0;
+    const size_t ns = std::max<size_t>(c->src.size(), 1);
+    const int32_t nf = c->n_free;
+    if (inj_blocks && c->poy_cap < n * inj_blocks) {
+        c->poy.alloc(n * inj_blocks * sizeof(double));
+        c->poy_cap = n * inj_blocks;
+    }
+    CgState init{};
+    init.tol = cg ? cg->tol : 1e-8;
+    init.maxiter = cg ? cg->maxiter : 100;
+    std::vector<CgState> inits((size_t)2 * ni, init);
+    const CgBatch bt{c->bt_inst.as<int32_t>(), c->bt_s0.as<int32_t>(), c->bt_s1.as<int32_t>(),
+                     c->bt_w0.as<int32_t>(), c->bt_w1.as<int32_t>()};
+    const int nb = c->bt_chunks;
+    double* part_del = c->cg_part.as<double>();
+    double* part_rr = part_del + nb;
+    double* part_gj[2] = {part_rr + nb, part_rr + 2 * nb};
+    CgState* states = c->cg_state.as<CgState>();
+    const int gb_n = tuned_blocks(c, TK_N, c->N, kSpmvCap);
+    const double* fdt = c->row_dt_f.as<double>();
+    const SellView<T> A = c->A.vals<T>();
+    auto dir_terms = [&](int64_t k) { return !(c->dir_zero_static && (k > 0 || c->ud_zero)); };
+    // the pass before the loop (D:1434-1453), when the run starts at step 1
+    if (first == 1) {
+        if (c->n_inj)
+            gf_inject_kernel<T><<<inj_blocks, kBlock, 0, c->stream>>>(
+                c->n_inj, c->inj_edge.as<int32_t>(), c->inj_kind.as<int8_t>(), c->inj_tb.as<int32_t>(),
+                c->inj_te.as<int32_t>(), c->term_src.as<int32_t>(), c->term_w.as<double>(), c->carr.as<double>(),
+                nullptr, c->fb.as<T>(), c->dir_new.as<T>(), c->U.as<T>(), c->poy.as<double>(), nullptr, 0);
+        if (c->n_dir > 0)
+            gf_scatter_kernel<T><<<blocks_for(c->n_dir), kBlock, 0, c->stream>>>(
+                0, c->free_edge.as<int32_t>(), c->x.as<T>(), c->U.as<T>(), c->n_dir, c->dir_edge.as<int32_t>(),
+                c->dir_new.as<T>(), nullptr, nullptr);
+        c->ud_zero = c->dir_zero_static;
+    }
+    for (int64_t k = 0; k < n; ++k) {
+        const int64_t i = first + k;
+        const double t = (double)i * c->dt;
+        const double* cs = c->carr.as<double>() + (k + 1) * ns;
+        const double* cp = c->carr.as<double>() + k * ns;
+        const CurlSources<T> inj = c->n_inj ? curl_sources<T>(c, 2, cs, i > 1 ? cp : nullptr,
+                                                              c->poy.as<double>() + k * inj_blocks)
+                                            : CurlSources<T>{};
+        launch_timed(c, "dga_curl_kernel", dga_curl_kernel<T>, blocks_for(c->F) + (c->n_inj ? inj_blocks : 0), c->F,
+                     c->c_col.as<int32_t>(), c->c_fixed_signs ? nullptr : c->c_sg.as<int8_t>(), c->U.as<T>(),
+                     c->curl_u.as<T>(), c->B.as<T>(), c->dt, fdt, (const CgState*)nullptr, inj);
+        TF_LAUNCH_TIMED(c, "sell_n_kernel", c->tune[TK_N].unroll, sell_n_kernel, T, gb_n,
+                        tuned_view(c, TK_N, c->N.vals<T>()), c->B.as<T>(), c->Fv.as<T>(), c->dt, 1, nullptr,
+                        nullptr);
+        // the state before the solve, at t - dt (as run_gfdtd samples it)
+        if (want_sample(c, i - 1, t - c->dt)) sample<T>(c, t - c->dt);
+        TF_HIP(hipMemcpyAsync(states, inits.data(), inits.size() * sizeof(CgState), hipMemcpyHostToDevice,
+                              c->stream));
+        // x = U_old on the free rows is the warm start (x keeps the last solve)
+        TF_LAUNCH_TIMED(c, "gf_rhs_init_kernel", 8, gf_rhs_init_kernel, T, nb, c->Kf.vals<T>(), c->R1f.vals<T>(),
+                        c->Ctf.signs(), dir_terms(k) ? 1 : 0, c->free_edge.as<int32_t>(), c->U.as<T>(),
+                        c->dir_new.as<T>(), c->Fv.as<T>(), c->fb.as<T>(), c->r0.as<T>(), RhsCoef{1.0, -1.0, 0.0, 0.0},
+                        c->pre.as<T>(), c->r.as<T>(), 1, c->z.as<T>(), part_rr, part_gj[0], states, 1,
+                        (const CgState*)nullptr, A, (const T*)nullptr, (double*)nullptr, bt,
+                        c->row_dt_cg.as<double>());
+        int it = 0;
+        auto passes = [&](int m) {
+            const int stop = std::min(it + m, init.maxiter + 1);
+            for (; it < stop; ++it) {
+                TF_LAUNCH_TIMED(c, "cg_w_kernel", c->tune[TK_W].unroll, cg_w_kernel, T, nb, it, A,
+                                (const T*)c->z.as<T>(), c->q.as<T>(), 1, (const double*)part_rr, nb, part_del, states,
+                                bt);
+                if (it == init.maxiter) { ++it; break; }
+                launch_timed(c, "cg_upd_kernel", cg_upd_kernel<T>, nb, it, nf, (const T*)c->pre.as<T>(),
+                             c->x.as<T>(), c->r.as<T>(), c->z.as<T>(), (const T*)c->q.as<T>(), c->p0.as<T>(),
+                             c->p1.as<T>(), (const double*)part_gj[it & 1], nb, (const double*)part_del, nb, part_rr,
+                             part_gj[(it + 1) & 1], states, bt);
+            }
+            TF_HIP(hipMemcpyAsync(c->h_state, states, 2 * sizeof(CgState) * (size_t)ni, hipMemcpyDeviceToHost,
+                                  c->stream));
+            TF_HIP(hipStreamSynchronize(c->stream));
+        };
+        passes(std::max(1, c->cg_guess + 1));
+        for (;;) {
+            bool all = true;
+            for (int32_t q = 0; q < ni && all; ++q) all = c->h_state[2 * q].done != 0;
+            if (all || it > init.maxiter) break;
+            passes(4);
+        }
+        int32_t it_max = 0;
+        for (int32_t q = 0; q < ni; ++q) {
+            const CgState& h = c->h_state[2 * q];
+            const int64_t iters = h.done ? h.iter : init.maxiter;
+            it_max = std::max<int32_t>(it_max, (int32_t)iters);
+            st->cg_iters_total += iters;
+            st->cg_iters_max = std::max<int32_t>(st->cg_iters_max, (int32_t)iters);
+            if (std::isfinite(h.relres)) st->cg_max_relres = std::max(st->cg_max_relres, h.relres);
+            if (!h.done || !(h.relres <= init.tol)) st->cg_not_converged++;
+        }
+        c->cg_guess = it_max;
+        launch_timed(c, "gf_scatter_kernel", gf_scatter_edges_kernel<T>, blocks_for(c->E), c->E,
+                     c->edge_row.as<int32_t>(), (const T*)c->x.as<T>(), c->U.as<T>(), (const T*)c->dir_new.as<T>(),
+                     (const CgState*)nullptr, (T*)nullptr);
+        if (c->profiling) ev_flush(c);
+    }
+    if (n > 0) c->ud_zero = c->dir_zero_static;
+    TF_HIP(hipGetLastError());
+    std::vector<double> part((size_t)n * std::max(inj_blocks, 1), 0.0);
+    if (inj_blocks)
+        TF_HIP(hipMemcpyAsync(part.data(), c->poy.p, n * inj_blocks * sizeof(double), hipMemcpyDeviceToHost,
+                              c->stream));
+    TF_HIP(hipStreamSynchronize(c->stream));
+    for (int64_t k = 0; k < n; ++k) {
+        double s = 0;
+        for (int q = 0; q < inj_blocks; ++q) s += part[k * inj_blocks + q];
+        c->poynting.push_back(s);
+    }
+    st->steps_done += n;
+}
+
 }  // namespace
 
 // ====================================================================== C ABI
@@ -3021,6 +3303,33 @@ tf_status tf_create_batch(const tf_problem* const* probs, int32_t n, int32_t pre
     if (!probs || !out || n < 1) { g_err = "bad argument"; return TF_ERR_ARG; }
     *out = nullptr;
     return guarded([&] {
+        if (!probs[0]) throw std::runtime_error("null problem in batch");
+        if (probs[0]->method == TF_METHOD_GFDTD) {
+            // CG-solved instances: block-diagonal system, one Jacobi CG per
+            // instance per step (run_gfdtd_batch)
+            MergedGf m;
+            merge_gfdtd(probs, n, m);
+            auto c = make_ctx(&m.p, prec, &m.row0);
+            build_batch_table(c.get(), m.row0);
+            std::vector<double> cgdt((size_t)c->n_free);
+            if (cgdt.size() != m.cg_dt.size()) throw std::runtime_error("batch CG rows");
+            upload(c->row_dt_cg, m.cg_dt);  // CG rows keep the merged order (instance blocks stay in place)
+            upload(c->row_dt_f, m.f_dt);
+            auto bt = std::make_unique<tf_ctx::Batch>();
+            bt->n = n;
+            bt->gfdtd = true;
+            bt->eoff = m.eoff;
+            bt->foff = m.foff;
+            bt->ecnt = m.ecnt;
+            bt->poff.assign(n + 1, 0);
+            bt->src.resize(n);
+            bt->terms.resize(n);
+            bt->probes.resize(n);
+            for (int32_t k = 0; k < n; ++k) bt->dt.push_back(probs[k]->dt);
+            c->batch = std::move(bt);
+            *out = c.release();
+            return;
+        }
         MergedDga m;
         merge_dga(probs, n, m);
         auto c = make_ctx(&m.p, prec);
@@ -3031,6 +3340,7 @@ tf_status tf_create_batch(const tf_problem* const* probs, int32_t n, int32_t pre
         bt->n = n;
         bt->eoff = m.eoff;
         bt->foff = m.foff;
+        for (int32_t k = 0; k < n; ++k) bt->ecnt.push_back(m.eoff[k + 1] - m.eoff[k]);
         bt->poff.assign(n + 1, 0);
         bt->src.resize(n);
         bt->terms.resize(n);
@@ -3135,6 +3445,8 @@ tf_status tf_run(tf_ctx* c, int64_t first, int64_t n, const tf_cg_opts* cg, tf_s
             if (f64) run_fit<double>(c, first, n, st); else run_fit<float>(c, first, n, st);
         } else if (c->method == TF_METHOD_DGA) {
             if (f64) run_dga<double>(c, first, n, st); else run_dga<float>(c, first, n, st);
+        } else if (c->batch && c->batch->gfdtd) {
+            if (f64) run_gfdtd_batch<double>(c, first, n, cg, st); else run_gfdtd_batch<float>(c, first, n, cg, st);
         } else {
             if (f64) run_gfdtd<double>(c, first, n, cg, st); else run_gfdtd<float>(c, first, n, cg, st);
         }
@@ -3497,7 +3809,7 @@ tf_status tf_batch_set_sources(tf_ctx* c, int32_t inst, const tf_source* src, in
     return guarded([&] {
         drop_graph(c);
         auto& b = *c->batch;
-        const int32_t E = b.eoff[inst + 1] - b.eoff[inst];
+        const int32_t E = b.ecnt[inst];
         for (int64_t i = 0; i < n_terms; ++i)
             if (terms[i].edge < 0 || terms[i].edge >= E || terms[i].source < 0 || terms[i].source >= n_src)
                 throw std::runtime_error("source term out of range");
@@ -3515,7 +3827,7 @@ tf_status tf_batch_set_probes(tf_ctx* c, int32_t inst, const tf_probe* probes, i
     return guarded([&] {
         auto& b = *c->batch;
         const int32_t eo = b.eoff[inst], fo = b.foff[inst];
-        const int32_t E = b.eoff[inst + 1] - eo, F = b.foff[inst + 1] - fo;
+        const int32_t E = b.ecnt[inst], F = b.foff[inst + 1] - fo;
         std::vector<tf_probe> v(probes, probes + n);
         for (tf_probe& p : v) {
             if (p.ne < 0 || p.ne > TF_PROBE_MAX_E || p.nh < 0 || p.nh > TF_PROBE_MAX_H)
@@ -3575,7 +3887,7 @@ tf_status tf_batch_read_state(tf_ctx* c, int32_t inst, double* U, double* F, dou
         const tf_status rc = tf_read_state(c, u.data(), f.data(), bb.data());
         if (rc != TF_OK) throw std::runtime_error(g_err);
         const int32_t eo = b.eoff[inst], fo = b.foff[inst];
-        if (U) std::copy(u.begin() + eo, u.begin() + b.eoff[inst + 1], U);
+        if (U) std::copy(u.begin() + eo, u.begin() + eo + b.ecnt[inst], U);
         if (F) std::copy(f.begin() + fo, f.begin() + b.foff[inst + 1], F);
         if (B) std::copy(bb.begin() + fo, bb.begin() + b.foff[inst + 1], B);
     });

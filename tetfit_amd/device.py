@@ -211,9 +211,12 @@ class Simulation:
 
 
 class BatchSimulation:
-    """One device context stepping many independent dga simulations together
+    """One device context stepping many independent simulations together
     (tf_create_batch): operators concatenated block-diagonally, one launch per
-    phase for the whole batch, each instance on its own dt and sources."""
+    phase for the whole batch, each instance on its own dt and sources. dga
+    instances step explicitly; gfdtd instances (lossless) solve one Jacobi CG
+    per instance per step, each with its own scalars and stop test, at the
+    instances' common CG tolerance."""
 
     def __init__(self, builds, precision="fp32", sample_every=0):
         L = hip_lib()
@@ -244,12 +247,30 @@ class BatchSimulation:
         """tf_set_graphs: step the batch in HIP graphs when nothing is sampled (default on)."""
         _check(hip_lib().tf_set_graphs(self._h, int(bool(on))))
 
-    def run(self, n, first=None):
+    def run(self, n, first=None, cg_tol=None, cg_maxiter=100):
         first = self.next_step if first is None else first
         st = _ffi.tf_stats()
-        _check(hip_lib().tf_run(self._h, int(first), int(n), None, C.byref(st)))
+        opts = None
+        if self.builds[0].method == "gfdtd":
+            tols = {b.cg_tol for b in self.builds}
+            if cg_tol is None and len(tols) != 1:
+                raise ValueError("gfdtd batch instances need one CG tolerance")
+            opts = C.byref(_ffi.tf_cg_opts(tols.pop() if cg_tol is None else cg_tol, int(cg_maxiter),
+                                           PRECOND["jacobi"], 0, 0, 0.0, 0))
+        _check(hip_lib().tf_run(self._h, int(first), int(n), opts, C.byref(st)))
         self.next_step = first + n
-        return dict(steps=st.steps_done, seconds=st.seconds)
+        return dict(steps=st.steps_done, seconds=st.seconds, cg_iters_total=st.cg_iters_total,
+                    cg_iters_max=st.cg_iters_max, cg_not_converged=st.cg_not_converged,
+                    cg_max_relres=st.cg_max_relres)
+
+    def poynting(self):
+        """tf_read_poynting: the batch's Poynting scalar per step (all instances summed)."""
+        L = hip_lib()
+        n = C.c_int64()
+        _check(L.tf_read_poynting(self._h, None, 0, C.byref(n)))
+        v = np.zeros(max(n.value, 1))
+        _check(L.tf_read_poynting(self._h, v.ctypes.data_as(C.POINTER(C.c_double)), n.value, C.byref(n)))
+        return v[:n.value]
 
     def state(self, k):
         b = self.builds[k]
