@@ -201,11 +201,12 @@ def cpu_baseline(b, steps):
                 host=os.uname().nodename, nproc=os.cpu_count())
 
 
-def batch_workload(d, count=256, steps=2000, seed=4, workers=16):
-    """Batched dga workload: `count` Kuhn cavities with n ~ U{8..32}, jitter
+def batch_workload(d, count=256, steps=2000, seed=4, workers=16, method="dga"):
+    """Batched workload: `count` Kuhn cavities with n ~ U{8..32}, jitter
     0.2h, one eps_r ~ U[1, 10] and one source frequency ~ U[1e8, 5e8] Hz per
     instance (numpy default_rng(seed)); dt = 0.9 x each instance's own
-    stability estimate. Host builds run in a thread pool."""
+    stability estimate; method dga (explicit) or gfdtd (CG tol 1e-8). Host
+    builds run in a thread pool."""
     from concurrent.futures import ThreadPoolExecutor
     from tetfit_amd import Model, configs
     rng = np.random.default_rng(seed)
@@ -214,7 +215,7 @@ def batch_workload(d, count=256, steps=2000, seed=4, workers=16):
     freq = rng.uniform(1e8, 5e8, size=count)
 
     def one(k):
-        p = configs.kuhn_cavity(d, n=int(ns[k]), method="dga", n_steps=steps, n_probes=1, block=4,
+        p = configs.kuhn_cavity(d, n=int(ns[k]), method=method, n_steps=steps, n_probes=1, block=4,
                                 seed=1000 + k, eps_range=(float(eps[k]), float(eps[k])), freq=float(freq[k]),
                                 dt="courant", name=f"inst{k}")
         return Model(p).build(1)
@@ -222,7 +223,7 @@ def batch_workload(d, count=256, steps=2000, seed=4, workers=16):
     t = time.time()
     with ThreadPoolExecutor(max_workers=workers) as ex:
         builds = list(ex.map(one, range(count)))
-    log(f"[bench] built {count} dga instances, n in [{ns.min()}, {ns.max()}], "
+    log(f"[bench] built {count} {method} instances, n in [{ns.min()}, {ns.max()}], "
         f"sum E = {sum(b.n_edges for b in builds)}, sum F = {sum(b.n_faces for b in builds)} "
         f"({time.time() - t:.1f}s host set-up)")
     return builds
@@ -594,9 +595,10 @@ def bench_copy(args):
 
 def bench_batch(args):
     from tetfit_amd.device import BatchSimulation
-    steps = args.steps if args.steps_given else 2000
+    gf = args.batch_method == "gfdtd"
+    steps = args.steps if args.steps_given else (500 if gf else 2000)
     with tempfile.TemporaryDirectory() as d:
-        builds = batch_workload(d, count=args.batch, steps=steps)
+        builds = batch_workload(d, count=args.batch, steps=steps, method=args.batch_method)
         E = sum(b.n_edges for b in builds)
         res = {}
         for prec in ("fp32", "fp64"):
@@ -612,8 +614,32 @@ def bench_batch(args):
             sim.close()
             res[prec] = dict(seconds=st["seconds"], ms_per_step=1e3 * st["seconds"] / steps,
                              edge_dof_per_s=E * steps / st["seconds"], kernels=kern, tuning=tuned)
+            if gf:
+                res[prec].update(cg_iters_per_step_per_instance=st["cg_iters_total"] / steps / len(builds),
+                                 cg_iters_max=st["cg_iters_max"], cg_not_converged=st["cg_not_converged"],
+                                 cg_max_relres=st["cg_max_relres"])
             log(f"[bench] batch {prec}: {res[prec]['ms_per_step']:.3f} ms/step, "
-                f"{res[prec]['edge_dof_per_s'] / 1e9:.3f} G edge-DoF/s")
+                f"{res[prec]['edge_dof_per_s'] / 1e9:.3f} G edge-DoF/s"
+                + (f", CG {res[prec]['cg_iters_per_step_per_instance']:.2f} it/step/instance" if gf else ""))
+        if gf:
+            head = res["fp32"]
+            line = {
+                "metric": "edge-DoF updates/s, batched gfdtd (all instances), 1 GPU",
+                "value": head["edge_dof_per_s"], "unit": "edge-DoF/s", "n_gpus": 1, "steps": steps,
+                "warmup": args.warmup, "ms_per_step": head["ms_per_step"], "higher_is_better": True,
+                "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+                "config": {"workload": f"{len(builds)} Kuhn gfdtd cavities, n~U{{8..32}}, eps~U[1,10], "
+                                       f"f~U[1e8,5e8] Hz, CG tol 1e-8 (Jacobi, one solve per instance)",
+                           "n_edges_total": E, "n_faces_total": sum(b.n_faces for b in builds),
+                           "parallelism": "single GPU, one launch per phase and CG pass for the whole batch"},
+                "fp64": {k: res["fp64"][k] for k in ("edge_dof_per_s", "ms_per_step", "cg_iters_per_step_per_instance",
+                                                     "cg_not_converged", "cg_max_relres")},
+                "cg_iters_per_step_per_instance": head["cg_iters_per_step_per_instance"],
+                "cg_not_converged": head["cg_not_converged"], "cg_max_relres": head["cg_max_relres"],
+                "roofline": None, "cpu_baseline": None, "kernels": head["kernels"], "tuning": head["tuning"],
+            }
+            print(json.dumps(line), flush=True)
+            return
         head = res["fp32"]
         nnz_h = sum(int(b.dga_ops()["H"]["rowptr"][-1]) for b in builds)
         n_h = sum(int(b.dga_ops()["H"]["nrows"]) for b in builds)
@@ -659,6 +685,7 @@ def main():
                          "fit: FIT 64^3; fitprec: FIT random-media fp32 vs fp64 study (--n, default 256); copy: "
                          "a 1 GiB device copy (the reference kernel for HBM counter scaling).")
     ap.add_argument("--batch", type=int, default=256, help="instances of the batch workload")
+    ap.add_argument("--batch-method", choices=["dga", "gfdtd"], default="dga", help="method of the batch workload")
     ap.add_argument("--paths", default="plain64,jacobi32,learned32,learned64",
                     help="gfdtd paths to run (profiling runs pick one)")
     ap.add_argument("--learn-steps", type=int, default=200, help="collection solves of the learned paths")
