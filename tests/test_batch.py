@@ -63,6 +63,14 @@ def gf_instances(tmp_path_factory):
     for k, (n, seed, er, f) in enumerate(specs):
         p = configs.kuhn_cavity(str(d), n=n, method="gfdtd", n_steps=STEPS, n_probes=3, block=2, seed=seed,
                                 eps_range=er, freq=f, name=f"g{k}")
+        if k == 1:
+            # an e source (Dirichlet values through dir_new) and a j dipole beside the h source
+            from test_gpu_sources import add_source, edit
+            edit(p, lambda s: add_source(s, 2, ["type j", "location {0.43,0.57,0.61}", "direction z",
+                                                "carrier ricker", "frequency %r" % f, "delay %r" % (1.5 / f),
+                                                "amplitude 2", "impedance 3"]))
+            edit(p, lambda s: add_source(s, 3, ["type e", "surface 4", "direction y", "carrier gaussian",
+                                                "width %r" % (1.0 / f), "delay %r" % (2.0 / f)]))
         out.append(Model(p).build(1))
     return out
 
