@@ -3311,9 +3311,10 @@ tf_status tf_create_batch(const tf_problem* const* probs, int32_t n, int32_t pre
             merge_gfdtd(probs, n, m);
             auto c = make_ctx(&m.p, prec, &m.row0);
             build_batch_table(c.get(), m.row0);
-            std::vector<double> cgdt((size_t)c->n_free);
-            if (cgdt.size() != m.cg_dt.size()) throw std::runtime_error("batch CG rows");
-            upload(c->row_dt_cg, m.cg_dt);  // CG rows keep the merged order (instance blocks stay in place)
+            // per CG row dt: the CG rows permute only inside each instance's
+            // block (batch_row_order), so the merged order's values hold
+            if ((size_t)c->n_free != m.cg_dt.size()) throw std::runtime_error("batch CG rows");
+            upload(c->row_dt_cg, m.cg_dt);
             upload(c->row_dt_f, m.f_dt);
             auto bt = std::make_unique<tf_ctx::Batch>();
             bt->n = n;
