@@ -1259,17 +1259,16 @@ void build_tet(const Model& m, const SimulationDef& sim, Build& b) {
                 }
             };
             std::vector<double> kx(nfree), w(nfree);
-            auto apply = [&](const std::vector<double>& x, std::vector<double>& yy) {
-                Kx(x, kx);
-                solveA(kx, yy);
-            };
             // Rayleigh quotient in the A inner product after power steps
             std::vector<double> x(nfree);
             uint64_t st = 0x9E3779B97F4A7C15ull;
             for (auto& v : x) { st ^= st << 13; st ^= st >> 7; st ^= st << 17; v = (double)(st >> 11) * 1.1102230246251565e-16 - 0.5; }
             double lam = 0;
+            // K x of the iterate is kept from the Rayleigh quotient of the
+            // step before, so each step applies K once
+            Kx(x, kx);
             for (int it = 0; it < 25; ++it) {
-                apply(x, y);
+                solveA(kx, y);
                 double n2 = std::sqrt(dotv(y, y));
                 if (n2 == 0) break;
 #pragma omp parallel for schedule(static)
