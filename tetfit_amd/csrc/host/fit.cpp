@@ -351,14 +351,23 @@ void build_fit(const Model& m, const SimulationDef& sim, Build& b) {
     std::vector<int32_t>().swap(ct_face);
 
     clk.mark("Ct");
-    // ---- edge classes (D:5382-5471)
-    std::vector<std::vector<uint32_t>> edge_src(totE);
+    // ---- edge classes (D:5382-5471). An edge none of whose faces lies on
+    // the boundary is common; the others (a thin shell) run the reference's
+    // per-face classification in edge order. The sources of an edge are kept
+    // only for the edges that have some (a sparse map).
+    std::map<uint32_t, std::vector<uint32_t>> edge_src;
     std::vector<uint32_t> edge_bcs(totE, 0);
     auto bc_of = [&](uint32_t label) -> const BoundaryCondition& {
         uint32_t l = label < m.bnd_to_bc.size() ? m.bnd_to_bc[label] : 0;
         return bcs[l];
     };
+    std::vector<uint8_t> common(totE, 1), shell(totE, 0);
+#pragma omp parallel for schedule(static)
+    for (int64_t e = 0; e < (int64_t)totE; ++e)
+        for (int64_t q = b.ct_ptr[e]; q < b.ct_ptr[e + 1]; ++q)
+            if (boundary_face[b.ct_face[q]] > 0) { shell[e] = 1; break; }
     for (uint32_t e = 0; e < totE; ++e) {
+        if (!shell[e]) continue;
         uint32_t in_b = 0, brk = 0;
         for (int64_t q = b.ct_ptr[e]; q < b.ct_ptr[e + 1]; ++q) {
             uint32_t ff = (uint32_t)b.ct_face[q];
@@ -367,7 +376,7 @@ void build_fit(const Model& m, const SimulationDef& sim, Build& b) {
                 if (bc_of(bid).type == "pec") {
                     ++in_b;
                     edge_bcs[e] = bid < m.bnd_to_bc.size() ? m.bnd_to_bc[bid] : 0;
-                    edge_src[e].clear();
+                    edge_src.erase(e);
                     b.bc_edges.push_back((int32_t)e);
                     if (brk == 1) b.src_edges.pop_back();
                     else if (brk == 2) b.uncommon_edges.pop_back();
@@ -380,15 +389,17 @@ void build_fit(const Model& m, const SimulationDef& sim, Build& b) {
                         ++in_b;
                         if (src.st == "e") {
                             if (brk == 2) break;
-                            if (std::find(edge_src[e].begin(), edge_src[e].end(), sl) == edge_src[e].end()) {
-                                edge_src[e].push_back(sl);
+                            std::vector<uint32_t>& es = edge_src[e];
+                            if (std::find(es.begin(), es.end(), sl) == es.end()) {
+                                es.push_back(sl);
                                 brk = 1;
                                 if (b.src_edges.empty() || b.src_edges.back() != (int32_t)e) b.src_edges.push_back((int32_t)e);
                             }
                         } else if (src.st == "h") {
                             if (brk == 1) break;
-                            if (std::find(edge_src[e].begin(), edge_src[e].end(), sl) == edge_src[e].end()) {
-                                edge_src[e].push_back(sl);
+                            std::vector<uint32_t>& es = edge_src[e];
+                            if (std::find(es.begin(), es.end(), sl) == es.end()) {
+                                es.push_back(sl);
                                 brk = 2;
                                 if (b.uncommon_edges.empty() || b.uncommon_edges.back() != (int32_t)e)
                                     b.uncommon_edges.push_back((int32_t)e);
@@ -398,7 +409,27 @@ void build_fit(const Model& m, const SimulationDef& sim, Build& b) {
                 }
             }
         }
-        if (in_b == 0) b.common_edges.push_back((int32_t)e);
+        common[e] = in_b == 0;
+    }
+    {
+        // common edges in edge order: counts per chunk, then a parallel fill
+        constexpr int64_t kChunk = 1 << 20;
+        const int64_t nch = ((int64_t)totE + kChunk - 1) / kChunk;
+        std::vector<int64_t> off(nch + 1, 0);
+#pragma omp parallel for schedule(static)
+        for (int64_t c = 0; c < nch; ++c) {
+            int64_t k = 0;
+            for (int64_t e = c * kChunk; e < std::min<int64_t>(totE, (c + 1) * kChunk); ++e) k += common[e];
+            off[c + 1] = k;
+        }
+        for (int64_t c = 0; c < nch; ++c) off[c + 1] += off[c];
+        b.common_edges.resize(off[nch]);
+#pragma omp parallel for schedule(static)
+        for (int64_t c = 0; c < nch; ++c) {
+            int64_t k = off[c];
+            for (int64_t e = c * kChunk; e < std::min<int64_t>(totE, (c + 1) * kChunk); ++e)
+                if (common[e]) b.common_edges[k++] = (int32_t)e;
+        }
     }
     for (int32_t e : b.common_edges)
         if (b.ct_ptr[e + 1] - b.ct_ptr[e] != 4)
@@ -578,7 +609,8 @@ void build_fit(const Model& m, const SimulationDef& sim, Build& b) {
     clk.mark("dt estimate");
     // ---- coefficients (D:5487-5649). PML factors are 1 without PML.
     b.M_nu.resize(totF); b.M_mu.resize(totF); b.fpml.assign(totF, 1.0); b.Nu_vec.assign(totF, 0.0);
-    for (uint32_t f = 0; f < totF; ++f) {
+#pragma omp parallel for schedule(static)
+    for (int64_t f = 0; f < (int64_t)totF; ++f) {
         if (avg_ni[f] != 0) {
             b.Nu_vec[f] = avg_ni[f] / face_area[f];
             if (mag_lossy[f]) {
@@ -590,19 +622,22 @@ void build_fit(const Model& m, const SimulationDef& sim, Build& b) {
             }
         } else { b.M_nu[f] = 0; b.M_mu[f] = 0; }
     }
-    b.M_h.resize(totE); b.M_q.resize(totE); b.M_e.resize(totE); b.upml.assign(totE, 1.0);
-    for (uint32_t e = 0; e < totE; ++e) {
-        if (!edge_src[e].empty()) {  // source consistency (D:5573-5586)
-            std::vector<uint32_t> keep;
-            const SourceDef& s0 = sources[edge_src[e].front()];
-            uint32_t surf0 = s0.surface_label.empty() ? 0 : s0.surface_label.front();
-            for (uint32_t k : edge_src[e]) {
-                const SourceDef& sk = sources[k];
-                uint32_t sf = sk.surface_label.empty() ? 0 : sk.surface_label.front();
-                if (sk.st == s0.st && sf == surf0) keep.push_back(k);
-            }
-            edge_src[e] = keep;
+    for (auto& kv : edge_src) {  // source consistency (D:5573-5586)
+        std::vector<uint32_t>& es = kv.second;
+        if (es.empty()) continue;
+        std::vector<uint32_t> keep;
+        const SourceDef& s0 = sources[es.front()];
+        uint32_t surf0 = s0.surface_label.empty() ? 0 : s0.surface_label.front();
+        for (uint32_t k : es) {
+            const SourceDef& sk = sources[k];
+            uint32_t sf = sk.surface_label.empty() ? 0 : sk.surface_label.front();
+            if (sk.st == s0.st && sf == surf0) keep.push_back(k);
         }
+        es = keep;
+    }
+    b.M_h.resize(totE); b.M_q.resize(totE); b.M_e.resize(totE); b.upml.assign(totE, 1.0);
+#pragma omp parallel for schedule(static)
+    for (int64_t e = 0; e < (int64_t)totE; ++e) {
         if (avg_eps[e] != 0) {
             if (ele_lossy[e]) {
                 b.M_h[e] = edge_len[e] / (avg_eps[e] + 0.5 * t_step * avg_sigma[e]);
@@ -768,19 +803,21 @@ void build_fit(const Model& m, const SimulationDef& sim, Build& b) {
         return v;
     };
     auto add_edge = [&](uint32_t e) {
-        if (edge_src[e].empty()) return;
+        auto es_it = edge_src.find(e);
+        if (es_it == edge_src.end() || es_it->second.empty()) return;
+        const std::vector<uint32_t>& es = es_it->second;
         tf_source_edge d{};
         d.edge = (int32_t)e;
-        const SourceDef& s0 = sources[edge_src[e].front()];
+        const SourceDef& s0 = sources[es.front()];
         d.field = make_source(s0).field;  // only e, h (D:5431-5455) and j dipoles reach edge_src here
-        if (edge_src[e].size() > 4)
+        if (es.size() > 4)
             throw Error("more than 4 sources on one edge (edge " + std::to_string(e) + ")");
         Vec3 eb = edge_bar(e);
         Vec3 ev = P(G[(size_t)e * 2 + 1]) - P(G[(size_t)e * 2]);
         Vec3 vec = d.field == TF_FIELD_H ? bnd_dual(e) : ev;
         d.bary[0] = eb.x; d.bary[1] = eb.y; d.bary[2] = eb.z;
         d.vec[0] = vec.x; d.vec[1] = vec.y; d.vec[2] = vec.z;
-        for (uint32_t sl : edge_src[e]) {
+        for (uint32_t sl : es) {
             const SourceDef& s = sources[sl];
             d.src[d.n_src++] = src_index[sl];
             if (make_source(s).field != d.field) continue;  // the loop only evaluates sources of the edge's kind
