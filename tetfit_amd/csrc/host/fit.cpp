@@ -45,6 +45,25 @@ const std::vector<Src> kNodeReuse[8] = {{{0, 4}, {1, 2}, {2, 1}}, {{0, 5}, {1, 3
                                         {{1, 6}, {2, 5}},         {{1, 7}},         {{2, 7}},         {}};
 const std::vector<Src> kFaceReuse[6] = {{{0, 5}}, {{1, 4}}, {{2, 3}}, {}, {}, {}};
 
+// A table of n values filled on all host threads (the mesher's per-entity
+// tables take several GB at 256^3 cells; a sequential zero fill of them cost
+// more than building them).
+template <class T>
+struct ParTable {
+    std::unique_ptr<T[]> p;
+    size_t n = 0;
+    ParTable() = default;
+    ParTable(size_t n_, T v) : p(new T[n_]), n(n_) {  // trivial T: left uninitialised by new
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < (int64_t)n_; ++i) p[i] = v;
+    }
+    T& operator[](size_t i) { return p[i]; }
+    const T& operator[](size_t i) const { return p[i]; }
+    size_t size() const { return n; }
+    T* begin() { return p.get(); }
+    T* end() { return p.get() + n; }
+};
+
 // The 12 E-probe weights use the dual area of each edge's axis (D:3555-3568).
 }  // namespace
 
@@ -152,14 +171,13 @@ void build_fit(const Model& m, const SimulationDef& sim, Build& b) {
 
     std::vector<int32_t>& E_cluster = b.e_cluster;  // 12 per cube
     std::vector<int32_t>& D = b.d_faces;            // 6 per cube
-    std::vector<int32_t> P_cluster;                 // 8 per cube
-    E_cluster.reserve(totV * 12);
-    D.reserve(totV * 6);
-    P_cluster.reserve(totV * 8);
-    b.pts.reserve(totP * 3);
-    std::vector<int32_t> Dt(totF * 2, -1);   // signed cube ids, up to 2 per face: value = cube, sign separate
-    std::vector<int8_t> Dt_sgn(totF * 2, 0);
-    std::vector<uint8_t> Dt_n(totF, 0);
+    ParTable<int32_t> P_cluster(totV * 8, 0);       // 8 per cube
+    E_cluster.assign(totV * 12, 0);
+    D.assign(totV * 6, 0);
+    b.pts.assign(totP * 3, 0.0);
+    ParTable<int32_t> Dt(totF * 2, -1);   // signed cube ids, up to 2 per face: value = cube, sign separate
+    ParTable<int8_t> Dt_sgn(totF * 2, 0);
+    ParTable<uint8_t> Dt_n(totF, 0);
     std::vector<int32_t>& G = b.g_nodes;  // 2 per edge
     G.assign(totE * 2, -1);
     // grid positions of edges and faces (tf_fit_ops edge_sid / face_sid)
@@ -178,163 +196,230 @@ void build_fit(const Model& m, const SimulationDef& sim, Build& b) {
         if (a == 1) return (int32_t)(nFx + (k * (Ny + 1) + j) * Nx + i);
         return (int32_t)(nFx + nFy + (k * Ny + j) * Nx + i);
     };
-    std::vector<uint8_t> edge_has_G(totE, 0), face_has_C(totF, 0);
     b.c_edge.assign(totF * 4, 0);
-    std::vector<int32_t> ct_face(totE * 4, 0);
-    std::vector<int8_t> ct_sgn(totE * 4, 0);
-    std::vector<uint8_t> ct_n(totE, 0);
+    ParTable<int32_t> ct_face(totE * 4, 0);
+    ParTable<int8_t> ct_sgn(totE * 4, 0);
+    ParTable<uint8_t> ct_n(totE, 0);
     b.curl.assign(totF, 1);
     b.dual_curl.assign(totE, 0);
-    std::vector<double> edge_len(totE, 0), face_area(totF, 0);
-    std::vector<double> avg_eps(totE, 0), avg_sigma(totE, 0), avg_ni(totF, 0), avg_mag_sigma(totF, 0);
-    std::vector<uint8_t> ele_lossy(totE, 0), mag_lossy(totF, 0);
-    std::vector<uint32_t> boundary_face;
-    boundary_face.reserve(totF);
-    std::vector<uint8_t> tbc;
-    tbc.reserve(totF);
-    std::vector<uint32_t> vol_material;
-    vol_material.reserve(totV);
-
-    uint32_t nv = 0, nf = 0, ne = 0, np = 0;
-    double px, py, pz = zmin;
-    for (uint32_t k = 0; k < Nz; ++k) {
-        py = ymin;
-        for (uint32_t j = 0; j < Ny; ++j) {
-            px = xmin;
-            for (uint32_t i = 0; i < Nx; ++i) {
-                // WhichSolid D:10110: first solid (by label) containing the centre
-                uint32_t mat = 0;
-                for (auto& s : solids)
-                    if (s.second.rule(px + 0.5 * Lx, py + 0.5 * Ly, pz + 0.5 * Lz)) { mat = s.second.material; break; }
-                vol_material.push_back(mat);
-                const Material& mt = materials[mat];
-
-                uint32_t nb[3] = {0, 0, 0};  // 1-based neighbour cube ids
-                if (pz > zmin) nb[0] = nv + 1 - Nx * Ny;
-                if (py > ymin) nb[1] = nv + 1 - Nx;
-                if (px > xmin) nb[2] = nv;
-
-                int32_t dcube[6], ecube[12], pcube[8];
-                for (int f = 0; f < 6; ++f) {
-                    dcube[f] = -1;
-                    for (const Src& s : kFaceReuse[f])
-                        if (nb[s.nb]) { dcube[f] = D[(size_t)(nb[s.nb] - 1) * 6 + s.pos]; break; }
-                }
-                for (int f = 0; f < 6; ++f)
-                    if (dcube[f] < 0) dcube[f] = (int32_t)nf++;
-                for (int e = 0; e < 12; ++e) {
-                    ecube[e] = -1;
-                    for (const Src& s : kEdgeReuse[e])
-                        if (nb[s.nb]) { ecube[e] = E_cluster[(size_t)(nb[s.nb] - 1) * 12 + s.pos]; break; }
-                }
-                for (int e = 0; e < 12; ++e)
-                    if (ecube[e] < 0) ecube[e] = (int32_t)ne++;
-                const Vec3 pp(px, py, pz);
-                for (int q = 0; q < 8; ++q) {
-                    pcube[q] = -1;
-                    for (const Src& s : kNodeReuse[q])
-                        if (nb[s.nb]) { pcube[q] = P_cluster[(size_t)(nb[s.nb] - 1) * 8 + s.pos]; break; }
-                }
-                for (int q = 0; q < 8; ++q)
-                    if (pcube[q] < 0) {
-                        pcube[q] = (int32_t)np++;
-                        Vec3 p = pp;
-                        if (q & 4) p = p + Vec3(0, 0, Lz);
-                        if (q & 1) p = p + Vec3(Lx, 0, 0);
-                        if (q & 2) p = p + Vec3(0, Ly, 0);
-                        b.pts.push_back(p.x); b.pts.push_back(p.y); b.pts.push_back(p.z);
-                    }
-                for (int f = 0; f < 6; ++f) D.push_back(dcube[f]);
-                for (int e = 0; e < 12; ++e) E_cluster.push_back(ecube[e]);
-                for (int q = 0; q < 8; ++q) P_cluster.push_back(pcube[q]);
-                for (int e = 0; e < 12; ++e) {
-                    const int q0 = kEdgeNodes[e][0];  // the edge's lower node
-                    b.edge_sid[ecube[e]] = edge_pos(kEdgeAxis[e], i + (q0 & 1), j + (q0 >> 1 & 1), k + (q0 >> 2 & 1));
-                }
-                for (int f = 0; f < 6; ++f) {
-                    const int a = kFaceAxis[f], o = f >= 3 ? 1 : 0;
-                    b.face_sid[dcube[f]] = face_pos(a, i + (a == 0 ? o : 0), j + (a == 1 ? o : 0), k + (a == 2 ? o : 0));
-                }
-
-                for (int f = 0; f < 6; ++f) {
-                    int32_t fid = dcube[f];
-                    uint8_t& n = Dt_n[fid];
-                    Dt[(size_t)fid * 2 + n] = (int32_t)nv;
-                    Dt_sgn[(size_t)fid * 2 + n] = f < 3 ? -1 : 1;
-                    ++n;
-                }
-                for (int e = 0; e < 12; ++e) {
-                    int32_t eid = ecube[e];
-                    if (!edge_has_G[eid]) {
-                        edge_has_G[eid] = 1;
-                        G[(size_t)eid * 2] = pcube[kEdgeNodes[e][0]];
-                        G[(size_t)eid * 2 + 1] = pcube[kEdgeNodes[e][1]];
-                        edge_len[eid] = len[kEdgeAxis[e]];
-                        b.dual_curl[eid] = (int8_t)kDualCurl[kEdgeAxis[e]];
-                    }
-                }
-                for (int f = 0; f < 6; ++f) {
-                    int32_t fid = dcube[f];
-                    if (!face_has_C[fid]) {
-                        face_has_C[fid] = 1;
-                        for (int q = 0; q < 4; ++q) {
-                            int32_t eid = ecube[kFaceEdges[f][q]];
-                            b.c_edge[(size_t)fid * 4 + q] = eid;
-                            uint8_t& n = ct_n[eid];
-                            ct_face[(size_t)eid * 4 + n] = fid;
-                            ct_sgn[(size_t)eid * 4 + n] = (int8_t)kFaceSigns[f][q];
-                            ++n;
-                        }
-                        b.curl[fid] = (int8_t)kFaceCurl[f];
-                        face_area[fid] = area[kFaceAxis[f]];
-                        boundary_face.push_back((uint32_t)kFaceLabel[f]);
-                        tbc.push_back(mat != 0);
-                    } else {
-                        uint32_t v1 = (uint32_t)Dt[(size_t)fid * 2], v2 = (uint32_t)Dt[(size_t)fid * 2 + 1];
-                        boundary_face[fid] = matad_at(vol_material[v1], vol_material[v2]);
-                        tbc[fid] = tbc[fid] || mat != 0;
-                    }
-                }
-                // material averages (D:5244-5330)
-                const double mu[3] = {mt.mu(0, 0), mt.mu(1, 1), mt.mu(2, 2)};
-                const double ch[3] = {mt.chi(0, 0), mt.chi(1, 1), mt.chi(2, 2)};
-                double ep[3] = {mt.epsilon(0, 0), mt.epsilon(1, 1), mt.epsilon(2, 2)};
-                if (mt.eps_random) {
-                    const double er = mt.eps_lo + (mt.eps_hi - mt.eps_lo) * cell_uniform(mt.eps_seed, nv);
-                    ep[0] = ep[1] = ep[2] = EPSILON0 * er;
-                }
-                const double si[3] = {mt.sigma(0, 0), mt.sigma(1, 1), mt.sigma(2, 2)};
-                // faces by normal axis: x -> 2,3 ; y -> 1,4 ; z -> 0,5
-                const int fax[3][2] = {{2, 3}, {1, 4}, {0, 5}};
-                for (int a = 0; a < 3; ++a) {
-                    if (mu[a] != 0)
-                        for (int q = 0; q < 2; ++q) avg_ni[dcube[fax[a][q]]] += len[a] / 2 / mu[a];
-                }
-                for (int a = 0; a < 3; ++a) {
-                    if (ch[a] != 0)
-                        for (int q = 0; q < 2; ++q) {
-                            avg_mag_sigma[dcube[fax[a][q]]] += len[a] / 2 / ch[a];
-                            mag_lossy[dcube[fax[a][q]]]++;
-                        }
-                }
-                const int eax[3][4] = {{0, 5, 8, 11}, {1, 3, 9, 10}, {2, 4, 6, 7}};
-                for (int a = 0; a < 3; ++a)
-                    if (ep[a] != 0)
-                        for (int q = 0; q < 4; ++q) avg_eps[ecube[eax[a][q]]] += da[a] * ep[a];
-                for (int a = 0; a < 3; ++a)
-                    if (si[a] != 0)
-                        for (int q = 0; q < 4; ++q) {
-                            avg_sigma[ecube[eax[a][q]]] += da[a] * si[a];
-                            ele_lossy[ecube[eax[a][q]]]++;
-                        }
-                ++nv;
-                px += Lx;
-            }
-            py += Ly;
-        }
-        pz += Lz;
+    ParTable<double> edge_len(totE, 0), face_area(totF, 0);
+    ParTable<double> avg_eps(totE, 0), avg_sigma(totE, 0), avg_ni(totF, 0), avg_mag_sigma(totF, 0);
+    ParTable<uint8_t> ele_lossy(totE, 0), mag_lossy(totF, 0);
+    ParTable<uint32_t> boundary_face(totF, 0);
+    ParTable<uint8_t> tbc(totF, 0);
+    // Material of every cube (WhichSolid D:10110: the first solid by label
+    // that holds the cube's centre), on all host threads. The corners are the
+    // running sums of the steps (the reference's cube loop adds the step
+    // once per cube), so the solid tests see the same coordinates bit for bit.
+    std::vector<double> xs(Nx), ys(Ny), zs(Nz);
+    {
+        double v = xmin;
+        for (uint32_t i = 0; i < Nx; ++i) { xs[i] = v; v += Lx; }
+        v = ymin;
+        for (uint32_t j = 0; j < Ny; ++j) { ys[j] = v; v += Ly; }
+        v = zmin;
+        for (uint32_t k = 0; k < Nz; ++k) { zs[k] = v; v += Lz; }
     }
-    if (nf != totF || ne != totE || np != totP) throw Error("cartesian mesher: entity count mismatch");
+    std::vector<const SolidDef*> solid_list;
+    for (auto& s : solids) solid_list.push_back(&s.second);
+    if (solid_list.size() >= 65535) throw Error("cartesian mesher: too many solids");
+    ParTable<uint32_t> vol_material(totV, 0);
+    ParTable<uint16_t> vol_solid(totV, 0);  // 0: no solid, s + 1: solid_list[s]
+    std::vector<uint8_t> solid_used(solid_list.size() + 1, 0);
+    const int64_t nrow = (int64_t)Ny * Nz;  // grid rows (j, k), row = k Ny + j
+#pragma omp parallel for schedule(static)
+    for (int64_t row = 0; row < nrow; ++row) {
+        const uint32_t j = (uint32_t)(row % Ny), k = (uint32_t)(row / Ny);
+        for (uint32_t i = 0; i < Nx; ++i) {
+            uint32_t mat = 0, sid = 0;
+            for (size_t q = 0; q < solid_list.size(); ++q)
+                if (solid_list[q]->rule(xs[i] + 0.5 * Lx, ys[j] + 0.5 * Ly, zs[k] + 0.5 * Lz)) {
+                    mat = solid_list[q]->material;
+                    sid = (uint32_t)q + 1;
+                    break;
+                }
+            vol_material[(size_t)row * Nx + i] = mat;
+            vol_solid[(size_t)row * Nx + i] = (uint16_t)sid;
+            __atomic_store_n(&solid_used[sid], (uint8_t)1, __ATOMIC_RELAXED);
+        }
+    }
+    // the material of each solid that holds a cube (operator[] inserts the
+    // default material for an undefined label, as the reference's lookup does)
+    std::vector<const Material*> solid_mat(solid_list.size() + 1, nullptr);
+    for (size_t q = 0; q <= solid_list.size(); ++q)
+        if (solid_used[q]) solid_mat[q] = &materials[q ? solid_list[q - 1]->material : 0u];
+
+    // ---- numbering (D:4621-4934) on all host threads. The reference walks
+    // the cubes in k, j, i order, takes a face / edge / node from the bottom,
+    // left or back neighbour when that neighbour exists and numbers it next
+    // otherwise. Which local entities a cube numbers thus depends only on
+    // which of its three neighbours exist: each grid row's first numbers are a
+    // prefix sum over rows, and an entity a cube takes from a neighbour holds
+    // the number its grid position got from the cube that created it. The
+    // first cube to hold an entity creates it, so the creating cube sets what
+    // the reference sets on an entity's first visit.
+    struct NewSet { int nf = 0, ne = 0, np = 0; int f[6], e[12], p[8]; uint8_t fnew = 0; };
+    NewSet nset[8];  // by neighbour mask: 1 bottom, 2 left, 4 back
+    for (int m = 0; m < 8; ++m) {
+        auto reused = [m](const std::vector<Src>& r) {
+            for (const Src& x : r)
+                if (m >> x.nb & 1) return true;
+            return false;
+        };
+        NewSet& t = nset[m];
+        for (int f = 0; f < 6; ++f)
+            if (!reused(kFaceReuse[f])) { t.f[t.nf++] = f; t.fnew |= (uint8_t)(1u << f); }
+        for (int e = 0; e < 12; ++e)
+            if (!reused(kEdgeReuse[e])) t.e[t.ne++] = e;
+        for (int q = 0; q < 8; ++q)
+            if (!reused(kNodeReuse[q])) t.p[t.np++] = q;
+    }
+    std::vector<uint8_t> has_back(Nx);
+    int64_t n_back = 0;  // cubes of a row with a back neighbour
+    for (uint32_t i = 0; i < Nx; ++i) n_back += has_back[i] = xs[i] > xmin;
+    auto row_mask = [&](uint32_t j, uint32_t k) { return (zs[k] > zmin ? 1 : 0) | (ys[j] > ymin ? 2 : 0); };
+    std::vector<int64_t> rowF(nrow + 1, 0), rowE(nrow + 1, 0), rowP(nrow + 1, 0);
+    for (int64_t r = 0; r < nrow; ++r) {
+        const int m = row_mask((uint32_t)(r % Ny), (uint32_t)(r / Ny));
+        const NewSet &t0 = nset[m], &t1 = nset[m | 4];
+        rowF[r + 1] = rowF[r] + ((int64_t)Nx - n_back) * t0.nf + n_back * t1.nf;
+        rowE[r + 1] = rowE[r] + ((int64_t)Nx - n_back) * t0.ne + n_back * t1.ne;
+        rowP[r + 1] = rowP[r] + ((int64_t)Nx - n_back) * t0.np + n_back * t1.np;
+    }
+    if ((uint64_t)rowF[nrow] != totF || (uint64_t)rowE[nrow] != totE || (uint64_t)rowP[nrow] != totP)
+        throw Error("cartesian mesher: entity count mismatch");
+    auto cube_face_pos = [&](int f, int64_t i, int64_t j, int64_t k) {
+        const int a = kFaceAxis[f], o = f >= 3 ? 1 : 0;
+        return face_pos(a, i + (a == 0 ? o : 0), j + (a == 1 ? o : 0), k + (a == 2 ? o : 0));
+    };
+    auto cube_edge_pos = [&](int e, int64_t i, int64_t j, int64_t k) {
+        const int q0 = kEdgeNodes[e][0];  // the edge's lower node
+        return edge_pos(kEdgeAxis[e], i + (q0 & 1), j + (q0 >> 1 & 1), k + (q0 >> 2 & 1));
+    };
+    auto cube_node_pos = [&](int q, int64_t i, int64_t j, int64_t k) {
+        return ((k + (q >> 2 & 1)) * (Ny + 1) + (j + (q >> 1 & 1))) * (Nx + 1) + (i + (q & 1));
+    };
+    // number of the entity at each grid position
+    ParTable<int32_t> f_at(totF, 0), e_at(totE, 0), p_at(totP, 0);
+#pragma omp parallel for schedule(static)
+    for (int64_t r = 0; r < nrow; ++r) {
+        const uint32_t j = (uint32_t)(r % Ny), k = (uint32_t)(r / Ny);
+        const int m = row_mask(j, k);
+        int64_t nf = rowF[r], ne = rowE[r], np = rowP[r];
+        for (uint32_t i = 0; i < Nx; ++i) {
+            const NewSet& t = nset[m | (has_back[i] ? 4 : 0)];
+            for (int u = 0; u < t.nf; ++u) f_at[cube_face_pos(t.f[u], i, j, k)] = (int32_t)nf++;
+            for (int u = 0; u < t.ne; ++u) e_at[cube_edge_pos(t.e[u], i, j, k)] = (int32_t)ne++;
+            const Vec3 pp(xs[i], ys[j], zs[k]);
+            for (int u = 0; u < t.np; ++u) {
+                const int q = t.p[u];
+                const int64_t id = np++;
+                p_at[cube_node_pos(q, i, j, k)] = (int32_t)id;
+                Vec3 p = pp;
+                if (q & 4) p = p + Vec3(0, 0, Lz);
+                if (q & 1) p = p + Vec3(Lx, 0, 0);
+                if (q & 2) p = p + Vec3(0, Ly, 0);
+                b.pts[id * 3] = p.x; b.pts[id * 3 + 1] = p.y; b.pts[id * 3 + 2] = p.z;
+            }
+        }
+    }
+    // cube tables, incidences and what the creating cube sets
+#pragma omp parallel for schedule(static)
+    for (int64_t r = 0; r < nrow; ++r) {
+        const uint32_t j = (uint32_t)(r % Ny), k = (uint32_t)(r / Ny);
+        const int m = row_mask(j, k);
+        for (uint32_t i = 0; i < Nx; ++i) {
+            const NewSet& t = nset[m | (has_back[i] ? 4 : 0)];
+            const size_t c = (size_t)r * Nx + i;
+            int32_t dcube[6], ecube[12], pcube[8];
+            for (int f = 0; f < 6; ++f) D[c * 6 + f] = dcube[f] = f_at[cube_face_pos(f, i, j, k)];
+            for (int e = 0; e < 12; ++e) E_cluster[c * 12 + e] = ecube[e] = e_at[cube_edge_pos(e, i, j, k)];
+            for (int q = 0; q < 8; ++q) P_cluster[c * 8 + q] = pcube[q] = p_at[cube_node_pos(q, i, j, k)];
+            // the lower cube of a face comes first in the walk: a face taken
+            // from the neighbour below / left / behind is its second visit
+            for (int f = 0; f < 6; ++f) {
+                const size_t slot = (size_t)dcube[f] * 2 + (t.fnew >> f & 1 ? 0 : 1);
+                Dt[slot] = (int32_t)c;
+                Dt_sgn[slot] = f < 3 ? -1 : 1;
+            }
+            for (int u = 0; u < t.ne; ++u) {
+                const int e = t.e[u];
+                const int32_t eid = ecube[e];
+                b.edge_sid[eid] = cube_edge_pos(e, i, j, k);
+                G[(size_t)eid * 2] = pcube[kEdgeNodes[e][0]];
+                G[(size_t)eid * 2 + 1] = pcube[kEdgeNodes[e][1]];
+                edge_len[eid] = len[kEdgeAxis[e]];
+                b.dual_curl[eid] = (int8_t)kDualCurl[kEdgeAxis[e]];
+            }
+            for (int u = 0; u < t.nf; ++u) {
+                const int f = t.f[u];
+                const int32_t fid = dcube[f];
+                b.face_sid[fid] = cube_face_pos(f, i, j, k);
+                for (int q = 0; q < 4; ++q) {
+                    const int32_t eid = ecube[kFaceEdges[f][q]];
+                    b.c_edge[(size_t)fid * 4 + q] = eid;
+                    const uint8_t n = __atomic_fetch_add(&ct_n[eid], (uint8_t)1, __ATOMIC_RELAXED);
+                    ct_face[(size_t)eid * 4 + n] = fid;
+                    ct_sgn[(size_t)eid * 4 + n] = (int8_t)kFaceSigns[f][q];
+                }
+                b.curl[fid] = (int8_t)kFaceCurl[f];
+                face_area[fid] = area[kFaceAxis[f]];
+                boundary_face[fid] = (uint32_t)kFaceLabel[f];
+            }
+        }
+    }
+    // faces: cubes, labels and material averages (D:5244-5330), cube by cube
+    // in walk order like the reference's sums
+#pragma omp parallel for schedule(static)
+    for (int64_t fid = 0; fid < (int64_t)totF; ++fid) {
+        const int32_t c0 = Dt[(size_t)fid * 2], c1 = Dt[(size_t)fid * 2 + 1];
+        Dt_n[fid] = c1 >= 0 ? 2 : 1;
+        tbc[fid] = vol_material[c0] != 0;
+        if (c1 >= 0) {
+            boundary_face[fid] = matad_at(vol_material[c0], vol_material[c1]);
+            tbc[fid] = tbc[fid] || vol_material[c1] != 0;
+        }
+        const int64_t fp = b.face_sid[fid];
+        const int a = fp < nFx ? 0 : fp < nFx + nFy ? 1 : 2;  // normal axis
+        for (int32_t c : {c0, c1}) {
+            if (c < 0) continue;
+            const Material& mt = *solid_mat[vol_solid[c]];
+            const double mu = mt.mu(a, a), ch = mt.chi(a, a);
+            if (mu != 0) avg_ni[fid] += len[a] / 2 / mu;
+            if (ch != 0) { avg_mag_sigma[fid] += len[a] / 2 / ch; mag_lossy[fid]++; }
+        }
+    }
+    // edges: faces in creation (= number) order, and material averages over
+    // the up to four cubes around the edge in walk order
+#pragma omp parallel for schedule(static)
+    for (int64_t eid = 0; eid < (int64_t)totE; ++eid) {
+        int32_t* cf = &ct_face[(size_t)eid * 4];
+        int8_t* cs = &ct_sgn[(size_t)eid * 4];
+        for (int x = 1; x < ct_n[eid]; ++x)
+            for (int y = x; y > 0 && cf[y - 1] > cf[y]; --y) { std::swap(cf[y - 1], cf[y]); std::swap(cs[y - 1], cs[y]); }
+        int64_t rem = b.edge_sid[eid];
+        const int a = rem < nEx ? 0 : rem < nEx + nEy ? 1 : 2;
+        rem -= a == 0 ? 0 : a == 1 ? nEx : nEx + nEy;
+        const int64_t ni = Nx + (a != 0), nj = Ny + (a != 1);  // positions per line and per plane row
+        const int64_t i = rem % ni, j = rem / ni % nj, k = rem / ni / nj;
+        // the two axes across the edge, lower one first: the cubes at
+        // (-1,-1), (0,-1), (-1,0), (0,0) in them come in walk order
+        const int u = a == 0 ? 1 : 0, v = a == 2 ? 1 : 2;
+        for (int s = 0; s < 4; ++s) {
+            int64_t q[3] = {i, j, k};
+            q[u] -= (s & 1) ? 0 : 1;
+            q[v] -= (s & 2) ? 0 : 1;
+            if (q[0] < 0 || q[1] < 0 || q[2] < 0 || q[0] >= Nx || q[1] >= Ny || q[2] >= Nz) continue;
+            const size_t c = ((size_t)q[2] * Ny + q[1]) * Nx + q[0];
+            const Material& mt = *solid_mat[vol_solid[c]];
+            double ep = mt.epsilon(a, a);
+            if (mt.eps_random) ep = EPSILON0 * (mt.eps_lo + (mt.eps_hi - mt.eps_lo) * cell_uniform(mt.eps_seed, c));
+            const double si = mt.sigma(a, a);
+            if (ep != 0) avg_eps[eid] += da[a] * ep;
+            if (si != 0) { avg_sigma[eid] += da[a] * si; ele_lossy[eid]++; }
+        }
+    }
+    const uint32_t nv = (uint32_t)totV, np = (uint32_t)totP;  // cubes and nodes
     b.n_cubes = (int32_t)nv;
 
     clk.mark("mesher (cubes, numbering, averages)");
@@ -343,12 +428,13 @@ void build_fit(const Model& m, const SimulationDef& sim, Build& b) {
     for (uint64_t e = 0; e < totE; ++e) b.ct_ptr[e + 1] = b.ct_ptr[e] + ct_n[e];
     b.ct_face.resize(b.ct_ptr[totE]);
     b.ct_sgn.resize(b.ct_ptr[totE]);
-    for (uint64_t e = 0; e < totE; ++e)
+#pragma omp parallel for schedule(static)
+    for (int64_t e = 0; e < (int64_t)totE; ++e)
         for (int q = 0; q < ct_n[e]; ++q) {
             b.ct_face[b.ct_ptr[e] + q] = ct_face[e * 4 + q];
             b.ct_sgn[b.ct_ptr[e] + q] = ct_sgn[e * 4 + q];
         }
-    std::vector<int32_t>().swap(ct_face);
+    ct_face = ParTable<int32_t>();
 
     clk.mark("Ct");
     // ---- edge classes (D:5382-5471). An edge none of whose faces lies on
